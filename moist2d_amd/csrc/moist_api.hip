@@ -1,0 +1,762 @@
+// moist_api.hip — host runtime and C ABI of libmoist2d_hip.so (include/moist2d.h).
+//
+// A handle owns one HIP stream, the device pools of every ensemble member,
+// the tile work-lists and (optionally) a hipGraph of one large step.  The
+// large step is the device restatement of rk3_step! (moist.jl:1589-1683):
+//   surface fluxes, radiation (flag-gated), Kessler + sedimentation + source
+//   terms; then three stages (dt/3, dt/2, dt with max(1,ceil(ns/3)),
+//   max(1,ceil(ns/2)), ns substeps), each: diagnostics, slow tendencies,
+//   the acoustic substeps (horizontal kernel + column kernel), scalar
+//   transport + final update.
+// State buffer rotation inside a step (stage input -> output):
+//   S0 -> SA, SA -> SB, SB -> S0; S0 always holds the step-start state s0,
+//   and stage 3 writes S0 in place because that kernel reads S0 only at the
+//   cell it writes.  After a step SB still holds the stage-3 input, whose
+//   periodic partner columns are what the reference leaves in its halos.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/moist2d.h"
+#include "moist_kernels.h"
+
+using namespace mo;
+
+namespace {
+
+constexpr int TILE_X = 64, TILE_Z = 4, COL_TILE = 64;
+
+}  // namespace
+
+struct moist_handle {
+    int precision = MOIST_FP64;
+    int nm = 0;
+    std::string err;
+    hipStream_t stream = nullptr;
+    std::vector<Dom> dom_h;
+    Dom *dom_d = nullptr;
+    std::vector<int> ns_stage;  // substeps per stage
+    int ns = 0;
+    int any_sf = 0, any_rd = 0;
+    int store_diag = 1;
+    int64_t n2 = 0, nxs = 0, nzs = 0;  // pool sizes
+    int max_ka = 0, max_ia = 0;
+    int4 *tiles2d = nullptr;
+    int ntiles2d = 0;
+    int2 *ctiles = nullptr;
+    int nctiles = 0;
+    // 1-D z data (double)
+    double *z = nullptr, *zf = nullptr, *dz = nullptr, *theta_ref = nullptr;
+    double *rho_bar = nullptr, *rt_bar = nullptr, *p_bar = nullptr, *L_bar = nullptr, *B_bar = nullptr;
+    std::vector<double> rho_bar_h, rt_bar_h;
+    // pools (void: typed views below)
+    void *pool = nullptr;        // all 2-D fields of type S
+    size_t elem = 8;
+    double *prec = nullptr, *sfh = nullptr, *cpr = nullptr, *dpr = nullptr;
+    void *pool1 = nullptr;       // 1-D x fields of type S
+    int *nan_key = nullptr, *step_ctr = nullptr;
+    void *staging = nullptr;
+    double *diag_out = nullptr;
+    int halo_src = -1;  // index of the buffer set whose partner columns fill halos (-1: none)
+    // graph
+    int use_graph = 0;
+    hipGraphExec_t gexec = nullptr;
+    // timing of the acoustic column kernel
+    int ev_on = 0;
+    std::vector<hipEvent_t> ev_pool;
+    int ev_used = 0;
+    double ac_ms = 0.0;
+    int64_t ac_launches = 0;
+};
+
+namespace {
+
+#define HIPCHK(h, x)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) {                                                            \
+            (h)->err = std::string(#x) + ": " + hipGetErrorString(e_);                     \
+            return -1;                                                                     \
+        }                                                                                  \
+    } while (0)
+
+// Field slots of the 2-D pool.
+enum Slot {
+    S0_ = 0, SA_ = 7, SB_ = 14,
+    THETA = 21, PRES, RT2, UF, WF, R_RHO, R_RHOU, R_RHOW, R_RT, RP, RUP, RWP, RTP0, RTP1, MOMX, MOMZ,
+    D_RHO, D_RT, D_QV, D_QC, D_QR, D_U2D, DG_TH_MP, DG_QV_MP, DG_QR_MP, DG_QR_SED, D_TH_RD,
+    NSLOT
+};
+enum Slot1 { X_DU_SF = 0, X_DTH_SF, X_DQV_SF, X_MFLUX, X_SHFLUX, X_LHFLUX, NSLOT1 };
+
+template <class S> S *slot(moist_handle *h, int s) { return (S *)h->pool + (size_t)s * h->n2; }
+template <class S> S *slot1(moist_handle *h, int s) { return (S *)h->pool1 + (size_t)s * h->nxs; }
+
+template <class S> SSet<S> sset(moist_handle *h, int first) {
+    SSet<S> s;
+    for (int f = 0; f < 7; ++f) s.f[f] = slot<S>(h, first + f);
+    return s;
+}
+
+template <class S> Work<S> work(moist_handle *h) {
+    Work<S> w;
+    w.theta = slot<S>(h, THETA);
+    w.pres = slot<S>(h, PRES);
+    w.rt2 = slot<S>(h, RT2);
+    w.uf = slot<S>(h, UF);
+    w.wf = slot<S>(h, WF);
+    w.R_rho = slot<S>(h, R_RHO);
+    w.R_rhou = slot<S>(h, R_RHOU);
+    w.R_rhow = slot<S>(h, R_RHOW);
+    w.R_rt = slot<S>(h, R_RT);
+    w.rp = slot<S>(h, RP);
+    w.rup = slot<S>(h, RUP);
+    w.rwp = slot<S>(h, RWP);
+    w.mom_x = slot<S>(h, MOMX);
+    w.mom_z = slot<S>(h, MOMZ);
+    w.cpr = h->cpr;
+    w.dpr = h->dpr;
+    w.d_rho = slot<S>(h, D_RHO);
+    w.d_rt = slot<S>(h, D_RT);
+    w.d_qv = slot<S>(h, D_QV);
+    w.d_qc = slot<S>(h, D_QC);
+    w.d_qr = slot<S>(h, D_QR);
+    w.d_u2d = slot<S>(h, D_U2D);
+    w.d_u_sf = slot1<S>(h, X_DU_SF);
+    w.d_th_sf = slot1<S>(h, X_DTH_SF);
+    w.d_qv_sf = slot1<S>(h, X_DQV_SF);
+    w.mflux = slot1<S>(h, X_MFLUX);
+    w.shflux = slot1<S>(h, X_SHFLUX);
+    w.lhflux = slot1<S>(h, X_LHFLUX);
+    w.dg_th_mp = slot<S>(h, DG_TH_MP);
+    w.dg_qv_mp = slot<S>(h, DG_QV_MP);
+    w.dg_qr_mp = slot<S>(h, DG_QR_MP);
+    w.dg_qr_sed = slot<S>(h, DG_QR_SED);
+    w.d_th_rd = slot<S>(h, D_TH_RD);
+    w.prec = h->prec;
+    w.sfh = h->sfh;
+    w.theta_ref = h->theta_ref;
+    w.nan_key = h->nan_key;
+    w.step_ctr = h->step_ctr;
+    return w;
+}
+
+ZVec zvec(moist_handle *h) {
+    ZVec z;
+    z.z = h->z;
+    z.zf = h->zf;
+    z.dz = h->dz;
+    z.rho_bar = h->rho_bar;
+    z.rt_bar = h->rt_bar;
+    z.p_bar = h->p_bar;
+    z.L_bar = h->L_bar;
+    z.B_bar = h->B_bar;
+    return z;
+}
+
+hipEvent_t next_event(moist_handle *h) {
+    if (h->ev_used >= (int)h->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        h->ev_pool.push_back(e);
+    }
+    return h->ev_pool[h->ev_used++];
+}
+
+// ---- one RK stage ------------------------------------------------------------
+template <class S>
+void enqueue_stage(moist_handle *h, int in_set, int out_set, int s0_set, int div, double dt_ovr, int nsub,
+                   int stage, int u_phys_2d) {
+    const dim3 b2(TILE_X, TILE_Z);
+    const ZVec zv = zvec(h);
+    const Work<S> w = work<S>(h);
+    const SSet<S> in = sset<S>(h, in_set), out = sset<S>(h, out_set), s0 = sset<S>(h, s0_set);
+    S *rtp[2] = {slot<S>(h, RTP0), slot<S>(h, RTP1)};
+    hipLaunchKernelGGL(k_diag<S>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, w);
+    hipLaunchKernelGGL(k_slow<S>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, s0, w,
+                       rtp[0], u_phys_2d);
+    int cur = 0, old = 0;
+    for (int n = 0; n < nsub; ++n) {
+        hipLaunchKernelGGL(k_acoustic_u<S>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, w,
+                           (const S *)rtp[cur], (const S *)rtp[old], div, dt_ovr, nsub);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (h->ev_on) {
+            e0 = next_event(h);
+            e1 = next_event(h);
+            if (e0) hipEventRecord(e0, h->stream);
+        }
+        hipLaunchKernelGGL(k_acoustic_col<S>, dim3(h->nctiles), dim3(COL_TILE), 0, h->stream, h->dom_d,
+                           h->ctiles, zv, in, w, (const S *)rtp[cur], rtp[1 - cur], div, dt_ovr, nsub);
+        if (h->ev_on && e1) hipEventRecord(e1, h->stream);
+        old = cur;
+        cur = 1 - cur;
+    }
+    hipLaunchKernelGGL(k_scalar<S>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, s0, out, w,
+                       (const S *)rtp[cur], div, dt_ovr, stage);
+}
+
+template <class S> void enqueue_step(moist_handle *h) {
+    const ZVec zv = zvec(h);
+    const Work<S> w = work<S>(h);
+    const SSet<S> s = sset<S>(h, S0_);
+    if (h->any_sf)
+        hipLaunchKernelGGL(k_surface<S>, dim3(h->nctiles), dim3(COL_TILE), 0, h->stream, h->dom_d, h->ctiles, zv, s, w);
+    if (h->any_rd)
+        hipLaunchKernelGGL(k_radiation<S>, dim3(h->nctiles), dim3(COL_TILE), 0, h->stream, h->dom_d, h->ctiles, zv,
+                           s, w);
+    hipLaunchKernelGGL(k_physics<S>, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d, h->tiles2d,
+                       zv, s, w, h->store_diag);
+    enqueue_stage<S>(h, S0_, SA_, S0_, 3, 0.0, h->ns_stage[0], 0, 0);
+    enqueue_stage<S>(h, SA_, SB_, S0_, 2, 0.0, h->ns_stage[1], 1, 0);
+    enqueue_stage<S>(h, SB_, S0_, S0_, 1, 0.0, h->ns_stage[2], 2, 0);
+    hipLaunchKernelGGL(k_step_done, dim3(1), dim3(64), 0, h->stream, h->step_ctr);
+}
+
+void enqueue_step_any(moist_handle *h) {
+    if (h->precision == MOIST_MIXED) enqueue_step<float>(h);
+    else enqueue_step<double>(h);
+}
+
+int build_graph(moist_handle *h) {
+    if (h->gexec) return 0;
+    hipGraph_t g;
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step_any(h);
+    HIPCHK(h, hipStreamEndCapture(h->stream, &g));
+    hipError_t e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        h->err = std::string("hipGraphInstantiate: ") + hipGetErrorString(e);
+        h->gexec = nullptr;
+        return -1;
+    }
+    return 0;
+}
+
+int run_steps(moist_handle *h, int nsteps) {
+    if (h->use_graph && !h->ev_on) {
+        if (build_graph(h)) return -1;
+        for (int n = 0; n < nsteps; ++n) HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+    } else {
+        for (int n = 0; n < nsteps; ++n) enqueue_step_any(h);
+    }
+    HIPCHK(h, hipGetLastError());
+    h->halo_src = SB_;
+    return 0;
+}
+
+int decode_nan(moist_handle *h) {
+    int key = INT_MAX;
+    HIPCHK(h, hipMemcpyAsync(&key, h->nan_key, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (key == INT_MAX) return 0;
+    return key & 15;
+}
+
+int reset_nan(moist_handle *h) {
+    int key = INT_MAX;
+    HIPCHK(h, hipMemcpyAsync(h->nan_key, &key, sizeof(int), hipMemcpyHostToDevice, h->stream));
+    return 0;
+}
+
+int check_member(moist_handle *h, int m) {
+    if (m < 0 || m >= h->nm) {
+        h->err = "member index out of range";
+        return -2;
+    }
+    return 0;
+}
+
+// ---- base state of the mixed form -----------------------------------------
+void base_from_state(const Dom &D, const double *z, const std::vector<double> &rb, const std::vector<double> &rtb,
+                     double *pb, double *Lb, double *Bb) {
+    const double Rd = 287.04, Cpd = 1004.64, Cvd = Cpd - Rd, P0 = 1e5, G = 9.81;
+    for (int k = 0; k < D.nz; ++k) {
+        Lb[k] = std::log(Rd * rtb[k] / P0);
+        pb[k] = P0 * std::exp(Cpd / Cvd * Lb[k]);
+    }
+    for (int k = 0; k < D.nz; ++k) {
+        if (k < D.nz - 1)
+            Bb[k] = -(pb[k + 1] - pb[k]) / (z[k + 1] - z[k]) - G * (rb[k] + rb[k + 1]) * 0.5;
+        else
+            Bb[k] = 0.0;
+    }
+}
+
+template <class S, class H>
+int upload_t(moist_handle *h, int m, const void *const fields[7], const void *prec, const void *theta_ref) {
+    const Dom &D = h->dom_h[m];
+    const int ka = D.nz, ia = D.nx + 2 * D.halo;
+    const size_t bytes = (size_t)ka * ia * sizeof(H);
+    const bool mixed = h->precision == MOIST_MIXED;
+    if (mixed) {
+        // base state: horizontal means of rho and rho*theta (FP64)
+        std::vector<double> rb(ka, 0.0), rtb(ka, 0.0), pb(ka), Lb(ka), Bb(ka), zz(ka);
+        const H *r = (const H *)fields[0], *rt = (const H *)fields[3];
+        for (int i = 0; i < D.nx; ++i)
+            for (int k = 0; k < ka; ++k) {
+                rb[k] += double(r[(size_t)(i + D.halo) * ka + k]);
+                rtb[k] += double(rt[(size_t)(i + D.halo) * ka + k]);
+            }
+        for (int k = 0; k < ka; ++k) {
+            rb[k] /= D.nx;
+            rtb[k] /= D.nx;
+        }
+        HIPCHK(h, hipMemcpy(zz.data(), h->z + D.offz, ka * sizeof(double), hipMemcpyDeviceToHost));
+        base_from_state(D, zz.data(), rb, rtb, pb.data(), Lb.data(), Bb.data());
+        HIPCHK(h, hipMemcpy(h->rho_bar + D.offz, rb.data(), ka * 8, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->rt_bar + D.offz, rtb.data(), ka * 8, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->p_bar + D.offz, pb.data(), ka * 8, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->L_bar + D.offz, Lb.data(), ka * 8, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->B_bar + D.offz, Bb.data(), ka * 8, hipMemcpyHostToDevice));
+    }
+    for (int f = 0; f < 7; ++f) {
+        HIPCHK(h, hipMemcpyAsync(h->staging, fields[f], bytes, hipMemcpyHostToDevice, h->stream));
+        const double *base = nullptr;
+        if (mixed && f == RHO) base = h->rho_bar + D.offz;
+        if (mixed && f == RT) base = h->rt_bar + D.offz;
+        hipLaunchKernelGGL((k_upload<S, H>), dim3((D.nx + 255) / 256, ka), dim3(256), 0, h->stream, D,
+                           (const H *)h->staging, slot<S>(h, S0_ + f), base, D.halo, ka);
+        HIPCHK(h, hipGetLastError());
+    }
+    // stale halo reads of surface_fluxes!: rho_u[ks, is-1], rho[ks, ie+1]
+    double sf[2] = {double(((const H *)fields[1])[(size_t)(D.halo - 1) * ka]),
+                    double(((const H *)fields[0])[(size_t)(D.halo + D.nx) * ka])};
+    HIPCHK(h, hipMemcpyAsync(h->sfh + 2 * m, sf, sizeof(sf), hipMemcpyHostToDevice, h->stream));
+    std::vector<double> pr(D.nx), tr(ka);
+    if (prec)
+        for (int i = 0; i < D.nx; ++i) pr[i] = double(((const H *)prec)[i + D.halo]);
+    else
+        std::fill(pr.begin(), pr.end(), 0.0);
+    HIPCHK(h, hipMemcpyAsync(h->prec + D.offx, pr.data(), D.nx * 8, hipMemcpyHostToDevice, h->stream));
+    if (theta_ref) {
+        for (int k = 0; k < ka; ++k) tr[k] = double(((const H *)theta_ref)[k]);
+        HIPCHK(h, hipMemcpyAsync(h->theta_ref + D.offz, tr.data(), ka * 8, hipMemcpyHostToDevice, h->stream));
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->halo_src = -1;
+    return 0;
+}
+
+template <class S, class H>
+int download_field(moist_handle *h, const Dom &D, const S *src, const S *hsrc, void *dst, const double *base,
+                   bool halos) {
+    const int ka = D.nz, ia = D.nx + 2 * D.halo;
+    hipLaunchKernelGGL((k_download<S, H>), dim3((ia + 255) / 256, ka), dim3(256), 0, h->stream, D, src,
+                       halos ? hsrc : (const S *)nullptr, (H *)h->staging, base, D.halo, ka);
+    HIPCHK(h, hipGetLastError());
+    char *d = (char *)dst;
+    const size_t col = (size_t)ka * sizeof(H);
+    if (halos && hsrc) {
+        HIPCHK(h, hipMemcpyAsync(d, h->staging, col * D.halo, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipMemcpyAsync(d + col * (D.halo + D.nx), (char *)h->staging + col * (D.halo + D.nx),
+                                 col * D.halo, hipMemcpyDeviceToHost, h->stream));
+    }
+    HIPCHK(h, hipMemcpyAsync(d + col * D.halo, (char *)h->staging + col * D.halo, col * D.nx,
+                             hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+template <class S, class H> int download_t(moist_handle *h, int m, void *const fields[7], void *prec) {
+    const Dom &D = h->dom_h[m];
+    const bool mixed = h->precision == MOIST_MIXED;
+    for (int f = 0; f < 7; ++f) {
+        if (!fields[f]) continue;
+        const double *base = nullptr;
+        if (mixed && f == RHO) base = h->rho_bar + D.offz;
+        if (mixed && f == RT) base = h->rt_bar + D.offz;
+        const S *hs = h->halo_src >= 0 ? slot<S>(h, h->halo_src + f) : nullptr;
+        int rc = download_field<S, H>(h, D, slot<S>(h, S0_ + f), hs, fields[f], base, f != RT);
+        if (rc) return rc;
+    }
+    if (prec) {
+        std::vector<double> pr(D.nx);
+        HIPCHK(h, hipMemcpy(pr.data(), h->prec + D.offx, D.nx * 8, hipMemcpyDeviceToHost));
+        for (int i = 0; i < D.nx; ++i) ((H *)prec)[i + D.halo] = H(pr[i]);
+    }
+    return 0;
+}
+
+template <class S, class H> int download_phys_t(moist_handle *h, int m, void *const o2[6], void *const o1[6]) {
+    const Dom &D = h->dom_h[m];
+    const int s2[6] = {DG_TH_MP, DG_QV_MP, D_QC, DG_QR_MP, DG_QR_SED, D_TH_RD};
+    for (int f = 0; f < 6; ++f) {
+        if (!o2 || !o2[f]) continue;
+        int rc = download_field<S, H>(h, D, slot<S>(h, s2[f]), nullptr, o2[f], nullptr, false);
+        if (rc) return rc;
+    }
+    const int s1[6] = {X_DU_SF, X_DTH_SF, X_DQV_SF, X_MFLUX, X_SHFLUX, X_LHFLUX};
+    for (int f = 0; f < 6; ++f) {
+        if (!o1 || !o1[f]) continue;
+        std::vector<S> v(D.nx);
+        HIPCHK(h, hipMemcpy(v.data(), slot1<S>(h, s1[f]) + D.offx, D.nx * sizeof(S), hipMemcpyDeviceToHost));
+        for (int i = 0; i < D.nx; ++i) ((H *)o1[f])[i + D.halo] = H(v[i]);
+    }
+    return 0;
+}
+
+template <class S, class H>
+int compute_step_t(moist_handle *h, int m, const void *const phys[6], const void *const s0[7], double dt, int ns) {
+    const Dom &D = h->dom_h[m];
+    const int ka = D.nz, ia = D.nx + 2 * D.halo;
+    const size_t bytes = (size_t)ka * ia * sizeof(H);
+    const bool mixed = h->precision == MOIST_MIXED;
+    const int pslot[6] = {D_RHO, D_U2D, D_RT, D_QV, D_QC, D_QR};
+    for (int f = 0; f < 6; ++f) {
+        HIPCHK(h, hipMemcpyAsync(h->staging, phys[f], bytes, hipMemcpyHostToDevice, h->stream));
+        hipLaunchKernelGGL((k_upload<S, H>), dim3((D.nx + 255) / 256, ka), dim3(256), 0, h->stream, D,
+                           (const H *)h->staging, slot<S>(h, pslot[f]), (const double *)nullptr, D.halo, ka);
+    }
+    for (int f = 0; f < 7; ++f) {
+        HIPCHK(h, hipMemcpyAsync(h->staging, s0[f], bytes, hipMemcpyHostToDevice, h->stream));
+        const double *base = nullptr;
+        if (mixed && f == RHO) base = h->rho_bar + D.offz;
+        if (mixed && f == RT) base = h->rt_bar + D.offz;
+        hipLaunchKernelGGL((k_upload<S, H>), dim3((D.nx + 255) / 256, ka), dim3(256), 0, h->stream, D,
+                           (const H *)h->staging, slot<S>(h, SB_ + f), base, D.halo, ka);
+    }
+    HIPCHK(h, hipGetLastError());
+    if (reset_nan(h)) return -1;
+    enqueue_stage<S>(h, S0_, SA_, SB_, 1, dt, ns, 0, 1);
+    // the output becomes the current state; the input's partner columns are the halos
+    for (int f = 0; f < 7; ++f)
+        HIPCHK(h, hipMemcpyAsync(slot<S>(h, SB_ + f), slot<S>(h, S0_ + f), h->n2 * sizeof(S),
+                                 hipMemcpyDeviceToDevice, h->stream));
+    for (int f = 0; f < 7; ++f)
+        HIPCHK(h, hipMemcpyAsync(slot<S>(h, S0_ + f), slot<S>(h, SA_ + f), h->n2 * sizeof(S),
+                                 hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipGetLastError());
+    h->halo_src = SB_;
+    int code = decode_nan(h);
+    if (code) return code;
+    // NaN in the given tendencies (moist.jl:1564-1578), checked after the state
+    const int order[5] = {1, 2, 3, 4, 5};  // d_rho_u, d_rho_theta, d_qv, d_qc, d_qr
+    for (int f = 0; f < 5; ++f) {
+        const H *a = (const H *)phys[order[f]];
+        for (int i = D.halo; i < D.halo + D.nx; ++i)
+            for (int k = 0; k < ka; ++k)
+                if (std::isnan(double(a[(size_t)i * ka + k]))) return 8 + f;
+    }
+    return 0;
+}
+
+}  // namespace
+
+// =============================================================================
+// C ABI
+// =============================================================================
+extern "C" {
+
+int moist_create(const moist_params *const *members, int n_members, int precision, moist_handle **out) {
+    if (!out) return -2;
+    *out = nullptr;
+    if (!members || n_members < 1 || (precision != MOIST_FP64 && precision != MOIST_MIXED)) return -2;
+    auto h = std::make_unique<moist_handle>();
+    h->precision = precision;
+    h->nm = n_members;
+    h->elem = precision == MOIST_MIXED ? 4 : 8;
+    std::vector<double> zh, zfh, dzh;
+    int64_t off2 = 0, offx = 0, offz = 0;
+    for (int m = 0; m < n_members; ++m) {
+        const moist_params *p = members[m];
+        if (!p || p->nx < 2 || p->nz < 1 || p->halo < 1 || p->halo > p->nx || p->ns < 1 || !p->z || !p->zf ||
+            !p->dz) {
+            h->err = "invalid member params";
+            *out = h.release();
+            return -2;
+        }
+        if (m == 0) h->ns = p->ns;
+        if (p->ns != h->ns) {
+            h->err = "all members must share ns";
+            *out = h.release();
+            return -2;
+        }
+        Dom D{};
+        D.nx = p->nx;
+        D.nz = p->nz;
+        D.halo = p->halo;
+        D.ns = p->ns;
+        D.flags = (p->do_mp ? F_MP : 0) | (p->do_auto ? F_AUTO : 0) | (p->do_accr ? F_ACCR : 0) |
+                  (p->do_revap ? F_REVAP : 0) | (p->do_sed ? F_SED : 0) | (p->do_rd ? F_RD : 0) |
+                  (p->do_sf ? F_SF : 0);
+        D.lin_col = ((p->nx - p->halo) % p->nx + p->nx) % p->nx;
+        D.off2 = off2;
+        D.offx = offx;
+        D.offz = offz;
+        D.dx = p->dx;
+        D.dt = p->dt;
+        D.H = p->H;
+        D.nu = p->nu;
+        D.kappa = p->kappa;
+        D.gamma_d = p->gamma_d;
+        D.beta = p->beta_offcenter;
+        D.zf_sponge = p->zf_sponge;
+        D.Cd = p->Cd;
+        D.Ch = p->Ch;
+        D.Ce = p->Ce;
+        D.uabs_min = p->uabs_min;
+        D.temp_surf = p->temp_surf;
+        D.k_abs_qv = p->k_abs_qv;
+        D.k_abs_qc = p->k_abs_qc;
+        D.k_abs_qr = p->k_abs_qr;
+        D.surf_emissivity = p->surf_emissivity;
+        D.tau_cooling = p->tau_cooling;
+        D.z_tropopose = p->z_tropopose;
+        D.K_auto = p->K_auto;
+        D.Qc_threshold = p->Qc_threshold;
+        D.K_accr = p->K_accr;
+        h->any_sf |= p->do_sf;
+        h->any_rd |= p->do_rd;
+        for (int k = 0; k < p->nz; ++k) {
+            zh.push_back(p->z[k]);
+            zfh.push_back(p->zf[k]);
+            dzh.push_back(p->dz[k]);
+        }
+        off2 += (int64_t)p->nx * p->nz;
+        offx += p->nx;
+        offz += p->nz;
+        h->max_ka = std::max(h->max_ka, p->nz);
+        h->max_ia = std::max(h->max_ia, p->nx + 2 * p->halo);
+        h->dom_h.push_back(D);
+    }
+    h->n2 = off2;
+    h->nxs = offx;
+    h->nzs = offz;
+    h->ns_stage = {std::max(1, (h->ns + 2) / 3), std::max(1, (h->ns + 1) / 2), h->ns};
+
+    std::vector<int4> t2;
+    std::vector<int2> tc;
+    for (int m = 0; m < n_members; ++m) {
+        const Dom &D = h->dom_h[m];
+        for (int k0 = 0; k0 < D.nz; k0 += TILE_Z)
+            for (int i0 = 0; i0 < D.nx; i0 += TILE_X) t2.push_back(make_int4(m, i0, k0, 0));
+        for (int i0 = 0; i0 < D.nx; i0 += COL_TILE) tc.push_back(make_int2(m, i0));
+    }
+    h->ntiles2d = (int)t2.size();
+    h->nctiles = (int)tc.size();
+
+    moist_handle *hp = h.get();
+#define ALLOC(ptr, bytes) HIPCHK(hp, hipMalloc((void **)&(ptr), (bytes)))
+    HIPCHK(hp, hipStreamCreate(&hp->stream));
+    ALLOC(hp->dom_d, sizeof(Dom) * n_members);
+    ALLOC(hp->tiles2d, sizeof(int4) * t2.size());
+    ALLOC(hp->ctiles, sizeof(int2) * tc.size());
+    HIPCHK(hp, hipMemcpy(hp->dom_d, hp->dom_h.data(), sizeof(Dom) * n_members, hipMemcpyHostToDevice));
+    HIPCHK(hp, hipMemcpy(hp->tiles2d, t2.data(), sizeof(int4) * t2.size(), hipMemcpyHostToDevice));
+    HIPCHK(hp, hipMemcpy(hp->ctiles, tc.data(), sizeof(int2) * tc.size(), hipMemcpyHostToDevice));
+    double **zarr[] = {&hp->z, &hp->zf, &hp->dz, &hp->theta_ref, &hp->rho_bar, &hp->rt_bar, &hp->p_bar, &hp->L_bar,
+                       &hp->B_bar};
+    for (double **a : zarr) {
+        ALLOC(*a, sizeof(double) * offz);
+        HIPCHK(hp, hipMemset(*a, 0, sizeof(double) * offz));
+    }
+    HIPCHK(hp, hipMemcpy(hp->z, zh.data(), 8 * offz, hipMemcpyHostToDevice));
+    HIPCHK(hp, hipMemcpy(hp->zf, zfh.data(), 8 * offz, hipMemcpyHostToDevice));
+    HIPCHK(hp, hipMemcpy(hp->dz, dzh.data(), 8 * offz, hipMemcpyHostToDevice));
+    const size_t pool_bytes = (size_t)NSLOT * hp->n2 * hp->elem;
+    ALLOC(hp->pool, pool_bytes);
+    HIPCHK(hp, hipMemset(hp->pool, 0, pool_bytes));
+    ALLOC(hp->pool1, (size_t)NSLOT1 * offx * hp->elem);
+    HIPCHK(hp, hipMemset(hp->pool1, 0, (size_t)NSLOT1 * offx * hp->elem));
+    ALLOC(hp->cpr, sizeof(double) * hp->n2);
+    ALLOC(hp->dpr, sizeof(double) * hp->n2);
+    ALLOC(hp->prec, sizeof(double) * offx);
+    HIPCHK(hp, hipMemset(hp->prec, 0, sizeof(double) * offx));
+    ALLOC(hp->sfh, sizeof(double) * 2 * n_members);
+    HIPCHK(hp, hipMemset(hp->sfh, 0, sizeof(double) * 2 * n_members));
+    ALLOC(hp->nan_key, sizeof(int));
+    ALLOC(hp->step_ctr, sizeof(int));
+    HIPCHK(hp, hipMemset(hp->step_ctr, 0, sizeof(int)));
+    ALLOC(hp->staging, (size_t)hp->max_ka * hp->max_ia * 8);
+    ALLOC(hp->diag_out, sizeof(double) * 4);
+#undef ALLOC
+    if (reset_nan(hp)) return -1;
+    HIPCHK(hp, hipStreamSynchronize(hp->stream));
+    *out = h.release();
+    return 0;
+}
+
+int moist_upload(moist_handle *h, int member, const void *const fields[7], const void *prec, const void *theta_ref,
+                 int elem_bytes) {
+    if (!h || !fields) return -2;
+    if (int rc = check_member(h, member)) return rc;
+    for (int f = 0; f < 7; ++f)
+        if (!fields[f]) return -2;
+    const bool mixed = h->precision == MOIST_MIXED;
+    if (elem_bytes == 8)
+        return mixed ? upload_t<float, double>(h, member, fields, prec, theta_ref)
+                     : upload_t<double, double>(h, member, fields, prec, theta_ref);
+    if (elem_bytes == 4)
+        return mixed ? upload_t<float, float>(h, member, fields, prec, theta_ref)
+                     : upload_t<double, float>(h, member, fields, prec, theta_ref);
+    h->err = "elem_bytes must be 4 or 8";
+    return -2;
+}
+
+int moist_download(moist_handle *h, int member, void *const fields[7], void *prec, int elem_bytes) {
+    if (!h || !fields) return -2;
+    if (int rc = check_member(h, member)) return rc;
+    const bool mixed = h->precision == MOIST_MIXED;
+    if (elem_bytes == 8)
+        return mixed ? download_t<float, double>(h, member, fields, prec) : download_t<double, double>(h, member, fields, prec);
+    if (elem_bytes == 4)
+        return mixed ? download_t<float, float>(h, member, fields, prec) : download_t<double, float>(h, member, fields, prec);
+    h->err = "elem_bytes must be 4 or 8";
+    return -2;
+}
+
+int moist_download_phys(moist_handle *h, int member, void *const out2d[6], void *const out1d[6], int elem_bytes) {
+    if (!h) return -2;
+    if (int rc = check_member(h, member)) return rc;
+    const bool mixed = h->precision == MOIST_MIXED;
+    if (elem_bytes == 8)
+        return mixed ? download_phys_t<float, double>(h, member, out2d, out1d)
+                     : download_phys_t<double, double>(h, member, out2d, out1d);
+    if (elem_bytes == 4)
+        return mixed ? download_phys_t<float, float>(h, member, out2d, out1d)
+                     : download_phys_t<double, float>(h, member, out2d, out1d);
+    h->err = "elem_bytes must be 4 or 8";
+    return -2;
+}
+
+int moist_rk3_step(moist_handle *h, int nsteps) {
+    if (!h || nsteps < 0) return -2;
+    if (reset_nan(h)) return -1;
+    if (run_steps(h, nsteps)) return -1;
+    return decode_nan(h);
+}
+
+int moist_compute_step(moist_handle *h, int member, const void *const phys[6], const void *const s0[7], double dt,
+                       int ns, int elem_bytes) {
+    if (!h || !phys || !s0 || ns < 1 || !(dt > 0.0)) return -2;
+    if (int rc = check_member(h, member)) return rc;
+    if (h->nm != 1) {
+        h->err = "moist_compute_step takes a single-domain handle";
+        return -2;
+    }
+    const bool mixed = h->precision == MOIST_MIXED;
+    if (elem_bytes == 8)
+        return mixed ? compute_step_t<float, double>(h, member, phys, s0, dt, ns)
+                     : compute_step_t<double, double>(h, member, phys, s0, dt, ns);
+    if (elem_bytes == 4)
+        return mixed ? compute_step_t<float, float>(h, member, phys, s0, dt, ns)
+                     : compute_step_t<double, float>(h, member, phys, s0, dt, ns);
+    h->err = "elem_bytes must be 4 or 8";
+    return -2;
+}
+
+int moist_diagnostics(moist_handle *h, int member, double *out) {
+    if (!h || !out) return -2;
+    if (int rc = check_member(h, member)) return rc;
+    const Dom &D = h->dom_h[member];
+    const ZVec zv = zvec(h);
+    if (h->precision == MOIST_MIXED)
+        hipLaunchKernelGGL(k_diagnostics<float>, dim3(1), dim3(256), 0, h->stream, D, zv, sset<float>(h, S0_),
+                           (const double *)h->prec, h->diag_out);
+    else
+        hipLaunchKernelGGL(k_diagnostics<double>, dim3(1), dim3(256), 0, h->stream, D, zv, sset<double>(h, S0_),
+                           (const double *)h->prec, h->diag_out);
+    HIPCHK(h, hipGetLastError());
+    double v[4];
+    HIPCHK(h, hipMemcpyAsync(v, h->diag_out, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+    int key = INT_MAX;
+    HIPCHK(h, hipMemcpyAsync(&key, h->nan_key, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    out[MOIST_DIAG_DRY_MASS] = v[0];
+    out[MOIST_DIAG_TOTAL_WATER] = v[1];
+    out[MOIST_DIAG_RAIN] = v[2];
+    out[MOIST_DIAG_MASS] = v[3];
+    out[MOIST_DIAG_NAN] = key == INT_MAX ? 0.0 : double(key & 15);
+    return 0;
+}
+
+int moist_use_graph(moist_handle *h, int on) {
+    if (!h) return -2;
+    h->use_graph = on ? 1 : 0;
+    return 0;
+}
+
+int moist_set_option(moist_handle *h, int option, int value) {
+    if (!h) return -2;
+    if (option == 0) {
+        h->store_diag = value ? 1 : 0;
+        if (h->gexec) {
+            hipGraphExecDestroy(h->gexec);
+            h->gexec = nullptr;
+        }
+        return 0;
+    }
+    h->err = "unknown option";
+    return -2;
+}
+
+int moist_time_steps(moist_handle *h, int nsteps, double *ms) {
+    if (!h || !ms || nsteps < 1) return -2;
+    hipEvent_t a, b;
+    HIPCHK(h, hipEventCreate(&a));
+    HIPCHK(h, hipEventCreate(&b));
+    if (reset_nan(h)) return -1;
+    if (h->use_graph && build_graph(h)) return -1;
+    h->ev_used = 0;
+    HIPCHK(h, hipEventRecord(a, h->stream));
+    if (run_steps(h, nsteps)) return -1;
+    HIPCHK(h, hipEventRecord(b, h->stream));
+    HIPCHK(h, hipEventSynchronize(b));
+    float f = 0.0f;
+    HIPCHK(h, hipEventElapsedTime(&f, a, b));
+    *ms = f;
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (h->ev_on) {
+        double tot = 0.0;
+        int64_t n = 0;
+        for (int e = 0; e + 1 < h->ev_used; e += 2) {
+            float d = 0.0f;
+            if (hipEventElapsedTime(&d, h->ev_pool[e], h->ev_pool[e + 1]) == hipSuccess) {
+                tot += d;
+                ++n;
+            }
+        }
+        h->ac_ms = n ? tot / n : 0.0;
+        h->ac_launches = n;
+    }
+    return decode_nan(h);
+}
+
+int moist_kernel_events(moist_handle *h, int on) {
+    if (!h) return -2;
+    h->ev_on = on ? 1 : 0;
+    return 0;
+}
+
+int moist_kernel_stats(moist_handle *h, double *acoustic_ms_per_launch, int64_t *acoustic_launches) {
+    if (!h) return -2;
+    if (acoustic_ms_per_launch) *acoustic_ms_per_launch = h->ac_ms;
+    if (acoustic_launches) *acoustic_launches = h->ac_launches;
+    return 0;
+}
+
+const char *moist_last_error(const moist_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+void moist_destroy(moist_handle *h) {
+    if (!h) return;
+    if (h->gexec) hipGraphExecDestroy(h->gexec);
+    for (auto e : h->ev_pool) hipEventDestroy(e);
+    void *ptrs[] = {h->dom_d, h->tiles2d, h->ctiles, h->z, h->zf, h->dz, h->theta_ref, h->rho_bar, h->rt_bar,
+                    h->p_bar, h->L_bar, h->B_bar, h->pool, h->pool1, h->cpr, h->dpr, h->prec, h->sfh, h->nan_key,
+                    h->step_ctr, h->staging, h->diag_out};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+// moist_device.h — device-side types and point functions of the gfx950
+// time-stepping core.  Everything here is __device__ inline and shared by
+// the kernels in moist_kernels.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mo {
+
+// ---- physical constants in the compute type (moist.jl:30-59) --------------
+template <class R> struct K {
+    static constexpr R GRAV = R(9.81);
+    static constexpr R Rd = R(287.04);
+    static constexpr R Rv = R(461.5);
+    static constexpr R Cpd = R(1004.64);
+    static constexpr R Cpv = R(1846.0);
+    static constexpr R Cpl = R(4218.0);
+    static constexpr R Cvd = Cpd - Rd;
+    static constexpr R P0 = R(1e5);
+    static constexpr R Lv = R(2.501e6);
+    static constexpr R SIGMA = R(5.67e-8);
+    static constexpr R EPSvap = Rd / Rv;
+    static constexpr R K_evap = R(1e-2);
+    static constexpr R A_term = R(36.3);
+    static constexpr R B_term = R(0.1346);
+    static constexpr R T0 = R(273.15);
+    static constexpr R ES0 = R(610.78);
+    static constexpr R A_SAT = R(17.67);
+    static constexpr R B_SAT = R(243.5);
+    static constexpr R EPS = R(1e-12);
+};
+
+// ---- per-domain descriptor (one per ensemble member) -----------------------
+enum : int {
+    F_MP = 1, F_AUTO = 2, F_ACCR = 4, F_REVAP = 8, F_SED = 16, F_RD = 32, F_SF = 64
+};
+
+struct Dom {
+    int nx, nz, halo, ns;
+    int flags;
+    int lin_col;        // interior column the positivity limiters read (halo column 1)
+    int64_t off2;       // first cell of this domain in every 2-D pool
+    int64_t offx;       // first column in the 1-D x pools
+    int64_t offz;       // first level in the 1-D z pools
+    double dx, dt, H, nu, kappa, gamma_d, beta, zf_sponge;
+    double Cd, Ch, Ce, uabs_min, temp_surf;
+    double k_abs_qv, k_abs_qc, k_abs_qr, surf_emissivity, tau_cooling, z_tropopose;
+    double K_auto, Qc_threshold, K_accr;
+};
+
+// 1-D z vectors of all domains, concatenated (offset Dom::offz).
+struct ZVec {
+    const double *z, *zf, *dz;
+    // base state of the mixed form (zero/unused in the FP64 form)
+    const double *rho_bar, *rt_bar, *p_bar, *L_bar, *B_bar;
+};
+
+// ---- Julia Base.min/max for IEEE floats -------------------------------------
+template <class R> __device__ __forceinline__ R jmax(R x, R y) {
+    R diff = x - y;
+    R am = signbit(diff) ? y : x;
+    return (isnan(x) || isnan(y)) ? diff : am;
+}
+template <class R> __device__ __forceinline__ R jmin(R x, R y) {
+    R diff = x - y;
+    R am = signbit(diff) ? x : y;
+    return (isnan(x) || isnan(y)) ? diff : am;
+}
+
+template <class R> __device__ __forceinline__ R f2h(R v1, R v2, R d1, R d2) {
+    return (v1 * d2 + v2 * d1) / (d1 + d2);
+}
+
+// Koren (1993) limiter (moist.jl:844-847)
+template <class R> __device__ __forceinline__ R koren(R r) {
+    return jmax(R(0.0), jmin(R(2.0) * r, jmin((R(1.0) + R(2.0) * r) / R(3.0), R(2.0))));
+}
+
+// (-m + 5c + 2p), the numerator of the 3rd-order upwind face value
+template <class R> __device__ __forceinline__ R upw3(R m, R c, R p) {
+    return -m + R(5.0) * c + R(2.0) * p;
+}
+
+// Periodic wrap of i + o for |o| <= 2 (exchange_halo!, moist.jl:233-241).
+__device__ __forceinline__ int wrapx(int j, int nx) {
+    if (j < 0) j += nx;
+    if (j < 0) j += nx;
+    if (j >= nx) j -= nx;
+    if (j >= nx) j -= nx;
+    return j;
+}
+
+// ---- math in the compute type -----------------------------------------------
+__device__ __forceinline__ double m_pow(double a, double b) { return pow(a, b); }
+__device__ __forceinline__ float m_pow(float a, float b) { return powf(a, b); }
+__device__ __forceinline__ double m_exp(double a) { return exp(a); }
+__device__ __forceinline__ float m_exp(float a) { return expf(a); }
+__device__ __forceinline__ double m_sqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ float m_sqrt(float a) { return sqrtf(a); }
+__device__ __forceinline__ double m_tanh(double a) { return tanh(a); }
+__device__ __forceinline__ float m_tanh(float a) { return tanhf(a); }
+__device__ __forceinline__ double m_fabs(double a) { return fabs(a); }
+__device__ __forceinline__ float m_fabs(float a) { return fabsf(a); }
+__device__ __forceinline__ double m_log1p(double a) { return log1p(a); }
+__device__ __forceinline__ float m_log1p(float a) { return log1pf(a); }
+__device__ __forceinline__ double m_expm1(double a) { return expm1(a); }
+__device__ __forceinline__ float m_expm1(float a) { return expm1f(a); }
+
+// Tetens saturation vapour pressure and its T derivative (moist.jl:254-260)
+template <class R> __device__ __forceinline__ void sat_vp(R Tk, R &es, R &des) {
+    R Tc = Tk - K<R>::T0;
+    R tmp = K<R>::B_SAT + Tc;
+    es = K<R>::ES0 * m_exp(K<R>::A_SAT * Tc / tmp);
+    des = es * K<R>::A_SAT * (tmp - Tc) / (tmp * tmp);
+}
+
+// Saturation specific humidity and d qsat / d es (moist.jl:263-273)
+template <class R> __device__ __forceinline__ void sat_q(R p, R es, R &qs, R &dqs) {
+    R tmp = p - (R(1.0) - K<R>::EPSvap) * es;
+    if (tmp > K<R>::EPS) {
+        qs = K<R>::EPSvap * es / tmp;
+        dqs = K<R>::EPSvap * p / (tmp * tmp);
+    } else {
+        qs = R(0.0);
+        dqs = R(1.0);
+    }
+}
+
+// NaN report: the smallest (step, stage, field) key wins, like the
+// reference's first error() (moist.jl:1543-1578).
+__device__ __forceinline__ void report_nan(int *nan_key, int key) { atomicMin(nan_key, key); }
+
+}  // namespace mo

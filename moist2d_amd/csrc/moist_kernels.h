@@ -1,0 +1,966 @@
+// moist_kernels.h — gfx950 kernels of one large step (rk3_step!, moist.jl:1589-1683).
+//
+// Device layout: every 2-D field is a pool holding all ensemble members;
+// member m's cell (k, i) is at Dom::off2 + k*nx + i (x fastest, no halo
+// columns; periodic x is an index wrap).  A block takes one entry of a tile
+// work-list: 2-D tiles are 64 columns x 4 levels (blockDim 64x4, one wave
+// per level row, coalesced along x); column tiles are 64 columns (one thread
+// per column, sweeping k).
+//
+// S is the storage type.  S = double is the FP64 form, which restates the
+// reference's arithmetic operation by operation.  S = float is the mixed
+// form: rho and rho*theta are stored as deviations from an FP64 1-D base
+// state (rho_bar, rt_bar), pressure is carried as its deviation from
+// p_bar = P0 (Rd rt_bar / P0)^(Cpd/Cvd) computed through log1p/expm1 so that
+// horizontal and vertical pressure differences keep full FP32 precision,
+// and the hydrostatic residual of the base state (B_bar) is added in FP64.
+// The w-p tridiagonal system is assembled and eliminated in FP64 in both forms.
+#pragma once
+
+#include "moist_device.h"
+
+namespace mo {
+
+template <class S> struct SSet {  // the 7 prognostic fields (moist.jl:295-301)
+    S *f[7];
+};
+enum { RHO = 0, RHOU = 1, RHOW = 2, RT = 3, QV = 4, QC = 5, QR = 6 };
+
+template <class S> struct Work {
+    S *theta, *pres, *rt2, *uf, *wf;      // stage diagnostics (moist.jl:1044-1097,1184-1189)
+    S *R_rho, *R_rhou, *R_rhow, *R_rt;    // slow tendencies (moist.jl:1071-1332)
+    S *rp, *rup, *rwp;                    // acoustic perturbations (moist.jl:1339-1342)
+    S *mom_x, *mom_z;                     // time-averaged mass fluxes (moist.jl:1347-1348)
+    double *cpr, *dpr;                    // Thomas factors of the current substep
+    S *d_rho, *d_rt, *d_qv, *d_qc, *d_qr; // physics tendencies, frozen over a step (moist.jl:1632-1644)
+    S *d_u2d;                             // 2-D d_rho_u_phys (stage-level entry only)
+    S *d_u_sf, *d_th_sf, *d_qv_sf, *mflux, *shflux, *lhflux;  // 1-D x (moist.jl:710-752)
+    S *dg_th_mp, *dg_qv_mp, *dg_qr_mp, *dg_qr_sed, *d_th_rd;  // State diagnostics
+    double *prec;                         // accumulated surface rain (1-D x)
+    double *sfh;                          // stale halo reads of surface_fluxes!, 2 per member
+    const double *theta_ref;              // 1-D z (gray radiation)
+    int *nan_key;
+    const int *step_ctr;
+};
+
+template <class S> constexpr bool is_mixed() { return sizeof(S) == 4; }
+
+// Stage time step: dt/3, dt/2, dt in the compute type (moist.jl:1649-1657),
+// or an explicit dt for the stage-level entry.
+template <class R> __device__ __forceinline__ R stage_dt(const Dom &D, int div, double dt_ovr) {
+    if (dt_ovr > 0.0) return R(dt_ovr);
+    return R(D.dt) / R(div);
+}
+
+template <class S>
+__device__ __forceinline__ S full_rho(const SSet<S> &s, int64_t c, const ZVec &zv, const Dom &D, int k) {
+    if constexpr (is_mixed<S>()) return s.f[RHO][c] + S(zv.rho_bar[D.offz + k]);
+    else return s.f[RHO][c];
+}
+template <class S>
+__device__ __forceinline__ S full_rt(const SSet<S> &s, int64_t c, const ZVec &zv, const Dom &D, int k) {
+    if constexpr (is_mixed<S>()) return s.f[RT][c] + S(zv.rt_bar[D.offz + k]);
+    else return s.f[RT][c];
+}
+
+// =============================================================================
+// Stage diagnostics: theta, pressure (or its deviation), rt2pres, u and w at
+// faces (moist.jl:1048-1062, 1091-1097, 1184-1189).
+// =============================================================================
+template <class S>
+__global__ void __launch_bounds__(256) k_diag(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
+                                              ZVec zv, SSet<S> in, Work<S> w) {
+    using R = S;
+    using KC = K<R>;
+    const int4 t = tiles[blockIdx.x];
+    const Dom &D = doms[t.x];
+    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+    const int nx = D.nx, nz = D.nz;
+    if (i >= nx || k >= nz) return;
+    const int64_t c = D.off2 + (int64_t)k * nx + i;
+    const int ip = wrapx(i + 1, nx);
+    const R rho = full_rho(in, c, zv, D, k);
+    const R rt = full_rt(in, c, zv, D, k);
+    const R qv = in.f[QV][c] / rho, qc = in.f[QC][c] / rho, qr = in.f[QR][c] / rho;
+    const R qd = R(1.0) - qv - qc - qr;
+    const R r = KC::Rd * qd + KC::Rv * qv;
+    const R cp = KC::Cpd * qd + KC::Cpv * qv + KC::Cpl * (qc + qr);
+    const R cv = cp - r;
+    R pres, pfull;
+    if constexpr (is_mixed<S>()) {
+        // p = P0 exp(g ln(r rt/P0)), p_bar = P0 exp(g_bar ln(Rd rt_bar/P0));
+        // ln(r rt / (Rd rt_bar)) = log1p(dr) + log1p(rt'/rt_bar),
+        // g - g_bar = (Cpd dR - Rd dCp) / (cv Cvd).
+        const R dR = (KC::Rv - KC::Rd) * qv - KC::Rd * (qc + qr);
+        const R dCp = (KC::Cpv - KC::Cpd) * qv + (KC::Cpl - KC::Cpd) * (qc + qr);
+        const R dg = (KC::Cpd * dR - KC::Rd * dCp) / (cv * KC::Cvd);
+        const R gam = cp / cv;
+        const R rtb = R(zv.rt_bar[D.offz + k]);
+        const R E = dg * R(zv.L_bar[D.offz + k]) + gam * (m_log1p(dR / KC::Rd) + m_log1p(in.f[RT][c] / rtb));
+        const double pb = zv.p_bar[D.offz + k];
+        pres = R(pb) * m_expm1(E);
+        pfull = R(pb) + pres;
+    } else {
+        pres = KC::P0 * m_pow(r * rt / KC::P0, cp / cv);
+        pfull = pres;
+    }
+    w.pres[c] = pres;
+    w.theta[c] = rt / rho;
+    w.rt2[c] = cp / cv * pfull / rt;
+    const int64_t cip = D.off2 + (int64_t)k * nx + ip;
+    w.uf[c] = in.f[RHOU][c] * R(2.0) / (rho + full_rho(in, cip, zv, D, k));
+    if (k < nz - 1) {
+        const int64_t ckp = c + nx;
+        const double *dz = zv.dz + D.offz;
+        w.wf[c] = in.f[RHOW][c] / f2h(rho, full_rho(in, ckp, zv, D, k + 1), R(dz[k]), R(dz[k + 1]));
+    } else {
+        w.wf[c] = R(0.0);
+    }
+}
+
+// =============================================================================
+// Slow tendencies R_rho, R_rho_u, R_rho_w, R_rho_theta (moist.jl:1071-1332)
+// and the acoustic perturbation start values (moist.jl:1339-1348).
+// =============================================================================
+template <class S>
+__global__ void __launch_bounds__(256) k_slow(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
+                                              ZVec zv, SSet<S> in, SSet<S> s0, Work<S> w, S *rtp0,
+                                              int u_phys_2d) {
+    using R = S;
+    using KC = K<R>;
+    const int4 t = tiles[blockIdx.x];
+    const Dom &D = doms[t.x];
+    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+    const int nx = D.nx, nz = D.nz, ke = nz - 1;
+    if (i >= nx || k >= nz) return;
+    const int64_t row = D.off2 + (int64_t)k * nx;
+    const int64_t c = row + i;
+    const int im = wrapx(i - 1, nx), im2 = wrapx(i - 2, nx), ip = wrapx(i + 1, nx), ip2 = wrapx(i + 2, nx);
+    const double *zd = zv.z + D.offz, *zfd = zv.zf + D.offz, *dzd = zv.dz + D.offz;
+    const R dx = R(D.dx);
+    const R EPS = KC::EPS;
+    const S *rho_u = in.f[RHOU], *rho_w = in.f[RHOW];
+    auto at = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + ii; };
+    auto RHOF = [&](int kk, int ii) -> R { return full_rho(in, at(kk, ii), zv, D, kk); };
+    auto U = [&](int kk, int ii) -> R { return w.uf[at(kk, ii)]; };
+    auto W = [&](int kk, int ii) -> R { return w.wf[at(kk, ii)]; };
+    auto TH = [&](int kk, int ii) -> R { return w.theta[at(kk, ii)]; };
+    auto DZ = [&](int kk) -> R { return R(dzd[kk]); };
+    auto Z = [&](int kk) -> R { return R(zd[kk]); };
+
+    // ---- R_rho (moist.jl:1079-1086)
+    {
+        R fkp = k == ke ? R(0.0) : rho_w[c];
+        R fkm = k == 0 ? R(0.0) : rho_w[c - nx];
+        R adv = -((rho_u[c] - rho_u[row + im]) / dx + (fkp - fkm) / DZ(k));
+        w.R_rho[c] = adv + w.d_rho[c];
+    }
+    // ---- R_rho_u (moist.jl:1098-1179)
+    {
+        R rho_u_ip = R(0.5) * (rho_u[c] + rho_u[row + ip]);
+        R fip, fim;
+        if (rho_u_ip > EPS) fip = rho_u_ip * upw3(U(k, im), U(k, i), U(k, ip)) / R(6.0);
+        else if (rho_u_ip < -EPS) fip = rho_u_ip * upw3(U(k, ip2), U(k, ip), U(k, i)) / R(6.0);
+        else fip = rho_u_ip * (U(k, i) + U(k, ip)) * R(0.5);
+        R rho_u_im = R(0.5) * (rho_u[row + im] + rho_u[c]);
+        if (rho_u_im > EPS) fim = rho_u_im * upw3(U(k, im2), U(k, im), U(k, i)) / R(6.0);
+        else if (rho_u_im < -EPS) fim = rho_u_im * upw3(U(k, ip), U(k, i), U(k, im)) / R(6.0);
+        else fim = rho_u_im * (U(k, im) + U(k, i)) * R(0.5);
+        R druudx = (fip - fim) / dx;
+        R fkp, fkm;
+        if (k == ke) fkp = R(0.0);
+        else {
+            R rw = R(0.5) * (rho_w[c] + rho_w[row + ip]);
+            if (rw > EPS && k > 0) fkp = rw * upw3(U(k - 1, i), U(k, i), U(k + 1, i)) / R(6.0);
+            else if (rw < -EPS && k < ke - 1) fkp = rw * upw3(U(k + 2, i), U(k + 1, i), U(k, i)) / R(6.0);
+            else fkp = rw * f2h(U(k, i), U(k + 1, i), DZ(k), DZ(k + 1));
+        }
+        if (k == 0) fkm = R(0.0);
+        else {
+            R rw = R(0.5) * (rho_w[c - nx] + rho_w[row - nx + ip]);
+            if (rw > EPS && k > 1) fkm = rw * upw3(U(k - 2, i), U(k - 1, i), U(k, i)) / R(6.0);
+            else if (rw < -EPS && k < ke) fkm = rw * upw3(U(k + 1, i), U(k, i), U(k - 1, i)) / R(6.0);
+            else fkm = rw * f2h(U(k - 1, i), U(k, i), DZ(k - 1), DZ(k));
+        }
+        R drwudz = (fkp - fkm) / DZ(k);
+        R adv = -druudx - drwudz;
+        R u_ip = (U(k, ip) - U(k, i)) / dx;
+        R u_im = (U(k, i) - U(k, im)) / dx;
+        R u_xx = (u_ip * RHOF(k, ip) - u_im * RHOF(k, i)) / dx;
+        R u_kp = k == ke ? R(0.0) : (U(k + 1, i) - U(k, i)) / (Z(k + 1) - Z(k));
+        R u_km = k == 0 ? R(0.0) : (U(k, i) - U(k - 1, i)) / (Z(k) - Z(k - 1));
+        R rho_kp = k == ke ? R(0.0) : R(0.25) * (RHOF(k, i) + RHOF(k + 1, i) + RHOF(k, ip) + RHOF(k + 1, ip));
+        R rho_km = k == 0 ? R(0.0) : R(0.25) * (RHOF(k - 1, i) + RHOF(k, i) + RHOF(k - 1, ip) + RHOF(k, ip));
+        R u_zz = (u_kp * rho_kp - u_km * rho_km) / DZ(k);
+        R diff = R(D.nu) * (u_xx + u_zz);
+        R dpdx = (w.pres[row + ip] - w.pres[c]) / dx;
+        R dup = u_phys_2d ? w.d_u2d[c] : (k == 0 ? w.d_u_sf[D.offx + i] : R(0.0));
+        w.R_rhou[c] = adv - dpdx + diff + dup;
+    }
+    // ---- R_rho_w (moist.jl:1190-1279), faces k+1/2 for k < ke
+    if (k < ke) {
+        R rho_u_ip = f2h(rho_u[c], rho_u[c + nx], Z(k), Z(k + 1));
+        R fip, fim;
+        if (rho_u_ip > EPS) fip = rho_u_ip * upw3(W(k, im), W(k, i), W(k, ip)) / R(6.0);
+        else if (rho_u_ip < -EPS) fip = rho_u_ip * upw3(W(k, ip2), W(k, ip), W(k, i)) / R(6.0);
+        else fip = rho_u_ip * (W(k, i) + W(k, ip)) * R(0.5);
+        R rho_u_im = f2h(rho_u[row + im], rho_u[row + nx + im], DZ(k), DZ(k + 1));
+        if (rho_u_im > EPS) fim = rho_u_im * upw3(W(k, im2), W(k, im), W(k, i)) / R(6.0);
+        else if (rho_u_im < -EPS) fim = rho_u_im * upw3(W(k, ip), W(k, i), W(k, im)) / R(6.0);
+        else fim = rho_u_im * (W(k, im) + W(k, i)) * R(0.5);
+        R drwudx = (fip - fim) / dx;
+        R fkp, fkm;
+        if (k == ke - 1) fkp = R(0.0);
+        else {
+            R rw = f2h(rho_w[c], rho_w[c + nx], DZ(k), DZ(k + 1));
+            if (rw > EPS && k > 0) fkp = rw * upw3(W(k - 1, i), W(k, i), W(k + 1, i)) / R(6.0);
+            else if (rw < -EPS && k < ke - 2) fkp = rw * upw3(W(k + 2, i), W(k + 1, i), W(k, i)) / R(6.0);
+            else fkp = rw * f2h(W(k, i), W(k + 1, i), DZ(k), DZ(k + 1));
+        }
+        if (k == 0) fkm = R(0.0);
+        else {
+            R rw = f2h(rho_w[c - nx], rho_w[c], DZ(k - 1), DZ(k));
+            if (rw > EPS && k > 1) fkm = rw * upw3(W(k - 2, i), W(k - 1, i), W(k, i)) / R(6.0);
+            else if (rw < -EPS && k < ke - 1) fkm = rw * upw3(W(k + 1, i), W(k, i), W(k - 1, i)) / R(6.0);
+            else fkm = rw * f2h(W(k - 1, i), W(k, i), DZ(k - 1), DZ(k));
+        }
+        R zz = Z(k + 1) - Z(k);
+        R drwwdz = (fkp - fkm) / zz;
+        R adv = -drwudx - drwwdz;
+        R w_ip = (W(k, ip) - W(k, i)) / dx;
+        R w_im = (W(k, i) - W(k, im)) / dx;
+        R rho_ip = R(0.25) * (RHOF(k, i) + RHOF(k + 1, i) + RHOF(k, ip) + RHOF(k + 1, ip));
+        R rho_im = R(0.25) * (RHOF(k, im) + RHOF(k + 1, im) + RHOF(k, i) + RHOF(k + 1, i));
+        R w_xx = (w_ip * rho_ip - w_im * rho_im) / dx;
+        R w_kp = k == ke - 1 ? R(0.0) : (W(k + 1, i) - W(k, i)) / DZ(k + 1);
+        R w_km = k == 0 ? R(0.0) : (W(k, i) - W(k - 1, i)) / DZ(k);
+        R w_zz = (w_kp * RHOF(k + 1, i) - w_km * RHOF(k, i)) / zz;
+        R diff = R(D.nu) * (w_xx + w_zz);
+        R sigma = R(0.0);
+        const R zf_sp = R(D.zf_sponge), H = R(D.H);
+        if (R(zfd[k]) > (R(1.0) - zf_sp) * H)
+            sigma = (KC::GRAV / R(30.0)) * ((R(zfd[k]) - (R(1.0) - zf_sp) * H) / (zf_sp * H));
+        R sponge = -sigma * rho_w[c];
+        R res;
+        if constexpr (is_mixed<S>()) {
+            // -dp/dz + buoyancy = B_bar - dp'/dz - g (rho'_k + rho'_k+1)/2
+            R dpdz = (w.pres[c + nx] - w.pres[c]) / zz;
+            R buoy = -KC::GRAV * (in.f[RHO][c] + in.f[RHO][c + nx]) * R(0.5);
+            res = adv + ((R(zv.B_bar[D.offz + k]) - dpdz) + buoy) + diff + sponge;
+        } else {
+            R dpdz = (w.pres[c + nx] - w.pres[c]) / zz;
+            R buoy = -KC::GRAV * (RHOF(k, i) + RHOF(k + 1, i)) * R(0.5);
+            res = adv - dpdz + buoy + diff + sponge;
+        }
+        w.R_rhow[c] = res;
+    } else {
+        w.R_rhow[c] = R(0.0);
+    }
+    // ---- R_rho_theta (moist.jl:1284-1332)
+    {
+        R ru = rho_u[c], rum = rho_u[row + im];
+        R fip, fim, fkp, fkm;
+        if (ru > EPS) fip = ru * upw3(TH(k, im), TH(k, i), TH(k, ip)) / R(6.0);
+        else if (ru < -EPS) fip = ru * upw3(TH(k, ip2), TH(k, ip), TH(k, i)) / R(6.0);
+        else fip = ru * (TH(k, i) + TH(k, ip)) * R(0.5);
+        if (rum > EPS) fim = rum * upw3(TH(k, im2), TH(k, im), TH(k, i)) / R(6.0);
+        else if (rum < -EPS) fim = rum * upw3(TH(k, ip), TH(k, i), TH(k, im)) / R(6.0);
+        else fim = rum * (TH(k, im) + TH(k, i)) * R(0.5);
+        if (k == ke) fkp = R(0.0);
+        else {
+            R rw = rho_w[c];
+            if (rw > EPS && k > 0) fkp = rw * upw3(TH(k - 1, i), TH(k, i), TH(k + 1, i)) / R(6.0);
+            else if (rw < -EPS && k < ke - 1) fkp = rw * upw3(TH(k + 2, i), TH(k + 1, i), TH(k, i)) / R(6.0);
+            else fkp = rw * f2h(TH(k, i), TH(k + 1, i), DZ(k), DZ(k + 1));
+        }
+        if (k == 0) fkm = R(0.0);
+        else {
+            R rw = rho_w[c - nx];
+            if (rw > EPS && k > 1) fkm = rw * upw3(TH(k - 2, i), TH(k - 1, i), TH(k, i)) / R(6.0);
+            else if (rw < -EPS && k < ke) fkm = rw * upw3(TH(k + 1, i), TH(k, i), TH(k - 1, i)) / R(6.0);
+            else fkm = rw * f2h(TH(k - 1, i), TH(k, i), DZ(k - 1), DZ(k));
+        }
+        R adv = -((fip - fim) / dx + (fkp - fkm) / DZ(k));
+        w.R_rt[c] = adv + w.d_rt[c];
+    }
+    // ---- perturbations from the step start, mass-flux accumulators
+    w.rp[c] = s0.f[RHO][c] - in.f[RHO][c];
+    w.rup[c] = s0.f[RHOU][c] - in.f[RHOU][c];
+    w.rwp[c] = s0.f[RHOW][c] - in.f[RHOW][c];
+    rtp0[c] = s0.f[RT][c] - in.f[RT][c];
+    w.mom_x[c] = in.f[RHOU][c];
+    w.mom_z[c] = in.f[RHOW][c];
+}
+
+// =============================================================================
+// Acoustic substep, horizontal part: rho_u' with divergence damping
+// (moist.jl:1360-1369) and its running mean (moist.jl:1453-1455).
+// =============================================================================
+template <class S>
+__global__ void __launch_bounds__(256) k_acoustic_u(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
+                                                    Work<S> w, const S *__restrict__ rtp_cur,
+                                                    const S *__restrict__ rtp_old, int div, double dt_ovr,
+                                                    int nsub) {
+    using R = S;
+    const int4 t = tiles[blockIdx.x];
+    const Dom &D = doms[t.x];
+    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+    const int nx = D.nx;
+    if (i >= nx || k >= D.nz) return;
+    const int64_t row = D.off2 + (int64_t)k * nx;
+    const int64_t c = row + i, cp = row + wrapx(i + 1, nx);
+    const R dx = R(D.dx);
+    const R dt_s = stage_dt<R>(D, div, dt_ovr) / R(nsub);
+    R gx = (rtp_cur[cp] - rtp_cur[c]) / dx;
+    R gxo = (rtp_old[cp] - rtp_old[c]) / dx;
+    gx = gx + R(D.gamma_d) * (gx - gxo);
+    R rt2h = (w.rt2[c] + w.rt2[cp]) * R(0.5);
+    R v = w.rup[c] + dt_s * (w.R_rhou[c] - rt2h * gx);
+    w.rup[c] = v;
+    w.mom_x[c] = w.mom_x[c] + v / R(nsub);
+}
+
+// =============================================================================
+// Acoustic substep, column part (moist.jl:1377-1447, 1456-1458):
+// explicit rho'/rho_theta', the w-p tridiagonal (implicit_correction!,
+// moist.jl:974-1030, solved by the Thomas algorithm, moist.jl:952-971,
+// in FP64), then the vertical flux updates.  One thread per column.
+// =============================================================================
+template <class S>
+__global__ void __launch_bounds__(64) k_acoustic_col(const Dom *__restrict__ doms, const int2 *__restrict__ ctiles,
+                                                     ZVec zv, SSet<S> in, Work<S> w,
+                                                     const S *__restrict__ rtp_cur, S *__restrict__ rtp_out,
+                                                     int div, double dt_ovr, int nsub) {
+    using R = S;
+    const int2 t = ctiles[blockIdx.x];
+    const Dom &D = doms[t.x];
+    const int i = t.y + threadIdx.x;
+    const int nx = D.nx, nz = D.nz, ke = nz - 1, n = nz - 1;
+    if (i >= nx) return;
+    const int im = wrapx(i - 1, nx), ip = wrapx(i + 1, nx);
+    const int64_t base = D.off2;
+    const double *zd = zv.z + D.offz, *dzd = zv.dz + D.offz;
+    const R dx = R(D.dx);
+    const R EPS = K<R>::EPS;
+    const R dt_s = stage_dt<R>(D, div, dt_ovr) / R(nsub);
+    // fact_tp of the explicit part is a Float64 expression in the reference
+    // (moist.jl:1375); in the FP64 form that is the same double.
+    const R beta = R(D.beta);
+    const double fact_tp_d = double(R(0.5)) * (1.0 + double(beta));
+    // implicit_correction!'s own factors, in T (moist.jl:987-989)
+    const R ftp = R(0.5) * (R(1.0) + beta);
+    const R ftm = R(0.5) * (R(1.0) - beta);
+    const double dt_tp2 = double(dt_s * ftp) * double(dt_s * ftp);
+    const double fact = double(ftm / ftp);
+    const double G = K<double>::GRAV;
+    auto at = [&](int kk) -> int64_t { return base + (int64_t)kk * nx + i; };
+    auto TH = [&](int kk) -> R { return w.theta[at(kk)]; };
+    auto DZ = [&](int kk) -> R { return R(dzd[kk]); };
+
+    // ---- pass 1 (k ascending): explicit updates, RHS, forward elimination
+    R rp_tp_prev = R(0), rtp_tp_prev = R(0);
+    double thh_jm1 = 0.0, thh_j = 0.0, cprev = 0.0, dprev = 0.0;
+    for (int k = 0; k < nz; ++k) {
+        const int64_t c = at(k);
+        const int64_t cm = base + (int64_t)k * nx + im, cpx = base + (int64_t)k * nx + ip;
+        const R rupc = w.rup[c], rupm = w.rup[cm];
+        R adv = -(rupc - rupm) / dx;
+        R inc = dt_s * (w.R_rho[c] + adv);
+        const R rpc = w.rp[c];
+        R rp_tp;
+        if constexpr (is_mixed<S>()) rp_tp = rpc + R(fact_tp_d) * inc;
+        else rp_tp = R(double(rpc) + fact_tp_d * double(inc));
+        w.rp[c] = rpc + inc;
+        const R th = w.theta[c];
+        R fip = (w.theta[cpx] + th) * R(0.5) * rupc;
+        R fim = (th + w.theta[cm]) * R(0.5) * rupm;
+        adv = -(fip - fim) / dx;
+        inc = dt_s * (w.R_rt[c] + adv);
+        const R rtc = rtp_cur[c];
+        R rtp_tp;
+        if constexpr (is_mixed<S>()) rtp_tp = rtc + R(fact_tp_d) * inc;
+        else rtp_tp = R(double(rtc) + fact_tp_d * double(inc));
+        rtp_out[c] = rtc + inc;
+        if (k > 0) {
+            const int j = k - 1;  // face j+1/2 between levels j and k
+            const int64_t cj = at(j);
+            const R zzR = R(zd[k]) - R(zd[j]);
+            R buoy = (rtp_tp - rtp_tp_prev) / zzR;
+            R grav = -K<R>::GRAV * R(0.5) * (rp_tp + rp_tp_prev);
+            const R rt2j = w.rt2[cj], rt2k = w.rt2[c];
+            R rt2h = (rt2j + rt2k) * R(0.5);
+            R drw = dt_s * (w.R_rhow[cj] - rt2h * buoy + grav);
+            // tridiagonal row j (moist.jl:997-1021), FP64
+            const double dzj = double(DZ(j)), dzk = double(DZ(k));
+            const double zz = double(zzR);
+            if (j == 0) thh_j = double(f2h(TH(0), TH(1), DZ(0), DZ(1)));
+            double thh_jp1 = 0.0;
+            if (j < n - 1) thh_jp1 = double(f2h(th, TH(k + 1), DZ(k), DZ(k + 1)));
+            double b = dt_tp2 * (double(rt2j) / dzj + double(rt2k) / dzk) * thh_j / zz;
+            double a = 0.0, cc = 0.0;
+            if (j > 0) a = -dt_tp2 * (double(rt2j) * thh_jm1 / (zz * dzj) - G / (dzj + dzk));
+            if (j < n - 1) cc = -dt_tp2 * (double(rt2k) * thh_jp1 / (zz * dzk) + G / (dzj + dzk));
+            const double rwj = double(w.rwp[cj]);
+            double d = rwj + double(drw);
+            if (j > 0) d -= a * fact * rwj;
+            d -= b * fact * rwj;
+            b += 1.0;
+            if (j < n - 1) d -= cc * fact * double(w.rwp[at(j + 1)]);
+            double cpj, dpj;
+            if (j == 0) {
+                cpj = cc / b;
+                dpj = d / b;
+            } else {
+                double temp = b - a * cprev;
+                cpj = cc / temp;
+                dpj = (d - a * dprev) / temp;
+            }
+            w.cpr[cj] = cpj;
+            w.dpr[cj] = dpj;
+            cprev = cpj;
+            dprev = dpj;
+            thh_jm1 = thh_j;
+            thh_j = thh_jp1;
+        }
+        rp_tp_prev = rp_tp;
+        rtp_tp_prev = rtp_tp;
+    }
+
+    // ---- pass 2 (descending): back substitution and vertical updates
+    auto vert = [&](int k, R wkp, R wkm) {
+        const int64_t c = at(k);
+        R fkp = k == ke ? R(0.0) : wkp;
+        R fkm = k == 0 ? R(0.0) : wkm;
+        R adv = -(fkp - fkm) / DZ(k);
+        w.rp[c] = w.rp[c] + dt_s * adv;
+        R Fkp, Fkm;
+        if (k == ke) Fkp = R(0.0);
+        else {
+            if (wkp > EPS && k > 0) Fkp = wkp * upw3(TH(k - 1), TH(k), TH(k + 1)) / R(6.0);
+            else if (in.f[RHOW][c] < -EPS && k < ke - 1) Fkp = wkp * upw3(TH(k + 2), TH(k + 1), TH(k)) / R(6.0);
+            else Fkp = wkp * f2h(TH(k), TH(k + 1), DZ(k), DZ(k + 1));
+        }
+        if (k == 0) Fkm = R(0.0);
+        else {
+            R sw = in.f[RHOW][c - nx];
+            if (sw > EPS && k > 1) Fkm = wkm * upw3(TH(k - 2), TH(k - 1), TH(k)) / R(6.0);
+            else if (sw < -EPS && k < ke) Fkm = wkm * upw3(TH(k + 1), TH(k), TH(k - 1)) / R(6.0);
+            else Fkm = wkm * f2h(TH(k - 1), TH(k), DZ(k - 1), DZ(k));
+        }
+        adv = -(Fkp - Fkm) / DZ(k);
+        rtp_out[c] = rtp_out[c] + dt_s * adv;
+    };
+    R x_next = R(0.0);
+    double xd_next = 0.0;
+    for (int j = n - 1; j >= 0; --j) {
+        const int64_t cj = at(j);
+        double xd = (j == n - 1) ? w.dpr[cj] : w.dpr[cj] - w.cpr[cj] * xd_next;
+        const R x = R(xd);
+        w.rwp[cj] = x;
+        const int k = j + 1;
+        vert(k, x_next, x);
+        if (k <= nz - 2) w.mom_z[at(k)] = w.mom_z[at(k)] + x_next / R(nsub);
+        x_next = x;
+        xd_next = double(x);
+    }
+    vert(0, x_next, R(0.0));
+    if (nz >= 2) w.mom_z[at(0)] = w.mom_z[at(0)] + x_next / R(nsub);
+}
+
+// =============================================================================
+// Koren flux at x-face ii+1/2 (compute_limited_flux_x, moist.jl:850-892).
+// =============================================================================
+template <class S>
+__device__ __forceinline__ S flux_x(const S *q, const SSet<S> &in, const ZVec &zv, const Dom &D, int64_t row,
+                                    int k, int ii, S rhou_face) {
+    using R = S;
+    const R EPS = K<R>::EPS;
+    const int nx = D.nx;
+    auto Q = [&](int jj) -> R {
+        const int64_t c = row + jj;
+        return q[c] / full_rho(in, c, zv, D, k);
+    };
+    const int i0 = wrapx(ii, nx), i1 = wrapx(ii + 1, nx);
+    R s_i = Q(i0), s_ip1 = Q(i1), s_face;
+    if (rhou_face > EPS) {
+        R s_im1 = Q(wrapx(ii - 1, nx));
+        R r_num = s_ip1 - s_i, r_den = s_i - s_im1;
+        R r = m_fabs(r_den) < EPS ? R(1.0) : r_num / r_den;
+        s_face = s_i + R(0.5) * koren(r) * r_den;
+    } else if (rhou_face < -EPS) {
+        R s_ip2 = Q(wrapx(ii + 2, nx));
+        R r_num = s_i - s_ip1, r_den = s_ip1 - s_ip2;
+        R r = m_fabs(r_den) < EPS ? R(1.0) : r_num / r_den;
+        s_face = s_ip1 + R(0.5) * koren(r) * r_den;
+    } else {
+        s_face = R(0.5) * (s_i + s_ip1);
+    }
+    return s_face * rhou_face;
+}
+
+// Koren flux at z-face kk+1/2 of column i (compute_limited_flux_z,
+// moist.jl:895-945), including the positivity limiters that read the first
+// halo column through a linear index and divide by the large-step dt.
+template <class S>
+__device__ __forceinline__ S flux_z(const S *q, const SSet<S> &in, const ZVec &zv, const Dom &D, int kk, int i,
+                                    S rhow_face) {
+    using R = S;
+    const R EPS = K<R>::EPS;
+    const int nx = D.nx, ke = D.nz - 1;
+    const double *dzd = zv.dz + D.offz;
+    auto Q = [&](int k2) -> R {
+        const int64_t c = D.off2 + (int64_t)k2 * nx + i;
+        return q[c] / full_rho(in, c, zv, D, k2);
+    };
+    auto LIN = [&](int k2) -> R { return q[D.off2 + (int64_t)k2 * nx + D.lin_col]; };
+    const R dt = R(D.dt);
+    R s_k = Q(kk), s_kp1 = Q(kk + 1), s_face;
+    if (rhow_face > EPS) {
+        if (kk == 0) {
+            s_face = f2h(s_k, s_kp1, R(dzd[kk]), R(dzd[kk + 1]));
+            s_face = jmin(s_face, LIN(kk) / (rhow_face * dt));
+        } else {
+            R s_km1 = Q(kk - 1);
+            R r_num = s_kp1 - s_k, r_den = s_k - s_km1;
+            R r = m_fabs(r_den) < EPS ? R(1.0) : r_num / r_den;
+            s_face = s_k + R(0.5) * koren(r) * r_den;
+        }
+    } else if (rhow_face < -EPS) {
+        if (kk == ke - 1) {
+            s_face = f2h(s_k, s_kp1, R(dzd[kk]), R(dzd[kk + 1]));
+            s_face = jmin(s_face, LIN(kk + 1) / (-rhow_face * dt));
+        } else {
+            R s_kp2 = Q(kk + 2);
+            R r_num = s_k - s_kp1, r_den = s_kp1 - s_kp2;
+            R r = m_fabs(r_den) < EPS ? R(1.0) : r_num / r_den;
+            s_face = s_kp1 - R(0.5) * koren(r) * r_den;
+        }
+    } else {
+        s_face = f2h(s_k, s_kp1, R(dzd[kk]), R(dzd[kk + 1]));
+    }
+    return s_face * rhow_face;
+}
+
+// =============================================================================
+// Scalar transport (moist.jl:1469-1518, read from the stage-input snapshot)
+// and the final update (moist.jl:1522-1539), with the NaN checks
+// (moist.jl:1543-1564).  Writes the stage output set.
+// =============================================================================
+template <class S>
+__global__ void __launch_bounds__(256) k_scalar(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
+                                                ZVec zv, SSet<S> in, SSet<S> s0, SSet<S> out, Work<S> w,
+                                                const S *__restrict__ rtp_fin, int div, double dt_ovr,
+                                                int stage) {
+    using R = S;
+    const int4 t = tiles[blockIdx.x];
+    const int m = t.x;
+    const Dom &D = doms[m];
+    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+    const int nx = D.nx, nz = D.nz, ke = nz - 1;
+    if (i >= nx || k >= nz) return;
+    const int64_t row = D.off2 + (int64_t)k * nx;
+    const int64_t c = row + i;
+    const int im = wrapx(i - 1, nx), ip = wrapx(i + 1, nx);
+    const double *zd = zv.z + D.offz, *dzd = zv.dz + D.offz;
+    const R dx = R(D.dx), kap = R(D.kappa);
+    const R dt = stage_dt<R>(D, div, dt_ovr);
+    auto RHOF = [&](int kk, int ii) -> R { return full_rho(in, D.off2 + (int64_t)kk * nx + ii, zv, D, kk); };
+    auto DZ = [&](int kk) -> R { return R(dzd[kk]); };
+    const R mxp = w.mom_x[c], mxm = w.mom_x[row + im];
+    const R mzp = k < ke ? w.mom_z[c] : R(0.0);
+    const R mzm = k > 0 ? w.mom_z[c - nx] : R(0.0);
+    const R rc = RHOF(k, i), rip = RHOF(k, ip), rim = RHOF(k, im);
+    const R rkp = k < ke ? RHOF(k + 1, i) : R(0.0);
+    const R rkm = k > 0 ? RHOF(k - 1, i) : R(0.0);
+    const S *dph[3] = {w.d_qv, w.d_qc, w.d_qr};
+    int nan_f = 0;
+#pragma unroll
+    for (int sc = 0; sc < 3; ++sc) {
+        const S *q = in.f[QV + sc];
+        R fxp = flux_x(q, in, zv, D, row, k, i, mxp);
+        R fxm = flux_x(q, in, zv, D, row, k, i - 1, mxm);
+        R adv_x = (fxp - fxm) / dx;
+        R fzp = k == ke ? R(0.0) : flux_z(q, in, zv, D, k, i, mzp);
+        R fzm = k == 0 ? R(0.0) : flux_z(q, in, zv, D, k - 1, i, mzm);
+        R adv_z = (fzp - fzm) / DZ(k);
+        R adv = -(adv_x + adv_z);
+        R sc_c = q[c] / rc;
+        R sc_ip = q[row + ip] / rip;
+        R sc_im = q[row + im] / rim;
+        R sc_kp = k == ke ? R(0.0) : q[c + nx] / rkp;
+        R sc_km = k == 0 ? R(0.0) : q[c - nx] / rkm;
+        R dip = (rip + rc) * R(0.5) * (sc_ip - sc_c) / dx;
+        R dim = (rc + rim) * R(0.5) * (sc_c - sc_im) / dx;
+        R diff_x = kap * (dip - dim) / dx;
+        R dkp = k == ke ? R(0.0) : f2h(rkp, rc, DZ(k + 1), DZ(k)) * (sc_kp - sc_c) / (R(zd[k + 1]) - R(zd[k]));
+        R dkm = k == 0 ? R(0.0) : f2h(rc, rkm, DZ(k), DZ(k - 1)) * (sc_c - sc_km) / (R(zd[k]) - R(zd[k - 1]));
+        R diff_z = kap * (dkp - dkm) / DZ(k);
+        R v = s0.f[QV + sc][c] + dt * (adv + (diff_x + diff_z) + dph[sc][c]);
+        out.f[QV + sc][c] = v;
+        if (isnan(v) && !nan_f) nan_f = 5 + sc;
+    }
+    const R vr = in.f[RHO][c] + w.rp[c];
+    const R vu = in.f[RHOU][c] + w.rup[c];
+    const R vt = in.f[RT][c] + rtp_fin[c];
+    const R vw = k < ke ? in.f[RHOW][c] + w.rwp[c] : in.f[RHOW][c];
+    out.f[RHO][c] = vr;
+    out.f[RHOU][c] = vu;
+    out.f[RT][c] = vt;
+    out.f[RHOW][c] = vw;
+    int f = 0;
+    if (isnan(vr)) f = 1;
+    else if (isnan(vu)) f = 2;
+    else if (isnan(vw)) f = 3;
+    else if (isnan(vt)) f = 4;
+    else f = nan_f;
+    if (f) report_nan(w.nan_key, ((*w.step_ctr * 3 + stage) << 4) | f);
+    // stale halo values surface_fluxes! reads at the next step (moist.jl:723,736)
+    if (k == 0 && i == nx - 1) w.sfh[2 * m + 0] = double(in.f[RHOU][c]);
+    if (k == 0 && i == 0) w.sfh[2 * m + 1] = double(rc);
+}
+
+// =============================================================================
+// Surface fluxes (surface_fluxes!, moist.jl:710-752), one thread per column.
+// =============================================================================
+template <class S>
+__global__ void __launch_bounds__(64) k_surface(const Dom *__restrict__ doms, const int2 *__restrict__ ctiles,
+                                                ZVec zv, SSet<S> s, Work<S> w) {
+    using R = S;
+    using KC = K<R>;
+    const int2 t = ctiles[blockIdx.x];
+    const int m = t.x;
+    const Dom &D = doms[m];
+    const int i = t.y + threadIdx.x;
+    const int nx = D.nx;
+    if (i >= nx || !(D.flags & F_SF)) return;
+    const int64_t c = D.off2 + i;
+    const R rho = full_rho(s, c, zv, D, 0), rt = full_rt(s, c, zv, D, 0);
+    const R qv = s.f[QV][c] / rho, qc = s.f[QC][c] / rho, qr = s.f[QR][c] / rho;
+    const R qd = R(1.0) - qv - qc - qr;
+    const R r = KC::Rd * qd + KC::Rv * qv;
+    const R cp = KC::Cpd * qd + KC::Cpv * qv + KC::Cpl * (qc + qr);
+    const R cv = cp - r;
+    const R ru_left = i == 0 ? R(w.sfh[2 * m + 0]) : s.f[RHOU][c - 1];
+    const R rho_right = i == nx - 1 ? R(w.sfh[2 * m + 1]) : full_rho(s, c + 1, zv, D, 0);
+    R uabs = m_fabs(ru_left + s.f[RHOU][c]) * R(0.5) / rho;
+    uabs = jmax(uabs, R(D.uabs_min));
+    const R theta_ks = rt / rho;
+    const R pi_ks = m_pow(r * rt / KC::P0, r / cv);
+    const R dz0 = R(zv.dz[D.offz]);
+    const R pi_surf = pi_ks + dz0 * R(0.5) * KC::GRAV / (cp * theta_ks);
+    const R p_surf = KC::P0 * m_pow(pi_surf, cp / r);
+    const R Ts = R(D.temp_surf);
+    const R rho_surf = p_surf / (r * Ts);
+    const R theta_surf = Ts / pi_surf;
+    const R u = s.f[RHOU][c] / (rho + rho_right) * R(2.0);
+    const R mflux = -rho_surf * R(D.Cd) * u * m_fabs(u);
+    const R d_u = mflux / dz0;
+    const R sh = rho_surf * R(D.Ch) * uabs * (theta_surf - theta_ks) * pi_surf;
+    R es, des, qs, dq;
+    sat_vp(Ts, es, des);
+    sat_q(p_surf, es, qs, dq);
+    const R lh = rho_surf * R(D.Ce) * uabs * (qs - qv);
+    const int64_t x = D.offx + i;
+    w.mflux[x] = mflux;
+    w.d_u_sf[x] = d_u;
+    w.d_th_sf[x] = sh / (pi_ks * dz0);
+    w.shflux[x] = sh * cp;
+    w.d_qv_sf[x] = lh / dz0;
+    w.lhflux[x] = lh * KC::Lv;
+    if (isnan(d_u)) report_nan(w.nan_key, ((*w.step_ctr * 3) << 4) | 8);
+}
+
+// =============================================================================
+// Gray long-wave radiation (gray_radiation!, moist.jl:776-840), one thread
+// per column.  The upward flux is parked in d_th_rd during the sweep up and
+// replaced by the tendency on the sweep down.
+// =============================================================================
+template <class S>
+__global__ void __launch_bounds__(64) k_radiation(const Dom *__restrict__ doms, const int2 *__restrict__ ctiles,
+                                                  ZVec zv, SSet<S> s, Work<S> w) {
+    using R = S;
+    using KC = K<R>;
+    const int2 t = ctiles[blockIdx.x];
+    const Dom &D = doms[t.x];
+    const int i = t.y + threadIdx.x;
+    const int nx = D.nx, nz = D.nz;
+    if (i >= nx || !(D.flags & F_RD)) return;
+    const double *zd = zv.z + D.offz, *dzd = zv.dz + D.offz;
+    const double *th_ref = w.theta_ref + D.offz;
+    auto cellv = [&](int k, R &eps, R &tau, R &T4, R &cpk, R &pex, R &rho, R &rt) {
+        const int64_t c = D.off2 + (int64_t)k * nx + i;
+        rho = full_rho(s, c, zv, D, k);
+        rt = full_rt(s, c, zv, D, k);
+        const R qv = s.f[QV][c] / rho, qc = s.f[QC][c] / rho, qr = s.f[QR][c] / rho;
+        const R qd = R(1.0) - qv - qc - qr;
+        const R r = KC::Rd * qd + KC::Rv * qv;
+        cpk = KC::Cpd * qd + KC::Cpv * qv + KC::Cpl * (qc + qr);
+        const R cv = cpk - r;
+        pex = m_pow(r * rt / KC::P0, r / cv);
+        const R temp = rt / rho * pex;
+        const R dzk = R(dzd[k]);
+        const R dopt = R(D.k_abs_qv) * s.f[QV][c] * dzk + R(D.k_abs_qc) * s.f[QC][c] * dzk +
+                       R(D.k_abs_qr) * s.f[QR][c] * dzk;
+        eps = R(1.0) - m_exp(-dopt);
+        tau = R(1.0) - eps;
+        T4 = temp * temp * temp * temp;
+    };
+    const R Ts = R(D.temp_surf);
+    const R up0 = R(D.surf_emissivity) * KC::SIGMA * (Ts * Ts * Ts * Ts);
+    R up = up0;
+    for (int k = 0; k < nz; ++k) {
+        R eps, tau, T4, cpk, pex, rho, rt;
+        cellv(k, eps, tau, T4, cpk, pex, rho, rt);
+        up = up * tau + eps * KC::SIGMA * T4;
+        w.d_th_rd[D.off2 + (int64_t)k * nx + i] = up;  // LW_up at face k+1
+    }
+    R dn_above = R(0.0);  // LW_dn[k+1]
+    const R dz_trans = R(2.0e3);
+    for (int k = nz - 1; k >= 0; --k) {
+        R eps, tau, T4, cpk, pex, rho, rt;
+        cellv(k, eps, tau, T4, cpk, pex, rho, rt);
+        const R dn = dn_above * tau + eps * KC::SIGMA * T4;
+        const int64_t c = D.off2 + (int64_t)k * nx + i;
+        const R up_kp = w.d_th_rd[c];
+        const R up_k = k == 0 ? up0 : w.d_th_rd[c - nx];
+        const R divF = ((up_kp - dn_above) - (up_k - dn)) / R(dzd[k]);
+        const R hr = -divF / (cpk * pex);
+        const R alpha = R(0.5) * (R(1.0) + m_tanh((R(zd[k]) - (R(D.z_tropopose) + dz_trans * R(0.5))) / dz_trans)) /
+                        R(D.tau_cooling);
+        const R cr = -(rt - rho * R(th_ref[k])) * alpha;
+        w.d_th_rd[c] = hr + cr;
+        dn_above = dn;
+    }
+}
+
+// =============================================================================
+// Kessler microphysics (cloud_microphysics!, moist.jl:554-671), rain
+// sedimentation (rain_sedimentation!, moist.jl:674-707) and the physics
+// source terms (moist.jl:1632-1644); accumulates surface rain
+// (moist.jl:1680-1682).  One thread per cell of the step-start state.
+// =============================================================================
+template <class S>
+__device__ __forceinline__ S term_velocity(S rho_qr, S rho) {
+    using R = S;
+    const R eff = jmax(R(0.0), rho_qr);
+    const R v = K<R>::A_term * m_pow(eff / rho, K<R>::B_term);
+    return eff > R(1e-12) ? v : R(0.0);
+}
+
+template <class S>
+__global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
+                                                 ZVec zv, SSet<S> s, Work<S> w, int store_diag) {
+    using R = S;
+    using KC = K<R>;
+    const int4 t = tiles[blockIdx.x];
+    const Dom &D = doms[t.x];
+    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+    const int nx = D.nx, nz = D.nz, ke = nz - 1;
+    if (i >= nx || k >= nz) return;
+    const int64_t c = D.off2 + (int64_t)k * nx + i;
+    const R dt = R(D.dt);
+    const int fl = D.flags;
+    const R rho_k = full_rho(s, c, zv, D, k);
+    const R rt = full_rt(s, c, zv, D, k);
+    R dqv = R(0.0), dqc = R(0.0), dqr = R(0.0), dth = R(0.0);
+    if (fl & F_MP) {
+        const R rinv = R(1.0) / rho_k;
+        R theta_k = rt * rinv;
+        R qv = s.f[QV][c] * rinv, qc = s.f[QC][c] * rinv, qr = s.f[QR][c] * rinv;
+        const R qd = R(1.0) - qv - qc - qr;
+        R r = KC::Rd * qd + KC::Rv * qv;
+        R cp = KC::Cpd * qd + KC::Cpv * qv + KC::Cpl * (qc + qr);
+        R cv = cp - r;
+        R Pi = m_pow(r * rt / KC::P0, r / cv);
+        R Tk = theta_k * Pi;
+        R pk = KC::P0 * m_pow(Pi, cp / r);
+        R lhe = KC::Lv / (cp * Pi);
+        const R qv_init = qv;
+        R qsat = R(0.0);
+        for (int it = 0; it < 3; ++it) {
+            R es, des, dqs_des;
+            sat_vp(Tk, es, des);
+            sat_q(pk, es, qsat, dqs_des);
+            const R dqs_dT = dqs_des * des;
+            R ce = (qv - qsat) / (R(1.0) + (KC::Lv / cp) * dqs_dT);
+            ce = jmin(ce, qv);
+            ce = jmax(ce, -qc);
+            qv -= ce;
+            qc += ce;
+            theta_k += ce * lhe;
+            r = KC::Rd * qd + KC::Rv * qv;
+            cp = KC::Cpd * qd + KC::Cpv * qv + KC::Cpl * (qc + qr);
+            cv = cp - r;
+            Pi = m_pow(r * rho_k * theta_k / KC::P0, r / cv);
+            Tk = theta_k * Pi;
+            pk = KC::P0 * m_pow(Pi, cp / r);
+            lhe = KC::Lv / (cp * Pi);
+        }
+        const R ce = qv_init - qv;
+        const R rate = ce * rho_k / dt;
+        dqv -= rate;
+        dqc += rate;
+        dth += rate * lhe;
+        if (fl & F_AUTO) {
+            R ra = R(D.K_auto) * jmax(R(0.0), qc - R(D.Qc_threshold));
+            ra = jmin(ra, qc / dt);
+            qc -= ra * dt;
+            qr += ra * dt;
+            ra *= rho_k;
+            dqc -= ra;
+            dqr += ra;
+        }
+        if (fl & F_ACCR) {
+            R ra = R(0.0);
+            if (qr > R(1e-12) && qc > R(1e-12)) {
+                ra = R(D.K_accr) * qc * qr;
+                ra = jmin(ra, qc / dt);
+            }
+            qc -= ra * dt;
+            qr += ra * dt;
+            ra *= rho_k;
+            dqc -= ra;
+            dqr += ra;
+        }
+        if (fl & F_REVAP) {
+            R re = R(0.0);
+            if (qr > R(1e-12) && qv < qsat) {
+                re = KC::K_evap * (qsat - qv) * m_sqrt(rho_k * qr);
+                re = jmin(re, qr) / dt;
+            }
+            re *= rho_k;
+            dqr -= re;
+            dqv += re;
+            r = KC::Rd * qd + KC::Rv * qv;
+            cp = KC::Cpd * qd + KC::Cpv * qv + KC::Cpl * (qc + qr);
+            cv = cp - r;
+            Pi = m_pow(r * rho_k * theta_k / KC::P0, r / cv);
+            lhe = KC::Lv / (cp * Pi);
+            dth -= re * lhe;
+        }
+    }
+    R dsed = R(0.0);
+    if ((fl & F_MP) && (fl & F_SED)) {
+        const R vt_k = term_velocity(s.f[QR][c], rho_k);
+        const R fkm = s.f[QR][c] * vt_k;
+        R fkp = R(0.0);
+        if (k < ke) {
+            const int64_t cu = c + nx;
+            fkp = s.f[QR][cu] * term_velocity(s.f[QR][cu], full_rho(s, cu, zv, D, k + 1));
+        }
+        dsed = (fkp - fkm) / R(zv.dz[D.offz + k]);
+        if (k == 0) {
+            const int64_t x = D.offx + i;
+            if constexpr (is_mixed<S>()) w.prec[x] += double(fkm) * D.dt;
+            else w.prec[x] += fkm * dt;
+        }
+    }
+    const R th_rd = (fl & F_RD) ? w.d_th_rd[c] : R(0.0);
+    const bool sf = (k == 0) && (fl & F_SF);
+    const int64_t x = D.offx + i;
+    R d_rho = dsed;
+    if (sf) d_rho += w.d_qv_sf[x];
+    R d_rt = dth + th_rd + d_rho * rt / rho_k;
+    if (sf) d_rt += w.d_th_sf[x];
+    R d_qv = dqv;
+    if (sf) d_qv += w.d_qv_sf[x];
+    const R d_qr = dqr + dsed;
+    w.d_rho[c] = d_rho;
+    w.d_rt[c] = d_rt;
+    w.d_qv[c] = d_qv;
+    w.d_qc[c] = dqc;
+    w.d_qr[c] = d_qr;
+    if (store_diag) {
+        w.dg_th_mp[c] = dth;
+        w.dg_qv_mp[c] = dqv;
+        w.dg_qr_mp[c] = dqr;
+        w.dg_qr_sed[c] = dsed;
+    }
+    int f = 0;
+    if (isnan(d_rt)) f = 9;
+    else if (isnan(d_qv)) f = 10;
+    else if (isnan(dqc)) f = 11;
+    else if (isnan(d_qr)) f = 12;
+    if (f) report_nan(w.nan_key, ((*w.step_ctr * 3) << 4) | f);
+}
+
+__global__ void k_step_done(int *step_ctr) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step_ctr += 1;
+}
+
+// =============================================================================
+// Layout conversion between the host State arrays ((ka, ia) column-major,
+// x halos) and the device pools.
+// =============================================================================
+template <class S, class H>
+__global__ void k_upload(Dom D, const H *__restrict__ host, S *__restrict__ dst, const double *__restrict__ base,
+                         int halo, int ka) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (i >= D.nx || k >= D.nz) return;
+    double v = double(host[(int64_t)(i + halo) * ka + k]);
+    if (base) v -= base[k];
+    dst[D.off2 + (int64_t)k * D.nx + i] = S(v);
+}
+
+// Writes host-layout columns: interior from `src`; halo columns from the
+// periodic partner column of `halo_src` (when given).
+template <class S, class H>
+__global__ void k_download(Dom D, const S *__restrict__ src, const S *__restrict__ halo_src, H *__restrict__ host,
+                           const double *__restrict__ base, int halo, int ka) {
+    const int ih = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    const int ia = D.nx + 2 * halo;
+    if (ih >= ia || k >= D.nz) return;
+    const int i = ih - halo;
+    const S *from;
+    int j;
+    if (i >= 0 && i < D.nx) {
+        from = src;
+        j = i;
+    } else {
+        if (!halo_src) return;
+        from = halo_src;
+        j = ((i % D.nx) + D.nx) % D.nx;
+    }
+    double v = double(from[D.off2 + (int64_t)k * D.nx + j]);
+    if (base) v += base[k];
+    host[(int64_t)ih * ka + k] = H(v);
+}
+
+// FP64 reductions of one member: dry mass, total water, rain, mass.
+template <class S>
+__global__ void __launch_bounds__(256) k_diagnostics(Dom D, ZVec zv, SSet<S> s, const double *__restrict__ prec,
+                                                     double *__restrict__ out) {
+    __shared__ double red[4][256];
+    const int tid = threadIdx.x;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int64_t ncell = (int64_t)D.nx * D.nz;
+    for (int64_t q = tid; q < ncell; q += blockDim.x) {
+        const int k = int(q / D.nx);
+        const int64_t c = D.off2 + q;
+        double rho = double(s.f[RHO][c]);
+        if (is_mixed<S>()) rho += zv.rho_bar[D.offz + k];
+        const double wat = double(s.f[QV][c]) + double(s.f[QC][c]) + double(s.f[QR][c]);
+        const double dz = zv.dz[D.offz + k];
+        acc[0] += (rho - wat) * dz;
+        acc[1] += wat * dz;
+        acc[3] += rho * dz;
+    }
+    for (int i = tid; i < D.nx; i += blockDim.x) acc[2] += prec[D.offx + i];
+    for (int a = 0; a < 4; ++a) red[a][tid] = acc[a];
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if (tid < st)
+            for (int a = 0; a < 4; ++a) red[a][tid] += red[a][tid + st];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        out[0] = red[0][0] * D.dx;
+        out[1] = red[1][0] * D.dx;
+        out[2] = red[2][0] * D.dx;
+        out[3] = red[3][0] * D.dx;
+    }
+}
+
+}  // namespace mo

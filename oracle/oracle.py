@@ -1,0 +1,197 @@
+"""ctypes front end of the CPU oracle (liboracle.so).  TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module; it is the checker, never the thing measured or shipped.  The
+functions mirror the reference's routines one to one (moist.jl line ranges
+in moist_oracle.cpp) and take host State/Params objects of moist2d_amd.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+ORDER_SNAPSHOT = 0
+ORDER_SERIAL = 1
+
+NAN_FIELDS = ("rho", "rho_u", "rho_w", "rho_theta", "rho_qv", "rho_qc", "rho_qr", "d_rho_u_phys",
+              "d_rho_theta_phys", "d_rho_qv_phys", "d_rho_qc_phys", "d_rho_qr_phys")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        for sfx, ct in (("f64", ctypes.c_double), ("f32", ctypes.c_float)):
+            P = ctypes.POINTER(ct)
+            for name, res, args in (
+                ("f2h", ct, [ct, ct, ct, ct]),
+                ("koren", ct, [ct]),
+                ("sat_vp", None, [ct, P, P]),
+                ("sat_q", None, [ct, ct, P, P]),
+                ("thomas", None, [ctypes.c_int, P, P, P, P]),
+                ("implicit_correction", None, [ctypes.c_void_p, P, P, P, P, ct]),
+                ("flux_x", ct, [ctypes.c_void_p, P, P, P, ctypes.c_int, ctypes.c_int]),
+                ("flux_z", ct, [ctypes.c_void_p, P, P, P, ctypes.c_int, ctypes.c_int]),
+                ("exchange_halo", None, [ctypes.c_void_p, P]),
+                ("cloud_microphysics", None, [ctypes.c_void_p] + [P] * 9),
+                ("rain_sedimentation", None, [ctypes.c_void_p, P, P, P, P]),
+                ("surface_fluxes", None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+                ("gray_radiation", None, [ctypes.c_void_p, P, ctypes.c_void_p, P]),
+                ("compute_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                ctypes.c_void_p, ct, ctypes.c_int, ctypes.c_int]),
+                ("rk3_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+            ):
+                f = getattr(_lib, f"orc_{name}_{sfx}")
+                f.restype = res
+                f.argtypes = args
+        _lib.orc_set_threads.restype = ctypes.c_int
+        _lib.orc_set_threads.argtypes = [ctypes.c_int]
+    return _lib
+
+
+def _sfx(T):
+    T = np.dtype(T)
+    if T == np.float64:
+        return "f64", ctypes.c_double
+    if T == np.float32:
+        return "f32", ctypes.c_float
+    raise TypeError(T)
+
+
+def _ptr(a, ct):
+    assert a.flags["F_CONTIGUOUS"] or a.ndim == 1
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+def _fn(name, T):
+    sfx, ct = _sfx(T)
+    return getattr(lib(), f"orc_{name}_{sfx}"), ct
+
+
+def set_threads(n: int) -> int:
+    return lib().orc_set_threads(int(n))
+
+
+# ---- scalar helpers ---------------------------------------------------------
+def f2h(a, b, c, d, T=np.float64):
+    f, ct = _fn("f2h", T)
+    return np.dtype(T).type(f(a, b, c, d))
+
+
+def koren_limiter(r, T=np.float64):
+    f, ct = _fn("koren", T)
+    return np.dtype(T).type(f(r))
+
+
+def saturation_vapor_pressure(Tk, T=np.float64):
+    f, ct = _fn("sat_vp", T)
+    es, des = ct(), ct()
+    f(Tk, ctypes.byref(es), ctypes.byref(des))
+    return np.dtype(T).type(es.value), np.dtype(T).type(des.value)
+
+
+def saturation_specific_humidity(p, es, T=np.float64):
+    f, ct = _fn("sat_q", T)
+    q, dq = ct(), ct()
+    f(p, es, ctypes.byref(q), ctypes.byref(dq))
+    return np.dtype(T).type(q.value), np.dtype(T).type(dq.value)
+
+
+def solve_tridiagonal(a, b, c, d):
+    """solve_tridiagonal!(a, b, c, d): modifies c and d in place."""
+    f, ct = _fn("thomas", d.dtype)
+    f(len(b), _ptr(a, ct), _ptr(b, ct), _ptr(c, ct), _ptr(d, ct))
+
+
+def implicit_correction(rho_w, drho_w, theta, rt2pres, dt, p):
+    f, ct = _fn("implicit_correction", rho_w.dtype)
+    cp = p.to_c()
+    f(ctypes.byref(cp), _ptr(rho_w, ct), _ptr(drho_w, ct), _ptr(theta, ct), _ptr(rt2pres, ct), dt)
+
+
+def compute_limited_flux_x(scalar_rho, rho, rho_u, k, i, p):
+    f, ct = _fn("flux_x", rho.dtype)
+    cp = p.to_c()
+    return f(ctypes.byref(cp), _ptr(scalar_rho, ct), _ptr(rho, ct), _ptr(rho_u, ct), k, i)
+
+
+def compute_limited_flux_z(scalar_rho, rho, rho_w, k, i, p):
+    f, ct = _fn("flux_z", rho.dtype)
+    cp = p.to_c()
+    return f(ctypes.byref(cp), _ptr(scalar_rho, ct), _ptr(rho, ct), _ptr(rho_w, ct), k, i)
+
+
+def exchange_halo(A, p):
+    f, ct = _fn("exchange_halo", A.dtype)
+    cp = p.to_c()
+    f(ctypes.byref(cp), _ptr(A, ct))
+
+
+def cloud_microphysics(d_qv, d_qc, d_qr, d_th, rho, rho_theta, rho_qv, rho_qc, rho_qr, p):
+    f, ct = _fn("cloud_microphysics", rho.dtype)
+    cp = p.to_c()
+    f(ctypes.byref(cp), *[_ptr(a, ct) for a in (d_qv, d_qc, d_qr, d_th, rho, rho_theta, rho_qv, rho_qc, rho_qr)])
+
+
+def rain_sedimentation(d_sed, d_prec, rho_qr, rho, p):
+    f, ct = _fn("rain_sedimentation", rho.dtype)
+    cp = p.to_c()
+    f(ctypes.byref(cp), *[_ptr(a, ct) for a in (d_sed, d_prec, rho_qr, rho)])
+
+
+def _ptr_array(arrs, ct):
+    arr = (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    return arr
+
+
+def surface_fluxes(out, s, p):
+    """out = (d_rho_u_sf, d_rho_theta_sf, d_rho_qv_sf, mflux, shflux, lhflux)."""
+    f, ct = _fn("surface_fluxes", s.rho.dtype)
+    cp = p.to_c()
+    o = _ptr_array(out, ct)
+    ss = _ptr_array([getattr(s, n) for n in ("rho", "rho_u", "rho_w", "rho_theta", "rho_qv", "rho_qc", "rho_qr")], ct)
+    f(ctypes.byref(cp), o, ss)
+
+
+def gray_radiation(d_rd, s, p):
+    f, ct = _fn("gray_radiation", s.rho.dtype)
+    cp = p.to_c()
+    ss = _ptr_array([getattr(s, n) for n in ("rho", "rho_u", "rho_w", "rho_theta", "rho_qv", "rho_qc", "rho_qr")], ct)
+    f(ctypes.byref(cp), _ptr(d_rd, ct), ss, _ptr(s.theta_ref, ct))
+
+
+_PROG = ("rho", "rho_u", "rho_w", "rho_theta", "rho_qv", "rho_qc", "rho_qr")
+
+
+def compute_step(phys, s, s0, p, dt, ns, order=ORDER_SNAPSHOT):
+    """compute_step!(phys..., s, s0, p, dt, ns); returns the NaN code (0 = none)."""
+    f, ct = _fn("compute_step", s.rho.dtype)
+    cp = p.to_c()
+    ph = _ptr_array(list(phys), ct)
+    a = _ptr_array([getattr(s, n) for n in _PROG], ct)
+    b = _ptr_array([getattr(s0, n) for n in _PROG], ct)
+    return f(ctypes.byref(cp), ph, a, b, dt, ns, order)
+
+
+def rk3_step(s, p, nsteps=1, order=ORDER_SNAPSHOT):
+    """nsteps x rk3_step!(s, p); returns the NaN code of the failing stage (0 = none)."""
+    f, ct = _fn("rk3_step", s.rho.dtype)
+    cp = p.to_c()
+    arrs = s.arrays()
+    for a in arrs:
+        assert a.dtype == s.rho.dtype and (a.flags["F_CONTIGUOUS"])
+    return f(ctypes.byref(cp), _ptr_array(arrs, ct), nsteps, order)
