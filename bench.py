@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Benchmark: cell-steps/s of the 8192x256 cold-pool run on one MI355X.
+
+Prints one JSON line.  `value` is the mixed form (FP32 prognostic deviations,
+FP64 tridiagonal/base state/accumulators); the FP64 form of the same run is
+timed beside it (`fp64_form`), and the mixed form's error against the FP64
+form after the timed steps is stated (`mixed_vs_fp64`; the FP64 form itself
+matches the CPU oracle to <= 1e-11 per field in the parity tests).
+
+Timing: W warm-up steps, then exactly K large steps between two HIP events on
+the handle's stream with the state resident in HBM (the C ABI synchronizes the
+stream before returning).  The roofline object is for the acoustic column
+kernel, timed live with a HIP event pair around each of its launches in a
+separate pass of K steps.  `cpu_baseline` times the CPU oracle (FP64,
+snapshot order, OpenMP over columns) on a bounded sample of the same
+workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from moist2d_amd import core, ic  # noqa: E402
+from moist2d_amd.state import PROGNOSTIC  # noqa: E402
+
+METRIC = "cell-steps/sec on 1 GPU (FP32 vs FP64 form, stated err) + adjoint/forward time"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes of the acoustic column kernel per cell and launch, in
+# field passes (reads: rho', rho_theta', rho_u' (i-1,i), theta, R_rho,
+# R_rho_theta, R_rho_w, rt2pres, rho_w', s.rho_w, mom_z; writes: rho',
+# rho_theta', rho_w', mom_z).
+ACOUSTIC_COL_PASSES = 15
+
+
+def run_form(p, s, precision, steps, warmup, graph=True):
+    eng = core.Engine(p, precision)
+    eng.store_phys_diag(False)
+    eng.upload(s)
+    eng.use_graph(graph)
+    if warmup:
+        eng.time_steps(warmup)
+    t0 = time.perf_counter()
+    ms = eng.time_steps(steps)
+    wall = time.perf_counter() - t0
+    return eng, ms, wall
+
+
+def kernel_pass(p, s, precision, steps):
+    eng = core.Engine(p, precision)
+    eng.store_phys_diag(False)
+    eng.upload(s)
+    eng.kernel_events(True)
+    eng.time_steps(1)
+    eng.time_steps(steps)
+    ms, n = eng.kernel_stats()
+    eng.close()
+    return ms, n
+
+
+def cpu_baseline(p, s, steps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # the checker, timed here as the CPU baseline only
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = O.set_threads(threads)
+    ref = s.copy()
+    O.rk3_step(ref, p, 1)  # warm-up
+    t0 = time.perf_counter()
+    rc = O.rk3_step(ref, p, steps)
+    dt = time.perf_counter() - t0
+    cells = p.Nx * p.Nz
+    return {"value": cells * steps / dt, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{p.Nx}x{p.Nz} cold pool, {steps} large steps after 1 warm-up step, FP64, snapshot order, "
+                      f"OpenMP over columns ({dt:.1f} s)", "rc": rc}
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a - b))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--nx", type=int, default=8192)
+    ap.add_argument("--nz", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    p, s = ic.coldpool(np.float64, nx=args.nx, nz=args.nz)
+    cells = p.Nx * p.Nz
+    K, W = args.steps, args.warmup
+
+    eng_m, ms_m, wall_m = run_form(p, s, core.MIXED, K, W)
+    mixed = {"cell_steps_per_s": cells * K / (ms_m * 1e-3), "ms_per_step": ms_m / K, "wall_s": wall_m}
+    out_m = s.copy()
+    eng_m.download(out_m, phys=False)
+    diag_m = eng_m.diagnostics()
+    eng_m.close()
+
+    fp64 = None
+    err = None
+    if not args.no_fp64:
+        eng_d, ms_d, wall_d = run_form(p, s, core.FP64, K, W)
+        fp64 = {"value": cells * K / (ms_d * 1e-3), "ms_per_step": ms_d / K, "wall_s": wall_d}
+        out_d = s.copy()
+        eng_d.download(out_d, phys=False)
+        diag_d = eng_d.diagnostics()
+        eng_d.close()
+        is0 = p.is_ - 1
+        sl = np.s_[:, is0:is0 + p.Nx]
+        err = {n: rel_l2(getattr(out_m, n)[sl], getattr(out_d, n)[sl]) for n in PROGNOSTIC}
+        w0 = diag_d["total_water"] + diag_d["rain"]
+        err["total_water_plus_rain_rel_diff"] = (diag_m["total_water"] + diag_m["rain"] - w0) / w0
+        err["steps"] = W + K
+        mixed["speedup_vs_fp64"] = mixed["cell_steps_per_s"] / fp64["value"]
+
+    # dominant kernel: acoustic column kernel, timed live with HIP events
+    kms, nlaunch = kernel_pass(p, s, core.MIXED, min(K, 50))
+    alg_bytes = ACOUSTIC_COL_PASSES * cells * 4
+    achieved = alg_bytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    roof = {"kernel": "k_acoustic_col<float>", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "avg_launch_ms": kms, "launches": nlaunch, "alg_bytes_per_launch": alg_bytes}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(p, s, args.cpu_steps)
+
+    line = {
+        "metric": METRIC,
+        "value": mixed["cell_steps_per_s"],
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": mixed["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 tridiagonal, base state, accumulators)",
+        "data": "synthetic",
+        "config": {"workload": f"coldpool_{p.Nx}x{p.Nz}", "nx": p.Nx, "nz": p.Nz, "dx": float(p.dx),
+                   "dt": float(p.dt), "ns": p.ns, "form": "mixed", "hipgraph": True},
+        "fp64_form": fp64,
+        "mixed_vs_fp64": err,
+        "mixed_wall_s": mixed["wall_s"],
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "adjoint_forward_ratio": None,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

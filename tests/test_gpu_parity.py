@@ -35,3 +35,132 @@ def test_fp64_bubble_10_steps():
     for n, e in errs.items():
         assert e <= TOL_FP64, (n, e)
     np.testing.assert_allclose(out.prec, ref.prec, rtol=1e-11, atol=1e-300)
+
+
+def _run_gpu(p, s, steps, precision=core.FP64, graph=False):
+    eng = core.Engine(p, precision)
+    eng.use_graph(graph)
+    eng.upload(s)
+    eng.step(steps)
+    out = s.copy()
+    eng.download(out)
+    d = eng.diagnostics()
+    eng.close()
+    return out, d
+
+
+def test_mixed_bubble_10_steps_vs_fp64_oracle():
+    p, s = ic.bubble(np.float64)
+    ref = s.copy()
+    assert O.rk3_step(ref, p, 10) == 0
+    out, _ = _run_gpu(p, s, 10, core.MIXED)
+    errs = field_errors(p, out, ref)
+    _print("mixed bubble 10 steps vs FP64 oracle", errs)
+    # stated error of the mixed form against the FP64 oracle
+    for n in ("rho", "rho_theta", "rho_qv"):
+        assert errs[n] <= 1e-6, (n, errs[n])
+    for n in ("rho_u", "rho_w"):
+        assert errs[n] <= 1e-4, (n, errs[n])
+
+
+def test_fp64_compute_step_entry():
+    """compute_step! with non-zero tendencies, s != s0, ns = 3, in a uniform
+    5 m/s wind.  (Two steps from rest |rho_u| <= 6e-3 and the oracle's own
+    rho_u moves by 4e-10 under a 1-ulp input perturbation; with the wind that
+    sensitivity is 4e-15, so the 1e-11 bar measures the kernels.)"""
+    p, s = ic.bubble(np.float64, nx=64, nz=32, d=156.25)
+    s.rho_u[...] = 5.0 * s.rho
+    rng = np.random.default_rng(5)
+    s0 = s.copy()
+    O.rk3_step(s, p, 2)  # s moved away from s0
+    phys = [np.asfortranarray(rng.normal(0, sc, size=s.rho.shape)) for sc in (1e-6, 1e-4, 1e-4, 1e-7, 1e-7, 1e-7)]
+    ref = s.copy()
+    assert O.compute_step(phys, ref, s0, p, 0.5, 3) == 0
+    eng = core.Engine(p, core.FP64)
+    eng.upload(s)
+    eng.compute_step(phys, s0, 0.5, 3)
+    out = s.copy()
+    eng.download(out, phys=False)
+    eng.close()
+    errs = field_errors(p, out, ref)
+    _print("fp64 compute_step", errs)
+    for n, e in errs.items():
+        assert e <= TOL_FP64, (n, e)
+    # halo columns: the reference leaves periodic copies of the stage input
+    for n in ("rho", "rho_u", "rho_w", "rho_qv"):
+        np.testing.assert_array_equal(getattr(out, n)[:, :p.halo], getattr(ref, n)[:, :p.halo])
+
+
+def test_fp64_default_physics_surface_and_radiation():
+    """Default Params switches (surface fluxes and gray radiation on) with the
+    reference's init_state! profile."""
+    from moist2d_amd.params import make_params
+    p = make_params(np.float64, Nx=32, Nz=40, Lx=32 * 500.0, H=20e3, dz0=200.0, z_fact=1.05, ns=3, tout_int=2.0)
+    s = ic.init_state(allocate_state(p), p, seed=7)
+    ref = s.copy()
+    assert O.rk3_step(ref, p, 3) == 0
+    out, _ = _run_gpu(p, s, 3)
+    errs = field_errors(p, out, ref)
+    _print("fp64 sf+rd 3 steps", errs)
+    for n, e in errs.items():
+        assert e <= TOL_FP64, (n, e)
+    for n in ("mflux", "shflux", "lhflux", "d_rho_u_sf", "d_rho_theta_sf", "d_rho_qv_sf"):
+        a, b = getattr(out, n)[p.halo:p.halo + p.Nx], getattr(ref, n)[p.halo:p.halo + p.Nx]
+        assert rel_l2(a, b) <= 1e-11, n
+    for n in ("d_rho_theta_mp", "d_rho_qv_mp", "d_rho_qc_mp", "d_rho_qr_mp", "d_rho_qr_sed", "d_rho_theta_rd"):
+        e = rel_l2(interior(p, getattr(out, n)), interior(p, getattr(ref, n)))
+        assert e <= 1e-10, (n, e)
+
+
+def test_ensemble_members_match_single_runs():
+    members = ic.ensemble(np.float64, members=3, nx_choices=[64, 128, 192], nz_choices=(16, 24), steps=5)
+    eng = core.Engine([p for p, _ in members], core.FP64)
+    for m, (p, s) in enumerate(members):
+        eng.upload(s, member=m)
+    eng.step(3)
+    for m, (p, s) in enumerate(members):
+        ref = s.copy()
+        assert O.rk3_step(ref, p, 3) == 0
+        out = s.copy()
+        eng.download(out, member=m)
+        errs = field_errors(p, out, ref)
+        _print(f"ensemble member {m} ({p.Nx}x{p.Nz})", errs)
+        for n, e in errs.items():
+            assert e <= TOL_FP64, (m, n, e)
+    eng.close()
+
+
+@pytest.mark.parametrize("precision", [core.FP64, core.MIXED])
+def test_graph_replay_matches_direct_and_is_deterministic(precision):
+    p, s = ic.bubble(np.float64, nx=128, nz=32)
+    a, _ = _run_gpu(p, s, 4, precision, graph=False)
+    b, _ = _run_gpu(p, s, 4, precision, graph=True)
+    c, _ = _run_gpu(p, s, 4, precision, graph=True)
+    for n in PROG:
+        np.testing.assert_array_equal(getattr(a, n), getattr(b, n), err_msg=n)
+        np.testing.assert_array_equal(getattr(b, n), getattr(c, n), err_msg=n)
+
+
+def test_nan_is_reported_like_the_reference():
+    p, s = ic.bubble(np.float64, nx=64, nz=16)
+    s.rho[5, p.halo + 3] = np.nan
+    with pytest.raises(RuntimeError, match="NaN detected in rho$"):
+        core.rk3_step(s, p)
+
+
+def test_conservation_coldpool_1024x256():
+    p, s = ic.coldpool(np.float64, nx=1024, nz=256)
+    for prec, tol in ((core.FP64, 1e-13), (core.MIXED, 1e-6)):
+        eng = core.Engine(p, prec)
+        eng.upload(s)
+        d0 = eng.diagnostics()
+        eng.step(40)
+        d1 = eng.diagnostics()
+        eng.close()
+        w0 = d0["total_water"] + d0["rain"]
+        dw = (d1["total_water"] + d1["rain"] - w0) / w0
+        m0 = d0["mass"] + d0["rain"]
+        dm = (d1["mass"] + d1["rain"] - m0) / m0
+        dd = (d1["dry_mass"] - d0["dry_mass"]) / d0["dry_mass"]
+        print(f"conservation prec={prec}: water+rain {dw:.2e} mass+rain {dm:.2e} dry {dd:.2e}", flush=True)
+        assert abs(dw) <= tol and abs(dm) <= tol and abs(dd) <= tol
