@@ -180,7 +180,7 @@ def _workload_params(T, nx, nz, d, dt, ns, steps, **kw):
 
 
 def bubble(T=np.float64, nx=128, nz=64, d=156.25, dt=1.0, ns=6, steps=600, amp=2.0, radius=2000.0,
-           xc=None, zc=2000.0, noise=0.01, seed=0, member=0, rh_low=0.90, z_low=2000.0, **kw):
+           xc=None, zc=2000.0, noise=0.01, seed=0, member=0, rh_low=0.90, z_low=2000.0, rh_top=0.20, **kw):
     """Moist thermal bubble (the 128x64 workload by default)."""
     p = _workload_params(T, nx, nz, d, dt, ns, steps, **kw)
     X, Z, I, K = _grid(p)
@@ -188,7 +188,7 @@ def bubble(T=np.float64, nx=128, nz=64, d=156.25, dt=1.0, ns=6, steps=600, amp=2
     th_p = _cos2_bump(X, Z, xc, zc, radius, radius, amp)
     th_p = th_p + noise * (2.0 * rng_uniform(seed, member, 0, I, K) - 1.0)
     z = p.z.astype(np.float64)
-    s = _fill_state(p, theta_profile(z), th_p, _rh_profile(z, rh_low, z_low))
+    s = _fill_state(p, theta_profile(z), th_p, _rh_profile(z, rh_low, z_low, rh_top=rh_top))
     return p, s
 
 
