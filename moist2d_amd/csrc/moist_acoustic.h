@@ -1,0 +1,388 @@
+// moist_acoustic.h — one acoustic substep as one kernel (moist.jl:1351-1461).
+//
+// Included by moist_kernels.h inside namespace mo.
+//
+// A workgroup of AC_THREADS threads owns a tile of TX columns and every level
+// of those columns.  Thread t handles column t % TX and chunk t / TX, a run
+// of L consecutive levels (and the faces above them) kept in registers, so
+// one wave reads 64/TX level rows of TX consecutive columns (full 128-byte
+// lines for TX >= 32 in FP32).  nch = AC_THREADS / TX chunks cover nz.
+//
+// Per substep, for every cell of the chunk:
+//   rho_u' at faces i-1/2 and i+1/2 with divergence damping (moist.jl:1360-1369;
+//   the i-1/2 value is recomputed, not read back, so the kernel needs no
+//   grid-wide barrier), its running mean mom_x (moist.jl:1453-1455),
+//   the explicit rho'/rho_theta' increments and their off-centred values
+//   (moist.jl:1381-1396), d rho_w' (moist.jl:1398-1405), the w-p system
+//   (implicit_correction!, moist.jl:974-1030) and the vertical flux updates
+//   (moist.jl:1411-1446), the running mean mom_z (moist.jl:1456-1458).
+//
+// The tridiagonal system is the reference's Thomas algorithm (moist.jl:952-971)
+// in FP64, split over the chunks of a column:
+//   * the LU factor recurrence c'_j = c_j / (b_j - a_j c'_{j-1}) is a Moebius
+//     map of c'_{j-1}; the chunk composites (2x2 matrices) are scanned once per
+//     stage (the matrix does not change between substeps) to give the c'
+//     entering each chunk, stored in `cin`; each substep re-runs the recurrence
+//     inside its chunk from that value;
+//   * forward and back substitution are affine recurrences: each chunk solves
+//     with a zero carry and keeps the carry coefficients, the carries are
+//     scanned across chunks through LDS, and every level is corrected.
+
+constexpr int AC_THREADS = 1024;
+constexpr int AC_L = 8;  // levels per chunk
+
+template <class S> struct AcBuf {
+    const S *rup_in;   // rho_u' before this substep
+    S *rup_out;        // rho_u' after it
+    const S *rtp_cur;  // rho_theta' at the start of the substep
+    const S *rtp_old;  // rho_theta' at the start of the previous substep
+    S *rtp_out;        // rho_theta' after the substep
+    double *cin;       // c' entering each chunk, [chunk][column]
+    int64_t cin_stride;
+};
+
+template <class S, int L>
+__global__ void __launch_bounds__(AC_THREADS) k_acoustic(const Dom *__restrict__ doms, const int4 *__restrict__ atiles,
+                                                         ZVec zv, SSet<S> in, Work<S> w, AcBuf<S> ab, int div,
+                                                         double dt_ovr, int nsub, int first) {
+    using R = S;
+    __shared__ double lds[4 * AC_THREADS];
+    __shared__ S ldt[2 * AC_THREADS];
+
+    const int4 tl = atiles[blockIdx.x];
+    if (tl.x < 0) return;  // padding tile of the XCD ordering (uniform per block)
+    const Dom &D = doms[tl.x];
+    const int TX = tl.z, nch = tl.w;
+    const int t = threadIdx.x;
+    const int c = t / TX, jj = t - c * TX;
+    const int nx = D.nx, nz = D.nz, ke = nz - 1;
+    const int i = tl.y + jj;
+    const bool col_ok = i < nx;
+    const int k0 = c * L;
+    const int nl = (col_ok && k0 < nz) ? min(L, nz - k0) : 0;  // levels owned
+    const int nf = max(0, min(nl, nz - 1 - k0));                // faces owned (j <= nz-2)
+    const int im = col_ok ? wrapx(i - 1, nx) : 0, ip = col_ok ? wrapx(i + 1, nx) : 0;
+    const int64_t base = D.off2;
+    const double *zd = zv.z + D.offz, *dzd = zv.dz + D.offz;
+    const R dx = R(D.dx);
+    const R EPS = K<R>::EPS;
+    const R dt_s = stage_dt<R>(D, div, dt_ovr) / R(nsub);
+    const R gam = R(D.gamma_d);
+    const R beta = R(D.beta);
+    // fact_tp of the explicit part is a Float64 expression (moist.jl:1375)
+    const double fact_tp_d = double(R(0.5)) * (1.0 + double(beta));
+    // implicit_correction!'s own factors, in T (moist.jl:987-989)
+    const R ftp = R(0.5) * (R(1.0) + beta);
+    const R ftm = R(0.5) * (R(1.0) - beta);
+    const double dt_tp2 = double(dt_s * ftp) * double(dt_s * ftp);
+    const double fact = double(ftm / ftp);
+    const double G = K<double>::GRAV;
+    const int64_t cidx = (int64_t)c * ab.cin_stride + D.offx + i;
+    double cin_v = 0.0;
+    if (!first && nl > 0) cin_v = ab.cin[cidx];
+
+    R rp_e[L], rtp_e[L], drw[L], thx[L + 4], rt2c[L + 1];
+    double dd[L], cpv[L], pv[L];
+#pragma unroll
+    for (int q = 0; q < L + 4; ++q) thx[q] = R(0.0);
+#pragma unroll
+    for (int q = 0; q < L + 1; ++q) rt2c[q] = R(0.0);
+
+    // ---- phase 1: rho_u', explicit rho' and rho_theta', d rho_w'
+    R tp_r0 = R(0.0), tp_t0 = R(0.0), tp_rp = R(0.0), tp_tp = R(0.0);
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        rp_e[l] = R(0.0);
+        rtp_e[l] = R(0.0);
+        drw[l] = R(0.0);
+        if (l < nl) {
+            const int k = k0 + l;
+            const int64_t row = base + (int64_t)k * nx;
+            const int64_t cc = row + i, cm = row + im, cx = row + ip;
+            const R cur_c = ab.rtp_cur[cc], cur_m = ab.rtp_cur[cm], cur_p = ab.rtp_cur[cx];
+            const R old_c = ab.rtp_old[cc], old_m = ab.rtp_old[cm], old_p = ab.rtp_old[cx];
+            const R rt2_c = w.rt2[cc], rt2_m = w.rt2[cm], rt2_p = w.rt2[cx];
+            // rho_u' at i+1/2 and i-1/2 (moist.jl:1362-1367)
+            R gx = (cur_p - cur_c) / dx;
+            R gxo = (old_p - old_c) / dx;
+            gx = gx + gam * (gx - gxo);
+            const R ru_i = ab.rup_in[cc] + dt_s * (w.R_rhou[cc] - (rt2_c + rt2_p) * R(0.5) * gx);
+            R gm = (cur_c - cur_m) / dx;
+            R gmo = (old_c - old_m) / dx;
+            gm = gm + gam * (gm - gmo);
+            const R ru_m = ab.rup_in[cm] + dt_s * (w.R_rhou[cm] - (rt2_m + rt2_c) * R(0.5) * gm);
+            ab.rup_out[cc] = ru_i;
+            w.mom_x[cc] = w.mom_x[cc] + ru_i / R(nsub);
+            // explicit rho' (moist.jl:1383-1386)
+            R adv = -(ru_i - ru_m) / dx;
+            R inc = dt_s * (w.R_rho[cc] + adv);
+            const R rpc = w.rp[cc];
+            R rp_tp;
+            if constexpr (is_mixed<S>()) rp_tp = rpc + R(fact_tp_d) * inc;
+            else rp_tp = R(double(rpc) + fact_tp_d * double(inc));
+            rp_e[l] = rpc + inc;
+            // explicit rho_theta' (moist.jl:1389-1394)
+            const R th_c = w.theta[cc];
+            const R fip = (w.theta[cx] + th_c) * R(0.5) * ru_i;
+            const R fim = (th_c + w.theta[cm]) * R(0.5) * ru_m;
+            adv = -(fip - fim) / dx;
+            inc = dt_s * (w.R_rt[cc] + adv);
+            R rtp_tp;
+            if constexpr (is_mixed<S>()) rtp_tp = cur_c + R(fact_tp_d) * inc;
+            else rtp_tp = R(double(cur_c) + fact_tp_d * double(inc));
+            rtp_e[l] = cur_c + inc;
+            thx[l + 2] = th_c;
+            rt2c[l] = rt2_c;
+            if (l == 0) {
+                tp_r0 = rp_tp;
+                tp_t0 = rtp_tp;
+            } else {
+                // d rho_w' at face k-1 (moist.jl:1399-1404)
+                const R buoy = (rtp_tp - tp_tp) / (R(zd[k]) - R(zd[k - 1]));
+                const R grav = -K<R>::GRAV * R(0.5) * (rp_tp + tp_rp);
+                const R rt2h = (rt2c[l - 1] + rt2_c) * R(0.5);
+                drw[l - 1] = dt_s * (w.R_rhow[cc - nx] - rt2h * buoy + grav);
+            }
+            tp_rp = rp_tp;
+            tp_tp = rtp_tp;
+        }
+    }
+    // theta two levels below and above the chunk, rt2pres one level above
+    if (nl > 0) {
+        const int64_t ci = base + i;
+        if (k0 >= 2) thx[0] = w.theta[ci + (int64_t)(k0 - 2) * nx];
+        if (k0 >= 1) thx[1] = w.theta[ci + (int64_t)(k0 - 1) * nx];
+        if (nl == L) {
+            if (k0 + L < nz) {
+                thx[L + 2] = w.theta[ci + (int64_t)(k0 + L) * nx];
+                rt2c[L] = w.rt2[ci + (int64_t)(k0 + L) * nx];
+            }
+            if (k0 + L + 1 < nz) thx[L + 3] = w.theta[ci + (int64_t)(k0 + L + 1) * nx];
+        }
+    }
+
+    // ---- LDS round A: off-centred values of the first level of each chunk
+    ldt[t] = tp_r0;
+    ldt[AC_THREADS + t] = tp_t0;
+    __syncthreads();
+    if (nl == L && k0 + L < nz) {
+        const int k = k0 + L;
+        const R up_r = ldt[t + TX], up_t = ldt[AC_THREADS + t + TX];
+        const R buoy = (up_t - tp_tp) / (R(zd[k]) - R(zd[k - 1]));
+        const R grav = -K<R>::GRAV * R(0.5) * (up_r + tp_rp);
+        const R rt2h = (rt2c[L - 1] + rt2c[L]) * R(0.5);
+        drw[L - 1] = dt_s * (w.R_rhow[base + (int64_t)(k - 1) * nx + i] - rt2h * buoy + grav);
+    }
+
+    // tridiagonal coefficients of face j = k0 + l before the off-centering
+    // update b += 1 (moist.jl:997-1006)
+    auto coef = [&](int l, double &a, double &b, double &cc) {
+        const int j = k0 + l;
+        const double dzj = double(R(dzd[j])), dzk = double(R(dzd[j + 1]));
+        const double zz = double(R(zd[j + 1]) - R(zd[j]));
+        const double thh_j = double(f2h(thx[l + 2], thx[l + 3], R(dzd[j]), R(dzd[j + 1])));
+        b = dt_tp2 * (double(rt2c[l]) / dzj + double(rt2c[l + 1]) / dzk) * thh_j / zz;
+        a = 0.0;
+        cc = 0.0;
+        if (j > 0) {
+            const double thh_m = double(f2h(thx[l + 1], thx[l + 2], R(dzd[j - 1]), R(dzd[j])));
+            a = -dt_tp2 * (double(rt2c[l]) * thh_m / (zz * dzj) - G / (dzj + dzk));
+        }
+        if (j < nz - 2) {
+            const double thh_p = double(f2h(thx[l + 3], thx[l + 4], R(dzd[j + 1]), R(dzd[j + 2])));
+            cc = -dt_tp2 * (double(rt2c[l + 1]) * thh_p / (zz * dzk) + G / (dzj + dzk));
+        }
+    };
+
+    // ---- right-hand sides (moist.jl:1008-1021)
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        dd[l] = 0.0;
+        if (l < nf) {
+            const int j = k0 + l;
+            double a, b, cc;
+            coef(l, a, b, cc);
+            const double rwj = double(w.rwp[base + (int64_t)j * nx + i]);
+            double d = rwj + double(drw[l]);
+            if (j > 0) d -= a * fact * rwj;
+            d -= b * fact * rwj;
+            if (j < nz - 2) d -= cc * fact * double(w.rwp[base + (int64_t)(j + 1) * nx + i]);
+            dd[l] = d;
+        }
+    }
+
+    // ---- c' entering the chunk: Moebius composites scanned once per stage
+    if (first) {
+        double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            if (l < nf) {
+                double a, b, cc;
+                coef(l, a, b, cc);
+                b += 1.0;
+                // [num; den] <- [[0, c], [-a, b]] [num; den]
+                const double n00 = cc * m10, n01 = cc * m11;
+                const double n10 = -a * m00 + b * m10, n11 = -a * m01 + b * m11;
+                const double sc = 1.0 / fmax(fmax(fabs(n00), fabs(n01)), fmax(fabs(n10), fabs(n11)));
+                m00 = n00 * sc;
+                m01 = n01 * sc;
+                m10 = n10 * sc;
+                m11 = n11 * sc;
+            }
+        }
+        lds[4 * t + 0] = m00;
+        lds[4 * t + 1] = m01;
+        lds[4 * t + 2] = m10;
+        lds[4 * t + 3] = m11;
+        __syncthreads();
+        if (c == 0 && col_ok) {
+            double v0 = 0.0, v1 = 1.0;  // c' = 0 before the first face
+            for (int q = 0; q < nch; ++q) {
+                const int e = 4 * (q * TX + jj);
+                const double a0 = lds[e], a1 = lds[e + 1], a2 = lds[e + 2], a3 = lds[e + 3];
+                const double cinq = v0 / v1;
+                lds[e] = cinq;
+                if (q * L < nz) ab.cin[(int64_t)q * ab.cin_stride + D.offx + i] = cinq;
+                const double u0 = a0 * v0 + a1 * v1, u1 = a2 * v0 + a3 * v1;
+                const double sc = 1.0 / fmax(fabs(u0), fabs(u1));
+                v0 = u0 * sc;
+                v1 = u1 * sc;
+            }
+        }
+        __syncthreads();
+        cin_v = lds[4 * t];
+        __syncthreads();
+    }
+
+    // ---- forward substitution inside the chunk with a zero carry
+    double ylast = 0.0, plast = 1.0;
+    {
+        double cprev = cin_v, yprev = 0.0, P = 1.0;
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            cpv[l] = 0.0;
+            pv[l] = 0.0;
+            if (l < nf) {
+                double a, b, cc;
+                coef(l, a, b, cc);
+                b += 1.0;
+                const double m = b - a * cprev;
+                const double cp = cc / m;
+                const double y = (dd[l] - a * yprev) / m;
+                P = (-a * P) / m;
+                cpv[l] = cp;
+                dd[l] = y;
+                pv[l] = P;
+                cprev = cp;
+                yprev = y;
+                ylast = y;
+                plast = P;
+            }
+        }
+    }
+    lds[2 * t + 0] = ylast;
+    lds[2 * t + 1] = plast;
+    __syncthreads();
+    if (c == 0 && col_ok) {
+        double din = 0.0;
+        for (int q = 0; q < nch; ++q) {
+            const int e = 2 * (q * TX + jj);
+            const double y = lds[e], P = lds[e + 1];
+            lds[e] = din;
+            din = y + P * din;
+        }
+    }
+    __syncthreads();
+    const double din = lds[2 * t];
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+        if (l < nf) dd[l] = dd[l] + pv[l] * din;
+    __syncthreads();
+
+    // ---- back substitution inside the chunk with a zero carry from above
+    double zfirst = 0.0, qfirst = 1.0, zlast = 0.0, qlast = 1.0;
+    {
+        double znext = 0.0, Q = 1.0;
+#pragma unroll
+        for (int l = L - 1; l >= 0; --l) {
+            if (l < nf) {
+                const double zl = dd[l] - cpv[l] * znext;
+                Q = -cpv[l] * Q;
+                if (l == nf - 1) {
+                    zlast = zl;
+                    qlast = Q;
+                }
+                dd[l] = zl;
+                pv[l] = Q;
+                znext = zl;
+                zfirst = zl;
+                qfirst = Q;
+            }
+        }
+    }
+    lds[4 * t + 0] = zfirst;
+    lds[4 * t + 1] = qfirst;
+    lds[4 * t + 2] = zlast;
+    lds[4 * t + 3] = qlast;
+    __syncthreads();
+    if (c == 0 && col_ok) {
+        double xin = 0.0;  // above the top face
+        for (int q = nch - 1; q >= 0; --q) {
+            const int e = 4 * (q * TX + jj);
+            const double zf = lds[e], qf = lds[e + 1];
+            lds[e + 1] = xin;
+            xin = zf + qf * xin;
+        }
+    }
+    __syncthreads();
+    const double xin = lds[4 * t + 1];
+    double x_below = 0.0;
+    if (c > 0 && nl > 0) {
+        const int e = 4 * (t - TX);
+        x_below = lds[e + 2] + lds[e + 3] * lds[e + 1];
+    }
+    R xr[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        xr[l] = R(0.0);
+        if (l < nf) {
+            const double x = dd[l] + pv[l] * xin;
+            const int64_t cj = base + (int64_t)(k0 + l) * nx + i;
+            xr[l] = R(x);
+            w.rwp[cj] = xr[l];
+            w.mom_z[cj] = w.mom_z[cj] + xr[l] / R(nsub);
+        }
+    }
+    const R xb = R(x_below);
+
+    // ---- vertical flux updates of rho' and rho_theta' (moist.jl:1412-1446)
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        if (l < nl) {
+            const int k = k0 + l;
+            const int64_t cc = base + (int64_t)k * nx + i;
+            const R wkp = k == ke ? R(0.0) : xr[l];
+            const R wkm = k == 0 ? R(0.0) : (l > 0 ? xr[l > 0 ? l - 1 : 0] : xb);
+            const R dzk = R(dzd[k]);
+            R adv = -(wkp - wkm) / dzk;
+            w.rp[cc] = rp_e[l] + dt_s * adv;
+            const R t_m2 = thx[l], t_m1 = thx[l + 1], t_0 = thx[l + 2], t_p1 = thx[l + 3], t_p2 = thx[l + 4];
+            R Fkp, Fkm;
+            if (k == ke) Fkp = R(0.0);
+            else {
+                if (wkp > EPS && k > 0) Fkp = wkp * upw3(t_m1, t_0, t_p1) / R(6.0);
+                else if (in.f[RHOW][cc] < -EPS && k < ke - 1) Fkp = wkp * upw3(t_p2, t_p1, t_0) / R(6.0);
+                else Fkp = wkp * f2h(t_0, t_p1, dzk, R(dzd[k + 1]));
+            }
+            if (k == 0) Fkm = R(0.0);
+            else {
+                const R sw = in.f[RHOW][cc - nx];
+                if (sw > EPS && k > 1) Fkm = wkm * upw3(t_m2, t_m1, t_0) / R(6.0);
+                else if (sw < -EPS && k < ke) Fkm = wkm * upw3(t_p1, t_0, t_m1) / R(6.0);
+                else Fkm = wkm * f2h(t_m1, t_0, R(dzd[k - 1]), dzk);
+            }
+            adv = -(Fkp - Fkm) / dzk;
+            ab.rtp_out[cc] = rtp_e[l] + dt_s * adv;
+        }
+    }
+}

@@ -32,6 +32,30 @@ using namespace mo;
 namespace {
 
 constexpr int TILE_X = 64, TILE_Z = 4, COL_TILE = 64;
+// levels per acoustic chunk (FP64 keeps fewer levels in registers)
+#ifndef MOIST_AC_L_MIXED
+#define MOIST_AC_L_MIXED 8
+#endif
+#ifndef MOIST_AC_L_FP64
+#define MOIST_AC_L_FP64 4
+#endif
+template <class S> constexpr int ac_levels() { return sizeof(S) == 4 ? MOIST_AC_L_MIXED : MOIST_AC_L_FP64; }
+
+// Workgroups are handed to the 8 XCDs of the chip round-robin by block index.
+// Reorder a tile list (natural order: neighbours adjacent) so that block b
+// runs tile (b % 8) * per + b / 8: each XCD then works on one contiguous
+// band of tiles and the halo lines its tiles share stay in that XCD's L2.
+// The list is padded to a multiple of 8 with tiles whose member is -1.
+constexpr int NXCD = 8;
+template <class T> std::vector<T> xcd_order(const std::vector<T> &v, T pad) {
+    const size_t n = v.size(), per = (n + NXCD - 1) / NXCD, np = per * NXCD;
+    std::vector<T> out(np, pad);
+    for (size_t b = 0; b < np; ++b) {
+        const size_t t = (b % NXCD) * per + b / NXCD;
+        if (t < n) out[b] = v[t];
+    }
+    return out;
+}
 
 }  // namespace
 
@@ -52,6 +76,10 @@ struct moist_handle {
     int ntiles2d = 0;
     int2 *ctiles = nullptr;
     int nctiles = 0;
+    int4 *atiles = nullptr;  // acoustic tiles {member, i0, TX, nch}
+    int natiles = 0;
+    int nch_max = 0;
+    double *cin = nullptr;   // c' entering each acoustic chunk, [chunk][column]
     // 1-D z data (double)
     double *z = nullptr, *zf = nullptr, *dz = nullptr, *theta_ref = nullptr;
     double *rho_bar = nullptr, *rt_bar = nullptr, *p_bar = nullptr, *L_bar = nullptr, *B_bar = nullptr;
@@ -59,7 +87,7 @@ struct moist_handle {
     // pools (void: typed views below)
     void *pool = nullptr;        // all 2-D fields of type S
     size_t elem = 8;
-    double *prec = nullptr, *sfh = nullptr, *cpr = nullptr, *dpr = nullptr;
+    double *prec = nullptr, *sfh = nullptr;
     void *pool1 = nullptr;       // 1-D x fields of type S
     int *nan_key = nullptr, *step_ctr = nullptr;
     void *staging = nullptr;
@@ -90,7 +118,7 @@ namespace {
 // Field slots of the 2-D pool.
 enum Slot {
     S0_ = 0, SA_ = 7, SB_ = 14,
-    THETA = 21, PRES, RT2, UF, WF, R_RHO, R_RHOU, R_RHOW, R_RT, RP, RUP, RWP, RTP0, RTP1, MOMX, MOMZ,
+    THETA = 21, PRES, RT2, UF, WF, R_RHO, R_RHOU, R_RHOW, R_RT, RP, RUP, RUP2, RWP, RTP0, RTP1, RTP2, MOMX, MOMZ,
     D_RHO, D_RT, D_QV, D_QC, D_QR, D_U2D, DG_TH_MP, DG_QV_MP, DG_QR_MP, DG_QR_SED, D_TH_RD,
     NSLOT
 };
@@ -121,8 +149,6 @@ template <class S> Work<S> work(moist_handle *h) {
     w.rwp = slot<S>(h, RWP);
     w.mom_x = slot<S>(h, MOMX);
     w.mom_z = slot<S>(h, MOMZ);
-    w.cpr = h->cpr;
-    w.dpr = h->dpr;
     w.d_rho = slot<S>(h, D_RHO);
     w.d_rt = slot<S>(h, D_RT);
     w.d_qv = slot<S>(h, D_QV);
@@ -178,28 +204,39 @@ void enqueue_stage(moist_handle *h, int in_set, int out_set, int s0_set, int div
     const ZVec zv = zvec(h);
     const Work<S> w = work<S>(h);
     const SSet<S> in = sset<S>(h, in_set), out = sset<S>(h, out_set), s0 = sset<S>(h, s0_set);
-    S *rtp[2] = {slot<S>(h, RTP0), slot<S>(h, RTP1)};
+    S *rtp[3] = {slot<S>(h, RTP0), slot<S>(h, RTP1), slot<S>(h, RTP2)};
+    S *rup[2] = {slot<S>(h, RUP), slot<S>(h, RUP2)};
     hipLaunchKernelGGL(k_diag<S>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, w);
     hipLaunchKernelGGL(k_slow<S>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, s0, w,
                        rtp[0], u_phys_2d);
-    int cur = 0, old = 0;
+    // rho_theta' rotates over three buffers (current, previous, next) and
+    // rho_u' over two, because every column reads its neighbours' values
+    int cur = 0, old = 0, ru = 0;
     for (int n = 0; n < nsub; ++n) {
-        hipLaunchKernelGGL(k_acoustic_u<S>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, w,
-                           (const S *)rtp[cur], (const S *)rtp[old], div, dt_ovr, nsub);
+        const int nxt = cur == old ? (cur + 1) % 3 : 3 - cur - old;
+        AcBuf<S> ab;
+        ab.rup_in = rup[ru];
+        ab.rup_out = rup[1 - ru];
+        ab.rtp_cur = rtp[cur];
+        ab.rtp_old = rtp[old];
+        ab.rtp_out = rtp[nxt];
+        ab.cin = h->cin;
+        ab.cin_stride = h->nxs;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (h->ev_on) {
             e0 = next_event(h);
             e1 = next_event(h);
             if (e0) hipEventRecord(e0, h->stream);
         }
-        hipLaunchKernelGGL(k_acoustic_col<S>, dim3(h->nctiles), dim3(COL_TILE), 0, h->stream, h->dom_d,
-                           h->ctiles, zv, in, w, (const S *)rtp[cur], rtp[1 - cur], div, dt_ovr, nsub);
+        hipLaunchKernelGGL((k_acoustic<S, ac_levels<S>()>), dim3(h->natiles), dim3(AC_THREADS), 0, h->stream,
+                           h->dom_d, h->atiles, zv, in, w, ab, div, dt_ovr, nsub, n == 0 ? 1 : 0);
         if (h->ev_on && e1) hipEventRecord(e1, h->stream);
         old = cur;
-        cur = 1 - cur;
+        cur = nxt;
+        ru = 1 - ru;
     }
     hipLaunchKernelGGL(k_scalar<S>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, s0, out, w,
-                       (const S *)rtp[cur], div, dt_ovr, stage);
+                       (const S *)rup[ru], (const S *)rtp[cur], div, dt_ovr, stage);
 }
 
 template <class S> void enqueue_step(moist_handle *h) {
@@ -532,14 +569,29 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
     h->nzs = offz;
     h->ns_stage = {std::max(1, (h->ns + 2) / 3), std::max(1, (h->ns + 1) / 2), h->ns};
 
-    std::vector<int4> t2;
+    std::vector<int4> t2, ta;
     std::vector<int2> tc;
+    const int acL = precision == MOIST_MIXED ? ac_levels<float>() : ac_levels<double>();
     for (int m = 0; m < n_members; ++m) {
         const Dom &D = h->dom_h[m];
         for (int k0 = 0; k0 < D.nz; k0 += TILE_Z)
             for (int i0 = 0; i0 < D.nx; i0 += TILE_X) t2.push_back(make_int4(m, i0, k0, 0));
         for (int i0 = 0; i0 < D.nx; i0 += COL_TILE) tc.push_back(make_int2(m, i0));
+        int nch = 1;
+        while (nch * acL < D.nz) nch *= 2;
+        if (nch > AC_THREADS) {
+            h->err = "nz too large for the acoustic kernel";
+            *out = h.release();
+            return -2;
+        }
+        const int TX = AC_THREADS / nch;
+        h->nch_max = std::max(h->nch_max, nch);
+        for (int i0 = 0; i0 < D.nx; i0 += TX) ta.push_back(make_int4(m, i0, TX, nch));
     }
+    t2 = xcd_order(t2, make_int4(-1, 0, 0, 0));
+    tc = xcd_order(tc, make_int2(-1, 0));
+    ta = xcd_order(ta, make_int4(-1, 0, 0, 0));
+    h->natiles = (int)ta.size();
     h->ntiles2d = (int)t2.size();
     h->nctiles = (int)tc.size();
 
@@ -549,6 +601,8 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
     ALLOC(hp->dom_d, sizeof(Dom) * n_members);
     ALLOC(hp->tiles2d, sizeof(int4) * t2.size());
     ALLOC(hp->ctiles, sizeof(int2) * tc.size());
+    ALLOC(hp->atiles, sizeof(int4) * ta.size());
+    HIPCHK(hp, hipMemcpy(hp->atiles, ta.data(), sizeof(int4) * ta.size(), hipMemcpyHostToDevice));
     HIPCHK(hp, hipMemcpy(hp->dom_d, hp->dom_h.data(), sizeof(Dom) * n_members, hipMemcpyHostToDevice));
     HIPCHK(hp, hipMemcpy(hp->tiles2d, t2.data(), sizeof(int4) * t2.size(), hipMemcpyHostToDevice));
     HIPCHK(hp, hipMemcpy(hp->ctiles, tc.data(), sizeof(int2) * tc.size(), hipMemcpyHostToDevice));
@@ -566,8 +620,8 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
     HIPCHK(hp, hipMemset(hp->pool, 0, pool_bytes));
     ALLOC(hp->pool1, (size_t)NSLOT1 * offx * hp->elem);
     HIPCHK(hp, hipMemset(hp->pool1, 0, (size_t)NSLOT1 * offx * hp->elem));
-    ALLOC(hp->cpr, sizeof(double) * hp->n2);
-    ALLOC(hp->dpr, sizeof(double) * hp->n2);
+    ALLOC(hp->cin, sizeof(double) * (size_t)hp->nch_max * offx);
+    HIPCHK(hp, hipMemset(hp->cin, 0, sizeof(double) * (size_t)hp->nch_max * offx));
     ALLOC(hp->prec, sizeof(double) * offx);
     HIPCHK(hp, hipMemset(hp->prec, 0, sizeof(double) * offx));
     ALLOC(hp->sfh, sizeof(double) * 2 * n_members);
@@ -751,7 +805,7 @@ void moist_destroy(moist_handle *h) {
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     for (auto e : h->ev_pool) hipEventDestroy(e);
     void *ptrs[] = {h->dom_d, h->tiles2d, h->ctiles, h->z, h->zf, h->dz, h->theta_ref, h->rho_bar, h->rt_bar,
-                    h->p_bar, h->L_bar, h->B_bar, h->pool, h->pool1, h->cpr, h->dpr, h->prec, h->sfh, h->nan_key,
+                    h->p_bar, h->L_bar, h->B_bar, h->pool, h->pool1, h->atiles, h->cin, h->prec, h->sfh, h->nan_key,
                     h->step_ctr, h->staging, h->diag_out};
     for (void *p : ptrs)
         if (p) hipFree(p);

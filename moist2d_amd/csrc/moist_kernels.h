@@ -31,7 +31,6 @@ template <class S> struct Work {
     S *R_rho, *R_rhou, *R_rhow, *R_rt;    // slow tendencies (moist.jl:1071-1332)
     S *rp, *rup, *rwp;                    // acoustic perturbations (moist.jl:1339-1342)
     S *mom_x, *mom_z;                     // time-averaged mass fluxes (moist.jl:1347-1348)
-    double *cpr, *dpr;                    // Thomas factors of the current substep
     S *d_rho, *d_rt, *d_qv, *d_qc, *d_qr; // physics tendencies, frozen over a step (moist.jl:1632-1644)
     S *d_u2d;                             // 2-D d_rho_u_phys (stage-level entry only)
     S *d_u_sf, *d_th_sf, *d_qv_sf, *mflux, *shflux, *lhflux;  // 1-D x (moist.jl:710-752)
@@ -73,6 +72,7 @@ __global__ void __launch_bounds__(256) k_diag(const Dom *__restrict__ doms, cons
     using R = S;
     using KC = K<R>;
     const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;  // padding tile of the XCD ordering
     const Dom &D = doms[t.x];
     const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
     const int nx = D.nx, nz = D.nz;
@@ -129,6 +129,7 @@ __global__ void __launch_bounds__(256) k_slow(const Dom *__restrict__ doms, cons
     using R = S;
     using KC = K<R>;
     const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;  // padding tile of the XCD ordering
     const Dom &D = doms[t.x];
     const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
     const int nx = D.nx, nz = D.nz, ke = nz - 1;
@@ -292,180 +293,7 @@ __global__ void __launch_bounds__(256) k_slow(const Dom *__restrict__ doms, cons
     w.mom_z[c] = in.f[RHOW][c];
 }
 
-// =============================================================================
-// Acoustic substep, horizontal part: rho_u' with divergence damping
-// (moist.jl:1360-1369) and its running mean (moist.jl:1453-1455).
-// =============================================================================
-template <class S>
-__global__ void __launch_bounds__(256) k_acoustic_u(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
-                                                    Work<S> w, const S *__restrict__ rtp_cur,
-                                                    const S *__restrict__ rtp_old, int div, double dt_ovr,
-                                                    int nsub) {
-    using R = S;
-    const int4 t = tiles[blockIdx.x];
-    const Dom &D = doms[t.x];
-    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
-    const int nx = D.nx;
-    if (i >= nx || k >= D.nz) return;
-    const int64_t row = D.off2 + (int64_t)k * nx;
-    const int64_t c = row + i, cp = row + wrapx(i + 1, nx);
-    const R dx = R(D.dx);
-    const R dt_s = stage_dt<R>(D, div, dt_ovr) / R(nsub);
-    R gx = (rtp_cur[cp] - rtp_cur[c]) / dx;
-    R gxo = (rtp_old[cp] - rtp_old[c]) / dx;
-    gx = gx + R(D.gamma_d) * (gx - gxo);
-    R rt2h = (w.rt2[c] + w.rt2[cp]) * R(0.5);
-    R v = w.rup[c] + dt_s * (w.R_rhou[c] - rt2h * gx);
-    w.rup[c] = v;
-    w.mom_x[c] = w.mom_x[c] + v / R(nsub);
-}
-
-// =============================================================================
-// Acoustic substep, column part (moist.jl:1377-1447, 1456-1458):
-// explicit rho'/rho_theta', the w-p tridiagonal (implicit_correction!,
-// moist.jl:974-1030, solved by the Thomas algorithm, moist.jl:952-971,
-// in FP64), then the vertical flux updates.  One thread per column.
-// =============================================================================
-template <class S>
-__global__ void __launch_bounds__(64) k_acoustic_col(const Dom *__restrict__ doms, const int2 *__restrict__ ctiles,
-                                                     ZVec zv, SSet<S> in, Work<S> w,
-                                                     const S *__restrict__ rtp_cur, S *__restrict__ rtp_out,
-                                                     int div, double dt_ovr, int nsub) {
-    using R = S;
-    const int2 t = ctiles[blockIdx.x];
-    const Dom &D = doms[t.x];
-    const int i = t.y + threadIdx.x;
-    const int nx = D.nx, nz = D.nz, ke = nz - 1, n = nz - 1;
-    if (i >= nx) return;
-    const int im = wrapx(i - 1, nx), ip = wrapx(i + 1, nx);
-    const int64_t base = D.off2;
-    const double *zd = zv.z + D.offz, *dzd = zv.dz + D.offz;
-    const R dx = R(D.dx);
-    const R EPS = K<R>::EPS;
-    const R dt_s = stage_dt<R>(D, div, dt_ovr) / R(nsub);
-    // fact_tp of the explicit part is a Float64 expression in the reference
-    // (moist.jl:1375); in the FP64 form that is the same double.
-    const R beta = R(D.beta);
-    const double fact_tp_d = double(R(0.5)) * (1.0 + double(beta));
-    // implicit_correction!'s own factors, in T (moist.jl:987-989)
-    const R ftp = R(0.5) * (R(1.0) + beta);
-    const R ftm = R(0.5) * (R(1.0) - beta);
-    const double dt_tp2 = double(dt_s * ftp) * double(dt_s * ftp);
-    const double fact = double(ftm / ftp);
-    const double G = K<double>::GRAV;
-    auto at = [&](int kk) -> int64_t { return base + (int64_t)kk * nx + i; };
-    auto TH = [&](int kk) -> R { return w.theta[at(kk)]; };
-    auto DZ = [&](int kk) -> R { return R(dzd[kk]); };
-
-    // ---- pass 1 (k ascending): explicit updates, RHS, forward elimination
-    R rp_tp_prev = R(0), rtp_tp_prev = R(0);
-    double thh_jm1 = 0.0, thh_j = 0.0, cprev = 0.0, dprev = 0.0;
-    for (int k = 0; k < nz; ++k) {
-        const int64_t c = at(k);
-        const int64_t cm = base + (int64_t)k * nx + im, cpx = base + (int64_t)k * nx + ip;
-        const R rupc = w.rup[c], rupm = w.rup[cm];
-        R adv = -(rupc - rupm) / dx;
-        R inc = dt_s * (w.R_rho[c] + adv);
-        const R rpc = w.rp[c];
-        R rp_tp;
-        if constexpr (is_mixed<S>()) rp_tp = rpc + R(fact_tp_d) * inc;
-        else rp_tp = R(double(rpc) + fact_tp_d * double(inc));
-        w.rp[c] = rpc + inc;
-        const R th = w.theta[c];
-        R fip = (w.theta[cpx] + th) * R(0.5) * rupc;
-        R fim = (th + w.theta[cm]) * R(0.5) * rupm;
-        adv = -(fip - fim) / dx;
-        inc = dt_s * (w.R_rt[c] + adv);
-        const R rtc = rtp_cur[c];
-        R rtp_tp;
-        if constexpr (is_mixed<S>()) rtp_tp = rtc + R(fact_tp_d) * inc;
-        else rtp_tp = R(double(rtc) + fact_tp_d * double(inc));
-        rtp_out[c] = rtc + inc;
-        if (k > 0) {
-            const int j = k - 1;  // face j+1/2 between levels j and k
-            const int64_t cj = at(j);
-            const R zzR = R(zd[k]) - R(zd[j]);
-            R buoy = (rtp_tp - rtp_tp_prev) / zzR;
-            R grav = -K<R>::GRAV * R(0.5) * (rp_tp + rp_tp_prev);
-            const R rt2j = w.rt2[cj], rt2k = w.rt2[c];
-            R rt2h = (rt2j + rt2k) * R(0.5);
-            R drw = dt_s * (w.R_rhow[cj] - rt2h * buoy + grav);
-            // tridiagonal row j (moist.jl:997-1021), FP64
-            const double dzj = double(DZ(j)), dzk = double(DZ(k));
-            const double zz = double(zzR);
-            if (j == 0) thh_j = double(f2h(TH(0), TH(1), DZ(0), DZ(1)));
-            double thh_jp1 = 0.0;
-            if (j < n - 1) thh_jp1 = double(f2h(th, TH(k + 1), DZ(k), DZ(k + 1)));
-            double b = dt_tp2 * (double(rt2j) / dzj + double(rt2k) / dzk) * thh_j / zz;
-            double a = 0.0, cc = 0.0;
-            if (j > 0) a = -dt_tp2 * (double(rt2j) * thh_jm1 / (zz * dzj) - G / (dzj + dzk));
-            if (j < n - 1) cc = -dt_tp2 * (double(rt2k) * thh_jp1 / (zz * dzk) + G / (dzj + dzk));
-            const double rwj = double(w.rwp[cj]);
-            double d = rwj + double(drw);
-            if (j > 0) d -= a * fact * rwj;
-            d -= b * fact * rwj;
-            b += 1.0;
-            if (j < n - 1) d -= cc * fact * double(w.rwp[at(j + 1)]);
-            double cpj, dpj;
-            if (j == 0) {
-                cpj = cc / b;
-                dpj = d / b;
-            } else {
-                double temp = b - a * cprev;
-                cpj = cc / temp;
-                dpj = (d - a * dprev) / temp;
-            }
-            w.cpr[cj] = cpj;
-            w.dpr[cj] = dpj;
-            cprev = cpj;
-            dprev = dpj;
-            thh_jm1 = thh_j;
-            thh_j = thh_jp1;
-        }
-        rp_tp_prev = rp_tp;
-        rtp_tp_prev = rtp_tp;
-    }
-
-    // ---- pass 2 (descending): back substitution and vertical updates
-    auto vert = [&](int k, R wkp, R wkm) {
-        const int64_t c = at(k);
-        R fkp = k == ke ? R(0.0) : wkp;
-        R fkm = k == 0 ? R(0.0) : wkm;
-        R adv = -(fkp - fkm) / DZ(k);
-        w.rp[c] = w.rp[c] + dt_s * adv;
-        R Fkp, Fkm;
-        if (k == ke) Fkp = R(0.0);
-        else {
-            if (wkp > EPS && k > 0) Fkp = wkp * upw3(TH(k - 1), TH(k), TH(k + 1)) / R(6.0);
-            else if (in.f[RHOW][c] < -EPS && k < ke - 1) Fkp = wkp * upw3(TH(k + 2), TH(k + 1), TH(k)) / R(6.0);
-            else Fkp = wkp * f2h(TH(k), TH(k + 1), DZ(k), DZ(k + 1));
-        }
-        if (k == 0) Fkm = R(0.0);
-        else {
-            R sw = in.f[RHOW][c - nx];
-            if (sw > EPS && k > 1) Fkm = wkm * upw3(TH(k - 2), TH(k - 1), TH(k)) / R(6.0);
-            else if (sw < -EPS && k < ke) Fkm = wkm * upw3(TH(k + 1), TH(k), TH(k - 1)) / R(6.0);
-            else Fkm = wkm * f2h(TH(k - 1), TH(k), DZ(k - 1), DZ(k));
-        }
-        adv = -(Fkp - Fkm) / DZ(k);
-        rtp_out[c] = rtp_out[c] + dt_s * adv;
-    };
-    R x_next = R(0.0);
-    double xd_next = 0.0;
-    for (int j = n - 1; j >= 0; --j) {
-        const int64_t cj = at(j);
-        double xd = (j == n - 1) ? w.dpr[cj] : w.dpr[cj] - w.cpr[cj] * xd_next;
-        const R x = R(xd);
-        w.rwp[cj] = x;
-        const int k = j + 1;
-        vert(k, x_next, x);
-        if (k <= nz - 2) w.mom_z[at(k)] = w.mom_z[at(k)] + x_next / R(nsub);
-        x_next = x;
-        xd_next = double(x);
-    }
-    vert(0, x_next, R(0.0));
-    if (nz >= 2) w.mom_z[at(0)] = w.mom_z[at(0)] + x_next / R(nsub);
-}
+#include "moist_acoustic.h"
 
 // =============================================================================
 // Koren flux at x-face ii+1/2 (compute_limited_flux_x, moist.jl:850-892).
@@ -549,10 +377,11 @@ __device__ __forceinline__ S flux_z(const S *q, const SSet<S> &in, const ZVec &z
 template <class S>
 __global__ void __launch_bounds__(256) k_scalar(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
                                                 ZVec zv, SSet<S> in, SSet<S> s0, SSet<S> out, Work<S> w,
-                                                const S *__restrict__ rtp_fin, int div, double dt_ovr,
-                                                int stage) {
+                                                const S *__restrict__ rup_fin, const S *__restrict__ rtp_fin,
+                                                int div, double dt_ovr, int stage) {
     using R = S;
     const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;  // padding tile of the XCD ordering
     const int m = t.x;
     const Dom &D = doms[m];
     const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
@@ -600,7 +429,7 @@ __global__ void __launch_bounds__(256) k_scalar(const Dom *__restrict__ doms, co
         if (isnan(v) && !nan_f) nan_f = 5 + sc;
     }
     const R vr = in.f[RHO][c] + w.rp[c];
-    const R vu = in.f[RHOU][c] + w.rup[c];
+    const R vu = in.f[RHOU][c] + rup_fin[c];
     const R vt = in.f[RT][c] + rtp_fin[c];
     const R vw = k < ke ? in.f[RHOW][c] + w.rwp[c] : in.f[RHOW][c];
     out.f[RHO][c] = vr;
@@ -628,6 +457,7 @@ __global__ void __launch_bounds__(64) k_surface(const Dom *__restrict__ doms, co
     using R = S;
     using KC = K<R>;
     const int2 t = ctiles[blockIdx.x];
+    if (t.x < 0) return;  // padding tile of the XCD ordering
     const int m = t.x;
     const Dom &D = doms[m];
     const int i = t.y + threadIdx.x;
@@ -681,6 +511,7 @@ __global__ void __launch_bounds__(64) k_radiation(const Dom *__restrict__ doms, 
     using R = S;
     using KC = K<R>;
     const int2 t = ctiles[blockIdx.x];
+    if (t.x < 0) return;  // padding tile of the XCD ordering
     const Dom &D = doms[t.x];
     const int i = t.y + threadIdx.x;
     const int nx = D.nx, nz = D.nz;
@@ -753,6 +584,7 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
     using R = S;
     using KC = K<R>;
     const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;  // padding tile of the XCD ordering
     const Dom &D = doms[t.x];
     const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
     const int nx = D.nx, nz = D.nz, ke = nz - 1;
