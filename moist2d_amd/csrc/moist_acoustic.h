@@ -103,18 +103,18 @@ __global__ void __launch_bounds__(AC_THREADS) k_acoustic(const Dom *__restrict__
             const R old_c = ab.rtp_old[cc], old_m = ab.rtp_old[cm], old_p = ab.rtp_old[cx];
             const R rt2_c = w.rt2[cc], rt2_m = w.rt2[cm], rt2_p = w.rt2[cx];
             // rho_u' at i+1/2 and i-1/2 (moist.jl:1362-1367)
-            R gx = (cur_p - cur_c) / dx;
-            R gxo = (old_p - old_c) / dx;
+            R gx = fdiv(cur_p - cur_c, dx);
+            R gxo = fdiv(old_p - old_c, dx);
             gx = gx + gam * (gx - gxo);
             const R ru_i = ab.rup_in[cc] + dt_s * (w.R_rhou[cc] - (rt2_c + rt2_p) * R(0.5) * gx);
-            R gm = (cur_c - cur_m) / dx;
-            R gmo = (old_c - old_m) / dx;
+            R gm = fdiv(cur_c - cur_m, dx);
+            R gmo = fdiv(old_c - old_m, dx);
             gm = gm + gam * (gm - gmo);
             const R ru_m = ab.rup_in[cm] + dt_s * (w.R_rhou[cm] - (rt2_m + rt2_c) * R(0.5) * gm);
             ab.rup_out[cc] = ru_i;
-            w.mom_x[cc] = w.mom_x[cc] + ru_i / R(nsub);
+            w.mom_x[cc] = w.mom_x[cc] + fdiv(ru_i, R(nsub));
             // explicit rho' (moist.jl:1383-1386)
-            R adv = -(ru_i - ru_m) / dx;
+            R adv = fdiv(-(ru_i - ru_m), dx);
             R inc = dt_s * (w.R_rho[cc] + adv);
             const R rpc = w.rp[cc];
             R rp_tp;
@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(AC_THREADS) k_acoustic(const Dom *__restrict__
             const R th_c = w.theta[cc];
             const R fip = (w.theta[cx] + th_c) * R(0.5) * ru_i;
             const R fim = (th_c + w.theta[cm]) * R(0.5) * ru_m;
-            adv = -(fip - fim) / dx;
+            adv = fdiv(-(fip - fim), dx);
             inc = dt_s * (w.R_rt[cc] + adv);
             R rtp_tp;
             if constexpr (is_mixed<S>()) rtp_tp = cur_c + R(fact_tp_d) * inc;
@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(AC_THREADS) k_acoustic(const Dom *__restrict__
                 tp_t0 = rtp_tp;
             } else {
                 // d rho_w' at face k-1 (moist.jl:1399-1404)
-                const R buoy = (rtp_tp - tp_tp) / (R(zd[k]) - R(zd[k - 1]));
+                const R buoy = fdiv(rtp_tp - tp_tp, R(zd[k]) - R(zd[k - 1]));
                 const R grav = -K<R>::GRAV * R(0.5) * (rp_tp + tp_rp);
                 const R rt2h = (rt2c[l - 1] + rt2_c) * R(0.5);
                 drw[l - 1] = dt_s * (w.R_rhow[cc - nx] - rt2h * buoy + grav);
@@ -168,30 +168,34 @@ __global__ void __launch_bounds__(AC_THREADS) k_acoustic(const Dom *__restrict__
     if (nl == L && k0 + L < nz) {
         const int k = k0 + L;
         const R up_r = ldt[t + TX], up_t = ldt[AC_THREADS + t + TX];
-        const R buoy = (up_t - tp_tp) / (R(zd[k]) - R(zd[k - 1]));
+        const R buoy = fdiv(up_t - tp_tp, R(zd[k]) - R(zd[k - 1]));
         const R grav = -K<R>::GRAV * R(0.5) * (up_r + tp_rp);
         const R rt2h = (rt2c[L - 1] + rt2c[L]) * R(0.5);
         drw[L - 1] = dt_s * (w.R_rhow[base + (int64_t)(k - 1) * nx + i] - rt2h * buoy + grav);
     }
 
+    // theta at the faces k0-1 .. k0+L (f2h in T, moist.jl:982-985): thh[q] is face k0-1+q
+    double thh[L + 2];
+#pragma unroll
+    for (int q = 0; q < L + 2; ++q) {
+        const int j = k0 - 1 + q;
+        thh[q] = 0.0;
+        // faces up to nz-2 exist; their theta levels are all in thx (a partial
+        // chunk is the top chunk, so it never needs levels above its own)
+        if (nl > 0 && j >= 0 && j <= nz - 2) thh[q] = double(f2h(thx[q + 1], thx[q + 2], R(dzd[j]), R(dzd[j + 1])));
+    }
+    const double4 *geo = zv.geo + D.offz;
     // tridiagonal coefficients of face j = k0 + l before the off-centering
-    // update b += 1 (moist.jl:997-1006)
+    // update b += 1 (moist.jl:997-1006), with the grid factors 1/dz, 1/(z_j+1 - z_j)
+    // and g/(dz_j + dz_j+1) taken from the per-level table
     auto coef = [&](int l, double &a, double &b, double &cc) {
         const int j = k0 + l;
-        const double dzj = double(R(dzd[j])), dzk = double(R(dzd[j + 1]));
-        const double zz = double(R(zd[j + 1]) - R(zd[j]));
-        const double thh_j = double(f2h(thx[l + 2], thx[l + 3], R(dzd[j]), R(dzd[j + 1])));
-        b = dt_tp2 * (double(rt2c[l]) / dzj + double(rt2c[l + 1]) / dzk) * thh_j / zz;
-        a = 0.0;
-        cc = 0.0;
-        if (j > 0) {
-            const double thh_m = double(f2h(thx[l + 1], thx[l + 2], R(dzd[j - 1]), R(dzd[j])));
-            a = -dt_tp2 * (double(rt2c[l]) * thh_m / (zz * dzj) - G / (dzj + dzk));
-        }
-        if (j < nz - 2) {
-            const double thh_p = double(f2h(thx[l + 3], thx[l + 4], R(dzd[j + 1]), R(dzd[j + 2])));
-            cc = -dt_tp2 * (double(rt2c[l + 1]) * thh_p / (zz * dzk) + G / (dzj + dzk));
-        }
+        const double4 gj = geo[j];
+        const double rdz1 = geo[j + 1].x;
+        const double r2j = double(rt2c[l]), r2k = double(rt2c[l + 1]);
+        b = dt_tp2 * (r2j * gj.x + r2k * rdz1) * thh[l + 1] * gj.y;
+        a = j > 0 ? -dt_tp2 * (r2j * thh[l] * (gj.y * gj.x) - gj.w) : 0.0;
+        cc = j < nz - 2 ? -dt_tp2 * (r2k * thh[l + 2] * (gj.y * rdz1) + gj.w) : 0.0;
     };
 
     // ---- right-hand sides (moist.jl:1008-1021)
@@ -266,10 +270,10 @@ __global__ void __launch_bounds__(AC_THREADS) k_acoustic(const Dom *__restrict__
                 double a, b, cc;
                 coef(l, a, b, cc);
                 b += 1.0;
-                const double m = b - a * cprev;
-                const double cp = cc / m;
-                const double y = (dd[l] - a * yprev) / m;
-                P = (-a * P) / m;
+                const double rm = 1.0 / (b - a * cprev);
+                const double cp = cc * rm;
+                const double y = (dd[l] - a * yprev) * rm;
+                P = (-a * P) * rm;
                 cpv[l] = cp;
                 dd[l] = y;
                 pv[l] = P;
@@ -350,7 +354,7 @@ __global__ void __launch_bounds__(AC_THREADS) k_acoustic(const Dom *__restrict__
             const int64_t cj = base + (int64_t)(k0 + l) * nx + i;
             xr[l] = R(x);
             w.rwp[cj] = xr[l];
-            w.mom_z[cj] = w.mom_z[cj] + xr[l] / R(nsub);
+            w.mom_z[cj] = w.mom_z[cj] + fdiv(xr[l], R(nsub));
         }
     }
     const R xb = R(x_below);
@@ -364,24 +368,24 @@ __global__ void __launch_bounds__(AC_THREADS) k_acoustic(const Dom *__restrict__
             const R wkp = k == ke ? R(0.0) : xr[l];
             const R wkm = k == 0 ? R(0.0) : (l > 0 ? xr[l > 0 ? l - 1 : 0] : xb);
             const R dzk = R(dzd[k]);
-            R adv = -(wkp - wkm) / dzk;
+            R adv = fdiv(-(wkp - wkm), dzk);
             w.rp[cc] = rp_e[l] + dt_s * adv;
             const R t_m2 = thx[l], t_m1 = thx[l + 1], t_0 = thx[l + 2], t_p1 = thx[l + 3], t_p2 = thx[l + 4];
             R Fkp, Fkm;
             if (k == ke) Fkp = R(0.0);
             else {
-                if (wkp > EPS && k > 0) Fkp = wkp * upw3(t_m1, t_0, t_p1) / R(6.0);
-                else if (in.f[RHOW][cc] < -EPS && k < ke - 1) Fkp = wkp * upw3(t_p2, t_p1, t_0) / R(6.0);
+                if (wkp > EPS && k > 0) Fkp = fdiv(wkp * upw3(t_m1, t_0, t_p1), R(6.0));
+                else if (in.f[RHOW][cc] < -EPS && k < ke - 1) Fkp = fdiv(wkp * upw3(t_p2, t_p1, t_0), R(6.0));
                 else Fkp = wkp * f2h(t_0, t_p1, dzk, R(dzd[k + 1]));
             }
             if (k == 0) Fkm = R(0.0);
             else {
                 const R sw = in.f[RHOW][cc - nx];
-                if (sw > EPS && k > 1) Fkm = wkm * upw3(t_m2, t_m1, t_0) / R(6.0);
-                else if (sw < -EPS && k < ke) Fkm = wkm * upw3(t_p1, t_0, t_m1) / R(6.0);
+                if (sw > EPS && k > 1) Fkm = fdiv(wkm * upw3(t_m2, t_m1, t_0), R(6.0));
+                else if (sw < -EPS && k < ke) Fkm = fdiv(wkm * upw3(t_p1, t_0, t_m1), R(6.0));
                 else Fkm = wkm * f2h(t_m1, t_0, R(dzd[k - 1]), dzk);
             }
-            adv = -(Fkp - Fkm) / dzk;
+            adv = fdiv(-(Fkp - Fkm), dzk);
             ab.rtp_out[cc] = rtp_e[l] + dt_s * adv;
         }
     }

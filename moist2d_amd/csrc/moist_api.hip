@@ -34,7 +34,7 @@ namespace {
 constexpr int TILE_X = 64, TILE_Z = 4, COL_TILE = 64;
 // levels per acoustic chunk (FP64 keeps fewer levels in registers)
 #ifndef MOIST_AC_L_MIXED
-#define MOIST_AC_L_MIXED 8
+#define MOIST_AC_L_MIXED 4
 #endif
 #ifndef MOIST_AC_L_FP64
 #define MOIST_AC_L_FP64 4
@@ -82,6 +82,7 @@ struct moist_handle {
     double *cin = nullptr;   // c' entering each acoustic chunk, [chunk][column]
     // 1-D z data (double)
     double *z = nullptr, *zf = nullptr, *dz = nullptr, *theta_ref = nullptr;
+    double4 *geo = nullptr;
     double *rho_bar = nullptr, *rt_bar = nullptr, *p_bar = nullptr, *L_bar = nullptr, *B_bar = nullptr;
     std::vector<double> rho_bar_h, rt_bar_h;
     // pools (void: typed views below)
@@ -179,6 +180,7 @@ ZVec zvec(moist_handle *h) {
     z.z = h->z;
     z.zf = h->zf;
     z.dz = h->dz;
+    z.geo = h->geo;
     z.rho_bar = h->rho_bar;
     z.rt_bar = h->rt_bar;
     z.p_bar = h->p_bar;
@@ -615,6 +617,22 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
     HIPCHK(hp, hipMemcpy(hp->z, zh.data(), 8 * offz, hipMemcpyHostToDevice));
     HIPCHK(hp, hipMemcpy(hp->zf, zfh.data(), 8 * offz, hipMemcpyHostToDevice));
     HIPCHK(hp, hipMemcpy(hp->dz, dzh.data(), 8 * offz, hipMemcpyHostToDevice));
+    {
+        std::vector<double4> g(offz);
+        for (int m = 0; m < n_members; ++m) {
+            const Dom &D = hp->dom_h[m];
+            for (int k = 0; k < D.nz; ++k) {
+                const size_t q = D.offz + k;
+                if (k < D.nz - 1)
+                    g[q] = make_double4(1.0 / dzh[q], 1.0 / (zh[q + 1] - zh[q]), 1.0 / (dzh[q] + dzh[q + 1]),
+                                        9.81 / (dzh[q] + dzh[q + 1]));
+                else
+                    g[q] = make_double4(1.0 / dzh[q], 0.0, 0.0, 0.0);
+            }
+        }
+        ALLOC(hp->geo, sizeof(double4) * offz);
+        HIPCHK(hp, hipMemcpy(hp->geo, g.data(), sizeof(double4) * offz, hipMemcpyHostToDevice));
+    }
     const size_t pool_bytes = (size_t)NSLOT * hp->n2 * hp->elem;
     ALLOC(hp->pool, pool_bytes);
     HIPCHK(hp, hipMemset(hp->pool, 0, pool_bytes));
@@ -804,7 +822,7 @@ void moist_destroy(moist_handle *h) {
     if (!h) return;
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     for (auto e : h->ev_pool) hipEventDestroy(e);
-    void *ptrs[] = {h->dom_d, h->tiles2d, h->ctiles, h->z, h->zf, h->dz, h->theta_ref, h->rho_bar, h->rt_bar,
+    void *ptrs[] = {h->dom_d, h->tiles2d, h->ctiles, h->z, h->zf, h->dz, h->geo, h->theta_ref, h->rho_bar, h->rt_bar,
                     h->p_bar, h->L_bar, h->B_bar, h->pool, h->pool1, h->atiles, h->cin, h->prec, h->sfh, h->nan_key,
                     h->step_ctr, h->staging, h->diag_out};
     for (void *p : ptrs)

@@ -52,6 +52,9 @@ struct Dom {
 // 1-D z vectors of all domains, concatenated (offset Dom::offz).
 struct ZVec {
     const double *z, *zf, *dz;
+    // per-level geometry of the w-p system: {1/dz_k, 1/(z_k+1 - z_k),
+    // 1/(dz_k + dz_k+1), g/(dz_k + dz_k+1)} (zeros on the top level)
+    const double4 *geo;
     // base state of the mixed form (zero/unused in the FP64 form)
     const double *rho_bar, *rt_bar, *p_bar, *L_bar, *B_bar;
 };
@@ -68,13 +71,20 @@ template <class R> __device__ __forceinline__ R jmin(R x, R y) {
     return (isnan(x) || isnan(y)) ? diff : am;
 }
 
+// ---- math in the compute type -----------------------------------------------
+// Division: IEEE division in double (the FP64 form performs the reference's
+// operations); the reciprocal-based fast division in float (mixed form).
+__device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float fdiv(float a, float b) { return __fdividef(a, b); }
+
+// dz-weighted face-to-centre interpolation (f2h, moist.jl:243-251)
 template <class R> __device__ __forceinline__ R f2h(R v1, R v2, R d1, R d2) {
-    return (v1 * d2 + v2 * d1) / (d1 + d2);
+    return fdiv(v1 * d2 + v2 * d1, d1 + d2);
 }
 
 // Koren (1993) limiter (moist.jl:844-847)
 template <class R> __device__ __forceinline__ R koren(R r) {
-    return jmax(R(0.0), jmin(R(2.0) * r, jmin((R(1.0) + R(2.0) * r) / R(3.0), R(2.0))));
+    return jmax(R(0.0), jmin(R(2.0) * r, jmin(fdiv(R(1.0) + R(2.0) * r, R(3.0)), R(2.0))));
 }
 
 // (-m + 5c + 2p), the numerator of the 3rd-order upwind face value
@@ -91,7 +101,6 @@ __device__ __forceinline__ int wrapx(int j, int nx) {
     return j;
 }
 
-// ---- math in the compute type -----------------------------------------------
 __device__ __forceinline__ double m_pow(double a, double b) { return pow(a, b); }
 __device__ __forceinline__ float m_pow(float a, float b) { return powf(a, b); }
 __device__ __forceinline__ double m_exp(double a) { return exp(a); }
@@ -111,16 +120,16 @@ __device__ __forceinline__ float m_expm1(float a) { return expm1f(a); }
 template <class R> __device__ __forceinline__ void sat_vp(R Tk, R &es, R &des) {
     R Tc = Tk - K<R>::T0;
     R tmp = K<R>::B_SAT + Tc;
-    es = K<R>::ES0 * m_exp(K<R>::A_SAT * Tc / tmp);
-    des = es * K<R>::A_SAT * (tmp - Tc) / (tmp * tmp);
+    es = K<R>::ES0 * m_exp(fdiv(K<R>::A_SAT * Tc, tmp));
+    des = fdiv(es * K<R>::A_SAT * (tmp - Tc), tmp * tmp);
 }
 
 // Saturation specific humidity and d qsat / d es (moist.jl:263-273)
 template <class R> __device__ __forceinline__ void sat_q(R p, R es, R &qs, R &dqs) {
     R tmp = p - (R(1.0) - K<R>::EPSvap) * es;
     if (tmp > K<R>::EPS) {
-        qs = K<R>::EPSvap * es / tmp;
-        dqs = K<R>::EPSvap * p / (tmp * tmp);
+        qs = fdiv(K<R>::EPSvap * es, tmp);
+        dqs = fdiv(K<R>::EPSvap * p, tmp * tmp);
     } else {
         qs = R(0.0);
         dqs = R(1.0);
