@@ -28,7 +28,10 @@
 //     with a zero carry and keeps the carry coefficients, the carries are
 //     scanned across chunks through LDS, and every level is corrected.
 
-constexpr int AC_THREADS = 1024;
+#ifndef MOIST_AC_THREADS
+#define MOIST_AC_THREADS 1024
+#endif
+constexpr int AC_THREADS = MOIST_AC_THREADS;
 constexpr int AC_L = 8;  // levels per chunk
 
 template <class S> struct AcBuf {

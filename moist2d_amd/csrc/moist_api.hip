@@ -74,6 +74,8 @@ struct moist_handle {
     int max_ka = 0, max_ia = 0;
     int4 *tiles2d = nullptr;
     int ntiles2d = 0;
+    int4 *stiles = nullptr;  // scalar-transport tiles (ST_X x ST_Z)
+    int nstiles = 0;
     int2 *ctiles = nullptr;
     int nctiles = 0;
     int4 *atiles = nullptr;  // acoustic tiles {member, i0, TX, nch}
@@ -237,7 +239,8 @@ void enqueue_stage(moist_handle *h, int in_set, int out_set, int s0_set, int div
         cur = nxt;
         ru = 1 - ru;
     }
-    hipLaunchKernelGGL(k_scalar<S>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, s0, out, w,
+    hipLaunchKernelGGL(k_scalar<S>, dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles, zv, in,
+                       s0, out, w,
                        (const S *)rup[ru], (const S *)rtp[cur], div, dt_ovr, stage);
 }
 
@@ -571,13 +574,15 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
     h->nzs = offz;
     h->ns_stage = {std::max(1, (h->ns + 2) / 3), std::max(1, (h->ns + 1) / 2), h->ns};
 
-    std::vector<int4> t2, ta;
+    std::vector<int4> t2, ta, ts;
     std::vector<int2> tc;
     const int acL = precision == MOIST_MIXED ? ac_levels<float>() : ac_levels<double>();
     for (int m = 0; m < n_members; ++m) {
         const Dom &D = h->dom_h[m];
         for (int k0 = 0; k0 < D.nz; k0 += TILE_Z)
             for (int i0 = 0; i0 < D.nx; i0 += TILE_X) t2.push_back(make_int4(m, i0, k0, 0));
+        for (int k0 = 0; k0 < D.nz; k0 += ST_Z)
+            for (int i0 = 0; i0 < D.nx; i0 += ST_X) ts.push_back(make_int4(m, i0, k0, 0));
         for (int i0 = 0; i0 < D.nx; i0 += COL_TILE) tc.push_back(make_int2(m, i0));
         int nch = 1;
         while (nch * acL < D.nz) nch *= 2;
@@ -591,6 +596,8 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
         for (int i0 = 0; i0 < D.nx; i0 += TX) ta.push_back(make_int4(m, i0, TX, nch));
     }
     t2 = xcd_order(t2, make_int4(-1, 0, 0, 0));
+    ts = xcd_order(ts, make_int4(-1, 0, 0, 0));
+    h->nstiles = (int)ts.size();
     tc = xcd_order(tc, make_int2(-1, 0));
     ta = xcd_order(ta, make_int4(-1, 0, 0, 0));
     h->natiles = (int)ta.size();
@@ -604,6 +611,8 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
     ALLOC(hp->tiles2d, sizeof(int4) * t2.size());
     ALLOC(hp->ctiles, sizeof(int2) * tc.size());
     ALLOC(hp->atiles, sizeof(int4) * ta.size());
+    ALLOC(hp->stiles, sizeof(int4) * ts.size());
+    HIPCHK(hp, hipMemcpy(hp->stiles, ts.data(), sizeof(int4) * ts.size(), hipMemcpyHostToDevice));
     HIPCHK(hp, hipMemcpy(hp->atiles, ta.data(), sizeof(int4) * ta.size(), hipMemcpyHostToDevice));
     HIPCHK(hp, hipMemcpy(hp->dom_d, hp->dom_h.data(), sizeof(Dom) * n_members, hipMemcpyHostToDevice));
     HIPCHK(hp, hipMemcpy(hp->tiles2d, t2.data(), sizeof(int4) * t2.size(), hipMemcpyHostToDevice));
@@ -822,7 +831,7 @@ void moist_destroy(moist_handle *h) {
     if (!h) return;
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     for (auto e : h->ev_pool) hipEventDestroy(e);
-    void *ptrs[] = {h->dom_d, h->tiles2d, h->ctiles, h->z, h->zf, h->dz, h->geo, h->theta_ref, h->rho_bar, h->rt_bar,
+    void *ptrs[] = {h->dom_d, h->tiles2d, h->stiles, h->ctiles, h->z, h->zf, h->dz, h->geo, h->theta_ref, h->rho_bar, h->rt_bar,
                     h->p_bar, h->L_bar, h->B_bar, h->pool, h->pool1, h->atiles, h->cin, h->prec, h->sfh, h->nan_key,
                     h->step_ctr, h->staging, h->diag_out};
     for (void *p : ptrs)

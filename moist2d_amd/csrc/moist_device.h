@@ -73,9 +73,10 @@ template <class R> __device__ __forceinline__ R jmin(R x, R y) {
 
 // ---- math in the compute type -----------------------------------------------
 // Division: IEEE division in double (the FP64 form performs the reference's
-// operations); the reciprocal-based fast division in float (mixed form).
+// operations); in float (mixed form) a times the hardware reciprocal
+// v_rcp_f32 (about 1 ulp).
 __device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
-__device__ __forceinline__ float fdiv(float a, float b) { return __fdividef(a, b); }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 // dz-weighted face-to-centre interpolation (f2h, moist.jl:243-251)
 template <class R> __device__ __forceinline__ R f2h(R v1, R v2, R d1, R d2) {
@@ -102,9 +103,13 @@ __device__ __forceinline__ int wrapx(int j, int nx) {
 }
 
 __device__ __forceinline__ double m_pow(double a, double b) { return pow(a, b); }
-__device__ __forceinline__ float m_pow(float a, float b) { return powf(a, b); }
+// float pow/exp of the mixed form from the hardware base-2 log/exp
+// (v_log_f32, v_exp_f32); arguments of pow here are positive or zero.
+__device__ __forceinline__ float m_pow(float a, float b) {
+    return __builtin_amdgcn_exp2f(b * __builtin_amdgcn_logf(a));
+}
 __device__ __forceinline__ double m_exp(double a) { return exp(a); }
-__device__ __forceinline__ float m_exp(float a) { return expf(a); }
+__device__ __forceinline__ float m_exp(float a) { return __builtin_amdgcn_exp2f(a * 1.4426950408889634f); }
 __device__ __forceinline__ double m_sqrt(double a) { return sqrt(a); }
 __device__ __forceinline__ float m_sqrt(float a) { return sqrtf(a); }
 __device__ __forceinline__ double m_tanh(double a) { return tanh(a); }

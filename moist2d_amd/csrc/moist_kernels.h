@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(256) k_diag(const Dom *__restrict__ doms, cons
     const int ip = wrapx(i + 1, nx);
     const R rho = full_rho(in, c, zv, D, k);
     const R rt = full_rt(in, c, zv, D, k);
-    const R qv = in.f[QV][c] / rho, qc = in.f[QC][c] / rho, qr = in.f[QR][c] / rho;
+    const R qv = fdiv(in.f[QV][c], rho), qc = fdiv(in.f[QC][c], rho), qr = fdiv(in.f[QR][c], rho);
     const R qd = R(1.0) - qv - qc - qr;
     const R r = KC::Rd * qd + KC::Rv * qv;
     const R cp = KC::Cpd * qd + KC::Cpv * qv + KC::Cpl * (qc + qr);
@@ -93,10 +93,10 @@ __global__ void __launch_bounds__(256) k_diag(const Dom *__restrict__ doms, cons
         // g - g_bar = (Cpd dR - Rd dCp) / (cv Cvd).
         const R dR = (KC::Rv - KC::Rd) * qv - KC::Rd * (qc + qr);
         const R dCp = (KC::Cpv - KC::Cpd) * qv + (KC::Cpl - KC::Cpd) * (qc + qr);
-        const R dg = (KC::Cpd * dR - KC::Rd * dCp) / (cv * KC::Cvd);
-        const R gam = cp / cv;
+        const R dg = fdiv(KC::Cpd * dR - KC::Rd * dCp, cv * KC::Cvd);
+        const R gam = fdiv(cp, cv);
         const R rtb = R(zv.rt_bar[D.offz + k]);
-        const R E = dg * R(zv.L_bar[D.offz + k]) + gam * (m_log1p(dR / KC::Rd) + m_log1p(in.f[RT][c] / rtb));
+        const R E = dg * R(zv.L_bar[D.offz + k]) + gam * (m_log1p(fdiv(dR, KC::Rd)) + m_log1p(fdiv(in.f[RT][c], rtb)));
         const double pb = zv.p_bar[D.offz + k];
         pres = R(pb) * m_expm1(E);
         pfull = R(pb) + pres;
@@ -105,14 +105,14 @@ __global__ void __launch_bounds__(256) k_diag(const Dom *__restrict__ doms, cons
         pfull = pres;
     }
     w.pres[c] = pres;
-    w.theta[c] = rt / rho;
-    w.rt2[c] = cp / cv * pfull / rt;
+    w.theta[c] = fdiv(rt, rho);
+    w.rt2[c] = fdiv(fdiv(cp, cv) * pfull, rt);
     const int64_t cip = D.off2 + (int64_t)k * nx + ip;
-    w.uf[c] = in.f[RHOU][c] * R(2.0) / (rho + full_rho(in, cip, zv, D, k));
+    w.uf[c] = fdiv(in.f[RHOU][c] * R(2.0), rho + full_rho(in, cip, zv, D, k));
     if (k < nz - 1) {
         const int64_t ckp = c + nx;
         const double *dz = zv.dz + D.offz;
-        w.wf[c] = in.f[RHOW][c] / f2h(rho, full_rho(in, ckp, zv, D, k + 1), R(dz[k]), R(dz[k + 1]));
+        w.wf[c] = fdiv(in.f[RHOW][c], f2h(rho, full_rho(in, ckp, zv, D, k + 1), R(dz[k]), R(dz[k + 1])));
     } else {
         w.wf[c] = R(0.0);
     }
@@ -153,48 +153,48 @@ __global__ void __launch_bounds__(256) k_slow(const Dom *__restrict__ doms, cons
     {
         R fkp = k == ke ? R(0.0) : rho_w[c];
         R fkm = k == 0 ? R(0.0) : rho_w[c - nx];
-        R adv = -((rho_u[c] - rho_u[row + im]) / dx + (fkp - fkm) / DZ(k));
+        R adv = -(fdiv(rho_u[c] - rho_u[row + im], dx) + fdiv(fkp - fkm, DZ(k)));
         w.R_rho[c] = adv + w.d_rho[c];
     }
     // ---- R_rho_u (moist.jl:1098-1179)
     {
         R rho_u_ip = R(0.5) * (rho_u[c] + rho_u[row + ip]);
         R fip, fim;
-        if (rho_u_ip > EPS) fip = rho_u_ip * upw3(U(k, im), U(k, i), U(k, ip)) / R(6.0);
-        else if (rho_u_ip < -EPS) fip = rho_u_ip * upw3(U(k, ip2), U(k, ip), U(k, i)) / R(6.0);
+        if (rho_u_ip > EPS) fip = fdiv(rho_u_ip * upw3(U(k, im), U(k, i), U(k, ip)), R(6.0));
+        else if (rho_u_ip < -EPS) fip = fdiv(rho_u_ip * upw3(U(k, ip2), U(k, ip), U(k, i)), R(6.0));
         else fip = rho_u_ip * (U(k, i) + U(k, ip)) * R(0.5);
         R rho_u_im = R(0.5) * (rho_u[row + im] + rho_u[c]);
-        if (rho_u_im > EPS) fim = rho_u_im * upw3(U(k, im2), U(k, im), U(k, i)) / R(6.0);
-        else if (rho_u_im < -EPS) fim = rho_u_im * upw3(U(k, ip), U(k, i), U(k, im)) / R(6.0);
+        if (rho_u_im > EPS) fim = fdiv(rho_u_im * upw3(U(k, im2), U(k, im), U(k, i)), R(6.0));
+        else if (rho_u_im < -EPS) fim = fdiv(rho_u_im * upw3(U(k, ip), U(k, i), U(k, im)), R(6.0));
         else fim = rho_u_im * (U(k, im) + U(k, i)) * R(0.5);
-        R druudx = (fip - fim) / dx;
+        R druudx = fdiv(fip - fim, dx);
         R fkp, fkm;
         if (k == ke) fkp = R(0.0);
         else {
             R rw = R(0.5) * (rho_w[c] + rho_w[row + ip]);
-            if (rw > EPS && k > 0) fkp = rw * upw3(U(k - 1, i), U(k, i), U(k + 1, i)) / R(6.0);
-            else if (rw < -EPS && k < ke - 1) fkp = rw * upw3(U(k + 2, i), U(k + 1, i), U(k, i)) / R(6.0);
+            if (rw > EPS && k > 0) fkp = fdiv(rw * upw3(U(k - 1, i), U(k, i), U(k + 1, i)), R(6.0));
+            else if (rw < -EPS && k < ke - 1) fkp = fdiv(rw * upw3(U(k + 2, i), U(k + 1, i), U(k, i)), R(6.0));
             else fkp = rw * f2h(U(k, i), U(k + 1, i), DZ(k), DZ(k + 1));
         }
         if (k == 0) fkm = R(0.0);
         else {
             R rw = R(0.5) * (rho_w[c - nx] + rho_w[row - nx + ip]);
-            if (rw > EPS && k > 1) fkm = rw * upw3(U(k - 2, i), U(k - 1, i), U(k, i)) / R(6.0);
-            else if (rw < -EPS && k < ke) fkm = rw * upw3(U(k + 1, i), U(k, i), U(k - 1, i)) / R(6.0);
+            if (rw > EPS && k > 1) fkm = fdiv(rw * upw3(U(k - 2, i), U(k - 1, i), U(k, i)), R(6.0));
+            else if (rw < -EPS && k < ke) fkm = fdiv(rw * upw3(U(k + 1, i), U(k, i), U(k - 1, i)), R(6.0));
             else fkm = rw * f2h(U(k - 1, i), U(k, i), DZ(k - 1), DZ(k));
         }
-        R drwudz = (fkp - fkm) / DZ(k);
+        R drwudz = fdiv(fkp - fkm, DZ(k));
         R adv = -druudx - drwudz;
-        R u_ip = (U(k, ip) - U(k, i)) / dx;
-        R u_im = (U(k, i) - U(k, im)) / dx;
-        R u_xx = (u_ip * RHOF(k, ip) - u_im * RHOF(k, i)) / dx;
-        R u_kp = k == ke ? R(0.0) : (U(k + 1, i) - U(k, i)) / (Z(k + 1) - Z(k));
-        R u_km = k == 0 ? R(0.0) : (U(k, i) - U(k - 1, i)) / (Z(k) - Z(k - 1));
+        R u_ip = fdiv(U(k, ip) - U(k, i), dx);
+        R u_im = fdiv(U(k, i) - U(k, im), dx);
+        R u_xx = fdiv(u_ip * RHOF(k, ip) - u_im * RHOF(k, i), dx);
+        R u_kp = k == ke ? R(0.0) : fdiv(U(k + 1, i) - U(k, i), (Z(k + 1) - Z(k)));
+        R u_km = k == 0 ? R(0.0) : fdiv(U(k, i) - U(k - 1, i), (Z(k) - Z(k - 1)));
         R rho_kp = k == ke ? R(0.0) : R(0.25) * (RHOF(k, i) + RHOF(k + 1, i) + RHOF(k, ip) + RHOF(k + 1, ip));
         R rho_km = k == 0 ? R(0.0) : R(0.25) * (RHOF(k - 1, i) + RHOF(k, i) + RHOF(k - 1, ip) + RHOF(k, ip));
-        R u_zz = (u_kp * rho_kp - u_km * rho_km) / DZ(k);
+        R u_zz = fdiv(u_kp * rho_kp - u_km * rho_km, DZ(k));
         R diff = R(D.nu) * (u_xx + u_zz);
-        R dpdx = (w.pres[row + ip] - w.pres[c]) / dx;
+        R dpdx = fdiv(w.pres[row + ip] - w.pres[c], dx);
         R dup = u_phys_2d ? w.d_u2d[c] : (k == 0 ? w.d_u_sf[D.offx + i] : R(0.0));
         w.R_rhou[c] = adv - dpdx + diff + dup;
     }
@@ -202,40 +202,40 @@ __global__ void __launch_bounds__(256) k_slow(const Dom *__restrict__ doms, cons
     if (k < ke) {
         R rho_u_ip = f2h(rho_u[c], rho_u[c + nx], Z(k), Z(k + 1));
         R fip, fim;
-        if (rho_u_ip > EPS) fip = rho_u_ip * upw3(W(k, im), W(k, i), W(k, ip)) / R(6.0);
-        else if (rho_u_ip < -EPS) fip = rho_u_ip * upw3(W(k, ip2), W(k, ip), W(k, i)) / R(6.0);
+        if (rho_u_ip > EPS) fip = fdiv(rho_u_ip * upw3(W(k, im), W(k, i), W(k, ip)), R(6.0));
+        else if (rho_u_ip < -EPS) fip = fdiv(rho_u_ip * upw3(W(k, ip2), W(k, ip), W(k, i)), R(6.0));
         else fip = rho_u_ip * (W(k, i) + W(k, ip)) * R(0.5);
         R rho_u_im = f2h(rho_u[row + im], rho_u[row + nx + im], DZ(k), DZ(k + 1));
-        if (rho_u_im > EPS) fim = rho_u_im * upw3(W(k, im2), W(k, im), W(k, i)) / R(6.0);
-        else if (rho_u_im < -EPS) fim = rho_u_im * upw3(W(k, ip), W(k, i), W(k, im)) / R(6.0);
+        if (rho_u_im > EPS) fim = fdiv(rho_u_im * upw3(W(k, im2), W(k, im), W(k, i)), R(6.0));
+        else if (rho_u_im < -EPS) fim = fdiv(rho_u_im * upw3(W(k, ip), W(k, i), W(k, im)), R(6.0));
         else fim = rho_u_im * (W(k, im) + W(k, i)) * R(0.5);
-        R drwudx = (fip - fim) / dx;
+        R drwudx = fdiv(fip - fim, dx);
         R fkp, fkm;
         if (k == ke - 1) fkp = R(0.0);
         else {
             R rw = f2h(rho_w[c], rho_w[c + nx], DZ(k), DZ(k + 1));
-            if (rw > EPS && k > 0) fkp = rw * upw3(W(k - 1, i), W(k, i), W(k + 1, i)) / R(6.0);
-            else if (rw < -EPS && k < ke - 2) fkp = rw * upw3(W(k + 2, i), W(k + 1, i), W(k, i)) / R(6.0);
+            if (rw > EPS && k > 0) fkp = fdiv(rw * upw3(W(k - 1, i), W(k, i), W(k + 1, i)), R(6.0));
+            else if (rw < -EPS && k < ke - 2) fkp = fdiv(rw * upw3(W(k + 2, i), W(k + 1, i), W(k, i)), R(6.0));
             else fkp = rw * f2h(W(k, i), W(k + 1, i), DZ(k), DZ(k + 1));
         }
         if (k == 0) fkm = R(0.0);
         else {
             R rw = f2h(rho_w[c - nx], rho_w[c], DZ(k - 1), DZ(k));
-            if (rw > EPS && k > 1) fkm = rw * upw3(W(k - 2, i), W(k - 1, i), W(k, i)) / R(6.0);
-            else if (rw < -EPS && k < ke - 1) fkm = rw * upw3(W(k + 1, i), W(k, i), W(k - 1, i)) / R(6.0);
+            if (rw > EPS && k > 1) fkm = fdiv(rw * upw3(W(k - 2, i), W(k - 1, i), W(k, i)), R(6.0));
+            else if (rw < -EPS && k < ke - 1) fkm = fdiv(rw * upw3(W(k + 1, i), W(k, i), W(k - 1, i)), R(6.0));
             else fkm = rw * f2h(W(k - 1, i), W(k, i), DZ(k - 1), DZ(k));
         }
         R zz = Z(k + 1) - Z(k);
-        R drwwdz = (fkp - fkm) / zz;
+        R drwwdz = fdiv(fkp - fkm, zz);
         R adv = -drwudx - drwwdz;
-        R w_ip = (W(k, ip) - W(k, i)) / dx;
-        R w_im = (W(k, i) - W(k, im)) / dx;
+        R w_ip = fdiv(W(k, ip) - W(k, i), dx);
+        R w_im = fdiv(W(k, i) - W(k, im), dx);
         R rho_ip = R(0.25) * (RHOF(k, i) + RHOF(k + 1, i) + RHOF(k, ip) + RHOF(k + 1, ip));
         R rho_im = R(0.25) * (RHOF(k, im) + RHOF(k + 1, im) + RHOF(k, i) + RHOF(k + 1, i));
-        R w_xx = (w_ip * rho_ip - w_im * rho_im) / dx;
-        R w_kp = k == ke - 1 ? R(0.0) : (W(k + 1, i) - W(k, i)) / DZ(k + 1);
-        R w_km = k == 0 ? R(0.0) : (W(k, i) - W(k - 1, i)) / DZ(k);
-        R w_zz = (w_kp * RHOF(k + 1, i) - w_km * RHOF(k, i)) / zz;
+        R w_xx = fdiv(w_ip * rho_ip - w_im * rho_im, dx);
+        R w_kp = k == ke - 1 ? R(0.0) : fdiv(W(k + 1, i) - W(k, i), DZ(k + 1));
+        R w_km = k == 0 ? R(0.0) : fdiv(W(k, i) - W(k - 1, i), DZ(k));
+        R w_zz = fdiv(w_kp * RHOF(k + 1, i) - w_km * RHOF(k, i), zz);
         R diff = R(D.nu) * (w_xx + w_zz);
         R sigma = R(0.0);
         const R zf_sp = R(D.zf_sponge), H = R(D.H);
@@ -245,11 +245,11 @@ __global__ void __launch_bounds__(256) k_slow(const Dom *__restrict__ doms, cons
         R res;
         if constexpr (is_mixed<S>()) {
             // -dp/dz + buoyancy = B_bar - dp'/dz - g (rho'_k + rho'_k+1)/2
-            R dpdz = (w.pres[c + nx] - w.pres[c]) / zz;
+            R dpdz = fdiv(w.pres[c + nx] - w.pres[c], zz);
             R buoy = -KC::GRAV * (in.f[RHO][c] + in.f[RHO][c + nx]) * R(0.5);
             res = adv + ((R(zv.B_bar[D.offz + k]) - dpdz) + buoy) + diff + sponge;
         } else {
-            R dpdz = (w.pres[c + nx] - w.pres[c]) / zz;
+            R dpdz = fdiv(w.pres[c + nx] - w.pres[c], zz);
             R buoy = -KC::GRAV * (RHOF(k, i) + RHOF(k + 1, i)) * R(0.5);
             res = adv - dpdz + buoy + diff + sponge;
         }
@@ -261,27 +261,27 @@ __global__ void __launch_bounds__(256) k_slow(const Dom *__restrict__ doms, cons
     {
         R ru = rho_u[c], rum = rho_u[row + im];
         R fip, fim, fkp, fkm;
-        if (ru > EPS) fip = ru * upw3(TH(k, im), TH(k, i), TH(k, ip)) / R(6.0);
-        else if (ru < -EPS) fip = ru * upw3(TH(k, ip2), TH(k, ip), TH(k, i)) / R(6.0);
+        if (ru > EPS) fip = fdiv(ru * upw3(TH(k, im), TH(k, i), TH(k, ip)), R(6.0));
+        else if (ru < -EPS) fip = fdiv(ru * upw3(TH(k, ip2), TH(k, ip), TH(k, i)), R(6.0));
         else fip = ru * (TH(k, i) + TH(k, ip)) * R(0.5);
-        if (rum > EPS) fim = rum * upw3(TH(k, im2), TH(k, im), TH(k, i)) / R(6.0);
-        else if (rum < -EPS) fim = rum * upw3(TH(k, ip), TH(k, i), TH(k, im)) / R(6.0);
+        if (rum > EPS) fim = fdiv(rum * upw3(TH(k, im2), TH(k, im), TH(k, i)), R(6.0));
+        else if (rum < -EPS) fim = fdiv(rum * upw3(TH(k, ip), TH(k, i), TH(k, im)), R(6.0));
         else fim = rum * (TH(k, im) + TH(k, i)) * R(0.5);
         if (k == ke) fkp = R(0.0);
         else {
             R rw = rho_w[c];
-            if (rw > EPS && k > 0) fkp = rw * upw3(TH(k - 1, i), TH(k, i), TH(k + 1, i)) / R(6.0);
-            else if (rw < -EPS && k < ke - 1) fkp = rw * upw3(TH(k + 2, i), TH(k + 1, i), TH(k, i)) / R(6.0);
+            if (rw > EPS && k > 0) fkp = fdiv(rw * upw3(TH(k - 1, i), TH(k, i), TH(k + 1, i)), R(6.0));
+            else if (rw < -EPS && k < ke - 1) fkp = fdiv(rw * upw3(TH(k + 2, i), TH(k + 1, i), TH(k, i)), R(6.0));
             else fkp = rw * f2h(TH(k, i), TH(k + 1, i), DZ(k), DZ(k + 1));
         }
         if (k == 0) fkm = R(0.0);
         else {
             R rw = rho_w[c - nx];
-            if (rw > EPS && k > 1) fkm = rw * upw3(TH(k - 2, i), TH(k - 1, i), TH(k, i)) / R(6.0);
-            else if (rw < -EPS && k < ke) fkm = rw * upw3(TH(k + 1, i), TH(k, i), TH(k - 1, i)) / R(6.0);
+            if (rw > EPS && k > 1) fkm = fdiv(rw * upw3(TH(k - 2, i), TH(k - 1, i), TH(k, i)), R(6.0));
+            else if (rw < -EPS && k < ke) fkm = fdiv(rw * upw3(TH(k + 1, i), TH(k, i), TH(k - 1, i)), R(6.0));
             else fkm = rw * f2h(TH(k - 1, i), TH(k, i), DZ(k - 1), DZ(k));
         }
-        R adv = -((fip - fim) / dx + (fkp - fkm) / DZ(k));
+        R adv = -(fdiv(fip - fim, dx) + fdiv(fkp - fkm, DZ(k)));
         w.R_rt[c] = adv + w.d_rt[c];
     }
     // ---- perturbations from the step start, mass-flux accumulators
@@ -295,158 +295,7 @@ __global__ void __launch_bounds__(256) k_slow(const Dom *__restrict__ doms, cons
 
 #include "moist_acoustic.h"
 
-// =============================================================================
-// Koren flux at x-face ii+1/2 (compute_limited_flux_x, moist.jl:850-892).
-// =============================================================================
-template <class S>
-__device__ __forceinline__ S flux_x(const S *q, const SSet<S> &in, const ZVec &zv, const Dom &D, int64_t row,
-                                    int k, int ii, S rhou_face) {
-    using R = S;
-    const R EPS = K<R>::EPS;
-    const int nx = D.nx;
-    auto Q = [&](int jj) -> R {
-        const int64_t c = row + jj;
-        return q[c] / full_rho(in, c, zv, D, k);
-    };
-    const int i0 = wrapx(ii, nx), i1 = wrapx(ii + 1, nx);
-    R s_i = Q(i0), s_ip1 = Q(i1), s_face;
-    if (rhou_face > EPS) {
-        R s_im1 = Q(wrapx(ii - 1, nx));
-        R r_num = s_ip1 - s_i, r_den = s_i - s_im1;
-        R r = m_fabs(r_den) < EPS ? R(1.0) : r_num / r_den;
-        s_face = s_i + R(0.5) * koren(r) * r_den;
-    } else if (rhou_face < -EPS) {
-        R s_ip2 = Q(wrapx(ii + 2, nx));
-        R r_num = s_i - s_ip1, r_den = s_ip1 - s_ip2;
-        R r = m_fabs(r_den) < EPS ? R(1.0) : r_num / r_den;
-        s_face = s_ip1 + R(0.5) * koren(r) * r_den;
-    } else {
-        s_face = R(0.5) * (s_i + s_ip1);
-    }
-    return s_face * rhou_face;
-}
-
-// Koren flux at z-face kk+1/2 of column i (compute_limited_flux_z,
-// moist.jl:895-945), including the positivity limiters that read the first
-// halo column through a linear index and divide by the large-step dt.
-template <class S>
-__device__ __forceinline__ S flux_z(const S *q, const SSet<S> &in, const ZVec &zv, const Dom &D, int kk, int i,
-                                    S rhow_face) {
-    using R = S;
-    const R EPS = K<R>::EPS;
-    const int nx = D.nx, ke = D.nz - 1;
-    const double *dzd = zv.dz + D.offz;
-    auto Q = [&](int k2) -> R {
-        const int64_t c = D.off2 + (int64_t)k2 * nx + i;
-        return q[c] / full_rho(in, c, zv, D, k2);
-    };
-    auto LIN = [&](int k2) -> R { return q[D.off2 + (int64_t)k2 * nx + D.lin_col]; };
-    const R dt = R(D.dt);
-    R s_k = Q(kk), s_kp1 = Q(kk + 1), s_face;
-    if (rhow_face > EPS) {
-        if (kk == 0) {
-            s_face = f2h(s_k, s_kp1, R(dzd[kk]), R(dzd[kk + 1]));
-            s_face = jmin(s_face, LIN(kk) / (rhow_face * dt));
-        } else {
-            R s_km1 = Q(kk - 1);
-            R r_num = s_kp1 - s_k, r_den = s_k - s_km1;
-            R r = m_fabs(r_den) < EPS ? R(1.0) : r_num / r_den;
-            s_face = s_k + R(0.5) * koren(r) * r_den;
-        }
-    } else if (rhow_face < -EPS) {
-        if (kk == ke - 1) {
-            s_face = f2h(s_k, s_kp1, R(dzd[kk]), R(dzd[kk + 1]));
-            s_face = jmin(s_face, LIN(kk + 1) / (-rhow_face * dt));
-        } else {
-            R s_kp2 = Q(kk + 2);
-            R r_num = s_k - s_kp1, r_den = s_kp1 - s_kp2;
-            R r = m_fabs(r_den) < EPS ? R(1.0) : r_num / r_den;
-            s_face = s_kp1 - R(0.5) * koren(r) * r_den;
-        }
-    } else {
-        s_face = f2h(s_k, s_kp1, R(dzd[kk]), R(dzd[kk + 1]));
-    }
-    return s_face * rhow_face;
-}
-
-// =============================================================================
-// Scalar transport (moist.jl:1469-1518, read from the stage-input snapshot)
-// and the final update (moist.jl:1522-1539), with the NaN checks
-// (moist.jl:1543-1564).  Writes the stage output set.
-// =============================================================================
-template <class S>
-__global__ void __launch_bounds__(256) k_scalar(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
-                                                ZVec zv, SSet<S> in, SSet<S> s0, SSet<S> out, Work<S> w,
-                                                const S *__restrict__ rup_fin, const S *__restrict__ rtp_fin,
-                                                int div, double dt_ovr, int stage) {
-    using R = S;
-    const int4 t = tiles[blockIdx.x];
-    if (t.x < 0) return;  // padding tile of the XCD ordering
-    const int m = t.x;
-    const Dom &D = doms[m];
-    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
-    const int nx = D.nx, nz = D.nz, ke = nz - 1;
-    if (i >= nx || k >= nz) return;
-    const int64_t row = D.off2 + (int64_t)k * nx;
-    const int64_t c = row + i;
-    const int im = wrapx(i - 1, nx), ip = wrapx(i + 1, nx);
-    const double *zd = zv.z + D.offz, *dzd = zv.dz + D.offz;
-    const R dx = R(D.dx), kap = R(D.kappa);
-    const R dt = stage_dt<R>(D, div, dt_ovr);
-    auto RHOF = [&](int kk, int ii) -> R { return full_rho(in, D.off2 + (int64_t)kk * nx + ii, zv, D, kk); };
-    auto DZ = [&](int kk) -> R { return R(dzd[kk]); };
-    const R mxp = w.mom_x[c], mxm = w.mom_x[row + im];
-    const R mzp = k < ke ? w.mom_z[c] : R(0.0);
-    const R mzm = k > 0 ? w.mom_z[c - nx] : R(0.0);
-    const R rc = RHOF(k, i), rip = RHOF(k, ip), rim = RHOF(k, im);
-    const R rkp = k < ke ? RHOF(k + 1, i) : R(0.0);
-    const R rkm = k > 0 ? RHOF(k - 1, i) : R(0.0);
-    const S *dph[3] = {w.d_qv, w.d_qc, w.d_qr};
-    int nan_f = 0;
-#pragma unroll
-    for (int sc = 0; sc < 3; ++sc) {
-        const S *q = in.f[QV + sc];
-        R fxp = flux_x(q, in, zv, D, row, k, i, mxp);
-        R fxm = flux_x(q, in, zv, D, row, k, i - 1, mxm);
-        R adv_x = (fxp - fxm) / dx;
-        R fzp = k == ke ? R(0.0) : flux_z(q, in, zv, D, k, i, mzp);
-        R fzm = k == 0 ? R(0.0) : flux_z(q, in, zv, D, k - 1, i, mzm);
-        R adv_z = (fzp - fzm) / DZ(k);
-        R adv = -(adv_x + adv_z);
-        R sc_c = q[c] / rc;
-        R sc_ip = q[row + ip] / rip;
-        R sc_im = q[row + im] / rim;
-        R sc_kp = k == ke ? R(0.0) : q[c + nx] / rkp;
-        R sc_km = k == 0 ? R(0.0) : q[c - nx] / rkm;
-        R dip = (rip + rc) * R(0.5) * (sc_ip - sc_c) / dx;
-        R dim = (rc + rim) * R(0.5) * (sc_c - sc_im) / dx;
-        R diff_x = kap * (dip - dim) / dx;
-        R dkp = k == ke ? R(0.0) : f2h(rkp, rc, DZ(k + 1), DZ(k)) * (sc_kp - sc_c) / (R(zd[k + 1]) - R(zd[k]));
-        R dkm = k == 0 ? R(0.0) : f2h(rc, rkm, DZ(k), DZ(k - 1)) * (sc_c - sc_km) / (R(zd[k]) - R(zd[k - 1]));
-        R diff_z = kap * (dkp - dkm) / DZ(k);
-        R v = s0.f[QV + sc][c] + dt * (adv + (diff_x + diff_z) + dph[sc][c]);
-        out.f[QV + sc][c] = v;
-        if (isnan(v) && !nan_f) nan_f = 5 + sc;
-    }
-    const R vr = in.f[RHO][c] + w.rp[c];
-    const R vu = in.f[RHOU][c] + rup_fin[c];
-    const R vt = in.f[RT][c] + rtp_fin[c];
-    const R vw = k < ke ? in.f[RHOW][c] + w.rwp[c] : in.f[RHOW][c];
-    out.f[RHO][c] = vr;
-    out.f[RHOU][c] = vu;
-    out.f[RT][c] = vt;
-    out.f[RHOW][c] = vw;
-    int f = 0;
-    if (isnan(vr)) f = 1;
-    else if (isnan(vu)) f = 2;
-    else if (isnan(vw)) f = 3;
-    else if (isnan(vt)) f = 4;
-    else f = nan_f;
-    if (f) report_nan(w.nan_key, ((*w.step_ctr * 3 + stage) << 4) | f);
-    // stale halo values surface_fluxes! reads at the next step (moist.jl:723,736)
-    if (k == 0 && i == nx - 1) w.sfh[2 * m + 0] = double(in.f[RHOU][c]);
-    if (k == 0 && i == 0) w.sfh[2 * m + 1] = double(rc);
-}
+#include "moist_scalar.h"
 
 // =============================================================================
 // Surface fluxes (surface_fluxes!, moist.jl:710-752), one thread per column.
@@ -574,7 +423,7 @@ template <class S>
 __device__ __forceinline__ S term_velocity(S rho_qr, S rho) {
     using R = S;
     const R eff = jmax(R(0.0), rho_qr);
-    const R v = K<R>::A_term * m_pow(eff / rho, K<R>::B_term);
+    const R v = K<R>::A_term * m_pow(fdiv(eff, rho), K<R>::B_term);
     return eff > R(1e-12) ? v : R(0.0);
 }
 
@@ -596,17 +445,17 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
     const R rt = full_rt(s, c, zv, D, k);
     R dqv = R(0.0), dqc = R(0.0), dqr = R(0.0), dth = R(0.0);
     if (fl & F_MP) {
-        const R rinv = R(1.0) / rho_k;
+        const R rinv = fdiv(R(1.0), rho_k);
         R theta_k = rt * rinv;
         R qv = s.f[QV][c] * rinv, qc = s.f[QC][c] * rinv, qr = s.f[QR][c] * rinv;
         const R qd = R(1.0) - qv - qc - qr;
         R r = KC::Rd * qd + KC::Rv * qv;
         R cp = KC::Cpd * qd + KC::Cpv * qv + KC::Cpl * (qc + qr);
         R cv = cp - r;
-        R Pi = m_pow(r * rt / KC::P0, r / cv);
+        R Pi = m_pow(fdiv(r * rt, KC::P0), fdiv(r, cv));
         R Tk = theta_k * Pi;
-        R pk = KC::P0 * m_pow(Pi, cp / r);
-        R lhe = KC::Lv / (cp * Pi);
+        R pk = KC::P0 * m_pow(Pi, fdiv(cp, r));
+        R lhe = fdiv(KC::Lv, cp * Pi);
         const R qv_init = qv;
         R qsat = R(0.0);
         for (int it = 0; it < 3; ++it) {
@@ -614,7 +463,7 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
             sat_vp(Tk, es, des);
             sat_q(pk, es, qsat, dqs_des);
             const R dqs_dT = dqs_des * des;
-            R ce = (qv - qsat) / (R(1.0) + (KC::Lv / cp) * dqs_dT);
+            R ce = fdiv(qv - qsat, R(1.0) + fdiv(KC::Lv, cp) * dqs_dT);
             ce = jmin(ce, qv);
             ce = jmax(ce, -qc);
             qv -= ce;
@@ -623,19 +472,19 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
             r = KC::Rd * qd + KC::Rv * qv;
             cp = KC::Cpd * qd + KC::Cpv * qv + KC::Cpl * (qc + qr);
             cv = cp - r;
-            Pi = m_pow(r * rho_k * theta_k / KC::P0, r / cv);
+            Pi = m_pow(fdiv(r * rho_k * theta_k, KC::P0), fdiv(r, cv));
             Tk = theta_k * Pi;
-            pk = KC::P0 * m_pow(Pi, cp / r);
-            lhe = KC::Lv / (cp * Pi);
+            pk = KC::P0 * m_pow(Pi, fdiv(cp, r));
+            lhe = fdiv(KC::Lv, cp * Pi);
         }
         const R ce = qv_init - qv;
-        const R rate = ce * rho_k / dt;
+        const R rate = fdiv(ce * rho_k, dt);
         dqv -= rate;
         dqc += rate;
         dth += rate * lhe;
         if (fl & F_AUTO) {
             R ra = R(D.K_auto) * jmax(R(0.0), qc - R(D.Qc_threshold));
-            ra = jmin(ra, qc / dt);
+            ra = jmin(ra, fdiv(qc, dt));
             qc -= ra * dt;
             qr += ra * dt;
             ra *= rho_k;
@@ -646,7 +495,7 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
             R ra = R(0.0);
             if (qr > R(1e-12) && qc > R(1e-12)) {
                 ra = R(D.K_accr) * qc * qr;
-                ra = jmin(ra, qc / dt);
+                ra = jmin(ra, fdiv(qc, dt));
             }
             qc -= ra * dt;
             qr += ra * dt;
@@ -658,7 +507,7 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
             R re = R(0.0);
             if (qr > R(1e-12) && qv < qsat) {
                 re = KC::K_evap * (qsat - qv) * m_sqrt(rho_k * qr);
-                re = jmin(re, qr) / dt;
+                re = fdiv(jmin(re, qr), dt);
             }
             re *= rho_k;
             dqr -= re;
@@ -666,8 +515,8 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
             r = KC::Rd * qd + KC::Rv * qv;
             cp = KC::Cpd * qd + KC::Cpv * qv + KC::Cpl * (qc + qr);
             cv = cp - r;
-            Pi = m_pow(r * rho_k * theta_k / KC::P0, r / cv);
-            lhe = KC::Lv / (cp * Pi);
+            Pi = m_pow(fdiv(r * rho_k * theta_k, KC::P0), fdiv(r, cv));
+            lhe = fdiv(KC::Lv, cp * Pi);
             dth -= re * lhe;
         }
     }
@@ -680,7 +529,7 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
             const int64_t cu = c + nx;
             fkp = s.f[QR][cu] * term_velocity(s.f[QR][cu], full_rho(s, cu, zv, D, k + 1));
         }
-        dsed = (fkp - fkm) / R(zv.dz[D.offz + k]);
+        dsed = fdiv(fkp - fkm, R(zv.dz[D.offz + k]));
         if (k == 0) {
             const int64_t x = D.offx + i;
             if constexpr (is_mixed<S>()) w.prec[x] += double(fkm) * D.dt;
@@ -692,7 +541,7 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
     const int64_t x = D.offx + i;
     R d_rho = dsed;
     if (sf) d_rho += w.d_qv_sf[x];
-    R d_rt = dth + th_rd + d_rho * rt / rho_k;
+    R d_rt = dth + th_rd + fdiv(d_rho * rt, rho_k);
     if (sf) d_rt += w.d_th_sf[x];
     R d_qv = dqv;
     if (sf) d_qv += w.d_qv_sf[x];
