@@ -211,8 +211,8 @@ void enqueue_stage(moist_handle *h, int in_set, int out_set, int s0_set, int div
     S *rtp[3] = {slot<S>(h, RTP0), slot<S>(h, RTP1), slot<S>(h, RTP2)};
     S *rup[2] = {slot<S>(h, RUP), slot<S>(h, RUP2)};
     hipLaunchKernelGGL(k_diag<S>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, w);
-    hipLaunchKernelGGL(k_slow<S>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, s0, w,
-                       rtp[0], u_phys_2d);
+    hipLaunchKernelGGL(k_slow<S>, dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles, zv, in, s0,
+                       w, rtp[0], u_phys_2d);
     // rho_theta' rotates over three buffers (current, previous, next) and
     // rho_u' over two, because every column reads its neighbours' values
     int cur = 0, old = 0, ru = 0;

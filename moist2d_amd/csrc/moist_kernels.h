@@ -118,184 +118,10 @@ __global__ void __launch_bounds__(256) k_diag(const Dom *__restrict__ doms, cons
     }
 }
 
-// =============================================================================
-// Slow tendencies R_rho, R_rho_u, R_rho_w, R_rho_theta (moist.jl:1071-1332)
-// and the acoustic perturbation start values (moist.jl:1339-1348).
-// =============================================================================
-template <class S>
-__global__ void __launch_bounds__(256) k_slow(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
-                                              ZVec zv, SSet<S> in, SSet<S> s0, Work<S> w, S *rtp0,
-                                              int u_phys_2d) {
-    using R = S;
-    using KC = K<R>;
-    const int4 t = tiles[blockIdx.x];
-    if (t.x < 0) return;  // padding tile of the XCD ordering
-    const Dom &D = doms[t.x];
-    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
-    const int nx = D.nx, nz = D.nz, ke = nz - 1;
-    if (i >= nx || k >= nz) return;
-    const int64_t row = D.off2 + (int64_t)k * nx;
-    const int64_t c = row + i;
-    const int im = wrapx(i - 1, nx), im2 = wrapx(i - 2, nx), ip = wrapx(i + 1, nx), ip2 = wrapx(i + 2, nx);
-    const double *zd = zv.z + D.offz, *zfd = zv.zf + D.offz, *dzd = zv.dz + D.offz;
-    const R dx = R(D.dx);
-    const R EPS = KC::EPS;
-    const S *rho_u = in.f[RHOU], *rho_w = in.f[RHOW];
-    auto at = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + ii; };
-    auto RHOF = [&](int kk, int ii) -> R { return full_rho(in, at(kk, ii), zv, D, kk); };
-    auto U = [&](int kk, int ii) -> R { return w.uf[at(kk, ii)]; };
-    auto W = [&](int kk, int ii) -> R { return w.wf[at(kk, ii)]; };
-    auto TH = [&](int kk, int ii) -> R { return w.theta[at(kk, ii)]; };
-    auto DZ = [&](int kk) -> R { return R(dzd[kk]); };
-    auto Z = [&](int kk) -> R { return R(zd[kk]); };
-
-    // ---- R_rho (moist.jl:1079-1086)
-    {
-        R fkp = k == ke ? R(0.0) : rho_w[c];
-        R fkm = k == 0 ? R(0.0) : rho_w[c - nx];
-        R adv = -(fdiv(rho_u[c] - rho_u[row + im], dx) + fdiv(fkp - fkm, DZ(k)));
-        w.R_rho[c] = adv + w.d_rho[c];
-    }
-    // ---- R_rho_u (moist.jl:1098-1179)
-    {
-        R rho_u_ip = R(0.5) * (rho_u[c] + rho_u[row + ip]);
-        R fip, fim;
-        if (rho_u_ip > EPS) fip = fdiv(rho_u_ip * upw3(U(k, im), U(k, i), U(k, ip)), R(6.0));
-        else if (rho_u_ip < -EPS) fip = fdiv(rho_u_ip * upw3(U(k, ip2), U(k, ip), U(k, i)), R(6.0));
-        else fip = rho_u_ip * (U(k, i) + U(k, ip)) * R(0.5);
-        R rho_u_im = R(0.5) * (rho_u[row + im] + rho_u[c]);
-        if (rho_u_im > EPS) fim = fdiv(rho_u_im * upw3(U(k, im2), U(k, im), U(k, i)), R(6.0));
-        else if (rho_u_im < -EPS) fim = fdiv(rho_u_im * upw3(U(k, ip), U(k, i), U(k, im)), R(6.0));
-        else fim = rho_u_im * (U(k, im) + U(k, i)) * R(0.5);
-        R druudx = fdiv(fip - fim, dx);
-        R fkp, fkm;
-        if (k == ke) fkp = R(0.0);
-        else {
-            R rw = R(0.5) * (rho_w[c] + rho_w[row + ip]);
-            if (rw > EPS && k > 0) fkp = fdiv(rw * upw3(U(k - 1, i), U(k, i), U(k + 1, i)), R(6.0));
-            else if (rw < -EPS && k < ke - 1) fkp = fdiv(rw * upw3(U(k + 2, i), U(k + 1, i), U(k, i)), R(6.0));
-            else fkp = rw * f2h(U(k, i), U(k + 1, i), DZ(k), DZ(k + 1));
-        }
-        if (k == 0) fkm = R(0.0);
-        else {
-            R rw = R(0.5) * (rho_w[c - nx] + rho_w[row - nx + ip]);
-            if (rw > EPS && k > 1) fkm = fdiv(rw * upw3(U(k - 2, i), U(k - 1, i), U(k, i)), R(6.0));
-            else if (rw < -EPS && k < ke) fkm = fdiv(rw * upw3(U(k + 1, i), U(k, i), U(k - 1, i)), R(6.0));
-            else fkm = rw * f2h(U(k - 1, i), U(k, i), DZ(k - 1), DZ(k));
-        }
-        R drwudz = fdiv(fkp - fkm, DZ(k));
-        R adv = -druudx - drwudz;
-        R u_ip = fdiv(U(k, ip) - U(k, i), dx);
-        R u_im = fdiv(U(k, i) - U(k, im), dx);
-        R u_xx = fdiv(u_ip * RHOF(k, ip) - u_im * RHOF(k, i), dx);
-        R u_kp = k == ke ? R(0.0) : fdiv(U(k + 1, i) - U(k, i), (Z(k + 1) - Z(k)));
-        R u_km = k == 0 ? R(0.0) : fdiv(U(k, i) - U(k - 1, i), (Z(k) - Z(k - 1)));
-        R rho_kp = k == ke ? R(0.0) : R(0.25) * (RHOF(k, i) + RHOF(k + 1, i) + RHOF(k, ip) + RHOF(k + 1, ip));
-        R rho_km = k == 0 ? R(0.0) : R(0.25) * (RHOF(k - 1, i) + RHOF(k, i) + RHOF(k - 1, ip) + RHOF(k, ip));
-        R u_zz = fdiv(u_kp * rho_kp - u_km * rho_km, DZ(k));
-        R diff = R(D.nu) * (u_xx + u_zz);
-        R dpdx = fdiv(w.pres[row + ip] - w.pres[c], dx);
-        R dup = u_phys_2d ? w.d_u2d[c] : (k == 0 ? w.d_u_sf[D.offx + i] : R(0.0));
-        w.R_rhou[c] = adv - dpdx + diff + dup;
-    }
-    // ---- R_rho_w (moist.jl:1190-1279), faces k+1/2 for k < ke
-    if (k < ke) {
-        R rho_u_ip = f2h(rho_u[c], rho_u[c + nx], Z(k), Z(k + 1));
-        R fip, fim;
-        if (rho_u_ip > EPS) fip = fdiv(rho_u_ip * upw3(W(k, im), W(k, i), W(k, ip)), R(6.0));
-        else if (rho_u_ip < -EPS) fip = fdiv(rho_u_ip * upw3(W(k, ip2), W(k, ip), W(k, i)), R(6.0));
-        else fip = rho_u_ip * (W(k, i) + W(k, ip)) * R(0.5);
-        R rho_u_im = f2h(rho_u[row + im], rho_u[row + nx + im], DZ(k), DZ(k + 1));
-        if (rho_u_im > EPS) fim = fdiv(rho_u_im * upw3(W(k, im2), W(k, im), W(k, i)), R(6.0));
-        else if (rho_u_im < -EPS) fim = fdiv(rho_u_im * upw3(W(k, ip), W(k, i), W(k, im)), R(6.0));
-        else fim = rho_u_im * (W(k, im) + W(k, i)) * R(0.5);
-        R drwudx = fdiv(fip - fim, dx);
-        R fkp, fkm;
-        if (k == ke - 1) fkp = R(0.0);
-        else {
-            R rw = f2h(rho_w[c], rho_w[c + nx], DZ(k), DZ(k + 1));
-            if (rw > EPS && k > 0) fkp = fdiv(rw * upw3(W(k - 1, i), W(k, i), W(k + 1, i)), R(6.0));
-            else if (rw < -EPS && k < ke - 2) fkp = fdiv(rw * upw3(W(k + 2, i), W(k + 1, i), W(k, i)), R(6.0));
-            else fkp = rw * f2h(W(k, i), W(k + 1, i), DZ(k), DZ(k + 1));
-        }
-        if (k == 0) fkm = R(0.0);
-        else {
-            R rw = f2h(rho_w[c - nx], rho_w[c], DZ(k - 1), DZ(k));
-            if (rw > EPS && k > 1) fkm = fdiv(rw * upw3(W(k - 2, i), W(k - 1, i), W(k, i)), R(6.0));
-            else if (rw < -EPS && k < ke - 1) fkm = fdiv(rw * upw3(W(k + 1, i), W(k, i), W(k - 1, i)), R(6.0));
-            else fkm = rw * f2h(W(k - 1, i), W(k, i), DZ(k - 1), DZ(k));
-        }
-        R zz = Z(k + 1) - Z(k);
-        R drwwdz = fdiv(fkp - fkm, zz);
-        R adv = -drwudx - drwwdz;
-        R w_ip = fdiv(W(k, ip) - W(k, i), dx);
-        R w_im = fdiv(W(k, i) - W(k, im), dx);
-        R rho_ip = R(0.25) * (RHOF(k, i) + RHOF(k + 1, i) + RHOF(k, ip) + RHOF(k + 1, ip));
-        R rho_im = R(0.25) * (RHOF(k, im) + RHOF(k + 1, im) + RHOF(k, i) + RHOF(k + 1, i));
-        R w_xx = fdiv(w_ip * rho_ip - w_im * rho_im, dx);
-        R w_kp = k == ke - 1 ? R(0.0) : fdiv(W(k + 1, i) - W(k, i), DZ(k + 1));
-        R w_km = k == 0 ? R(0.0) : fdiv(W(k, i) - W(k - 1, i), DZ(k));
-        R w_zz = fdiv(w_kp * RHOF(k + 1, i) - w_km * RHOF(k, i), zz);
-        R diff = R(D.nu) * (w_xx + w_zz);
-        R sigma = R(0.0);
-        const R zf_sp = R(D.zf_sponge), H = R(D.H);
-        if (R(zfd[k]) > (R(1.0) - zf_sp) * H)
-            sigma = (KC::GRAV / R(30.0)) * ((R(zfd[k]) - (R(1.0) - zf_sp) * H) / (zf_sp * H));
-        R sponge = -sigma * rho_w[c];
-        R res;
-        if constexpr (is_mixed<S>()) {
-            // -dp/dz + buoyancy = B_bar - dp'/dz - g (rho'_k + rho'_k+1)/2
-            R dpdz = fdiv(w.pres[c + nx] - w.pres[c], zz);
-            R buoy = -KC::GRAV * (in.f[RHO][c] + in.f[RHO][c + nx]) * R(0.5);
-            res = adv + ((R(zv.B_bar[D.offz + k]) - dpdz) + buoy) + diff + sponge;
-        } else {
-            R dpdz = fdiv(w.pres[c + nx] - w.pres[c], zz);
-            R buoy = -KC::GRAV * (RHOF(k, i) + RHOF(k + 1, i)) * R(0.5);
-            res = adv - dpdz + buoy + diff + sponge;
-        }
-        w.R_rhow[c] = res;
-    } else {
-        w.R_rhow[c] = R(0.0);
-    }
-    // ---- R_rho_theta (moist.jl:1284-1332)
-    {
-        R ru = rho_u[c], rum = rho_u[row + im];
-        R fip, fim, fkp, fkm;
-        if (ru > EPS) fip = fdiv(ru * upw3(TH(k, im), TH(k, i), TH(k, ip)), R(6.0));
-        else if (ru < -EPS) fip = fdiv(ru * upw3(TH(k, ip2), TH(k, ip), TH(k, i)), R(6.0));
-        else fip = ru * (TH(k, i) + TH(k, ip)) * R(0.5);
-        if (rum > EPS) fim = fdiv(rum * upw3(TH(k, im2), TH(k, im), TH(k, i)), R(6.0));
-        else if (rum < -EPS) fim = fdiv(rum * upw3(TH(k, ip), TH(k, i), TH(k, im)), R(6.0));
-        else fim = rum * (TH(k, im) + TH(k, i)) * R(0.5);
-        if (k == ke) fkp = R(0.0);
-        else {
-            R rw = rho_w[c];
-            if (rw > EPS && k > 0) fkp = fdiv(rw * upw3(TH(k - 1, i), TH(k, i), TH(k + 1, i)), R(6.0));
-            else if (rw < -EPS && k < ke - 1) fkp = fdiv(rw * upw3(TH(k + 2, i), TH(k + 1, i), TH(k, i)), R(6.0));
-            else fkp = rw * f2h(TH(k, i), TH(k + 1, i), DZ(k), DZ(k + 1));
-        }
-        if (k == 0) fkm = R(0.0);
-        else {
-            R rw = rho_w[c - nx];
-            if (rw > EPS && k > 1) fkm = fdiv(rw * upw3(TH(k - 2, i), TH(k - 1, i), TH(k, i)), R(6.0));
-            else if (rw < -EPS && k < ke) fkm = fdiv(rw * upw3(TH(k + 1, i), TH(k, i), TH(k - 1, i)), R(6.0));
-            else fkm = rw * f2h(TH(k - 1, i), TH(k, i), DZ(k - 1), DZ(k));
-        }
-        R adv = -(fdiv(fip - fim, dx) + fdiv(fkp - fkm, DZ(k)));
-        w.R_rt[c] = adv + w.d_rt[c];
-    }
-    // ---- perturbations from the step start, mass-flux accumulators
-    w.rp[c] = s0.f[RHO][c] - in.f[RHO][c];
-    w.rup[c] = s0.f[RHOU][c] - in.f[RHOU][c];
-    w.rwp[c] = s0.f[RHOW][c] - in.f[RHOW][c];
-    rtp0[c] = s0.f[RT][c] - in.f[RT][c];
-    w.mom_x[c] = in.f[RHOU][c];
-    w.mom_z[c] = in.f[RHOW][c];
-}
-
 #include "moist_acoustic.h"
 
 #include "moist_scalar.h"
+#include "moist_slow.h"
 
 // =============================================================================
 // Surface fluxes (surface_fluxes!, moist.jl:710-752), one thread per column.

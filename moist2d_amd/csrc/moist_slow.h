@@ -1,0 +1,258 @@
+// moist_slow.h — slow tendencies of one RK stage (moist.jl:1071-1332) and the
+// acoustic start values (moist.jl:1339-1348).  Included by moist_kernels.h
+// inside namespace mo.
+//
+// A block owns ST_X columns x ST_Z levels.  It stages u and w at faces, theta,
+// rho, rho_u and rho_w of the tile plus a 2-cell halo in LDS, then evaluates
+// every flux once at the face, centre or corner where the reference defines
+// it and forms each tendency from flux differences:
+//   R_rho_theta: x- and z-face fluxes (moist.jl:1287-1329)
+//   R_rho_u:     centre momentum fluxes, corner fluxes, viscous fluxes (1102-1172)
+//   R_rho_w:     corner fluxes with height weights (left of the face) and with
+//                dz weights (right of the face) — the reference interpolates
+//                rho_u differently on the two sides (moist.jl:1196, 1205), so
+//                both are kept — centre fluxes and viscous fluxes (1193-1267)
+// Each shared flux is the same expression the reference evaluates in both
+// neighbouring cells, so evaluating it once gives the same numbers.
+
+constexpr int SL_HX = ST_X + 4, SL_HZ = ST_Z + 4;
+constexpr int SL_FLUX = 5 * (ST_Z + 1) * (ST_X + 1);  // largest flux phase (w)
+
+template <class S>
+__global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
+                                                     ZVec zv, SSet<S> in, SSet<S> s0, Work<S> w, S *rtp0,
+                                                     int u_phys_2d) {
+    using R = S;
+    using KC = K<R>;
+    __shared__ R lU[SL_HZ][SL_HX], lW[SL_HZ][SL_HX], lT[SL_HZ][SL_HX];
+    __shared__ R lR[SL_HZ][SL_HX], lRU[SL_HZ][SL_HX], lRW[SL_HZ][SL_HX];
+    __shared__ R lF[SL_FLUX];
+
+    const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;  // padding tile of the XCD ordering (uniform per block)
+    const Dom &D = doms[t.x];
+    const int i0 = t.y, k0 = t.z;
+    const int nx = D.nx, nz = D.nz, ke = nz - 1;
+    const int tid = threadIdx.x;
+    const double *zd = zv.z + D.offz, *zfd = zv.zf + D.offz, *dzd = zv.dz + D.offz;
+    const R dx = R(D.dx), EPS = KC::EPS, nu = R(D.nu);
+    auto gidx = [&](int k, int i) -> int64_t { return D.off2 + (int64_t)k * nx + i; };
+    auto DZ = [&](int k) -> R { return R(dzd[k]); };
+    auto Z = [&](int k) -> R { return R(zd[k]); };
+    // LDS accessors in absolute (k, i) of the tile neighbourhood
+    auto U = [&](int k, int i) -> R { return lU[k - k0 + 2][i - i0 + 2]; };
+    auto W = [&](int k, int i) -> R { return lW[k - k0 + 2][i - i0 + 2]; };
+    auto TH = [&](int k, int i) -> R { return lT[k - k0 + 2][i - i0 + 2]; };
+    auto RHL = [&](int k, int i) -> R { return lR[k - k0 + 2][i - i0 + 2]; };
+    auto RU = [&](int k, int i) -> R { return lRU[k - k0 + 2][i - i0 + 2]; };
+    auto RW = [&](int k, int i) -> R { return lRW[k - k0 + 2][i - i0 + 2]; };
+
+    for (int e = tid; e < SL_HZ * SL_HX; e += ST_THREADS) {
+        const int r = e / SL_HX, c = e - r * SL_HX;
+        const int k = k0 - 2 + r, i = wrapx(i0 - 2 + c, nx);
+        R u = R(0), ww = R(0), th = R(0), rho = R(0), ru = R(0), rw = R(0);
+        if (k >= 0 && k < nz) {
+            const int64_t q = gidx(k, i);
+            u = w.uf[q];
+            ww = w.wf[q];
+            th = w.theta[q];
+            rho = full_rho(in, q, zv, D, k);
+            ru = in.f[RHOU][q];
+            rw = in.f[RHOW][q];
+        }
+        lU[r][c] = u;
+        lW[r][c] = ww;
+        lT[r][c] = th;
+        lR[r][c] = rho;
+        lRU[r][c] = ru;
+        lRW[r][c] = rw;
+    }
+    __syncthreads();
+
+    // ---------------------------------------------------------------- R_rho_theta
+    R *FTx = lF;                          // [ST_Z][ST_X+1]: x-face between i0+f-1 and i0+f
+    R *FTz = lF + ST_Z * (ST_X + 1);      // [ST_Z+1][ST_X]: z-face between k0+g-1 and k0+g
+    for (int e = tid; e < ST_Z * (ST_X + 1); e += ST_THREADS) {
+        const int r = e / (ST_X + 1), f = e - r * (ST_X + 1);
+        const int k = k0 + r, L = i0 + f - 1;
+        R fl = R(0);
+        if (k < nz) {
+            const R ru = RU(k, L);
+            if (ru > EPS) fl = fdiv(ru * upw3(TH(k, L - 1), TH(k, L), TH(k, L + 1)), R(6.0));
+            else if (ru < -EPS) fl = fdiv(ru * upw3(TH(k, L + 2), TH(k, L + 1), TH(k, L)), R(6.0));
+            else fl = ru * (TH(k, L) + TH(k, L + 1)) * R(0.5);
+        }
+        FTx[e] = fl;
+    }
+    for (int e = tid; e < (ST_Z + 1) * ST_X; e += ST_THREADS) {
+        const int g = e / ST_X, c = e - g * ST_X;
+        const int k = k0 + g - 1, i = i0 + c;
+        R fl = R(0);
+        if (k >= 0 && k <= ke - 1) {
+            const R rw = RW(k, i);
+            if (rw > EPS && k > 0) fl = fdiv(rw * upw3(TH(k - 1, i), TH(k, i), TH(k + 1, i)), R(6.0));
+            else if (rw < -EPS && k < ke - 1) fl = fdiv(rw * upw3(TH(k + 2, i), TH(k + 1, i), TH(k, i)), R(6.0));
+            else fl = rw * f2h(TH(k, i), TH(k + 1, i), DZ(k), DZ(k + 1));
+        }
+        FTz[e] = fl;
+    }
+    __syncthreads();
+    for (int e = tid; e < ST_Z * ST_X; e += ST_THREADS) {
+        const int r = e / ST_X, c = e - r * ST_X;
+        const int k = k0 + r, i = i0 + c;
+        if (k >= nz || i >= nx) continue;
+        const int64_t q = gidx(k, i);
+        // R_rho (moist.jl:1079-1086)
+        const R fkp = k == ke ? R(0.0) : RW(k, i);
+        const R fkm = k == 0 ? R(0.0) : RW(k - 1, i);
+        w.R_rho[q] = -(fdiv(RU(k, i) - RU(k, i - 1), dx) + fdiv(fkp - fkm, DZ(k))) + w.d_rho[q];
+        // R_rho_theta
+        const R adv = -(fdiv(FTx[r * (ST_X + 1) + c + 1] - FTx[r * (ST_X + 1) + c], dx) +
+                        fdiv(FTz[(r + 1) * ST_X + c] - FTz[r * ST_X + c], DZ(k)));
+        w.R_rt[q] = adv + w.d_rt[q];
+        // perturbations from the step start, mass-flux accumulators (moist.jl:1339-1348)
+        w.rp[q] = s0.f[RHO][q] - in.f[RHO][q];
+        w.rup[q] = s0.f[RHOU][q] - in.f[RHOU][q];
+        w.rwp[q] = s0.f[RHOW][q] - in.f[RHOW][q];
+        rtp0[q] = s0.f[RT][q] - in.f[RT][q];
+        w.mom_x[q] = RU(k, i);
+        w.mom_z[q] = RW(k, i);
+    }
+    __syncthreads();
+
+    // ---------------------------------------------------------------- R_rho_u
+    R *CUx = lF;                                // [ST_Z][ST_X+1]: momentum flux at centre (k, i0+cc)
+    R *GUx = lF + ST_Z * (ST_X + 1);            // [ST_Z][ST_X+1]: rho du/dx at centre (k, i0+cc)
+    R *CUz = lF + 2 * ST_Z * (ST_X + 1);        // [ST_Z+1][ST_X]: corner flux at (k0+g-1/2, i+1/2)
+    R *GUz = CUz + (ST_Z + 1) * ST_X;           // [ST_Z+1][ST_X]: rho du/dz at the same corners
+    for (int e = tid; e < ST_Z * (ST_X + 1); e += ST_THREADS) {
+        const int r = e / (ST_X + 1), cc = e - r * (ST_X + 1);
+        const int k = k0 + r, i = i0 + cc;  // centre (k, i) lies between faces i-1/2 and i+1/2
+        R fl = R(0), gx = R(0);
+        if (k < nz) {
+            const R rc = R(0.5) * (RU(k, i - 1) + RU(k, i));
+            if (rc > EPS) fl = fdiv(rc * upw3(U(k, i - 2), U(k, i - 1), U(k, i)), R(6.0));
+            else if (rc < -EPS) fl = fdiv(rc * upw3(U(k, i + 1), U(k, i), U(k, i - 1)), R(6.0));
+            else fl = rc * (U(k, i - 1) + U(k, i)) * R(0.5);
+            gx = fdiv(U(k, i) - U(k, i - 1), dx) * RHL(k, i);
+        }
+        CUx[e] = fl;
+        GUx[e] = gx;
+    }
+    for (int e = tid; e < (ST_Z + 1) * ST_X; e += ST_THREADS) {
+        const int g = e / ST_X, c = e - g * ST_X;
+        const int k = k0 + g - 1, i = i0 + c;  // corner (k+1/2, i+1/2)
+        R fl = R(0), gz = R(0);
+        if (k >= 0 && k <= ke - 1) {
+            const R rw = R(0.5) * (RW(k, i) + RW(k, i + 1));
+            if (rw > EPS && k > 0) fl = fdiv(rw * upw3(U(k - 1, i), U(k, i), U(k + 1, i)), R(6.0));
+            else if (rw < -EPS && k < ke - 1) fl = fdiv(rw * upw3(U(k + 2, i), U(k + 1, i), U(k, i)), R(6.0));
+            else fl = rw * f2h(U(k, i), U(k + 1, i), DZ(k), DZ(k + 1));
+            const R u_kp = fdiv(U(k + 1, i) - U(k, i), Z(k + 1) - Z(k));
+            const R rho_kp = R(0.25) * (RHL(k, i) + RHL(k + 1, i) + RHL(k, i + 1) + RHL(k + 1, i + 1));
+            gz = u_kp * rho_kp;
+        }
+        CUz[e] = fl;
+        GUz[e] = gz;
+    }
+    __syncthreads();
+    for (int e = tid; e < ST_Z * ST_X; e += ST_THREADS) {
+        const int r = e / ST_X, c = e - r * ST_X;
+        const int k = k0 + r, i = i0 + c;
+        if (k >= nz || i >= nx) continue;
+        const int64_t q = gidx(k, i);
+        const int x0 = r * (ST_X + 1) + c;
+        const R druudx = fdiv(CUx[x0 + 1] - CUx[x0], dx);
+        const R drwudz = fdiv(CUz[(r + 1) * ST_X + c] - CUz[r * ST_X + c], DZ(k));
+        const R adv = -druudx - drwudz;
+        const R u_xx = fdiv(GUx[x0 + 1] - GUx[x0], dx);
+        const R u_zz = fdiv(GUz[(r + 1) * ST_X + c] - GUz[r * ST_X + c], DZ(k));
+        const R diff = nu * (u_xx + u_zz);
+        const int ip = wrapx(i + 1, nx);
+        const R dpdx = fdiv(w.pres[gidx(k, ip)] - w.pres[q], dx);
+        const R dup = u_phys_2d ? w.d_u2d[q] : (k == 0 ? w.d_u_sf[D.offx + i] : R(0.0));
+        w.R_rhou[q] = adv - dpdx + diff + dup;
+    }
+    __syncthreads();
+
+    // ---------------------------------------------------------------- R_rho_w
+    const int NX1 = ST_Z * (ST_X + 1);
+    R *XZ = lF;                  // [ST_Z][ST_X+1] corner (k+1/2, i0+f-1/2), rho_u weighted by heights
+    R *XD = lF + NX1;            // same corners, rho_u weighted by dz
+    R *VX = lF + 2 * NX1;        // rho dw/dx at the same corners
+    R *ZC = lF + 3 * NX1;        // [ST_Z+1][ST_X] centre flux at (k0+g, i): kp of face k0+g-1
+    R *VZ = ZC + (ST_Z + 1) * ST_X;
+    for (int e = tid; e < NX1; e += ST_THREADS) {
+        const int r = e / (ST_X + 1), f = e - r * (ST_X + 1);
+        const int k = k0 + r, L = i0 + f - 1;  // corner between columns L and L+1 on face k+1/2
+        R xz = R(0), xd = R(0), vx = R(0);
+        if (k <= ke - 1) {
+            const R s_p = upw3(W(k, L - 1), W(k, L), W(k, L + 1));
+            const R s_m = upw3(W(k, L + 2), W(k, L + 1), W(k, L));
+            const R s_c = (W(k, L) + W(k, L + 1)) * R(0.5);
+            const R uz = f2h(RU(k, L), RU(k + 1, L), Z(k), Z(k + 1));
+            if (uz > EPS) xz = fdiv(uz * s_p, R(6.0));
+            else if (uz < -EPS) xz = fdiv(uz * s_m, R(6.0));
+            else xz = uz * s_c;
+            const R ud = f2h(RU(k, L), RU(k + 1, L), DZ(k), DZ(k + 1));
+            if (ud > EPS) xd = fdiv(ud * s_p, R(6.0));
+            else if (ud < -EPS) xd = fdiv(ud * s_m, R(6.0));
+            else xd = ud * s_c;
+            const R w_ip = fdiv(W(k, L + 1) - W(k, L), dx);
+            const R rho_ip = R(0.25) * (RHL(k, L) + RHL(k + 1, L) + RHL(k, L + 1) + RHL(k + 1, L + 1));
+            vx = w_ip * rho_ip;
+        }
+        XZ[e] = xz;
+        XD[e] = xd;
+        VX[e] = vx;
+    }
+    for (int e = tid; e < (ST_Z + 1) * ST_X; e += ST_THREADS) {
+        const int g = e / ST_X, c = e - g * ST_X;
+        const int k = k0 + g - 1, i = i0 + c;  // face k, its kp centre is (k+1, i)
+        R zc = R(0), vz = R(0);
+        if (k >= 0 && k < ke - 1) {
+            const R rw = f2h(RW(k, i), RW(k + 1, i), DZ(k), DZ(k + 1));
+            if (rw > EPS && k > 0) zc = fdiv(rw * upw3(W(k - 1, i), W(k, i), W(k + 1, i)), R(6.0));
+            else if (rw < -EPS && k < ke - 2) zc = fdiv(rw * upw3(W(k + 2, i), W(k + 1, i), W(k, i)), R(6.0));
+            else zc = rw * f2h(W(k, i), W(k + 1, i), DZ(k), DZ(k + 1));
+            vz = fdiv(W(k + 1, i) - W(k, i), DZ(k + 1)) * RHL(k + 1, i);
+        }
+        ZC[e] = zc;
+        VZ[e] = vz;
+    }
+    __syncthreads();
+    const R zf_sp = R(D.zf_sponge), H = R(D.H);
+    for (int e = tid; e < ST_Z * ST_X; e += ST_THREADS) {
+        const int r = e / ST_X, c = e - r * ST_X;
+        const int k = k0 + r, i = i0 + c;
+        if (k >= nz || i >= nx) continue;
+        const int64_t q = gidx(k, i);
+        if (k == ke) {
+            w.R_rhow[q] = R(0.0);
+            continue;
+        }
+        const int x0 = r * (ST_X + 1) + c;
+        const R zz = Z(k + 1) - Z(k);
+        const R drwudx = fdiv(XZ[x0 + 1] - XD[x0], dx);
+        const R drwwdz = fdiv(ZC[(r + 1) * ST_X + c] - ZC[r * ST_X + c], zz);
+        const R adv = -drwudx - drwwdz;
+        const R w_xx = fdiv(VX[x0 + 1] - VX[x0], dx);
+        const R w_zz = fdiv(VZ[(r + 1) * ST_X + c] - VZ[r * ST_X + c], zz);
+        const R diff = nu * (w_xx + w_zz);
+        R sigma = R(0.0);
+        if (R(zfd[k]) > (R(1.0) - zf_sp) * H)
+            sigma = (KC::GRAV / R(30.0)) * ((R(zfd[k]) - (R(1.0) - zf_sp) * H) / (zf_sp * H));
+        const R sponge = -sigma * RW(k, i);
+        const R dpdz = fdiv(w.pres[q + nx] - w.pres[q], zz);
+        R res;
+        if constexpr (is_mixed<S>()) {
+            // -dp/dz + buoyancy = B_bar - dp'/dz - g (rho'_k + rho'_k+1)/2
+            const R buoy = -KC::GRAV * (in.f[RHO][q] + in.f[RHO][q + nx]) * R(0.5);
+            res = adv + ((R(zv.B_bar[D.offz + k]) - dpdz) + buoy) + diff + sponge;
+        } else {
+            const R buoy = -KC::GRAV * (RHL(k, i) + RHL(k + 1, i)) * R(0.5);
+            res = adv - dpdz + buoy + diff + sponge;
+        }
+        w.R_rhow[q] = res;
+    }
+}
