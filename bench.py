@@ -74,6 +74,29 @@ def kernel_pass(p, s, precision, steps):
     return ms, n
 
 
+def ensemble_run(steps, members=256, seed=2):
+    """The batched workload: 256 bubbles on domains of uneven nx (256..2048) and
+    nz (64, 128), all advanced by the same launches (one handle)."""
+    mem = ic.ensemble(np.float64, members=members, seed=seed)
+    params = [p for p, _ in mem]
+    cells = sum(p.Nx * p.Nz for p in params)
+    out = {"workload": f"ensemble_{members}", "members": members, "cells": cells,
+           "columns": sum(p.Nx for p in params), "dt": float(params[0].dt), "ns": params[0].ns, "steps": steps}
+    for name, prec in (("mixed", core.MIXED), ("fp64", core.FP64)):
+        eng = core.Engine(params, prec)
+        eng.store_phys_diag(False)
+        for m, (_, s) in enumerate(mem):
+            eng.upload(s, member=m)
+        eng.use_graph(True)
+        eng.time_steps(2)
+        ms = eng.time_steps(steps)
+        d = [eng.diagnostics(m) for m in (0, members - 1)]
+        eng.close()
+        out[name] = {"cell_steps_per_s": cells * steps / (ms * 1e-3), "ms_per_step": ms / steps,
+                     "first_last_member_total_water": [x["total_water"] for x in d]}
+    return out
+
+
 def cpu_baseline(p, s, steps):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # the checker, timed here as the CPU baseline only
@@ -121,6 +144,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=30, help="0 skips the 256-member ensemble")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -183,6 +207,8 @@ def main():
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
             "avg_launch_ms": kms, "launches": nlaunch, "alg_bytes_per_launch": alg_bytes}
 
+    ens = ensemble_run(args.ensemble_steps) if args.ensemble_steps > 0 else None
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(p, s, args.cpu_steps)
@@ -208,6 +234,7 @@ def main():
         "mixed_wall_s": mixed["wall_s"],
         "roofline": roof,
         "cpu_baseline": cpu,
+        "ensemble": ens,
         "adjoint_forward_ratio": None,
     }
     print(json.dumps(line))

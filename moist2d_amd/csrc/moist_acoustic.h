@@ -26,7 +26,9 @@
 //     inside its chunk from that value;
 //   * forward and back substitution are affine recurrences: each chunk solves
 //     with a zero carry and keeps the carry coefficients, the carries are
-//     scanned across chunks through LDS, and every level is corrected.
+//     combined across chunks, and every level is corrected.
+// The cross-chunk combinations are Hillis-Steele scans in LDS (log2(nch)
+// steps, every thread active).
 
 #ifndef MOIST_AC_THREADS
 #define MOIST_AC_THREADS 1024
@@ -49,7 +51,7 @@ __global__ void __launch_bounds__(AC_THREADS) k_acoustic(const Dom *__restrict__
                                                          ZVec zv, SSet<S> in, Work<S> w, AcBuf<S> ab, int div,
                                                          double dt_ovr, int nsub, int first) {
     using R = S;
-    __shared__ double lds[4 * AC_THREADS];
+    __shared__ double lds[8 * AC_THREADS];  // two buffers of 4 doubles per thread for the scans
     __shared__ S ldt[2 * AC_THREADS];
 
     const int4 tl = atiles[blockIdx.x];
@@ -237,27 +239,42 @@ __global__ void __launch_bounds__(AC_THREADS) k_acoustic(const Dom *__restrict__
                 m11 = n11 * sc;
             }
         }
-        lds[4 * t + 0] = m00;
-        lds[4 * t + 1] = m01;
-        lds[4 * t + 2] = m10;
-        lds[4 * t + 3] = m11;
+        // inclusive Hillis-Steele scan over the chunks of each column:
+        // S_c = M_c S_{c-1} ... M_0; the c' entering chunk c is S_{c-1} applied to (0, 1)
+        double *A = lds, *B = lds + 4 * AC_THREADS;
+        A[4 * t + 0] = m00;
+        A[4 * t + 1] = m01;
+        A[4 * t + 2] = m10;
+        A[4 * t + 3] = m11;
         __syncthreads();
-        if (c == 0 && col_ok) {
-            double v0 = 0.0, v1 = 1.0;  // c' = 0 before the first face
-            for (int q = 0; q < nch; ++q) {
-                const int e = 4 * (q * TX + jj);
-                const double a0 = lds[e], a1 = lds[e + 1], a2 = lds[e + 2], a3 = lds[e + 3];
-                const double cinq = v0 / v1;
-                lds[e] = cinq;
-                if (q * L < nz) ab.cin[(int64_t)q * ab.cin_stride + D.offx + i] = cinq;
-                const double u0 = a0 * v0 + a1 * v1, u1 = a2 * v0 + a3 * v1;
-                const double sc = 1.0 / fmax(fabs(u0), fabs(u1));
-                v0 = u0 * sc;
-                v1 = u1 * sc;
+        for (int dst = 1; dst < nch; dst <<= 1) {
+            double a0 = A[4 * t], a1 = A[4 * t + 1], a2 = A[4 * t + 2], a3 = A[4 * t + 3];
+            if (c >= dst) {
+                const int o = 4 * (t - dst * TX);
+                const double b0 = A[o], b1 = A[o + 1], b2 = A[o + 2], b3 = A[o + 3];
+                const double n0 = a0 * b0 + a1 * b2, n1 = a0 * b1 + a1 * b3;
+                const double n2 = a2 * b0 + a3 * b2, n3 = a2 * b1 + a3 * b3;
+                const double sc = 1.0 / fmax(fmax(fabs(n0), fabs(n1)), fmax(fabs(n2), fabs(n3)));
+                a0 = n0 * sc;
+                a1 = n1 * sc;
+                a2 = n2 * sc;
+                a3 = n3 * sc;
             }
+            B[4 * t + 0] = a0;
+            B[4 * t + 1] = a1;
+            B[4 * t + 2] = a2;
+            B[4 * t + 3] = a3;
+            __syncthreads();
+            double *tmp = A;
+            A = B;
+            B = tmp;
         }
-        __syncthreads();
-        cin_v = lds[4 * t];
+        cin_v = 0.0;
+        if (c > 0) {
+            const int o = 4 * (t - TX);
+            cin_v = A[o + 1] / A[o + 3];
+        }
+        if (nl > 0) ab.cin[cidx] = cin_v;
         __syncthreads();
     }
 
@@ -287,24 +304,33 @@ __global__ void __launch_bounds__(AC_THREADS) k_acoustic(const Dom *__restrict__
             }
         }
     }
-    lds[2 * t + 0] = ylast;
-    lds[2 * t + 1] = plast;
-    __syncthreads();
-    if (c == 0 && col_ok) {
-        double din = 0.0;
-        for (int q = 0; q < nch; ++q) {
-            const int e = 2 * (q * TX + jj);
-            const double y = lds[e], P = lds[e + 1];
-            lds[e] = din;
-            din = y + P * din;
+    // carries of the forward substitution: inclusive scan of the chunk maps
+    // x -> y + P x from the bottom; the carry into chunk c is S_{c-1}(0)
+    {
+        double *A = lds, *B = lds + 4 * AC_THREADS;
+        A[2 * t + 0] = ylast;
+        A[2 * t + 1] = plast;
+        __syncthreads();
+        for (int dst = 1; dst < nch; dst <<= 1) {
+            double y = A[2 * t], P = A[2 * t + 1];
+            if (c >= dst) {
+                const int o = 2 * (t - dst * TX);
+                y = y + P * A[o];
+                P = P * A[o + 1];
+            }
+            B[2 * t + 0] = y;
+            B[2 * t + 1] = P;
+            __syncthreads();
+            double *tmp = A;
+            A = B;
+            B = tmp;
         }
-    }
-    __syncthreads();
-    const double din = lds[2 * t];
+        const double din = c > 0 ? A[2 * (t - TX)] : 0.0;
 #pragma unroll
-    for (int l = 0; l < L; ++l)
-        if (l < nf) dd[l] = dd[l] + pv[l] * din;
-    __syncthreads();
+        for (int l = 0; l < L; ++l)
+            if (l < nf) dd[l] = dd[l] + pv[l] * din;
+        __syncthreads();
+    }
 
     // ---- back substitution inside the chunk with a zero carry from above
     double zfirst = 0.0, qfirst = 1.0, zlast = 0.0, qlast = 1.0;
@@ -327,26 +353,39 @@ __global__ void __launch_bounds__(AC_THREADS) k_acoustic(const Dom *__restrict__
             }
         }
     }
-    lds[4 * t + 0] = zfirst;
-    lds[4 * t + 1] = qfirst;
-    lds[4 * t + 2] = zlast;
-    lds[4 * t + 3] = qlast;
-    __syncthreads();
-    if (c == 0 && col_ok) {
-        double xin = 0.0;  // above the top face
-        for (int q = nch - 1; q >= 0; --q) {
-            const int e = 4 * (q * TX + jj);
-            const double zf = lds[e], qf = lds[e + 1];
-            lds[e + 1] = xin;
-            xin = zf + qf * xin;
+    // carries of the back substitution: inclusive scan of the maps
+    // x_in -> z_first + Q_first x_in from the top; T_c(0) is x at chunk c's
+    // first face, the carry into chunk c is T_{c+1}(0)
+    double xin, x_below = 0.0;
+    {
+        double *A = lds, *B = lds + 4 * AC_THREADS;
+        A[2 * t + 0] = zfirst;
+        A[2 * t + 1] = qfirst;
+        __syncthreads();
+        for (int dst = 1; dst < nch; dst <<= 1) {
+            double y = A[2 * t], P = A[2 * t + 1];
+            if (c + dst < nch) {
+                const int o = 2 * (t + dst * TX);
+                y = y + P * A[o];
+                P = P * A[o + 1];
+            }
+            B[2 * t + 0] = y;
+            B[2 * t + 1] = P;
+            __syncthreads();
+            double *tmp = A;
+            A = B;
+            B = tmp;
         }
-    }
-    __syncthreads();
-    const double xin = lds[4 * t + 1];
-    double x_below = 0.0;
-    if (c > 0 && nl > 0) {
-        const int e = 4 * (t - TX);
-        x_below = lds[e + 2] + lds[e + 3] * lds[e + 1];
+        xin = c + 1 < nch ? A[2 * (t + TX)] : 0.0;
+        const double xfirst = A[2 * t];  // x at this chunk's first face
+        // x at the last face of the chunk below = z_last + Q_last * (x at this chunk's first face)
+        B[2 * t + 0] = zlast;
+        B[2 * t + 1] = qlast;
+        __syncthreads();
+        if (c > 0 && nl > 0) {
+            const int o = 2 * (t - TX);
+            x_below = B[o] + B[o + 1] * xfirst;
+        }
     }
     R xr[L];
 #pragma unroll

@@ -130,6 +130,28 @@ def test_ensemble_members_match_single_runs():
     eng.close()
 
 
+def test_ensemble_workload_members_match_oracle():
+    """Members 0, 1 and 255 of the 256-member workload (seed 2) in one handle
+    with all 256 domains, graph replay, 2 steps, FP64 form against the oracle."""
+    members = ic.ensemble(np.float64, members=256, steps=2)
+    eng = core.Engine([p for p, _ in members], core.FP64)
+    for m, (p, s) in enumerate(members):
+        eng.upload(s, member=m)
+    eng.use_graph(True)
+    eng.step(2)
+    for m in (0, 1, 255):
+        p, s = members[m]
+        ref = s.copy()
+        assert O.rk3_step(ref, p, 2) == 0
+        out = s.copy()
+        eng.download(out, member=m)
+        errs = field_errors(p, out, ref)
+        _print(f"ensemble-256 member {m} ({p.Nx}x{p.Nz})", errs)
+        for n, e in errs.items():
+            assert e <= TOL_FP64, (m, n, e)
+    eng.close()
+
+
 @pytest.mark.parametrize("precision", [core.FP64, core.MIXED])
 def test_graph_replay_matches_direct_and_is_deterministic(precision):
     p, s = ic.bubble(np.float64, nx=128, nz=32)
