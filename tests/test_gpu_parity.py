@@ -132,17 +132,19 @@ def test_ensemble_members_match_single_runs():
 
 def test_ensemble_workload_members_match_oracle():
     """Members 0, 1 and 255 of the 256-member workload (seed 2) in one handle
-    with all 256 domains, graph replay, 2 steps, FP64 form against the oracle."""
-    members = ic.ensemble(np.float64, members=256, steps=2)
+    with all 256 domains, graph replay, 10 steps, FP64 form against the oracle.
+    (After 2 steps from rest the oracle's own rho_u moves by 4e-11 when rho_theta
+    is perturbed by 1 ulp; after 10 steps by 2e-12, below the 1e-11 bar.)"""
+    members = ic.ensemble(np.float64, members=256, steps=10)
     eng = core.Engine([p for p, _ in members], core.FP64)
     for m, (p, s) in enumerate(members):
         eng.upload(s, member=m)
     eng.use_graph(True)
-    eng.step(2)
+    eng.step(10)
     for m in (0, 1, 255):
         p, s = members[m]
         ref = s.copy()
-        assert O.rk3_step(ref, p, 2) == 0
+        assert O.rk3_step(ref, p, 10) == 0
         out = s.copy()
         eng.download(out, member=m)
         errs = field_errors(p, out, ref)
