@@ -277,12 +277,21 @@ template <class T> static inline T upw3(T m, T c0, T p1) {  // (-m + 5c + 2p)/6 
     return -m + T(5.0) * c0 + T(2.0) * p1;
 }
 
+// What the adjoint of one compute_step! needs from its forward pass: the
+// stage diagnostics and slow tendencies, the averaged momenta, and per
+// acoustic substep rho_theta' and rho_w' at its start, rho_u' after its
+// update and rho_w' after the implicit solve.
+template <class T> struct Rec {
+    std::vector<T> pres, theta, rt2pres, uf, wf, R_rho, R_rho_u, R_rho_w, R_rho_theta, mom_x, mom_z;
+    std::vector<std::vector<T>> rtp_start, rup_new, rwp_start, rwp_new;
+};
+
 // ---- compute_step! (moist.jl:1039-1580) -----------------------------------
 template <class T>
 static int compute_step(const G<T> &p, const T *d_rho_phys, const T *d_rho_u_phys,
                         const T *d_rho_theta_phys, const T *d_qv_phys, const T *d_qc_phys,
                         const T *d_qr_phys, const St<T> &s, const St<T> &s0, T dt, int ns,
-                        int order) {
+                        int order, Rec<T> *rec = nullptr) {
     const C<T> &c = p.c;
     const int is = p.is, ie = p.ie, ks = p.ks, ke = p.ke;
     const size_t N = p.n2();
@@ -312,6 +321,15 @@ static int compute_step(const G<T> &p, const T *d_rho_phys, const T *d_rho_u_phy
     exchange_halo(rho, p);
     exchange_halo(rho_u, p);
     exchange_halo(rho_w, p);
+    if (rec) {
+        rec->pres = pres;
+        rec->theta = theta;
+        rec->rt2pres = rt2pres;
+        rec->rtp_start.clear();
+        rec->rup_new.clear();
+        rec->rwp_start.clear();
+        rec->rwp_new.clear();
+    }
 
     std::vector<T> R_rho(N, T(0)), R_rho_u(N, T(0)), R_rho_w(N, T(0)), R_rho_theta(N, T(0));
 
@@ -487,12 +505,24 @@ static int compute_step(const G<T> &p, const T *d_rho_phys, const T *d_rho_u_phy
     exchange_halo(rtp.data(), p);
     std::vector<T> rtp_old(rtp);
     std::vector<T> mom_x(rho_u, rho_u + N), mom_z(rho_w, rho_w + N);
+    if (rec) {
+        rec->uf = uf;
+        rec->wf = wf;
+        rec->R_rho = R_rho;
+        rec->R_rho_u = R_rho_u;
+        rec->R_rho_w = R_rho_w;
+        rec->R_rho_theta = R_rho_theta;
+    }
 
     // acoustic substeps (moist.jl:1351-1461)
     const T dt_s = dt / T(ns);
     const double fact_tp = double(T(0.5)) * (1.0 + double(p.beta));  // Float64 (moist.jl:1375)
     for (int n = 1; n <= ns; ++n) {
         if (n > 1) exchange_halo(rtp.data(), p);
+        if (rec) {
+            rec->rtp_start.push_back(rtp);
+            rec->rwp_start.push_back(rwp);
+        }
 #pragma omp parallel for schedule(static)
         for (int i = is; i <= ie; ++i) {
             for (int k = ks; k <= ke; ++k) {
@@ -504,6 +534,7 @@ static int compute_step(const G<T> &p, const T *d_rho_phys, const T *d_rho_u_phy
             }
         }
         exchange_halo(rup.data(), p);
+        if (rec) rec->rup_new.push_back(rup);
         rtp_old = rtp;
 
 #pragma omp parallel for schedule(static)
@@ -567,9 +598,14 @@ static int compute_step(const G<T> &p, const T *d_rho_phys, const T *d_rho_u_phy
             for (int k = ks; k <= ke; ++k) mom_x[p.ix(k, i)] += rup[p.ix(k, i)] / T(ns);
             for (int k = ks; k <= ke - 1; ++k) mom_z[p.ix(k, i)] += rwp[p.ix(k, i)] / T(ns);
         }
+        if (rec) rec->rwp_new.push_back(rwp);
     }
 
     exchange_halo(mom_x.data(), p);
+    if (rec) {
+        rec->mom_x = mom_x;
+        rec->mom_z = mom_z;
+    }
     exchange_halo(s.rho_qv(), p);
     exchange_halo(s.rho_qc(), p);
     exchange_halo(s.rho_qr(), p);
@@ -904,6 +940,8 @@ template <class T> static int rk3_step(const G<T> &p, T *const *F, int order) {
     return 0;
 }
 
+#include "moist_adjoint.inc"
+
 }  // namespace orc
 
 // ============================================================================
@@ -992,6 +1030,66 @@ template <class T> static int rk3_step(const G<T> &p, T *const *F, int order) {
     }
 
 ORC_BOTH(DEF_SCALARS)
+
+// ---- adjoint (FP64 only) ------------------------------------------------------
+// One compute_step!: forward with record on copies of s, then the adjoint.
+ORC_EXPORT int orc_compute_step_adj_f64(const moist_params *mp, const double *const *phys, const double *const *s,
+                                        const double *const *s0, double dt, int ns, const double *const *lo,
+                                        double *const *ls, double *const *ls0, double *const *lph) {
+    orc::G<double> g(mp);
+    const size_t N = g.n2();
+    orc::StBuf a, b;
+    for (int f = 0; f < 7; ++f) {
+        a.f[f].assign(s[f], s[f] + N);
+        b.f[f].assign(s0[f], s0[f] + N);
+    }
+    orc::StBuf in = a;
+    orc::St<double> av = a.view(), bv = b.view();
+    orc::Rec<double> rec;
+    int rc = orc::compute_step(g, phys[0], phys[1], phys[2], phys[3], phys[4], phys[5], av, bv, dt, ns,
+                               orc::ORDER_SNAPSHOT, &rec);
+    if (rc) return rc;
+    orc::St<double> iv = in.view();
+    orc::compute_step_adj(g, iv, bv, dt, ns, rec, lo, ls, ls0, lph);
+    return 0;
+}
+
+ORC_EXPORT void orc_physics_fwd_f64(const moist_params *mp, const double *const *s, double *const *ph,
+                                    double *dprec) {
+    orc::G<double> g(mp);
+    orc::St<double> sv;
+    for (int f = 0; f < 7; ++f) sv.f[f] = const_cast<double *>(s[f]);
+    std::vector<double> v[6], dp;
+    orc::physics_fwd(g, sv, v, dp);
+    for (int f = 0; f < 6; ++f) std::copy(v[f].begin(), v[f].end(), ph[f]);
+    std::copy(dp.begin(), dp.end(), dprec);
+}
+
+ORC_EXPORT void orc_physics_adj_f64(const moist_params *mp, const double *const *s, const double *const *lph,
+                                    const double *ldprec, double *const *ls, double *lK) {
+    orc::G<double> g(mp);
+    orc::St<double> sv;
+    for (int f = 0; f < 7; ++f) sv.f[f] = const_cast<double *>(s[f]);
+    orc::physics_adj(g, sv, lph, ldprec, ls, lK);
+}
+
+ORC_EXPORT int orc_rk3_step_adj_f64(const moist_params *mp, const double *const *s, double *const *lam,
+                                    double ldprec, double *lK) {
+    orc::G<double> g(mp);
+    orc::StBuf a;
+    for (int f = 0; f < 7; ++f) {
+        a.f[f].assign(s[f], s[f] + g.n2());
+        orc::exchange_halo(a.f[f].data(), g);
+    }
+    orc::St<double> av = a.view();
+    return orc::rk3_step_adj(g, av, lam, ldprec, lK);
+}
+
+ORC_EXPORT int orc_adjoint_run_f64(const moist_params *mp, double *const *fields, int nsteps, int ckpt,
+                                   const double *theta_tgt, double *J, double *const *grad, double *gK) {
+    orc::G<double> g(mp);
+    return orc::adjoint_run(g, fields, nsteps, ckpt, theta_tgt, J, grad, gK);
+}
 
 ORC_EXPORT int orc_set_threads(int n) {
 #ifdef _OPENMP

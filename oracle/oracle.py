@@ -58,6 +58,17 @@ def lib():
                 f = getattr(_lib, f"orc_{name}_{sfx}")
                 f.restype = res
                 f.argtypes = args
+        V = ctypes.c_void_p
+        for name, res, args in (
+            ("compute_step_adj", ctypes.c_int, [V, V, V, V, ctypes.c_double, ctypes.c_int, V, V, V, V]),
+            ("physics_fwd", None, [V, V, V, V]),
+            ("physics_adj", None, [V, V, V, V, V, V]),
+            ("rk3_step_adj", ctypes.c_int, [V, V, V, ctypes.c_double, V]),
+            ("adjoint_run", ctypes.c_int, [V, V, ctypes.c_int, ctypes.c_int, V, V, V, V]),
+        ):
+            f = getattr(_lib, f"orc_{name}_f64")
+            f.restype = res
+            f.argtypes = args
         _lib.orc_set_threads.restype = ctypes.c_int
         _lib.orc_set_threads.argtypes = [ctypes.c_int]
     return _lib
@@ -195,3 +206,80 @@ def rk3_step(s, p, nsteps=1, order=ORDER_SNAPSHOT):
     for a in arrs:
         assert a.dtype == s.rho.dtype and (a.flags["F_CONTIGUOUS"])
     return f(ctypes.byref(cp), _ptr_array(arrs, ct), nsteps, order)
+
+
+# ---- adjoint (FP64) ---------------------------------------------------------
+def _f64(arrs):
+    for a in arrs:
+        assert a.dtype == np.float64 and (a.flags["F_CONTIGUOUS"] or a.ndim == 1), a.dtype
+    return _ptr_array(list(arrs), ctypes.c_double)
+
+
+def compute_step_adj(phys, s, s0, p, dt, ns, lam_out):
+    """Adjoint of compute_step! at (phys, s, s0): returns (lam_s[7], lam_s0[7], lam_phys[6])
+    for the adjoint `lam_out` (7 fields) of the stage output."""
+    shape = s.rho.shape
+    ls = [np.zeros(shape, order="F") for _ in range(7)]
+    ls0 = [np.zeros(shape, order="F") for _ in range(7)]
+    lph = [np.zeros(shape, order="F") for _ in range(6)]
+    cp = p.to_c()
+    rc = lib().orc_compute_step_adj_f64(ctypes.byref(cp), _f64(phys), _f64([getattr(s, n) for n in _PROG]),
+                                        _f64([getattr(s0, n) for n in _PROG]), dt, ns, _f64(lam_out), _f64(ls),
+                                        _f64(ls0), _f64(lph))
+    if rc:
+        raise RuntimeError(f"compute_step_adj failed ({rc})")
+    return ls, ls0, lph
+
+
+def physics_fwd(s, p):
+    """The six combined tendencies of rk3_step! (do_sf = do_rd = false) and d_prec."""
+    shape = s.rho.shape
+    ph = [np.zeros(shape, order="F") for _ in range(6)]
+    dprec = np.zeros(shape[1])
+    cp = p.to_c()
+    lib().orc_physics_fwd_f64(ctypes.byref(cp), _f64([getattr(s, n) for n in _PROG]), _f64(ph),
+                              dprec.ctypes.data_as(ctypes.c_void_p))
+    return ph, dprec
+
+
+def physics_adj(s, p, lam_phys, lam_dprec):
+    """Adjoint of physics_fwd: returns (lam_s[7], lam_K[3])."""
+    shape = s.rho.shape
+    ls = [np.zeros(shape, order="F") for _ in range(7)]
+    lK = np.zeros(3)
+    cp = p.to_c()
+    lam_dprec = np.ascontiguousarray(lam_dprec, dtype=np.float64)
+    lib().orc_physics_adj_f64(ctypes.byref(cp), _f64([getattr(s, n) for n in _PROG]), _f64(lam_phys),
+                              lam_dprec.ctypes.data_as(ctypes.c_void_p), _f64(ls),
+                              lK.ctypes.data_as(ctypes.c_void_p))
+    return ls, lK
+
+
+def rk3_step_adj(s, p, lam, lam_dprec=0.0):
+    """Adjoint of one rk3_step! at s: `lam` (7 fields, adjoint of the step output)
+    is replaced by the adjoint of the step input; returns lam_K[3]."""
+    lK = np.zeros(3)
+    cp = p.to_c()
+    rc = lib().orc_rk3_step_adj_f64(ctypes.byref(cp), _f64([getattr(s, n) for n in _PROG]), _f64(lam),
+                                    float(lam_dprec), lK.ctypes.data_as(ctypes.c_void_p))
+    if rc:
+        raise RuntimeError(f"rk3_step_adj failed ({rc})")
+    return lK
+
+
+def adjoint_run(s, p, nsteps, theta_target, ckpt=50):
+    """Loss J = dx sum(prec) + 1/2 sum((theta - theta_target)^2) after nsteps from s
+    (s is advanced in place) and its gradient: (J, grad[7] w.r.t. the initial
+    prognostic fields, grad_K[3] w.r.t. K_auto, Qc_threshold, K_accr)."""
+    shape = s.rho.shape
+    grad = [np.zeros(shape, order="F") for _ in range(7)]
+    gK = np.zeros(3)
+    J = ctypes.c_double()
+    cp = p.to_c()
+    arrs = s.arrays()
+    tt = np.asfortranarray(theta_target, dtype=np.float64)
+    rc = lib().orc_adjoint_run_f64(ctypes.byref(cp), _f64(arrs), nsteps, ckpt, tt.ctypes.data_as(ctypes.c_void_p),
+                                   ctypes.byref(J), _f64(grad), gK.ctypes.data_as(ctypes.c_void_p))
+    if rc:
+        raise RuntimeError(f"adjoint_run failed ({rc})")
+    return J.value, grad, gK

@@ -1,0 +1,241 @@
+"""The oracle's discrete adjoint (FP64, CPU) against central finite differences
+of the oracle's own forward step.
+
+Pins: compute_step! adjoint per input field, physics adjoint (Kessler +
+sedimentation, with the three Kessler parameters), the adjoint of one
+rk3_step!, and the checkpointed gradient of the adjoint workload's loss
+(initial theta' and qv at fixed density, K_auto, Qc_threshold, K_accr),
+plus a Taylor-remainder test and checkpoint-interval invariance.
+
+States carry a smooth cloud and rain background so that no perturbation
+switches a branch the adjoint freezes (a field that is identically zero
+takes the |r_den| < EPS branch of the Koren limiter, which a finite
+difference of any size leaves).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from moist2d_amd import ic
+
+PROG = ("rho", "rho_u", "rho_w", "rho_theta", "rho_qv", "rho_qc", "rho_qr")
+SCALES = {"rho": 1e-3, "rho_u": 1e-2, "rho_w": 1e-2, "rho_theta": 1e-1, "rho_qv": 1e-5, "rho_qc": 1e-6,
+          "rho_qr": 1e-6}
+
+
+def _wrap(p, a):
+    h = p.halo
+    a[:, :h] = a[:, p.Nx:p.Nx + h]
+    a[:, h + p.Nx:] = a[:, h:2 * h]
+    return a
+
+
+def _state(seed=1):
+    rng = np.random.default_rng(seed)
+    p, s = ic.bubble(np.float64, nx=16, nz=12, d=200.0)
+    s.rho_u[...] = 3.0 * s.rho
+    O.rk3_step(s, p, 3)
+    s.rho_qc[...] += 1.5e-3 * s.rho * (1 + 0.2 * rng.random(s.rho.shape))
+    s.rho_qr[...] += 1e-4 * s.rho * (1 + 0.2 * rng.random(s.rho.shape))
+    for n in PROG:
+        _wrap(p, getattr(s, n))
+    return p, s, rng
+
+
+def _dir(p, rng, shape):
+    return np.asfortranarray(_wrap(p, rng.normal(size=shape)))
+
+
+def _inner(p, a):
+    return a[:, p.halo:p.halo + p.Nx]
+
+
+def test_compute_step_adjoint_matches_fd():
+    p, s, rng = _state()
+    s0 = s.copy()
+    O.rk3_step(s, p, 1)
+    phys = [np.asfortranarray(rng.normal(0, sc, size=s.rho.shape)) for sc in (1e-6, 1e-4, 1e-4, 1e-7, 1e-7, 1e-7)]
+    W = [np.asfortranarray(rng.normal(size=s.rho.shape)) for _ in range(7)]
+    for w in W:
+        w[:, :p.halo] = 0
+        w[:, p.halo + p.Nx:] = 0
+    dt, ns, eps = 0.5, 3, 1e-4
+    ls, ls0, lph = O.compute_step_adj(phys, s, s0, p, dt, ns, W)
+
+    def out(s_, s0_, ph_):
+        a = s_.copy()
+        assert O.compute_step(ph_, a, s0_, p, dt, ns) == 0
+        return a
+
+    def jd(a, b):
+        return sum(float(np.sum(W[f] * (getattr(a, n) - getattr(b, n)))) for f, n in enumerate(PROG))
+
+    for f, n in enumerate(PROG):
+        for which in ("s", "s0"):
+            v = _dir(p, rng, s.rho.shape) * SCALES[n]
+            sp, sm, s0p, s0m = s.copy(), s.copy(), s0.copy(), s0.copy()
+            tp, tm = (sp, sm) if which == "s" else (s0p, s0m)
+            getattr(tp, n)[...] += eps * v
+            getattr(tm, n)[...] -= eps * v
+            fd = jd(out(sp, s0p, phys), out(sm, s0m, phys)) / (2 * eps)
+            ad = float(np.sum(_inner(p, (ls if which == "s" else ls0)[f] * v)))
+            assert abs(fd - ad) <= 2e-5 * abs(fd), (which, n, fd, ad)
+    for f in range(6):
+        v = _dir(p, rng, s.rho.shape) * [1e-5, 1e-3, 1e-3, 1e-6, 1e-6, 1e-6][f]
+        pp = [a.copy(order="F") for a in phys]
+        pm = [a.copy(order="F") for a in phys]
+        pp[f] += eps * v
+        pm[f] -= eps * v
+        fd = jd(out(s, s0, pp), out(s, s0, pm)) / (2 * eps)
+        ad = float(np.sum(_inner(p, lph[f] * v)))
+        assert abs(fd - ad) <= 1e-4 * abs(fd), (f, fd, ad)
+
+
+def test_physics_adjoint_matches_fd():
+    p, s, rng = _state(3)
+    s.rho_qv[...] *= 1 + 0.05 * rng.random(s.rho.shape)
+    _wrap(p, s.rho_qv)
+    Wp = [np.asfortranarray(rng.normal(size=s.rho.shape)) for _ in range(6)]
+    wd = rng.normal(size=p.Nx + 2 * p.halo)
+    ls, lK = O.physics_adj(s, p, Wp, wd)
+    eps = 1e-4
+
+    def jd(a, b):
+        (pa, da), (pb, db) = a, b
+        h = p.halo
+        return (sum(float(np.sum(_inner(p, Wp[f] * (pa[f] - pb[f])))) for f in range(6))
+                + float(np.sum((wd * (da - db))[h:h + p.Nx])))
+
+    for f, n in enumerate(PROG):
+        if n in ("rho_u", "rho_w"):
+            assert not np.any(ls[f]), n
+            continue
+        # rho_qc, rho_qr directions at 1e-4 so that h v stays well above FD roundoff
+        v = _dir(p, rng, s.rho.shape) * (1e-4 if n in ("rho_qc", "rho_qr") else SCALES[n])
+        sp, sm = s.copy(), s.copy()
+        getattr(sp, n)[...] += eps * v
+        getattr(sm, n)[...] -= eps * v
+        fd = jd(O.physics_fwd(sp, p), O.physics_fwd(sm, p)) / (2 * eps)
+        ad = float(np.sum(_inner(p, ls[f] * v)))
+        assert abs(fd - ad) <= 5e-5 * abs(fd), (n, fd, ad)
+    base = p.to_c()
+    eps = 1e-5  # one cell's qc lies within 1e-4 relative of Qc_threshold (a kink of max(0, qc - Qc0))
+    for j, name in enumerate(("K_auto", "Qc_threshold", "K_accr")):
+        v0 = getattr(base, name)
+        a = O.physics_fwd(s, p.replace(**{name: v0 * (1 + eps)}))
+        b = O.physics_fwd(s, p.replace(**{name: v0 * (1 - eps)}))
+        fd = jd(a, b) / (2 * eps * v0)
+        assert abs(fd - lK[j]) <= 5e-5 * abs(fd), (name, fd, lK[j])
+
+
+def test_rk3_step_adjoint_matches_fd():
+    p, s, rng = _state(4)
+    W = [np.asfortranarray(rng.normal(size=s.rho.shape)) for _ in range(7)]
+    for w in W:
+        w[:, :p.halo] = 0
+        w[:, p.halo + p.Nx:] = 0
+    lam = [w.copy(order="F") for w in W]
+    O.rk3_step_adj(s, p, lam)
+    eps = 1e-4
+
+    def step(s_):
+        a = s_.copy()
+        assert O.rk3_step(a, p, 1) == 0
+        return a
+
+    for f, n in enumerate(PROG):
+        v = _dir(p, rng, s.rho.shape) * SCALES[n]
+        sp, sm = s.copy(), s.copy()
+        getattr(sp, n)[...] += eps * v
+        getattr(sm, n)[...] -= eps * v
+        a, b = step(sp), step(sm)
+        fd = sum(float(np.sum(W[g] * (getattr(a, m) - getattr(b, m)))) for g, m in enumerate(PROG)) / (2 * eps)
+        ad = float(np.sum(_inner(p, lam[f] * v)))
+        assert abs(fd - ad) <= 5e-5 * abs(fd), (n, fd, ad)
+
+
+def _loss_setup():
+    p, s, rng = _state(2)
+    tt = np.asfortranarray(s.rho_theta / s.rho + rng.normal(0, 0.5, size=s.rho.shape))
+    return p, s, rng, tt
+
+
+def _loss(p, s, tt, nsteps):
+    a = s.copy()
+    assert O.rk3_step(a, p, nsteps) == 0
+    h = p.halo
+    th = (a.rho_theta / a.rho)[:, h:h + p.Nx]
+    return p.dx * a.prec[h:h + p.Nx].sum() + 0.5 * np.sum((th - tt[:, h:h + p.Nx]) ** 2)
+
+
+def _smooth_dir(p, rng, shape):
+    """A seeded random direction of low wavenumbers (x modes 1-3, z modes 0-2)."""
+    x = np.arange(p.Nx) / p.Nx
+    z = (np.arange(p.Nz) + 0.5) / p.Nz
+    v = np.zeros((p.Nz, p.Nx))
+    for m in range(1, 4):
+        for n in range(3):
+            a, b = rng.normal(size=2)
+            v += np.outer(np.sin(np.pi * n * z + 3 * rng.random()),
+                          a * np.cos(2 * np.pi * m * x) + b * np.sin(2 * np.pi * m * x))
+    full = np.zeros(shape)
+    full[:, p.halo:p.halo + p.Nx] = v
+    return np.asfortranarray(_wrap(p, full))
+
+
+@pytest.mark.parametrize("nsteps,ckpt", [(12, 5)])
+def test_loss_gradient_matches_central_fd(nsteps, ckpt):
+    """Central FD with h = 1e-6 relative on 8 seeded random directions of the
+    initial theta' and qv (at fixed density).  Over 12 steps the map is only
+    piecewise smooth (Koren limiter pieces, saturation clipping): a finite
+    difference that straddles a kink is off by up to ~1e-3 at h = 1e-4 and
+    ~1e-5 at h = 1e-6, so the bar is a median of 1e-5 and a maximum of 1e-4."""
+    p, s, rng, tt = _loss_setup()
+    J, grad, gK = O.adjoint_run(s.copy(), p, nsteps, tt, ckpt=ckpt)
+    assert abs(J - _loss(p, s, tt, nsteps)) <= 1e-12 * abs(J)
+    eps = 1e-6
+    errs = []
+    for _ in range(4):
+        for f, n, sc in ((3, "rho_theta", 0.01), (4, "rho_qv", 1e-5)):
+            v = _smooth_dir(p, rng, s.rho.shape) * sc
+            sp, sm = s.copy(), s.copy()
+            getattr(sp, n)[...] += eps * v * s.rho
+            getattr(sm, n)[...] -= eps * v * s.rho
+            fd = (_loss(p, sp, tt, nsteps) - _loss(p, sm, tt, nsteps)) / (2 * eps)
+            ad = float(np.sum(_inner(p, grad[f] * s.rho * v)))
+            errs.append(abs(fd - ad) / abs(ad))
+    assert np.median(errs) <= 1e-5 and max(errs) <= 1e-4, errs
+    eps = 1e-5
+    base = p.to_c()
+    for j, name in enumerate(("K_auto", "Qc_threshold", "K_accr")):
+        v0 = getattr(base, name)
+        fd = (_loss(p.replace(**{name: v0 * (1 + eps)}), s, tt, nsteps)
+              - _loss(p.replace(**{name: v0 * (1 - eps)}), s, tt, nsteps)) / (2 * eps * v0)
+        assert abs(fd - gK[j]) <= 1e-5 * abs(fd), (name, fd, gK[j])
+
+
+def test_taylor_remainder_is_second_order():
+    """|J(x + h v) - J(x) - h <g, v>| falls by ~100 when h falls by 10."""
+    p, s, rng, tt = _loss_setup()
+    nsteps = 6
+    J, grad, _ = O.adjoint_run(s.copy(), p, nsteps, tt, ckpt=nsteps)
+    v = [_dir(p, rng, s.rho.shape) * SCALES[n] for n in PROG]
+    gv = sum(float(np.sum(_inner(p, grad[f] * v[f]))) for f in range(7))
+    rem = []
+    for h in (1e-1, 1e-2):
+        sp = s.copy()
+        for f, n in enumerate(PROG):
+            getattr(sp, n)[...] += h * v[f]
+        rem.append(abs(_loss(p, sp, tt, nsteps) - J - h * gv))
+    ratio = rem[0] / rem[1]
+    assert 60 < ratio < 140, (rem, ratio)
+
+
+def test_checkpoint_interval_does_not_change_the_gradient():
+    p, s, rng, tt = _loss_setup()
+    a = O.adjoint_run(s.copy(), p, 7, tt, ckpt=7)
+    b = O.adjoint_run(s.copy(), p, 7, tt, ckpt=2)
+    assert a[0] == b[0]
+    for f in range(7):
+        np.testing.assert_array_equal(a[1][f], b[1][f])
+    np.testing.assert_array_equal(a[2], b[2])
