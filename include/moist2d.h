@@ -144,6 +144,26 @@ int moist_kernel_events(moist_handle *h, int on);
 int moist_kernel_stats(moist_handle *h, double *acoustic_ms_per_launch,
                        int64_t *acoustic_launches);
 
+/* Reverse mode (FP64-form handle, do_sf = do_rd = false).  Replaces
+ * ForwardDiff / a ChainRulesCore rrule around the reference's driver loop
+ * (run_sim, moist.jl:1930-1963, calling rk3_step!, moist.jl:1589): runs
+ * nsteps forward from the uploaded state with checkpoints every `ckpt`
+ * steps, evaluates per member
+ *   J = dx * sum_i prec_i + 1/2 * sum_cells (rho_theta/rho - theta_target)^2
+ * and sweeps back, recomputing each segment from its checkpoint and each
+ * step's acoustic substeps.  theta_target[m]: host array (ka, ia) in the
+ * reference layout (halo columns ignored).  J[m] receives the loss.  The
+ * handle's state afterwards is the final forward state. */
+int moist_adjoint_run(moist_handle *h, int nsteps, int ckpt, const void *const *theta_target, int elem_bytes,
+                      double *J);
+/* dJ/d(initial rho, rho_u, rho_w, rho_theta, rho_qv, rho_qc, rho_qr) of one
+ * member into host arrays (ka, ia) (interior columns written), and
+ * dJ/d(K_auto, Qc_threshold, K_accr) into grad_params[3]; either may be NULL. */
+int moist_adjoint_grad(moist_handle *h, int member, void *const grad[7], double *grad_params, int elem_bytes);
+/* Device time of the last adjoint run: forward pass (with checkpoint copies)
+ * and reverse sweep (segment recomputation included), in ms. */
+int moist_adjoint_times(moist_handle *h, double *ms_forward, double *ms_reverse);
+
 const char *moist_last_error(const moist_handle *h);
 void moist_destroy(moist_handle *h);
 

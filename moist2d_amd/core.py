@@ -54,6 +54,9 @@ def lib():
             "moist_time_steps": (ip, [vp, ip, ctypes.POINTER(ctypes.c_double)]),
             "moist_kernel_events": (ip, [vp, ip]),
             "moist_kernel_stats": (ip, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
+            "moist_adjoint_run": (ip, [vp, ip, ip, vp, ip, ctypes.POINTER(ctypes.c_double)]),
+            "moist_adjoint_grad": (ip, [vp, ip, vp, ctypes.POINTER(ctypes.c_double), ip]),
+            "moist_adjoint_times": (ip, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
             "moist_last_error": (ctypes.c_char_p, [vp]),
             "moist_destroy": (None, [vp]),
         }
@@ -157,6 +160,31 @@ class Engine:
         ms = ctypes.c_double()
         self._check(lib().moist_time_steps(self._h, nsteps, ctypes.byref(ms)), "moist_time_steps")
         return ms.value
+
+    def adjoint(self, nsteps, theta_target, ckpt=50):
+        """Loss and reverse sweep over `nsteps` large steps from the uploaded
+        state (FP64 form).  theta_target: one (ka, ia) array per member.
+        Returns J per member; gradients come from `adjoint_grad`."""
+        tt = [np.asfortranarray(t, dtype=np.float64) for t in theta_target]
+        self._keep_tt = tt
+        J = (ctypes.c_double * len(self.params))()
+        self._check(lib().moist_adjoint_run(self._h, nsteps, ckpt, _ptrs(tt), 8, J), "moist_adjoint_run")
+        return list(J)
+
+    def adjoint_grad(self, member=0):
+        """(dJ/d initial prognostic fields (7 arrays (ka, ia), interior columns),
+        dJ/d (K_auto, Qc_threshold, K_accr))."""
+        p = self.params[member]
+        g = [np.zeros((p.Nz, p.Nx + 2 * p.halo), order="F") for _ in range(7)]
+        gk = (ctypes.c_double * 3)()
+        self._check(lib().moist_adjoint_grad(self._h, member, _ptrs(g), gk, 8), "moist_adjoint_grad")
+        return g, np.array(list(gk))
+
+    def adjoint_times(self):
+        """(forward ms, reverse ms) of the last adjoint run."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().moist_adjoint_times(self._h, ctypes.byref(a), ctypes.byref(b)), "moist_adjoint_times")
+        return a.value, b.value
 
     def kernel_stats(self):
         ms = ctypes.c_double()

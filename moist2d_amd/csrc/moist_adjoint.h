@@ -1,0 +1,1110 @@
+// moist_adjoint.h — reverse-mode (adjoint) kernels of the FP64 time step.
+// Included by moist_api.hip after moist_kernels.h, inside namespace mo.
+//
+// The adjoint of rk3_step! (moist.jl:1589-1683) runs stage by stage in
+// reverse: scalar transport + final update (moist.jl:1469-1539), the
+// acoustic substeps (moist.jl:1351-1461, with the transposed tridiagonal
+// solve of implicit_correction!, moist.jl:974-1030), the perturbation
+// initialisation, the slow tendencies (moist.jl:1071-1332) and the stage
+// diagnostics (moist.jl:1044-1069); then Kessler microphysics and rain
+// sedimentation (moist.jl:554-707).  Every branch the forward pass takes
+// (upwind direction, Koren limiter piece, positivity limiter, saturation
+// clipping, caps) is frozen: the kernels differentiate the piece taken.
+//
+// All kernels are gathers: the thread that owns an input cell sums the
+// partial derivatives of every flux or update that reads it, so results are
+// deterministic and no atomics are needed.  Partial derivatives of the
+// nonlinear point functions (Koren faces, diagnostics, Kessler) come from
+// forward-mode dual numbers evaluated on the same formula.
+//
+// Adjoint arrays live in the same x-fastest pools as the state (Dom::off2,
+// periodic x through wrapn).  FP64 only.
+
+constexpr int AD_X = 64, AD_Z = 4;  // cell tile of the gather kernels
+
+// ---- dual numbers -------------------------------------------------------------
+template <int N> struct Dl {
+    double v;
+    double d[N];
+    __device__ Dl() : v(0.0) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) d[j] = 0.0;
+    }
+    __device__ Dl(double a) : v(a) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) d[j] = 0.0;
+    }
+    __device__ static Dl var(double a, int j) {
+        Dl r(a);
+        r.d[j] = 1.0;
+        return r;
+    }
+};
+template <int N> __device__ __forceinline__ Dl<N> operator+(const Dl<N> &a, const Dl<N> &b) {
+    Dl<N> r(a.v + b.v);
+#pragma unroll
+    for (int j = 0; j < N; ++j) r.d[j] = a.d[j] + b.d[j];
+    return r;
+}
+template <int N> __device__ __forceinline__ Dl<N> operator-(const Dl<N> &a, const Dl<N> &b) {
+    Dl<N> r(a.v - b.v);
+#pragma unroll
+    for (int j = 0; j < N; ++j) r.d[j] = a.d[j] - b.d[j];
+    return r;
+}
+template <int N> __device__ __forceinline__ Dl<N> operator-(const Dl<N> &a) {
+    Dl<N> r(-a.v);
+#pragma unroll
+    for (int j = 0; j < N; ++j) r.d[j] = -a.d[j];
+    return r;
+}
+template <int N> __device__ __forceinline__ Dl<N> operator*(const Dl<N> &a, const Dl<N> &b) {
+    Dl<N> r(a.v * b.v);
+#pragma unroll
+    for (int j = 0; j < N; ++j) r.d[j] = a.d[j] * b.v + a.v * b.d[j];
+    return r;
+}
+template <int N> __device__ __forceinline__ Dl<N> operator/(const Dl<N> &a, const Dl<N> &b) {
+    Dl<N> r(a.v / b.v);
+#pragma unroll
+    for (int j = 0; j < N; ++j) r.d[j] = (a.d[j] - r.v * b.d[j]) / b.v;
+    return r;
+}
+template <int N> __device__ __forceinline__ Dl<N> operator+(const Dl<N> &a, double b) { return a + Dl<N>(b); }
+template <int N> __device__ __forceinline__ Dl<N> operator+(double a, const Dl<N> &b) { return Dl<N>(a) + b; }
+template <int N> __device__ __forceinline__ Dl<N> operator-(const Dl<N> &a, double b) { return a - Dl<N>(b); }
+template <int N> __device__ __forceinline__ Dl<N> operator-(double a, const Dl<N> &b) { return Dl<N>(a) - b; }
+template <int N> __device__ __forceinline__ Dl<N> operator*(const Dl<N> &a, double b) { return a * Dl<N>(b); }
+template <int N> __device__ __forceinline__ Dl<N> operator*(double a, const Dl<N> &b) { return Dl<N>(a) * b; }
+template <int N> __device__ __forceinline__ Dl<N> operator/(const Dl<N> &a, double b) { return a / Dl<N>(b); }
+template <int N> __device__ __forceinline__ Dl<N> operator/(double a, const Dl<N> &b) { return Dl<N>(a) / b; }
+template <int N> __device__ __forceinline__ Dl<N> d_exp(const Dl<N> &a) {
+    Dl<N> r(exp(a.v));
+#pragma unroll
+    for (int j = 0; j < N; ++j) r.d[j] = r.v * a.d[j];
+    return r;
+}
+template <int N> __device__ __forceinline__ Dl<N> d_sqrt(const Dl<N> &a) {
+    Dl<N> r(sqrt(a.v));
+#pragma unroll
+    for (int j = 0; j < N; ++j) r.d[j] = a.d[j] / (2.0 * r.v);
+    return r;
+}
+// a^b for a > 0
+template <int N> __device__ __forceinline__ Dl<N> d_pow(const Dl<N> &a, const Dl<N> &b) {
+    Dl<N> r(pow(a.v, b.v));
+    const double la = log(a.v);
+#pragma unroll
+    for (int j = 0; j < N; ++j) r.d[j] = r.v * (b.d[j] * la + b.v * a.d[j] / a.v);
+    return r;
+}
+template <int N> __device__ __forceinline__ double val(const Dl<N> &a) { return a.v; }
+// Julia Base.min/max on values; the derivative follows the operand chosen
+template <class V> __device__ __forceinline__ V vmin(const V &x, const V &y) {
+    return signbit(val(x) - val(y)) ? x : y;
+}
+template <class V> __device__ __forceinline__ V vmax(const V &x, const V &y) {
+    return signbit(val(x) - val(y)) ? y : x;
+}
+
+__device__ __forceinline__ int wrapn(int j, int nx) {
+    j %= nx;
+    return j < 0 ? j + nx : j;
+}
+
+// ---- Koren faces with dual numbers (moist.jl:850-945) ---------------------------
+template <class V> __device__ __forceinline__ V koren_v(V r) {
+    return vmax(V(0.0), vmin(V(2.0) * r, vmin((V(1.0) + V(2.0) * r) / V(3.0), V(2.0))));
+}
+// face between cells holding q1 and q2 (q0 upstream of q1, q3 downstream of q2)
+template <class V> __device__ __forceinline__ V kface_x(V q0, V q1, V q2, V q3, V m) {
+    const double EPS = 1e-12;
+    V s;
+    if (val(m) > EPS) {
+        V rn = q2 - q1, rd = q1 - q0;
+        V r = fabs(val(rd)) < EPS ? V(1.0) : rn / rd;
+        s = q1 + V(0.5) * koren_v(r) * rd;
+    } else if (val(m) < -EPS) {
+        V rn = q1 - q2, rd = q2 - q3;
+        V r = fabs(val(rd)) < EPS ? V(1.0) : rn / rd;
+        s = q2 + V(0.5) * koren_v(r) * rd;
+    } else {
+        s = V(0.5) * (q1 + q2);
+    }
+    return s * m;
+}
+template <class V>
+__device__ __forceinline__ V kface_z(V q0, V q1, V q2, V q3, V m, V lo, V hi, bool bottom, bool top, double dzk,
+                                     double dzk1, double dtL) {
+    const double EPS = 1e-12;
+    V s;
+    if (val(m) > EPS) {
+        if (bottom) {
+            s = (q1 * dzk1 + q2 * dzk) / (dzk + dzk1);
+            s = vmin(s, lo / (m * dtL));
+        } else {
+            V rn = q2 - q1, rd = q1 - q0;
+            V r = fabs(val(rd)) < EPS ? V(1.0) : rn / rd;
+            s = q1 + V(0.5) * koren_v(r) * rd;
+        }
+    } else if (val(m) < -EPS) {
+        if (top) {
+            s = (q1 * dzk1 + q2 * dzk) / (dzk + dzk1);
+            s = vmin(s, hi / (-m * dtL));
+        } else {
+            V rn = q1 - q2, rd = q2 - q3;
+            V r = fabs(val(rd)) < EPS ? V(1.0) : rn / rd;
+            s = q2 - V(0.5) * koren_v(r) * rd;
+        }
+    } else {
+        s = (q1 * dzk1 + q2 * dzk) / (dzk + dzk1);
+    }
+    return s * m;
+}
+
+// ---- third-order upwind fluxes: coefficient of one input ------------------------
+// A flux m * g at the interface between cell slots 0 and 1 with g from the
+// values at slots -1, 0, 1, 2: positive m uses (-v[-1] + 5 v[0] + 2 v[1]) / 6,
+// negative m (-v[2] + 5 v[1] + 2 v[0]) / 6, otherwise w0 v[0] + w1 v[1]
+// (1/2, 1/2 or the f2h weights).  `pos` / `neg` carry the forward pass's
+// branch decision (including its boundary conditions).
+__device__ __forceinline__ double upw_coef(bool pos, bool neg, int off, double w0, double w1) {
+    if (pos) return off == -1 ? -1.0 / 6.0 : off == 0 ? 5.0 / 6.0 : off == 1 ? 2.0 / 6.0 : 0.0;
+    if (neg) return off == 2 ? -1.0 / 6.0 : off == 1 ? 5.0 / 6.0 : off == 0 ? 2.0 / 6.0 : 0.0;
+    return off == 0 ? w0 : off == 1 ? w1 : 0.0;
+}
+__device__ __forceinline__ double upw_val(bool pos, bool neg, double vm1, double v0, double v1, double v2, double w0,
+                                          double w1) {
+    if (pos) return upw3(vm1, v0, v1) / 6.0;
+    if (neg) return upw3(v2, v1, v0) / 6.0;
+    return v0 * w0 + v1 * w1;
+}
+
+// ---- adjoint work arrays of one stage --------------------------------------------
+struct AdjW {
+    // forward record of the stage (FP64)
+    const double *theta, *pres, *rt2, *R_rho, *R_rhou, *R_rhow, *R_rt, *mom_x, *mom_z;
+    // adjoints
+    double *Lth, *Lrt, *Lpres, *LU, *LW, *LRr, *LRu, *LRw, *LRt, *Lmx, *Lmz;
+    double *Lrp, *Lrwp, *Lrtp, *pend, *Linc_r, *Linc_t;
+    double *Y, *cpr, *rmi;     // column scratch: transposed-solve vector, c', 1/m of the forward Thomas
+    double *linb, *lint;       // per column: positivity-limiter adjoints (3 scalars x columns)
+};
+
+// Fill of `n` doubles (graph-safe zeroing).
+__global__ void ka_fill(double *p, int64_t n, double v) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        p[q] = v;
+}
+
+// ---- loss: J = dx sum prec + 1/2 sum (theta - theta_target)^2 -------------------
+// One block per member: writes J[m] and the adjoint of the final state.
+__global__ void __launch_bounds__(256) ka_loss(const Dom *__restrict__ doms, SSet<double> s, const double *prec,
+                                               const double *__restrict__ tgt, double *lam_rho, double *lam_rt,
+                                               double *J) {
+    const int m = blockIdx.x;
+    const Dom &D = doms[m];
+    const int64_t n = (int64_t)D.nx * D.nz;
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
+        const int64_t q = D.off2 + e;
+        const double th = s.f[RT][q] / s.f[RHO][q];
+        const double d = th - tgt[q];
+        acc += 0.5 * d * d;
+        lam_rt[q] = d / s.f[RHO][q];
+        lam_rho[q] = -d * th / s.f[RHO][q];
+    }
+    for (int i = threadIdx.x; i < D.nx; i += blockDim.x) acc += D.dx * prec[D.offx + i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) J[m] = red[0];
+}
+
+// ---- scalar transport + final update (moist.jl:1469-1539) ------------------------
+// lo: adjoint of the stage output; writes ls (stage input, first writer of the
+// stage), adds into ls0 (rho_q) and lph (q tendencies); writes Lmx, Lmz and
+// the substep adjoints Lrp, Lrtp, Lrwp, Lrup (= lo of rho, rho_theta, rho_w, rho_u).
+__global__ void __launch_bounds__(AD_X *AD_Z)
+    ka_scalar(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s, SSet<double> lo,
+              SSet<double> ls, SSet<double> ls0, SSet<double> lph, AdjW a, double *Lrup, int div) {
+    const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;
+    const Dom &D = doms[t.x];
+    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+    const int nx = D.nx, nz = D.nz, ke = nz - 1;
+    if (i >= nx || k >= nz) return;
+    const double *zd = zv.z + D.offz, *dzd = zv.dz + D.offz;
+    const double dx = D.dx, kap = D.kappa, dt = D.dt / double(div);
+    auto G = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
+    const int64_t c = G(k, i);
+    const double *rho = s.f[RHO];
+    double lrho = lo.f[RHO][c];  // final update: rho_out = rho + rho'
+    double lmx = 0.0, lmz = 0.0;
+    // ---- the three scalars
+    for (int sc = 0; sc < 3; ++sc) {
+        const double *qr = s.f[QV + sc];
+        const double *L = lo.f[QV + sc];
+        auto Q = [&](int kk, int ii) { const int64_t g = G(kk, ii); return qr[g] / rho[g]; };
+        auto LFx = [&](int j) { return dt / dx * (L[G(k, j + 1)] - L[G(k, j)]); };
+        auto LFz = [&](int kk) { return dt * (L[G(kk + 1, i)] / dzd[kk + 1] - L[G(kk, i)] / dzd[kk]); };
+        ls0.f[QV + sc][c] += L[c];
+        lph.f[3 + sc][c] += dt * L[c];
+        double Lq = 0.0;
+        // x faces j+1/2, j = i-2..i+1, read q(j-1..j+2); slot of cell i is i-j+1
+        for (int j = i - 2; j <= i + 1; ++j) {
+            const int slot = i - j + 1;
+            typedef Dl<1> V;
+            V q0(Q(k, j - 1)), q1(Q(k, j)), q2(Q(k, j + 1)), q3(Q(k, j + 2));
+            if (slot == 0) q0.d[0] = 1.0;
+            if (slot == 1) q1.d[0] = 1.0;
+            if (slot == 2) q2.d[0] = 1.0;
+            if (slot == 3) q3.d[0] = 1.0;
+            const V F = kface_x(q0, q1, q2, q3, V(a.mom_x[G(k, j)]));
+            Lq += LFx(j) * F.d[0];
+        }
+        // z faces kk+1/2, kk = k-2..k+1 within 0..ke-1, read q(kk-1..kk+2)
+        for (int kk = k - 2; kk <= k + 1; ++kk) {
+            if (kk < 0 || kk > ke - 1) continue;
+            const int slot = k - kk + 1;
+            typedef Dl<1> V;
+            V q0(kk > 0 ? Q(kk - 1, i) : 0.0), q1(Q(kk, i)), q2(Q(kk + 1, i)), q3(kk + 2 <= ke ? Q(kk + 2, i) : 0.0);
+            if (slot == 0) q0.d[0] = 1.0;
+            if (slot == 1) q1.d[0] = 1.0;
+            if (slot == 2) q2.d[0] = 1.0;
+            if (slot == 3) q3.d[0] = 1.0;
+            const V F = kface_z(q0, q1, q2, q3, V(a.mom_z[G(kk, i)]), V(qr[G(kk, D.lin_col)]),
+                                V(qr[G(kk + 1, D.lin_col)]), kk == 0, kk == ke - 1, dzd[kk], dzd[kk + 1], D.dt);
+            Lq += LFz(kk) * F.d[0];
+        }
+        // diffusive x faces j+1/2, j = i-1, i: D = (rho_j + rho_j+1) / 2 (q_j+1 - q_j) / dx
+        for (int j = i - 1; j <= i; ++j) {
+            const double lD = kap * dt / dx * (L[G(k, j)] - L[G(k, j + 1)]);
+            const double r0 = rho[G(k, j)], r1 = rho[G(k, j + 1)];
+            Lq += lD * (r1 + r0) * 0.5 / dx * (j == i - 1 ? 1.0 : -1.0);
+            lrho += lD * 0.5 * (Q(k, j + 1) - Q(k, j)) / dx;
+        }
+        // diffusive z faces kk+1/2, kk = k-1, k: D = f2h(rho) (q_kk+1 - q_kk) / (z_kk+1 - z_kk)
+        for (int kk = k - 1; kk <= k; ++kk) {
+            if (kk < 0 || kk > ke - 1) continue;
+            const double lD = kap * dt * (L[G(kk, i)] / dzd[kk] - L[G(kk + 1, i)] / dzd[kk + 1]);
+            const double r0 = rho[G(kk, i)], r1 = rho[G(kk + 1, i)];
+            const double den = dzd[kk + 1] + dzd[kk];
+            const double rf = (r1 * dzd[kk] + r0 * dzd[kk + 1]) / den;
+            const double zz = zd[kk + 1] - zd[kk];
+            const double gq = (Q(kk + 1, i) - Q(kk, i)) / zz;
+            Lq += lD * rf / zz * (kk == k - 1 ? 1.0 : -1.0);
+            lrho += lD * gq * (kk == k - 1 ? dzd[kk] : dzd[kk + 1]) / den;
+        }
+        const double qc = qr[c] / rho[c];
+        ls.f[QV + sc][c] = Lq / rho[c];
+        lrho -= Lq * qc / rho[c];
+        // this cell owns face i+1/2 (mom_x) and face k+1/2 (mom_z)
+        {
+            typedef Dl<1> V;
+            const V F = kface_x(V(Q(k, i - 1)), V(Q(k, i)), V(Q(k, i + 1)), V(Q(k, i + 2)), V::var(a.mom_x[c], 0));
+            lmx += LFx(i) * F.d[0];
+        }
+        if (k <= ke - 1) {
+            typedef Dl<3> V;
+            const V F = kface_z(V(k > 0 ? Q(k - 1, i) : 0.0), V(Q(k, i)), V(Q(k + 1, i)),
+                                V(k + 2 <= ke ? Q(k + 2, i) : 0.0), V::var(a.mom_z[c], 0),
+                                V::var(qr[G(k, D.lin_col)], 1), V::var(qr[G(k + 1, D.lin_col)], 2), k == 0,
+                                k == ke - 1, dzd[k], dzd[k + 1], D.dt);
+            const double lf = LFz(k);
+            lmz += lf * F.d[0];
+            // linear-index positivity limiters: bottom face reads (0, lin_col), top face (ke, lin_col)
+            if (k == 0) a.linb[D.offx * 3 + (int64_t)sc * nx + i] = lf * F.d[1];
+            if (k == ke - 1) a.lint[D.offx * 3 + (int64_t)sc * nx + i] = lf * F.d[2];
+        }
+    }
+    a.Lmx[c] = lmx;
+    a.Lmz[c] = lmz;
+    if (ke == 0) {
+        for (int sc = 0; sc < 3; ++sc) {
+            a.linb[D.offx * 3 + (int64_t)sc * nx + i] = 0.0;
+            a.lint[D.offx * 3 + (int64_t)sc * nx + i] = 0.0;
+        }
+    }
+    // final update (moist.jl:1522-1539): identity parts and the perturbation adjoints
+    ls.f[RHO][c] = lrho;
+    ls.f[RHOU][c] = lo.f[RHOU][c];
+    ls.f[RT][c] = lo.f[RT][c];
+    ls.f[RHOW][c] = lo.f[RHOW][c];
+    a.Lrp[c] = lo.f[RHO][c];
+    Lrup[c] = lo.f[RHOU][c];
+    a.Lrtp[c] = lo.f[RT][c];
+    a.Lrwp[c] = k <= ke - 1 ? lo.f[RHOW][c] : 0.0;
+}
+
+// Sum of the positivity-limiter adjoints of every column into rho_q at
+// (0, lin_col) and (ke, lin_col).  One block per member.
+__global__ void __launch_bounds__(256) ka_lin(const Dom *__restrict__ doms, AdjW a, SSet<double> ls) {
+    const Dom &D = doms[blockIdx.x];
+    __shared__ double rb[256], rt[256];
+    for (int sc = 0; sc < 3; ++sc) {
+        double b = 0.0, tp = 0.0;
+        for (int i = threadIdx.x; i < D.nx; i += blockDim.x) {
+            b += a.linb[D.offx * 3 + (int64_t)sc * D.nx + i];
+            tp += a.lint[D.offx * 3 + (int64_t)sc * D.nx + i];
+        }
+        rb[threadIdx.x] = b;
+        rt[threadIdx.x] = tp;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) {
+                rb[threadIdx.x] += rb[threadIdx.x + w];
+                rt[threadIdx.x] += rt[threadIdx.x + w];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0 && D.nz >= 2) {
+            ls.f[QV + sc][D.off2 + D.lin_col] += rb[0];
+            ls.f[QV + sc][D.off2 + (int64_t)(D.nz - 1) * D.nx + D.lin_col] += rt[0];
+        }
+        __syncthreads();
+    }
+}
+
+// ---- acoustic substep, column part (moist.jl:1377-1446) ---------------------------
+// One thread per column.  In: Lrp, Lrtp, Lrwp (adjoints after the substep),
+// Lmz.  Out: Lrwp (adjoint of rho_w' before the solve), Lrp / Lrtp with the
+// off-centred contributions, Linc_r / Linc_t, and Lth, Lrt, LRr, LRw, LRt
+// accumulated.  `factor`: first reverse substep of the stage, which also
+// factors the stage's tridiagonal matrix into c' and 1/m.
+__global__ void __launch_bounds__(64)
+    ka_ac_col(const Dom *__restrict__ doms, const int2 *__restrict__ ctiles, ZVec zv, SSet<double> s, AdjW a,
+              const double *__restrict__ rtp_s, const double *__restrict__ rup_n, const double *__restrict__ rwp_s,
+              const double *__restrict__ rwp_n, int div, int nsub, int factor) {
+    const int2 t = ctiles[blockIdx.x];
+    if (t.x < 0) return;
+    const Dom &D = doms[t.x];
+    const int i = t.y + threadIdx.x;
+    const int nx = D.nx, nz = D.nz, ke = nz - 1;
+    if (i >= nx) return;
+    const double *zd = zv.z + D.offz, *dzd = zv.dz + D.offz;
+    const double dx = D.dx, EPS = 1e-12, GRAV = K<double>::GRAV;
+    auto G = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
+    const double *TH = a.theta, *RT = a.rt2;
+    const double dt_s = D.dt / double(div) / double(nsub);
+    const double fact_tp = 0.5 * (1.0 + D.beta), fact_tm = 0.5 * (1.0 - D.beta);
+    const double fact = fact_tm / fact_tp;
+    const double dt_tp2 = (dt_s * fact_tp) * (dt_s * fact_tp);
+    const int n = nz - 1;
+    auto th_f = [&](int kk) { return f2h(TH[G(kk, i)], TH[G(kk + 1, i)], dzd[kk], dzd[kk + 1]); };
+    auto coef_a = [&](int kk) {  // a_j, j = kk (kk > 0)
+        const double z1 = zd[kk + 1] - zd[kk];
+        return -dt_tp2 * (RT[G(kk, i)] * th_f(kk - 1) / (z1 * dzd[kk]) - GRAV / (dzd[kk] + dzd[kk + 1]));
+    };
+    auto coef_b = [&](int kk) {
+        const double z1 = zd[kk + 1] - zd[kk];
+        return dt_tp2 * (RT[G(kk, i)] / dzd[kk] + RT[G(kk + 1, i)] / dzd[kk + 1]) * th_f(kk) / z1;
+    };
+    auto coef_c = [&](int kk) {  // kk < ke - 1
+        const double z1 = zd[kk + 1] - zd[kk];
+        return -dt_tp2 * (RT[G(kk + 1, i)] * th_f(kk + 1) / (z1 * dzd[kk + 1]) + GRAV / (dzd[kk] + dzd[kk + 1]));
+    };
+    if (factor && n > 0) {
+        // forward Thomas factorization: m_j = b_j + 1 - a_j c'_j-1, c'_j = c_j / m_j
+        double cprev = 0.0;
+        for (int kk = 0; kk <= ke - 1; ++kk) {
+            const double m = coef_b(kk) + 1.0 - (kk > 0 ? coef_a(kk) * cprev : 0.0);
+            const double rm = 1.0 / m;
+            const double cp = kk < ke - 1 ? coef_c(kk) * rm : 0.0;
+            a.cpr[G(kk, i)] = cp;
+            a.rmi[G(kk, i)] = rm;
+            cprev = cp;
+        }
+    }
+    // momentum average (mom_z += rwp / ns) and the two vertical updates
+    for (int k = 0; k <= ke - 1; ++k) a.Lrwp[G(k, i)] += a.Lmz[G(k, i)] / double(nsub);
+    for (int k = 0; k <= ke; ++k) {
+        const double l = a.Lrtp[G(k, i)];
+        const double lr = a.Lrp[G(k, i)];
+        if (k < ke) {  // fkp of cell k: w = rwp_n[k], positive branch tests w, negative tests s.rho_w[k]
+            const double lf = -dt_s / dzd[k] * l;
+            const double w = rwp_n[G(k, i)];
+            const bool pos = w > EPS && k > 0, neg = !pos && s.f[RHOW][G(k, i)] < -EPS && k < ke - 1;
+            const double den = dzd[k] + dzd[k + 1];
+            const double w0 = dzd[k + 1] / den, w1 = dzd[k] / den;
+            const double g = upw_val(pos, neg, k > 0 ? TH[G(k - 1, i)] : 0.0, TH[G(k, i)], TH[G(k + 1, i)],
+                                     k + 2 <= ke ? TH[G(k + 2, i)] : 0.0, w0, w1);
+            for (int o = -1; o <= 2; ++o) {
+                const int kk = k + o;
+                if (kk < 0 || kk > ke) continue;
+                const double cf = upw_coef(pos, neg, o, w0, w1);
+                if (cf != 0.0) a.Lth[G(kk, i)] += lf * w * cf;
+            }
+            a.Lrwp[G(k, i)] += lf * g - dt_s / dzd[k] * lr;
+        }
+        if (k > 0) {  // fkm of cell k: face k-1/2, w = rwp_n[k-1], both branches test s.rho_w[k-1]
+            const double lf = dt_s / dzd[k] * l;
+            const int kb = k - 1;
+            const double w = rwp_n[G(kb, i)], sw = s.f[RHOW][G(kb, i)];
+            const bool pos = sw > EPS && k > 1, neg = !pos && sw < -EPS && k < ke;
+            const double den = dzd[kb] + dzd[k];
+            const double w0 = dzd[k] / den, w1 = dzd[kb] / den;
+            const double g = upw_val(pos, neg, kb > 0 ? TH[G(kb - 1, i)] : 0.0, TH[G(kb, i)], TH[G(k, i)],
+                                     k + 1 <= ke ? TH[G(k + 1, i)] : 0.0, w0, w1);
+            for (int o = -1; o <= 2; ++o) {
+                const int kk = kb + o;
+                if (kk < 0 || kk > ke) continue;
+                const double cf = upw_coef(pos, neg, o, w0, w1);
+                if (cf != 0.0) a.Lth[G(kk, i)] += lf * w * cf;
+            }
+            a.Lrwp[G(kb, i)] += lf * g + dt_s / dzd[k] * lr;
+        }
+    }
+    if (n <= 0) return;
+    // transposed solve A^T y = Lx with A = L U: U^T w = Lx (forward), L^T y = w (backward)
+    {
+        double wprev = 0.0;
+        for (int kk = 0; kk <= ke - 1; ++kk) {
+            const double wv = a.Lrwp[G(kk, i)] - (kk > 0 ? a.cpr[G(kk - 1, i)] * wprev : 0.0);
+            a.Y[G(kk, i)] = wv;
+            wprev = wv;
+        }
+        double ynext = 0.0;
+        for (int kk = ke - 1; kk >= 0; --kk) {
+            const double yv = (a.Y[G(kk, i)] - (kk < ke - 1 ? coef_a(kk + 1) * ynext : 0.0)) * a.rmi[G(kk, i)];
+            a.Y[G(kk, i)] = yv;
+            ynext = yv;
+        }
+    }
+    // matrix and right-hand-side adjoints, rho_w' before the solve, drwp and the explicit part
+    double yprev = 0.0, Lthc_carry = 0.0;
+    double rtp_tp_k = 0.0;
+    auto rtp_tp = [&](int kk) {
+        const int64_t q = G(kk, i);
+        const double fip = (TH[G(kk, i + 1)] + TH[q]) * 0.5 * rup_n[q];
+        const double fim = (TH[q] + TH[G(kk, i - 1)]) * 0.5 * rup_n[G(kk, i - 1)];
+        const double inc = dt_s * (a.R_rt[q] + (-(fip - fim) / dx));
+        return rtp_s[q] + fact_tp * inc;
+    };
+    rtp_tp_k = rtp_tp(0);
+    // Lthc (adjoint of the face theta th_j) gathers from b_j, a_j+1 and c_j-1
+    for (int k = 0; k <= ke; ++k) {
+        const int64_t q = G(k, i);
+        const double y = k <= ke - 1 ? a.Y[q] : 0.0;
+        double Lrp_tp = 0.0, Lrtp_tp = 0.0;
+        if (k <= ke - 1) {
+            const double z1 = zd[k + 1] - zd[k];
+            const double x = rwp_n[q];
+            const double ws = rwp_s[q];
+            const double b = coef_b(k);
+            double La = 0.0, Lc = 0.0;
+            const double Lb = -y * x - fact * y * ws;
+            double lws = y * (1.0 - fact * b);
+            if (k > 0) {
+                La = -y * rwp_n[G(k - 1, i)] - fact * y * ws;
+                lws += -fact * coef_a(k) * y;
+                lws += -fact * coef_c(k - 1) * yprev;
+            }
+            if (k < ke - 1) Lc = -y * rwp_n[G(k + 1, i)] - fact * y * rwp_s[G(k + 1, i)];
+            a.Lrwp[q] = lws;
+            const double rk = RT[q], rk1 = RT[G(k + 1, i)];
+            const double thk = th_f(k);
+            a.Lrt[q] += Lb * dt_tp2 * thk / (z1 * dzd[k]);
+            a.Lrt[G(k + 1, i)] += Lb * dt_tp2 * thk / (z1 * dzd[k + 1]);
+            double Lthc_k = Lb * dt_tp2 * (rk / dzd[k] + rk1 / dzd[k + 1]) / z1 + Lthc_carry;
+            Lthc_carry = 0.0;
+            if (k > 0) {
+                a.Lrt[q] += La * (-dt_tp2 * th_f(k - 1) / (z1 * dzd[k]));
+                // th_(k-1): its own adjoint was completed at k-1 except this term
+                const double add = La * (-dt_tp2 * rk / (z1 * dzd[k]));
+                const double den = dzd[k - 1] + dzd[k];
+                a.Lth[G(k - 1, i)] += add * dzd[k] / den;
+                a.Lth[q] += add * dzd[k - 1] / den;
+            }
+            if (k < ke - 1) {
+                a.Lrt[G(k + 1, i)] += Lc * (-dt_tp2 * th_f(k + 1) / (z1 * dzd[k + 1]));
+                Lthc_carry = Lc * (-dt_tp2 * rk1 / (z1 * dzd[k + 1]));
+            }
+            {
+                const double den = dzd[k] + dzd[k + 1];
+                a.Lth[q] += Lthc_k * dzd[k + 1] / den;
+                a.Lth[G(k + 1, i)] += Lthc_k * dzd[k] / den;
+            }
+            // drwp = dt_s (R_rho_w - rt2h buoy + grav)
+            const double rtp_tp_k1 = rtp_tp(k + 1);
+            const double buoy = (rtp_tp_k1 - rtp_tp_k) / z1;
+            const double rt2h = (rk + rk1) * 0.5;
+            a.LRw[q] += dt_s * y;
+            a.Lrt[q] += -dt_s * y * buoy * 0.5;
+            a.Lrt[G(k + 1, i)] += -dt_s * y * buoy * 0.5;
+            Lrtp_tp += dt_s * y * rt2h / z1;
+            Lrp_tp += -dt_s * y * GRAV * 0.5;
+            rtp_tp_k = rtp_tp_k1;
+        }
+        if (k > 0) {
+            const int kb = k - 1;
+            const double z1 = zd[kb + 1] - zd[kb];
+            const double rt2h = (RT[G(kb, i)] + RT[q]) * 0.5;
+            Lrtp_tp += -dt_s * yprev * rt2h / z1;
+            Lrp_tp += -dt_s * yprev * GRAV * 0.5;
+        }
+        // explicit part: rp_tp = rp + f inc, rp += inc (the same for rtp)
+        const double lr = a.Lrp[q], lt = a.Lrtp[q];
+        const double lir = lr + fact_tp * Lrp_tp, lit = lt + fact_tp * Lrtp_tp;
+        a.Lrp[q] = lr + Lrp_tp;
+        a.Lrtp[q] = lt + Lrtp_tp;
+        a.Linc_r[q] = lir;
+        a.Linc_t[q] = lit;
+        a.LRr[q] += dt_s * lir;
+        a.LRt[q] += dt_s * lit;
+        yprev = y;
+    }
+}
+
+// ---- acoustic substep, horizontal part (moist.jl:1360-1408) -----------------------
+// Per cell.  Lrup_in: adjoint of rho_u' after this substep (from later
+// substeps and the final update); Lrup_out: adjoint before it.  pend: the
+// damping term's adjoint for rho_theta' at the start of the previous substep.
+__global__ void __launch_bounds__(AD_X *AD_Z)
+    ka_ac_hor(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, AdjW a, const double *__restrict__ Lrup_in,
+              double *__restrict__ Lrup_out, const double *__restrict__ rtp_s, const double *__restrict__ rtp_prev,
+              const double *__restrict__ rup_n, int div, int nsub, int first_substep) {
+    const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;
+    const Dom &D = doms[t.x];
+    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+    const int nx = D.nx, nz = D.nz;
+    if (i >= nx || k >= nz) return;
+    const double dx = D.dx, gam = D.gamma_d;
+    const double dt_s = D.dt / double(div) / double(nsub);
+    auto G = [&](int ii) -> int64_t { return D.off2 + (int64_t)k * nx + wrapn(ii, nx); };
+    const double *TH = a.theta, *RT = a.rt2;
+    // adjoint of rho_u'(j) after the substep, all contributions of this substep included
+    auto lrup = [&](int j) {
+        const int64_t q = G(j), qp = G(j + 1);
+        const double lF = dt_s / dx * (a.Linc_t[qp] - a.Linc_t[q]);
+        return Lrup_in[q] + a.Lmx[q] / double(nsub) - dt_s / dx * a.Linc_r[q] + dt_s / dx * a.Linc_r[qp] +
+               lF * 0.5 * (TH[q] + TH[qp]);
+    };
+    const int64_t c = G(i), cm = G(i - 1), cp = G(i + 1);
+    // theta adjoint of the rho_theta' flux (faces i+1/2 and i-1/2)
+    {
+        const double lFi = dt_s / dx * (a.Linc_t[cp] - a.Linc_t[c]);
+        const double lFm = dt_s / dx * (a.Linc_t[c] - a.Linc_t[cm]);
+        a.Lth[c] += 0.5 * (lFi * rup_n[c] + lFm * rup_n[cm]);
+    }
+    const double Li = lrup(i), Lm = lrup(i - 1);
+    // rho_theta' at the start of this substep: add the damping adjoint from the next substep
+    double lt = a.Lrtp[c] + a.pend[c];
+    // rho_u' update: rup_n = rup_s + dt_s (R_rho_u - rt2h ((1 + g) gx - g gxo))
+    auto Gx = [&](int j) {
+        const double gx = (rtp_s[G(j + 1)] - rtp_s[G(j)]) / dx;
+        const double gxo = (rtp_prev[G(j + 1)] - rtp_prev[G(j)]) / dx;
+        return gx + gam * (gx - gxo);
+    };
+    a.LRu[c] += dt_s * Li;
+    a.Lrt[c] += -0.5 * dt_s * (Li * Gx(i) + Lm * Gx(i - 1));
+    const double lGi = -dt_s * (RT[c] + RT[cp]) * 0.5 * Li;
+    const double lGm = -dt_s * (RT[cm] + RT[c]) * 0.5 * Lm;
+    lt += (1.0 + gam) / dx * (lGm - lGi);
+    const double lo = -gam / dx * (lGm - lGi);
+    if (first_substep) {
+        lt += lo;
+        a.pend[c] = 0.0;
+    } else {
+        a.pend[c] = lo;
+    }
+    a.Lrtp[c] = lt;
+    Lrup_out[c] = Li;
+}
+
+// ---- perturbation initialisation + slow tendencies (moist.jl:1071-1348) ----------
+// Gathers every contribution of R_rho, R_rho_u, R_rho_w, R_rho_theta to the
+// stage input (rho, rho_u, rho_w), to the face velocities (LU, LW), pres and
+// theta; adds the perturbation adjoints (rho' = s0 - s etc.) and the momentum
+// copies.
+__global__ void __launch_bounds__(AD_X *AD_Z)
+    ka_slow(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s, SSet<double> ls,
+            SSet<double> ls0, SSet<double> lph, AdjW a, const double *__restrict__ Lrup) {
+    const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;
+    const Dom &D = doms[t.x];
+    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+    const int nx = D.nx, nz = D.nz, ke = nz - 1;
+    if (i >= nx || k >= nz) return;
+    const double *zd = zv.z + D.offz, *zfd = zv.zf + D.offz, *dzd = zv.dz + D.offz;
+    const double dx = D.dx, EPS = 1e-12, GRAV = K<double>::GRAV, nu = D.nu;
+    auto G = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
+    const int64_t c = G(k, i);
+    const double *rho = s.f[RHO], *ru = s.f[RHOU], *rw = s.f[RHOW];
+    const double *TH = a.theta;
+    auto U = [&](int kk, int ii) {
+        const int64_t q = G(kk, ii);
+        return ru[q] * 2.0 / (rho[q] + rho[G(kk, ii + 1)]);
+    };
+    auto W = [&](int kk, int ii) {  // kk <= ke - 1
+        const int64_t q = G(kk, ii);
+        return rw[q] / f2h(rho[q], rho[G(kk + 1, ii)], dzd[kk], dzd[kk + 1]);
+    };
+    auto zz = [&](int kk) { return zd[kk + 1] - zd[kk]; };
+    double l_r = 0.0, l_u = 0.0, l_w = 0.0, LUc = 0.0, LWc = 0.0, lp = 0.0, lth = 0.0;
+
+    // perturbation init and momentum copies
+    {
+        const double Lrp = a.Lrp[c], Lrt = a.Lrtp[c], Lrw = k <= ke - 1 ? a.Lrwp[c] : 0.0, Lru = Lrup[c];
+        ls0.f[RHO][c] += Lrp;
+        ls0.f[RHOU][c] += Lru;
+        ls0.f[RT][c] += Lrt;
+        ls0.f[RHOW][c] += Lrw;
+        l_r -= Lrp;
+        l_u += -Lru + a.Lmx[c];
+        ls.f[RT][c] -= Lrt;
+        l_w += -Lrw + a.Lmz[c];
+    }
+    // physics tendencies enter R_rho, R_rho_u, R_rho_theta additively
+    lph.f[0][c] += a.LRr[c];
+    lph.f[1][c] += a.LRu[c];
+    lph.f[2][c] += a.LRt[c];
+
+    // ---- R_rho (moist.jl:1079-1086)
+    l_u += -a.LRr[c] / dx + a.LRr[G(k, i + 1)] / dx;
+    if (k < ke) l_w += -a.LRr[c] / dzd[k] + a.LRr[G(k + 1, i)] / dzd[k + 1];
+
+    // ---- R_rho_u (moist.jl:1098-1179)
+    // x flux F(j) between u-faces j and j+1, m = (rho_u_j + rho_u_j+1)/2
+    for (int j = i - 2; j <= i + 1; ++j) {
+        const double lF = (a.LRu[G(k, j + 1)] - a.LRu[G(k, j)]) / dx;
+        const double m = 0.5 * (ru[G(k, j)] + ru[G(k, j + 1)]);
+        const bool pos = m > EPS, neg = m < -EPS;
+        LUc += lF * m * upw_coef(pos, neg, i - j, 0.5, 0.5);
+        if (j == i - 1 || j == i) {
+            const double g = upw_val(pos, neg, U(k, j - 1), U(k, j), U(k, j + 1), U(k, j + 2), 0.5, 0.5);
+            l_u += lF * g * 0.5;
+        }
+    }
+    // z flux at corner (kk+1/2, face i), kk = k-2..k+1 within 0..ke-1
+    for (int kk = k - 2; kk <= k + 1; ++kk) {
+        if (kk < 0 || kk > ke - 1) continue;
+        const double lF = a.LRu[G(kk + 1, i)] / dzd[kk + 1] - a.LRu[G(kk, i)] / dzd[kk];
+        const double m = 0.5 * (rw[G(kk, i)] + rw[G(kk, i + 1)]);
+        const bool pos = m > EPS && kk > 0, neg = !pos && m < -EPS && kk < ke - 1;
+        const double den = dzd[kk] + dzd[kk + 1];
+        LUc += lF * m * upw_coef(pos, neg, k - kk, dzd[kk + 1] / den, dzd[kk] / den);
+    }
+    // rho_w at (k, i) enters the corner fluxes (k+1/2, i) and (k+1/2, i-1)
+    if (k <= ke - 1) {
+        for (int ii = i - 1; ii <= i; ++ii) {
+            const double lF = a.LRu[G(k + 1, ii)] / dzd[k + 1] - a.LRu[G(k, ii)] / dzd[k];
+            const double m = 0.5 * (rw[G(k, ii)] + rw[G(k, ii + 1)]);
+            const bool pos = m > EPS && k > 0, neg = !pos && m < -EPS && k < ke - 1;
+            const double den = dzd[k] + dzd[k + 1];
+            const double g = upw_val(pos, neg, k > 0 ? U(k - 1, ii) : 0.0, U(k, ii), U(k + 1, ii),
+                                     k + 2 <= ke ? U(k + 2, ii) : 0.0, dzd[k + 1] / den, dzd[k] / den);
+            l_w += lF * g * 0.5;
+        }
+    }
+    // viscous x flux Dx(j) = (U_j+1 - U_j) / dx * rho_j+1, j = i-1, i
+    for (int j = i - 1; j <= i; ++j) {
+        const double lD = nu * (a.LRu[G(k, j)] - a.LRu[G(k, j + 1)]) / dx;
+        LUc += lD * rho[G(k, j + 1)] / dx * (j == i - 1 ? 1.0 : -1.0);
+        if (j == i - 1) l_r += lD * (U(k, i) - U(k, i - 1)) / dx;
+    }
+    // viscous z flux Dz(kk) = (U_kk+1 - U_kk) / zz * rho at the corner, kk = k-1, k
+    for (int kk = k - 1; kk <= k; ++kk) {
+        if (kk < 0 || kk > ke - 1) continue;
+        const double lD = nu * (a.LRu[G(kk, i)] / dzd[kk] - a.LRu[G(kk + 1, i)] / dzd[kk + 1]);
+        const double rc = 0.25 * (rho[G(kk, i)] + rho[G(kk + 1, i)] + rho[G(kk, i + 1)] + rho[G(kk + 1, i + 1)]);
+        LUc += lD * rc / zz(kk) * (kk == k - 1 ? 1.0 : -1.0);
+    }
+    // rho(k, i) in the corner densities of the z viscous fluxes (kk, ii), kk = k-1, k; ii = i-1, i
+    for (int kk = k - 1; kk <= k; ++kk) {
+        if (kk < 0 || kk > ke - 1) continue;
+        for (int ii = i - 1; ii <= i; ++ii) {
+            const double lD = nu * (a.LRu[G(kk, ii)] / dzd[kk] - a.LRu[G(kk + 1, ii)] / dzd[kk + 1]);
+            l_r += lD * (U(kk + 1, ii) - U(kk, ii)) / zz(kk) * 0.25;
+        }
+    }
+    // pressure gradient (p_i+1 - p_i) / dx
+    lp += a.LRu[c] / dx - a.LRu[G(k, i - 1)] / dx;
+
+    // ---- R_rho_w (moist.jl:1190-1279), cells k' <= ke - 1
+    const double sponge_base = (1.0 - D.zf_sponge) * D.H;
+    auto LRw = [&](int kk, int ii) { return a.LRw[G(kk, ii)]; };
+    if (k <= ke - 1) {
+        // fip(k, j): m = f2h(rho_u(k,j), rho_u(k+1,j), z_k, z_k+1), reads W(j-1..j+2)
+        const double zs = zd[k] + zd[k + 1];
+        for (int j = i - 2; j <= i + 1; ++j) {
+            const double lF = -LRw(k, j) / dx;
+            const double m = (ru[G(k, j)] * zd[k + 1] + ru[G(k + 1, j)] * zd[k]) / zs;
+            LWc += lF * m * upw_coef(m > EPS, m < -EPS, i - j, 0.5, 0.5);
+        }
+        // fim(k, j): m = f2h(rho_u(k,j-1), rho_u(k+1,j-1), dz), reads W(j-2..j+1): interface between j-1 and j
+        const double ds = dzd[k] + dzd[k + 1];
+        for (int j = i - 1; j <= i + 2; ++j) {
+            const double lF = LRw(k, j) / dx;
+            const double m = (ru[G(k, j - 1)] * dzd[k + 1] + ru[G(k + 1, j - 1)] * dzd[k]) / ds;
+            LWc += lF * m * upw_coef(m > EPS, m < -EPS, i - (j - 1), 0.5, 0.5);
+        }
+        // viscous x flux Ex(j) = (W_j+1 - W_j) / dx * rho at corner (k+1/2, j+1/2), j = i-1, i
+        for (int j = i - 1; j <= i; ++j) {
+            const double lE = nu * (LRw(k, j) - LRw(k, j + 1)) / dx;
+            const double rc = 0.25 * (rho[G(k, j)] + rho[G(k + 1, j)] + rho[G(k, j + 1)] + rho[G(k + 1, j + 1)]);
+            LWc += lE * rc / dx * (j == i - 1 ? 1.0 : -1.0);
+        }
+        // sponge
+        double sigma = 0.0;
+        if (zfd[k] > sponge_base) sigma = (GRAV / 30.0) * ((zfd[k] - sponge_base) / (D.zf_sponge * D.H));
+        l_w += -sigma * LRw(k, i);
+    }
+    // z flux Fz(kk) between w levels kk and kk+1 (kk < ke - 1), shared by cells kk and kk+1
+    for (int kk = k - 2; kk <= k + 1; ++kk) {
+        if (kk < 0 || kk >= ke - 1) continue;
+        const double lF = LRw(kk + 1, i) / zz(kk + 1) - LRw(kk, i) / zz(kk);
+        const double ds = dzd[kk] + dzd[kk + 1];
+        const double m = (rw[G(kk, i)] * dzd[kk + 1] + rw[G(kk + 1, i)] * dzd[kk]) / ds;
+        const bool pos = m > EPS && kk > 0, neg = !pos && m < -EPS && kk < ke - 2;
+        if (k <= ke - 1) LWc += lF * m * upw_coef(pos, neg, k - kk, dzd[kk + 1] / ds, dzd[kk] / ds);
+        if (kk == k || kk == k - 1) {
+            const double g = upw_val(pos, neg, kk > 0 ? W(kk - 1, i) : 0.0, W(kk, i), W(kk + 1, i),
+                                     kk + 2 <= ke - 1 ? W(kk + 2, i) : 0.0, dzd[kk + 1] / ds, dzd[kk] / ds);
+            l_w += lF * g * (kk == k ? dzd[kk + 1] : dzd[kk]) / ds;
+        }
+    }
+    // viscous z flux Ez(kk) = (W_kk+1 - W_kk) / dz_kk+1 * rho_kk+1, kk = k-1, k (kk < ke - 1)
+    for (int kk = k - 1; kk <= k; ++kk) {
+        if (kk < 0 || kk >= ke - 1) continue;
+        const double lE = nu * (LRw(kk, i) / zz(kk) - LRw(kk + 1, i) / zz(kk + 1));
+        if (k <= ke - 1) LWc += lE * rho[G(kk + 1, i)] / dzd[kk + 1] * (kk == k - 1 ? 1.0 : -1.0);
+        if (kk == k - 1) l_r += lE * (W(k, i) - W(k - 1, i)) / dzd[k];
+    }
+    // rho(k, i) in the corner densities of the x viscous fluxes Ex(k', j), k' = k-1, k; j = i-1, i
+    for (int kp = k - 1; kp <= k; ++kp) {
+        if (kp < 0 || kp > ke - 1) continue;
+        for (int j = i - 1; j <= i; ++j) {
+            const double lE = nu * (LRw(kp, j) - LRw(kp, j + 1)) / dx;
+            l_r += lE * (W(kp, j + 1) - W(kp, j)) / dx * 0.25;
+        }
+    }
+    // rho_u(k, i) in the R_rho_w x fluxes: fip(k', i) and fim(k', i+1), k' = k-1, k
+    for (int kp = k - 1; kp <= k; ++kp) {
+        if (kp < 0 || kp > ke - 1) continue;
+        {
+            const double lF = -LRw(kp, i) / dx;
+            const double zs = zd[kp] + zd[kp + 1];
+            const double m = (ru[G(kp, i)] * zd[kp + 1] + ru[G(kp + 1, i)] * zd[kp]) / zs;
+            const double g = upw_val(m > EPS, m < -EPS, W(kp, i - 1), W(kp, i), W(kp, i + 1), W(kp, i + 2), 0.5, 0.5);
+            l_u += lF * g * (kp == k ? zd[kp + 1] : zd[kp]) / zs;
+        }
+        {
+            const double lF = LRw(kp, i + 1) / dx;
+            const double ds = dzd[kp] + dzd[kp + 1];
+            const double m = (ru[G(kp, i)] * dzd[kp + 1] + ru[G(kp + 1, i)] * dzd[kp]) / ds;
+            const double g = upw_val(m > EPS, m < -EPS, W(kp, i - 1), W(kp, i), W(kp, i + 1), W(kp, i + 2), 0.5, 0.5);
+            l_u += lF * g * (kp == k ? dzd[kp + 1] : dzd[kp]) / ds;
+        }
+    }
+    // pressure gradient (p_k+1 - p_k) / zz and buoyancy -g (rho_k + rho_k+1) / 2
+    if (k <= ke - 1) {
+        lp += LRw(k, i) / zz(k);
+        l_r += -GRAV * 0.5 * LRw(k, i);
+    }
+    if (k >= 1 && k - 1 <= ke - 1) {
+        lp += -LRw(k - 1, i) / zz(k - 1);
+        l_r += -GRAV * 0.5 * LRw(k - 1, i);
+    }
+
+    // ---- R_rho_theta (moist.jl:1284-1332)
+    for (int j = i - 2; j <= i + 1; ++j) {
+        const double lF = (a.LRt[G(k, j + 1)] - a.LRt[G(k, j)]) / dx;
+        const double m = ru[G(k, j)];
+        const bool pos = m > EPS, neg = m < -EPS;
+        lth += lF * m * upw_coef(pos, neg, i - j, 0.5, 0.5);
+        if (j == i) l_u += lF * upw_val(pos, neg, TH[G(k, i - 1)], TH[c], TH[G(k, i + 1)], TH[G(k, i + 2)], 0.5, 0.5);
+    }
+    for (int kk = k - 2; kk <= k + 1; ++kk) {
+        if (kk < 0 || kk > ke - 1) continue;
+        const double lF = a.LRt[G(kk + 1, i)] / dzd[kk + 1] - a.LRt[G(kk, i)] / dzd[kk];
+        const double m = rw[G(kk, i)];
+        const bool pos = m > EPS && kk > 0, neg = !pos && m < -EPS && kk < ke - 1;
+        const double den = dzd[kk] + dzd[kk + 1];
+        lth += lF * m * upw_coef(pos, neg, k - kk, dzd[kk + 1] / den, dzd[kk] / den);
+        if (kk == k)
+            l_w += lF * upw_val(pos, neg, kk > 0 ? TH[G(kk - 1, i)] : 0.0, TH[G(kk, i)], TH[G(kk + 1, i)],
+                                kk + 2 <= ke ? TH[G(kk + 2, i)] : 0.0, dzd[kk + 1] / den, dzd[kk] / den);
+    }
+
+    ls.f[RHO][c] += l_r;
+    ls.f[RHOU][c] += l_u;
+    ls.f[RHOW][c] += l_w;
+    a.LU[c] = LUc;
+    a.LW[c] = k <= ke - 1 ? LWc : 0.0;
+    a.Lpres[c] = lp;
+    a.Lth[c] += lth;
+}
+
+// ---- face velocities and stage diagnostics (moist.jl:1044-1069, 1091-1097, 1184-1189)
+__global__ void __launch_bounds__(AD_X *AD_Z)
+    ka_diag(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s, SSet<double> ls,
+            AdjW a) {
+    const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;
+    const Dom &D = doms[t.x];
+    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+    const int nx = D.nx, nz = D.nz, ke = nz - 1;
+    if (i >= nx || k >= nz) return;
+    const double *dzd = zv.dz + D.offz;
+    auto G = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
+    const int64_t c = G(k, i), cm = G(k, i - 1), cp = G(k, i + 1);
+    const double *rho = s.f[RHO], *ru = s.f[RHOU], *rw = s.f[RHOW];
+    double l_r = 0.0, l_u = 0.0, l_w = 0.0;
+    // U(j) = 2 rho_u(j) / (rho_j + rho_j+1)
+    {
+        const double den = rho[c] + rho[cp];
+        const double Uc = ru[c] * 2.0 / den;
+        l_u += a.LU[c] * 2.0 / den;
+        l_r += -a.LU[c] * Uc / den;
+        const double denm = rho[cm] + rho[c];
+        l_r += -a.LU[cm] * (ru[cm] * 2.0 / denm) / denm;
+    }
+    // W(kk) = rho_w(kk) / f2h(rho_kk, rho_kk+1)
+    if (k <= ke - 1) {
+        const double ds = dzd[k] + dzd[k + 1];
+        const double rf = (rho[c] * dzd[k + 1] + rho[G(k + 1, i)] * dzd[k]) / ds;
+        const double Wc = rw[c] / rf;
+        l_w += a.LW[c] / rf;
+        l_r += -a.LW[c] * Wc / rf * dzd[k + 1] / ds;
+    }
+    if (k >= 1) {
+        const int kb = k - 1;
+        const double ds = dzd[kb] + dzd[k];
+        const double rf = (rho[G(kb, i)] * dzd[k] + rho[c] * dzd[kb]) / ds;
+        const double Wb = rw[G(kb, i)] / rf;
+        l_r += -a.LW[G(kb, i)] * Wb / rf * dzd[kb] / ds;
+    }
+    // diagnostics pres, theta, rt2pres
+    typedef Dl<5> V;
+    const K<double> C;
+    V rr = V::var(rho[c], 0), rt = V::var(s.f[RT][c], 1), rqv = V::var(s.f[QV][c], 2), rqc = V::var(s.f[QC][c], 3),
+      rqr = V::var(s.f[QR][c], 4);
+    V qv = rqv / rr, qc = rqc / rr, qr = rqr / rr;
+    V qd = V(1.0) - qv - qc - qr;
+    V r = C.Rd * qd + C.Rv * qv;
+    V cp_ = C.Cpd * qd + C.Cpv * qv + C.Cpl * (qc + qr);
+    V cv = cp_ - r;
+    V pres = C.P0 * d_pow(r * rt / C.P0, cp_ / cv);
+    V theta = rt / rr;
+    V rt2 = cp_ / cv * pres / rt;
+    const double lp = a.Lpres[c], lt = a.Lth[c], lr2 = a.Lrt[c];
+    double g[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) g[j] = lp * pres.d[j] + lt * theta.d[j] + lr2 * rt2.d[j];
+    ls.f[RHO][c] += l_r + g[0];
+    ls.f[RHOU][c] += l_u;
+    ls.f[RHOW][c] += l_w;
+    ls.f[RT][c] += g[1];
+    ls.f[QV][c] += g[2];
+    ls.f[QC][c] += g[3];
+    ls.f[QR][c] += g[4];
+}
+
+// ---- Kessler microphysics of one cell with dual numbers (moist.jl:554-671) -------
+template <class V>
+__device__ __forceinline__ void kessler_v(const Dom &D, V rho_k, V rt, V rqv, V rqc, V rqr, V Ka, V Qc0, V Kacc,
+                                          V out[4]) {
+    const K<double> c;
+    const double dt = D.dt;
+    const bool do_auto = D.flags & F_AUTO, do_accr = D.flags & F_ACCR, do_revap = D.flags & F_REVAP;
+    V dqv(0.0), dqc(0.0), dqr(0.0), dth(0.0);
+    V rinv = V(1.0) / rho_k;
+    V theta_k = rt * rinv;
+    V qv = rqv * rinv, qc = rqc * rinv, qr = rqr * rinv;
+    V qd = V(1.0) - qv - qc - qr;
+    V r = c.Rd * qd + c.Rv * qv;
+    V cp = c.Cpd * qd + c.Cpv * qv + c.Cpl * (qc + qr);
+    V cv = cp - r;
+    V Pi = d_pow(r * rt / c.P0, r / cv);
+    V Tk = theta_k * Pi;
+    V pk = c.P0 * d_pow(Pi, cp / r);
+    V lhe = c.Lv / (cp * Pi);
+    V qv_init = qv;
+    V ce(0.0), qsat(0.0);
+    for (int it = 0; it < 3; ++it) {
+        V T_c = Tk - c.T0;
+        V tmp = c.B_SAT + T_c;
+        V es = c.ES0 * d_exp(c.A_SAT * T_c / tmp);
+        V des = es * c.A_SAT * (tmp - T_c) / (tmp * tmp);
+        V tq = pk - (1.0 - c.EPSvap) * es;
+        V dqs_des;
+        if (val(tq) > c.EPS) {
+            qsat = c.EPSvap * es / tq;
+            dqs_des = c.EPSvap * pk / (tq * tq);
+        } else {
+            qsat = V(0.0);
+            dqs_des = V(1.0);
+        }
+        V dqs_dT = dqs_des * des;
+        ce = (qv - qsat) / (V(1.0) + (c.Lv / cp) * dqs_dT);
+        ce = vmin(ce, qv);
+        ce = vmax(ce, -qc);
+        qv = qv - ce;
+        qc = qc + ce;
+        theta_k = theta_k + ce * lhe;
+        r = c.Rd * qd + c.Rv * qv;
+        cp = c.Cpd * qd + c.Cpv * qv + c.Cpl * (qc + qr);
+        cv = cp - r;
+        Pi = d_pow(r * rho_k * theta_k / c.P0, r / cv);
+        Tk = theta_k * Pi;
+        pk = c.P0 * d_pow(Pi, cp / r);
+        lhe = c.Lv / (cp * Pi);
+    }
+    ce = qv_init - qv;
+    V rate = ce * rho_k / dt;
+    dqv = dqv - rate;
+    dqc = dqc + rate;
+    dth = dth + rate * lhe;
+    if (do_auto) {
+        V ra = Ka * vmax(V(0.0), qc - Qc0);
+        ra = vmin(ra, qc / dt);
+        qc = qc - ra * dt;
+        qr = qr + ra * dt;
+        ra = ra * rho_k;
+        dqc = dqc - ra;
+        dqr = dqr + ra;
+    }
+    if (do_accr) {
+        V rc(0.0);
+        if (val(qr) > 1e-12 && val(qc) > 1e-12) {
+            rc = Kacc * qc * qr;
+            rc = vmin(rc, qc / dt);
+        }
+        qc = qc - rc * dt;
+        qr = qr + rc * dt;
+        rc = rc * rho_k;
+        dqc = dqc - rc;
+        dqr = dqr + rc;
+    }
+    if (do_revap) {
+        V re(0.0);
+        if (val(qr) > 1e-12 && val(qv) < val(qsat)) {
+            re = c.K_evap * (qsat - qv) * d_sqrt(rho_k * qr);
+            re = vmin(re, qr) / dt;
+        }
+        re = re * rho_k;
+        dqr = dqr - re;
+        dqv = dqv + re;
+        r = c.Rd * qd + c.Rv * qv;
+        cp = c.Cpd * qd + c.Cpv * qv + c.Cpl * (qc + qr);
+        cv = cp - r;
+        Pi = d_pow(r * rho_k * theta_k / c.P0, r / cv);
+        lhe = c.Lv / (cp * Pi);
+        dth = dth - re * lhe;
+    }
+    out[0] = dqv;
+    out[1] = dqc;
+    out[2] = dqr;
+    out[3] = dth;
+}
+
+// ---- physics adjoint (moist.jl:554-707, 1632-1644), do_sf = do_rd = false -------
+// lph: adjoint of the six tendencies; ldprec: adjoint of d_prec (dx * dt).
+// Adds into ls; accumulates dJ/dK per cell into gk (3 fields).
+__global__ void __launch_bounds__(AD_X *AD_Z)
+    ka_physics(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s,
+               SSet<double> lph, SSet<double> ls, double *gk0, double *gk1, double *gk2) {
+    const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;
+    const Dom &D = doms[t.x];
+    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+    const int nx = D.nx, nz = D.nz, ke = nz - 1;
+    if (i >= nx || k >= nz) return;
+    const double *dzd = zv.dz + D.offz;
+    const K<double> C;
+    auto G = [&](int kk) -> int64_t { return D.off2 + (int64_t)kk * nx + i; };
+    const int64_t c = G(k);
+    const double *rho = s.f[RHO], *rt = s.f[RT], *rqr = s.f[QR];
+    const bool mp = D.flags & F_MP, sed = mp && (D.flags & F_SED);
+    auto vt = [&](int kk) {
+        const int64_t q = G(kk);
+        const double eff = jmax(0.0, rqr[q]);
+        const double v = C.A_term * pow(eff / rho[q], C.B_term);
+        return eff > 1e-12 ? v : 0.0;
+    };
+    auto flux = [&](int kk) { return rqr[G(kk)] * vt(kk); };
+    double sedc = 0.0;
+    if (sed) sedc = ((k < ke ? flux(k + 1) : 0.0) - flux(k)) / dzd[k];
+    double l_r = 0.0, l_rt = 0.0, l_qr = 0.0;
+    // d_rho_theta = d_th_mp + d_rho rho_theta / rho with d_rho = d_qr_sed
+    l_rt += lph.f[2][c] * sedc / rho[c];
+    l_r += -lph.f[2][c] * sedc * rt[c] / (rho[c] * rho[c]);
+    if (mp) {
+        typedef Dl<8> V;
+        V out[4];
+        kessler_v<V>(D, V::var(rho[c], 0), V::var(rt[c], 1), V::var(s.f[QV][c], 2), V::var(s.f[QC][c], 3),
+                     V::var(rqr[c], 4), V::var(D.K_auto, 5), V::var(D.Qc_threshold, 6), V::var(D.K_accr, 7), out);
+        const double lo[4] = {lph.f[3][c], lph.f[4][c], lph.f[5][c], lph.f[2][c]};
+        double g[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            g[j] = 0.0;
+#pragma unroll
+            for (int o = 0; o < 4; ++o) g[j] += lo[o] * out[o].d[j];
+        }
+        l_r += g[0];
+        l_rt += g[1];
+        ls.f[QV][c] += g[2];
+        ls.f[QC][c] += g[3];
+        l_qr += g[4];
+        gk0[c] += g[5];
+        gk1[c] += g[6];
+        gk2[c] += g[7];
+    }
+    if (sed) {
+        // F(k) = rho_qr Vt; d_sed(k) = (F(k+1) [k<ke] - F(k)) / dz_k; d_prec = F(0)
+        auto lsed = [&](int kk) {
+            const int64_t q = G(kk);
+            return lph.f[0][q] + lph.f[5][q] + lph.f[2][q] * rt[q] / rho[q];
+        };
+        double lF = -lsed(k) / dzd[k];
+        if (k > 0) lF += lsed(k - 1) / dzd[k - 1];
+        if (k == 0) lF += D.dx * D.dt;  // the rain term of the loss: dJ/dd_prec = dx dt
+        const double eff = jmax(0.0, rqr[c]);
+        if (eff > 1e-12) {
+            const double v = vt(k);
+            l_qr += lF * v * (1.0 + C.B_term);
+            l_r += -lF * C.B_term * rqr[c] * v / rho[c];
+        }
+    }
+    ls.f[RHO][c] += l_r;
+    ls.f[RT][c] += l_rt;
+    ls.f[QR][c] += l_qr;
+}
+
+// Per-member sums of the parameter gradients (one block per member).
+__global__ void __launch_bounds__(256) ka_reduce_k(const Dom *__restrict__ doms, const double *gk0, const double *gk1,
+                                                   const double *gk2, double *out) {
+    const Dom &D = doms[blockIdx.x];
+    const int64_t n = (int64_t)D.nx * D.nz;
+    __shared__ double r[3][256];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int64_t e = threadIdx.x; e < n; e += blockDim.x) {
+        a0 += gk0[D.off2 + e];
+        a1 += gk1[D.off2 + e];
+        a2 += gk2[D.off2 + e];
+    }
+    r[0][threadIdx.x] = a0;
+    r[1][threadIdx.x] = a1;
+    r[2][threadIdx.x] = a2;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int j = 0; j < 3; ++j) r[j][threadIdx.x] += r[j][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int j = 0; j < 3; ++j) out[3 * blockIdx.x + j] = r[j][0];
+}
+
+// y += x over n doubles
+__global__ void ka_axpy(double *y, const double *x, int64_t n) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        y[q] += x[q];
+}

@@ -26,6 +26,9 @@
 
 #include "../../include/moist2d.h"
 #include "moist_kernels.h"
+namespace mo {
+#include "moist_adjoint.h"
+}
 
 using namespace mo;
 
@@ -58,6 +61,8 @@ template <class T> std::vector<T> xcd_order(const std::vector<T> &v, T pad) {
 }
 
 }  // namespace
+
+struct AdjState;
 
 struct moist_handle {
     int precision = MOIST_FP64;
@@ -105,6 +110,8 @@ struct moist_handle {
     int ev_used = 0;
     double ac_ms = 0.0;
     int64_t ac_launches = 0;
+    // reverse mode (allocated by the first moist_adjoint_run)
+    AdjState *adj = nullptr;
 };
 
 namespace {
@@ -490,6 +497,309 @@ int compute_step_t(moist_handle *h, int m, const void *const phys[6], const void
     return 0;
 }
 
+// ---- reverse mode (FP64 form) ------------------------------------------------------
+// Buffers of one adjoint run: checkpoints every `ckpt` steps, the states of
+// one segment (recomputed from its checkpoint), the forward record of one
+// step (stage diagnostics and tendencies, per-substep rho_theta', rho_u',
+// rho_w'), and the adjoint fields.
+enum AdjWork {
+    W_TH = 0, W_RT, W_PRES, W_LU, W_LW, W_LRR, W_LRU, W_LRW, W_LRT, W_LMX, W_LMZ, W_LRP, W_LRWP, W_LRTP, W_PEND,
+    W_LINCR, W_LINCT, W_Y, W_CPR, W_RMI, W_LRUP0, W_LRUP1, NWORK
+};
+enum { REC_TH = 0, REC_PRES, REC_RT2, REC_RR, REC_RRU, REC_RRW, REC_RRT, REC_MX, REC_MZ, NREC };
+
+}  // namespace
+
+struct AdjState {
+    int nsteps = 0, ckpt = 0, ncp = 0;
+    size_t cp_stride = 0;
+    double *cp = nullptr, *seg = nullptr, *SC = nullptr, *rst = nullptr, *rsub = nullptr;
+    double *lam = nullptr, *lam2 = nullptr, *ls0 = nullptr, *lph = nullptr, *wk = nullptr, *lim = nullptr;
+    double *gk = nullptr, *tgt = nullptr, *fin = nullptr, *J = nullptr, *gK = nullptr;
+    std::vector<double *> allocs;
+    std::vector<double> J_h, gK_h;
+    double ms_fwd = 0.0, ms_bwd = 0.0;
+    int sub_off[3] = {0, 0, 0};  // first substep record of each stage
+};
+
+namespace {
+
+void adj_free(moist_handle *h) {
+    if (!h->adj) return;
+    for (double *p : h->adj->allocs)
+        if (p) hipFree(p);
+    delete h->adj;
+    h->adj = nullptr;
+}
+
+int adj_alloc(moist_handle *h, int nsteps, int ckpt) {
+    if (h->adj && h->adj->nsteps == nsteps && h->adj->ckpt == ckpt) return 0;
+    adj_free(h);
+    h->adj = new AdjState();
+    AdjState &A = *h->adj;
+    A.nsteps = nsteps;
+    A.ckpt = ckpt;
+    A.ncp = (nsteps + ckpt - 1) / ckpt;
+    const size_t n2 = h->n2;
+    A.cp_stride = 7 * n2 + h->nxs;
+    int nrec_sub = 0;
+    for (int st = 0; st < 3; ++st) {
+        A.sub_off[st] = nrec_sub;
+        nrec_sub += h->ns_stage[st] + 1;
+    }
+    auto al = [&](double *&p, size_t n) -> int {
+        HIPCHK(h, hipMalloc((void **)&p, n * sizeof(double)));
+        A.allocs.push_back(p);
+        return 0;
+    };
+    if (al(A.cp, A.cp_stride * A.ncp) || al(A.seg, 7 * n2 * (size_t)ckpt) || al(A.SC, 7 * n2) ||
+        al(A.rst, 3 * NREC * n2) || al(A.rsub, 3 * n2 * (size_t)nrec_sub) || al(A.lam, 7 * n2) ||
+        al(A.lam2, 7 * n2) || al(A.ls0, 7 * n2) || al(A.lph, 7 * n2) || al(A.wk, NWORK * n2) ||
+        al(A.lim, 6 * h->nxs) || al(A.gk, 3 * n2) || al(A.tgt, n2) || al(A.fin, A.cp_stride) || al(A.J, h->nm) ||
+        al(A.gK, 3 * h->nm))
+        return -1;
+    A.J_h.assign(h->nm, 0.0);
+    A.gK_h.assign(3 * h->nm, 0.0);
+    return 0;
+}
+
+SSet<double> dset(double *base, size_t n2) {
+    SSet<double> s;
+    for (int f = 0; f < 7; ++f) s.f[f] = base + f * n2;
+    return s;
+}
+
+// substep records of stage st: rho_theta', rho_u', rho_w' after substep n (n = 0: initial)
+double *rec_rtp(moist_handle *h, int st, int n) { return h->adj->rsub + ((size_t)3 * (h->adj->sub_off[st] + n) + 0) * h->n2; }
+double *rec_rup(moist_handle *h, int st, int n) { return h->adj->rsub + ((size_t)3 * (h->adj->sub_off[st] + n) + 1) * h->n2; }
+double *rec_rwp(moist_handle *h, int st, int n) { return h->adj->rsub + ((size_t)3 * (h->adj->sub_off[st] + n) + 2) * h->n2; }
+double *rec_st(moist_handle *h, int st, int r) { return h->adj->rst + ((size_t)st * NREC + r) * h->n2; }
+double *wk(moist_handle *h, int w) { return h->adj->wk + (size_t)w * h->n2; }
+
+AdjW adj_w(moist_handle *h, int st) {
+    AdjW a;
+    a.theta = rec_st(h, st, REC_TH);
+    a.pres = rec_st(h, st, REC_PRES);
+    a.rt2 = rec_st(h, st, REC_RT2);
+    a.R_rho = rec_st(h, st, REC_RR);
+    a.R_rhou = rec_st(h, st, REC_RRU);
+    a.R_rhow = rec_st(h, st, REC_RRW);
+    a.R_rt = rec_st(h, st, REC_RRT);
+    a.mom_x = rec_st(h, st, REC_MX);
+    a.mom_z = rec_st(h, st, REC_MZ);
+    a.Lth = wk(h, W_TH);
+    a.Lrt = wk(h, W_RT);
+    a.Lpres = wk(h, W_PRES);
+    a.LU = wk(h, W_LU);
+    a.LW = wk(h, W_LW);
+    a.LRr = wk(h, W_LRR);
+    a.LRu = wk(h, W_LRU);
+    a.LRw = wk(h, W_LRW);
+    a.LRt = wk(h, W_LRT);
+    a.Lmx = wk(h, W_LMX);
+    a.Lmz = wk(h, W_LMZ);
+    a.Lrp = wk(h, W_LRP);
+    a.Lrwp = wk(h, W_LRWP);
+    a.Lrtp = wk(h, W_LRTP);
+    a.pend = wk(h, W_PEND);
+    a.Linc_r = wk(h, W_LINCR);
+    a.Linc_t = wk(h, W_LINCT);
+    a.Y = wk(h, W_Y);
+    a.cpr = wk(h, W_CPR);
+    a.rmi = wk(h, W_RMI);
+    a.linb = h->adj->lim;
+    a.lint = h->adj->lim + 3 * h->nxs;
+    return a;
+}
+
+void fill(moist_handle *h, double *p, int64_t n, double v = 0.0) {
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(ka_fill, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0, h->stream, p, n, v);
+}
+
+void copy_set(moist_handle *h, double *dst, const double *src, size_t n) {
+    hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream);
+}
+
+// One stage of the forward pass with every intermediate the adjoint reads
+// kept in the stage record (same kernels as enqueue_stage).
+void enqueue_stage_rec(moist_handle *h, int st, SSet<double> in, SSet<double> out, SSet<double> s0, int div) {
+    const int nsub = h->ns_stage[st];
+    const dim3 b2(TILE_X, TILE_Z);
+    const ZVec zv = zvec(h);
+    Work<double> w = work<double>(h);
+    w.theta = rec_st(h, st, REC_TH);
+    w.pres = rec_st(h, st, REC_PRES);
+    w.rt2 = rec_st(h, st, REC_RT2);
+    w.R_rho = rec_st(h, st, REC_RR);
+    w.R_rhou = rec_st(h, st, REC_RRU);
+    w.R_rhow = rec_st(h, st, REC_RRW);
+    w.R_rt = rec_st(h, st, REC_RRT);
+    w.mom_x = rec_st(h, st, REC_MX);
+    w.mom_z = rec_st(h, st, REC_MZ);
+    w.rup = rec_rup(h, st, 0);
+    hipLaunchKernelGGL(k_diag<double>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, w);
+    hipLaunchKernelGGL(k_slow<double>, dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles, zv, in,
+                       s0, w, rec_rtp(h, st, 0), 0);
+    copy_set(h, rec_rwp(h, st, 0), w.rwp, h->n2);
+    for (int n = 0; n < nsub; ++n) {
+        AcBuf<double> ab;
+        ab.rup_in = rec_rup(h, st, n);
+        ab.rup_out = rec_rup(h, st, n + 1);
+        ab.rtp_cur = rec_rtp(h, st, n);
+        ab.rtp_old = rec_rtp(h, st, n > 0 ? n - 1 : 0);
+        ab.rtp_out = rec_rtp(h, st, n + 1);
+        ab.cin = h->cin;
+        ab.cin_stride = h->nxs;
+        hipLaunchKernelGGL((k_acoustic<double, ac_levels<double>()>), dim3(h->natiles), dim3(AC_THREADS), 0,
+                           h->stream, h->dom_d, h->atiles, zv, in, w, ab, div, 0.0, nsub, n == 0 ? 1 : 0);
+        copy_set(h, rec_rwp(h, st, n + 1), w.rwp, h->n2);
+    }
+    hipLaunchKernelGGL(k_scalar<double>, dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles, zv, in,
+                       s0, out, w, (const double *)rec_rup(h, st, nsub), (const double *)rec_rtp(h, st, nsub), div, 0.0,
+                       st);
+}
+
+// Forward pass of one step from the state `sn` with its record: the stage
+// inputs stay in S0 (= sn), SA, SB; stage 3 writes the separate set SC.
+void enqueue_step_rec(moist_handle *h, const double *sn) {
+    const size_t n2 = h->n2;
+    for (int f = 0; f < 7; ++f) copy_set(h, slot<double>(h, S0_ + f), sn + f * n2, n2);
+    const ZVec zv = zvec(h);
+    const Work<double> w = work<double>(h);
+    hipLaunchKernelGGL(k_physics<double>, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d,
+                       h->tiles2d, zv, sset<double>(h, S0_), w, 0);
+    enqueue_stage_rec(h, 0, sset<double>(h, S0_), sset<double>(h, SA_), sset<double>(h, S0_), 3);
+    enqueue_stage_rec(h, 1, sset<double>(h, SA_), sset<double>(h, SB_), sset<double>(h, S0_), 2);
+    enqueue_stage_rec(h, 2, sset<double>(h, SB_), dset(h->adj->SC, n2), sset<double>(h, S0_), 1);
+}
+
+// Adjoint of one step: A.lam holds the adjoint of the step output on entry
+// and of the step input `sn` on exit.
+void enqueue_step_adj(moist_handle *h, const double *sn) {
+    AdjState &A = *h->adj;
+    const size_t n2 = h->n2;
+    const ZVec zv = zvec(h);
+    const dim3 b2(TILE_X, TILE_Z);
+    enqueue_step_rec(h, sn);
+    fill(h, A.ls0, 7 * n2);
+    fill(h, A.lph, 7 * n2);
+    const SSet<double> ls0 = dset(A.ls0, n2), lph = dset(A.lph, n2);
+    const int ins[3] = {S0_, SA_, SB_};
+    const int divs[3] = {3, 2, 1};
+    double *lo = A.lam, *li = A.lam2;
+    for (int st = 2; st >= 0; --st) {
+        const SSet<double> in = sset<double>(h, ins[st]);
+        const int nsub = h->ns_stage[st], div = divs[st];
+        const AdjW a = adj_w(h, st);
+        for (int wi : {W_TH, W_RT, W_LRR, W_LRU, W_LRW, W_LRT, W_PEND}) fill(h, wk(h, wi), n2);
+        double *Lrup[2] = {wk(h, W_LRUP0), wk(h, W_LRUP1)};
+        hipLaunchKernelGGL(ka_scalar, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, dset(lo, n2),
+                           dset(li, n2), ls0, lph, a, Lrup[0], div);
+        hipLaunchKernelGGL(ka_lin, dim3(h->nm), dim3(256), 0, h->stream, h->dom_d, a, dset(li, n2));
+        int cur = 0;
+        for (int n = nsub; n >= 1; --n) {
+            const int m = n - 1;
+            hipLaunchKernelGGL(ka_ac_col, dim3(h->nctiles), dim3(COL_TILE), 0, h->stream, h->dom_d, h->ctiles, zv, in,
+                               a, (const double *)rec_rtp(h, st, m), (const double *)rec_rup(h, st, m + 1),
+                               (const double *)rec_rwp(h, st, m), (const double *)rec_rwp(h, st, m + 1), div, nsub,
+                               n == nsub ? 1 : 0);
+            hipLaunchKernelGGL(ka_ac_hor, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, a,
+                               (const double *)Lrup[cur], Lrup[1 - cur], (const double *)rec_rtp(h, st, m),
+                               (const double *)rec_rtp(h, st, m > 0 ? m - 1 : 0),
+                               (const double *)rec_rup(h, st, m + 1), div, nsub, n == 1 ? 1 : 0);
+            cur = 1 - cur;
+        }
+        hipLaunchKernelGGL(ka_slow, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, dset(li, n2),
+                           ls0, lph, a, (const double *)Lrup[cur]);
+        hipLaunchKernelGGL(ka_diag, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, dset(li, n2), a);
+        std::swap(lo, li);
+    }
+    // lo: adjoint of the stage-1 input; the step input also enters as s0 of every stage and through physics
+    hipLaunchKernelGGL(ka_axpy, dim3(1024), dim3(256), 0, h->stream, lo, (const double *)A.ls0, (int64_t)(7 * n2));
+    hipLaunchKernelGGL(ka_physics, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv,
+                       sset<double>(h, S0_), lph, dset(lo, n2), A.gk, A.gk + n2, A.gk + 2 * n2);
+    if (lo != A.lam) copy_set(h, A.lam, lo, 7 * n2);
+}
+
+void save_state(moist_handle *h, double *dst) {
+    const size_t n2 = h->n2;
+    for (int f = 0; f < 7; ++f) copy_set(h, dst + f * n2, slot<double>(h, S0_ + f), n2);
+    copy_set(h, dst + 7 * n2, h->prec, h->nxs);
+}
+void load_state(moist_handle *h, const double *src) {
+    const size_t n2 = h->n2;
+    for (int f = 0; f < 7; ++f) copy_set(h, slot<double>(h, S0_ + f), src + f * n2, n2);
+    copy_set(h, h->prec, src + 7 * n2, h->nxs);
+}
+
+template <class H> int adjoint_run_t(moist_handle *h, int nsteps, int ckpt, const void *const *tgt, double *J) {
+    if (adj_alloc(h, nsteps, ckpt)) return -1;
+    AdjState &A = *h->adj;
+    const size_t n2 = h->n2;
+    // theta targets
+    for (int m = 0; m < h->nm; ++m) {
+        const Dom &D = h->dom_h[m];
+        const int ka = D.nz, ia = D.nx + 2 * D.halo;
+        HIPCHK(h, hipMemcpyAsync(h->staging, tgt[m], (size_t)ka * ia * sizeof(H), hipMemcpyHostToDevice, h->stream));
+        hipLaunchKernelGGL((k_upload<double, H>), dim3((D.nx + 255) / 256, ka), dim3(256), 0, h->stream, D,
+                           (const H *)h->staging, A.tgt, (const double *)nullptr, D.halo, ka);
+        HIPCHK(h, hipGetLastError());
+    }
+    hipEvent_t e0, e1, e2;
+    HIPCHK(h, hipEventCreate(&e0));
+    HIPCHK(h, hipEventCreate(&e1));
+    HIPCHK(h, hipEventCreate(&e2));
+    if (reset_nan(h)) return -1;
+    HIPCHK(h, hipEventRecord(e0, h->stream));
+    // forward with checkpoints
+    for (int n = 0; n < nsteps; ++n) {
+        if (n % ckpt == 0) save_state(h, A.cp + A.cp_stride * (n / ckpt));
+        if (run_steps(h, 1)) return -1;
+    }
+    HIPCHK(h, hipEventRecord(e1, h->stream));
+    int code = decode_nan(h);
+    if (code) return code;
+    save_state(h, A.fin);
+    // loss and the adjoint of the final state
+    fill(h, A.lam, 7 * n2);
+    fill(h, A.gk, 3 * n2);
+    hipLaunchKernelGGL(ka_loss, dim3(h->nm), dim3(256), 0, h->stream, h->dom_d, sset<double>(h, S0_),
+                       (const double *)h->prec, (const double *)A.tgt, A.lam + RHO * n2, A.lam + RT * n2, A.J);
+    // reverse sweep over the segments
+    for (int sg = A.ncp - 1; sg >= 0; --sg) {
+        const int n0 = sg * ckpt, n1 = std::min(nsteps, n0 + ckpt);
+        load_state(h, A.cp + A.cp_stride * sg);
+        for (int n = n0; n < n1; ++n) {
+            for (int f = 0; f < 7; ++f)
+                copy_set(h, A.seg + ((size_t)(n - n0) * 7 + f) * n2, slot<double>(h, S0_ + f), n2);
+            if (n + 1 < n1 && run_steps(h, 1)) return -1;
+        }
+        for (int n = n1 - 1; n >= n0; --n) enqueue_step_adj(h, A.seg + (size_t)(n - n0) * 7 * n2);
+        HIPCHK(h, hipGetLastError());
+    }
+    hipLaunchKernelGGL(ka_reduce_k, dim3(h->nm), dim3(256), 0, h->stream, h->dom_d, (const double *)A.gk,
+                       (const double *)(A.gk + n2), (const double *)(A.gk + 2 * n2), A.gK);
+    HIPCHK(h, hipEventRecord(e2, h->stream));
+    load_state(h, A.fin);
+    HIPCHK(h, hipMemcpyAsync(A.J_h.data(), A.J, h->nm * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(A.gK_h.data(), A.gK, 3 * h->nm * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipGetLastError());
+    float f1 = 0.f, f2 = 0.f;
+    hipEventElapsedTime(&f1, e0, e1);
+    hipEventElapsedTime(&f2, e1, e2);
+    A.ms_fwd = f1;
+    A.ms_bwd = f2;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipEventDestroy(e2);
+    if (J)
+        for (int m = 0; m < h->nm; ++m) J[m] = A.J_h[m];
+    h->halo_src = -1;
+    return decode_nan(h);
+}
+
 }  // namespace
 
 // =============================================================================
@@ -825,12 +1135,60 @@ int moist_kernel_stats(moist_handle *h, double *acoustic_ms_per_launch, int64_t 
     return 0;
 }
 
+int moist_adjoint_run(moist_handle *h, int nsteps, int ckpt, const void *const *theta_target, int elem_bytes,
+                      double *J) {
+    if (!h || !theta_target || nsteps < 1 || ckpt < 1) return -2;
+    if (h->precision != MOIST_FP64) {
+        h->err = "the adjoint runs on an FP64-form handle";
+        return -2;
+    }
+    if (h->any_sf || h->any_rd) {
+        h->err = "the adjoint covers do_sf = do_rd = false";
+        return -2;
+    }
+    for (int m = 0; m < h->nm; ++m)
+        if (!theta_target[m]) return -2;
+    if (elem_bytes == 8) return adjoint_run_t<double>(h, nsteps, ckpt, theta_target, J);
+    if (elem_bytes == 4) return adjoint_run_t<float>(h, nsteps, ckpt, theta_target, J);
+    h->err = "elem_bytes must be 4 or 8";
+    return -2;
+}
+
+int moist_adjoint_grad(moist_handle *h, int member, void *const grad[7], double *grad_params, int elem_bytes) {
+    if (!h) return -2;
+    if (int rc = check_member(h, member)) return rc;
+    if (!h->adj) {
+        h->err = "no adjoint run on this handle";
+        return -2;
+    }
+    const Dom &D = h->dom_h[member];
+    for (int f = 0; f < 7; ++f) {
+        if (!grad || !grad[f]) continue;
+        const double *src = h->adj->lam + (size_t)f * h->n2;
+        int rc = elem_bytes == 8 ? download_field<double, double>(h, D, src, nullptr, grad[f], nullptr, false)
+                 : elem_bytes == 4 ? download_field<double, float>(h, D, src, nullptr, grad[f], nullptr, false)
+                                   : -2;
+        if (rc) return rc;
+    }
+    if (grad_params)
+        for (int j = 0; j < 3; ++j) grad_params[j] = h->adj->gK_h[3 * member + j];
+    return 0;
+}
+
+int moist_adjoint_times(moist_handle *h, double *ms_forward, double *ms_reverse) {
+    if (!h || !h->adj) return -2;
+    if (ms_forward) *ms_forward = h->adj->ms_fwd;
+    if (ms_reverse) *ms_reverse = h->adj->ms_bwd;
+    return 0;
+}
+
 const char *moist_last_error(const moist_handle *h) { return h ? h->err.c_str() : "null handle"; }
 
 void moist_destroy(moist_handle *h) {
     if (!h) return;
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     for (auto e : h->ev_pool) hipEventDestroy(e);
+    adj_free(h);
     void *ptrs[] = {h->dom_d, h->tiles2d, h->stiles, h->ctiles, h->z, h->zf, h->dz, h->geo, h->theta_ref, h->rho_bar, h->rt_bar,
                     h->p_bar, h->L_bar, h->B_bar, h->pool, h->pool1, h->atiles, h->cin, h->prec, h->sfh, h->nan_key,
                     h->step_ctr, h->staging, h->diag_out};

@@ -1,0 +1,154 @@
+"""GPU reverse mode (moist_adjoint_run / moist_adjoint_grad, FP64 form)
+against the oracle's adjoint (itself pinned by central finite differences in
+test_adjoint_oracle.py) and against central finite differences of the GPU's
+own forward step.
+
+Bars: the loss to 1e-12 relative; per-field gradient rel-L2 <= 1e-8 (the
+oracle and the GPU run the same FP64 operations in a different order, and
+the reverse sweep amplifies their 1e-13 trajectory differences by at most
+the condition of the step); parameter gradients to 1e-8.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import PROG
+from moist2d_amd import core, ic
+
+pytestmark = pytest.mark.gpu
+
+PROG7 = ("rho", "rho_u", "rho_w", "rho_theta", "rho_qv", "rho_qc", "rho_qr")
+
+
+def _wrap(p, a):
+    h = p.halo
+    a[:, :h] = a[:, p.Nx:p.Nx + h]
+    a[:, h + p.Nx:] = a[:, h:2 * h]
+    return a
+
+
+def _case(nx=16, nz=12, seed=2, **kw):
+    rng = np.random.default_rng(seed)
+    p, s = ic.bubble(np.float64, nx=nx, nz=nz, d=200.0, **kw)
+    s.rho_u[...] = 3.0 * s.rho
+    O.rk3_step(s, p, 3)
+    s.rho_qc[...] += 1.5e-3 * s.rho * (1 + 0.2 * rng.random(s.rho.shape))
+    s.rho_qr[...] += 1e-4 * s.rho * (1 + 0.2 * rng.random(s.rho.shape))
+    for n in PROG7:
+        _wrap(p, getattr(s, n))
+    tt = np.asfortranarray(s.rho_theta / s.rho + rng.normal(0, 0.5, size=s.rho.shape))
+    return p, s, tt
+
+
+def _rel(a, b):
+    d = np.linalg.norm(a - b)
+    n = np.linalg.norm(b)
+    return d / n if n > 0 else d
+
+
+def _compare(tag, p, s, tt, nsteps, ckpt, tol=1e-8):
+    Jo, go, gKo = O.adjoint_run(s.copy(), p, nsteps, tt, ckpt=ckpt)
+    eng = core.Engine(p, core.FP64)
+    eng.upload(s)
+    J = eng.adjoint(nsteps, [tt], ckpt=ckpt)[0]
+    g, gK = eng.adjoint_grad(0)
+    fwd, rev = eng.adjoint_times()
+    eng.close()
+    h = p.halo
+    errs = {n: _rel(g[f][:, h:h + p.Nx], go[f][:, h:h + p.Nx]) for f, n in enumerate(PROG7)}
+    kerr = [abs(gK[j] - gKo[j]) / max(abs(gKo[j]), 1e-300) for j in range(3)]
+    print(f"{tag}: J {J:.12e} vs {Jo:.12e}; grad rel-L2 " + " ".join(f"{k}={v:.1e}" for k, v in errs.items())
+          + " dK " + " ".join(f"{e:.1e}" for e in kerr) + f"; fwd {fwd:.2f} ms rev {rev:.2f} ms", flush=True)
+    assert abs(J - Jo) <= 1e-12 * abs(Jo), (J, Jo)
+    for n, e in errs.items():
+        assert e <= tol, (tag, n, e)
+    for j in range(3):
+        assert kerr[j] <= tol or abs(gK[j] - gKo[j]) <= 1e-12, (tag, j, gK[j], gKo[j])
+    return g, gK
+
+
+def test_adjoint_one_step_dynamics_only():
+    p, s, tt = _case(do_mp=False)
+    _compare("dynamics 1 step", p, s, tt, 1, 1)
+
+
+def test_adjoint_one_step_with_physics():
+    p, s, tt = _case()
+    _compare("physics 1 step", p, s, tt, 1, 1)
+
+
+def test_adjoint_checkpointed_run():
+    p, s, tt = _case()
+    _compare("12 steps ckpt 5", p, s, tt, 12, 5)
+
+
+def test_adjoint_bubble_64x32():
+    p, s = ic.bubble(np.float64, nx=64, nz=32, d=312.5, dt=1.0)
+    O.rk3_step(s, p, 30)  # let cloud and rain form
+    rng = np.random.default_rng(9)
+    tt = np.asfortranarray(s.rho_theta / s.rho + rng.normal(0, 0.5, size=s.rho.shape))
+    _compare("bubble 64x32, 20 steps", p, s, tt, 20, 6, tol=1e-7)
+
+
+def test_adjoint_ensemble_members_match_single_runs():
+    cases = [_case(nx=16, nz=12, seed=2), _case(nx=24, nz=10, seed=5)]
+    eng = core.Engine([c[0] for c in cases], core.FP64)
+    for m, (p, s, tt) in enumerate(cases):
+        eng.upload(s, member=m)
+    J = eng.adjoint(6, [c[2] for c in cases], ckpt=4)
+    for m, (p, s, tt) in enumerate(cases):
+        Jo, go, gKo = O.adjoint_run(s.copy(), p, 6, tt, ckpt=4)
+        g, gK = eng.adjoint_grad(m)
+        h = p.halo
+        assert abs(J[m] - Jo) <= 1e-12 * abs(Jo)
+        for f, n in enumerate(PROG7):
+            e = _rel(g[f][:, h:h + p.Nx], go[f][:, h:h + p.Nx])
+            assert e <= 1e-8, (m, n, e)
+        np.testing.assert_allclose(gK, gKo, rtol=1e-8, atol=1e-12)
+    eng.close()
+
+
+def test_adjoint_matches_fd_of_gpu_forward():
+    """Central FD (h = 1e-6, smooth directions) of the GPU's own FP64 forward."""
+    p, s, tt = _case()
+    nsteps = 8
+    eng = core.Engine(p, core.FP64)
+    eng.upload(s)
+    eng.adjoint(nsteps, [tt], ckpt=3)
+    g, _ = eng.adjoint_grad(0)
+    eng.close()
+    h = p.halo
+    rng = np.random.default_rng(11)
+
+    def loss(s_):
+        e = core.Engine(p, core.FP64)
+        e.upload(s_)
+        e.step(nsteps)
+        a = s_.copy()
+        e.download(a)
+        e.close()
+        th = (a.rho_theta / a.rho)[:, h:h + p.Nx]
+        return p.dx * a.prec[h:h + p.Nx].sum() + 0.5 * np.sum((th - tt[:, h:h + p.Nx]) ** 2)
+
+    x = np.arange(p.Nx) / p.Nx
+    z = (np.arange(p.Nz) + 0.5) / p.Nz
+    errs = []
+    for f, n, sc in ((3, "rho_theta", 0.01), (4, "rho_qv", 1e-5), (3, "rho_theta", 0.01), (4, "rho_qv", 1e-5)):
+        v = np.zeros((p.Nz, p.Nx))
+        for m in range(1, 4):
+            for k in range(3):
+                a, b = rng.normal(size=2)
+                v += np.outer(np.sin(np.pi * k * z + 3 * rng.random()),
+                              a * np.cos(2 * np.pi * m * x) + b * np.sin(2 * np.pi * m * x))
+        full = np.zeros(s.rho.shape)
+        full[:, h:h + p.Nx] = v * sc
+        full = _wrap(p, full) * s.rho
+        eps = 1e-6
+        sp, sm = s.copy(), s.copy()
+        getattr(sp, n)[...] += eps * full
+        getattr(sm, n)[...] -= eps * full
+        fd = (loss(sp) - loss(sm)) / (2 * eps)
+        ad = float(np.sum(g[f][:, h:h + p.Nx] * full[:, h:h + p.Nx]))
+        errs.append(abs(fd - ad) / abs(ad))
+    print("GPU FD rel errors", " ".join(f"{e:.1e}" for e in errs), flush=True)
+    assert np.median(errs) <= 1e-5 and max(errs) <= 1e-4, errs
