@@ -187,7 +187,7 @@ struct AdjW {
     // adjoints
     double *Lth, *Lrt, *Lpres, *LU, *LW, *LRr, *LRu, *LRw, *LRt, *Lmx, *Lmz;
     double *Lrp, *Lrwp, *Lrtp, *pend, *Linc_r, *Linc_t;
-    double *Y, *cpr, *rmi;     // column scratch: transposed-solve vector, c', 1/m of the forward Thomas
+    double *Y, *cpr, *rmi, *aco;  // column scratch: transposed-solve vector; c', 1/m, a of the forward Thomas
     double *linb, *lint;       // per column: positivity-limiter adjoints (3 scalars x columns)
 };
 
@@ -370,194 +370,301 @@ __global__ void __launch_bounds__(256) ka_lin(const Dom *__restrict__ doms, AdjW
     }
 }
 
-// ---- acoustic substep, column part (moist.jl:1377-1446) ---------------------------
-// One thread per column.  In: Lrp, Lrtp, Lrwp (adjoints after the substep),
-// Lmz.  Out: Lrwp (adjoint of rho_w' before the solve), Lrp / Lrtp with the
-// off-centred contributions, Linc_r / Linc_t, and Lth, Lrt, LRr, LRw, LRt
-// accumulated.  `factor`: first reverse substep of the stage, which also
-// factors the stage's tridiagonal matrix into c' and 1/m.
-__global__ void __launch_bounds__(64)
-    ka_ac_col(const Dom *__restrict__ doms, const int2 *__restrict__ ctiles, ZVec zv, SSet<double> s, AdjW a,
-              const double *__restrict__ rtp_s, const double *__restrict__ rup_n, const double *__restrict__ rwp_s,
-              const double *__restrict__ rwp_n, int div, int nsub, int factor) {
-    const int2 t = ctiles[blockIdx.x];
+// ---- acoustic substep, vertical part (moist.jl:1377-1446) ---------------------------
+// Coefficients of the w-p system of column i at row kk (0 <= kk <= ke-1):
+// A = tridiag(a, b + 1, c) (implicit_correction!, moist.jl:974-1030).
+struct WpCoef {
+    const Dom &D;
+    const double *zd, *dzd, *TH, *RT;
+    int nx, i;
+    double dt_tp2;
+    __device__ int64_t G(int kk) const { return D.off2 + (int64_t)kk * nx + i; }
+    __device__ double th(int kk) const { return f2h(TH[G(kk)], TH[G(kk + 1)], dzd[kk], dzd[kk + 1]); }
+    __device__ double z1(int kk) const { return zd[kk + 1] - zd[kk]; }
+    __device__ double a(int kk) const {  // kk > 0
+        return -dt_tp2 * (RT[G(kk)] * th(kk - 1) / (z1(kk) * dzd[kk]) - K<double>::GRAV / (dzd[kk] + dzd[kk + 1]));
+    }
+    __device__ double b(int kk) const {
+        return dt_tp2 * (RT[G(kk)] / dzd[kk] + RT[G(kk + 1)] / dzd[kk + 1]) * th(kk) / z1(kk);
+    }
+    __device__ double c(int kk) const {  // kk < ke - 1
+        return -dt_tp2 * (RT[G(kk + 1)] * th(kk + 1) / (z1(kk) * dzd[kk + 1]) + K<double>::GRAV / (dzd[kk] + dzd[kk + 1]));
+    }
+};
+
+// (1) Adjoint of rho_w' after the solve: the momentum average, the rho'
+// and rho_theta' vertical updates (with the Q5 branch tests of the
+// forward); theta adjoint of the rho_theta' vertical flux.  Per cell.
+__global__ void __launch_bounds__(AD_X *AD_Z)
+    ka_ac_v1(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s, AdjW a,
+             const double *__restrict__ rwp_n, int div, int nsub) {
+    const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
-    const int i = t.y + threadIdx.x;
+    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
     const int nx = D.nx, nz = D.nz, ke = nz - 1;
-    if (i >= nx) return;
-    const double *zd = zv.z + D.offz, *dzd = zv.dz + D.offz;
-    const double dx = D.dx, EPS = 1e-12, GRAV = K<double>::GRAV;
-    auto G = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
-    const double *TH = a.theta, *RT = a.rt2;
+    if (i >= nx || k >= nz) return;
+    const double *dzd = zv.dz + D.offz;
+    const double EPS = 1e-12;
     const double dt_s = D.dt / double(div) / double(nsub);
-    const double fact_tp = 0.5 * (1.0 + D.beta), fact_tm = 0.5 * (1.0 - D.beta);
-    const double fact = fact_tm / fact_tp;
-    const double dt_tp2 = (dt_s * fact_tp) * (dt_s * fact_tp);
-    const int n = nz - 1;
-    auto th_f = [&](int kk) { return f2h(TH[G(kk, i)], TH[G(kk + 1, i)], dzd[kk], dzd[kk + 1]); };
-    auto coef_a = [&](int kk) {  // a_j, j = kk (kk > 0)
-        const double z1 = zd[kk + 1] - zd[kk];
-        return -dt_tp2 * (RT[G(kk, i)] * th_f(kk - 1) / (z1 * dzd[kk]) - GRAV / (dzd[kk] + dzd[kk + 1]));
+    auto G = [&](int kk) -> int64_t { return D.off2 + (int64_t)kk * nx + i; };
+    const double *TH = a.theta, *rw = s.f[RHOW];
+    auto THv = [&](int kk) { return kk >= 0 && kk <= ke ? TH[G(kk)] : 0.0; };
+    // face f+1/2 as fkp of cell f: positive branch tests rwp_n[f], negative s.rho_w[f]
+    auto brp = [&](int f, bool &pos, bool &neg) {
+        pos = rwp_n[G(f)] > EPS && f > 0;
+        neg = !pos && rw[G(f)] < -EPS && f < ke - 1;
     };
-    auto coef_b = [&](int kk) {
-        const double z1 = zd[kk + 1] - zd[kk];
-        return dt_tp2 * (RT[G(kk, i)] / dzd[kk] + RT[G(kk + 1, i)] / dzd[kk + 1]) * th_f(kk) / z1;
+    // face f+1/2 as fkm of cell f+1: both branches test s.rho_w[f]
+    auto brm = [&](int f, bool &pos, bool &neg) {
+        const double sw = rw[G(f)];
+        pos = sw > EPS && f + 1 > 1;
+        neg = !pos && sw < -EPS && f + 1 < ke;
     };
-    auto coef_c = [&](int kk) {  // kk < ke - 1
-        const double z1 = zd[kk + 1] - zd[kk];
-        return -dt_tp2 * (RT[G(kk + 1, i)] * th_f(kk + 1) / (z1 * dzd[kk + 1]) + GRAV / (dzd[kk] + dzd[kk + 1]));
-    };
-    if (factor && n > 0) {
-        // forward Thomas factorization: m_j = b_j + 1 - a_j c'_j-1, c'_j = c_j / m_j
-        double cprev = 0.0;
-        for (int kk = 0; kk <= ke - 1; ++kk) {
-            const double m = coef_b(kk) + 1.0 - (kk > 0 ? coef_a(kk) * cprev : 0.0);
+    double lth = 0.0;
+    for (int f = k - 2; f <= k + 1; ++f) {
+        if (f < 0 || f > ke - 1) continue;
+        const double den = dzd[f] + dzd[f + 1];
+        const double w0 = dzd[f + 1] / den, w1 = dzd[f] / den;
+        const double w = rwp_n[G(f)];
+        bool pos, neg;
+        brp(f, pos, neg);
+        lth += -dt_s / dzd[f] * a.Lrtp[G(f)] * w * upw_coef(pos, neg, k - f, w0, w1);
+        brm(f, pos, neg);
+        lth += dt_s / dzd[f + 1] * a.Lrtp[G(f + 1)] * w * upw_coef(pos, neg, k - f, w0, w1);
+    }
+    a.Lth[G(k)] += lth;
+    if (k <= ke - 1) {
+        const double den = dzd[k] + dzd[k + 1];
+        const double w0 = dzd[k + 1] / den, w1 = dzd[k] / den;
+        bool pos, neg;
+        brp(k, pos, neg);
+        const double gp = upw_val(pos, neg, THv(k - 1), THv(k), THv(k + 1), THv(k + 2), w0, w1);
+        brm(k, pos, neg);
+        const double gm = upw_val(pos, neg, THv(k - 1), THv(k), THv(k + 1), THv(k + 2), w0, w1);
+        const int64_t q = G(k), qp = G(k + 1);
+        a.Lrwp[q] += a.Lmz[q] / double(nsub) - dt_s / dzd[k] * a.Lrtp[q] * gp + dt_s / dzd[k + 1] * a.Lrtp[qp] * gm -
+                     dt_s / dzd[k] * a.Lrp[q] + dt_s / dzd[k + 1] * a.Lrp[qp];
+    }
+}
+
+// Wave-wide inclusive scans (lane 0 first / lane 63 first) of affine maps
+// x -> A x + B and of 2x2 matrices (Moebius maps, scaled to unit max norm).
+__device__ __forceinline__ void scan_affine_up(double &A, double &B, int lane) {
+    for (int d = 1; d < 64; d <<= 1) {
+        const double Ap = __shfl_up(A, d), Bp = __shfl_up(B, d);
+        if (lane >= d) {
+            B = A * Bp + B;
+            A = A * Ap;
+        }
+    }
+}
+__device__ __forceinline__ void scan_affine_down(double &A, double &B, int lane) {
+    for (int d = 1; d < 64; d <<= 1) {
+        const double Ap = __shfl_down(A, d), Bp = __shfl_down(B, d);
+        if (lane + d < 64) {
+            B = A * Bp + B;
+            A = A * Ap;
+        }
+    }
+}
+__device__ __forceinline__ void mob_norm(double &m00, double &m01, double &m10, double &m11) {
+    const double s = fmax(fmax(fabs(m00), fabs(m01)), fmax(fabs(m10), fabs(m11)));
+    if (s > 0.0) {
+        const double r = 1.0 / s;
+        m00 *= r;
+        m01 *= r;
+        m10 *= r;
+        m11 *= r;
+    }
+}
+
+// (2) Transposed solve A^T y = Lx of every column, one wave per column
+// (lane l owns rows l*L .. l*L+L-1).  With the forward factorization A = L U
+// (L: diagonal m_j, sub-diagonal a_j; U: unit, super-diagonal c'_j):
+// U^T w = Lx is w_j = Lx_j - c'_j-1 w_j-1, L^T y = w is
+// y_j = (w_j - a_j+1 y_j+1) / m_j; both are affine scans.  `factor` (first
+// reverse substep of a stage) first computes c'_j = c_j / (b_j + 1 - a_j c'_j-1)
+// as a Moebius scan and stores c', 1/m and a.
+__global__ void __launch_bounds__(256)
+    ka_ac_solve(const Dom *__restrict__ doms, const int2 *__restrict__ wcols, int ncols, ZVec zv, AdjW a, int div,
+                int nsub, int factor) {
+    const int wid = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (wid >= ncols) return;
+    const int2 t = wcols[wid];
+    if (t.x < 0) return;
+    const Dom &D = doms[t.x];
+    const int nx = D.nx, nz = D.nz, n = nz - 1;
+    if (n <= 0) return;
+    const double dt_s = D.dt / double(div) / double(nsub);
+    const double fact_tp = 0.5 * (1.0 + D.beta);
+    const WpCoef W{D, zv.z + D.offz, zv.dz + D.offz, a.theta, a.rt2, nx, t.y, (dt_s * fact_tp) * (dt_s * fact_tp)};
+    const int L = (n + 63) / 64, j0 = lane * L;
+    if (factor) {
+        double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;  // lane composite, identity
+        for (int j = j0; j < min(j0 + L, n); ++j) {
+            const double aj = j > 0 ? W.a(j) : 0.0, bj = W.b(j) + 1.0, cj = j < n - 1 ? W.c(j) : 0.0;
+            // M_j = [[0, c], [-a, b+1]] applied after the composite
+            const double n00 = cj * m10, n01 = cj * m11;
+            const double n10 = -aj * m00 + bj * m10, n11 = -aj * m01 + bj * m11;
+            m00 = n00;
+            m01 = n01;
+            m10 = n10;
+            m11 = n11;
+            mob_norm(m00, m01, m10, m11);
+        }
+        for (int d = 1; d < 64; d <<= 1) {
+            const double p00 = __shfl_up(m00, d), p01 = __shfl_up(m01, d), p10 = __shfl_up(m10, d),
+                         p11 = __shfl_up(m11, d);
+            if (lane >= d) {
+                const double n00 = m00 * p00 + m01 * p10, n01 = m00 * p01 + m01 * p11;
+                const double n10 = m10 * p00 + m11 * p10, n11 = m10 * p01 + m11 * p11;
+                m00 = n00;
+                m01 = n01;
+                m10 = n10;
+                m11 = n11;
+                mob_norm(m00, m01, m10, m11);
+            }
+        }
+        // c' entering this lane: composite of the lanes before, applied to c'_-1 = 0 (p, q) = (0, 1)
+        const double e01 = __shfl_up(m01, 1), e11 = __shfl_up(m11, 1);
+        double cprev = lane == 0 ? 0.0 : e01 / e11;
+        for (int j = j0; j < min(j0 + L, n); ++j) {
+            const double aj = j > 0 ? W.a(j) : 0.0;
+            const double m = W.b(j) + 1.0 - aj * cprev;
             const double rm = 1.0 / m;
-            const double cp = kk < ke - 1 ? coef_c(kk) * rm : 0.0;
-            a.cpr[G(kk, i)] = cp;
-            a.rmi[G(kk, i)] = rm;
+            const double cp = j < n - 1 ? W.c(j) * rm : 0.0;
+            a.cpr[W.G(j)] = cp;
+            a.rmi[W.G(j)] = rm;
+            a.aco[W.G(j)] = aj;
             cprev = cp;
         }
     }
-    // momentum average (mom_z += rwp / ns) and the two vertical updates
-    for (int k = 0; k <= ke - 1; ++k) a.Lrwp[G(k, i)] += a.Lmz[G(k, i)] / double(nsub);
-    for (int k = 0; k <= ke; ++k) {
-        const double l = a.Lrtp[G(k, i)];
-        const double lr = a.Lrp[G(k, i)];
-        if (k < ke) {  // fkp of cell k: w = rwp_n[k], positive branch tests w, negative tests s.rho_w[k]
-            const double lf = -dt_s / dzd[k] * l;
-            const double w = rwp_n[G(k, i)];
-            const bool pos = w > EPS && k > 0, neg = !pos && s.f[RHOW][G(k, i)] < -EPS && k < ke - 1;
-            const double den = dzd[k] + dzd[k + 1];
-            const double w0 = dzd[k + 1] / den, w1 = dzd[k] / den;
-            const double g = upw_val(pos, neg, k > 0 ? TH[G(k - 1, i)] : 0.0, TH[G(k, i)], TH[G(k + 1, i)],
-                                     k + 2 <= ke ? TH[G(k + 2, i)] : 0.0, w0, w1);
-            for (int o = -1; o <= 2; ++o) {
-                const int kk = k + o;
-                if (kk < 0 || kk > ke) continue;
-                const double cf = upw_coef(pos, neg, o, w0, w1);
-                if (cf != 0.0) a.Lth[G(kk, i)] += lf * w * cf;
-            }
-            a.Lrwp[G(k, i)] += lf * g - dt_s / dzd[k] * lr;
-        }
-        if (k > 0) {  // fkm of cell k: face k-1/2, w = rwp_n[k-1], both branches test s.rho_w[k-1]
-            const double lf = dt_s / dzd[k] * l;
-            const int kb = k - 1;
-            const double w = rwp_n[G(kb, i)], sw = s.f[RHOW][G(kb, i)];
-            const bool pos = sw > EPS && k > 1, neg = !pos && sw < -EPS && k < ke;
-            const double den = dzd[kb] + dzd[k];
-            const double w0 = dzd[k] / den, w1 = dzd[kb] / den;
-            const double g = upw_val(pos, neg, kb > 0 ? TH[G(kb - 1, i)] : 0.0, TH[G(kb, i)], TH[G(k, i)],
-                                     k + 1 <= ke ? TH[G(k + 1, i)] : 0.0, w0, w1);
-            for (int o = -1; o <= 2; ++o) {
-                const int kk = kb + o;
-                if (kk < 0 || kk > ke) continue;
-                const double cf = upw_coef(pos, neg, o, w0, w1);
-                if (cf != 0.0) a.Lth[G(kk, i)] += lf * w * cf;
-            }
-            a.Lrwp[G(kb, i)] += lf * g + dt_s / dzd[k] * lr;
-        }
+    // forward sweep w_j = Lx_j - c'_j-1 w_j-1
+    double A = 1.0, B = 0.0;
+    for (int j = j0; j < min(j0 + L, n); ++j) {
+        const double al = j > 0 ? -a.cpr[W.G(j - 1)] : 0.0;
+        B = al * B + a.Lrwp[W.G(j)];
+        A = al * A;
     }
-    if (n <= 0) return;
-    // transposed solve A^T y = Lx with A = L U: U^T w = Lx (forward), L^T y = w (backward)
-    {
-        double wprev = 0.0;
-        for (int kk = 0; kk <= ke - 1; ++kk) {
-            const double wv = a.Lrwp[G(kk, i)] - (kk > 0 ? a.cpr[G(kk - 1, i)] * wprev : 0.0);
-            a.Y[G(kk, i)] = wv;
-            wprev = wv;
-        }
-        double ynext = 0.0;
-        for (int kk = ke - 1; kk >= 0; --kk) {
-            const double yv = (a.Y[G(kk, i)] - (kk < ke - 1 ? coef_a(kk + 1) * ynext : 0.0)) * a.rmi[G(kk, i)];
-            a.Y[G(kk, i)] = yv;
-            ynext = yv;
-        }
+    scan_affine_up(A, B, lane);
+    double win = __shfl_up(B, 1);
+    if (lane == 0) win = 0.0;
+    double wv = win;
+    for (int j = j0; j < min(j0 + L, n); ++j) {
+        const double al = j > 0 ? -a.cpr[W.G(j - 1)] : 0.0;
+        wv = al * wv + a.Lrwp[W.G(j)];
+        a.Y[W.G(j)] = wv;  // w, replaced by y below
     }
-    // matrix and right-hand-side adjoints, rho_w' before the solve, drwp and the explicit part
-    double yprev = 0.0, Lthc_carry = 0.0;
-    double rtp_tp_k = 0.0;
+    // back substitution y_j = rm_j w_j - rm_j a_j+1 y_j+1, lanes from the top
+    A = 1.0;
+    B = 0.0;
+    for (int j = min(j0 + L, n) - 1; j >= j0; --j) {
+        const double rm = a.rmi[W.G(j)];
+        const double al = j < n - 1 ? -rm * a.aco[W.G(j + 1)] : 0.0;
+        B = al * B + rm * a.Y[W.G(j)];
+        A = al * A;
+    }
+    scan_affine_down(A, B, lane);
+    double yin = __shfl_down(B, 1);
+    if (lane == 63) yin = 0.0;
+    double yv = yin;
+    for (int j = min(j0 + L, n) - 1; j >= j0; --j) {
+        const double rm = a.rmi[W.G(j)];
+        const double al = j < n - 1 ? -rm * a.aco[W.G(j + 1)] : 0.0;
+        yv = al * yv + rm * a.Y[W.G(j)];
+        a.Y[W.G(j)] = yv;
+    }
+}
+
+// (3) Matrix and right-hand-side adjoints of the solve, the adjoint of
+// rho_w' before it, drwp = dt_s (R_rho_w - rt2h buoy + grav) and the explicit
+// part (rp_tp = rp + f inc, rp += inc; the same for rho_theta').  Per cell.
+__global__ void __launch_bounds__(AD_X *AD_Z)
+    ka_ac_v2(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, AdjW a,
+             const double *__restrict__ rtp_s, const double *__restrict__ rup_n, const double *__restrict__ rwp_s,
+             const double *__restrict__ rwp_n, int div, int nsub) {
+    const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;
+    const Dom &D = doms[t.x];
+    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+    const int nx = D.nx, nz = D.nz, ke = nz - 1, n = nz - 1;
+    if (i >= nx || k >= nz || n <= 0) return;
+    const double *zd = zv.z + D.offz, *dzd = zv.dz + D.offz;
+    const double GRAV = K<double>::GRAV, dx = D.dx;
+    const double dt_s = D.dt / double(div) / double(nsub);
+    const double fact_tp = 0.5 * (1.0 + D.beta), fact_tm = 0.5 * (1.0 - D.beta);
+    const double fact = fact_tm / fact_tp;
+    const WpCoef W{D, zd, dzd, a.theta, a.rt2, nx, i, (dt_s * fact_tp) * (dt_s * fact_tp)};
+    const double dt_tp2 = W.dt_tp2;
+    const double *RT = a.rt2, *TH = a.theta;
+    auto G = [&](int kk) -> int64_t { return W.G(kk); };
+    auto Gx = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
+    auto yv = [&](int j) { return j >= 0 && j <= n - 1 ? a.Y[G(j)] : 0.0; };
+    auto xv = [&](int j) { return j >= 0 && j <= n - 1 ? rwp_n[G(j)] : 0.0; };
+    auto La = [&](int j) { return j >= 1 && j <= n - 1 ? -yv(j) * xv(j - 1) - fact * yv(j) * rwp_s[G(j)] : 0.0; };
+    auto Lb = [&](int j) { return j >= 0 && j <= n - 1 ? -yv(j) * xv(j) - fact * yv(j) * rwp_s[G(j)] : 0.0; };
+    auto Lc = [&](int j) {
+        return j >= 0 && j < n - 1 ? -yv(j) * xv(j + 1) - fact * yv(j) * rwp_s[G(j + 1)] : 0.0;
+    };
+    // adjoint of the face theta th_j (j = 0..n-1)
+    auto Lthc = [&](int j) {
+        if (j < 0 || j > n - 1) return 0.0;
+        double v = Lb(j) * dt_tp2 * (RT[G(j)] / dzd[j] + RT[G(j + 1)] / dzd[j + 1]) / W.z1(j);
+        if (j + 1 <= n - 1) v += La(j + 1) * (-dt_tp2 * RT[G(j + 1)] / (W.z1(j + 1) * dzd[j + 1]));
+        if (j >= 1) v += Lc(j - 1) * (-dt_tp2 * RT[G(j)] / (W.z1(j - 1) * dzd[j]));
+        return v;
+    };
+    // off-centred rho_theta' of the forward explicit part
     auto rtp_tp = [&](int kk) {
-        const int64_t q = G(kk, i);
-        const double fip = (TH[G(kk, i + 1)] + TH[q]) * 0.5 * rup_n[q];
-        const double fim = (TH[q] + TH[G(kk, i - 1)]) * 0.5 * rup_n[G(kk, i - 1)];
+        const int64_t q = G(kk);
+        const double fip = (TH[Gx(kk, i + 1)] + TH[q]) * 0.5 * rup_n[q];
+        const double fim = (TH[q] + TH[Gx(kk, i - 1)]) * 0.5 * rup_n[Gx(kk, i - 1)];
         const double inc = dt_s * (a.R_rt[q] + (-(fip - fim) / dx));
         return rtp_s[q] + fact_tp * inc;
     };
-    rtp_tp_k = rtp_tp(0);
-    // Lthc (adjoint of the face theta th_j) gathers from b_j, a_j+1 and c_j-1
-    for (int k = 0; k <= ke; ++k) {
-        const int64_t q = G(k, i);
-        const double y = k <= ke - 1 ? a.Y[q] : 0.0;
-        double Lrp_tp = 0.0, Lrtp_tp = 0.0;
-        if (k <= ke - 1) {
-            const double z1 = zd[k + 1] - zd[k];
-            const double x = rwp_n[q];
-            const double ws = rwp_s[q];
-            const double b = coef_b(k);
-            double La = 0.0, Lc = 0.0;
-            const double Lb = -y * x - fact * y * ws;
-            double lws = y * (1.0 - fact * b);
-            if (k > 0) {
-                La = -y * rwp_n[G(k - 1, i)] - fact * y * ws;
-                lws += -fact * coef_a(k) * y;
-                lws += -fact * coef_c(k - 1) * yprev;
-            }
-            if (k < ke - 1) Lc = -y * rwp_n[G(k + 1, i)] - fact * y * rwp_s[G(k + 1, i)];
-            a.Lrwp[q] = lws;
-            const double rk = RT[q], rk1 = RT[G(k + 1, i)];
-            const double thk = th_f(k);
-            a.Lrt[q] += Lb * dt_tp2 * thk / (z1 * dzd[k]);
-            a.Lrt[G(k + 1, i)] += Lb * dt_tp2 * thk / (z1 * dzd[k + 1]);
-            double Lthc_k = Lb * dt_tp2 * (rk / dzd[k] + rk1 / dzd[k + 1]) / z1 + Lthc_carry;
-            Lthc_carry = 0.0;
-            if (k > 0) {
-                a.Lrt[q] += La * (-dt_tp2 * th_f(k - 1) / (z1 * dzd[k]));
-                // th_(k-1): its own adjoint was completed at k-1 except this term
-                const double add = La * (-dt_tp2 * rk / (z1 * dzd[k]));
-                const double den = dzd[k - 1] + dzd[k];
-                a.Lth[G(k - 1, i)] += add * dzd[k] / den;
-                a.Lth[q] += add * dzd[k - 1] / den;
-            }
-            if (k < ke - 1) {
-                a.Lrt[G(k + 1, i)] += Lc * (-dt_tp2 * th_f(k + 1) / (z1 * dzd[k + 1]));
-                Lthc_carry = Lc * (-dt_tp2 * rk1 / (z1 * dzd[k + 1]));
-            }
-            {
-                const double den = dzd[k] + dzd[k + 1];
-                a.Lth[q] += Lthc_k * dzd[k + 1] / den;
-                a.Lth[G(k + 1, i)] += Lthc_k * dzd[k] / den;
-            }
-            // drwp = dt_s (R_rho_w - rt2h buoy + grav)
-            const double rtp_tp_k1 = rtp_tp(k + 1);
-            const double buoy = (rtp_tp_k1 - rtp_tp_k) / z1;
-            const double rt2h = (rk + rk1) * 0.5;
-            a.LRw[q] += dt_s * y;
-            a.Lrt[q] += -dt_s * y * buoy * 0.5;
-            a.Lrt[G(k + 1, i)] += -dt_s * y * buoy * 0.5;
-            Lrtp_tp += dt_s * y * rt2h / z1;
-            Lrp_tp += -dt_s * y * GRAV * 0.5;
-            rtp_tp_k = rtp_tp_k1;
-        }
-        if (k > 0) {
-            const int kb = k - 1;
-            const double z1 = zd[kb + 1] - zd[kb];
-            const double rt2h = (RT[G(kb, i)] + RT[q]) * 0.5;
-            Lrtp_tp += -dt_s * yprev * rt2h / z1;
-            Lrp_tp += -dt_s * yprev * GRAV * 0.5;
-        }
-        // explicit part: rp_tp = rp + f inc, rp += inc (the same for rtp)
-        const double lr = a.Lrp[q], lt = a.Lrtp[q];
-        const double lir = lr + fact_tp * Lrp_tp, lit = lt + fact_tp * Lrtp_tp;
-        a.Lrp[q] = lr + Lrp_tp;
-        a.Lrtp[q] = lt + Lrtp_tp;
-        a.Linc_r[q] = lir;
-        a.Linc_t[q] = lit;
-        a.LRr[q] += dt_s * lir;
-        a.LRt[q] += dt_s * lit;
-        yprev = y;
+    const int64_t q = G(k);
+    double lrt = 0.0, lth = 0.0, Lrp_tp = 0.0, Lrtp_tp = 0.0;
+    if (k <= n - 1) {
+        const double y = yv(k);
+        const double thk = W.th(k), z1 = W.z1(k);
+        lrt += Lb(k) * dt_tp2 * thk / (z1 * dzd[k]);
+        if (k >= 1) lrt += La(k) * (-dt_tp2 * W.th(k - 1) / (z1 * dzd[k]));
+        const double buoy = (rtp_tp(k + 1) - rtp_tp(k)) / z1;
+        lrt += -0.5 * dt_s * y * buoy;
+        const double rt2h = (RT[q] + RT[G(k + 1)]) * 0.5;
+        Lrtp_tp += dt_s * y * rt2h / z1;
+        Lrp_tp += -dt_s * y * GRAV * 0.5;
+        a.LRw[q] += dt_s * y;
+        lth += Lthc(k) * dzd[k + 1] / (dzd[k] + dzd[k + 1]);
+        // rho_w' before the solve
+        double lws = y * (1.0 - fact * W.b(k));
+        if (k >= 1) lws += -fact * W.a(k) * y - fact * W.c(k - 1) * yv(k - 1);
+        a.Lrwp[q] = lws;
     }
+    if (k >= 1) {
+        const int kb = k - 1;
+        const double yb = yv(kb);
+        const double z1 = W.z1(kb);
+        lrt += Lb(kb) * dt_tp2 * W.th(kb) / (z1 * dzd[k]);
+        if (kb < n - 1) lrt += Lc(kb) * (-dt_tp2 * W.th(k) / (z1 * dzd[k]));
+        const double buoy = (rtp_tp(k) - rtp_tp(kb)) / z1;
+        lrt += -0.5 * dt_s * yb * buoy;
+        const double rt2h = (RT[G(kb)] + RT[q]) * 0.5;
+        Lrtp_tp += -dt_s * yb * rt2h / z1;
+        Lrp_tp += -dt_s * yb * GRAV * 0.5;
+        lth += Lthc(kb) * dzd[kb] / (dzd[kb] + dzd[k]);
+    }
+    a.Lrt[q] += lrt;
+    a.Lth[q] += lth;
+    const double lr = a.Lrp[q], lt = a.Lrtp[q];
+    const double lir = lr + fact_tp * Lrp_tp, lit = lt + fact_tp * Lrtp_tp;
+    a.Lrp[q] = lr + Lrp_tp;
+    a.Lrtp[q] = lt + Lrtp_tp;
+    a.Linc_r[q] = lir;
+    a.Linc_t[q] = lit;
+    a.LRr[q] += dt_s * lir;
+    a.LRt[q] += dt_s * lit;
 }
 
 // ---- acoustic substep, horizontal part (moist.jl:1360-1408) -----------------------

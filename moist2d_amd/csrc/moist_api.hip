@@ -84,6 +84,8 @@ struct moist_handle {
     int2 *ctiles = nullptr;
     int nctiles = 0;
     int4 *atiles = nullptr;  // acoustic tiles {member, i0, TX, nch}
+    int2 *wcols = nullptr;   // every column {member, i} (one wave each in the adjoint's column solve)
+    int nwcols = 0;
     int natiles = 0;
     int nch_max = 0;
     double *cin = nullptr;   // c' entering each acoustic chunk, [chunk][column]
@@ -504,7 +506,7 @@ int compute_step_t(moist_handle *h, int m, const void *const phys[6], const void
 // rho_w'), and the adjoint fields.
 enum AdjWork {
     W_TH = 0, W_RT, W_PRES, W_LU, W_LW, W_LRR, W_LRU, W_LRW, W_LRT, W_LMX, W_LMZ, W_LRP, W_LRWP, W_LRTP, W_PEND,
-    W_LINCR, W_LINCT, W_Y, W_CPR, W_RMI, W_LRUP0, W_LRUP1, NWORK
+    W_LINCR, W_LINCT, W_Y, W_CPR, W_RMI, W_ACO, W_LRUP0, W_LRUP1, NWORK
 };
 enum { REC_TH = 0, REC_PRES, REC_RT2, REC_RR, REC_RRU, REC_RRW, REC_RRT, REC_MX, REC_MZ, NREC };
 
@@ -607,6 +609,7 @@ AdjW adj_w(moist_handle *h, int st) {
     a.Y = wk(h, W_Y);
     a.cpr = wk(h, W_CPR);
     a.rmi = wk(h, W_RMI);
+    a.aco = wk(h, W_ACO);
     a.linb = h->adj->lim;
     a.lint = h->adj->lim + 3 * h->nxs;
     return a;
@@ -700,10 +703,13 @@ void enqueue_step_adj(moist_handle *h, const double *sn) {
         int cur = 0;
         for (int n = nsub; n >= 1; --n) {
             const int m = n - 1;
-            hipLaunchKernelGGL(ka_ac_col, dim3(h->nctiles), dim3(COL_TILE), 0, h->stream, h->dom_d, h->ctiles, zv, in,
-                               a, (const double *)rec_rtp(h, st, m), (const double *)rec_rup(h, st, m + 1),
-                               (const double *)rec_rwp(h, st, m), (const double *)rec_rwp(h, st, m + 1), div, nsub,
-                               n == nsub ? 1 : 0);
+            hipLaunchKernelGGL(ka_ac_v1, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, a,
+                               (const double *)rec_rwp(h, st, m + 1), div, nsub);
+            hipLaunchKernelGGL(ka_ac_solve, dim3((h->nwcols + 3) / 4), dim3(256), 0, h->stream, h->dom_d, h->wcols,
+                               h->nwcols, zv, a, div, nsub, n == nsub ? 1 : 0);
+            hipLaunchKernelGGL(ka_ac_v2, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, a,
+                               (const double *)rec_rtp(h, st, m), (const double *)rec_rup(h, st, m + 1),
+                               (const double *)rec_rwp(h, st, m), (const double *)rec_rwp(h, st, m + 1), div, nsub);
             hipLaunchKernelGGL(ka_ac_hor, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, a,
                                (const double *)Lrup[cur], Lrup[1 - cur], (const double *)rec_rtp(h, st, m),
                                (const double *)rec_rtp(h, st, m > 0 ? m - 1 : 0),
@@ -885,7 +891,7 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
     h->ns_stage = {std::max(1, (h->ns + 2) / 3), std::max(1, (h->ns + 1) / 2), h->ns};
 
     std::vector<int4> t2, ta, ts;
-    std::vector<int2> tc;
+    std::vector<int2> tc, wc;
     const int acL = precision == MOIST_MIXED ? ac_levels<float>() : ac_levels<double>();
     for (int m = 0; m < n_members; ++m) {
         const Dom &D = h->dom_h[m];
@@ -894,6 +900,7 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
         for (int k0 = 0; k0 < D.nz; k0 += ST_Z)
             for (int i0 = 0; i0 < D.nx; i0 += ST_X) ts.push_back(make_int4(m, i0, k0, 0));
         for (int i0 = 0; i0 < D.nx; i0 += COL_TILE) tc.push_back(make_int2(m, i0));
+        for (int i = 0; i < D.nx; ++i) wc.push_back(make_int2(m, i));
         int nch = 1;
         while (nch * acL < D.nz) nch *= 2;
         if (nch > AC_THREADS) {
@@ -909,6 +916,8 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
     ts = xcd_order(ts, make_int4(-1, 0, 0, 0));
     h->nstiles = (int)ts.size();
     tc = xcd_order(tc, make_int2(-1, 0));
+    wc = xcd_order(wc, make_int2(-1, 0));
+    h->nwcols = (int)wc.size();
     ta = xcd_order(ta, make_int4(-1, 0, 0, 0));
     h->natiles = (int)ta.size();
     h->ntiles2d = (int)t2.size();
@@ -922,6 +931,8 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
     ALLOC(hp->ctiles, sizeof(int2) * tc.size());
     ALLOC(hp->atiles, sizeof(int4) * ta.size());
     ALLOC(hp->stiles, sizeof(int4) * ts.size());
+    ALLOC(hp->wcols, sizeof(int2) * wc.size());
+    HIPCHK(hp, hipMemcpy(hp->wcols, wc.data(), sizeof(int2) * wc.size(), hipMemcpyHostToDevice));
     HIPCHK(hp, hipMemcpy(hp->stiles, ts.data(), sizeof(int4) * ts.size(), hipMemcpyHostToDevice));
     HIPCHK(hp, hipMemcpy(hp->atiles, ta.data(), sizeof(int4) * ta.size(), hipMemcpyHostToDevice));
     HIPCHK(hp, hipMemcpy(hp->dom_d, hp->dom_h.data(), sizeof(Dom) * n_members, hipMemcpyHostToDevice));
@@ -1189,7 +1200,7 @@ void moist_destroy(moist_handle *h) {
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     for (auto e : h->ev_pool) hipEventDestroy(e);
     adj_free(h);
-    void *ptrs[] = {h->dom_d, h->tiles2d, h->stiles, h->ctiles, h->z, h->zf, h->dz, h->geo, h->theta_ref, h->rho_bar, h->rt_bar,
+    void *ptrs[] = {h->dom_d, h->tiles2d, h->stiles, h->ctiles, h->wcols, h->z, h->zf, h->dz, h->geo, h->theta_ref, h->rho_bar, h->rt_bar,
                     h->p_bar, h->L_bar, h->B_bar, h->pool, h->pool1, h->atiles, h->cin, h->prec, h->sfh, h->nan_key,
                     h->step_ctr, h->staging, h->diag_out};
     for (void *p : ptrs)

@@ -228,6 +228,20 @@ def adjoint_bubble(T=np.float64, nx=512, nz=128, d=78.125, dt=0.5, ns=6, steps=6
     return bubble(T, nx=nx, nz=nz, d=d, dt=dt, ns=ns, steps=steps, seed=seed, **kw)
 
 
+def theta_target(p, seed=3, sigma=0.5, member=0):
+    """Target theta of the adjoint workload's loss: the base profile plus a
+    seeded Gaussian field of standard deviation `sigma` (K), in the (ka, ia)
+    layout (halo columns are periodic copies)."""
+    X, Z, I, K = _grid(p)
+    th = theta_profile(Z) + sigma * rng_normal(seed, member, 7, I, K)
+    out = np.zeros((p.Nz, p.Nx + 2 * p.halo), order="F")
+    h = p.halo
+    out[:, h:h + p.Nx] = th
+    out[:, :h] = th[:, p.Nx - h:]
+    out[:, h + p.Nx:] = th[:, :h]
+    return out
+
+
 def init_state(s: State, p, seed=0) -> State:
     """The reference's init_state!(s, p, rngs) (moist.jl:351-450) with the
     portable generator in place of the per-thread Xoshiro streams.  Float64
