@@ -14,6 +14,11 @@ kernel, timed live with a HIP event pair around each of its launches in a
 separate pass of K steps.  `cpu_baseline` times the CPU oracle (FP64,
 snapshot order, OpenMP over columns) on a bounded sample of the same
 workload.
+
+Beside the headline line: the 256-member ensemble (`ensemble`), the 512x128
+adjoint workload (`adjoint`: reverse-sweep time / forward time, central-FD
+check of the gradient) and the 64-member batched adjoint (`batched_adjoint`:
+gradients/s, device memory).
 """
 from __future__ import annotations
 
@@ -120,6 +125,125 @@ def ensemble_run(steps, members=256, seed=2):
     return out
 
 
+def _smooth_direction(p, rng):
+    """Seeded random field of low wavenumbers (x modes 1-3, z modes 0-2), (ka, ia) layout."""
+    x = np.arange(p.Nx) / p.Nx
+    z = (np.arange(p.Nz) + 0.5) / p.Nz
+    v = np.zeros((p.Nz, p.Nx))
+    for m in range(1, 4):
+        for n in range(3):
+            a, b = rng.normal(size=2)
+            v += np.outer(np.sin(np.pi * n * z + 3 * rng.random()),
+                          a * np.cos(2 * np.pi * m * x) + b * np.sin(2 * np.pi * m * x))
+    h = p.halo
+    out = np.zeros((p.Nz, p.Nx + 2 * h), order="F")
+    out[:, h:h + p.Nx] = v
+    out[:, :h] = v[:, p.Nx - h:]
+    out[:, h + p.Nx:] = v[:, :h]
+    return out
+
+
+def _loss_np(p, a, tt):
+    h = p.halo
+    th = (a.rho_theta / a.rho)[:, h:h + p.Nx]
+    return float(p.dx) * a.prec[h:h + p.Nx].sum() + 0.5 * np.sum((th - tt[:, h:h + p.Nx]) ** 2)
+
+
+def adjoint_bench(steps=600, ckpt=50, fd_dirs=8, cpu_steps=2):
+    """The 512x128 adjoint workload (FP64 form): loss J = dx sum(prec) +
+    1/2 |theta - theta_target|^2 after 600 steps, gradient w.r.t. the initial
+    theta' and qv (at fixed density) and K_auto, Qc_threshold, K_accr;
+    checkpoints every 50 steps.  Central FD (h = 1e-6) on seeded smooth
+    directions of the GPU's own FP64 forward checks the gradient."""
+    p, s = ic.adjoint_bubble(np.float64, steps=steps)
+    tt = ic.theta_target(p)
+    eng = core.Engine(p, core.FP64)
+    eng.store_phys_diag(False)
+    eng.use_graph(True)
+    eng.upload(s)
+    eng.time_steps(2)
+    eng.upload(s)
+    ms_plain = eng.time_steps(steps)
+    eng.upload(s)
+    J = eng.adjoint(steps, [tt], ckpt=ckpt)[0]
+    ms_fwd, ms_rev = eng.adjoint_times()
+    g, gK = eng.adjoint_grad(0)
+    st_b, ad_b = eng.device_bytes()
+    rng = np.random.default_rng(100)
+    errs = []
+    for d in range(fd_dirs):
+        f, n, sc = (3, "rho_theta", 0.01) if d % 2 == 0 else (4, "rho_qv", 1e-5)
+        v = _smooth_direction(p, rng) * sc * s.rho
+        eps = 1e-6
+        vals = []
+        for sgn in (1.0, -1.0):
+            sp = s.copy()
+            getattr(sp, n)[...] += sgn * eps * v
+            eng.upload(sp)
+            eng.step(steps)
+            a = s.copy()
+            eng.download(a, phys=False)
+            vals.append(_loss_np(p, a, tt))
+        fd = (vals[0] - vals[1]) / (2 * eps)
+        h = p.halo
+        ad = float(np.sum(g[f][:, h:h + p.Nx] * v[:, h:h + p.Nx]))
+        errs.append(abs(fd - ad) / abs(ad))
+    eng.close()
+    out = {"workload": f"adjoint_bubble_{p.Nx}x{p.Nz}", "form": "fp64", "steps": steps, "ckpt": ckpt,
+           "dt": float(p.dt), "ns": p.ns, "J": J, "dJ_dK": [float(x) for x in gK],
+           "forward_ms": ms_plain, "forward_with_checkpoints_ms": ms_fwd, "reverse_ms": ms_rev,
+           "adjoint_forward_ratio": ms_rev / ms_plain, "device_bytes": st_b + ad_b,
+           "fd_check": {"directions": fd_dirs, "h": 1e-6, "controls": "theta' and qv at fixed rho",
+                        "median_rel_err": float(np.median(errs)), "max_rel_err": float(max(errs))}}
+    if cpu_steps:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O  # the checker, timed here as the CPU baseline only
+        threads = O.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        out["cpu_oracle_threads"] = threads
+        t0 = time.perf_counter()
+        O.adjoint_run(s.copy(), p, cpu_steps, tt, ckpt=cpu_steps)
+        cpu = (time.perf_counter() - t0) / cpu_steps
+        out["cpu_oracle_ms_per_step"] = cpu * 1e3
+        out["gpu_ms_per_step"] = (ms_fwd + ms_rev) / steps
+    return out
+
+
+def batched_adjoint_bench(members=64, steps=300, ckpt=50, seed=4):
+    """64 bubbles at 1024x128 (dx = dz = 100 m, dt = 1 s, ns = 6) with seeded
+    amplitude / position and K_auto ~ U[5e-4, 2e-3]: forward + checkpointed
+    reverse sweep of every member's loss in one batched launch sequence
+    (FP64 form)."""
+    mem = []
+    for m in range(members):
+        u = [float(ic.rng_uniform(seed, m, 2000 + j, 0, 0)) for j in range(5)]
+        p, s = ic.bubble(np.float64, nx=1024, nz=128, d=100.0, dt=1.0, ns=6, steps=steps, amp=1.0 + 2.0 * u[0],
+                         xc=(0.3 + 0.4 * u[1]) * 1024 * 100.0, zc=1500.0 + 1000.0 * u[2], seed=seed, member=m,
+                         K_auto=5e-4 + 1.5e-3 * u[3])
+        mem.append((p, s, ic.theta_target(p, seed=seed, member=m)))
+    eng = core.Engine([p for p, _, _ in mem], core.FP64)
+    eng.store_phys_diag(False)
+    eng.use_graph(True)
+    for m, (p, s, _) in enumerate(mem):
+        eng.upload(s, member=m)
+    eng.time_steps(2)
+    for m, (p, s, _) in enumerate(mem):
+        eng.upload(s, member=m)
+    ms_plain = eng.time_steps(steps)
+    for m, (p, s, _) in enumerate(mem):
+        eng.upload(s, member=m)
+    J = eng.adjoint(steps, [t for _, _, t in mem], ckpt=ckpt)
+    ms_fwd, ms_rev = eng.adjoint_times()
+    _, gK0 = eng.adjoint_grad(0)
+    st_b, ad_b = eng.device_bytes()
+    eng.close()
+    cells = sum(p.Nx * p.Nz for p, _, _ in mem)
+    return {"workload": f"batched_adjoint_{members}x1024x128", "form": "fp64", "members": members, "cells": cells,
+            "steps": steps, "ckpt": ckpt, "forward_ms": ms_plain, "reverse_ms": ms_rev,
+            "adjoint_forward_ratio": ms_rev / ms_plain,
+            "gradients_per_s": members / ((ms_fwd + ms_rev) * 1e-3), "device_bytes": st_b + ad_b,
+            "device_GiB": (st_b + ad_b) / 2 ** 30, "J_member0": J[0], "dJ_dK_member0": [float(x) for x in gK0]}
+
+
 def cpu_baseline(p, s, steps):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # the checker, timed here as the CPU baseline only
@@ -168,6 +292,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=30, help="0 skips the 256-member ensemble")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workloads")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -236,6 +361,10 @@ def main():
                           "frac": step_gbs / HBM_PEAK_GBS}
 
     ens = ensemble_run(args.ensemble_steps) if args.ensemble_steps > 0 else None
+    adj = badj = None
+    if not args.no_adjoint:
+        adj = adjoint_bench(cpu_steps=0 if args.no_cpu else 2)
+        badj = batched_adjoint_bench()
 
     cpu = None
     if not args.no_cpu:
@@ -263,7 +392,9 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "ensemble": ens,
-        "adjoint_forward_ratio": None,
+        "adjoint_forward_ratio": adj["adjoint_forward_ratio"] if adj else None,
+        "adjoint": adj,
+        "batched_adjoint": badj,
     }
     print(json.dumps(line))
 

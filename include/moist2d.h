@@ -160,6 +160,10 @@ int moist_adjoint_run(moist_handle *h, int nsteps, int ckpt, const void *const *
  * member into host arrays (ka, ia) (interior columns written), and
  * dJ/d(K_auto, Qc_threshold, K_accr) into grad_params[3]; either may be NULL. */
 int moist_adjoint_grad(moist_handle *h, int member, void *const grad[7], double *grad_params, int elem_bytes);
+/* Device memory held by the handle: state pools and tables, and the
+ * buffers of the last adjoint run (checkpoints, segment states, forward
+ * record, adjoint fields). */
+int moist_device_bytes(moist_handle *h, int64_t *state_bytes, int64_t *adjoint_bytes);
 /* Device time of the last adjoint run: forward pass (with checkpoint copies)
  * and reverse sweep (segment recomputation included), in ms. */
 int moist_adjoint_times(moist_handle *h, double *ms_forward, double *ms_reverse);

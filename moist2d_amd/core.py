@@ -57,6 +57,7 @@ def lib():
             "moist_adjoint_run": (ip, [vp, ip, ip, vp, ip, ctypes.POINTER(ctypes.c_double)]),
             "moist_adjoint_grad": (ip, [vp, ip, vp, ctypes.POINTER(ctypes.c_double), ip]),
             "moist_adjoint_times": (ip, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+            "moist_device_bytes": (ip, [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
             "moist_last_error": (ctypes.c_char_p, [vp]),
             "moist_destroy": (None, [vp]),
         }
@@ -184,6 +185,12 @@ class Engine:
         """(forward ms, reverse ms) of the last adjoint run."""
         a, b = ctypes.c_double(), ctypes.c_double()
         self._check(lib().moist_adjoint_times(self._h, ctypes.byref(a), ctypes.byref(b)), "moist_adjoint_times")
+        return a.value, b.value
+
+    def device_bytes(self):
+        """(state bytes, adjoint bytes) of device memory held by the handle."""
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        self._check(lib().moist_device_bytes(self._h, ctypes.byref(a), ctypes.byref(b)), "moist_device_bytes")
         return a.value, b.value
 
     def kernel_stats(self):

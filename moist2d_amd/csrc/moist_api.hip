@@ -114,6 +114,7 @@ struct moist_handle {
     int64_t ac_launches = 0;
     // reverse mode (allocated by the first moist_adjoint_run)
     AdjState *adj = nullptr;
+    int64_t dev_bytes = 0;  // device memory of the state pools and tables
 };
 
 namespace {
@@ -519,6 +520,7 @@ struct AdjState {
     double *lam = nullptr, *lam2 = nullptr, *ls0 = nullptr, *lph = nullptr, *wk = nullptr, *lim = nullptr;
     double *gk = nullptr, *tgt = nullptr, *fin = nullptr, *J = nullptr, *gK = nullptr;
     std::vector<double *> allocs;
+    size_t bytes = 0;
     std::vector<double> J_h, gK_h;
     double ms_fwd = 0.0, ms_bwd = 0.0;
     int sub_off[3] = {0, 0, 0};  // first substep record of each stage
@@ -552,6 +554,7 @@ int adj_alloc(moist_handle *h, int nsteps, int ckpt) {
     auto al = [&](double *&p, size_t n) -> int {
         HIPCHK(h, hipMalloc((void **)&p, n * sizeof(double)));
         A.allocs.push_back(p);
+        A.bytes += n * sizeof(double);
         return 0;
     };
     if (al(A.cp, A.cp_stride * A.ncp) || al(A.seg, 7 * n2 * (size_t)ckpt) || al(A.SC, 7 * n2) ||
@@ -924,7 +927,11 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
     h->nctiles = (int)tc.size();
 
     moist_handle *hp = h.get();
-#define ALLOC(ptr, bytes) HIPCHK(hp, hipMalloc((void **)&(ptr), (bytes)))
+#define ALLOC(ptr, bytes)                                          \
+    do {                                                           \
+        HIPCHK(hp, hipMalloc((void **)&(ptr), (bytes)));           \
+        hp->dev_bytes += (int64_t)(bytes);                         \
+    } while (0)
     HIPCHK(hp, hipStreamCreate(&hp->stream));
     ALLOC(hp->dom_d, sizeof(Dom) * n_members);
     ALLOC(hp->tiles2d, sizeof(int4) * t2.size());
@@ -1183,6 +1190,13 @@ int moist_adjoint_grad(moist_handle *h, int member, void *const grad[7], double 
     }
     if (grad_params)
         for (int j = 0; j < 3; ++j) grad_params[j] = h->adj->gK_h[3 * member + j];
+    return 0;
+}
+
+int moist_device_bytes(moist_handle *h, int64_t *state_bytes, int64_t *adjoint_bytes) {
+    if (!h) return -2;
+    if (state_bytes) *state_bytes = h->dev_bytes;
+    if (adjoint_bytes) *adjoint_bytes = h->adj ? (int64_t)h->adj->bytes : 0;
     return 0;
 }
 
