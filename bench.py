@@ -71,18 +71,34 @@ def measured_traffic(kernel, nx, nz):
     return t["traffic_bytes"]
 
 
+AUTOTUNE = True
+
+
+def tune(eng):
+    """Pick the acoustic kernel's tile shape for this handle (None when disabled)."""
+    if not AUTOTUNE:
+        return None
+    L, NT, us = eng.autotune(reps=10)
+    return {"levels_per_chunk": L, "threads": NT, "us_per_launch": us}
+
+
 def run_form(p, s, precision, steps, warmup, graph=True):
     """Warm up on a throwaway engine, then time `steps` large steps from t = 0."""
+    shape = None
     if warmup:
         eng = core.Engine(p, precision)
         eng.store_phys_diag(False)
         eng.upload(s)
+        shape = tune(eng)
         eng.use_graph(graph)
         eng.time_steps(warmup)
         eng.close()
     eng = core.Engine(p, precision)
     eng.store_phys_diag(False)
     eng.upload(s)
+    if shape:
+        eng.set_acoustic_shape(shape["levels_per_chunk"], shape["threads"])
+    eng.shape = shape
     eng.use_graph(graph)
     t0 = time.perf_counter()
     ms = eng.time_steps(steps)
@@ -90,10 +106,12 @@ def run_form(p, s, precision, steps, warmup, graph=True):
     return eng, ms, wall
 
 
-def kernel_pass(p, s, precision, steps):
+def kernel_pass(p, s, precision, steps, shape=None):
     eng = core.Engine(p, precision)
     eng.store_phys_diag(False)
     eng.upload(s)
+    if shape:
+        eng.set_acoustic_shape(shape["levels_per_chunk"], shape["threads"])
     eng.kernel_events(True)
     eng.time_steps(1)
     eng.time_steps(steps)
@@ -115,12 +133,13 @@ def ensemble_run(steps, members=256, seed=2):
         eng.store_phys_diag(False)
         for m, (_, s) in enumerate(mem):
             eng.upload(s, member=m)
+        shape = tune(eng)
         eng.use_graph(True)
         eng.time_steps(2)
         ms = eng.time_steps(steps)
         d = [eng.diagnostics(m) for m in (0, members - 1)]
         eng.close()
-        out[name] = {"cell_steps_per_s": cells * steps / (ms * 1e-3), "ms_per_step": ms / steps,
+        out[name] = {"cell_steps_per_s": cells * steps / (ms * 1e-3), "ms_per_step": ms / steps, "acoustic_shape": shape,
                      "first_last_member_total_water": [x["total_water"] for x in d]}
     return out
 
@@ -149,17 +168,21 @@ def _loss_np(p, a, tt):
     return float(p.dx) * a.prec[h:h + p.Nx].sum() + 0.5 * np.sum((th - tt[:, h:h + p.Nx]) ** 2)
 
 
-def adjoint_bench(steps=600, ckpt=50, fd_dirs=8, cpu_steps=2):
+def adjoint_bench(steps=600, ckpt=50, fd_dirs=32, cpu_steps=2):
     """The 512x128 adjoint workload (FP64 form): loss J = dx sum(prec) +
     1/2 |theta - theta_target|^2 after 600 steps, gradient w.r.t. the initial
     theta' and qv (at fixed density) and K_auto, Qc_threshold, K_accr;
-    checkpoints every 50 steps.  Central FD (h = 1e-6) on seeded smooth
-    directions of the GPU's own FP64 forward checks the gradient."""
+    checkpoints every 50 steps.  Central FD of the GPU's own FP64 forward on
+    32 seeded directions checks the gradient: smooth fields of unit RMS
+    scaled by the RMS of the initial theta' (theta - base profile) or qv,
+    step h = 1e-6 relative."""
     p, s = ic.adjoint_bubble(np.float64, steps=steps)
     tt = ic.theta_target(p)
     eng = core.Engine(p, core.FP64)
     eng.store_phys_diag(False)
     eng.use_graph(True)
+    eng.upload(s)
+    shape = tune(eng)
     eng.upload(s)
     eng.time_steps(2)
     eng.upload(s)
@@ -170,10 +193,16 @@ def adjoint_bench(steps=600, ckpt=50, fd_dirs=8, cpu_steps=2):
     g, gK = eng.adjoint_grad(0)
     st_b, ad_b = eng.device_bytes()
     rng = np.random.default_rng(100)
+    hh = p.halo
+    th0 = (s.rho_theta / s.rho)[:, hh:hh + p.Nx] - ic.theta_profile(p.z)[:, None]
+    qv0 = (s.rho_qv / s.rho)[:, hh:hh + p.Nx]
+    rms = {3: float(np.sqrt(np.mean(th0 ** 2))), 4: float(np.sqrt(np.mean(qv0 ** 2)))}
     errs = []
     for d in range(fd_dirs):
-        f, n, sc = (3, "rho_theta", 0.01) if d % 2 == 0 else (4, "rho_qv", 1e-5)
-        v = _smooth_direction(p, rng) * sc * s.rho
+        f, n = (3, "rho_theta") if d % 2 == 0 else (4, "rho_qv")
+        v = _smooth_direction(p, rng)
+        v *= rms[f] / np.sqrt(np.mean(v[:, hh:hh + p.Nx] ** 2))
+        v = v * s.rho
         eps = 1e-6
         vals = []
         for sgn in (1.0, -1.0):
@@ -190,10 +219,12 @@ def adjoint_bench(steps=600, ckpt=50, fd_dirs=8, cpu_steps=2):
         errs.append(abs(fd - ad) / abs(ad))
     eng.close()
     out = {"workload": f"adjoint_bubble_{p.Nx}x{p.Nz}", "form": "fp64", "steps": steps, "ckpt": ckpt,
+           "acoustic_shape": shape,
            "dt": float(p.dt), "ns": p.ns, "J": J, "dJ_dK": [float(x) for x in gK],
            "forward_ms": ms_plain, "forward_with_checkpoints_ms": ms_fwd, "reverse_ms": ms_rev,
            "adjoint_forward_ratio": ms_rev / ms_plain, "device_bytes": st_b + ad_b,
            "fd_check": {"directions": fd_dirs, "h": 1e-6, "controls": "theta' and qv at fixed rho",
+                        "direction_scale": {"theta_K": rms[3], "qv": rms[4]},
                         "median_rel_err": float(np.median(errs)), "max_rel_err": float(max(errs))}}
     if cpu_steps:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -225,6 +256,9 @@ def batched_adjoint_bench(members=64, steps=300, ckpt=50, seed=4):
     eng.use_graph(True)
     for m, (p, s, _) in enumerate(mem):
         eng.upload(s, member=m)
+    shape = tune(eng)
+    for m, (p, s, _) in enumerate(mem):
+        eng.upload(s, member=m)
     eng.time_steps(2)
     for m, (p, s, _) in enumerate(mem):
         eng.upload(s, member=m)
@@ -238,6 +272,7 @@ def batched_adjoint_bench(members=64, steps=300, ckpt=50, seed=4):
     eng.close()
     cells = sum(p.Nx * p.Nz for p, _, _ in mem)
     return {"workload": f"batched_adjoint_{members}x1024x128", "form": "fp64", "members": members, "cells": cells,
+            "acoustic_shape": shape,
             "steps": steps, "ckpt": ckpt, "forward_ms": ms_plain, "reverse_ms": ms_rev,
             "adjoint_forward_ratio": ms_rev / ms_plain,
             "gradients_per_s": members / ((ms_fwd + ms_rev) * 1e-3), "device_bytes": st_b + ad_b,
@@ -293,6 +328,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=30, help="0 skips the 256-member ensemble")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workloads")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default acoustic tile shape")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -301,6 +337,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    global AUTOTUNE
+    AUTOTUNE = not args.no_autotune
     if args.copy_calibration:
         copy_calibration()
     p, s = ic.coldpool(np.float64, nx=args.nx, nz=args.nz)
@@ -308,7 +346,8 @@ def main():
     K, W = args.steps, args.warmup
 
     eng_m, ms_m, wall_m = run_form(p, s, core.MIXED, K, W)
-    mixed = {"cell_steps_per_s": cells * K / (ms_m * 1e-3), "ms_per_step": ms_m / K, "wall_s": wall_m}
+    mixed = {"cell_steps_per_s": cells * K / (ms_m * 1e-3), "ms_per_step": ms_m / K, "wall_s": wall_m,
+             "acoustic_shape": eng_m.shape}
     out_m = s.copy()
     eng_m.download(out_m, phys=False)
     diag_m = eng_m.diagnostics()
@@ -319,7 +358,8 @@ def main():
     floor = None
     if not args.no_fp64:
         eng_d, ms_d, wall_d = run_form(p, s, core.FP64, K, W)
-        fp64 = {"value": cells * K / (ms_d * 1e-3), "ms_per_step": ms_d / K, "wall_s": wall_d}
+        fp64 = {"value": cells * K / (ms_d * 1e-3), "ms_per_step": ms_d / K, "wall_s": wall_d,
+                "acoustic_shape": eng_d.shape}
         out_d = s.copy()
         eng_d.download(out_d, phys=False)
         diag_d = eng_d.diagnostics()
@@ -348,10 +388,11 @@ def main():
         floor["what"] = "FP64 form from the FP32-rounded initial state vs FP64 form, same time"
 
     # dominant kernel: acoustic column kernel, timed live with HIP events
-    kms, nlaunch = kernel_pass(p, s, core.MIXED, min(K, 40))
+    kms, nlaunch = kernel_pass(p, s, core.MIXED, min(K, 40), mixed["acoustic_shape"])
     alg_bytes = ACOUSTIC_PASSES * cells * 4
     achieved = alg_bytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
-    kname = "mo::k_acoustic<float, 4>"
+    sh = mixed["acoustic_shape"] or {"levels_per_chunk": 4, "threads": 1024}
+    kname = f"mo::k_acoustic<float, {sh['levels_per_chunk']}, {sh['threads']}>"
     roof = {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname, p.Nx, p.Nz),
             "avg_launch_ms": kms, "launches": nlaunch, "alg_bytes_per_launch": alg_bytes}
@@ -389,6 +430,7 @@ def main():
         "mixed_vs_fp64": err,
         "fp64_sensitivity_floor": floor,
         "mixed_wall_s": mixed["wall_s"],
+        "acoustic_shape": mixed["acoustic_shape"],
         "roofline": roof,
         "cpu_baseline": cpu,
         "ensemble": ens,

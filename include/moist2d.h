@@ -144,6 +144,19 @@ int moist_kernel_events(moist_handle *h, int on);
 int moist_kernel_stats(moist_handle *h, double *acoustic_ms_per_launch,
                        int64_t *acoustic_launches);
 
+/* Tile-shape autotuner (no counterpart in the reference, whose loops have
+ * no tiles): times the acoustic substep kernel on this handle's domains for
+ * every shape of L levels per chunk (2, 4) and workgroup size (256, 512,
+ * 1024 threads) that fits every member, `reps` launches each, and keeps the
+ * fastest.  Outputs (may be NULL): the chosen L and threads and the
+ * microseconds per launch.  Opt-in: different shapes combine the chunked
+ * tridiagonal solve in a different order (results differ at rounding
+ * level), so a handle keeps its default shape unless this is called. */
+int moist_autotune(moist_handle *h, int reps, int *levels, int *threads, double *us_per_launch);
+/* Sets the acoustic kernel's shape explicitly (levels 2 or 4, threads 256,
+ * 512 or 1024). */
+int moist_set_acoustic_shape(moist_handle *h, int levels, int threads);
+
 /* Reverse mode (FP64-form handle, do_sf = do_rd = false).  Replaces
  * ForwardDiff / a ChainRulesCore rrule around the reference's driver loop
  * (run_sim, moist.jl:1930-1963, calling rk3_step!, moist.jl:1589): runs

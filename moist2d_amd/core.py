@@ -57,6 +57,8 @@ def lib():
             "moist_adjoint_run": (ip, [vp, ip, ip, vp, ip, ctypes.POINTER(ctypes.c_double)]),
             "moist_adjoint_grad": (ip, [vp, ip, vp, ctypes.POINTER(ctypes.c_double), ip]),
             "moist_adjoint_times": (ip, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+            "moist_autotune": (ip, [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.POINTER(ctypes.c_double)]),
+            "moist_set_acoustic_shape": (ip, [vp, ip, ip]),
             "moist_device_bytes": (ip, [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
             "moist_last_error": (ctypes.c_char_p, [vp]),
             "moist_destroy": (None, [vp]),
@@ -186,6 +188,17 @@ class Engine:
         a, b = ctypes.c_double(), ctypes.c_double()
         self._check(lib().moist_adjoint_times(self._h, ctypes.byref(a), ctypes.byref(b)), "moist_adjoint_times")
         return a.value, b.value
+
+    def autotune(self, reps=20):
+        """Time every acoustic-kernel shape on this handle's domains and keep the
+        fastest; returns (levels per chunk, threads per workgroup, us per launch)."""
+        L, NT, us = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._check(lib().moist_autotune(self._h, reps, ctypes.byref(L), ctypes.byref(NT), ctypes.byref(us)),
+                    "moist_autotune")
+        return L.value, NT.value, us.value
+
+    def set_acoustic_shape(self, levels, threads):
+        self._check(lib().moist_set_acoustic_shape(self._h, levels, threads), "moist_set_acoustic_shape")
 
     def device_bytes(self):
         """(state bytes, adjoint bytes) of device memory held by the handle."""

@@ -2,11 +2,12 @@
 //
 // Included by moist_kernels.h inside namespace mo.
 //
-// A workgroup of AC_THREADS threads owns a tile of TX columns and every level
+// A workgroup of NT threads owns a tile of TX columns and every level
 // of those columns.  Thread t handles column t % TX and chunk t / TX, a run
 // of L consecutive levels (and the faces above them) kept in registers, so
 // one wave reads 64/TX level rows of TX consecutive columns (full 128-byte
-// lines for TX >= 32 in FP32).  nch = AC_THREADS / TX chunks cover nz.
+// lines for TX >= 32 in FP32).  nch = NT / TX chunks cover nz.  L and NT are
+// template parameters (the runtime picks them per handle, or times them).
 //
 // Per substep, for every cell of the chunk:
 //   rho_u' at faces i-1/2 and i+1/2 with divergence damping (moist.jl:1360-1369;
@@ -34,7 +35,6 @@
 #define MOIST_AC_THREADS 1024
 #endif
 constexpr int AC_THREADS = MOIST_AC_THREADS;
-constexpr int AC_L = 8;  // levels per chunk
 
 template <class S> struct AcBuf {
     const S *rup_in;   // rho_u' before this substep
@@ -46,13 +46,13 @@ template <class S> struct AcBuf {
     int64_t cin_stride;
 };
 
-template <class S, int L>
-__global__ void __launch_bounds__(AC_THREADS) k_acoustic(const Dom *__restrict__ doms, const int4 *__restrict__ atiles,
-                                                         ZVec zv, SSet<S> in, Work<S> w, AcBuf<S> ab, int div,
-                                                         double dt_ovr, int nsub, int first) {
+template <class S, int L, int NT>
+__global__ void __launch_bounds__(NT) k_acoustic(const Dom *__restrict__ doms, const int4 *__restrict__ atiles,
+                                                 ZVec zv, SSet<S> in, Work<S> w, AcBuf<S> ab, int div,
+                                                 double dt_ovr, int nsub, int first) {
     using R = S;
-    __shared__ double lds[8 * AC_THREADS];  // two buffers of 4 doubles per thread for the scans
-    __shared__ S ldt[2 * AC_THREADS];
+    __shared__ double lds[8 * NT];  // two buffers of 4 doubles per thread for the scans
+    __shared__ S ldt[2 * NT];
 
     const int4 tl = atiles[blockIdx.x];
     if (tl.x < 0) return;  // padding tile of the XCD ordering (uniform per block)
@@ -168,11 +168,11 @@ __global__ void __launch_bounds__(AC_THREADS) k_acoustic(const Dom *__restrict__
 
     // ---- LDS round A: off-centred values of the first level of each chunk
     ldt[t] = tp_r0;
-    ldt[AC_THREADS + t] = tp_t0;
+    ldt[NT + t] = tp_t0;
     __syncthreads();
     if (nl == L && k0 + L < nz) {
         const int k = k0 + L;
-        const R up_r = ldt[t + TX], up_t = ldt[AC_THREADS + t + TX];
+        const R up_r = ldt[t + TX], up_t = ldt[NT + t + TX];
         const R buoy = fdiv(up_t - tp_tp, R(zd[k]) - R(zd[k - 1]));
         const R grav = -K<R>::GRAV * R(0.5) * (up_r + tp_rp);
         const R rt2h = (rt2c[L - 1] + rt2c[L]) * R(0.5);
@@ -241,7 +241,7 @@ __global__ void __launch_bounds__(AC_THREADS) k_acoustic(const Dom *__restrict__
         }
         // inclusive Hillis-Steele scan over the chunks of each column:
         // S_c = M_c S_{c-1} ... M_0; the c' entering chunk c is S_{c-1} applied to (0, 1)
-        double *A = lds, *B = lds + 4 * AC_THREADS;
+        double *A = lds, *B = lds + 4 * NT;
         A[4 * t + 0] = m00;
         A[4 * t + 1] = m01;
         A[4 * t + 2] = m10;
@@ -307,7 +307,7 @@ __global__ void __launch_bounds__(AC_THREADS) k_acoustic(const Dom *__restrict__
     // carries of the forward substitution: inclusive scan of the chunk maps
     // x -> y + P x from the bottom; the carry into chunk c is S_{c-1}(0)
     {
-        double *A = lds, *B = lds + 4 * AC_THREADS;
+        double *A = lds, *B = lds + 4 * NT;
         A[2 * t + 0] = ylast;
         A[2 * t + 1] = plast;
         __syncthreads();
@@ -358,7 +358,7 @@ __global__ void __launch_bounds__(AC_THREADS) k_acoustic(const Dom *__restrict__
     // first face, the carry into chunk c is T_{c+1}(0)
     double xin, x_below = 0.0;
     {
-        double *A = lds, *B = lds + 4 * AC_THREADS;
+        double *A = lds, *B = lds + 4 * NT;
         A[2 * t + 0] = zfirst;
         A[2 * t + 1] = qfirst;
         __syncthreads();

@@ -448,21 +448,27 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     }
 }
 
-// Wave-wide inclusive scans (lane 0 first / lane 63 first) of affine maps
-// x -> A x + B and of 2x2 matrices (Moebius maps, scaled to unit max norm).
-__device__ __forceinline__ void scan_affine_up(double &A, double &B, int lane) {
-    for (int d = 1; d < 64; d <<= 1) {
-        const double Ap = __shfl_up(A, d), Bp = __shfl_up(B, d);
-        if (lane >= d) {
+// Inclusive scans over the CH chunks of one column inside a wave (lane =
+// column + CW * chunk, CW = 64 / CH): affine maps x -> A x + B from the
+// bottom chunk up or from the top chunk down, and 2x2 matrices (Moebius
+// maps, scaled to unit max norm).
+template <int CH> __device__ __forceinline__ void scan_affine_up(double &A, double &B, int ch) {
+    constexpr int CW = 64 / CH;
+#pragma unroll
+    for (int d = 1; d < CH; d <<= 1) {
+        const double Ap = __shfl_up(A, CW * d), Bp = __shfl_up(B, CW * d);
+        if (ch >= d) {
             B = A * Bp + B;
             A = A * Ap;
         }
     }
 }
-__device__ __forceinline__ void scan_affine_down(double &A, double &B, int lane) {
-    for (int d = 1; d < 64; d <<= 1) {
-        const double Ap = __shfl_down(A, d), Bp = __shfl_down(B, d);
-        if (lane + d < 64) {
+template <int CH> __device__ __forceinline__ void scan_affine_down(double &A, double &B, int ch) {
+    constexpr int CW = 64 / CH;
+#pragma unroll
+    for (int d = 1; d < CH; d <<= 1) {
+        const double Ap = __shfl_down(A, CW * d), Bp = __shfl_down(B, CW * d);
+        if (ch + d < CH) {
             B = A * Bp + B;
             A = A * Ap;
         }
@@ -479,32 +485,40 @@ __device__ __forceinline__ void mob_norm(double &m00, double &m01, double &m10, 
     }
 }
 
-// (2) Transposed solve A^T y = Lx of every column, one wave per column
-// (lane l owns rows l*L .. l*L+L-1).  With the forward factorization A = L U
-// (L: diagonal m_j, sub-diagonal a_j; U: unit, super-diagonal c'_j):
-// U^T w = Lx is w_j = Lx_j - c'_j-1 w_j-1, L^T y = w is
-// y_j = (w_j - a_j+1 y_j+1) / m_j; both are affine scans.  `factor` (first
-// reverse substep of a stage) first computes c'_j = c_j / (b_j + 1 - a_j c'_j-1)
-// as a Moebius scan and stores c', 1/m and a.
+// (2) Transposed solve A^T y = Lx of every column.  A wave owns CW = 64/CH
+// adjacent columns, each split into CH chunks of rows (lane = column +
+// CW * chunk), so a load instruction touches CH runs of CW adjacent columns.
+// With the forward factorization A = L U (L: diagonal m_j, sub-diagonal a_j;
+// U: unit, super-diagonal c'_j): U^T w = Lx is w_j = Lx_j - c'_j-1 w_j-1,
+// L^T y = w is y_j = (w_j - a_j+1 y_j+1) / m_j; both are affine scans over
+// the chunks.  `factor` (first reverse substep of a stage) first computes
+// c'_j = c_j / (b_j + 1 - a_j c'_j-1) with a Moebius scan and stores c', 1/m
+// and a.  groups: {member, first column} of every wave.
+template <int CH>
 __global__ void __launch_bounds__(256)
-    ka_ac_solve(const Dom *__restrict__ doms, const int2 *__restrict__ wcols, int ncols, ZVec zv, AdjW a, int div,
+    ka_ac_solve(const Dom *__restrict__ doms, const int2 *__restrict__ groups, int ngroups, ZVec zv, AdjW a, int div,
                 int nsub, int factor) {
+    constexpr int CW = 64 / CH;
     const int wid = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (wid >= ncols) return;
-    const int2 t = wcols[wid];
+    if (wid >= ngroups) return;
+    const int2 t = groups[wid];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
     const int nx = D.nx, nz = D.nz, n = nz - 1;
     if (n <= 0) return;
+    const int col = lane % CW, ch = lane / CW;
+    const int i = t.y + col;
+    const bool live = i < nx;
     const double dt_s = D.dt / double(div) / double(nsub);
     const double fact_tp = 0.5 * (1.0 + D.beta);
-    const WpCoef W{D, zv.z + D.offz, zv.dz + D.offz, a.theta, a.rt2, nx, t.y, (dt_s * fact_tp) * (dt_s * fact_tp)};
-    const int L = (n + 63) / 64, j0 = lane * L;
+    const WpCoef W{D, zv.z + D.offz, zv.dz + D.offz, a.theta, a.rt2, nx, live ? i : 0,
+                   (dt_s * fact_tp) * (dt_s * fact_tp)};
+    const int L = (n + CH - 1) / CH, j0 = ch * L, j1 = live ? min(j0 + L, n) : j0;
     if (factor) {
-        double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;  // lane composite, identity
-        for (int j = j0; j < min(j0 + L, n); ++j) {
+        double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;
+        for (int j = j0; j < j1; ++j) {
             const double aj = j > 0 ? W.a(j) : 0.0, bj = W.b(j) + 1.0, cj = j < n - 1 ? W.c(j) : 0.0;
-            // M_j = [[0, c], [-a, b+1]] applied after the composite
+            // M_j = [[0, c], [-a, b+1]] applied after the chunk composite
             const double n00 = cj * m10, n01 = cj * m11;
             const double n10 = -aj * m00 + bj * m10, n11 = -aj * m01 + bj * m11;
             m00 = n00;
@@ -513,10 +527,11 @@ __global__ void __launch_bounds__(256)
             m11 = n11;
             mob_norm(m00, m01, m10, m11);
         }
-        for (int d = 1; d < 64; d <<= 1) {
-            const double p00 = __shfl_up(m00, d), p01 = __shfl_up(m01, d), p10 = __shfl_up(m10, d),
-                         p11 = __shfl_up(m11, d);
-            if (lane >= d) {
+#pragma unroll
+        for (int d = 1; d < CH; d <<= 1) {
+            const double p00 = __shfl_up(m00, CW * d), p01 = __shfl_up(m01, CW * d), p10 = __shfl_up(m10, CW * d),
+                         p11 = __shfl_up(m11, CW * d);
+            if (ch >= d) {
                 const double n00 = m00 * p00 + m01 * p10, n01 = m00 * p01 + m01 * p11;
                 const double n10 = m10 * p00 + m11 * p10, n11 = m10 * p01 + m11 * p11;
                 m00 = n00;
@@ -526,10 +541,10 @@ __global__ void __launch_bounds__(256)
                 mob_norm(m00, m01, m10, m11);
             }
         }
-        // c' entering this lane: composite of the lanes before, applied to c'_-1 = 0 (p, q) = (0, 1)
-        const double e01 = __shfl_up(m01, 1), e11 = __shfl_up(m11, 1);
-        double cprev = lane == 0 ? 0.0 : e01 / e11;
-        for (int j = j0; j < min(j0 + L, n); ++j) {
+        // c' entering this chunk: the chunks below applied to c'_-1 = 0, i.e. (p, q) = (0, 1)
+        const double e01 = __shfl_up(m01, CW), e11 = __shfl_up(m11, CW);
+        double cprev = ch == 0 ? 0.0 : e01 / e11;
+        for (int j = j0; j < j1; ++j) {
             const double aj = j > 0 ? W.a(j) : 0.0;
             const double m = W.b(j) + 1.0 - aj * cprev;
             const double rm = 1.0 / m;
@@ -542,34 +557,32 @@ __global__ void __launch_bounds__(256)
     }
     // forward sweep w_j = Lx_j - c'_j-1 w_j-1
     double A = 1.0, B = 0.0;
-    for (int j = j0; j < min(j0 + L, n); ++j) {
+    for (int j = j0; j < j1; ++j) {
         const double al = j > 0 ? -a.cpr[W.G(j - 1)] : 0.0;
         B = al * B + a.Lrwp[W.G(j)];
         A = al * A;
     }
-    scan_affine_up(A, B, lane);
-    double win = __shfl_up(B, 1);
-    if (lane == 0) win = 0.0;
-    double wv = win;
-    for (int j = j0; j < min(j0 + L, n); ++j) {
+    scan_affine_up<CH>(A, B, ch);
+    double wv = __shfl_up(B, CW);
+    if (ch == 0) wv = 0.0;
+    for (int j = j0; j < j1; ++j) {
         const double al = j > 0 ? -a.cpr[W.G(j - 1)] : 0.0;
         wv = al * wv + a.Lrwp[W.G(j)];
         a.Y[W.G(j)] = wv;  // w, replaced by y below
     }
-    // back substitution y_j = rm_j w_j - rm_j a_j+1 y_j+1, lanes from the top
+    // back substitution y_j = rm_j w_j - rm_j a_j+1 y_j+1, chunks from the top
     A = 1.0;
     B = 0.0;
-    for (int j = min(j0 + L, n) - 1; j >= j0; --j) {
+    for (int j = j1 - 1; j >= j0; --j) {
         const double rm = a.rmi[W.G(j)];
         const double al = j < n - 1 ? -rm * a.aco[W.G(j + 1)] : 0.0;
         B = al * B + rm * a.Y[W.G(j)];
         A = al * A;
     }
-    scan_affine_down(A, B, lane);
-    double yin = __shfl_down(B, 1);
-    if (lane == 63) yin = 0.0;
-    double yv = yin;
-    for (int j = min(j0 + L, n) - 1; j >= j0; --j) {
+    scan_affine_down<CH>(A, B, ch);
+    double yv = __shfl_down(B, CW);
+    if (ch == CH - 1) yv = 0.0;
+    for (int j = j1 - 1; j >= j0; --j) {
         const double rm = a.rmi[W.G(j)];
         const double al = j < n - 1 ? -rm * a.aco[W.G(j + 1)] : 0.0;
         yv = al * yv + rm * a.Y[W.G(j)];

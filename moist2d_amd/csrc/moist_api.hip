@@ -84,8 +84,10 @@ struct moist_handle {
     int2 *ctiles = nullptr;
     int nctiles = 0;
     int4 *atiles = nullptr;  // acoustic tiles {member, i0, TX, nch}
-    int2 *wcols = nullptr;   // every column {member, i} (one wave each in the adjoint's column solve)
-    int nwcols = 0;
+    int ac_L = 4, ac_NT = 1024;  // levels per chunk and threads per workgroup of the acoustic kernel
+    int tuned = 0;
+    int2 *wgroups = nullptr;  // {member, i0} of the adjoint column solve's waves
+    int nwgroups = 0, wch = 8;  // chunks per column in those waves
     int natiles = 0;
     int nch_max = 0;
     double *cin = nullptr;   // c' entering each acoustic chunk, [chunk][column]
@@ -210,6 +212,71 @@ hipEvent_t next_event(moist_handle *h) {
     return h->ev_pool[h->ev_used++];
 }
 
+// ---- acoustic kernel shapes ------------------------------------------------------
+// Tiles {member, i0, TX, nch} for L levels per chunk and NT threads per
+// workgroup: nch = smallest power of two with nch * L >= nz, TX = NT / nch.
+int make_atiles(const std::vector<Dom> &doms, int L, int NT, std::vector<int4> &out, int &nch_max) {
+    out.clear();
+    nch_max = 0;
+    for (size_t m = 0; m < doms.size(); ++m) {
+        const Dom &D = doms[m];
+        int nch = 1;
+        while (nch * L < D.nz) nch *= 2;
+        if (nch > NT) return -1;
+        const int TX = NT / nch;
+        nch_max = std::max(nch_max, nch);
+        for (int i0 = 0; i0 < D.nx; i0 += TX) out.push_back(make_int4((int)m, i0, TX, nch));
+    }
+    out = xcd_order(out, make_int4(-1, 0, 0, 0));
+    return 0;
+}
+
+int set_acoustic_shape(moist_handle *h, int L, int NT) {
+    std::vector<int4> ta;
+    int nch_max = 0;
+    if (make_atiles(h->dom_h, L, NT, ta, nch_max)) {
+        h->err = "nz too large for the acoustic kernel shape";
+        return -2;
+    }
+    if (nch_max > h->nch_max) {
+        h->err = "acoustic shape needs more chunks than allocated";
+        return -2;
+    }
+    if (h->atiles) HIPCHK(h, hipFree(h->atiles));
+    h->atiles = nullptr;
+    HIPCHK(h, hipMalloc((void **)&h->atiles, sizeof(int4) * ta.size()));
+    HIPCHK(h, hipMemcpy(h->atiles, ta.data(), sizeof(int4) * ta.size(), hipMemcpyHostToDevice));
+    h->natiles = (int)ta.size();
+    h->ac_L = L;
+    h->ac_NT = NT;
+    if (h->gexec) {
+        hipGraphExecDestroy(h->gexec);
+        h->gexec = nullptr;
+    }
+    return 0;
+}
+
+template <class S, int L, int NT>
+void launch_ac(moist_handle *h, const ZVec &zv, const SSet<S> &in, const Work<S> &w, const AcBuf<S> &ab, int div,
+               double dt_ovr, int nsub, int first) {
+    hipLaunchKernelGGL((k_acoustic<S, L, NT>), dim3(h->natiles), dim3(NT), 0, h->stream, h->dom_d, h->atiles, zv, in,
+                       w, ab, div, dt_ovr, nsub, first);
+}
+
+template <class S>
+void launch_acoustic(moist_handle *h, const ZVec &zv, const SSet<S> &in, const Work<S> &w, const AcBuf<S> &ab,
+                     int div, double dt_ovr, int nsub, int first) {
+    const int key = h->ac_L * 10000 + h->ac_NT;
+    switch (key) {
+    case 2 * 10000 + 256: launch_ac<S, 2, 256>(h, zv, in, w, ab, div, dt_ovr, nsub, first); break;
+    case 2 * 10000 + 512: launch_ac<S, 2, 512>(h, zv, in, w, ab, div, dt_ovr, nsub, first); break;
+    case 2 * 10000 + 1024: launch_ac<S, 2, 1024>(h, zv, in, w, ab, div, dt_ovr, nsub, first); break;
+    case 4 * 10000 + 256: launch_ac<S, 4, 256>(h, zv, in, w, ab, div, dt_ovr, nsub, first); break;
+    case 4 * 10000 + 512: launch_ac<S, 4, 512>(h, zv, in, w, ab, div, dt_ovr, nsub, first); break;
+    default: launch_ac<S, 4, 1024>(h, zv, in, w, ab, div, dt_ovr, nsub, first); break;
+    }
+}
+
 // ---- one RK stage ------------------------------------------------------------
 template <class S>
 void enqueue_stage(moist_handle *h, int in_set, int out_set, int s0_set, int div, double dt_ovr, int nsub,
@@ -242,8 +309,7 @@ void enqueue_stage(moist_handle *h, int in_set, int out_set, int s0_set, int div
             e1 = next_event(h);
             if (e0) hipEventRecord(e0, h->stream);
         }
-        hipLaunchKernelGGL((k_acoustic<S, ac_levels<S>()>), dim3(h->natiles), dim3(AC_THREADS), 0, h->stream,
-                           h->dom_d, h->atiles, zv, in, w, ab, div, dt_ovr, nsub, n == 0 ? 1 : 0);
+        launch_acoustic<S>(h, zv, in, w, ab, div, dt_ovr, nsub, n == 0 ? 1 : 0);
         if (h->ev_on && e1) hipEventRecord(e1, h->stream);
         old = cur;
         cur = nxt;
@@ -516,7 +582,13 @@ enum { REC_TH = 0, REC_PRES, REC_RT2, REC_RR, REC_RRU, REC_RRW, REC_RRT, REC_MX,
 struct AdjState {
     int nsteps = 0, ckpt = 0, ncp = 0;
     size_t cp_stride = 0;
-    double *cp = nullptr, *seg = nullptr, *SC = nullptr, *rst = nullptr, *rsub = nullptr;
+    double *cp = nullptr, *seg = nullptr, *SC = nullptr;
+    // forward records: blocks of {stage-2 input, stage-3 input, stage records,
+    // substep records}; one block per step of a segment when they fit in
+    // memory (`segrec`), else one block recomputed per step
+    double *recs = nullptr, *cur = nullptr;
+    size_t rb_stride = 0;
+    int segrec = 0, nblocks = 1;
     double *lam = nullptr, *lam2 = nullptr, *ls0 = nullptr, *lph = nullptr, *wk = nullptr, *lim = nullptr;
     double *gk = nullptr, *tgt = nullptr, *fin = nullptr, *J = nullptr, *gK = nullptr;
     std::vector<double *> allocs;
@@ -551,6 +623,17 @@ int adj_alloc(moist_handle *h, int nsteps, int ckpt) {
         A.sub_off[st] = nrec_sub;
         nrec_sub += h->ns_stage[st] + 1;
     }
+    A.rb_stride = (14 + 3 * (size_t)NREC + 3 * (size_t)nrec_sub) * n2;
+    {
+        // records of a whole segment if they take at most a quarter of the free memory
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess) {
+            const size_t need = A.rb_stride * sizeof(double) * (size_t)ckpt;
+            const size_t other = (A.cp_stride * A.ncp + 7 * n2 * (size_t)(ckpt + 1) + 60 * n2) * sizeof(double);
+            A.segrec = need + other < fr / 4 ? 1 : 0;
+        }
+        A.nblocks = A.segrec ? ckpt : 1;
+    }
     auto al = [&](double *&p, size_t n) -> int {
         HIPCHK(h, hipMalloc((void **)&p, n * sizeof(double)));
         A.allocs.push_back(p);
@@ -558,11 +641,12 @@ int adj_alloc(moist_handle *h, int nsteps, int ckpt) {
         return 0;
     };
     if (al(A.cp, A.cp_stride * A.ncp) || al(A.seg, 7 * n2 * (size_t)ckpt) || al(A.SC, 7 * n2) ||
-        al(A.rst, 3 * NREC * n2) || al(A.rsub, 3 * n2 * (size_t)nrec_sub) || al(A.lam, 7 * n2) ||
+        al(A.recs, A.rb_stride * (size_t)A.nblocks) || al(A.lam, 7 * n2) ||
         al(A.lam2, 7 * n2) || al(A.ls0, 7 * n2) || al(A.lph, 7 * n2) || al(A.wk, NWORK * n2) ||
         al(A.lim, 6 * h->nxs) || al(A.gk, 3 * n2) || al(A.tgt, n2) || al(A.fin, A.cp_stride) || al(A.J, h->nm) ||
         al(A.gK, 3 * h->nm))
         return -1;
+    A.cur = A.recs;
     A.J_h.assign(h->nm, 0.0);
     A.gK_h.assign(3 * h->nm, 0.0);
     return 0;
@@ -574,11 +658,17 @@ SSet<double> dset(double *base, size_t n2) {
     return s;
 }
 
-// substep records of stage st: rho_theta', rho_u', rho_w' after substep n (n = 0: initial)
-double *rec_rtp(moist_handle *h, int st, int n) { return h->adj->rsub + ((size_t)3 * (h->adj->sub_off[st] + n) + 0) * h->n2; }
-double *rec_rup(moist_handle *h, int st, int n) { return h->adj->rsub + ((size_t)3 * (h->adj->sub_off[st] + n) + 1) * h->n2; }
-double *rec_rwp(moist_handle *h, int st, int n) { return h->adj->rsub + ((size_t)3 * (h->adj->sub_off[st] + n) + 2) * h->n2; }
-double *rec_st(moist_handle *h, int st, int r) { return h->adj->rst + ((size_t)st * NREC + r) * h->n2; }
+// Layout of a record block (current block h->adj->cur): stage-2 input (7
+// fields), stage-3 input (7), per stage the NREC stage records, then per
+// stage and substep n (n = 0: initial) rho_theta', rho_u', rho_w' after it.
+double *rec_in(moist_handle *h, int st) { return h->adj->cur + (size_t)(st - 1) * 7 * h->n2; }  // st = 1, 2
+double *rec_st(moist_handle *h, int st, int r) { return h->adj->cur + (14 + (size_t)st * NREC + r) * h->n2; }
+double *rec_sub(moist_handle *h, int st, int n, int f) {
+    return h->adj->cur + (14 + 3 * (size_t)NREC + (size_t)3 * (h->adj->sub_off[st] + n) + f) * h->n2;
+}
+double *rec_rtp(moist_handle *h, int st, int n) { return rec_sub(h, st, n, 0); }
+double *rec_rup(moist_handle *h, int st, int n) { return rec_sub(h, st, n, 1); }
+double *rec_rwp(moist_handle *h, int st, int n) { return rec_sub(h, st, n, 2); }
 double *wk(moist_handle *h, int w) { return h->adj->wk + (size_t)w * h->n2; }
 
 AdjW adj_w(moist_handle *h, int st) {
@@ -657,8 +747,7 @@ void enqueue_stage_rec(moist_handle *h, int st, SSet<double> in, SSet<double> ou
         ab.rtp_out = rec_rtp(h, st, n + 1);
         ab.cin = h->cin;
         ab.cin_stride = h->nxs;
-        hipLaunchKernelGGL((k_acoustic<double, ac_levels<double>()>), dim3(h->natiles), dim3(AC_THREADS), 0,
-                           h->stream, h->dom_d, h->atiles, zv, in, w, ab, div, 0.0, nsub, n == 0 ? 1 : 0);
+        launch_acoustic<double>(h, zv, in, w, ab, div, 0.0, nsub, n == 0 ? 1 : 0);
         copy_set(h, rec_rwp(h, st, n + 1), w.rwp, h->n2);
     }
     hipLaunchKernelGGL(k_scalar<double>, dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles, zv, in,
@@ -666,36 +755,36 @@ void enqueue_stage_rec(moist_handle *h, int st, SSet<double> in, SSet<double> ou
                        st);
 }
 
-// Forward pass of one step from the state `sn` with its record: the stage
-// inputs stay in S0 (= sn), SA, SB; stage 3 writes the separate set SC.
+// Forward pass of one step from the state `sn` into the current record
+// block: the stage-2 and -3 inputs are kept in the block; stage 3 writes the
+// separate set SC (the step's output).
 void enqueue_step_rec(moist_handle *h, const double *sn) {
     const size_t n2 = h->n2;
-    for (int f = 0; f < 7; ++f) copy_set(h, slot<double>(h, S0_ + f), sn + f * n2, n2);
+    copy_set(h, slot<double>(h, S0_), sn, 7 * n2);  // slots S0_..S0_+6 are contiguous
     const ZVec zv = zvec(h);
     const Work<double> w = work<double>(h);
     hipLaunchKernelGGL(k_physics<double>, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d,
                        h->tiles2d, zv, sset<double>(h, S0_), w, 0);
-    enqueue_stage_rec(h, 0, sset<double>(h, S0_), sset<double>(h, SA_), sset<double>(h, S0_), 3);
-    enqueue_stage_rec(h, 1, sset<double>(h, SA_), sset<double>(h, SB_), sset<double>(h, S0_), 2);
-    enqueue_stage_rec(h, 2, sset<double>(h, SB_), dset(h->adj->SC, n2), sset<double>(h, S0_), 1);
+    const SSet<double> s0 = sset<double>(h, S0_), ia = dset(rec_in(h, 1), n2), ib = dset(rec_in(h, 2), n2);
+    enqueue_stage_rec(h, 0, s0, ia, s0, 3);
+    enqueue_stage_rec(h, 1, ia, ib, s0, 2);
+    enqueue_stage_rec(h, 2, ib, dset(h->adj->SC, n2), s0, 1);
 }
 
-// Adjoint of one step: A.lam holds the adjoint of the step output on entry
-// and of the step input `sn` on exit.
+// Adjoint of one step from its record (the current block): A.lam holds the
+// adjoint of the step output on entry and of the step input `sn` on exit.
 void enqueue_step_adj(moist_handle *h, const double *sn) {
     AdjState &A = *h->adj;
     const size_t n2 = h->n2;
     const ZVec zv = zvec(h);
     const dim3 b2(TILE_X, TILE_Z);
-    enqueue_step_rec(h, sn);
     fill(h, A.ls0, 7 * n2);
     fill(h, A.lph, 7 * n2);
     const SSet<double> ls0 = dset(A.ls0, n2), lph = dset(A.lph, n2);
-    const int ins[3] = {S0_, SA_, SB_};
     const int divs[3] = {3, 2, 1};
     double *lo = A.lam, *li = A.lam2;
     for (int st = 2; st >= 0; --st) {
-        const SSet<double> in = sset<double>(h, ins[st]);
+        const SSet<double> in = dset(st == 0 ? const_cast<double *>(sn) : rec_in(h, st), n2);
         const int nsub = h->ns_stage[st], div = divs[st];
         const AdjW a = adj_w(h, st);
         for (int wi : {W_TH, W_RT, W_LRR, W_LRU, W_LRW, W_LRT, W_PEND}) fill(h, wk(h, wi), n2);
@@ -708,8 +797,15 @@ void enqueue_step_adj(moist_handle *h, const double *sn) {
             const int m = n - 1;
             hipLaunchKernelGGL(ka_ac_v1, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, a,
                                (const double *)rec_rwp(h, st, m + 1), div, nsub);
-            hipLaunchKernelGGL(ka_ac_solve, dim3((h->nwcols + 3) / 4), dim3(256), 0, h->stream, h->dom_d, h->wcols,
-                               h->nwcols, zv, a, div, nsub, n == nsub ? 1 : 0);
+            if (h->wch == 4)
+                hipLaunchKernelGGL(ka_ac_solve<4>, dim3((h->nwgroups + 3) / 4), dim3(256), 0, h->stream, h->dom_d,
+                                   h->wgroups, h->nwgroups, zv, a, div, nsub, n == nsub ? 1 : 0);
+            else if (h->wch == 64)
+                hipLaunchKernelGGL(ka_ac_solve<64>, dim3((h->nwgroups + 3) / 4), dim3(256), 0, h->stream, h->dom_d,
+                                   h->wgroups, h->nwgroups, zv, a, div, nsub, n == nsub ? 1 : 0);
+            else
+                hipLaunchKernelGGL(ka_ac_solve<8>, dim3((h->nwgroups + 3) / 4), dim3(256), 0, h->stream, h->dom_d,
+                                   h->wgroups, h->nwgroups, zv, a, div, nsub, n == nsub ? 1 : 0);
             hipLaunchKernelGGL(ka_ac_v2, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, a,
                                (const double *)rec_rtp(h, st, m), (const double *)rec_rup(h, st, m + 1),
                                (const double *)rec_rwp(h, st, m), (const double *)rec_rwp(h, st, m + 1), div, nsub);
@@ -727,7 +823,7 @@ void enqueue_step_adj(moist_handle *h, const double *sn) {
     // lo: adjoint of the stage-1 input; the step input also enters as s0 of every stage and through physics
     hipLaunchKernelGGL(ka_axpy, dim3(1024), dim3(256), 0, h->stream, lo, (const double *)A.ls0, (int64_t)(7 * n2));
     hipLaunchKernelGGL(ka_physics, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv,
-                       sset<double>(h, S0_), lph, dset(lo, n2), A.gk, A.gk + n2, A.gk + 2 * n2);
+                       dset(const_cast<double *>(sn), n2), lph, dset(lo, n2), A.gk, A.gk + n2, A.gk + 2 * n2);
     if (lo != A.lam) copy_set(h, A.lam, lo, 7 * n2);
 }
 
@@ -779,12 +875,31 @@ template <class H> int adjoint_run_t(moist_handle *h, int nsteps, int ckpt, cons
     for (int sg = A.ncp - 1; sg >= 0; --sg) {
         const int n0 = sg * ckpt, n1 = std::min(nsteps, n0 + ckpt);
         load_state(h, A.cp + A.cp_stride * sg);
-        for (int n = n0; n < n1; ++n) {
-            for (int f = 0; f < 7; ++f)
-                copy_set(h, A.seg + ((size_t)(n - n0) * 7 + f) * n2, slot<double>(h, S0_ + f), n2);
-            if (n + 1 < n1 && run_steps(h, 1)) return -1;
+        if (A.segrec) {
+            // one recording pass over the segment, then the reverse sweep from the records
+            for (int n = n0; n < n1; ++n) {
+                double *sn = A.seg + (size_t)(n - n0) * 7 * n2;
+                copy_set(h, sn, slot<double>(h, S0_), 7 * n2);
+                A.cur = A.recs + A.rb_stride * (size_t)(n - n0);
+                enqueue_step_rec(h, sn);
+                copy_set(h, slot<double>(h, S0_), A.SC, 7 * n2);
+            }
+            for (int n = n1 - 1; n >= n0; --n) {
+                A.cur = A.recs + A.rb_stride * (size_t)(n - n0);
+                enqueue_step_adj(h, A.seg + (size_t)(n - n0) * 7 * n2);
+            }
+        } else {
+            // states of the segment, then per step (in reverse) its record and adjoint
+            for (int n = n0; n < n1; ++n) {
+                copy_set(h, A.seg + (size_t)(n - n0) * 7 * n2, slot<double>(h, S0_), 7 * n2);
+                if (n + 1 < n1 && run_steps(h, 1)) return -1;
+            }
+            A.cur = A.recs;
+            for (int n = n1 - 1; n >= n0; --n) {
+                enqueue_step_rec(h, A.seg + (size_t)(n - n0) * 7 * n2);
+                enqueue_step_adj(h, A.seg + (size_t)(n - n0) * 7 * n2);
+            }
         }
-        for (int n = n1 - 1; n >= n0; --n) enqueue_step_adj(h, A.seg + (size_t)(n - n0) * 7 * n2);
         HIPCHK(h, hipGetLastError());
     }
     hipLaunchKernelGGL(ka_reduce_k, dim3(h->nm), dim3(256), 0, h->stream, h->dom_d, (const double *)A.gk,
@@ -903,25 +1018,38 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
         for (int k0 = 0; k0 < D.nz; k0 += ST_Z)
             for (int i0 = 0; i0 < D.nx; i0 += ST_X) ts.push_back(make_int4(m, i0, k0, 0));
         for (int i0 = 0; i0 < D.nx; i0 += COL_TILE) tc.push_back(make_int2(m, i0));
-        for (int i = 0; i < D.nx; ++i) wc.push_back(make_int2(m, i));
-        int nch = 1;
-        while (nch * acL < D.nz) nch *= 2;
-        if (nch > AC_THREADS) {
+        // c' storage for the largest chunk count any acoustic shape can use (L = 2)
+        int nch2 = 1;
+        while (nch2 * 2 < D.nz) nch2 *= 2;
+        h->nch_max = std::max(h->nch_max, std::min(nch2, AC_THREADS));
+    }
+    {
+        int nch_def = 0;
+        if (make_atiles(h->dom_h, acL, AC_THREADS, ta, nch_def)) {
             h->err = "nz too large for the acoustic kernel";
             *out = h.release();
             return -2;
         }
-        const int TX = AC_THREADS / nch;
-        h->nch_max = std::max(h->nch_max, nch);
-        for (int i0 = 0; i0 < D.nx; i0 += TX) ta.push_back(make_int4(m, i0, TX, nch));
+        h->nch_max = std::max(h->nch_max, nch_def);
+        h->ac_L = acL;
+        h->ac_NT = AC_THREADS;
     }
     t2 = xcd_order(t2, make_int4(-1, 0, 0, 0));
     ts = xcd_order(ts, make_int4(-1, 0, 0, 0));
     h->nstiles = (int)ts.size();
     tc = xcd_order(tc, make_int2(-1, 0));
-    wc = xcd_order(wc, make_int2(-1, 0));
-    h->nwcols = (int)wc.size();
-    ta = xcd_order(ta, make_int4(-1, 0, 0, 0));
+    {
+        // adjoint column solve: one wave per column for few columns (parallelism
+        // over rows), 8 columns x 8 chunks or 16 x 4 per wave for many (coalescing)
+        int64_t ncols = 0;
+        for (const Dom &D : h->dom_h) ncols += D.nx;
+        h->wch = ncols < 4096 ? 64 : ncols < 16384 ? 8 : 4;
+        const int cw = 64 / h->wch;
+        for (int m = 0; m < n_members; ++m)
+            for (int i0 = 0; i0 < h->dom_h[m].nx; i0 += cw) wc.push_back(make_int2(m, i0));
+        wc = xcd_order(wc, make_int2(-1, 0));
+        h->nwgroups = (int)wc.size();
+    }
     h->natiles = (int)ta.size();
     h->ntiles2d = (int)t2.size();
     h->nctiles = (int)tc.size();
@@ -938,8 +1066,8 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
     ALLOC(hp->ctiles, sizeof(int2) * tc.size());
     ALLOC(hp->atiles, sizeof(int4) * ta.size());
     ALLOC(hp->stiles, sizeof(int4) * ts.size());
-    ALLOC(hp->wcols, sizeof(int2) * wc.size());
-    HIPCHK(hp, hipMemcpy(hp->wcols, wc.data(), sizeof(int2) * wc.size(), hipMemcpyHostToDevice));
+    ALLOC(hp->wgroups, sizeof(int2) * wc.size());
+    HIPCHK(hp, hipMemcpy(hp->wgroups, wc.data(), sizeof(int2) * wc.size(), hipMemcpyHostToDevice));
     HIPCHK(hp, hipMemcpy(hp->stiles, ts.data(), sizeof(int4) * ts.size(), hipMemcpyHostToDevice));
     HIPCHK(hp, hipMemcpy(hp->atiles, ta.data(), sizeof(int4) * ta.size(), hipMemcpyHostToDevice));
     HIPCHK(hp, hipMemcpy(hp->dom_d, hp->dom_h.data(), sizeof(Dom) * n_members, hipMemcpyHostToDevice));
@@ -1153,6 +1281,75 @@ int moist_kernel_stats(moist_handle *h, double *acoustic_ms_per_launch, int64_t 
     return 0;
 }
 
+int moist_autotune(moist_handle *h, int reps, int *levels, int *threads, double *us_per_launch) {
+    if (!h || reps < 1) return -2;
+    const int Ls[2] = {2, 4}, NTs[3] = {256, 512, 1024};
+    const int L0 = h->ac_L, NT0 = h->ac_NT;
+    int bestL = L0, bestNT = NT0;
+    double best = 1e300, t0 = 1e300;
+    hipEvent_t a, b;
+    HIPCHK(h, hipEventCreate(&a));
+    HIPCHK(h, hipEventCreate(&b));
+    const ZVec zv = zvec(h);
+    for (int L : Ls)
+        for (int NT : NTs) {
+            if (set_acoustic_shape(h, L, NT)) continue;  // shape does not fit every member
+            // one stage's substeps on the current state and work arrays (outputs are work arrays only)
+            auto run = [&]() {
+                for (int n = 0; n < reps; ++n) {
+                    if (h->precision == MOIST_MIXED) {
+                        const Work<float> w = work<float>(h);
+                        AcBuf<float> ab{slot<float>(h, RUP), slot<float>(h, RUP2), slot<float>(h, RTP0),
+                                        slot<float>(h, RTP0), slot<float>(h, RTP1), h->cin, h->nxs};
+                        launch_acoustic<float>(h, zv, sset<float>(h, S0_), w, ab, 1, 0.0, 1, n == 0 ? 1 : 0);
+                    } else {
+                        const Work<double> w = work<double>(h);
+                        AcBuf<double> ab{slot<double>(h, RUP), slot<double>(h, RUP2), slot<double>(h, RTP0),
+                                         slot<double>(h, RTP0), slot<double>(h, RTP1), h->cin, h->nxs};
+                        launch_acoustic<double>(h, zv, sset<double>(h, S0_), w, ab, 1, 0.0, 1, n == 0 ? 1 : 0);
+                    }
+                }
+            };
+            run();  // warm-up
+            HIPCHK(h, hipEventRecord(a, h->stream));
+            run();
+            HIPCHK(h, hipEventRecord(b, h->stream));
+            HIPCHK(h, hipEventSynchronize(b));
+            HIPCHK(h, hipGetLastError());
+            float ms = 0.f;
+            HIPCHK(h, hipEventElapsedTime(&ms, a, b));
+            if (L == L0 && NT == NT0) t0 = ms;
+            if (ms < best) {
+                best = ms;
+                bestL = L;
+                bestNT = NT;
+            }
+        }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    // keep the current shape unless another is faster by more than 3 %
+    if (!(best < 0.97 * t0)) {
+        bestL = L0;
+        bestNT = NT0;
+        best = t0;
+    }
+    if (int rc = set_acoustic_shape(h, bestL, bestNT)) return rc;
+    h->tuned = 1;
+    if (levels) *levels = bestL;
+    if (threads) *threads = bestNT;
+    if (us_per_launch) *us_per_launch = best * 1e3 / reps;
+    return 0;
+}
+
+int moist_set_acoustic_shape(moist_handle *h, int levels, int threads) {
+    if (!h) return -2;
+    if ((levels != 2 && levels != 4) || (threads != 256 && threads != 512 && threads != 1024)) {
+        h->err = "acoustic shape: levels 2 or 4, threads 256, 512 or 1024";
+        return -2;
+    }
+    return set_acoustic_shape(h, levels, threads);
+}
+
 int moist_adjoint_run(moist_handle *h, int nsteps, int ckpt, const void *const *theta_target, int elem_bytes,
                       double *J) {
     if (!h || !theta_target || nsteps < 1 || ckpt < 1) return -2;
@@ -1214,7 +1411,7 @@ void moist_destroy(moist_handle *h) {
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     for (auto e : h->ev_pool) hipEventDestroy(e);
     adj_free(h);
-    void *ptrs[] = {h->dom_d, h->tiles2d, h->stiles, h->ctiles, h->wcols, h->z, h->zf, h->dz, h->geo, h->theta_ref, h->rho_bar, h->rt_bar,
+    void *ptrs[] = {h->dom_d, h->tiles2d, h->stiles, h->ctiles, h->wgroups, h->z, h->zf, h->dz, h->geo, h->theta_ref, h->rho_bar, h->rt_bar,
                     h->p_bar, h->L_bar, h->B_bar, h->pool, h->pool1, h->atiles, h->cin, h->prec, h->sfh, h->nan_key,
                     h->step_ctr, h->staging, h->diag_out};
     for (void *p : ptrs)

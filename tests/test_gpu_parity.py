@@ -188,3 +188,43 @@ def test_conservation_coldpool_1024x256():
         dd = (d1["dry_mass"] - d0["dry_mass"]) / d0["dry_mass"]
         print(f"conservation prec={prec}: water+rain {dw:.2e} mass+rain {dm:.2e} dry {dd:.2e}", flush=True)
         assert abs(dw) <= tol and abs(dm) <= tol and abs(dd) <= tol
+
+
+@pytest.mark.parametrize("levels,threads", [(2, 256), (2, 512), (2, 1024), (4, 256), (4, 512), (4, 1024)])
+def test_fp64_every_acoustic_shape(levels, threads):
+    """Every shape the autotuner may pick keeps the FP64 parity bar."""
+    p, s = ic.bubble(np.float64)
+    ref = s.copy()
+    assert O.rk3_step(ref, p, 10) == 0
+    eng = core.Engine(p, core.FP64)
+    eng.set_acoustic_shape(levels, threads)
+    eng.upload(s)
+    eng.step(10)
+    out = s.copy()
+    eng.download(out)
+    eng.close()
+    errs = field_errors(p, out, ref)
+    _print(f"fp64 shape L={levels} NT={threads}", errs)
+    for n, e in errs.items():
+        assert e <= TOL_FP64, (n, e)
+
+
+def test_autotune_keeps_parity():
+    p, s = ic.bubble(np.float64)
+    ref = s.copy()
+    assert O.rk3_step(ref, p, 10) == 0
+    for prec, tol in ((core.FP64, TOL_FP64), (core.MIXED, 1e-4)):
+        eng = core.Engine(p, prec)
+        eng.upload(s)
+        L, NT, us = eng.autotune(reps=5)
+        assert L in (2, 4) and NT in (256, 512, 1024) and us > 0
+        eng.upload(s)
+        eng.use_graph(True)
+        eng.step(10)
+        out = s.copy()
+        eng.download(out)
+        eng.close()
+        errs = field_errors(p, out, ref)
+        _print(f"autotuned prec={prec} L={L} NT={NT} ({us:.1f} us)", errs)
+        for n, e in errs.items():
+            assert e <= tol, (prec, n, e)
