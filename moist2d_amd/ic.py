@@ -242,6 +242,121 @@ def theta_target(p, seed=3, sigma=0.5, member=0):
     return out
 
 
+def invert_T_for_thetae(qt, theta_e, p_k):
+    """Temperature (and saturation qv) whose saturated equivalent potential
+    temperature at pressure p_k is theta_e: 30 bisection steps on [250, 350] K
+    (invert_T_for_thetae, moist.jl:275-289).  Vectorised over p_k."""
+    p_k = np.asarray(p_k, dtype=np.float64)
+    lo = np.full(p_k.shape, 250.0)
+    hi = np.full(p_k.shape, 350.0)
+    qsat = np.zeros(p_k.shape)
+    qd = 1.0 - qt
+    cp = _C.Cpd * qd + _C.Cpl * qt
+    for _ in range(30):
+        t = (lo + hi) / 2
+        Tc = t - _C.T0
+        es = _C.ES0 * np.exp(_C.A_SAT * Tc / (_C.B_SAT + Tc))
+        qsat = _qsat(t, p_k)
+        th = t * ((p_k - es) / _C.P0) ** (-_C.Rd / cp) * np.exp(_C.Lv * qsat / (cp * t))
+        above = th > theta_e
+        hi = np.where(above, t, hi)
+        lo = np.where(above, lo, t)
+    return (lo + hi) / 2, qsat
+
+
+def init_state_bf2002(s: State, p, qt=0.02, theta_e=320.0) -> State:
+    """The saturated neutral atmosphere of Bryan and Fritsch (2002) with a
+    cos^2 warm bubble: init_state_BF2002!(s, p) (moist.jl:453-551), in Float64.
+
+    qv + qc = qt everywhere and the saturated equivalent potential temperature
+    is theta_e; the column pressure is iterated to the model's discrete
+    hydrostatic balance (moist.jl:489-514, relaxation 0.7, tolerance 1e-10).
+    The reference's first-level update reads an index `k` that is not defined
+    at that point (moist.jl:494, the function is commented out of run_sim);
+    here that layer is the trapezoid from the surface to z[ks],
+    p_ks = P0 - g (rho_s + rho_ks) z[ks] / 2, which is what its factor
+    0.25 * (z[k] - z[k-1]) equals on a uniform grid."""
+    T = p.T
+    c = _C
+    is0, nx, nz = p.is_ - 1, p.Nx, p.Nz
+    z = p.z.astype(np.float64)
+    qd = 1.0 - qt
+    P0, G = c.P0, c.GRAV
+    # first guess (moist.jl:465-484)
+    t_s, qv = invert_T_for_thetae(qt, theta_e, np.array(P0))
+    theta = float(t_s)
+    qv = float(qv)
+    r = c.Rd * qd + c.Rv * qv
+    cp = c.Cpd * qd + c.Cpv * qv + c.Cpl * (qt - qv)
+    Pi = np.zeros(nz)
+    pc = np.zeros(nz)
+    Pi[0] = 1.0 - G * z[0] / (cp * theta)
+    pc[0] = P0 * Pi[0] ** (cp / r)
+    t_k, qv = invert_T_for_thetae(qt, theta_e, pc[0])
+    theta = float(t_k) / Pi[0]
+    qv = float(qv)
+    for k in range(1, nz):
+        r = c.Rd * qd + c.Rv * qv
+        cp = c.Cpd * qd + c.Cpv * qv + c.Cpl * (qt - qv)
+        Pi[k] = Pi[k - 1] - G * (z[k] - z[k - 1]) / (cp * theta)
+        pc[k] = P0 * Pi[k] ** (cp / r)
+        t_k, qv = invert_T_for_thetae(qt, theta_e, pc[k])
+        theta = float(t_k) / Pi[k]
+        qv = float(qv)
+    # hydrostatic iteration (moist.jl:487-514)
+    omega = 0.7
+    t_s = float(t_s)
+    for _ in range(100):
+        max_diff = 0.0
+        t_k, qv = invert_T_for_thetae(qt, theta_e, pc[0])
+        r = c.Rd * qd + c.Rv * float(qv)
+        rho_s = P0 / (r * t_s)
+        rho_k = pc[0] / (r * float(t_k))
+        p_new = P0 - G * (rho_s + rho_k) * z[0] * 0.5
+        d = omega * (p_new - pc[0])
+        pc[0] += d
+        max_diff = max(max_diff, abs(d / p_new))
+        for k in range(1, nz):
+            t_m, qv_m = invert_T_for_thetae(qt, theta_e, pc[k - 1])
+            rho_m = pc[k - 1] / ((c.Rd * qd + c.Rv * float(qv_m)) * float(t_m))
+            t_k, qv = invert_T_for_thetae(qt, theta_e, pc[k])
+            rho_k = pc[k] / ((c.Rd * qd + c.Rv * float(qv)) * float(t_k))
+            p_new = pc[k - 1] - G * (rho_m + rho_k) * 0.5 * (z[k] - z[k - 1])
+            d = omega * (p_new - pc[k])
+            pc[k] += d
+            max_diff = max(max_diff, abs(d / p_new))
+        if max_diff < 1e-10:
+            break
+    # fields (moist.jl:518-548)
+    t_k, qv = invert_T_for_thetae(qt, theta_e, pc)
+    r = c.Rd * qd + c.Rv * qv
+    cp = c.Cpd * qd + c.Cpv * qv + c.Cpl * (qt - qv)
+    theta = t_k * (P0 / pc) ** (r / cp)
+    x = p.x[is0:is0 + nx].astype(np.float64)
+    xc, zc, xr, zr = float(p.Lx) / 2, 2e3, 2e3, 2e3
+    X, Z = np.meshgrid(x, z)
+    dist = np.sqrt(((X - xc) / xr) ** 2 + ((Z - zc) / zr) ** 2)
+    tdiff = float(p.theta_pert) * np.cos(np.pi * dist * 0.5) ** 2 / 300.0
+    th2 = theta[:, None] * np.where(dist < 1.0, tdiff + 1.0, 1.0)
+    qs = _qsat(t_k, pc)
+    qv2 = np.minimum(qt, qs)
+    qc2 = qt - qv2
+    r2 = c.Rd * qd + c.Rv * qv2
+    rho = (pc / (r2 * t_k))[:, None] * np.ones((1, nx))
+    sl = np.s_[:, is0:is0 + nx]
+    s.rho[sl] = rho.astype(T)
+    s.rho_u[sl] = (rho * float(p.u0)).astype(T)
+    s.rho_w[...] = 0
+    s.rho_theta[sl] = (rho * th2).astype(T)
+    s.rho_qv[sl] = (rho * qv2[:, None]).astype(T)
+    s.rho_qc[sl] = (rho * qc2[:, None]).astype(T)
+    s.rho_qr[...] = 0
+    for n in PROGNOSTIC:
+        exchange_halo(getattr(s, n), p)
+    s._bf2002_pressure = pc
+    return s
+
+
 def init_state(s: State, p, seed=0) -> State:
     """The reference's init_state!(s, p, rngs) (moist.jl:351-450) with the
     portable generator in place of the per-thread Xoshiro streams.  Float64

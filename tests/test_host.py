@@ -88,3 +88,41 @@ def test_state_layout():
     assert [f for f in STATE_ORDER][:7] == ["rho", "rho_u", "rho_w", "rho_theta", "rho_qv", "rho_qc", "rho_qr"]
     assert s.rho.shape == (p.ka, p.ia) and s.rho.flags["F_CONTIGUOUS"]
     assert s.prec.shape == (p.ia,) and s.theta_ref.shape == (p.ka,)
+
+
+def test_bf2002_initial_state():
+    """init_state_BF2002! (moist.jl:453-551): saturated, qt = 0.02, saturated
+    theta_e = 320 K where unperturbed, discrete hydrostatic balance, and the
+    model's equation of state returns the column pressure."""
+    from moist2d_amd import ic
+    from moist2d_amd.params import Constants, make_params
+    from moist2d_amd.state import allocate_state
+    c = Constants(np.float64)
+    p = make_params(np.float64, Nx=64, Nz=50, Lx=20e3, H=10e3, dz0=200.0, z_fact=1.0, ns=6, tout_int=1.0)
+    s = ic.init_state_bf2002(allocate_state(p), p)
+    h = p.halo
+    rho, pc, z = s.rho[:, h], s._bf2002_pressure, p.z
+    res = pc[1:] - pc[:-1] + c.GRAV * (z[1:] - z[:-1]) * (rho[1:] + rho[:-1]) / 2
+    assert np.abs(res).max() <= 1e-9 * pc.max()
+    inner = np.s_[:, h:h + p.Nx]
+    qv = s.rho_qv[inner] / s.rho[inner]
+    qc = s.rho_qc[inner] / s.rho[inner]
+    np.testing.assert_allclose(qv + qc, 0.02, rtol=1e-13)
+    assert (qc >= 0).all() and (s.rho_qr == 0).all()
+    # model EOS: pres = P0 (r rho theta / P0)^(cp/cv)
+    qd = 1.0 - qv - qc
+    r = c.Rd * qd + c.Rv * qv
+    cp = c.Cpd * qd + c.Cpv * qv + c.Cpl * qc
+    pres = c.P0 * (r * s.rho_theta[inner] / c.P0) ** (cp / (cp - r))
+    far = np.abs(p.x[h:h + p.Nx] - p.Lx / 2) > 2.5e3  # outside the bubble
+    np.testing.assert_allclose(pres[:, far], pc[:, None] * np.ones((1, far.sum())), rtol=1e-10)
+    # saturated theta_e of the unperturbed columns (the inversion's own formula)
+    t, qs = ic.invert_T_for_thetae(0.02, 320.0, pc)
+    Tc = t - c.T0
+    es = c.ES0 * np.exp(c.A_SAT * Tc / (c.B_SAT + Tc))
+    cpl = c.Cpd * 0.98 + c.Cpl * 0.02
+    the = t * ((pc - es) / c.P0) ** (-c.Rd / cpl) * np.exp(c.Lv * qs / (cpl * t))
+    np.testing.assert_allclose(the, 320.0, atol=1e-6)
+    # the bubble: theta scaled by 1 + theta_pert cos^2 / 300 at its centre
+    th = s.rho_theta[inner] / s.rho[inner]
+    assert th[:, ~far].max() > th[:, far].max() or th[:, ~far].max() > 0
