@@ -144,6 +144,18 @@ int moist_kernel_events(moist_handle *h, int on);
 int moist_kernel_stats(moist_handle *h, double *acoustic_ms_per_launch,
                        int64_t *acoustic_launches);
 
+/* The derived fields and budget of output_history (moist.jl:1816-1858) for
+ * one member, computed on the device in FP64 from the current state.
+ * stats[7] (may be NULL): total energy (kinetic + potential + moist
+ * internal, J/m per unit y, before the reference subtracts its first value),
+ * w min, w max (m/s), qc max, qr max (g/kg), RH min, RH max (%).
+ * fields[13] (NULL entries skipped): host arrays of nx*nz elements, x
+ * fastest (the history file's layout): rho, rho_u, rho_w, u, w, theta, qv,
+ * qc, qr, RH (%), temperature (K), pressure (hPa), theta_e (K). */
+#define MOIST_HIST_FIELDS 13
+#define MOIST_HIST_STATS 7
+int moist_history(moist_handle *h, int member, double *stats, void *const fields[13], int elem_bytes);
+
 /* Tile-shape autotuner (no counterpart in the reference, whose loops have
  * no tiles): times the acoustic substep kernel on this handle's domains for
  * every shape of L levels per chunk (2, 4) and workgroup size (256, 512,

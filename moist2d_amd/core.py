@@ -59,6 +59,7 @@ def lib():
             "moist_adjoint_times": (ip, [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
             "moist_autotune": (ip, [vp, ip, ctypes.POINTER(ip), ctypes.POINTER(ip), ctypes.POINTER(ctypes.c_double)]),
             "moist_set_acoustic_shape": (ip, [vp, ip, ip]),
+            "moist_history": (ip, [vp, ip, ctypes.POINTER(ctypes.c_double), vp, ip]),
             "moist_device_bytes": (ip, [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
             "moist_last_error": (ctypes.c_char_p, [vp]),
             "moist_destroy": (None, [vp]),
@@ -199,6 +200,19 @@ class Engine:
 
     def set_acoustic_shape(self, levels, threads):
         self._check(lib().moist_set_acoustic_shape(self._h, levels, threads), "moist_set_acoustic_shape")
+
+    HIST_FIELDS = ("rho", "rhou", "rhow", "u", "w", "theta", "qv", "qc", "qr", "RH", "temp", "pres", "thetae")
+    HIST_STATS = ("Etot", "wmin", "wmax", "qcmax", "qrmax", "RHmin", "RHmax")
+
+    def history(self, member=0, fields=True):
+        """output_history's derived fields ((Nz, Nx) arrays, x fastest, dict keyed
+        by the history variable names) and stats (dict), computed on the device."""
+        p = self.params[member]
+        st = (ctypes.c_double * 7)()
+        out = {n: np.zeros((p.Nz, p.Nx)) for n in self.HIST_FIELDS} if fields else None
+        ptrs = _ptrs([out[n] for n in self.HIST_FIELDS]) if fields else None
+        self._check(lib().moist_history(self._h, member, st, ptrs, 8), "moist_history")
+        return out, dict(zip(self.HIST_STATS, list(st)))
 
     def device_bytes(self):
         """(state bytes, adjoint bytes) of device memory held by the handle."""

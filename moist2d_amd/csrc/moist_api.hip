@@ -28,6 +28,7 @@
 #include "moist_kernels.h"
 namespace mo {
 #include "moist_adjoint.h"
+#include "moist_history.h"
 }
 
 using namespace mo;
@@ -117,6 +118,8 @@ struct moist_handle {
     // reverse mode (allocated by the first moist_adjoint_run)
     AdjState *adj = nullptr;
     int64_t dev_bytes = 0;  // device memory of the state pools and tables
+    // output_history: derived fields (allocated on first request), block partials, per-member stats
+    double *hist_fields = nullptr, *hist_part = nullptr, *hist_stats = nullptr;
 };
 
 namespace {
@@ -563,6 +566,36 @@ int compute_step_t(moist_handle *h, int m, const void *const phys[6], const void
             for (int k = 0; k < ka; ++k)
                 if (std::isnan(double(a[(size_t)i * ka + k]))) return 8 + f;
     }
+    return 0;
+}
+
+// ---- output_history's derived fields and budget -------------------------------------
+template <class S, class H> int history_t(moist_handle *h, int m, double *stats, void *const fields[HF_COUNT]) {
+    bool want = false;
+    for (int f = 0; fields && f < HF_COUNT; ++f) want |= fields[f] != nullptr;
+    if (want && !h->hist_fields) HIPCHK(h, hipMalloc((void **)&h->hist_fields, sizeof(double) * HF_COUNT * h->n2));
+    if (!h->hist_part) HIPCHK(h, hipMalloc((void **)&h->hist_part, sizeof(double) * HS_COUNT * h->ntiles2d));
+    if (!h->hist_stats) HIPCHK(h, hipMalloc((void **)&h->hist_stats, sizeof(double) * HS_COUNT * h->nm));
+    hipLaunchKernelGGL(k_history<S>, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d, h->tiles2d,
+                       zvec(h), sset<S>(h, S0_), want ? h->hist_fields : (double *)nullptr, (int64_t)h->n2,
+                       h->hist_part);
+    hipLaunchKernelGGL(k_history_reduce, dim3(h->nm), dim3(256), 0, h->stream, (const int4 *)h->tiles2d,
+                       h->ntiles2d, (const double *)h->hist_part, h->hist_stats);
+    HIPCHK(h, hipGetLastError());
+    if (stats)
+        HIPCHK(h, hipMemcpyAsync(stats, h->hist_stats + (size_t)HS_COUNT * m, sizeof(double) * HS_COUNT,
+                                 hipMemcpyDeviceToHost, h->stream));
+    const Dom &D = h->dom_h[m];
+    const size_t n = (size_t)D.nx * D.nz;
+    std::vector<double> tmp(n);
+    for (int f = 0; want && f < HF_COUNT; ++f) {
+        if (!fields[f]) continue;
+        HIPCHK(h, hipMemcpyAsync(tmp.data(), h->hist_fields + (size_t)f * h->n2 + D.off2, n * sizeof(double),
+                                 hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        for (size_t q = 0; q < n; ++q) ((H *)fields[f])[q] = H(tmp[q]);
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return 0;
 }
 
@@ -1390,6 +1423,18 @@ int moist_adjoint_grad(moist_handle *h, int member, void *const grad[7], double 
     return 0;
 }
 
+int moist_history(moist_handle *h, int member, double *stats, void *const fields[13], int elem_bytes) {
+    if (!h) return -2;
+    if (int rc = check_member(h, member)) return rc;
+    const bool mixed = h->precision == MOIST_MIXED;
+    if (elem_bytes == 8)
+        return mixed ? history_t<float, double>(h, member, stats, fields) : history_t<double, double>(h, member, stats, fields);
+    if (elem_bytes == 4)
+        return mixed ? history_t<float, float>(h, member, stats, fields) : history_t<double, float>(h, member, stats, fields);
+    h->err = "elem_bytes must be 4 or 8";
+    return -2;
+}
+
 int moist_device_bytes(moist_handle *h, int64_t *state_bytes, int64_t *adjoint_bytes) {
     if (!h) return -2;
     if (state_bytes) *state_bytes = h->dev_bytes;
@@ -1411,7 +1456,7 @@ void moist_destroy(moist_handle *h) {
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     for (auto e : h->ev_pool) hipEventDestroy(e);
     adj_free(h);
-    void *ptrs[] = {h->dom_d, h->tiles2d, h->stiles, h->ctiles, h->wgroups, h->z, h->zf, h->dz, h->geo, h->theta_ref, h->rho_bar, h->rt_bar,
+    void *ptrs[] = {h->dom_d, h->tiles2d, h->stiles, h->ctiles, h->wgroups, h->hist_fields, h->hist_part, h->hist_stats, h->z, h->zf, h->dz, h->geo, h->theta_ref, h->rho_bar, h->rt_bar,
                     h->p_bar, h->L_bar, h->B_bar, h->pool, h->pool1, h->atiles, h->cin, h->prec, h->sfh, h->nan_key,
                     h->step_ctr, h->staging, h->diag_out};
     for (void *p : ptrs)

@@ -942,6 +942,61 @@ template <class T> static int rk3_step(const G<T> &p, T *const *F, int order) {
 
 #include "moist_adjoint.inc"
 
+// ---- output_history's derived fields and budget (moist.jl:1816-1858) -------
+// out[13]: nx*nz arrays, x fastest; stats[7]: Etot sum, w min/max, qc/qr max
+// (g/kg), RH min/max.  rho_u's halo is exchanged first, as the reference does.
+template <class T>
+static void output_history(const G<T> &p, T *const *F, T *const *out, double *stats) {
+    const C<T> &c = p.c;
+    T *rho = F[0], *rho_u = F[1], *rho_w = F[2], *rho_theta = F[3];
+    exchange_halo(rho_u, p);
+    const int nx = p.nx, nz = p.nz;
+    double ek = 0.0, epot = 0.0, ei = 0.0;
+    double wmin = 1e300, wmax = -1e300, qcm = -1e300, qrm = -1e300, rhmin = 1e300, rhmax = -1e300;
+    for (int i = 0; i < nx; ++i)
+        for (int k = 0; k < nz; ++k) {
+            const size_t q = p.ix(k, p.is + i);
+            const T rinv = T(1.0) / rho[q];
+            const T th = rho_theta[q] * rinv;
+            const T qv = F[4][q] * rinv, qc = F[5][q] * rinv, qr = F[6][q] * rinv;
+            const T u = (rho_u[q] + rho_u[p.ix(k, p.is + i - 1)]) * T(0.5) * rinv;
+            const T w = k > 0 ? (rho_w[q] + rho_w[p.ix(k - 1, p.is + i)]) * T(0.5) * rinv : rho_w[q] * T(0.5) * rinv;
+            const T qt = qv + qc + qr, qd = T(1.0) - qt;
+            const T r = c.Rd * qd + c.Rv * qv;
+            T cp = c.Cpd * qd + c.Cpv * qv + c.Cpl * (qc + qr);
+            const T cv = cp - r;
+            const T pi = std::pow(r * rho_theta[q] / c.P0, r / cv);
+            const T pres = c.P0 * std::pow(pi, cp / r);
+            const T temp = th * pi;
+            T es, des, qs, dqs;
+            sat_vp(c, temp, es, des);
+            sat_q(c, pres, es, qs, dqs);
+            const T rh = qs > T(1e-10) ? qv / qs * T(100.0) : T(0.0);
+            const T presv = rho[q] * qv * c.Rv * temp;
+            const T presd = pres - presv;
+            cp = c.Cpd * qd + c.Cpl * qt;
+            const T the = temp * std::pow(presd / c.P0, -c.Rd / cp) * std::exp(c.Lv * qv / (cp * temp));
+            ek += double(T(0.5) * rho[q] * (u * u + w * w));
+            epot += double(rho[q] * c.GRAV * p.z[k]);
+            ei += double(rho[q] * (cv * temp + qv * c.Lv));
+            wmin = std::min(wmin, double(w));
+            wmax = std::max(wmax, double(w));
+            qcm = std::max(qcm, double(qc) * 1000.0);
+            qrm = std::max(qrm, double(qr) * 1000.0);
+            rhmin = std::min(rhmin, double(rh));
+            rhmax = std::max(rhmax, double(rh));
+            const T v[13] = {rho[q], rho_u[q], rho_w[q], u, w, th, qv, qc, qr, rh, temp, pres / T(100.0), the};
+            for (int f = 0; f < 13; ++f) out[f][(size_t)k * nx + i] = v[f];
+        }
+    stats[0] = ek + epot + ei;
+    stats[1] = wmin;
+    stats[2] = wmax;
+    stats[3] = qcm;
+    stats[4] = qrm;
+    stats[5] = rhmin;
+    stats[6] = rhmax;
+}
+
 }  // namespace orc
 
 // ============================================================================
@@ -1083,6 +1138,12 @@ ORC_EXPORT int orc_rk3_step_adj_f64(const moist_params *mp, const double *const 
     }
     orc::St<double> av = a.view();
     return orc::rk3_step_adj(g, av, lam, ldprec, lK);
+}
+
+ORC_EXPORT void orc_output_history_f64(const moist_params *mp, double *const *fields, double *const *out,
+                                       double *stats) {
+    orc::G<double> g(mp);
+    orc::output_history(g, fields, out, stats);
 }
 
 ORC_EXPORT int orc_adjoint_run_f64(const moist_params *mp, double *const *fields, int nsteps, int ckpt,

@@ -65,6 +65,7 @@ def lib():
             ("physics_adj", None, [V, V, V, V, V, V]),
             ("rk3_step_adj", ctypes.c_int, [V, V, V, ctypes.c_double, V]),
             ("adjoint_run", ctypes.c_int, [V, V, ctypes.c_int, ctypes.c_int, V, V, V, V]),
+            ("output_history", None, [V, V, V, V]),
         ):
             f = getattr(_lib, f"orc_{name}_f64")
             f.restype = res
@@ -283,3 +284,17 @@ def adjoint_run(s, p, nsteps, theta_target, ckpt=50):
     if rc:
         raise RuntimeError(f"adjoint_run failed ({rc})")
     return J.value, grad, gK
+
+
+HIST_FIELDS = ("rho", "rhou", "rhow", "u", "w", "theta", "qv", "qc", "qr", "RH", "temp", "pres", "thetae")
+
+
+def output_history(s, p):
+    """output_history's derived fields ((Nz, Nx), x fastest) and stats
+    (Etot sum, w min/max, qc/qr max g/kg, RH min/max) of a Float64 State."""
+    out = [np.zeros((p.Nz, p.Nx)) for _ in HIST_FIELDS]
+    st = np.zeros(7)
+    cp = p.to_c()
+    lib().orc_output_history_f64(ctypes.byref(cp), _f64([getattr(s, n) for n in _PROG]),
+                                 _ptr_array(out, ctypes.c_double), st.ctypes.data_as(ctypes.c_void_p))
+    return dict(zip(HIST_FIELDS, out)), st
