@@ -119,7 +119,7 @@ struct moist_handle {
     AdjState *adj = nullptr;
     int64_t dev_bytes = 0;  // device memory of the state pools and tables
     // output_history: derived fields (allocated on first request), block partials, per-member stats
-    double *hist_fields = nullptr, *hist_part = nullptr, *hist_stats = nullptr;
+    double *hist_fields = nullptr, *hist_part = nullptr, *hist_stats = nullptr, *diag_all = nullptr;
 };
 
 namespace {
@@ -1226,17 +1226,20 @@ int moist_compute_step(moist_handle *h, int member, const void *const phys[6], c
 int moist_diagnostics(moist_handle *h, int member, double *out) {
     if (!h || !out) return -2;
     if (int rc = check_member(h, member)) return rc;
-    const Dom &D = h->dom_h[member];
     const ZVec zv = zvec(h);
+    if (!h->hist_part) HIPCHK(h, hipMalloc((void **)&h->hist_part, sizeof(double) * HS_COUNT * h->ntiles2d));
+    if (!h->diag_all) HIPCHK(h, hipMalloc((void **)&h->diag_all, sizeof(double) * 4 * h->nm));
     if (h->precision == MOIST_MIXED)
-        hipLaunchKernelGGL(k_diagnostics<float>, dim3(1), dim3(256), 0, h->stream, D, zv, sset<float>(h, S0_),
-                           (const double *)h->prec, h->diag_out);
+        hipLaunchKernelGGL(k_budget<float>, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d,
+                           h->tiles2d, zv, sset<float>(h, S0_), h->hist_part);
     else
-        hipLaunchKernelGGL(k_diagnostics<double>, dim3(1), dim3(256), 0, h->stream, D, zv, sset<double>(h, S0_),
-                           (const double *)h->prec, h->diag_out);
+        hipLaunchKernelGGL(k_budget<double>, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d,
+                           h->tiles2d, zv, sset<double>(h, S0_), h->hist_part);
+    hipLaunchKernelGGL(k_budget_reduce, dim3(h->nm), dim3(256), 0, h->stream, h->dom_d, (const int4 *)h->tiles2d,
+                       h->ntiles2d, (const double *)h->hist_part, (const double *)h->prec, h->diag_all);
     HIPCHK(h, hipGetLastError());
     double v[4];
-    HIPCHK(h, hipMemcpyAsync(v, h->diag_out, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(v, h->diag_all + 4 * member, sizeof(v), hipMemcpyDeviceToHost, h->stream));
     int key = INT_MAX;
     HIPCHK(h, hipMemcpyAsync(&key, h->nan_key, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1456,7 +1459,7 @@ void moist_destroy(moist_handle *h) {
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     for (auto e : h->ev_pool) hipEventDestroy(e);
     adj_free(h);
-    void *ptrs[] = {h->dom_d, h->tiles2d, h->stiles, h->ctiles, h->wgroups, h->hist_fields, h->hist_part, h->hist_stats, h->z, h->zf, h->dz, h->geo, h->theta_ref, h->rho_bar, h->rt_bar,
+    void *ptrs[] = {h->dom_d, h->tiles2d, h->stiles, h->ctiles, h->wgroups, h->hist_fields, h->hist_part, h->hist_stats, h->diag_all, h->z, h->zf, h->dz, h->geo, h->theta_ref, h->rho_bar, h->rt_bar,
                     h->p_bar, h->L_bar, h->B_bar, h->pool, h->pool1, h->atiles, h->cin, h->prec, h->sfh, h->nan_key,
                     h->step_ctr, h->staging, h->diag_out};
     for (void *p : ptrs)

@@ -115,3 +115,63 @@ __global__ void __launch_bounds__(256)
     }
     if (tid < HS_COUNT) stats[m * HS_COUNT + tid] = red[tid][0];
 }
+
+// ---- water and mass budget (moist_diagnostics): FP64 block partials -----------------
+// Per tile: sum (rho - water) dz, sum water dz, sum rho dz.
+template <class S>
+__global__ void __launch_bounds__(256)
+    k_budget(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<S> s, double *partial) {
+    const int4 t = tiles[blockIdx.x];
+    __shared__ double red[3][256];
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    if (t.x >= 0) {
+        const Dom &D = doms[t.x];
+        const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+        if (i < D.nx && k < D.nz) {
+            const int64_t c = D.off2 + (int64_t)k * D.nx + i;
+            double rho = double(s.f[RHO][c]);
+            if (is_mixed<S>()) rho += zv.rho_bar[D.offz + k];
+            const double wat = double(s.f[QV][c]) + double(s.f[QC][c]) + double(s.f[QR][c]);
+            const double dz = zv.dz[D.offz + k];
+            a0 = (rho - wat) * dz;
+            a1 = wat * dz;
+            a2 = rho * dz;
+        }
+    }
+    red[0][tid] = a0;
+    red[1][tid] = a1;
+    red[2][tid] = a2;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (tid < st)
+            for (int j = 0; j < 3; ++j) red[j][tid] += red[j][tid + st];
+        __syncthreads();
+    }
+    if (tid < 3) partial[(int64_t)blockIdx.x * 3 + tid] = red[tid][0];
+}
+
+// One block per member: dry mass, total water, accumulated rain, total mass (times dx).
+__global__ void __launch_bounds__(256) k_budget_reduce(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
+                                                       int ntiles, const double *partial, const double *prec,
+                                                       double *out) {
+    const int m = blockIdx.x, tid = threadIdx.x;
+    const Dom &D = doms[m];
+    __shared__ double red[4][256];
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = tid; b < ntiles; b += blockDim.x) {
+        if (tiles[b].x != m) continue;
+        a[0] += partial[(int64_t)b * 3 + 0];
+        a[1] += partial[(int64_t)b * 3 + 1];
+        a[3] += partial[(int64_t)b * 3 + 2];
+    }
+    for (int i = tid; i < D.nx; i += blockDim.x) a[2] += prec[D.offx + i];
+    for (int j = 0; j < 4; ++j) red[j][tid] = a[j];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (tid < st)
+            for (int j = 0; j < 4; ++j) red[j][tid] += red[j][tid + st];
+        __syncthreads();
+    }
+    if (tid < 4) out[4 * m + tid] = red[tid][0] * D.dx;
+}

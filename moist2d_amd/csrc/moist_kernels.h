@@ -435,39 +435,4 @@ __global__ void k_download(Dom D, const S *__restrict__ src, const S *__restrict
     host[(int64_t)ih * ka + k] = H(v);
 }
 
-// FP64 reductions of one member: dry mass, total water, rain, mass.
-template <class S>
-__global__ void __launch_bounds__(256) k_diagnostics(Dom D, ZVec zv, SSet<S> s, const double *__restrict__ prec,
-                                                     double *__restrict__ out) {
-    __shared__ double red[4][256];
-    const int tid = threadIdx.x;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    const int64_t ncell = (int64_t)D.nx * D.nz;
-    for (int64_t q = tid; q < ncell; q += blockDim.x) {
-        const int k = int(q / D.nx);
-        const int64_t c = D.off2 + q;
-        double rho = double(s.f[RHO][c]);
-        if (is_mixed<S>()) rho += zv.rho_bar[D.offz + k];
-        const double wat = double(s.f[QV][c]) + double(s.f[QC][c]) + double(s.f[QR][c]);
-        const double dz = zv.dz[D.offz + k];
-        acc[0] += (rho - wat) * dz;
-        acc[1] += wat * dz;
-        acc[3] += rho * dz;
-    }
-    for (int i = tid; i < D.nx; i += blockDim.x) acc[2] += prec[D.offx + i];
-    for (int a = 0; a < 4; ++a) red[a][tid] = acc[a];
-    __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-        if (tid < st)
-            for (int a = 0; a < 4; ++a) red[a][tid] += red[a][tid + st];
-        __syncthreads();
-    }
-    if (tid == 0) {
-        out[0] = red[0][0] * D.dx;
-        out[1] = red[1][0] * D.dx;
-        out[2] = red[2][0] * D.dx;
-        out[3] = red[3][0] * D.dx;
-    }
-}
-
 }  // namespace mo
