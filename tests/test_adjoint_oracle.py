@@ -30,9 +30,9 @@ def _wrap(p, a):
     return a
 
 
-def _state(seed=1):
+def _state(seed=1, **kw):
     rng = np.random.default_rng(seed)
-    p, s = ic.bubble(np.float64, nx=16, nz=12, d=200.0)
+    p, s = ic.bubble(np.float64, nx=16, nz=12, d=200.0, **kw)
     s.rho_u[...] = 3.0 * s.rho
     O.rk3_step(s, p, 3)
     s.rho_qc[...] += 1.5e-3 * s.rho * (1 + 0.2 * rng.random(s.rho.shape))
@@ -128,8 +128,11 @@ def test_physics_adjoint_matches_fd():
         assert abs(fd - lK[j]) <= 5e-5 * abs(fd), (name, fd, lK[j])
 
 
-def test_rk3_step_adjoint_matches_fd():
-    p, s, rng = _state(4)
+@pytest.mark.parametrize("kw", [{}, dict(beta_offcenter=0.6), dict(ns=8)])
+def test_rk3_step_adjoint_matches_fd(kw):
+    """One step; beta = 0.6 makes the off-centering terms of the w-p right-hand
+    side (zero at the default beta = 1) and their adjoint active."""
+    p, s, rng = _state(4, **kw)
     W = [np.asfortranarray(rng.normal(size=s.rho.shape)) for _ in range(7)]
     for w in W:
         w[:, :p.halo] = 0

@@ -152,3 +152,11 @@ def test_adjoint_matches_fd_of_gpu_forward():
         errs.append(abs(fd - ad) / abs(ad))
     print("GPU FD rel errors", " ".join(f"{e:.1e}" for e in errs), flush=True)
     assert np.median(errs) <= 1e-5 and max(errs) <= 1e-4, errs
+
+
+@pytest.mark.parametrize("kw", [dict(beta_offcenter=0.6), dict(ns=8), dict(ns=1), dict(gamma_d=0.0)])
+def test_adjoint_edge_cases(kw):
+    """Off-centering (fact = (1 - beta)/(1 + beta) != 0 in the w-p right-hand side and its
+    adjoint), unequal stage substeps, one substep, no divergence damping."""
+    p, s, tt = _case(**kw)
+    _compare(f"edge {kw}", p, s, tt, 4, 2)

@@ -228,3 +228,33 @@ def test_autotune_keeps_parity():
         _print(f"autotuned prec={prec} L={L} NT={NT} ({us:.1f} us)", errs)
         for n, e in errs.items():
             assert e <= tol, (prec, n, e)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(beta_offcenter=0.6),           # off-centering terms active (zero at the default beta = 1)
+    dict(ns=1),                         # one substep in every stage
+    dict(ns=8),                         # 3, 4, 8 substeps: unequal stage dt_s
+    dict(gamma_d=0.0),                  # no divergence damping
+    dict(nz=3),                         # two unknowns in the w-p system
+    dict(nx=3),                         # halo of 2 on 3 columns (periodic wrap of the stencils)
+    dict(nx=70, nz=17),                 # ragged tiles in x and z
+])
+def test_fp64_edge_cases(kw):
+    nx, nz = kw.pop("nx", 48), kw.pop("nz", 24)
+    p, s = ic.bubble(np.float64, nx=nx, nz=nz, d=250.0, **kw)
+    s.rho_u[...] = 4.0 * s.rho  # a mean wind exercises both upwind directions' branches
+    ref = s.copy()
+    assert O.rk3_step(ref, p, 6) == 0
+    # the oracle's own change under a 1-ulp perturbation of rho_theta: on tiny
+    # columns (nz = 3) rho_w moves by 2e-11, so the bar is the larger of 1e-11
+    # and twice that sensitivity
+    pert = s.copy()
+    pert.rho_theta[...] = np.nextafter(pert.rho_theta, np.inf)
+    assert O.rk3_step(pert, p, 6) == 0
+    sens = field_errors(p, pert, ref)
+    out, _ = _run_gpu(p, s, 6)
+    errs = field_errors(p, out, ref)
+    _print(f"fp64 edge {kw} {nx}x{nz}", errs)
+    _print("   oracle 1-ulp sensitivity", sens)
+    for n, e in errs.items():
+        assert e <= max(TOL_FP64, 2.0 * sens[n]), (n, e, sens[n])
