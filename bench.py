@@ -392,7 +392,8 @@ def main():
     alg_bytes = ACOUSTIC_PASSES * cells * 4
     achieved = alg_bytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     sh = mixed["acoustic_shape"] or {"levels_per_chunk": 4, "threads": 1024}
-    kname = f"mo::k_acoustic<float, {sh['levels_per_chunk']}, {sh['threads']}>"
+    nt = sh["threads"]
+    kname = f"mo::k_acoustic<float, {sh['levels_per_chunk']}, {abs(nt)}, {8 if nt < 0 else 1}>"
     roof = {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname, p.Nx, p.Nz),
             "avg_launch_ms": kms, "launches": nlaunch, "alg_bytes_per_launch": alg_bytes}

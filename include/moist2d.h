@@ -159,14 +159,17 @@ int moist_history(moist_handle *h, int member, double *stats, void *const fields
 /* Tile-shape autotuner (no counterpart in the reference, whose loops have
  * no tiles): times the acoustic substep kernel on this handle's domains for
  * every shape of L levels per chunk (2, 4) and workgroup size (256, 512,
- * 1024 threads) that fits every member, `reps` launches each, and keeps the
+ * 1024 threads, and 512 / 1024 with registers for 8 waves per SIMD) that fits
+ * every member, `reps` launches each, and keeps the
  * fastest.  Outputs (may be NULL): the chosen L and threads and the
  * microseconds per launch.  Opt-in: different shapes combine the chunked
  * tridiagonal solve in a different order (results differ at rounding
  * level), so a handle keeps its default shape unless this is called. */
 int moist_autotune(moist_handle *h, int reps, int *levels, int *threads, double *us_per_launch);
-/* Sets the acoustic kernel's shape explicitly (levels 2 or 4, threads 256,
- * 512 or 1024). */
+/* Sets the acoustic kernel's shape explicitly: levels 2 or 4; threads 256,
+ * 512 or 1024, or -512 / -1024 for the same workgroup size compiled with
+ * registers for 8 waves per SIMD (two or four workgroups per CU).  The
+ * autotuner reports threads the same way. */
 int moist_set_acoustic_shape(moist_handle *h, int levels, int threads);
 
 /* Reverse mode (FP64-form handle, do_sf = do_rd = false).  Replaces

@@ -46,8 +46,10 @@ template <class S> struct AcBuf {
     int64_t cin_stride;
 };
 
-template <class S, int L, int NT>
-__global__ void __launch_bounds__(NT) k_acoustic(const Dom *__restrict__ doms, const int4 *__restrict__ atiles,
+// WPE: minimum waves per SIMD the register allocation must allow (1: no
+// constraint; 8 lets two 1024-thread workgroups share a CU).
+template <class S, int L, int NT, int WPE>
+__global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ doms, const int4 *__restrict__ atiles,
                                                  ZVec zv, SSet<S> in, Work<S> w, AcBuf<S> ab, int div,
                                                  double dt_ovr, int nsub, int first) {
     using R = S;

@@ -85,7 +85,8 @@ struct moist_handle {
     int2 *ctiles = nullptr;
     int nctiles = 0;
     int4 *atiles = nullptr;  // acoustic tiles {member, i0, TX, nch}
-    int ac_L = 4, ac_NT = 1024;  // levels per chunk and threads per workgroup of the acoustic kernel
+    int ac_L = 4, ac_NT = 1024, ac_W = 1;  // acoustic kernel: levels per chunk, threads per workgroup,
+                                           // minimum waves per SIMD of its register budget
     int tuned = 0;
     int2 *wgroups = nullptr;  // {member, i0} of the adjoint column solve's waves
     int nwgroups = 0, wch = 8;  // chunks per column in those waves
@@ -234,7 +235,7 @@ int make_atiles(const std::vector<Dom> &doms, int L, int NT, std::vector<int4> &
     return 0;
 }
 
-int set_acoustic_shape(moist_handle *h, int L, int NT) {
+int set_acoustic_shape(moist_handle *h, int L, int NT, int W = 1) {
     std::vector<int4> ta;
     int nch_max = 0;
     if (make_atiles(h->dom_h, L, NT, ta, nch_max)) {
@@ -252,6 +253,7 @@ int set_acoustic_shape(moist_handle *h, int L, int NT) {
     h->natiles = (int)ta.size();
     h->ac_L = L;
     h->ac_NT = NT;
+    h->ac_W = W;
     if (h->gexec) {
         hipGraphExecDestroy(h->gexec);
         h->gexec = nullptr;
@@ -259,25 +261,34 @@ int set_acoustic_shape(moist_handle *h, int L, int NT) {
     return 0;
 }
 
-template <class S, int L, int NT>
+template <class S, int L, int NT, int W>
 void launch_ac(moist_handle *h, const ZVec &zv, const SSet<S> &in, const Work<S> &w, const AcBuf<S> &ab, int div,
                double dt_ovr, int nsub, int first) {
-    hipLaunchKernelGGL((k_acoustic<S, L, NT>), dim3(h->natiles), dim3(NT), 0, h->stream, h->dom_d, h->atiles, zv, in,
-                       w, ab, div, dt_ovr, nsub, first);
+    hipLaunchKernelGGL((k_acoustic<S, L, NT, W>), dim3(h->natiles), dim3(NT), 0, h->stream, h->dom_d, h->atiles, zv,
+                       in, w, ab, div, dt_ovr, nsub, first);
 }
 
 template <class S>
 void launch_acoustic(moist_handle *h, const ZVec &zv, const SSet<S> &in, const Work<S> &w, const AcBuf<S> &ab,
                      int div, double dt_ovr, int nsub, int first) {
-    const int key = h->ac_L * 10000 + h->ac_NT;
+    const int key = (h->ac_L * 10000 + h->ac_NT) * 10 + (h->ac_W == 8 ? 8 : 1);
+#define AC_CASE(L_, NT_, W_)                                                                 \
+    case (L_ * 10000 + NT_) * 10 + W_:                                                       \
+        launch_ac<S, L_, NT_, W_>(h, zv, in, w, ab, div, dt_ovr, nsub, first);               \
+        break;
     switch (key) {
-    case 2 * 10000 + 256: launch_ac<S, 2, 256>(h, zv, in, w, ab, div, dt_ovr, nsub, first); break;
-    case 2 * 10000 + 512: launch_ac<S, 2, 512>(h, zv, in, w, ab, div, dt_ovr, nsub, first); break;
-    case 2 * 10000 + 1024: launch_ac<S, 2, 1024>(h, zv, in, w, ab, div, dt_ovr, nsub, first); break;
-    case 4 * 10000 + 256: launch_ac<S, 4, 256>(h, zv, in, w, ab, div, dt_ovr, nsub, first); break;
-    case 4 * 10000 + 512: launch_ac<S, 4, 512>(h, zv, in, w, ab, div, dt_ovr, nsub, first); break;
-    default: launch_ac<S, 4, 1024>(h, zv, in, w, ab, div, dt_ovr, nsub, first); break;
+        AC_CASE(2, 256, 1)
+        AC_CASE(2, 512, 1)
+        AC_CASE(2, 1024, 1)
+        AC_CASE(4, 256, 1)
+        AC_CASE(4, 512, 1)
+        AC_CASE(2, 512, 8)
+        AC_CASE(2, 1024, 8)
+        AC_CASE(4, 512, 8)
+        AC_CASE(4, 1024, 8)
+    default: launch_ac<S, 4, 1024, 1>(h, zv, in, w, ab, div, dt_ovr, nsub, first); break;
     }
+#undef AC_CASE
 }
 
 // ---- one RK stage ------------------------------------------------------------
@@ -1319,17 +1330,20 @@ int moist_kernel_stats(moist_handle *h, double *acoustic_ms_per_launch, int64_t 
 
 int moist_autotune(moist_handle *h, int reps, int *levels, int *threads, double *us_per_launch) {
     if (!h || reps < 1) return -2;
-    const int Ls[2] = {2, 4}, NTs[3] = {256, 512, 1024};
-    const int L0 = h->ac_L, NT0 = h->ac_NT;
-    int bestL = L0, bestNT = NT0;
+    // shapes: {L, NT, W}; W = 8 asks for registers that let 8 waves share a SIMD
+    const int shapes[10][3] = {{2, 256, 1}, {2, 512, 1}, {2, 1024, 1}, {4, 256, 1}, {4, 512, 1},
+                               {4, 1024, 1}, {2, 512, 8}, {2, 1024, 8}, {4, 512, 8}, {4, 1024, 8}};
+    const int L0 = h->ac_L, NT0 = h->ac_NT, W0 = h->ac_W;
+    int bestL = L0, bestNT = NT0, bestW = W0;
     double best = 1e300, t0 = 1e300;
     hipEvent_t a, b;
     HIPCHK(h, hipEventCreate(&a));
     HIPCHK(h, hipEventCreate(&b));
     const ZVec zv = zvec(h);
-    for (int L : Ls)
-        for (int NT : NTs) {
-            if (set_acoustic_shape(h, L, NT)) continue;  // shape does not fit every member
+    for (const auto &sh : shapes) {
+        const int L = sh[0], NT = sh[1], W = sh[2];
+        {
+            if (set_acoustic_shape(h, L, NT, W)) continue;  // shape does not fit every member
             // one stage's substeps on the current state and work arrays (outputs are work arrays only)
             auto run = [&]() {
                 for (int n = 0; n < reps; ++n) {
@@ -1354,36 +1368,43 @@ int moist_autotune(moist_handle *h, int reps, int *levels, int *threads, double 
             HIPCHK(h, hipGetLastError());
             float ms = 0.f;
             HIPCHK(h, hipEventElapsedTime(&ms, a, b));
-            if (L == L0 && NT == NT0) t0 = ms;
+            if (L == L0 && NT == NT0 && W == W0) t0 = ms;
             if (ms < best) {
                 best = ms;
                 bestL = L;
                 bestNT = NT;
+                bestW = W;
             }
         }
+    }
     hipEventDestroy(a);
     hipEventDestroy(b);
     // keep the current shape unless another is faster by more than 3 %
     if (!(best < 0.97 * t0)) {
         bestL = L0;
         bestNT = NT0;
+        bestW = W0;
         best = t0;
     }
-    if (int rc = set_acoustic_shape(h, bestL, bestNT)) return rc;
+    if (int rc = set_acoustic_shape(h, bestL, bestNT, bestW)) return rc;
     h->tuned = 1;
     if (levels) *levels = bestL;
-    if (threads) *threads = bestNT;
+    if (threads) *threads = bestNT * (bestW == 8 ? -1 : 1);
     if (us_per_launch) *us_per_launch = best * 1e3 / reps;
     return 0;
 }
 
 int moist_set_acoustic_shape(moist_handle *h, int levels, int threads) {
     if (!h) return -2;
-    if ((levels != 2 && levels != 4) || (threads != 256 && threads != 512 && threads != 1024)) {
-        h->err = "acoustic shape: levels 2 or 4, threads 256, 512 or 1024";
+    // threads < 0: the same workgroup size with registers for 8 waves per SIMD
+    const int W = threads < 0 ? 8 : 1, NT = threads < 0 ? -threads : threads;
+    const bool ok = (levels == 2 || levels == 4) &&
+                    (W == 1 ? (NT == 256 || NT == 512 || NT == 1024) : (NT == 512 || NT == 1024));
+    if (!ok) {
+        h->err = "acoustic shape: levels 2 or 4; threads 256, 512 or 1024, or -512 / -1024 for 8 waves per SIMD";
         return -2;
     }
-    return set_acoustic_shape(h, levels, threads);
+    return set_acoustic_shape(h, levels, NT, W);
 }
 
 int moist_adjoint_run(moist_handle *h, int nsteps, int ckpt, const void *const *theta_target, int elem_bytes,

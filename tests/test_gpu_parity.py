@@ -190,7 +190,8 @@ def test_conservation_coldpool_1024x256():
         assert abs(dw) <= tol and abs(dm) <= tol and abs(dd) <= tol
 
 
-@pytest.mark.parametrize("levels,threads", [(2, 256), (2, 512), (2, 1024), (4, 256), (4, 512), (4, 1024)])
+@pytest.mark.parametrize("levels,threads", [(2, 256), (2, 512), (2, 1024), (4, 256), (4, 512), (4, 1024),
+                                            (2, -512), (2, -1024), (4, -512), (4, -1024)])
 def test_fp64_every_acoustic_shape(levels, threads):
     """Every shape the autotuner may pick keeps the FP64 parity bar."""
     p, s = ic.bubble(np.float64)
@@ -217,7 +218,7 @@ def test_autotune_keeps_parity():
         eng = core.Engine(p, prec)
         eng.upload(s)
         L, NT, us = eng.autotune(reps=5)
-        assert L in (2, 4) and NT in (256, 512, 1024) and us > 0
+        assert L in (2, 4) and NT in (256, 512, 1024, -512, -1024) and us > 0
         eng.upload(s)
         eng.use_graph(True)
         eng.step(10)
