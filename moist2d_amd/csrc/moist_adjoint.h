@@ -188,6 +188,7 @@ struct AdjW {
     double *Lth, *Lrt, *Lpres, *LU, *LW, *LRr, *LRu, *LRw, *LRt, *Lmx, *Lmz;
     double *Lrp, *Lrwp, *Lrtp, *pend, *Linc_r, *Linc_t;
     double *Y, *cpr, *rmi, *aco;  // column scratch: transposed-solve vector; c', 1/m, a of the forward Thomas
+    double *q[3], *uf, *wf;       // stage input's q = rho_q / rho and face velocities (moist.jl:1091-1097, 1184-1189)
     double *linb, *lint;       // per column: positivity-limiter adjoints (3 scalars x columns)
 };
 
@@ -225,6 +226,26 @@ __global__ void __launch_bounds__(256) ka_loss(const Dom *__restrict__ doms, SSe
     if (threadIdx.x == 0) J[m] = red[0];
 }
 
+// ---- per-stage point values the gathers read many times -----------------------------
+// q = rho_q / rho of the three scalars, u = 2 rho_u / (rho_i + rho_i+1) and
+// w = rho_w / f2h(rho_k, rho_k+1) of the stage input, with the forward's
+// expressions.
+__global__ void __launch_bounds__(AD_X *AD_Z)
+    ka_pre(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s, AdjW a) {
+    const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;
+    const Dom &D = doms[t.x];
+    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+    const int nx = D.nx, nz = D.nz;
+    if (i >= nx || k >= nz) return;
+    const double *dzd = zv.dz + D.offz;
+    const int64_t c = D.off2 + (int64_t)k * nx + i, cp = D.off2 + (int64_t)k * nx + wrapn(i + 1, nx);
+    const double *rho = s.f[RHO];
+    for (int sc = 0; sc < 3; ++sc) a.q[sc][c] = s.f[QV + sc][c] / rho[c];
+    a.uf[c] = s.f[RHOU][c] * 2.0 / (rho[c] + rho[cp]);
+    a.wf[c] = k < nz - 1 ? s.f[RHOW][c] / f2h(rho[c], rho[c + nx], dzd[k], dzd[k + 1]) : 0.0;
+}
+
 // ---- scalar transport + final update (moist.jl:1469-1539) ------------------------
 // lo: adjoint of the stage output; writes ls (stage input, first writer of the
 // stage), adds into ls0 (rho_q) and lph (q tendencies); writes Lmx, Lmz and
@@ -249,7 +270,8 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     for (int sc = 0; sc < 3; ++sc) {
         const double *qr = s.f[QV + sc];
         const double *L = lo.f[QV + sc];
-        auto Q = [&](int kk, int ii) { const int64_t g = G(kk, ii); return qr[g] / rho[g]; };
+        const double *qa = a.q[sc];
+        auto Q = [&](int kk, int ii) { return qa[G(kk, ii)]; };
         auto LFx = [&](int j) { return dt / dx * (L[G(k, j + 1)] - L[G(k, j)]); };
         auto LFz = [&](int kk) { return dt * (L[G(kk + 1, i)] / dzd[kk + 1] - L[G(kk, i)] / dzd[kk]); };
         ls0.f[QV + sc][c] += L[c];
@@ -757,14 +779,8 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     const int64_t c = G(k, i);
     const double *rho = s.f[RHO], *ru = s.f[RHOU], *rw = s.f[RHOW];
     const double *TH = a.theta;
-    auto U = [&](int kk, int ii) {
-        const int64_t q = G(kk, ii);
-        return ru[q] * 2.0 / (rho[q] + rho[G(kk, ii + 1)]);
-    };
-    auto W = [&](int kk, int ii) {  // kk <= ke - 1
-        const int64_t q = G(kk, ii);
-        return rw[q] / f2h(rho[q], rho[G(kk + 1, ii)], dzd[kk], dzd[kk + 1]);
-    };
+    auto U = [&](int kk, int ii) { return a.uf[G(kk, ii)]; };
+    auto W = [&](int kk, int ii) { return a.wf[G(kk, ii)]; };  // kk <= ke - 1
     auto zz = [&](int kk) { return zd[kk + 1] - zd[kk]; };
     double l_r = 0.0, l_u = 0.0, l_w = 0.0, LUc = 0.0, LWc = 0.0, lp = 0.0, lth = 0.0;
 
@@ -1156,26 +1172,43 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     l_rt += lph.f[2][c] * sedc / rho[c];
     l_r += -lph.f[2][c] * sedc * rt[c] / (rho[c] * rho[c]);
     if (mp) {
-        typedef Dl<8> V;
-        V out[4];
-        kessler_v<V>(D, V::var(rho[c], 0), V::var(rt[c], 1), V::var(s.f[QV][c], 2), V::var(s.f[QC][c], 3),
-                     V::var(rqr[c], 4), V::var(D.K_auto, 5), V::var(D.Qc_threshold, 6), V::var(D.K_accr, 7), out);
+        // partials w.r.t. the five state fields, then w.r.t. the three parameters
+        // (two narrower dual evaluations of the same formula keep the registers in bounds)
         const double lo[4] = {lph.f[3][c], lph.f[4][c], lph.f[5][c], lph.f[2][c]};
-        double g[8];
+        {
+            typedef Dl<5> V;
+            V out[4];
+            kessler_v<V>(D, V::var(rho[c], 0), V::var(rt[c], 1), V::var(s.f[QV][c], 2), V::var(s.f[QC][c], 3),
+                         V::var(rqr[c], 4), V(D.K_auto), V(D.Qc_threshold), V(D.K_accr), out);
+            double g[5];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            g[j] = 0.0;
+            for (int j = 0; j < 5; ++j) {
+                g[j] = 0.0;
 #pragma unroll
-            for (int o = 0; o < 4; ++o) g[j] += lo[o] * out[o].d[j];
+                for (int o = 0; o < 4; ++o) g[j] += lo[o] * out[o].d[j];
+            }
+            l_r += g[0];
+            l_rt += g[1];
+            ls.f[QV][c] += g[2];
+            ls.f[QC][c] += g[3];
+            l_qr += g[4];
         }
-        l_r += g[0];
-        l_rt += g[1];
-        ls.f[QV][c] += g[2];
-        ls.f[QC][c] += g[3];
-        l_qr += g[4];
-        gk0[c] += g[5];
-        gk1[c] += g[6];
-        gk2[c] += g[7];
+        {
+            typedef Dl<3> V;
+            V out[4];
+            kessler_v<V>(D, V(rho[c]), V(rt[c]), V(s.f[QV][c]), V(s.f[QC][c]), V(rqr[c]), V::var(D.K_auto, 0),
+                         V::var(D.Qc_threshold, 1), V::var(D.K_accr, 2), out);
+            double g[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                g[j] = 0.0;
+#pragma unroll
+                for (int o = 0; o < 4; ++o) g[j] += lo[o] * out[o].d[j];
+            }
+            gk0[c] += g[0];
+            gk1[c] += g[1];
+            gk2[c] += g[2];
+        }
     }
     if (sed) {
         // F(k) = rho_qr Vt; d_sed(k) = (F(k+1) [k<ke] - F(k)) / dz_k; d_prec = F(0)

@@ -617,7 +617,7 @@ template <class S, class H> int history_t(moist_handle *h, int m, double *stats,
 // rho_w'), and the adjoint fields.
 enum AdjWork {
     W_TH = 0, W_RT, W_PRES, W_LU, W_LW, W_LRR, W_LRU, W_LRW, W_LRT, W_LMX, W_LMZ, W_LRP, W_LRWP, W_LRTP, W_PEND,
-    W_LINCR, W_LINCT, W_Y, W_CPR, W_RMI, W_ACO, W_LRUP0, W_LRUP1, NWORK
+    W_LINCR, W_LINCT, W_Y, W_CPR, W_RMI, W_ACO, W_LRUP0, W_LRUP1, W_Q0, W_Q1, W_Q2, W_UF, W_WF, NWORK
 };
 enum { REC_TH = 0, REC_PRES, REC_RT2, REC_RR, REC_RRU, REC_RRW, REC_RRT, REC_MX, REC_MZ, NREC };
 
@@ -747,6 +747,9 @@ AdjW adj_w(moist_handle *h, int st) {
     a.cpr = wk(h, W_CPR);
     a.rmi = wk(h, W_RMI);
     a.aco = wk(h, W_ACO);
+    for (int sc = 0; sc < 3; ++sc) a.q[sc] = wk(h, W_Q0 + sc);
+    a.uf = wk(h, W_UF);
+    a.wf = wk(h, W_WF);
     a.linb = h->adj->lim;
     a.lint = h->adj->lim + 3 * h->nxs;
     return a;
@@ -833,6 +836,7 @@ void enqueue_step_adj(moist_handle *h, const double *sn) {
         const AdjW a = adj_w(h, st);
         for (int wi : {W_TH, W_RT, W_LRR, W_LRU, W_LRW, W_LRT, W_PEND}) fill(h, wk(h, wi), n2);
         double *Lrup[2] = {wk(h, W_LRUP0), wk(h, W_LRUP1)};
+        hipLaunchKernelGGL(ka_pre, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, a);
         hipLaunchKernelGGL(ka_scalar, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, dset(lo, n2),
                            dset(li, n2), ls0, lph, a, Lrup[0], div);
         hipLaunchKernelGGL(ka_lin, dim3(h->nm), dim3(256), 0, h->stream, h->dom_d, a, dset(li, n2));
