@@ -168,17 +168,9 @@ def _loss_np(p, a, tt):
     return float(p.dx) * a.prec[h:h + p.Nx].sum() + 0.5 * np.sum((th - tt[:, h:h + p.Nx]) ** 2)
 
 
-def adjoint_bench(steps=600, ckpt=50, fd_dirs=32, cpu_steps=2):
-    """The 512x128 adjoint workload (FP64 form): loss J = dx sum(prec) +
-    1/2 |theta - theta_target|^2 after 600 steps, gradient w.r.t. the initial
-    theta' and qv (at fixed density) and K_auto, Qc_threshold, K_accr;
-    checkpoints every 50 steps.  Central FD of the GPU's own FP64 forward on
-    32 seeded directions checks the gradient: smooth fields of unit RMS
-    scaled by the RMS of the initial theta' (theta - base profile) or qv,
-    step h = 1e-6 relative."""
-    p, s = ic.adjoint_bubble(np.float64, steps=steps)
-    tt = ic.theta_target(p)
-    eng = core.Engine(p, core.FP64)
+def _adjoint_form(p, s, tt, precision, steps, ckpt):
+    """Forward timing and one checkpointed adjoint run in one form."""
+    eng = core.Engine(p, precision)
     eng.store_phys_diag(False)
     eng.use_graph(True)
     eng.upload(s)
@@ -192,12 +184,31 @@ def adjoint_bench(steps=600, ckpt=50, fd_dirs=32, cpu_steps=2):
     ms_fwd, ms_rev = eng.adjoint_times()
     g, gK = eng.adjoint_grad(0)
     st_b, ad_b = eng.device_bytes()
+    res = {"acoustic_shape": shape, "J": J, "dJ_dK": [float(x) for x in gK], "forward_ms": ms_plain,
+           "forward_with_checkpoints_ms": ms_fwd, "reverse_ms": ms_rev, "adjoint_forward_ratio": ms_rev / ms_plain,
+           "device_bytes": st_b + ad_b}
+    return eng, res, g, gK
+
+
+def adjoint_bench(steps=600, ckpt=50, fd_dirs=32, cpu_steps=2):
+    """The 512x128 adjoint workload: loss J = dx sum(prec) + 1/2 |theta -
+    theta_target|^2 after 600 steps, gradient w.r.t. the initial theta' and qv
+    (at fixed density) and K_auto, Qc_threshold, K_accr; checkpoints every 50
+    steps; FP64 form, and the mixed form (mixed-form trajectory, FP64 step
+    adjoints along it).  Central FD of the GPU's own FP64 forward on 32
+    seeded directions checks both gradients: smooth fields of unit RMS scaled
+    by the RMS of the initial theta' (theta - base profile) or qv, step
+    h = 1e-6 relative (an FP32 trajectory cannot resolve h = 1e-6, so the
+    mixed gradient is checked against the FP64 forward's differences)."""
+    p, s = ic.adjoint_bubble(np.float64, steps=steps)
+    tt = ic.theta_target(p)
+    eng, out, g, gK = _adjoint_form(p, s, tt, core.FP64, steps, ckpt)
     rng = np.random.default_rng(100)
     hh = p.halo
     th0 = (s.rho_theta / s.rho)[:, hh:hh + p.Nx] - ic.theta_profile(p.z)[:, None]
     qv0 = (s.rho_qv / s.rho)[:, hh:hh + p.Nx]
     rms = {3: float(np.sqrt(np.mean(th0 ** 2))), 4: float(np.sqrt(np.mean(qv0 ** 2)))}
-    errs = []
+    errs, fds = [], []
     for d in range(fd_dirs):
         f, n = (3, "rho_theta") if d % 2 == 0 else (4, "rho_qv")
         v = _smooth_direction(p, rng)
@@ -214,18 +225,30 @@ def adjoint_bench(steps=600, ckpt=50, fd_dirs=32, cpu_steps=2):
             eng.download(a, phys=False)
             vals.append(_loss_np(p, a, tt))
         fd = (vals[0] - vals[1]) / (2 * eps)
-        h = p.halo
-        ad = float(np.sum(g[f][:, h:h + p.Nx] * v[:, h:h + p.Nx]))
+        fds.append((f, v, fd))
+        ad = float(np.sum(g[f][:, hh:hh + p.Nx] * v[:, hh:hh + p.Nx]))
         errs.append(abs(fd - ad) / abs(ad))
     eng.close()
-    out = {"workload": f"adjoint_bubble_{p.Nx}x{p.Nz}", "form": "fp64", "steps": steps, "ckpt": ckpt,
-           "acoustic_shape": shape,
-           "dt": float(p.dt), "ns": p.ns, "J": J, "dJ_dK": [float(x) for x in gK],
-           "forward_ms": ms_plain, "forward_with_checkpoints_ms": ms_fwd, "reverse_ms": ms_rev,
-           "adjoint_forward_ratio": ms_rev / ms_plain, "device_bytes": st_b + ad_b,
-           "fd_check": {"directions": fd_dirs, "h": 1e-6, "controls": "theta' and qv at fixed rho",
-                        "direction_scale": {"theta_K": rms[3], "qv": rms[4]},
-                        "median_rel_err": float(np.median(errs)), "max_rel_err": float(max(errs))}}
+    out = dict({"workload": f"adjoint_bubble_{p.Nx}x{p.Nz}", "form": "fp64", "steps": steps, "ckpt": ckpt,
+                "dt": float(p.dt), "ns": p.ns}, **out)
+    out["fd_check"] = {"directions": fd_dirs, "h": 1e-6, "controls": "theta' and qv at fixed rho",
+                       "direction_scale": {"theta_K": rms[3], "qv": rms[4]},
+                       "median_rel_err": float(np.median(errs)), "max_rel_err": float(max(errs))}
+    # mixed form
+    eng, mx, gm, gKm = _adjoint_form(p, s, tt, core.MIXED, steps, ckpt)
+    eng.close()
+    errs_m = []
+    for f, v, fd in fds:
+        ad = float(np.sum(gm[f][:, hh:hh + p.Nx] * v[:, hh:hh + p.Nx]))
+        errs_m.append(abs(fd - ad) / abs(ad))
+    mx["form"] = "mixed"
+    mx["vs_fp64"] = {"J_rel": abs(mx["J"] - out["J"]) / abs(out["J"]),
+                     "grad_theta_rel_l2": rel_l2(gm[3][:, hh:hh + p.Nx], g[3][:, hh:hh + p.Nx]),
+                     "grad_qv_rel_l2": rel_l2(gm[4][:, hh:hh + p.Nx], g[4][:, hh:hh + p.Nx]),
+                     "dJ_dK_rel": [abs(a - b) / max(abs(b), 1e-300) for a, b in zip(gKm, gK)]}
+    mx["fd_check_vs_fp64_forward"] = {"directions": fd_dirs, "median_rel_err": float(np.median(errs_m)),
+                                      "max_rel_err": float(max(errs_m))}
+    out["mixed"] = mx
     if cpu_steps:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O  # the checker, timed here as the CPU baseline only
@@ -235,15 +258,17 @@ def adjoint_bench(steps=600, ckpt=50, fd_dirs=32, cpu_steps=2):
         O.adjoint_run(s.copy(), p, cpu_steps, tt, ckpt=cpu_steps)
         cpu = (time.perf_counter() - t0) / cpu_steps
         out["cpu_oracle_ms_per_step"] = cpu * 1e3
-        out["gpu_ms_per_step"] = (ms_fwd + ms_rev) / steps
+        out["gpu_ms_per_step"] = (out["forward_with_checkpoints_ms"] + out["reverse_ms"]) / steps
+        out["gpu_ms_per_step_mixed"] = (mx["forward_with_checkpoints_ms"] + mx["reverse_ms"]) / steps
     return out
 
 
-def batched_adjoint_bench(members=64, steps=300, ckpt=50, seed=4):
+def batched_adjoint_bench(members=64, steps=300, ckpt=50, seed=4, forms=("mixed", "fp64")):
     """64 bubbles at 1024x128 (dx = dz = 100 m, dt = 1 s, ns = 6) with seeded
     amplitude / position and K_auto ~ U[5e-4, 2e-3]: forward + checkpointed
-    reverse sweep of every member's loss in one batched launch sequence
-    (FP64 form)."""
+    reverse sweep of every member's loss in one batched launch sequence, in
+    the mixed form (FP32 bulk, FP64 tridiagonal and step adjoints) and in the
+    FP64 form."""
     mem = []
     for m in range(members):
         u = [float(ic.rng_uniform(seed, m, 2000 + j, 0, 0)) for j in range(5)]
@@ -251,32 +276,39 @@ def batched_adjoint_bench(members=64, steps=300, ckpt=50, seed=4):
                          xc=(0.3 + 0.4 * u[1]) * 1024 * 100.0, zc=1500.0 + 1000.0 * u[2], seed=seed, member=m,
                          K_auto=5e-4 + 1.5e-3 * u[3])
         mem.append((p, s, ic.theta_target(p, seed=seed, member=m)))
-    eng = core.Engine([p for p, _, _ in mem], core.FP64)
-    eng.store_phys_diag(False)
-    eng.use_graph(True)
-    for m, (p, s, _) in enumerate(mem):
-        eng.upload(s, member=m)
-    shape = tune(eng)
-    for m, (p, s, _) in enumerate(mem):
-        eng.upload(s, member=m)
-    eng.time_steps(2)
-    for m, (p, s, _) in enumerate(mem):
-        eng.upload(s, member=m)
-    ms_plain = eng.time_steps(steps)
-    for m, (p, s, _) in enumerate(mem):
-        eng.upload(s, member=m)
-    J = eng.adjoint(steps, [t for _, _, t in mem], ckpt=ckpt)
-    ms_fwd, ms_rev = eng.adjoint_times()
-    _, gK0 = eng.adjoint_grad(0)
-    st_b, ad_b = eng.device_bytes()
-    eng.close()
     cells = sum(p.Nx * p.Nz for p, _, _ in mem)
-    return {"workload": f"batched_adjoint_{members}x1024x128", "form": "fp64", "members": members, "cells": cells,
-            "acoustic_shape": shape,
-            "steps": steps, "ckpt": ckpt, "forward_ms": ms_plain, "reverse_ms": ms_rev,
-            "adjoint_forward_ratio": ms_rev / ms_plain,
-            "gradients_per_s": members / ((ms_fwd + ms_rev) * 1e-3), "device_bytes": st_b + ad_b,
-            "device_GiB": (st_b + ad_b) / 2 ** 30, "J_member0": J[0], "dJ_dK_member0": [float(x) for x in gK0]}
+    res = {}
+    for form in forms:
+        eng = core.Engine([p for p, _, _ in mem], core.MIXED if form == "mixed" else core.FP64)
+        eng.store_phys_diag(False)
+        eng.use_graph(True)
+        for m, (p, s, _) in enumerate(mem):
+            eng.upload(s, member=m)
+        shape = tune(eng)
+        for m, (p, s, _) in enumerate(mem):
+            eng.upload(s, member=m)
+        eng.time_steps(2)
+        for m, (p, s, _) in enumerate(mem):
+            eng.upload(s, member=m)
+        ms_plain = eng.time_steps(steps)
+        for m, (p, s, _) in enumerate(mem):
+            eng.upload(s, member=m)
+        J = eng.adjoint(steps, [t for _, _, t in mem], ckpt=ckpt)
+        ms_fwd, ms_rev = eng.adjoint_times()
+        _, gK0 = eng.adjoint_grad(0)
+        st_b, ad_b = eng.device_bytes()
+        eng.close()
+        res[form] = {"workload": f"batched_adjoint_{members}x1024x128", "form": form, "members": members,
+                     "cells": cells, "acoustic_shape": shape, "steps": steps, "ckpt": ckpt, "forward_ms": ms_plain,
+                     "forward_with_checkpoints_ms": ms_fwd, "reverse_ms": ms_rev,
+                     "adjoint_forward_ratio": ms_rev / ms_plain,
+                     "gradients_per_s": members / ((ms_fwd + ms_rev) * 1e-3), "device_bytes": st_b + ad_b,
+                     "device_GiB": (st_b + ad_b) / 2 ** 30, "J_member0": J[0],
+                     "dJ_dK_member0": [float(x) for x in gK0]}
+    out = dict(res[forms[0]])
+    for f in forms[1:]:
+        out[f] = res[f]
+    return out
 
 
 def cpu_baseline(p, s, steps):

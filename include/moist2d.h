@@ -127,8 +127,14 @@ int moist_diagnostics(moist_handle *h, int member, double *out);
 int moist_use_graph(moist_handle *h, int on);
 
 /* Options: MOIST_OPT_PHYS_DIAG (default 1) stores the State physics
- * diagnostics of moist_download_phys at every step; 0 skips those writes.  */
+ * diagnostics of moist_download_phys at every step; 0 skips those writes.
+ * MOIST_OPT_ADJ_RECORDS selects how moist_adjoint_run keeps the forward
+ * records of its reverse sweep: 0 (default) whole-segment records when they
+ * take at most a quarter of the free device memory, else per step; 1 per
+ * step (each step's record recomputed from the segment's states); 2 whole
+ * segment.  Both give the same gradient.                                    */
 #define MOIST_OPT_PHYS_DIAG 0
+#define MOIST_OPT_ADJ_RECORDS 1
 int moist_set_option(moist_handle *h, int option, int value);
 
 /* Time nsteps large steps with HIP events on the handle's stream (state
@@ -172,7 +178,7 @@ int moist_autotune(moist_handle *h, int reps, int *levels, int *threads, double 
  * autotuner reports threads the same way. */
 int moist_set_acoustic_shape(moist_handle *h, int levels, int threads);
 
-/* Reverse mode (FP64-form handle, do_sf = do_rd = false).  Replaces
+/* Reverse mode (do_sf = do_rd = false).  Replaces
  * ForwardDiff / a ChainRulesCore rrule around the reference's driver loop
  * (run_sim, moist.jl:1930-1963, calling rk3_step!, moist.jl:1589): runs
  * nsteps forward from the uploaded state with checkpoints every `ckpt`
@@ -181,7 +187,11 @@ int moist_set_acoustic_shape(moist_handle *h, int levels, int threads);
  * and sweeps back, recomputing each segment from its checkpoint and each
  * step's acoustic substeps.  theta_target[m]: host array (ka, ia) in the
  * reference layout (halo columns ignored).  J[m] receives the loss.  The
- * handle's state afterwards is the final forward state. */
+ * handle's state afterwards is the final forward state.  The trajectory
+ * (forward pass, checkpoints, segment recomputation) advances in the
+ * handle's form; each step is recorded and reversed in FP64 from that
+ * trajectory's state, so a mixed-form handle gives the FP64 step's adjoint
+ * along its mixed-form trajectory (checkpoints take half the memory). */
 int moist_adjoint_run(moist_handle *h, int nsteps, int ckpt, const void *const *theta_target, int elem_bytes,
                       double *J);
 /* dJ/d(initial rho, rho_u, rho_w, rho_theta, rho_qv, rho_qc, rho_qr) of one

@@ -157,6 +157,11 @@ class Engine:
     def store_phys_diag(self, on=True):
         self._check(lib().moist_set_option(self._h, 0, 1 if on else 0), "moist_set_option")
 
+    def adjoint_records(self, mode):
+        """Forward records of the reverse sweep: "auto", "step" or "segment"."""
+        v = {"auto": 0, "step": 1, "segment": 2}[mode]
+        self._check(lib().moist_set_option(self._h, 1, v), "moist_set_option")
+
     def kernel_events(self, on=True):
         self._check(lib().moist_kernel_events(self._h, 1 if on else 0), "moist_kernel_events")
 
@@ -167,7 +172,9 @@ class Engine:
 
     def adjoint(self, nsteps, theta_target, ckpt=50):
         """Loss and reverse sweep over `nsteps` large steps from the uploaded
-        state (FP64 form).  theta_target: one (ka, ia) array per member.
+        state.  The trajectory advances in the handle's form (FP64 or mixed);
+        each step is linearised in FP64 about that trajectory's state.
+        theta_target: one (ka, ia) array per member.
         Returns J per member; gradients come from `adjoint_grad`."""
         tt = [np.asfortranarray(t, dtype=np.float64) for t in theta_target]
         self._keep_tt = tt

@@ -198,6 +198,26 @@ __global__ void ka_fill(double *p, int64_t n, double v) {
         p[q] = v;
 }
 
+// Full FP64 state from a mixed-form state (FP32 deviations; rho and
+// rho_theta about the FP64 horizontal-mean base state), one cell per thread.
+__global__ void __launch_bounds__(AD_X *AD_Z)
+    ka_widen(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<float> in,
+             SSet<double> out) {
+    const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;
+    const Dom &D = doms[t.x];
+    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+    if (i >= D.nx || k >= D.nz) return;
+    const int64_t c = D.off2 + (int64_t)k * D.nx + i;
+#pragma unroll
+    for (int f = 0; f < 7; ++f) {
+        double v = double(in.f[f][c]);
+        if (f == RHO) v += zv.rho_bar[D.offz + k];
+        if (f == RT) v += zv.rt_bar[D.offz + k];
+        out.f[f][c] = v;
+    }
+}
+
 // ---- loss: J = dx sum prec + 1/2 sum (theta - theta_target)^2 -------------------
 // One block per member: writes J[m] and the adjoint of the final state.
 __global__ void __launch_bounds__(256) ka_loss(const Dom *__restrict__ doms, SSet<double> s, const double *prec,

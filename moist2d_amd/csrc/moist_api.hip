@@ -118,6 +118,7 @@ struct moist_handle {
     int64_t ac_launches = 0;
     // reverse mode (allocated by the first moist_adjoint_run)
     AdjState *adj = nullptr;
+    int adj_mode = 0;  // forward records of the reverse sweep: 0 automatic, 1 per step, 2 per segment
     int64_t dev_bytes = 0;  // device memory of the state pools and tables
     // output_history: derived fields (allocated on first request), block partials, per-member stats
     double *hist_fields = nullptr, *hist_part = nullptr, *hist_stats = nullptr, *diag_all = nullptr;
@@ -152,39 +153,44 @@ template <class S> SSet<S> sset(moist_handle *h, int first) {
     return s;
 }
 
-template <class S> Work<S> work(moist_handle *h) {
+// Views of the work slots in the handle's pools, or in pools of the same
+// layout given by `pool` / `pool1`.
+template <class S> Work<S> work(moist_handle *h, void *pool = nullptr, void *pool1 = nullptr) {
+    S *const p2 = pool ? (S *)pool : (S *)h->pool, *const p1 = pool1 ? (S *)pool1 : (S *)h->pool1;
+    auto P = [&](int s) { return p2 + (size_t)s * h->n2; };
+    auto P1 = [&](int s) { return p1 + (size_t)s * h->nxs; };
     Work<S> w;
-    w.theta = slot<S>(h, THETA);
-    w.pres = slot<S>(h, PRES);
-    w.rt2 = slot<S>(h, RT2);
-    w.uf = slot<S>(h, UF);
-    w.wf = slot<S>(h, WF);
-    w.R_rho = slot<S>(h, R_RHO);
-    w.R_rhou = slot<S>(h, R_RHOU);
-    w.R_rhow = slot<S>(h, R_RHOW);
-    w.R_rt = slot<S>(h, R_RT);
-    w.rp = slot<S>(h, RP);
-    w.rup = slot<S>(h, RUP);
-    w.rwp = slot<S>(h, RWP);
-    w.mom_x = slot<S>(h, MOMX);
-    w.mom_z = slot<S>(h, MOMZ);
-    w.d_rho = slot<S>(h, D_RHO);
-    w.d_rt = slot<S>(h, D_RT);
-    w.d_qv = slot<S>(h, D_QV);
-    w.d_qc = slot<S>(h, D_QC);
-    w.d_qr = slot<S>(h, D_QR);
-    w.d_u2d = slot<S>(h, D_U2D);
-    w.d_u_sf = slot1<S>(h, X_DU_SF);
-    w.d_th_sf = slot1<S>(h, X_DTH_SF);
-    w.d_qv_sf = slot1<S>(h, X_DQV_SF);
-    w.mflux = slot1<S>(h, X_MFLUX);
-    w.shflux = slot1<S>(h, X_SHFLUX);
-    w.lhflux = slot1<S>(h, X_LHFLUX);
-    w.dg_th_mp = slot<S>(h, DG_TH_MP);
-    w.dg_qv_mp = slot<S>(h, DG_QV_MP);
-    w.dg_qr_mp = slot<S>(h, DG_QR_MP);
-    w.dg_qr_sed = slot<S>(h, DG_QR_SED);
-    w.d_th_rd = slot<S>(h, D_TH_RD);
+    w.theta = P(THETA);
+    w.pres = P(PRES);
+    w.rt2 = P(RT2);
+    w.uf = P(UF);
+    w.wf = P(WF);
+    w.R_rho = P(R_RHO);
+    w.R_rhou = P(R_RHOU);
+    w.R_rhow = P(R_RHOW);
+    w.R_rt = P(R_RT);
+    w.rp = P(RP);
+    w.rup = P(RUP);
+    w.rwp = P(RWP);
+    w.mom_x = P(MOMX);
+    w.mom_z = P(MOMZ);
+    w.d_rho = P(D_RHO);
+    w.d_rt = P(D_RT);
+    w.d_qv = P(D_QV);
+    w.d_qc = P(D_QC);
+    w.d_qr = P(D_QR);
+    w.d_u2d = P(D_U2D);
+    w.d_u_sf = P1(X_DU_SF);
+    w.d_th_sf = P1(X_DTH_SF);
+    w.d_qv_sf = P1(X_DQV_SF);
+    w.mflux = P1(X_MFLUX);
+    w.shflux = P1(X_SHFLUX);
+    w.lhflux = P1(X_LHFLUX);
+    w.dg_th_mp = P(DG_TH_MP);
+    w.dg_qv_mp = P(DG_QV_MP);
+    w.dg_qr_mp = P(DG_QR_MP);
+    w.dg_qr_sed = P(DG_QR_SED);
+    w.d_th_rd = P(D_TH_RD);
     w.prec = h->prec;
     w.sfh = h->sfh;
     w.theta_ref = h->theta_ref;
@@ -610,11 +616,16 @@ template <class S, class H> int history_t(moist_handle *h, int m, double *stats,
     return 0;
 }
 
-// ---- reverse mode (FP64 form) ------------------------------------------------------
+// ---- reverse mode ------------------------------------------------------------------
 // Buffers of one adjoint run: checkpoints every `ckpt` steps, the states of
 // one segment (recomputed from its checkpoint), the forward record of one
 // step (stage diagnostics and tendencies, per-substep rho_theta', rho_u',
 // rho_w'), and the adjoint fields.
+//
+// The trajectory (forward pass, checkpoints, segment recomputation) advances
+// in the handle's form.  Each step is then recorded by an FP64 forward step
+// from that trajectory's state and reversed in FP64, so in the mixed form the
+// gradient is the FP64 step's adjoint along the mixed-form trajectory.
 enum AdjWork {
     W_TH = 0, W_RT, W_PRES, W_LU, W_LW, W_LRR, W_LRU, W_LRW, W_LRT, W_LMX, W_LMZ, W_LRP, W_LRWP, W_LRTP, W_PEND,
     W_LINCR, W_LINCT, W_Y, W_CPR, W_RMI, W_ACO, W_LRUP0, W_LRUP1, W_Q0, W_Q1, W_Q2, W_UF, W_WF, NWORK
@@ -624,9 +635,13 @@ enum { REC_TH = 0, REC_PRES, REC_RT2, REC_RR, REC_RRU, REC_RRW, REC_RRT, REC_MX,
 }  // namespace
 
 struct AdjState {
-    int nsteps = 0, ckpt = 0, ncp = 0;
-    size_t cp_stride = 0;
+    int nsteps = 0, ckpt = 0, ncp = 0, mode = 0;
+    // checkpoint: the 7 state slots in the handle's form (cp_words doubles), then prec
+    size_t cp_stride = 0, cp_words = 0;
     double *cp = nullptr, *seg = nullptr, *SC = nullptr;
+    // FP64 pools of the recording forward pass: the handle's own pools in the
+    // FP64 form; separate pools of the same layout in the mixed form
+    double *pool64 = nullptr, *pool1_64 = nullptr;
     // forward records: blocks of {stage-2 input, stage-3 input, stage records,
     // substep records}; one block per step of a segment when they fit in
     // memory (`segrec`), else one block recomputed per step
@@ -653,15 +668,18 @@ void adj_free(moist_handle *h) {
 }
 
 int adj_alloc(moist_handle *h, int nsteps, int ckpt) {
-    if (h->adj && h->adj->nsteps == nsteps && h->adj->ckpt == ckpt) return 0;
+    if (h->adj && h->adj->nsteps == nsteps && h->adj->ckpt == ckpt && h->adj->mode == h->adj_mode) return 0;
     adj_free(h);
     h->adj = new AdjState();
     AdjState &A = *h->adj;
     A.nsteps = nsteps;
     A.ckpt = ckpt;
+    A.mode = h->adj_mode;
     A.ncp = (nsteps + ckpt - 1) / ckpt;
     const size_t n2 = h->n2;
-    A.cp_stride = 7 * n2 + h->nxs;
+    const bool mixed = h->precision == MOIST_MIXED;
+    A.cp_words = (7 * n2 * h->elem + 7) / 8;
+    A.cp_stride = A.cp_words + h->nxs;
     int nrec_sub = 0;
     for (int st = 0; st < 3; ++st) {
         A.sub_off[st] = nrec_sub;
@@ -673,9 +691,11 @@ int adj_alloc(moist_handle *h, int nsteps, int ckpt) {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) == hipSuccess) {
             const size_t need = A.rb_stride * sizeof(double) * (size_t)ckpt;
-            const size_t other = (A.cp_stride * A.ncp + 7 * n2 * (size_t)(ckpt + 1) + 60 * n2) * sizeof(double);
+            const size_t other =
+                (A.cp_stride * A.ncp + 7 * n2 * (size_t)(ckpt + 1) + (mixed ? NSLOT + 60 : 60) * n2) * sizeof(double);
             A.segrec = need + other < fr / 4 ? 1 : 0;
         }
+        if (A.mode) A.segrec = A.mode == 2 ? 1 : 0;
         A.nblocks = A.segrec ? ckpt : 1;
     }
     auto al = [&](double *&p, size_t n) -> int {
@@ -690,6 +710,13 @@ int adj_alloc(moist_handle *h, int nsteps, int ckpt) {
         al(A.lim, 6 * h->nxs) || al(A.gk, 3 * n2) || al(A.tgt, n2) || al(A.fin, A.cp_stride) || al(A.J, h->nm) ||
         al(A.gK, 3 * h->nm))
         return -1;
+    if (mixed) {
+        if (al(A.pool64, (size_t)NSLOT * n2) || al(A.pool1_64, (size_t)NSLOT1 * h->nxs)) return -1;
+        HIPCHK(h, hipMemset(A.pool1_64, 0, (size_t)NSLOT1 * h->nxs * sizeof(double)));
+    } else {
+        A.pool64 = (double *)h->pool;
+        A.pool1_64 = (double *)h->pool1;
+    }
     A.cur = A.recs;
     A.J_h.assign(h->nm, 0.0);
     A.gK_h.assign(3 * h->nm, 0.0);
@@ -714,6 +741,9 @@ double *rec_rtp(moist_handle *h, int st, int n) { return rec_sub(h, st, n, 0); }
 double *rec_rup(moist_handle *h, int st, int n) { return rec_sub(h, st, n, 1); }
 double *rec_rwp(moist_handle *h, int st, int n) { return rec_sub(h, st, n, 2); }
 double *wk(moist_handle *h, int w) { return h->adj->wk + (size_t)w * h->n2; }
+// FP64 slots of the recording forward pass
+double *slot64(moist_handle *h, int s) { return h->adj->pool64 + (size_t)s * h->n2; }
+Work<double> work64(moist_handle *h) { return work<double>(h, h->adj->pool64, h->adj->pool1_64); }
 
 AdjW adj_w(moist_handle *h, int st) {
     AdjW a;
@@ -770,7 +800,7 @@ void enqueue_stage_rec(moist_handle *h, int st, SSet<double> in, SSet<double> ou
     const int nsub = h->ns_stage[st];
     const dim3 b2(TILE_X, TILE_Z);
     const ZVec zv = zvec(h);
-    Work<double> w = work<double>(h);
+    Work<double> w = work64(h);
     w.theta = rec_st(h, st, REC_TH);
     w.pres = rec_st(h, st, REC_PRES);
     w.rt2 = rec_st(h, st, REC_RT2);
@@ -807,12 +837,12 @@ void enqueue_stage_rec(moist_handle *h, int st, SSet<double> in, SSet<double> ou
 // separate set SC (the step's output).
 void enqueue_step_rec(moist_handle *h, const double *sn) {
     const size_t n2 = h->n2;
-    copy_set(h, slot<double>(h, S0_), sn, 7 * n2);  // slots S0_..S0_+6 are contiguous
+    copy_set(h, slot64(h, S0_), sn, 7 * n2);  // slots S0_..S0_+6 are contiguous
     const ZVec zv = zvec(h);
-    const Work<double> w = work<double>(h);
+    const Work<double> w = work64(h);
+    const SSet<double> s0 = dset(slot64(h, S0_), n2), ia = dset(rec_in(h, 1), n2), ib = dset(rec_in(h, 2), n2);
     hipLaunchKernelGGL(k_physics<double>, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d,
-                       h->tiles2d, zv, sset<double>(h, S0_), w, 0);
-    const SSet<double> s0 = sset<double>(h, S0_), ia = dset(rec_in(h, 1), n2), ib = dset(rec_in(h, 2), n2);
+                       h->tiles2d, zv, s0, w, 0);
     enqueue_stage_rec(h, 0, s0, ia, s0, 3);
     enqueue_stage_rec(h, 1, ia, ib, s0, 2);
     enqueue_stage_rec(h, 2, ib, dset(h->adj->SC, n2), s0, 1);
@@ -875,21 +905,30 @@ void enqueue_step_adj(moist_handle *h, const double *sn) {
     if (lo != A.lam) copy_set(h, A.lam, lo, 7 * n2);
 }
 
+// Checkpoints hold the state slots (S0_.. S0_+6, contiguous at the front of
+// the pool) in the handle's form, and the accumulated rain.
 void save_state(moist_handle *h, double *dst) {
-    const size_t n2 = h->n2;
-    for (int f = 0; f < 7; ++f) copy_set(h, dst + f * n2, slot<double>(h, S0_ + f), n2);
-    copy_set(h, dst + 7 * n2, h->prec, h->nxs);
+    hipMemcpyAsync(dst, h->pool, 7 * h->n2 * h->elem, hipMemcpyDeviceToDevice, h->stream);
+    copy_set(h, dst + h->adj->cp_words, h->prec, h->nxs);
 }
 void load_state(moist_handle *h, const double *src) {
-    const size_t n2 = h->n2;
-    for (int f = 0; f < 7; ++f) copy_set(h, slot<double>(h, S0_ + f), src + f * n2, n2);
-    copy_set(h, h->prec, src + 7 * n2, h->nxs);
+    hipMemcpyAsync(h->pool, src, 7 * h->n2 * h->elem, hipMemcpyDeviceToDevice, h->stream);
+    copy_set(h, h->prec, src + h->adj->cp_words, h->nxs);
+}
+// The current state as a full FP64 state (7 fields) at `dst`.
+void state64(moist_handle *h, double *dst) {
+    if (h->precision == MOIST_MIXED)
+        hipLaunchKernelGGL(ka_widen, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d, h->tiles2d,
+                           zvec(h), sset<float>(h, S0_), dset(dst, h->n2));
+    else
+        copy_set(h, dst, slot<double>(h, S0_), 7 * h->n2);
 }
 
 template <class H> int adjoint_run_t(moist_handle *h, int nsteps, int ckpt, const void *const *tgt, double *J) {
     if (adj_alloc(h, nsteps, ckpt)) return -1;
     AdjState &A = *h->adj;
     const size_t n2 = h->n2;
+    const bool mixed = h->precision == MOIST_MIXED;
     // theta targets
     for (int m = 0; m < h->nm; ++m) {
         const Dom &D = h->dom_h[m];
@@ -917,7 +956,8 @@ template <class H> int adjoint_run_t(moist_handle *h, int nsteps, int ckpt, cons
     // loss and the adjoint of the final state
     fill(h, A.lam, 7 * n2);
     fill(h, A.gk, 3 * n2);
-    hipLaunchKernelGGL(ka_loss, dim3(h->nm), dim3(256), 0, h->stream, h->dom_d, sset<double>(h, S0_),
+    state64(h, A.SC);
+    hipLaunchKernelGGL(ka_loss, dim3(h->nm), dim3(256), 0, h->stream, h->dom_d, dset(A.SC, n2),
                        (const double *)h->prec, (const double *)A.tgt, A.lam + RHO * n2, A.lam + RT * n2, A.J);
     // reverse sweep over the segments
     for (int sg = A.ncp - 1; sg >= 0; --sg) {
@@ -927,10 +967,14 @@ template <class H> int adjoint_run_t(moist_handle *h, int nsteps, int ckpt, cons
             // one recording pass over the segment, then the reverse sweep from the records
             for (int n = n0; n < n1; ++n) {
                 double *sn = A.seg + (size_t)(n - n0) * 7 * n2;
-                copy_set(h, sn, slot<double>(h, S0_), 7 * n2);
+                state64(h, sn);
                 A.cur = A.recs + A.rb_stride * (size_t)(n - n0);
                 enqueue_step_rec(h, sn);
-                copy_set(h, slot<double>(h, S0_), A.SC, 7 * n2);
+                // next state: the recorded step's output (FP64), or a step of the mixed form
+                if (!mixed)
+                    copy_set(h, slot<double>(h, S0_), A.SC, 7 * n2);
+                else if (n + 1 < n1 && run_steps(h, 1))
+                    return -1;
             }
             for (int n = n1 - 1; n >= n0; --n) {
                 A.cur = A.recs + A.rb_stride * (size_t)(n - n0);
@@ -939,7 +983,7 @@ template <class H> int adjoint_run_t(moist_handle *h, int nsteps, int ckpt, cons
         } else {
             // states of the segment, then per step (in reverse) its record and adjoint
             for (int n = n0; n < n1; ++n) {
-                copy_set(h, A.seg + (size_t)(n - n0) * 7 * n2, slot<double>(h, S0_), 7 * n2);
+                state64(h, A.seg + (size_t)(n - n0) * 7 * n2);
                 if (n + 1 < n1 && run_steps(h, 1)) return -1;
             }
             A.cur = A.recs;
@@ -1274,12 +1318,20 @@ int moist_use_graph(moist_handle *h, int on) {
 
 int moist_set_option(moist_handle *h, int option, int value) {
     if (!h) return -2;
-    if (option == 0) {
+    if (option == MOIST_OPT_PHYS_DIAG) {
         h->store_diag = value ? 1 : 0;
         if (h->gexec) {
             hipGraphExecDestroy(h->gexec);
             h->gexec = nullptr;
         }
+        return 0;
+    }
+    if (option == MOIST_OPT_ADJ_RECORDS) {
+        if (value < 0 || value > 2) {
+            h->err = "adjoint record mode must be 0, 1 or 2";
+            return -2;
+        }
+        h->adj_mode = value;
         return 0;
     }
     h->err = "unknown option";
@@ -1414,10 +1466,6 @@ int moist_set_acoustic_shape(moist_handle *h, int levels, int threads) {
 int moist_adjoint_run(moist_handle *h, int nsteps, int ckpt, const void *const *theta_target, int elem_bytes,
                       double *J) {
     if (!h || !theta_target || nsteps < 1 || ckpt < 1) return -2;
-    if (h->precision != MOIST_FP64) {
-        h->err = "the adjoint runs on an FP64-form handle";
-        return -2;
-    }
     if (h->any_sf || h->any_rd) {
         h->err = "the adjoint covers do_sf = do_rd = false";
         return -2;

@@ -242,3 +242,24 @@ def test_checkpoint_interval_does_not_change_the_gradient():
     for f in range(7):
         np.testing.assert_array_equal(a[1][f], b[1][f])
     np.testing.assert_array_equal(a[2], b[2])
+
+
+def test_chain_of_step_adjoints_is_the_checkpointed_gradient():
+    """The chain of one-step adjoints along the oracle's trajectory (the
+    reference the device's mixed-form reverse sweep is checked with) equals
+    adjoint_run."""
+    from adjoint_chain import chain_adjoint
+
+    p, s, rng, tt = _loss_setup()
+    states = [s.copy()]
+    for _ in range(4):
+        x = states[-1].copy()
+        O.rk3_step(x, p, 1)
+        states.append(x)
+    J, g, gK = chain_adjoint(p, states, tt)
+    Jo, go, gKo = O.adjoint_run(s.copy(), p, 4, tt, ckpt=3)
+    assert abs(J - Jo) <= 1e-13 * abs(Jo)
+    h = p.halo
+    for f in range(7):
+        np.testing.assert_allclose(g[f][:, h:h + p.Nx], go[f][:, h:h + p.Nx], rtol=1e-12, atol=1e-14 * np.abs(go[f]).max())
+    np.testing.assert_allclose(gK, gKo, rtol=1e-12)

@@ -1,17 +1,25 @@
-"""GPU reverse mode (moist_adjoint_run / moist_adjoint_grad, FP64 form)
-against the oracle's adjoint (itself pinned by central finite differences in
+"""GPU reverse mode (moist_adjoint_run / moist_adjoint_grad) against the
+oracle's adjoint (itself pinned by central finite differences in
 test_adjoint_oracle.py) and against central finite differences of the GPU's
 own forward step.
 
-Bars: the loss to 1e-12 relative; per-field gradient rel-L2 <= 1e-8 (the
-oracle and the GPU run the same FP64 operations in a different order, and
-the reverse sweep amplifies their 1e-13 trajectory differences by at most
-the condition of the step); parameter gradients to 1e-8.
+FP64 form: bars are the loss to 1e-12 relative; per-field gradient rel-L2
+<= 1e-8 (the oracle and the GPU run the same FP64 operations in a different
+order, and the reverse sweep amplifies their 1e-13 trajectory differences by
+at most the condition of the step); parameter gradients to 1e-8.  Both ways
+of keeping the forward records (whole segment, per step) are covered.
+
+Mixed form: the trajectory advances in the mixed form and each step is
+linearised in FP64 about it, so the reference is the chain of the oracle's
+one-step adjoints along the device's own mixed-form trajectory (downloaded
+step by step), to the same 1e-8 bar; its distance from the FP64-form
+gradient is the trajectory's precision error, checked separately.
 """
 import numpy as np
 import pytest
 
 import oracle as O
+from adjoint_chain import chain_adjoint
 from conftest import PROG
 from moist2d_amd import core, ic
 
@@ -46,9 +54,10 @@ def _rel(a, b):
     return d / n if n > 0 else d
 
 
-def _compare(tag, p, s, tt, nsteps, ckpt, tol=1e-8):
+def _compare(tag, p, s, tt, nsteps, ckpt, tol=1e-8, records="auto"):
     Jo, go, gKo = O.adjoint_run(s.copy(), p, nsteps, tt, ckpt=ckpt)
     eng = core.Engine(p, core.FP64)
+    eng.adjoint_records(records)
     eng.upload(s)
     J = eng.adjoint(nsteps, [tt], ckpt=ckpt)[0]
     g, gK = eng.adjoint_grad(0)
@@ -77,9 +86,74 @@ def test_adjoint_one_step_with_physics():
     _compare("physics 1 step", p, s, tt, 1, 1)
 
 
-def test_adjoint_checkpointed_run():
+@pytest.mark.parametrize("records", ["segment", "step"])
+def test_adjoint_checkpointed_run(records):
     p, s, tt = _case()
-    _compare("12 steps ckpt 5", p, s, tt, 12, 5)
+    _compare(f"12 steps ckpt 5, {records} records", p, s, tt, 12, 5, records=records)
+
+
+def _trajectory(p, s, nsteps, precision):
+    """x_0 .. x_nsteps of the device's forward in `precision`, as FP64 States."""
+    eng = core.Engine(p, precision)
+    eng.upload(s)
+    xs = []
+    for n in range(nsteps + 1):
+        a = s.copy()
+        eng.download(a, phys=False)
+        xs.append(a)
+        if n < nsteps:
+            eng.step(1)
+    eng.close()
+    return xs
+
+
+def _device_adjoint(p, s, tt, nsteps, ckpt, precision, records="auto"):
+    eng = core.Engine(p, precision)
+    eng.adjoint_records(records)
+    eng.upload(s)
+    J = eng.adjoint(nsteps, [tt], ckpt=ckpt)[0]
+    g, gK = eng.adjoint_grad(0)
+    eng.close()
+    return J, g, gK
+
+
+@pytest.mark.parametrize("records", ["segment", "step"])
+@pytest.mark.parametrize("do_mp", [False, True])
+def test_mixed_adjoint_is_the_step_adjoint_chain_along_its_trajectory(records, do_mp):
+    p, s, tt = _case(do_mp=do_mp)
+    nsteps, ckpt = 5, 2
+    Jc, gc, gKc = chain_adjoint(p, _trajectory(p, s, nsteps, core.MIXED), tt)
+    J, g, gK = _device_adjoint(p, s, tt, nsteps, ckpt, core.MIXED, records)
+    h = p.halo
+    errs = {n: _rel(g[f][:, h:h + p.Nx], gc[f][:, h:h + p.Nx]) for f, n in enumerate(PROG7)}
+    print(f"mixed {records} do_mp={do_mp}: J {J:.12e} vs chain {Jc:.12e}; rel-L2 "
+          + " ".join(f"{k}={v:.1e}" for k, v in errs.items()), flush=True)
+    assert abs(J - Jc) <= 1e-12 * abs(Jc), (J, Jc)
+    for n, e in errs.items():
+        assert e <= 1e-8, (n, e)
+    for j in range(3):
+        assert abs(gK[j] - gKc[j]) <= 1e-8 * abs(gKc[j]) or abs(gK[j] - gKc[j]) <= 1e-12, (j, gK[j], gKc[j])
+
+
+def test_mixed_adjoint_close_to_fp64_adjoint():
+    """The mixed-form gradient differs from the FP64 form's by what the
+    trajectory's FP32 deviations change.  Measured on this case: the loss
+    2.5e-6 relative, per-field gradient rel-L2 <= 3e-4; bars 1e-5 and 1e-3
+    (the mixed form's forward bar is 1e-4 state rel-L2; a gradient, being a
+    derivative along that trajectory, carries a few times its error)."""
+    p, s, tt = _case()
+    nsteps, ckpt = 12, 5
+    J64, g64, gK64 = _device_adjoint(p, s, tt, nsteps, ckpt, core.FP64)
+    Jm, gm, gKm = _device_adjoint(p, s, tt, nsteps, ckpt, core.MIXED)
+    h = p.halo
+    errs = {n: _rel(gm[f][:, h:h + p.Nx], g64[f][:, h:h + p.Nx]) for f, n in enumerate(PROG7)}
+    kerr = [abs(gKm[j] - gK64[j]) / max(abs(gK64[j]), 1e-300) for j in range(3)]
+    print(f"mixed vs fp64: J rel {abs(Jm - J64) / abs(J64):.1e}; rel-L2 "
+          + " ".join(f"{k}={v:.1e}" for k, v in errs.items()) + " dK " + " ".join(f"{e:.1e}" for e in kerr),
+          flush=True)
+    assert abs(Jm - J64) <= 1e-5 * abs(J64)
+    for n, e in errs.items():
+        assert e <= 1e-3, (n, e)
 
 
 def test_adjoint_bubble_64x32():
