@@ -28,8 +28,8 @@
 //   * forward and back substitution are affine recurrences: each chunk solves
 //     with a zero carry and keeps the carry coefficients, the carries are
 //     combined across chunks, and every level is corrected.
-// The cross-chunk combinations are Hillis-Steele scans in LDS (log2(nch)
-// steps, every thread active).
+// The cross-chunk combinations are two-level scans: lane shuffles over the
+// chunks one wave holds, then one LDS exchange of the waves' aggregates.
 
 #ifndef MOIST_AC_THREADS
 #define MOIST_AC_THREADS 1024
@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
                                                  ZVec zv, SSet<S> in, Work<S> w, AcBuf<S> ab, int div,
                                                  double dt_ovr, int nsub, int first) {
     using R = S;
-    __shared__ double lds[8 * NT];  // two buffers of 4 doubles per thread for the scans
+    __shared__ double lds[6 * NT];  // group aggregates of the cross-chunk scans
     __shared__ S ldt[2 * NT];
 
     const int4 tl = atiles[blockIdx.x];
@@ -83,7 +83,6 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
     const R ftm = R(0.5) * (R(1.0) - beta);
     const double dt_tp2 = double(dt_s * ftp) * double(dt_s * ftp);
     const double fact = double(ftm / ftp);
-    const double G = K<double>::GRAV;
     const int64_t cidx = (int64_t)c * ab.cin_stride + D.offx + i;
     double cin_v = 0.0;
     if (!first && nl > 0) cin_v = ab.cin[cidx];
@@ -222,6 +221,17 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
         }
     }
 
+    // ---- cross-chunk scans: two levels.  A wave holds CPW = 64 / TX
+    // consecutive chunks of TX columns (CPW = 1 when TX >= 64); the chunks of
+    // a wave combine with lane shuffles (no barrier), the groups of CPW chunks
+    // through one LDS exchange per scan, each thread folding the aggregates of
+    // the groups below (or above) its own in order.  G = nch / CPW <= NT / 64.
+    const int CPW = TX >= 64 ? 1 : 64 / TX;
+    const int cw = c % CPW;    // chunk within the wave
+    const int grp = c / CPW;   // group of the chunk
+    const int G = nch / CPW;
+    const int gi = grp * TX + jj;  // this thread's (group, column) slot
+
     // ---- c' entering the chunk: Moebius composites scanned once per stage
     if (first) {
         double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;
@@ -241,43 +251,48 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
                 m11 = n11 * sc;
             }
         }
-        // inclusive Hillis-Steele scan over the chunks of each column:
-        // S_c = M_c S_{c-1} ... M_0; the c' entering chunk c is S_{c-1} applied to (0, 1)
-        double *A = lds, *B = lds + 4 * NT;
-        A[4 * t + 0] = m00;
-        A[4 * t + 1] = m01;
-        A[4 * t + 2] = m10;
-        A[4 * t + 3] = m11;
-        __syncthreads();
-        for (int dst = 1; dst < nch; dst <<= 1) {
-            double a0 = A[4 * t], a1 = A[4 * t + 1], a2 = A[4 * t + 2], a3 = A[4 * t + 3];
-            if (c >= dst) {
-                const int o = 4 * (t - dst * TX);
-                const double b0 = A[o], b1 = A[o + 1], b2 = A[o + 2], b3 = A[o + 3];
-                const double n0 = a0 * b0 + a1 * b2, n1 = a0 * b1 + a1 * b3;
-                const double n2 = a2 * b0 + a3 * b2, n3 = a2 * b1 + a3 * b3;
+        // in-wave inclusive scan S_c = M_c S_c-1 ... M_first of the group
+        for (int d = 1; d < CPW; d <<= 1) {
+            const double b0 = __shfl_up(m00, d * TX), b1 = __shfl_up(m01, d * TX), b2 = __shfl_up(m10, d * TX),
+                         b3 = __shfl_up(m11, d * TX);
+            if (cw >= d) {
+                const double n0 = m00 * b0 + m01 * b2, n1 = m00 * b1 + m01 * b3;
+                const double n2 = m10 * b0 + m11 * b2, n3 = m10 * b1 + m11 * b3;
                 const double sc = 1.0 / fmax(fmax(fabs(n0), fabs(n1)), fmax(fabs(n2), fabs(n3)));
-                a0 = n0 * sc;
-                a1 = n1 * sc;
-                a2 = n2 * sc;
-                a3 = n3 * sc;
+                m00 = n0 * sc;
+                m01 = n1 * sc;
+                m10 = n2 * sc;
+                m11 = n3 * sc;
             }
-            B[4 * t + 0] = a0;
-            B[4 * t + 1] = a1;
-            B[4 * t + 2] = a2;
-            B[4 * t + 3] = a3;
-            __syncthreads();
-            double *tmp = A;
-            A = B;
-            B = tmp;
         }
-        cin_v = 0.0;
-        if (c > 0) {
-            const int o = 4 * (t - TX);
-            cin_v = A[o + 1] / A[o + 3];
+        // the group composite (last chunk of the group) to LDS
+        double *agm = lds;  // 4 doubles per (group, column)
+        if (cw == CPW - 1) {
+            agm[4 * gi + 0] = m00;
+            agm[4 * gi + 1] = m01;
+            agm[4 * gi + 2] = m10;
+            agm[4 * gi + 3] = m11;
         }
-        if (nl > 0) ab.cin[cidx] = cin_v;
+        // composite of the chunks of this group below this one (identity for the first)
+        double e00 = 1.0, e01 = 0.0, e10 = 0.0, e11 = 1.0;
+        if (CPW > 1) {
+            const double p0 = __shfl_up(m00, TX), p1 = __shfl_up(m01, TX), p2 = __shfl_up(m10, TX),
+                         p3 = __shfl_up(m11, TX);
+            if (cw > 0) e00 = p0, e01 = p1, e10 = p2, e11 = p3;
+        }
         __syncthreads();
+        // the groups below applied to (0, 1), then the chunks of this group below this one
+        double u0 = 0.0, u1 = 1.0;
+        for (int gp = 0; gp < grp; ++gp) {
+            const double *M = agm + 4 * (gp * TX + jj);
+            const double v0 = M[0] * u0 + M[1] * u1, v1 = M[2] * u0 + M[3] * u1;
+            const double sc = 1.0 / fmax(fabs(v0), fabs(v1));
+            u0 = v0 * sc;
+            u1 = v1 * sc;
+        }
+        const double s0 = e00 * u0 + e01 * u1, s1 = e10 * u0 + e11 * u1;
+        cin_v = c > 0 ? s0 / s1 : 0.0;
+        if (nl > 0) ab.cin[cidx] = cin_v;
     }
 
     // ---- forward substitution inside the chunk with a zero carry
@@ -306,32 +321,34 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
             }
         }
     }
-    // carries of the forward substitution: inclusive scan of the chunk maps
-    // x -> y + P x from the bottom; the carry into chunk c is S_{c-1}(0)
+    // carries of the forward substitution: the chunk maps x -> y + P x
+    // composed from the bottom; the carry into chunk c is S_c-1(0)
     {
-        double *A = lds, *B = lds + 4 * NT;
-        A[2 * t + 0] = ylast;
-        A[2 * t + 1] = plast;
-        __syncthreads();
-        for (int dst = 1; dst < nch; dst <<= 1) {
-            double y = A[2 * t], P = A[2 * t + 1];
-            if (c >= dst) {
-                const int o = 2 * (t - dst * TX);
-                y = y + P * A[o];
-                P = P * A[o + 1];
+        double y = ylast, P = plast;
+        for (int d = 1; d < CPW; d <<= 1) {
+            const double yp = __shfl_up(y, d * TX), Pp = __shfl_up(P, d * TX);
+            if (cw >= d) {
+                y = y + P * yp;
+                P = P * Pp;
             }
-            B[2 * t + 0] = y;
-            B[2 * t + 1] = P;
-            __syncthreads();
-            double *tmp = A;
-            A = B;
-            B = tmp;
         }
-        const double din = c > 0 ? A[2 * (t - TX)] : 0.0;
+        double *agf = lds + 4 * NT;  // 2 doubles per (group, column)
+        if (cw == CPW - 1) {
+            agf[2 * gi + 0] = y;
+            agf[2 * gi + 1] = P;
+        }
+        double ye = 0.0, Pe = 1.0;
+        if (CPW > 1) {
+            const double yq = __shfl_up(y, TX), Pq = __shfl_up(P, TX);
+            if (cw > 0) ye = yq, Pe = Pq;
+        }
+        __syncthreads();
+        double E = 0.0;
+        for (int gp = 0; gp < grp; ++gp) E = agf[2 * (gp * TX + jj)] + agf[2 * (gp * TX + jj) + 1] * E;
+        const double din = ye + Pe * E;
 #pragma unroll
         for (int l = 0; l < L; ++l)
             if (l < nf) dd[l] = dd[l] + pv[l] * din;
-        __syncthreads();
     }
 
     // ---- back substitution inside the chunk with a zero carry from above
@@ -355,39 +372,51 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
             }
         }
     }
-    // carries of the back substitution: inclusive scan of the maps
-    // x_in -> z_first + Q_first x_in from the top; T_c(0) is x at chunk c's
-    // first face, the carry into chunk c is T_{c+1}(0)
+    // carries of the back substitution: the maps x_in -> z_first + Q_first x_in
+    // composed from the top; T_c(0) is x at chunk c's first face, the carry
+    // into chunk c is T_c+1(0)
     double xin, x_below = 0.0;
     {
-        double *A = lds, *B = lds + 4 * NT;
-        A[2 * t + 0] = zfirst;
-        A[2 * t + 1] = qfirst;
-        __syncthreads();
-        for (int dst = 1; dst < nch; dst <<= 1) {
-            double y = A[2 * t], P = A[2 * t + 1];
-            if (c + dst < nch) {
-                const int o = 2 * (t + dst * TX);
-                y = y + P * A[o];
-                P = P * A[o + 1];
+        double y = zfirst, P = qfirst;
+        for (int d = 1; d < CPW; d <<= 1) {
+            const double yn = __shfl_down(y, d * TX), Pn = __shfl_down(P, d * TX);
+            if (cw + d < CPW) {
+                y = y + P * yn;
+                P = P * Pn;
             }
-            B[2 * t + 0] = y;
-            B[2 * t + 1] = P;
-            __syncthreads();
-            double *tmp = A;
-            A = B;
-            B = tmp;
         }
-        xin = c + 1 < nch ? A[2 * (t + TX)] : 0.0;
-        const double xfirst = A[2 * t];  // x at this chunk's first face
-        // x at the last face of the chunk below = z_last + Q_last * (x at this chunk's first face)
-        B[2 * t + 0] = zlast;
-        B[2 * t + 1] = qlast;
+        // group composites (first chunk of each group) and the (z_last, Q_last)
+        // of each group's last chunk; the Moebius area is free again (the
+        // forward scan's barrier lies between its reads and these writes)
+        double *agb = lds, *lzq = lds + 2 * NT;
+        if (cw == 0) {
+            agb[2 * gi + 0] = y;
+            agb[2 * gi + 1] = P;
+        }
+        if (cw == CPW - 1) {
+            lzq[2 * gi + 0] = zlast;
+            lzq[2 * gi + 1] = qlast;
+        }
+        double zl_b = 0.0, ql_b = 1.0;  // (z_last, Q_last) of chunk c-1 when it is in this wave
+        if (CPW > 1) {
+            const double zq = __shfl_up(zlast, TX), qq = __shfl_up(qlast, TX);
+            if (cw > 0) zl_b = zq, ql_b = qq;
+        }
         __syncthreads();
-        if (c > 0 && nl > 0) {
-            const int o = 2 * (t - TX);
-            x_below = B[o] + B[o + 1] * xfirst;
+        double F = 0.0;
+        for (int gp = G - 1; gp > grp; --gp) F = agb[2 * (gp * TX + jj)] + agb[2 * (gp * TX + jj) + 1] * F;
+        const double xfirst = y + P * F;  // x at this chunk's first face
+        xin = F;
+        if (CPW > 1) {
+            const double xn = __shfl_down(xfirst, TX);
+            if (cw < CPW - 1) xin = xn;
         }
+        if (cw == 0 && grp > 0) {
+            zl_b = lzq[2 * ((grp - 1) * TX + jj)];
+            ql_b = lzq[2 * ((grp - 1) * TX + jj) + 1];
+        }
+        // x at the last face of the chunk below = z_last + Q_last * (x at this chunk's first face)
+        if (c > 0 && nl > 0) x_below = zl_b + ql_b * xfirst;
     }
     R xr[L];
 #pragma unroll

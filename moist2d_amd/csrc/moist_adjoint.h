@@ -64,10 +64,14 @@ template <int N> __device__ __forceinline__ Dl<N> operator*(const Dl<N> &a, cons
     for (int j = 0; j < N; ++j) r.d[j] = a.d[j] * b.v + a.v * b.d[j];
     return r;
 }
+// Quotients: the value is the correctly rounded quotient (the forward's
+// value, so frozen branch tests see the same numbers); the derivative parts
+// share one reciprocal.
 template <int N> __device__ __forceinline__ Dl<N> operator/(const Dl<N> &a, const Dl<N> &b) {
     Dl<N> r(a.v / b.v);
+    const double rb = 1.0 / b.v;
 #pragma unroll
-    for (int j = 0; j < N; ++j) r.d[j] = (a.d[j] - r.v * b.d[j]) / b.v;
+    for (int j = 0; j < N; ++j) r.d[j] = (a.d[j] - r.v * b.d[j]) * rb;
     return r;
 }
 template <int N> __device__ __forceinline__ Dl<N> operator+(const Dl<N> &a, double b) { return a + Dl<N>(b); }
@@ -76,8 +80,20 @@ template <int N> __device__ __forceinline__ Dl<N> operator-(const Dl<N> &a, doub
 template <int N> __device__ __forceinline__ Dl<N> operator-(double a, const Dl<N> &b) { return Dl<N>(a) - b; }
 template <int N> __device__ __forceinline__ Dl<N> operator*(const Dl<N> &a, double b) { return a * Dl<N>(b); }
 template <int N> __device__ __forceinline__ Dl<N> operator*(double a, const Dl<N> &b) { return Dl<N>(a) * b; }
-template <int N> __device__ __forceinline__ Dl<N> operator/(const Dl<N> &a, double b) { return a / Dl<N>(b); }
-template <int N> __device__ __forceinline__ Dl<N> operator/(double a, const Dl<N> &b) { return Dl<N>(a) / b; }
+template <int N> __device__ __forceinline__ Dl<N> operator/(const Dl<N> &a, double b) {
+    Dl<N> r(a.v / b);
+    const double rb = 1.0 / b;
+#pragma unroll
+    for (int j = 0; j < N; ++j) r.d[j] = a.d[j] * rb;
+    return r;
+}
+template <int N> __device__ __forceinline__ Dl<N> operator/(double a, const Dl<N> &b) {
+    Dl<N> r(a / b.v);
+    const double s = -r.v / b.v;
+#pragma unroll
+    for (int j = 0; j < N; ++j) r.d[j] = s * b.d[j];
+    return r;
+}
 template <int N> __device__ __forceinline__ Dl<N> d_exp(const Dl<N> &a) {
     Dl<N> r(exp(a.v));
 #pragma unroll
@@ -86,16 +102,17 @@ template <int N> __device__ __forceinline__ Dl<N> d_exp(const Dl<N> &a) {
 }
 template <int N> __device__ __forceinline__ Dl<N> d_sqrt(const Dl<N> &a) {
     Dl<N> r(sqrt(a.v));
+    const double h = 0.5 / r.v;
 #pragma unroll
-    for (int j = 0; j < N; ++j) r.d[j] = a.d[j] / (2.0 * r.v);
+    for (int j = 0; j < N; ++j) r.d[j] = a.d[j] * h;
     return r;
 }
 // a^b for a > 0
 template <int N> __device__ __forceinline__ Dl<N> d_pow(const Dl<N> &a, const Dl<N> &b) {
     Dl<N> r(pow(a.v, b.v));
-    const double la = log(a.v);
+    const double la = log(a.v), ba = b.v / a.v;
 #pragma unroll
-    for (int j = 0; j < N; ++j) r.d[j] = r.v * (b.d[j] * la + b.v * a.d[j] / a.v);
+    for (int j = 0; j < N; ++j) r.d[j] = r.v * (b.d[j] * la + ba * a.d[j]);
     return r;
 }
 template <int N> __device__ __forceinline__ double val(const Dl<N> &a) { return a.v; }
@@ -112,54 +129,109 @@ __device__ __forceinline__ int wrapn(int j, int nx) {
     return j < 0 ? j + nx : j;
 }
 
-// ---- Koren faces with dual numbers (moist.jl:850-945) ---------------------------
-template <class V> __device__ __forceinline__ V koren_v(V r) {
-    return vmax(V(0.0), vmin(V(2.0) * r, vmin((V(1.0) + V(2.0) * r) / V(3.0), V(2.0))));
-}
-// face between cells holding q1 and q2 (q0 upstream of q1, q3 downstream of q2)
-template <class V> __device__ __forceinline__ V kface_x(V q0, V q1, V q2, V q3, V m) {
+// ---- Koren faces: value and partial derivatives (moist.jl:850-945) -------------
+// On each branch of the limiter the face value is linear in the four cell
+// values, so its partials are constants.  The branch itself is decided on the
+// forward's values: r = rn / rd and phi(r) = max(0, min(2r, min((1+2r)/3, 2)))
+// are evaluated exactly as the forward does, with Julia's min/max operand
+// choice (min(x, y) = signbit(x - y) ? x : y, max(x, y) = signbit(x - y) ? y : x).
+//
+// One-sided reconstruction s = u0 + sg * 0.5 * phi(r) * (u0 - um), r = (u1 - u0) / (u0 - um),
+// sg = +1 (moist.jl koren_face) or -1 (the z face's downward branch, which
+// subtracts).  Returns s; (am, a0, a1) = ds / d(um, u0, u1).
+__device__ __forceinline__ double koren_side(double um, double u0, double u1, double sg, double &am, double &a0,
+                                             double &a1) {
     const double EPS = 1e-12;
-    V s;
-    if (val(m) > EPS) {
-        V rn = q2 - q1, rd = q1 - q0;
-        V r = fabs(val(rd)) < EPS ? V(1.0) : rn / rd;
-        s = q1 + V(0.5) * koren_v(r) * rd;
-    } else if (val(m) < -EPS) {
-        V rn = q1 - q2, rd = q2 - q3;
-        V r = fabs(val(rd)) < EPS ? V(1.0) : rn / rd;
-        s = q2 + V(0.5) * koren_v(r) * rd;
+    const double rn = u1 - u0, rd = u0 - um;
+    double pm, p0, p1, phi;  // 0.5 phi rd = pm um + p0 u0 + p1 u1 on the branch taken
+    if (fabs(rd) < EPS) {    // r = 1, phi = 1
+        phi = 1.0;
+        pm = -0.5, p0 = 0.5, p1 = 0.0;
     } else {
-        s = V(0.5) * (q1 + q2);
+        const double r = rn / rd;
+        const double t = (1.0 + 2.0 * r) / 3.0, r2 = 2.0 * r;
+        int b1 = signbit(t - 2.0) ? 1 : 2;  // (1+2r)/3 or 2
+        const double t1 = b1 == 1 ? t : 2.0;
+        const int b2 = signbit(r2 - t1) ? 0 : b1;  // 2r or the above
+        const double t2 = b2 == 0 ? r2 : t1;
+        const int br = signbit(0.0 - t2) ? b2 : 3;  // phi > 0 or phi = 0
+        phi = br == 3 ? 0.0 : t2;
+        if (br == 0) pm = 0.0, p0 = -1.0, p1 = 1.0;                          // 0.5 (2r) rd = rn
+        else if (br == 1) pm = -1.0 / 6.0, p0 = -1.0 / 6.0, p1 = 2.0 / 6.0;  // (rd + 2 rn) / 6
+        else if (br == 2) pm = -1.0, p0 = 1.0, p1 = 0.0;                     // rd
+        else pm = 0.0, p0 = 0.0, p1 = 0.0;
     }
-    return s * m;
+    am = sg * pm;
+    a0 = 1.0 + sg * p0;
+    a1 = sg * p1;
+    return sg > 0.0 ? u0 + 0.5 * phi * rd : u0 - 0.5 * phi * rd;
 }
-template <class V>
-__device__ __forceinline__ V kface_z(V q0, V q1, V q2, V q3, V m, V lo, V hi, bool bottom, bool top, double dzk,
-                                     double dzk1, double dtL) {
+
+// x face between q1 and q2 (q0 upstream of q1, q3 downstream of q2), mass
+// flux m: the face value s (flux s m) and ds/dq0..dq3.
+__device__ __forceinline__ double kcoef_x(double q0, double q1, double q2, double q3, double m, double c[4]) {
     const double EPS = 1e-12;
-    V s;
-    if (val(m) > EPS) {
-        if (bottom) {
-            s = (q1 * dzk1 + q2 * dzk) / (dzk + dzk1);
-            s = vmin(s, lo / (m * dtL));
-        } else {
-            V rn = q2 - q1, rd = q1 - q0;
-            V r = fabs(val(rd)) < EPS ? V(1.0) : rn / rd;
-            s = q1 + V(0.5) * koren_v(r) * rd;
-        }
-    } else if (val(m) < -EPS) {
-        if (top) {
-            s = (q1 * dzk1 + q2 * dzk) / (dzk + dzk1);
-            s = vmin(s, hi / (-m * dtL));
-        } else {
-            V rn = q1 - q2, rd = q2 - q3;
-            V r = fabs(val(rd)) < EPS ? V(1.0) : rn / rd;
-            s = q2 - V(0.5) * koren_v(r) * rd;
-        }
-    } else {
+    if (m > EPS) {
+        c[3] = 0.0;
+        return koren_side(q0, q1, q2, 1.0, c[0], c[1], c[2]);
+    }
+    if (m < -EPS) {
+        c[0] = 0.0;
+        return koren_side(q3, q2, q1, 1.0, c[3], c[2], c[1]);
+    }
+    c[0] = c[3] = 0.0;
+    c[1] = c[2] = 0.5;
+    return 0.5 * (q1 + q2);
+}
+
+// z face kk+1/2 (moist.jl:903-945): first-order f2h faces with the
+// positivity limiters at the bottom (upward flow) and top (downward flow)
+// faces, which cap the face value by lo / (m dtL) or hi / (-m dtL).  Returns
+// s; c = ds/dq0..dq3; on a capped face `capped` is set and the flux s m has
+// d/dlo = clo, d/dhi = chi and no dependence on m or q.
+__device__ __forceinline__ double kcoef_z(double q0, double q1, double q2, double q3, double m, double lo, double hi,
+                                          bool bottom, bool top, double dzk, double dzk1, double rden, double dtL,
+                                          double c[4], bool &capped, double &clo, double &chi) {
+    const double EPS = 1e-12;
+    capped = false;
+    clo = chi = 0.0;
+    c[0] = c[1] = c[2] = c[3] = 0.0;
+    auto f2h = [&](double &s) {
         s = (q1 * dzk1 + q2 * dzk) / (dzk + dzk1);
+        c[1] = dzk1 * rden;
+        c[2] = dzk * rden;
+    };
+    double s;
+    if (m > EPS) {
+        if (bottom) {
+            f2h(s);
+            const double lim = lo / (m * dtL);
+            if (!signbit(s - lim)) {  // min(s, lim) takes lim
+                capped = true;
+                c[1] = c[2] = 0.0;
+                clo = 1.0 / dtL;
+                s = lim;
+            }
+        } else {
+            s = koren_side(q0, q1, q2, 1.0, c[0], c[1], c[2]);
+        }
+    } else if (m < -EPS) {
+        if (top) {
+            f2h(s);
+            const double lim = hi / (-m * dtL);
+            if (!signbit(s - lim)) {
+                capped = true;
+                c[1] = c[2] = 0.0;
+                chi = -1.0 / dtL;
+                s = lim;
+            }
+        } else {
+            s = koren_side(q3, q2, q1, -1.0, c[3], c[2], c[1]);
+        }
+    } else {
+        f2h(s);
     }
-    return s * m;
+    return s;
 }
 
 // ---- third-order upwind fluxes: coefficient of one input ------------------------
@@ -175,8 +247,8 @@ __device__ __forceinline__ double upw_coef(bool pos, bool neg, int off, double w
 }
 __device__ __forceinline__ double upw_val(bool pos, bool neg, double vm1, double v0, double v1, double v2, double w0,
                                           double w1) {
-    if (pos) return upw3(vm1, v0, v1) / 6.0;
-    if (neg) return upw3(v2, v1, v0) / 6.0;
+    if (pos) return upw3(vm1, v0, v1) * (1.0 / 6.0);
+    if (neg) return upw3(v2, v1, v0) * (1.0 / 6.0);
     return v0 * w0 + v1 * w1;
 }
 
@@ -279,89 +351,88 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
     const int nx = D.nx, nz = D.nz, ke = nz - 1;
     if (i >= nx || k >= nz) return;
-    const double *zd = zv.z + D.offz, *dzd = zv.dz + D.offz;
-    const double dx = D.dx, kap = D.kappa, dt = D.dt / double(div);
+    const double *dzd = zv.dz + D.offz;
+    const double dx = D.dx, kap = D.kappa, dt = D.dt / double(div), rdx = 1.0 / dx;
+    const double4 *gz = zv.geo + D.offz;  // per level: 1/dz, 1/(z_k+1 - z_k), 1/(dz_k + dz_k+1)
     auto G = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
     const int64_t c = G(k, i);
     const double *rho = s.f[RHO];
+    const double rrho = 1.0 / rho[c];
     double lrho = lo.f[RHO][c];  // final update: rho_out = rho + rho'
     double lmx = 0.0, lmz = 0.0;
     // ---- the three scalars
+#pragma unroll 1
     for (int sc = 0; sc < 3; ++sc) {
         const double *qr = s.f[QV + sc];
         const double *L = lo.f[QV + sc];
         const double *qa = a.q[sc];
         auto Q = [&](int kk, int ii) { return qa[G(kk, ii)]; };
-        auto LFx = [&](int j) { return dt / dx * (L[G(k, j + 1)] - L[G(k, j)]); };
-        auto LFz = [&](int kk) { return dt * (L[G(kk + 1, i)] / dzd[kk + 1] - L[G(kk, i)] / dzd[kk]); };
+        auto LFx = [&](int j) { return dt * rdx * (L[G(k, j + 1)] - L[G(k, j)]); };
+        auto LFz = [&](int kk) { return dt * (L[G(kk + 1, i)] * gz[kk + 1].x - L[G(kk, i)] * gz[kk].x); };
         ls0.f[QV + sc][c] += L[c];
         lph.f[3 + sc][c] += dt * L[c];
         double Lq = 0.0;
-        // x faces j+1/2, j = i-2..i+1, read q(j-1..j+2); slot of cell i is i-j+1
-        for (int j = i - 2; j <= i + 1; ++j) {
-            const int slot = i - j + 1;
-            typedef Dl<1> V;
-            V q0(Q(k, j - 1)), q1(Q(k, j)), q2(Q(k, j + 1)), q3(Q(k, j + 2));
-            if (slot == 0) q0.d[0] = 1.0;
-            if (slot == 1) q1.d[0] = 1.0;
-            if (slot == 2) q2.d[0] = 1.0;
-            if (slot == 3) q3.d[0] = 1.0;
-            const V F = kface_x(q0, q1, q2, q3, V(a.mom_x[G(k, j)]));
-            Lq += LFx(j) * F.d[0];
+        // x faces j+1/2, j = i-2..i+1, read q(j-1..j+2); cell i is q slot i-j+1.
+        // Face i+1/2 (o = 2) is this cell's own: its value is d flux / d mom_x.
+        double s_own_x = 0.0;
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+            const int j = i - 2 + o, slot = 3 - o;
+            double cf[4];
+            const double m = a.mom_x[G(k, j)];
+            const double sf = kcoef_x(Q(k, j - 1), Q(k, j), Q(k, j + 1), Q(k, j + 2), m, cf);
+            Lq += LFx(j) * m * cf[slot];
+            if (o == 2) s_own_x = sf;
         }
-        // z faces kk+1/2, kk = k-2..k+1 within 0..ke-1, read q(kk-1..kk+2)
-        for (int kk = k - 2; kk <= k + 1; ++kk) {
+        // z faces kk+1/2, kk = k-2..k+1 within 0..ke-1, read q(kk-1..kk+2); cell k is slot k-kk+1
+        double s_own_z = 0.0, clo_own = 0.0, chi_own = 0.0;
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+            const int kk = k - 2 + o, slot = 3 - o;
             if (kk < 0 || kk > ke - 1) continue;
-            const int slot = k - kk + 1;
-            typedef Dl<1> V;
-            V q0(kk > 0 ? Q(kk - 1, i) : 0.0), q1(Q(kk, i)), q2(Q(kk + 1, i)), q3(kk + 2 <= ke ? Q(kk + 2, i) : 0.0);
-            if (slot == 0) q0.d[0] = 1.0;
-            if (slot == 1) q1.d[0] = 1.0;
-            if (slot == 2) q2.d[0] = 1.0;
-            if (slot == 3) q3.d[0] = 1.0;
-            const V F = kface_z(q0, q1, q2, q3, V(a.mom_z[G(kk, i)]), V(qr[G(kk, D.lin_col)]),
-                                V(qr[G(kk + 1, D.lin_col)]), kk == 0, kk == ke - 1, dzd[kk], dzd[kk + 1], D.dt);
-            Lq += LFz(kk) * F.d[0];
+            double cf[4], clo, chi;
+            bool capped;
+            const double m = a.mom_z[G(kk, i)];
+            const double sf = kcoef_z(kk > 0 ? Q(kk - 1, i) : 0.0, Q(kk, i), Q(kk + 1, i),
+                                      kk + 2 <= ke ? Q(kk + 2, i) : 0.0, m, qr[G(kk, D.lin_col)],
+                                      qr[G(kk + 1, D.lin_col)], kk == 0, kk == ke - 1, dzd[kk], dzd[kk + 1],
+                                      gz[kk].z, D.dt, cf, capped, clo, chi);
+            Lq += LFz(kk) * m * cf[slot];
+            if (o == 2) {
+                s_own_z = capped ? 0.0 : sf;
+                clo_own = clo;
+                chi_own = chi;
+            }
         }
         // diffusive x faces j+1/2, j = i-1, i: D = (rho_j + rho_j+1) / 2 (q_j+1 - q_j) / dx
         for (int j = i - 1; j <= i; ++j) {
-            const double lD = kap * dt / dx * (L[G(k, j)] - L[G(k, j + 1)]);
+            const double lD = kap * dt * rdx * (L[G(k, j)] - L[G(k, j + 1)]);
             const double r0 = rho[G(k, j)], r1 = rho[G(k, j + 1)];
-            Lq += lD * (r1 + r0) * 0.5 / dx * (j == i - 1 ? 1.0 : -1.0);
-            lrho += lD * 0.5 * (Q(k, j + 1) - Q(k, j)) / dx;
+            Lq += lD * (r1 + r0) * 0.5 * rdx * (j == i - 1 ? 1.0 : -1.0);
+            lrho += lD * 0.5 * (Q(k, j + 1) - Q(k, j)) * rdx;
         }
         // diffusive z faces kk+1/2, kk = k-1, k: D = f2h(rho) (q_kk+1 - q_kk) / (z_kk+1 - z_kk)
         for (int kk = k - 1; kk <= k; ++kk) {
             if (kk < 0 || kk > ke - 1) continue;
-            const double lD = kap * dt * (L[G(kk, i)] / dzd[kk] - L[G(kk + 1, i)] / dzd[kk + 1]);
+            const double lD = kap * dt * (L[G(kk, i)] * gz[kk].x - L[G(kk + 1, i)] * gz[kk + 1].x);
             const double r0 = rho[G(kk, i)], r1 = rho[G(kk + 1, i)];
-            const double den = dzd[kk + 1] + dzd[kk];
-            const double rf = (r1 * dzd[kk] + r0 * dzd[kk + 1]) / den;
-            const double zz = zd[kk + 1] - zd[kk];
-            const double gq = (Q(kk + 1, i) - Q(kk, i)) / zz;
-            Lq += lD * rf / zz * (kk == k - 1 ? 1.0 : -1.0);
-            lrho += lD * gq * (kk == k - 1 ? dzd[kk] : dzd[kk + 1]) / den;
+            const double rden = gz[kk].z, rzz = gz[kk].y;
+            const double rf = (r1 * dzd[kk] + r0 * dzd[kk + 1]) * rden;
+            const double gq = (Q(kk + 1, i) - Q(kk, i)) * rzz;
+            Lq += lD * rf * rzz * (kk == k - 1 ? 1.0 : -1.0);
+            lrho += lD * gq * (kk == k - 1 ? dzd[kk] : dzd[kk + 1]) * rden;
         }
-        const double qc = qr[c] / rho[c];
-        ls.f[QV + sc][c] = Lq / rho[c];
-        lrho -= Lq * qc / rho[c];
+        const double Lqr = Lq * rrho;
+        ls.f[QV + sc][c] = Lqr;
+        lrho -= Lqr * qa[c];
         // this cell owns face i+1/2 (mom_x) and face k+1/2 (mom_z)
-        {
-            typedef Dl<1> V;
-            const V F = kface_x(V(Q(k, i - 1)), V(Q(k, i)), V(Q(k, i + 1)), V(Q(k, i + 2)), V::var(a.mom_x[c], 0));
-            lmx += LFx(i) * F.d[0];
-        }
+        lmx += LFx(i) * s_own_x;
         if (k <= ke - 1) {
-            typedef Dl<3> V;
-            const V F = kface_z(V(k > 0 ? Q(k - 1, i) : 0.0), V(Q(k, i)), V(Q(k + 1, i)),
-                                V(k + 2 <= ke ? Q(k + 2, i) : 0.0), V::var(a.mom_z[c], 0),
-                                V::var(qr[G(k, D.lin_col)], 1), V::var(qr[G(k + 1, D.lin_col)], 2), k == 0,
-                                k == ke - 1, dzd[k], dzd[k + 1], D.dt);
             const double lf = LFz(k);
-            lmz += lf * F.d[0];
+            lmz += lf * s_own_z;
             // linear-index positivity limiters: bottom face reads (0, lin_col), top face (ke, lin_col)
-            if (k == 0) a.linb[D.offx * 3 + (int64_t)sc * nx + i] = lf * F.d[1];
-            if (k == ke - 1) a.lint[D.offx * 3 + (int64_t)sc * nx + i] = lf * F.d[2];
+            if (k == 0) a.linb[D.offx * 3 + (int64_t)sc * nx + i] = lf * clo_own;
+            if (k == ke - 1) a.lint[D.offx * 3 + (int64_t)sc * nx + i] = lf * chi_own;
         }
     }
     a.Lmx[c] = lmx;
@@ -415,22 +486,27 @@ __global__ void __launch_bounds__(256) ka_lin(const Dom *__restrict__ doms, AdjW
 // ---- acoustic substep, vertical part (moist.jl:1377-1446) ---------------------------
 // Coefficients of the w-p system of column i at row kk (0 <= kk <= ke-1):
 // A = tridiag(a, b + 1, c) (implicit_correction!, moist.jl:974-1030).
+// Divisions by grid quantities use the per-level reciprocals of the geometry
+// table gz: {1/dz_k, 1/(z_k+1 - z_k), 1/(dz_k + dz_k+1), g/(dz_k + dz_k+1)}.
 struct WpCoef {
     const Dom &D;
-    const double *zd, *dzd, *TH, *RT;
+    const double *zd, *dzd;
+    const double4 *gz;
+    const double *TH, *RT;
     int nx, i;
     double dt_tp2;
     __device__ int64_t G(int kk) const { return D.off2 + (int64_t)kk * nx + i; }
-    __device__ double th(int kk) const { return f2h(TH[G(kk)], TH[G(kk + 1)], dzd[kk], dzd[kk + 1]); }
+    __device__ double th(int kk) const { return (TH[G(kk)] * dzd[kk + 1] + TH[G(kk + 1)] * dzd[kk]) * gz[kk].z; }
     __device__ double z1(int kk) const { return zd[kk + 1] - zd[kk]; }
+    __device__ double rz1(int kk) const { return gz[kk].y; }
     __device__ double a(int kk) const {  // kk > 0
-        return -dt_tp2 * (RT[G(kk)] * th(kk - 1) / (z1(kk) * dzd[kk]) - K<double>::GRAV / (dzd[kk] + dzd[kk + 1]));
+        return -dt_tp2 * (RT[G(kk)] * th(kk - 1) * (gz[kk].y * gz[kk].x) - gz[kk].w);
     }
     __device__ double b(int kk) const {
-        return dt_tp2 * (RT[G(kk)] / dzd[kk] + RT[G(kk + 1)] / dzd[kk + 1]) * th(kk) / z1(kk);
+        return dt_tp2 * (RT[G(kk)] * gz[kk].x + RT[G(kk + 1)] * gz[kk + 1].x) * th(kk) * gz[kk].y;
     }
     __device__ double c(int kk) const {  // kk < ke - 1
-        return -dt_tp2 * (RT[G(kk + 1)] * th(kk + 1) / (z1(kk) * dzd[kk + 1]) + K<double>::GRAV / (dzd[kk] + dzd[kk + 1]));
+        return -dt_tp2 * (RT[G(kk + 1)] * th(kk + 1) * (gz[kk].y * gz[kk + 1].x) + gz[kk].w);
     }
 };
 
@@ -447,7 +523,8 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     const int nx = D.nx, nz = D.nz, ke = nz - 1;
     if (i >= nx || k >= nz) return;
     const double *dzd = zv.dz + D.offz;
-    const double EPS = 1e-12;
+    const double4 *gz = zv.geo + D.offz;  // per level: 1/dz, 1/(z_k+1 - z_k), 1/(dz_k + dz_k+1)
+    const double EPS = 1e-12, rns = 1.0 / double(nsub);
     const double dt_s = D.dt / double(div) / double(nsub);
     auto G = [&](int kk) -> int64_t { return D.off2 + (int64_t)kk * nx + i; };
     const double *TH = a.theta, *rw = s.f[RHOW];
@@ -466,27 +543,27 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     double lth = 0.0;
     for (int f = k - 2; f <= k + 1; ++f) {
         if (f < 0 || f > ke - 1) continue;
-        const double den = dzd[f] + dzd[f + 1];
-        const double w0 = dzd[f + 1] / den, w1 = dzd[f] / den;
+        const double rden = gz[f].z;
+        const double w0 = dzd[f + 1] * rden, w1 = dzd[f] * rden;
         const double w = rwp_n[G(f)];
         bool pos, neg;
         brp(f, pos, neg);
-        lth += -dt_s / dzd[f] * a.Lrtp[G(f)] * w * upw_coef(pos, neg, k - f, w0, w1);
+        lth += -dt_s * gz[f].x * a.Lrtp[G(f)] * w * upw_coef(pos, neg, k - f, w0, w1);
         brm(f, pos, neg);
-        lth += dt_s / dzd[f + 1] * a.Lrtp[G(f + 1)] * w * upw_coef(pos, neg, k - f, w0, w1);
+        lth += dt_s * gz[f + 1].x * a.Lrtp[G(f + 1)] * w * upw_coef(pos, neg, k - f, w0, w1);
     }
     a.Lth[G(k)] += lth;
     if (k <= ke - 1) {
-        const double den = dzd[k] + dzd[k + 1];
-        const double w0 = dzd[k + 1] / den, w1 = dzd[k] / den;
+        const double rden = gz[k].z;
+        const double w0 = dzd[k + 1] * rden, w1 = dzd[k] * rden;
         bool pos, neg;
         brp(k, pos, neg);
         const double gp = upw_val(pos, neg, THv(k - 1), THv(k), THv(k + 1), THv(k + 2), w0, w1);
         brm(k, pos, neg);
         const double gm = upw_val(pos, neg, THv(k - 1), THv(k), THv(k + 1), THv(k + 2), w0, w1);
         const int64_t q = G(k), qp = G(k + 1);
-        a.Lrwp[q] += a.Lmz[q] / double(nsub) - dt_s / dzd[k] * a.Lrtp[q] * gp + dt_s / dzd[k + 1] * a.Lrtp[qp] * gm -
-                     dt_s / dzd[k] * a.Lrp[q] + dt_s / dzd[k + 1] * a.Lrp[qp];
+        a.Lrwp[q] += a.Lmz[q] * rns - dt_s * gz[k].x * a.Lrtp[q] * gp + dt_s * gz[k + 1].x * a.Lrtp[qp] * gm -
+                     dt_s * gz[k].x * a.Lrp[q] + dt_s * gz[k + 1].x * a.Lrp[qp];
     }
 }
 
@@ -553,7 +630,7 @@ __global__ void __launch_bounds__(256)
     const bool live = i < nx;
     const double dt_s = D.dt / double(div) / double(nsub);
     const double fact_tp = 0.5 * (1.0 + D.beta);
-    const WpCoef W{D, zv.z + D.offz, zv.dz + D.offz, a.theta, a.rt2, nx, live ? i : 0,
+    const WpCoef W{D, zv.z + D.offz, zv.dz + D.offz, zv.geo + D.offz, a.theta, a.rt2, nx, live ? i : 0,
                    (dt_s * fact_tp) * (dt_s * fact_tp)};
     const int L = (n + CH - 1) / CH, j0 = ch * L, j1 = live ? min(j0 + L, n) : j0;
     if (factor) {
@@ -643,14 +720,15 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
     const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
-    const int nx = D.nx, nz = D.nz, ke = nz - 1, n = nz - 1;
+    const int nx = D.nx, nz = D.nz, n = nz - 1;
     if (i >= nx || k >= nz || n <= 0) return;
     const double *zd = zv.z + D.offz, *dzd = zv.dz + D.offz;
-    const double GRAV = K<double>::GRAV, dx = D.dx;
+    const double GRAV = K<double>::GRAV, dx = D.dx, rdx = 1.0 / dx;
     const double dt_s = D.dt / double(div) / double(nsub);
     const double fact_tp = 0.5 * (1.0 + D.beta), fact_tm = 0.5 * (1.0 - D.beta);
     const double fact = fact_tm / fact_tp;
-    const WpCoef W{D, zd, dzd, a.theta, a.rt2, nx, i, (dt_s * fact_tp) * (dt_s * fact_tp)};
+    const double4 *gz = zv.geo + D.offz;
+    const WpCoef W{D, zd, dzd, gz, a.theta, a.rt2, nx, i, (dt_s * fact_tp) * (dt_s * fact_tp)};
     const double dt_tp2 = W.dt_tp2;
     const double *RT = a.rt2, *TH = a.theta;
     auto G = [&](int kk) -> int64_t { return W.G(kk); };
@@ -665,9 +743,9 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     // adjoint of the face theta th_j (j = 0..n-1)
     auto Lthc = [&](int j) {
         if (j < 0 || j > n - 1) return 0.0;
-        double v = Lb(j) * dt_tp2 * (RT[G(j)] / dzd[j] + RT[G(j + 1)] / dzd[j + 1]) / W.z1(j);
-        if (j + 1 <= n - 1) v += La(j + 1) * (-dt_tp2 * RT[G(j + 1)] / (W.z1(j + 1) * dzd[j + 1]));
-        if (j >= 1) v += Lc(j - 1) * (-dt_tp2 * RT[G(j)] / (W.z1(j - 1) * dzd[j]));
+        double v = Lb(j) * dt_tp2 * (RT[G(j)] * gz[j].x + RT[G(j + 1)] * gz[j + 1].x) * gz[j].y;
+        if (j + 1 <= n - 1) v += La(j + 1) * (-dt_tp2 * RT[G(j + 1)] * (gz[j + 1].y * gz[j + 1].x));
+        if (j >= 1) v += Lc(j - 1) * (-dt_tp2 * RT[G(j)] * (gz[j - 1].y * gz[j].x));
         return v;
     };
     // off-centred rho_theta' of the forward explicit part
@@ -675,23 +753,23 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
         const int64_t q = G(kk);
         const double fip = (TH[Gx(kk, i + 1)] + TH[q]) * 0.5 * rup_n[q];
         const double fim = (TH[q] + TH[Gx(kk, i - 1)]) * 0.5 * rup_n[Gx(kk, i - 1)];
-        const double inc = dt_s * (a.R_rt[q] + (-(fip - fim) / dx));
+        const double inc = dt_s * (a.R_rt[q] + (-(fip - fim) * rdx));
         return rtp_s[q] + fact_tp * inc;
     };
     const int64_t q = G(k);
     double lrt = 0.0, lth = 0.0, Lrp_tp = 0.0, Lrtp_tp = 0.0;
     if (k <= n - 1) {
         const double y = yv(k);
-        const double thk = W.th(k), z1 = W.z1(k);
-        lrt += Lb(k) * dt_tp2 * thk / (z1 * dzd[k]);
-        if (k >= 1) lrt += La(k) * (-dt_tp2 * W.th(k - 1) / (z1 * dzd[k]));
-        const double buoy = (rtp_tp(k + 1) - rtp_tp(k)) / z1;
+        const double thk = W.th(k), rz1 = gz[k].y, rzd = rz1 * gz[k].x;
+        lrt += Lb(k) * dt_tp2 * thk * rzd;
+        if (k >= 1) lrt += La(k) * (-dt_tp2 * W.th(k - 1) * rzd);
+        const double buoy = (rtp_tp(k + 1) - rtp_tp(k)) * rz1;
         lrt += -0.5 * dt_s * y * buoy;
         const double rt2h = (RT[q] + RT[G(k + 1)]) * 0.5;
-        Lrtp_tp += dt_s * y * rt2h / z1;
+        Lrtp_tp += dt_s * y * rt2h * rz1;
         Lrp_tp += -dt_s * y * GRAV * 0.5;
         a.LRw[q] += dt_s * y;
-        lth += Lthc(k) * dzd[k + 1] / (dzd[k] + dzd[k + 1]);
+        lth += Lthc(k) * dzd[k + 1] * gz[k].z;
         // rho_w' before the solve
         double lws = y * (1.0 - fact * W.b(k));
         if (k >= 1) lws += -fact * W.a(k) * y - fact * W.c(k - 1) * yv(k - 1);
@@ -700,15 +778,15 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     if (k >= 1) {
         const int kb = k - 1;
         const double yb = yv(kb);
-        const double z1 = W.z1(kb);
-        lrt += Lb(kb) * dt_tp2 * W.th(kb) / (z1 * dzd[k]);
-        if (kb < n - 1) lrt += Lc(kb) * (-dt_tp2 * W.th(k) / (z1 * dzd[k]));
-        const double buoy = (rtp_tp(k) - rtp_tp(kb)) / z1;
+        const double rz1 = gz[kb].y, rzd = rz1 * gz[k].x;
+        lrt += Lb(kb) * dt_tp2 * W.th(kb) * rzd;
+        if (kb < n - 1) lrt += Lc(kb) * (-dt_tp2 * W.th(k) * rzd);
+        const double buoy = (rtp_tp(k) - rtp_tp(kb)) * rz1;
         lrt += -0.5 * dt_s * yb * buoy;
         const double rt2h = (RT[G(kb)] + RT[q]) * 0.5;
-        Lrtp_tp += -dt_s * yb * rt2h / z1;
+        Lrtp_tp += -dt_s * yb * rt2h * rz1;
         Lrp_tp += -dt_s * yb * GRAV * 0.5;
-        lth += Lthc(kb) * dzd[kb] / (dzd[kb] + dzd[k]);
+        lth += Lthc(kb) * dzd[kb] * gz[kb].z;
     }
     a.Lrt[q] += lrt;
     a.Lth[q] += lth;
@@ -736,22 +814,22 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
     const int nx = D.nx, nz = D.nz;
     if (i >= nx || k >= nz) return;
-    const double dx = D.dx, gam = D.gamma_d;
-    const double dt_s = D.dt / double(div) / double(nsub);
+    const double dx = D.dx, gam = D.gamma_d, rdx = 1.0 / dx;
+    const double dt_s = D.dt / double(div) / double(nsub), rns = 1.0 / double(nsub);
     auto G = [&](int ii) -> int64_t { return D.off2 + (int64_t)k * nx + wrapn(ii, nx); };
     const double *TH = a.theta, *RT = a.rt2;
     // adjoint of rho_u'(j) after the substep, all contributions of this substep included
     auto lrup = [&](int j) {
         const int64_t q = G(j), qp = G(j + 1);
-        const double lF = dt_s / dx * (a.Linc_t[qp] - a.Linc_t[q]);
-        return Lrup_in[q] + a.Lmx[q] / double(nsub) - dt_s / dx * a.Linc_r[q] + dt_s / dx * a.Linc_r[qp] +
+        const double lF = dt_s * rdx * (a.Linc_t[qp] - a.Linc_t[q]);
+        return Lrup_in[q] + a.Lmx[q] * rns - dt_s * rdx * a.Linc_r[q] + dt_s * rdx * a.Linc_r[qp] +
                lF * 0.5 * (TH[q] + TH[qp]);
     };
     const int64_t c = G(i), cm = G(i - 1), cp = G(i + 1);
     // theta adjoint of the rho_theta' flux (faces i+1/2 and i-1/2)
     {
-        const double lFi = dt_s / dx * (a.Linc_t[cp] - a.Linc_t[c]);
-        const double lFm = dt_s / dx * (a.Linc_t[c] - a.Linc_t[cm]);
+        const double lFi = dt_s * rdx * (a.Linc_t[cp] - a.Linc_t[c]);
+        const double lFm = dt_s * rdx * (a.Linc_t[c] - a.Linc_t[cm]);
         a.Lth[c] += 0.5 * (lFi * rup_n[c] + lFm * rup_n[cm]);
     }
     const double Li = lrup(i), Lm = lrup(i - 1);
@@ -759,16 +837,16 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     double lt = a.Lrtp[c] + a.pend[c];
     // rho_u' update: rup_n = rup_s + dt_s (R_rho_u - rt2h ((1 + g) gx - g gxo))
     auto Gx = [&](int j) {
-        const double gx = (rtp_s[G(j + 1)] - rtp_s[G(j)]) / dx;
-        const double gxo = (rtp_prev[G(j + 1)] - rtp_prev[G(j)]) / dx;
+        const double gx = (rtp_s[G(j + 1)] - rtp_s[G(j)]) * rdx;
+        const double gxo = (rtp_prev[G(j + 1)] - rtp_prev[G(j)]) * rdx;
         return gx + gam * (gx - gxo);
     };
     a.LRu[c] += dt_s * Li;
     a.Lrt[c] += -0.5 * dt_s * (Li * Gx(i) + Lm * Gx(i - 1));
     const double lGi = -dt_s * (RT[c] + RT[cp]) * 0.5 * Li;
     const double lGm = -dt_s * (RT[cm] + RT[c]) * 0.5 * Lm;
-    lt += (1.0 + gam) / dx * (lGm - lGi);
-    const double lo = -gam / dx * (lGm - lGi);
+    lt += (1.0 + gam) * rdx * (lGm - lGi);
+    const double lo = -gam * rdx * (lGm - lGi);
     if (first_substep) {
         lt += lo;
         a.pend[c] = 0.0;
@@ -794,14 +872,14 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     const int nx = D.nx, nz = D.nz, ke = nz - 1;
     if (i >= nx || k >= nz) return;
     const double *zd = zv.z + D.offz, *zfd = zv.zf + D.offz, *dzd = zv.dz + D.offz;
-    const double dx = D.dx, EPS = 1e-12, GRAV = K<double>::GRAV, nu = D.nu;
+    const double dx = D.dx, EPS = 1e-12, GRAV = K<double>::GRAV, nu = D.nu, rdx = 1.0 / dx;
+    const double4 *gz = zv.geo + D.offz;  // per level: 1/dz, 1/(z_k+1 - z_k), 1/(dz_k + dz_k+1)
     auto G = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
     const int64_t c = G(k, i);
     const double *rho = s.f[RHO], *ru = s.f[RHOU], *rw = s.f[RHOW];
     const double *TH = a.theta;
     auto U = [&](int kk, int ii) { return a.uf[G(kk, ii)]; };
     auto W = [&](int kk, int ii) { return a.wf[G(kk, ii)]; };  // kk <= ke - 1
-    auto zz = [&](int kk) { return zd[kk + 1] - zd[kk]; };
     double l_r = 0.0, l_u = 0.0, l_w = 0.0, LUc = 0.0, LWc = 0.0, lp = 0.0, lth = 0.0;
 
     // perturbation init and momentum copies
@@ -822,13 +900,13 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     lph.f[2][c] += a.LRt[c];
 
     // ---- R_rho (moist.jl:1079-1086)
-    l_u += -a.LRr[c] / dx + a.LRr[G(k, i + 1)] / dx;
-    if (k < ke) l_w += -a.LRr[c] / dzd[k] + a.LRr[G(k + 1, i)] / dzd[k + 1];
+    l_u += -a.LRr[c] * rdx + a.LRr[G(k, i + 1)] * rdx;
+    if (k < ke) l_w += -a.LRr[c] * gz[k].x + a.LRr[G(k + 1, i)] * gz[k + 1].x;
 
     // ---- R_rho_u (moist.jl:1098-1179)
     // x flux F(j) between u-faces j and j+1, m = (rho_u_j + rho_u_j+1)/2
     for (int j = i - 2; j <= i + 1; ++j) {
-        const double lF = (a.LRu[G(k, j + 1)] - a.LRu[G(k, j)]) / dx;
+        const double lF = (a.LRu[G(k, j + 1)] - a.LRu[G(k, j)]) * rdx;
         const double m = 0.5 * (ru[G(k, j)] + ru[G(k, j + 1)]);
         const bool pos = m > EPS, neg = m < -EPS;
         LUc += lF * m * upw_coef(pos, neg, i - j, 0.5, 0.5);
@@ -840,71 +918,71 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     // z flux at corner (kk+1/2, face i), kk = k-2..k+1 within 0..ke-1
     for (int kk = k - 2; kk <= k + 1; ++kk) {
         if (kk < 0 || kk > ke - 1) continue;
-        const double lF = a.LRu[G(kk + 1, i)] / dzd[kk + 1] - a.LRu[G(kk, i)] / dzd[kk];
+        const double lF = a.LRu[G(kk + 1, i)] * gz[kk + 1].x - a.LRu[G(kk, i)] * gz[kk].x;
         const double m = 0.5 * (rw[G(kk, i)] + rw[G(kk, i + 1)]);
         const bool pos = m > EPS && kk > 0, neg = !pos && m < -EPS && kk < ke - 1;
-        const double den = dzd[kk] + dzd[kk + 1];
-        LUc += lF * m * upw_coef(pos, neg, k - kk, dzd[kk + 1] / den, dzd[kk] / den);
+        const double rden = gz[kk].z;
+        LUc += lF * m * upw_coef(pos, neg, k - kk, dzd[kk + 1] * rden, dzd[kk] * rden);
     }
     // rho_w at (k, i) enters the corner fluxes (k+1/2, i) and (k+1/2, i-1)
     if (k <= ke - 1) {
         for (int ii = i - 1; ii <= i; ++ii) {
-            const double lF = a.LRu[G(k + 1, ii)] / dzd[k + 1] - a.LRu[G(k, ii)] / dzd[k];
+            const double lF = a.LRu[G(k + 1, ii)] * gz[k + 1].x - a.LRu[G(k, ii)] * gz[k].x;
             const double m = 0.5 * (rw[G(k, ii)] + rw[G(k, ii + 1)]);
             const bool pos = m > EPS && k > 0, neg = !pos && m < -EPS && k < ke - 1;
-            const double den = dzd[k] + dzd[k + 1];
+            const double rden = gz[k].z;
             const double g = upw_val(pos, neg, k > 0 ? U(k - 1, ii) : 0.0, U(k, ii), U(k + 1, ii),
-                                     k + 2 <= ke ? U(k + 2, ii) : 0.0, dzd[k + 1] / den, dzd[k] / den);
+                                     k + 2 <= ke ? U(k + 2, ii) : 0.0, dzd[k + 1] * rden, dzd[k] * rden);
             l_w += lF * g * 0.5;
         }
     }
     // viscous x flux Dx(j) = (U_j+1 - U_j) / dx * rho_j+1, j = i-1, i
     for (int j = i - 1; j <= i; ++j) {
-        const double lD = nu * (a.LRu[G(k, j)] - a.LRu[G(k, j + 1)]) / dx;
-        LUc += lD * rho[G(k, j + 1)] / dx * (j == i - 1 ? 1.0 : -1.0);
-        if (j == i - 1) l_r += lD * (U(k, i) - U(k, i - 1)) / dx;
+        const double lD = nu * (a.LRu[G(k, j)] - a.LRu[G(k, j + 1)]) * rdx;
+        LUc += lD * rho[G(k, j + 1)] * rdx * (j == i - 1 ? 1.0 : -1.0);
+        if (j == i - 1) l_r += lD * (U(k, i) - U(k, i - 1)) * rdx;
     }
     // viscous z flux Dz(kk) = (U_kk+1 - U_kk) / zz * rho at the corner, kk = k-1, k
     for (int kk = k - 1; kk <= k; ++kk) {
         if (kk < 0 || kk > ke - 1) continue;
-        const double lD = nu * (a.LRu[G(kk, i)] / dzd[kk] - a.LRu[G(kk + 1, i)] / dzd[kk + 1]);
+        const double lD = nu * (a.LRu[G(kk, i)] * gz[kk].x - a.LRu[G(kk + 1, i)] * gz[kk + 1].x);
         const double rc = 0.25 * (rho[G(kk, i)] + rho[G(kk + 1, i)] + rho[G(kk, i + 1)] + rho[G(kk + 1, i + 1)]);
-        LUc += lD * rc / zz(kk) * (kk == k - 1 ? 1.0 : -1.0);
+        LUc += lD * rc * gz[kk].y * (kk == k - 1 ? 1.0 : -1.0);
     }
     // rho(k, i) in the corner densities of the z viscous fluxes (kk, ii), kk = k-1, k; ii = i-1, i
     for (int kk = k - 1; kk <= k; ++kk) {
         if (kk < 0 || kk > ke - 1) continue;
         for (int ii = i - 1; ii <= i; ++ii) {
-            const double lD = nu * (a.LRu[G(kk, ii)] / dzd[kk] - a.LRu[G(kk + 1, ii)] / dzd[kk + 1]);
-            l_r += lD * (U(kk + 1, ii) - U(kk, ii)) / zz(kk) * 0.25;
+            const double lD = nu * (a.LRu[G(kk, ii)] * gz[kk].x - a.LRu[G(kk + 1, ii)] * gz[kk + 1].x);
+            l_r += lD * (U(kk + 1, ii) - U(kk, ii)) * gz[kk].y * 0.25;
         }
     }
     // pressure gradient (p_i+1 - p_i) / dx
-    lp += a.LRu[c] / dx - a.LRu[G(k, i - 1)] / dx;
+    lp += a.LRu[c] * rdx - a.LRu[G(k, i - 1)] * rdx;
 
     // ---- R_rho_w (moist.jl:1190-1279), cells k' <= ke - 1
     const double sponge_base = (1.0 - D.zf_sponge) * D.H;
     auto LRw = [&](int kk, int ii) { return a.LRw[G(kk, ii)]; };
     if (k <= ke - 1) {
         // fip(k, j): m = f2h(rho_u(k,j), rho_u(k+1,j), z_k, z_k+1), reads W(j-1..j+2)
-        const double zs = zd[k] + zd[k + 1];
+        const double rzs = 1.0 / (zd[k] + zd[k + 1]);
         for (int j = i - 2; j <= i + 1; ++j) {
-            const double lF = -LRw(k, j) / dx;
-            const double m = (ru[G(k, j)] * zd[k + 1] + ru[G(k + 1, j)] * zd[k]) / zs;
+            const double lF = -LRw(k, j) * rdx;
+            const double m = (ru[G(k, j)] * zd[k + 1] + ru[G(k + 1, j)] * zd[k]) * rzs;
             LWc += lF * m * upw_coef(m > EPS, m < -EPS, i - j, 0.5, 0.5);
         }
         // fim(k, j): m = f2h(rho_u(k,j-1), rho_u(k+1,j-1), dz), reads W(j-2..j+1): interface between j-1 and j
-        const double ds = dzd[k] + dzd[k + 1];
+        const double rds = gz[k].z;
         for (int j = i - 1; j <= i + 2; ++j) {
-            const double lF = LRw(k, j) / dx;
-            const double m = (ru[G(k, j - 1)] * dzd[k + 1] + ru[G(k + 1, j - 1)] * dzd[k]) / ds;
+            const double lF = LRw(k, j) * rdx;
+            const double m = (ru[G(k, j - 1)] * dzd[k + 1] + ru[G(k + 1, j - 1)] * dzd[k]) * rds;
             LWc += lF * m * upw_coef(m > EPS, m < -EPS, i - (j - 1), 0.5, 0.5);
         }
         // viscous x flux Ex(j) = (W_j+1 - W_j) / dx * rho at corner (k+1/2, j+1/2), j = i-1, i
         for (int j = i - 1; j <= i; ++j) {
-            const double lE = nu * (LRw(k, j) - LRw(k, j + 1)) / dx;
+            const double lE = nu * (LRw(k, j) - LRw(k, j + 1)) * rdx;
             const double rc = 0.25 * (rho[G(k, j)] + rho[G(k + 1, j)] + rho[G(k, j + 1)] + rho[G(k + 1, j + 1)]);
-            LWc += lE * rc / dx * (j == i - 1 ? 1.0 : -1.0);
+            LWc += lE * rc * rdx * (j == i - 1 ? 1.0 : -1.0);
         }
         // sponge
         double sigma = 0.0;
@@ -914,63 +992,63 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     // z flux Fz(kk) between w levels kk and kk+1 (kk < ke - 1), shared by cells kk and kk+1
     for (int kk = k - 2; kk <= k + 1; ++kk) {
         if (kk < 0 || kk >= ke - 1) continue;
-        const double lF = LRw(kk + 1, i) / zz(kk + 1) - LRw(kk, i) / zz(kk);
-        const double ds = dzd[kk] + dzd[kk + 1];
-        const double m = (rw[G(kk, i)] * dzd[kk + 1] + rw[G(kk + 1, i)] * dzd[kk]) / ds;
+        const double lF = LRw(kk + 1, i) * gz[kk + 1].y - LRw(kk, i) * gz[kk].y;
+        const double rds = gz[kk].z;
+        const double m = (rw[G(kk, i)] * dzd[kk + 1] + rw[G(kk + 1, i)] * dzd[kk]) * rds;
         const bool pos = m > EPS && kk > 0, neg = !pos && m < -EPS && kk < ke - 2;
-        if (k <= ke - 1) LWc += lF * m * upw_coef(pos, neg, k - kk, dzd[kk + 1] / ds, dzd[kk] / ds);
+        if (k <= ke - 1) LWc += lF * m * upw_coef(pos, neg, k - kk, dzd[kk + 1] * rds, dzd[kk] * rds);
         if (kk == k || kk == k - 1) {
             const double g = upw_val(pos, neg, kk > 0 ? W(kk - 1, i) : 0.0, W(kk, i), W(kk + 1, i),
-                                     kk + 2 <= ke - 1 ? W(kk + 2, i) : 0.0, dzd[kk + 1] / ds, dzd[kk] / ds);
-            l_w += lF * g * (kk == k ? dzd[kk + 1] : dzd[kk]) / ds;
+                                     kk + 2 <= ke - 1 ? W(kk + 2, i) : 0.0, dzd[kk + 1] * rds, dzd[kk] * rds);
+            l_w += lF * g * (kk == k ? dzd[kk + 1] : dzd[kk]) * rds;
         }
     }
     // viscous z flux Ez(kk) = (W_kk+1 - W_kk) / dz_kk+1 * rho_kk+1, kk = k-1, k (kk < ke - 1)
     for (int kk = k - 1; kk <= k; ++kk) {
         if (kk < 0 || kk >= ke - 1) continue;
-        const double lE = nu * (LRw(kk, i) / zz(kk) - LRw(kk + 1, i) / zz(kk + 1));
-        if (k <= ke - 1) LWc += lE * rho[G(kk + 1, i)] / dzd[kk + 1] * (kk == k - 1 ? 1.0 : -1.0);
-        if (kk == k - 1) l_r += lE * (W(k, i) - W(k - 1, i)) / dzd[k];
+        const double lE = nu * (LRw(kk, i) * gz[kk].y - LRw(kk + 1, i) * gz[kk + 1].y);
+        if (k <= ke - 1) LWc += lE * rho[G(kk + 1, i)] * gz[kk + 1].x * (kk == k - 1 ? 1.0 : -1.0);
+        if (kk == k - 1) l_r += lE * (W(k, i) - W(k - 1, i)) * gz[k].x;
     }
     // rho(k, i) in the corner densities of the x viscous fluxes Ex(k', j), k' = k-1, k; j = i-1, i
     for (int kp = k - 1; kp <= k; ++kp) {
         if (kp < 0 || kp > ke - 1) continue;
         for (int j = i - 1; j <= i; ++j) {
-            const double lE = nu * (LRw(kp, j) - LRw(kp, j + 1)) / dx;
-            l_r += lE * (W(kp, j + 1) - W(kp, j)) / dx * 0.25;
+            const double lE = nu * (LRw(kp, j) - LRw(kp, j + 1)) * rdx;
+            l_r += lE * (W(kp, j + 1) - W(kp, j)) * rdx * 0.25;
         }
     }
     // rho_u(k, i) in the R_rho_w x fluxes: fip(k', i) and fim(k', i+1), k' = k-1, k
     for (int kp = k - 1; kp <= k; ++kp) {
         if (kp < 0 || kp > ke - 1) continue;
         {
-            const double lF = -LRw(kp, i) / dx;
-            const double zs = zd[kp] + zd[kp + 1];
-            const double m = (ru[G(kp, i)] * zd[kp + 1] + ru[G(kp + 1, i)] * zd[kp]) / zs;
+            const double lF = -LRw(kp, i) * rdx;
+            const double rzs = 1.0 / (zd[kp] + zd[kp + 1]);
+            const double m = (ru[G(kp, i)] * zd[kp + 1] + ru[G(kp + 1, i)] * zd[kp]) * rzs;
             const double g = upw_val(m > EPS, m < -EPS, W(kp, i - 1), W(kp, i), W(kp, i + 1), W(kp, i + 2), 0.5, 0.5);
-            l_u += lF * g * (kp == k ? zd[kp + 1] : zd[kp]) / zs;
+            l_u += lF * g * (kp == k ? zd[kp + 1] : zd[kp]) * rzs;
         }
         {
-            const double lF = LRw(kp, i + 1) / dx;
-            const double ds = dzd[kp] + dzd[kp + 1];
-            const double m = (ru[G(kp, i)] * dzd[kp + 1] + ru[G(kp + 1, i)] * dzd[kp]) / ds;
+            const double lF = LRw(kp, i + 1) * rdx;
+            const double rds = gz[kp].z;
+            const double m = (ru[G(kp, i)] * dzd[kp + 1] + ru[G(kp + 1, i)] * dzd[kp]) * rds;
             const double g = upw_val(m > EPS, m < -EPS, W(kp, i - 1), W(kp, i), W(kp, i + 1), W(kp, i + 2), 0.5, 0.5);
-            l_u += lF * g * (kp == k ? dzd[kp + 1] : dzd[kp]) / ds;
+            l_u += lF * g * (kp == k ? dzd[kp + 1] : dzd[kp]) * rds;
         }
     }
     // pressure gradient (p_k+1 - p_k) / zz and buoyancy -g (rho_k + rho_k+1) / 2
     if (k <= ke - 1) {
-        lp += LRw(k, i) / zz(k);
+        lp += LRw(k, i) * gz[k].y;
         l_r += -GRAV * 0.5 * LRw(k, i);
     }
     if (k >= 1 && k - 1 <= ke - 1) {
-        lp += -LRw(k - 1, i) / zz(k - 1);
+        lp += -LRw(k - 1, i) * gz[k - 1].y;
         l_r += -GRAV * 0.5 * LRw(k - 1, i);
     }
 
     // ---- R_rho_theta (moist.jl:1284-1332)
     for (int j = i - 2; j <= i + 1; ++j) {
-        const double lF = (a.LRt[G(k, j + 1)] - a.LRt[G(k, j)]) / dx;
+        const double lF = (a.LRt[G(k, j + 1)] - a.LRt[G(k, j)]) * rdx;
         const double m = ru[G(k, j)];
         const bool pos = m > EPS, neg = m < -EPS;
         lth += lF * m * upw_coef(pos, neg, i - j, 0.5, 0.5);
@@ -978,14 +1056,14 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     }
     for (int kk = k - 2; kk <= k + 1; ++kk) {
         if (kk < 0 || kk > ke - 1) continue;
-        const double lF = a.LRt[G(kk + 1, i)] / dzd[kk + 1] - a.LRt[G(kk, i)] / dzd[kk];
+        const double lF = a.LRt[G(kk + 1, i)] * gz[kk + 1].x - a.LRt[G(kk, i)] * gz[kk].x;
         const double m = rw[G(kk, i)];
         const bool pos = m > EPS && kk > 0, neg = !pos && m < -EPS && kk < ke - 1;
-        const double den = dzd[kk] + dzd[kk + 1];
-        lth += lF * m * upw_coef(pos, neg, k - kk, dzd[kk + 1] / den, dzd[kk] / den);
+        const double rden = gz[kk].z;
+        lth += lF * m * upw_coef(pos, neg, k - kk, dzd[kk + 1] * rden, dzd[kk] * rden);
         if (kk == k)
             l_w += lF * upw_val(pos, neg, kk > 0 ? TH[G(kk - 1, i)] : 0.0, TH[G(kk, i)], TH[G(kk + 1, i)],
-                                kk + 2 <= ke ? TH[G(kk + 2, i)] : 0.0, dzd[kk + 1] / den, dzd[kk] / den);
+                                kk + 2 <= ke ? TH[G(kk + 2, i)] : 0.0, dzd[kk + 1] * rden, dzd[kk] * rden);
     }
 
     ls.f[RHO][c] += l_r;
@@ -1014,40 +1092,42 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     double l_r = 0.0, l_u = 0.0, l_w = 0.0;
     // U(j) = 2 rho_u(j) / (rho_j + rho_j+1)
     {
-        const double den = rho[c] + rho[cp];
-        const double Uc = ru[c] * 2.0 / den;
-        l_u += a.LU[c] * 2.0 / den;
-        l_r += -a.LU[c] * Uc / den;
-        const double denm = rho[cm] + rho[c];
-        l_r += -a.LU[cm] * (ru[cm] * 2.0 / denm) / denm;
+        const double rden = 1.0 / (rho[c] + rho[cp]);
+        const double Uc = ru[c] * 2.0 * rden;
+        l_u += a.LU[c] * 2.0 * rden;
+        l_r += -a.LU[c] * Uc * rden;
+        const double rdm = 1.0 / (rho[cm] + rho[c]);
+        l_r += -a.LU[cm] * (ru[cm] * 2.0 * rdm) * rdm;
     }
     // W(kk) = rho_w(kk) / f2h(rho_kk, rho_kk+1)
+    const double4 *gz = zv.geo + D.offz;  // per level: 1/dz, 1/(z_k+1 - z_k), 1/(dz_k + dz_k+1)
     if (k <= ke - 1) {
-        const double ds = dzd[k] + dzd[k + 1];
-        const double rf = (rho[c] * dzd[k + 1] + rho[G(k + 1, i)] * dzd[k]) / ds;
-        const double Wc = rw[c] / rf;
-        l_w += a.LW[c] / rf;
-        l_r += -a.LW[c] * Wc / rf * dzd[k + 1] / ds;
+        const double rds = gz[k].z;
+        const double rrf = 1.0 / ((rho[c] * dzd[k + 1] + rho[G(k + 1, i)] * dzd[k]) * rds);
+        const double Wc = rw[c] * rrf;
+        l_w += a.LW[c] * rrf;
+        l_r += -a.LW[c] * Wc * rrf * dzd[k + 1] * rds;
     }
     if (k >= 1) {
         const int kb = k - 1;
-        const double ds = dzd[kb] + dzd[k];
-        const double rf = (rho[G(kb, i)] * dzd[k] + rho[c] * dzd[kb]) / ds;
-        const double Wb = rw[G(kb, i)] / rf;
-        l_r += -a.LW[G(kb, i)] * Wb / rf * dzd[kb] / ds;
+        const double rds = gz[kb].z;
+        const double rrf = 1.0 / ((rho[G(kb, i)] * dzd[k] + rho[c] * dzd[kb]) * rds);
+        const double Wb = rw[G(kb, i)] * rrf;
+        l_r += -a.LW[G(kb, i)] * Wb * rrf * dzd[kb] * rds;
     }
     // diagnostics pres, theta, rt2pres
     typedef Dl<5> V;
     const K<double> C;
     V rr = V::var(rho[c], 0), rt = V::var(s.f[RT][c], 1), rqv = V::var(s.f[QV][c], 2), rqc = V::var(s.f[QC][c], 3),
       rqr = V::var(s.f[QR][c], 4);
-    V qv = rqv / rr, qc = rqc / rr, qr = rqr / rr;
+    const V rinv = 1.0 / rr;
+    V qv = rqv * rinv, qc = rqc * rinv, qr = rqr * rinv;
     V qd = V(1.0) - qv - qc - qr;
     V r = C.Rd * qd + C.Rv * qv;
     V cp_ = C.Cpd * qd + C.Cpv * qv + C.Cpl * (qc + qr);
     V cv = cp_ - r;
     V pres = C.P0 * d_pow(r * rt / C.P0, cp_ / cv);
-    V theta = rt / rr;
+    V theta = rt * rinv;
     V rt2 = cp_ / cv * pres / rt;
     const double lp = a.Lpres[c], lt = a.Lth[c], lr2 = a.Lrt[c];
     double g[5];
