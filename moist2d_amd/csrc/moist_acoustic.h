@@ -35,6 +35,9 @@
 #define MOIST_AC_THREADS 1024
 #endif
 constexpr int AC_THREADS = MOIST_AC_THREADS;
+// tallest column the kernel takes: its per-level geometry (40 bytes a level)
+// is staged in dynamic LDS within the 64 KiB a launch gets without opt-in
+constexpr int AC_MAX_NZ = 1600;
 
 template <class S> struct AcBuf {
     const S *rup_in;   // rho_u' before this substep
@@ -55,6 +58,7 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
     using R = S;
     __shared__ double lds[6 * NT];  // group aggregates of the cross-chunk scans
     __shared__ S ldt[2 * NT];
+    extern __shared__ double lgeo[];  // 5 doubles per level of the tallest member (dynamic)
 
     const int4 tl = atiles[blockIdx.x];
     if (tl.x < 0) return;  // padding tile of the XCD ordering (uniform per block)
@@ -87,12 +91,34 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
     double cin_v = 0.0;
     if (!first && nl > 0) cin_v = ab.cin[cidx];
 
+    // Loads of the later phases (after the first barrier) are issued early:
+    // the per-level geometry goes to LDS, the few per-cell values to
+    // registers, so no global load waits behind a barrier or a store.
     R rp_e[L], rtp_e[L], drw[L], thx[L + 4], rt2c[L + 1];
+    R rwv[L + 1];  // rho_w' at faces k0 .. k0+L
+    R swv[L + 1];  // input rho_w at levels k0-1 .. k0+L-1 (upwind branch tests)
+    R mzv[L];      // mom_z at faces k0 .. k0+L-1
     double dd[L], cpv[L], pv[L];
 #pragma unroll
     for (int q = 0; q < L + 4; ++q) thx[q] = R(0.0);
 #pragma unroll
     for (int q = 0; q < L + 1; ++q) rt2c[q] = R(0.0);
+    {
+        const int ic = col_ok ? i : 0;
+        auto at = [&](int k) -> int64_t { return base + (int64_t)min(max(k, 0), ke) * nx + ic; };
+#pragma unroll
+        for (int q = 0; q < L + 1; ++q) {
+            const int j = k0 + q, kq = k0 - 1 + q;
+            const R rv = w.rwp[at(j)], sv = in.f[RHOW][at(kq)];
+            rwv[q] = (nl > 0 && j <= nz - 2) ? rv : R(0.0);
+            swv[q] = (nl > 0 && kq >= 0 && kq < nz) ? sv : R(0.0);
+        }
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const R mv = w.mom_z[at(k0 + l)];
+            mzv[l] = l < nf ? mv : R(0.0);
+        }
+    }
 
     // ---- phase 1: rho_u', explicit rho' and rho_theta', d rho_w'
     R tp_r0 = R(0.0), tp_t0 = R(0.0), tp_rp = R(0.0), tp_tp = R(0.0);
@@ -105,6 +131,7 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
             const int k = k0 + l;
             const int64_t row = base + (int64_t)k * nx;
             const int64_t cc = row + i, cm = row + im, cx = row + ip;
+            const R mx_in = w.mom_x[cc];
             const R cur_c = ab.rtp_cur[cc], cur_m = ab.rtp_cur[cm], cur_p = ab.rtp_cur[cx];
             const R old_c = ab.rtp_old[cc], old_m = ab.rtp_old[cm], old_p = ab.rtp_old[cx];
             const R rt2_c = w.rt2[cc], rt2_m = w.rt2[cm], rt2_p = w.rt2[cx];
@@ -117,8 +144,6 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
             R gmo = fdiv(old_c - old_m, dx);
             gm = gm + gam * (gm - gmo);
             const R ru_m = ab.rup_in[cm] + dt_s * (w.R_rhou[cm] - (rt2_m + rt2_c) * R(0.5) * gm);
-            ab.rup_out[cc] = ru_i;
-            w.mom_x[cc] = w.mom_x[cc] + fdiv(ru_i, R(nsub));
             // explicit rho' (moist.jl:1383-1386)
             R adv = fdiv(-(ru_i - ru_m), dx);
             R inc = dt_s * (w.R_rho[cc] + adv);
@@ -151,9 +176,13 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
             }
             tp_rp = rp_tp;
             tp_tp = rtp_tp;
+            // rho_u' and its running mean mom_x (moist.jl:1453-1455)
+            ab.rup_out[cc] = ru_i;
+            w.mom_x[cc] = mx_in + fdiv(ru_i, R(nsub));
         }
     }
     // theta two levels below and above the chunk, rt2pres one level above
+    R rrw_top = R(0.0);  // R_rho_w at the face below level k0+L
     if (nl > 0) {
         const int64_t ci = base + i;
         if (k0 >= 2) thx[0] = w.theta[ci + (int64_t)(k0 - 2) * nx];
@@ -162,22 +191,35 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
             if (k0 + L < nz) {
                 thx[L + 2] = w.theta[ci + (int64_t)(k0 + L) * nx];
                 rt2c[L] = w.rt2[ci + (int64_t)(k0 + L) * nx];
+                rrw_top = w.R_rhow[ci + (int64_t)(k0 + L - 1) * nx];
             }
             if (k0 + L + 1 < nz) thx[L + 3] = w.theta[ci + (int64_t)(k0 + L + 1) * nx];
         }
     }
 
+    // geometry for the phases after the barrier, per level: z, dz, 1/dz, 1/(z_k+1 - z_k), g/(dz_k + dz_k+1)
+    for (int q = t; q < nz; q += NT) {
+        const double4 g = zv.geo[D.offz + q];
+        lgeo[5 * q + 0] = zd[q];
+        lgeo[5 * q + 1] = dzd[q];
+        lgeo[5 * q + 2] = g.x;
+        lgeo[5 * q + 3] = g.y;
+        lgeo[5 * q + 4] = g.w;
+    }
     // ---- LDS round A: off-centred values of the first level of each chunk
+    // (and the geometry staged above)
     ldt[t] = tp_r0;
     ldt[NT + t] = tp_t0;
     __syncthreads();
+    auto LZ = [&](int k) { return lgeo[5 * k]; };
+    auto LDZ = [&](int k) { return lgeo[5 * k + 1]; };
     if (nl == L && k0 + L < nz) {
         const int k = k0 + L;
         const R up_r = ldt[t + TX], up_t = ldt[NT + t + TX];
-        const R buoy = fdiv(up_t - tp_tp, R(zd[k]) - R(zd[k - 1]));
+        const R buoy = fdiv(up_t - tp_tp, R(LZ(k)) - R(LZ(k - 1)));
         const R grav = -K<R>::GRAV * R(0.5) * (up_r + tp_rp);
         const R rt2h = (rt2c[L - 1] + rt2c[L]) * R(0.5);
-        drw[L - 1] = dt_s * (w.R_rhow[base + (int64_t)(k - 1) * nx + i] - rt2h * buoy + grav);
+        drw[L - 1] = dt_s * (rrw_top - rt2h * buoy + grav);
     }
 
     // theta at the faces k0-1 .. k0+L (f2h in T, moist.jl:982-985): thh[q] is face k0-1+q
@@ -188,20 +230,19 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
         thh[q] = 0.0;
         // faces up to nz-2 exist; their theta levels are all in thx (a partial
         // chunk is the top chunk, so it never needs levels above its own)
-        if (nl > 0 && j >= 0 && j <= nz - 2) thh[q] = double(f2h(thx[q + 1], thx[q + 2], R(dzd[j]), R(dzd[j + 1])));
+        if (nl > 0 && j >= 0 && j <= nz - 2) thh[q] = double(f2h(thx[q + 1], thx[q + 2], R(LDZ(j)), R(LDZ(j + 1))));
     }
-    const double4 *geo = zv.geo + D.offz;
     // tridiagonal coefficients of face j = k0 + l before the off-centering
     // update b += 1 (moist.jl:997-1006), with the grid factors 1/dz, 1/(z_j+1 - z_j)
-    // and g/(dz_j + dz_j+1) taken from the per-level table
+    // and g/(dz_j + dz_j+1) from the staged per-level table
     auto coef = [&](int l, double &a, double &b, double &cc) {
         const int j = k0 + l;
-        const double4 gj = geo[j];
-        const double rdz1 = geo[j + 1].x;
+        const double gx = lgeo[5 * j + 2], gy = lgeo[5 * j + 3], gw = lgeo[5 * j + 4];
+        const double rdz1 = lgeo[5 * (j + 1) + 2];
         const double r2j = double(rt2c[l]), r2k = double(rt2c[l + 1]);
-        b = dt_tp2 * (r2j * gj.x + r2k * rdz1) * thh[l + 1] * gj.y;
-        a = j > 0 ? -dt_tp2 * (r2j * thh[l] * (gj.y * gj.x) - gj.w) : 0.0;
-        cc = j < nz - 2 ? -dt_tp2 * (r2k * thh[l + 2] * (gj.y * rdz1) + gj.w) : 0.0;
+        b = dt_tp2 * (r2j * gx + r2k * rdz1) * thh[l + 1] * gy;
+        a = j > 0 ? -dt_tp2 * (r2j * thh[l] * (gy * gx) - gw) : 0.0;
+        cc = j < nz - 2 ? -dt_tp2 * (r2k * thh[l + 2] * (gy * rdz1) + gw) : 0.0;
     };
 
     // ---- right-hand sides (moist.jl:1008-1021)
@@ -212,11 +253,11 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
             const int j = k0 + l;
             double a, b, cc;
             coef(l, a, b, cc);
-            const double rwj = double(w.rwp[base + (int64_t)j * nx + i]);
+            const double rwj = double(rwv[l]);
             double d = rwj + double(drw[l]);
             if (j > 0) d -= a * fact * rwj;
             d -= b * fact * rwj;
-            if (j < nz - 2) d -= cc * fact * double(w.rwp[base + (int64_t)(j + 1) * nx + i]);
+            if (j < nz - 2) d -= cc * fact * double(rwv[l + 1]);
             dd[l] = d;
         }
     }
@@ -427,7 +468,7 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
             const int64_t cj = base + (int64_t)(k0 + l) * nx + i;
             xr[l] = R(x);
             w.rwp[cj] = xr[l];
-            w.mom_z[cj] = w.mom_z[cj] + fdiv(xr[l], R(nsub));
+            w.mom_z[cj] = mzv[l] + fdiv(xr[l], R(nsub));
         }
     }
     const R xb = R(x_below);
@@ -440,7 +481,7 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
             const int64_t cc = base + (int64_t)k * nx + i;
             const R wkp = k == ke ? R(0.0) : xr[l];
             const R wkm = k == 0 ? R(0.0) : (l > 0 ? xr[l > 0 ? l - 1 : 0] : xb);
-            const R dzk = R(dzd[k]);
+            const R dzk = R(LDZ(k));
             R adv = fdiv(-(wkp - wkm), dzk);
             w.rp[cc] = rp_e[l] + dt_s * adv;
             const R t_m2 = thx[l], t_m1 = thx[l + 1], t_0 = thx[l + 2], t_p1 = thx[l + 3], t_p2 = thx[l + 4];
@@ -448,15 +489,15 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
             if (k == ke) Fkp = R(0.0);
             else {
                 if (wkp > EPS && k > 0) Fkp = fdiv(wkp * upw3(t_m1, t_0, t_p1), R(6.0));
-                else if (in.f[RHOW][cc] < -EPS && k < ke - 1) Fkp = fdiv(wkp * upw3(t_p2, t_p1, t_0), R(6.0));
-                else Fkp = wkp * f2h(t_0, t_p1, dzk, R(dzd[k + 1]));
+                else if (swv[l + 1] < -EPS && k < ke - 1) Fkp = fdiv(wkp * upw3(t_p2, t_p1, t_0), R(6.0));
+                else Fkp = wkp * f2h(t_0, t_p1, dzk, R(LDZ(k + 1)));
             }
             if (k == 0) Fkm = R(0.0);
             else {
-                const R sw = in.f[RHOW][cc - nx];
+                const R sw = swv[l];
                 if (sw > EPS && k > 1) Fkm = fdiv(wkm * upw3(t_m2, t_m1, t_0), R(6.0));
                 else if (sw < -EPS && k < ke) Fkm = fdiv(wkm * upw3(t_p1, t_0, t_m1), R(6.0));
-                else Fkm = wkm * f2h(t_m1, t_0, R(dzd[k - 1]), dzk);
+                else Fkm = wkm * f2h(t_m1, t_0, R(LDZ(k - 1)), dzk);
             }
             adv = fdiv(-(Fkp - Fkm), dzk);
             ab.rtp_out[cc] = rtp_e[l] + dt_s * adv;

@@ -232,7 +232,7 @@ int make_atiles(const std::vector<Dom> &doms, int L, int NT, std::vector<int4> &
         const Dom &D = doms[m];
         int nch = 1;
         while (nch * L < D.nz) nch *= 2;
-        if (nch > NT) return -1;
+        if (nch > NT || D.nz > AC_MAX_NZ) return -1;
         const int TX = NT / nch;
         nch_max = std::max(nch_max, nch);
         for (int i0 = 0; i0 < D.nx; i0 += TX) out.push_back(make_int4((int)m, i0, TX, nch));
@@ -270,8 +270,9 @@ int set_acoustic_shape(moist_handle *h, int L, int NT, int W = 1) {
 template <class S, int L, int NT, int W>
 void launch_ac(moist_handle *h, const ZVec &zv, const SSet<S> &in, const Work<S> &w, const AcBuf<S> &ab, int div,
                double dt_ovr, int nsub, int first) {
-    hipLaunchKernelGGL((k_acoustic<S, L, NT, W>), dim3(h->natiles), dim3(NT), 0, h->stream, h->dom_d, h->atiles, zv,
-                       in, w, ab, div, dt_ovr, nsub, first);
+    // dynamic LDS: the per-level geometry (5 doubles per level of the tallest member)
+    hipLaunchKernelGGL((k_acoustic<S, L, NT, W>), dim3(h->natiles), dim3(NT), (unsigned)(40 * h->max_ka), h->stream,
+                       h->dom_d, h->atiles, zv, in, w, ab, div, dt_ovr, nsub, first);
 }
 
 template <class S>
