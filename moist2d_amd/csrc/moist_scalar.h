@@ -39,36 +39,117 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
     const R dtL = R(D.dt);  // the positivity limiters use the large step (moist.jl:917,930)
     auto gidx = [&](int k, int i) -> int64_t { return D.off2 + (int64_t)k * nx + i; };
 
-    // rho of the haloed tile (full density), once for the three scalars
-    for (int e = tid; e < ST_HZ * ST_HX; e += ST_THREADS) {
+    // Work items of the strided loops, fixed per thread: staged cells of the
+    // haloed tile, x faces, z faces and interior cells.  Their global indices
+    // and the face mass fluxes (the same for the three scalars) are set up
+    // once, so every loop issues its loads together.
+    constexpr int NSTG = (ST_HZ * ST_HX + ST_THREADS - 1) / ST_THREADS;
+    constexpr int NFX = (ST_Z * (ST_X + 1) + ST_THREADS - 1) / ST_THREADS;
+    constexpr int NFZ = ((ST_Z + 1) * ST_X + ST_THREADS - 1) / ST_THREADS;
+    constexpr int NCL = (ST_Z * ST_X + ST_THREADS - 1) / ST_THREADS;
+    int64_t gst[NSTG];  // staged cell (-1: outside the domain's levels or the tile)
+#pragma unroll
+    for (int u = 0; u < NSTG; ++u) {
+        const int e = tid + u * ST_THREADS;
         const int r = e / ST_HX, cidx = e - r * ST_HX;
         const int k = k0 - 2 + r, i = wrapx(i0 - 2 + cidx, nx);
-        R v = R(0.0);
-        if (k >= 0 && k < nz) v = full_rho(in, gidx(k, i), zv, D, k);
-        l_rho[r][cidx] = v;
+        gst[u] = (e < ST_HZ * ST_HX && k >= 0 && k < nz) ? gidx(k, i) : -1;
+    }
+    R mfx[NFX], mfz[NFZ];  // mass flux at the x and z faces of this thread
+    R fdz0[NFZ], fdz1[NFZ], fzz[NFZ], cdz[NCL];  // dz below / above each z face, z spacing, dz of each cell
+#pragma unroll
+    for (int u = 0; u < NFX; ++u) {
+        const int e = tid + u * ST_THREADS;
+        const int r = e / (ST_X + 1), f = e - r * (ST_X + 1);
+        const int k = k0 + r;
+        mfx[u] = (e < ST_Z * (ST_X + 1) && k < nz) ? w.mom_x[gidx(k, wrapx(i0 + f - 1, nx))] : R(0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < NFZ; ++u) {
+        const int e = tid + u * ST_THREADS;
+        const int g = e / ST_X, c = e - g * ST_X;
+        const int kk = k0 + g - 1, i = i0 + c;
+        const bool ok = e < (ST_Z + 1) * ST_X && i < nx && kk >= 0 && kk <= ke - 1;
+        mfz[u] = ok ? w.mom_z[gidx(kk, i)] : R(0.0);
+        const int kc = ok ? kk : 0, kc1 = ok ? kk + 1 : 0;
+        fdz0[u] = R(dzd[kc]);
+        fdz1[u] = R(dzd[kc1]);
+        fzz[u] = R(zd[kc1]) - R(zd[kc]);
+    }
+#pragma unroll
+    for (int u = 0; u < NCL; ++u) {
+        const int e = tid + u * ST_THREADS;
+        const int k = k0 + e / ST_X;
+        cdz[u] = R(dzd[min(k, ke)]);
+    }
+
+    // rho of the haloed tile (full density), once for the three scalars
+    {
+        R v[NSTG];
+#pragma unroll
+        for (int u = 0; u < NSTG; ++u) {
+            const int e = tid + u * ST_THREADS;
+            const int k = k0 - 2 + e / ST_HX;
+            v[u] = gst[u] >= 0 ? full_rho(in, gst[u], zv, D, k) : R(0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < NSTG; ++u) {
+            const int e = tid + u * ST_THREADS;
+            if (e < ST_HZ * ST_HX) (&l_rho[0][0])[e] = v[u];
+        }
+    }
+
+    // the final update's inputs (read-only here), loaded with the rest
+    R fin[NCL][4];
+#pragma unroll
+    for (int u = 0; u < NCL; ++u) {
+        const int e = tid + u * ST_THREADS;
+        const int r = e / ST_X, c = e - r * ST_X;
+        const int k = min(k0 + r, ke), i = min(i0 + c, nx - 1);
+        const int64_t q = gidx(k, i);
+        fin[u][0] = in.f[RHO][q] + w.rp[q];
+        fin[u][1] = in.f[RHOU][q] + rup_fin[q];
+        fin[u][2] = in.f[RT][q] + rtp_fin[q];
+        fin[u][3] = k < ke ? in.f[RHOW][q] + w.rwp[q] : in.f[RHOW][q];
     }
 
     const S *dph[3] = {w.d_qv, w.d_qc, w.d_qr};
     int nan_f = 0;
     for (int sc = 0; sc < 3; ++sc) {
         const S *qraw = in.f[QV + sc];
+        // this scalar's global values: staged cells, limiter inputs, and the
+        // s0 value and tendency of the interior cells
+        R qv_[NSTG];
+#pragma unroll
+        for (int u = 0; u < NSTG; ++u) qv_[u] = gst[u] >= 0 ? R(qraw[gst[u]]) : R(0.0);
+        const R q_lo = R(qraw[gidx(0, D.lin_col)]), q_hi = R(qraw[gidx(ke, D.lin_col)]);
+        R s0v[NCL], dpv[NCL];
+#pragma unroll
+        for (int u = 0; u < NCL; ++u) {
+            const int e = tid + u * ST_THREADS;
+            const int r = e / ST_X, c = e - r * ST_X;
+            const int k = k0 + r, i = i0 + c;
+            const bool ok = e < ST_Z * ST_X && k < nz && i < nx;
+            s0v[u] = ok ? s0.f[QV + sc][gidx(k, i)] : R(0.0);
+            dpv[u] = ok ? dph[sc][gidx(k, i)] : R(0.0);
+        }
         __syncthreads();  // l_rho ready; previous scalar's fluxes consumed
-        for (int e = tid; e < ST_HZ * ST_HX; e += ST_THREADS) {
-            const int r = e / ST_HX, cidx = e - r * ST_HX;
-            const int k = k0 - 2 + r, i = wrapx(i0 - 2 + cidx, nx);
-            R v = R(0.0);
-            if (k >= 0 && k < nz) v = fdiv(qraw[gidx(k, i)], l_rho[r][cidx]);
-            l_q[r][cidx] = v;
+#pragma unroll
+        for (int u = 0; u < NSTG; ++u) {
+            const int e = tid + u * ST_THREADS;
+            if (e < ST_HZ * ST_HX) (&l_q[0][0])[e] = gst[u] >= 0 ? fdiv(qv_[u], (&l_rho[0][0])[e]) : R(0.0);
         }
         __syncthreads();
         // x faces: face f lies between cells i0+f-1 and i0+f (compute_limited_flux_x at ii = i0+f-1)
-        for (int e = tid; e < ST_Z * (ST_X + 1); e += ST_THREADS) {
+#pragma unroll
+        for (int u = 0; u < NFX; ++u) {
+            const int e = tid + u * ST_THREADS;
+            if (e >= ST_Z * (ST_X + 1)) continue;
             const int r = e / (ST_X + 1), f = e - r * (ST_X + 1);
             const int k = k0 + r;
             R fx = R(0.0), dfx = R(0.0);
             if (k < nz) {
-                const int ii = wrapx(i0 + f - 1, nx);
-                const R rhou_face = w.mom_x[gidx(k, ii)];
+                const R rhou_face = mfx[u];
                 const int lr = r + 2, lc = f + 1;  // LDS column of cell ii
                 const R s_i = l_q[lr][lc], s_ip1 = l_q[lr][lc + 1];
                 R s_face;
@@ -93,21 +174,24 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
             l_dx[r][f] = dfx;
         }
         // z faces: face g lies between levels kk = k0+g-1 and kk+1 (compute_limited_flux_z at kk)
-        for (int e = tid; e < (ST_Z + 1) * ST_X; e += ST_THREADS) {
+#pragma unroll
+        for (int u = 0; u < NFZ; ++u) {
+            const int e = tid + u * ST_THREADS;
+            if (e >= (ST_Z + 1) * ST_X) continue;
             const int g = e / ST_X, c = e - g * ST_X;
             const int kk = k0 + g - 1, i = i0 + c;
             R fz = R(0.0), dfz = R(0.0);
             if (i < nx && kk >= 0 && kk <= ke - 1) {
-                const R rhow_face = w.mom_z[gidx(kk, i)];
+                const R rhow_face = mfz[u];
                 const int lr = g + 1, lc = c + 2;  // LDS row of level kk
                 const R s_k = l_q[lr][lc], s_kp1 = l_q[lr + 1][lc];
-                const R dzk = R(dzd[kk]), dzk1 = R(dzd[kk + 1]);
+                const R dzk = fdz0[u], dzk1 = fdz1[u];
                 R s_face;
                 if (rhow_face > EPS) {
                     if (kk == 0) {
                         s_face = f2h(s_k, s_kp1, dzk, dzk1);
                         // linear index into rho_q: row kk of the first halo column
-                        s_face = jmin(s_face, fdiv(R(qraw[gidx(kk, D.lin_col)]), rhow_face * dtL));
+                        s_face = jmin(s_face, fdiv(q_lo, rhow_face * dtL));
                     } else {
                         const R s_km1 = l_q[lr - 1][lc];
                         const R r_num = s_kp1 - s_k, r_den = s_k - s_km1;
@@ -117,7 +201,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
                 } else if (rhow_face < -EPS) {
                     if (kk == ke - 1) {
                         s_face = f2h(s_k, s_kp1, dzk, dzk1);
-                        s_face = jmin(s_face, fdiv(R(qraw[gidx(kk + 1, D.lin_col)]), -rhow_face * dtL));
+                        s_face = jmin(s_face, fdiv(q_hi, -rhow_face * dtL));
                     } else {
                         const R s_kp2 = l_q[lr + 2][lc];
                         const R r_num = s_k - s_kp1, r_den = s_kp1 - s_kp2;
@@ -129,41 +213,42 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
                 }
                 fz = s_face * rhow_face;
                 // rho * dq/dz at the face (moist.jl:1507-1508)
-                dfz = fdiv(f2h(l_rho[lr + 1][lc], l_rho[lr][lc], dzk1, dzk) * (s_kp1 - s_k),
-                           R(zd[kk + 1]) - R(zd[kk]));
+                dfz = fdiv(f2h(l_rho[lr + 1][lc], l_rho[lr][lc], dzk1, dzk) * (s_kp1 - s_k), fzz[u]);
             }
             l_fz[g][c] = fz;
             l_dz[g][c] = dfz;
         }
         __syncthreads();
         // cells (moist.jl:1480-1516)
-        for (int e = tid; e < ST_Z * ST_X; e += ST_THREADS) {
+#pragma unroll
+        for (int u = 0; u < NCL; ++u) {
+            const int e = tid + u * ST_THREADS;
+            if (e >= ST_Z * ST_X) continue;
             const int r = e / ST_X, c = e - r * ST_X;
             const int k = k0 + r, i = i0 + c;
             if (k >= nz || i >= nx) continue;
             const int64_t q = gidx(k, i);
-            const R dzk = R(dzd[k]);
+            const R dzk = cdz[u];
             const R adv_x = fdiv(l_fx[r][c + 1] - l_fx[r][c], dx);
             const R adv_z = fdiv(l_fz[r + 1][c] - l_fz[r][c], dzk);
             const R adv = -(adv_x + adv_z);
             const R diff_x = fdiv(kap * (l_dx[r][c + 1] - l_dx[r][c]), dx);
             const R diff_z = fdiv(kap * (l_dz[r + 1][c] - l_dz[r][c]), dzk);
-            const R v = s0.f[QV + sc][q] + dt * (adv + (diff_x + diff_z) + dph[sc][q]);
+            const R v = s0v[u] + dt * (adv + (diff_x + diff_z) + dpv[u]);
             out.f[QV + sc][q] = v;
             if (isnan(v) && !nan_f) nan_f = 5 + sc;
         }
     }
 
     // final update of rho, rho_u, rho_theta, rho_w (moist.jl:1522-1539) and NaN checks
-    for (int e = tid; e < ST_Z * ST_X; e += ST_THREADS) {
+#pragma unroll
+    for (int u = 0; u < NCL; ++u) {
+        const int e = tid + u * ST_THREADS;
         const int r = e / ST_X, c = e - r * ST_X;
         const int k = k0 + r, i = i0 + c;
-        if (k >= nz || i >= nx) continue;
+        if (e >= ST_Z * ST_X || k >= nz || i >= nx) continue;
         const int64_t q = gidx(k, i);
-        const R vr = in.f[RHO][q] + w.rp[q];
-        const R vu = in.f[RHOU][q] + rup_fin[q];
-        const R vt = in.f[RT][q] + rtp_fin[q];
-        const R vw = k < ke ? in.f[RHOW][q] + w.rwp[q] : in.f[RHOW][q];
+        const R vr = fin[u][0], vu = fin[u][1], vt = fin[u][2], vw = fin[u][3];
         out.f[RHO][q] = vr;
         out.f[RHOU][q] = vu;
         out.f[RT][q] = vt;

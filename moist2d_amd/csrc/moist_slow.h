@@ -27,6 +27,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
     __shared__ R lU[SL_HZ][SL_HX], lW[SL_HZ][SL_HX], lT[SL_HZ][SL_HX];
     __shared__ R lR[SL_HZ][SL_HX], lRU[SL_HZ][SL_HX], lRW[SL_HZ][SL_HX];
     __shared__ R lF[SL_FLUX];
+    __shared__ R lDZ[SL_HZ], lZ[SL_HZ];  // dz and z of levels k0-2 .. k0+ST_Z+1
 
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;  // padding tile of the XCD ordering (uniform per block)
@@ -37,8 +38,8 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
     const double *zd = zv.z + D.offz, *zfd = zv.zf + D.offz, *dzd = zv.dz + D.offz;
     const R dx = R(D.dx), EPS = KC::EPS, nu = R(D.nu);
     auto gidx = [&](int k, int i) -> int64_t { return D.off2 + (int64_t)k * nx + i; };
-    auto DZ = [&](int k) -> R { return R(dzd[k]); };
-    auto Z = [&](int k) -> R { return R(zd[k]); };
+    auto DZ = [&](int k) -> R { return lDZ[k - k0 + 2]; };
+    auto Z = [&](int k) -> R { return lZ[k - k0 + 2]; };
     // LDS accessors in absolute (k, i) of the tile neighbourhood
     auto U = [&](int k, int i) -> R { return lU[k - k0 + 2][i - i0 + 2]; };
     auto W = [&](int k, int i) -> R { return lW[k - k0 + 2][i - i0 + 2]; };
@@ -47,25 +48,69 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
     auto RU = [&](int k, int i) -> R { return lRU[k - k0 + 2][i - i0 + 2]; };
     auto RW = [&](int k, int i) -> R { return lRW[k - k0 + 2][i - i0 + 2]; };
 
-    for (int e = tid; e < SL_HZ * SL_HX; e += ST_THREADS) {
-        const int r = e / SL_HX, c = e - r * SL_HX;
-        const int k = k0 - 2 + r, i = wrapx(i0 - 2 + c, nx);
-        R u = R(0), ww = R(0), th = R(0), rho = R(0), ru = R(0), rw = R(0);
-        if (k >= 0 && k < nz) {
-            const int64_t q = gidx(k, i);
-            u = w.uf[q];
-            ww = w.wf[q];
-            th = w.theta[q];
-            rho = full_rho(in, q, zv, D, k);
-            ru = in.f[RHOU][q];
-            rw = in.f[RHOW][q];
+    // Staged tile and every global value the cell phases read are loaded
+    // up front (fixed trip counts), so the loads of a thread are in flight
+    // together instead of one loop iteration at a time.
+    constexpr int NSL = (SL_HZ * SL_HX + ST_THREADS - 1) / ST_THREADS;
+    constexpr int NCL = (ST_Z * ST_X + ST_THREADS - 1) / ST_THREADS;
+    {
+        R v[NSL][6];
+#pragma unroll
+        for (int u = 0; u < NSL; ++u) {
+            const int e = tid + u * ST_THREADS;
+            const int r = e / SL_HX, c = e - r * SL_HX;
+            const int k = k0 - 2 + r, i = wrapx(i0 - 2 + c, nx);
+            const bool ok = e < SL_HZ * SL_HX && k >= 0 && k < nz;
+            const int64_t q = gidx(ok ? k : 0, ok ? i : 0);
+            v[u][0] = ok ? w.uf[q] : R(0);
+            v[u][1] = ok ? w.wf[q] : R(0);
+            v[u][2] = ok ? w.theta[q] : R(0);
+            v[u][3] = ok ? full_rho(in, q, zv, D, k) : R(0);
+            v[u][4] = ok ? in.f[RHOU][q] : R(0);
+            v[u][5] = ok ? in.f[RHOW][q] : R(0);
         }
-        lU[r][c] = u;
-        lW[r][c] = ww;
-        lT[r][c] = th;
-        lR[r][c] = rho;
-        lRU[r][c] = ru;
-        lRW[r][c] = rw;
+        if (tid < SL_HZ) {
+            const int k = k0 - 2 + tid;
+            const bool ok = k >= 0 && k < nz;
+            lDZ[tid] = ok ? R(dzd[k]) : R(1);
+            lZ[tid] = ok ? R(zd[k]) : R(0);
+        }
+#pragma unroll
+        for (int u = 0; u < NSL; ++u) {
+            const int e = tid + u * ST_THREADS;
+            if (e >= SL_HZ * SL_HX) continue;
+            (&lU[0][0])[e] = v[u][0];
+            (&lW[0][0])[e] = v[u][1];
+            (&lT[0][0])[e] = v[u][2];
+            (&lR[0][0])[e] = v[u][3];
+            (&lRU[0][0])[e] = v[u][4];
+            (&lRW[0][0])[e] = v[u][5];
+        }
+    }
+    // per interior cell of this thread: the cell phases' global inputs
+    R c_drho[NCL], c_drt[NCL], c_rp[NCL], c_rup[NCL], c_rwp[NCL], c_rtp[NCL];
+    R c_pres[NCL], c_presx[NCL], c_presz[NCL], c_dup[NCL], c_rho[NCL], c_rhoz[NCL], c_zf[NCL], c_bb[NCL];
+#pragma unroll
+    for (int u = 0; u < NCL; ++u) {
+        const int e = tid + u * ST_THREADS;
+        const int r = e / ST_X, c = e - r * ST_X;
+        const int k = min(k0 + r, ke), i = min(i0 + c, nx - 1);
+        const int kz = min(k + 1, ke);
+        const int64_t q = gidx(k, i);
+        c_drho[u] = w.d_rho[q];
+        c_drt[u] = w.d_rt[q];
+        c_rp[u] = s0.f[RHO][q] - in.f[RHO][q];
+        c_rup[u] = s0.f[RHOU][q] - in.f[RHOU][q];
+        c_rwp[u] = s0.f[RHOW][q] - in.f[RHOW][q];
+        c_rtp[u] = s0.f[RT][q] - in.f[RT][q];
+        c_pres[u] = w.pres[q];
+        c_presx[u] = w.pres[gidx(k, wrapx(i + 1, nx))];
+        c_presz[u] = w.pres[gidx(kz, i)];
+        c_dup[u] = u_phys_2d ? w.d_u2d[q] : (k == 0 ? w.d_u_sf[D.offx + i] : R(0.0));
+        c_rho[u] = in.f[RHO][q];
+        c_rhoz[u] = in.f[RHO][gidx(kz, i)];
+        c_zf[u] = R(zfd[k]);
+        c_bb[u] = R(zv.B_bar[D.offz + k]);
     }
     __syncthreads();
 
@@ -97,24 +142,26 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         FTz[e] = fl;
     }
     __syncthreads();
-    for (int e = tid; e < ST_Z * ST_X; e += ST_THREADS) {
+#pragma unroll
+    for (int u = 0; u < NCL; ++u) {
+        const int e = tid + u * ST_THREADS;
         const int r = e / ST_X, c = e - r * ST_X;
         const int k = k0 + r, i = i0 + c;
-        if (k >= nz || i >= nx) continue;
+        if (e >= ST_Z * ST_X || k >= nz || i >= nx) continue;
         const int64_t q = gidx(k, i);
         // R_rho (moist.jl:1079-1086)
         const R fkp = k == ke ? R(0.0) : RW(k, i);
         const R fkm = k == 0 ? R(0.0) : RW(k - 1, i);
-        w.R_rho[q] = -(fdiv(RU(k, i) - RU(k, i - 1), dx) + fdiv(fkp - fkm, DZ(k))) + w.d_rho[q];
+        w.R_rho[q] = -(fdiv(RU(k, i) - RU(k, i - 1), dx) + fdiv(fkp - fkm, DZ(k))) + c_drho[u];
         // R_rho_theta
         const R adv = -(fdiv(FTx[r * (ST_X + 1) + c + 1] - FTx[r * (ST_X + 1) + c], dx) +
                         fdiv(FTz[(r + 1) * ST_X + c] - FTz[r * ST_X + c], DZ(k)));
-        w.R_rt[q] = adv + w.d_rt[q];
+        w.R_rt[q] = adv + c_drt[u];
         // perturbations from the step start, mass-flux accumulators (moist.jl:1339-1348)
-        w.rp[q] = s0.f[RHO][q] - in.f[RHO][q];
-        w.rup[q] = s0.f[RHOU][q] - in.f[RHOU][q];
-        w.rwp[q] = s0.f[RHOW][q] - in.f[RHOW][q];
-        rtp0[q] = s0.f[RT][q] - in.f[RT][q];
+        w.rp[q] = c_rp[u];
+        w.rup[q] = c_rup[u];
+        w.rwp[q] = c_rwp[u];
+        rtp0[q] = c_rtp[u];
         w.mom_x[q] = RU(k, i);
         w.mom_z[q] = RW(k, i);
     }
@@ -156,10 +203,12 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         GUz[e] = gz;
     }
     __syncthreads();
-    for (int e = tid; e < ST_Z * ST_X; e += ST_THREADS) {
+#pragma unroll
+    for (int u = 0; u < NCL; ++u) {
+        const int e = tid + u * ST_THREADS;
         const int r = e / ST_X, c = e - r * ST_X;
         const int k = k0 + r, i = i0 + c;
-        if (k >= nz || i >= nx) continue;
+        if (e >= ST_Z * ST_X || k >= nz || i >= nx) continue;
         const int64_t q = gidx(k, i);
         const int x0 = r * (ST_X + 1) + c;
         const R druudx = fdiv(CUx[x0 + 1] - CUx[x0], dx);
@@ -168,10 +217,8 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         const R u_xx = fdiv(GUx[x0 + 1] - GUx[x0], dx);
         const R u_zz = fdiv(GUz[(r + 1) * ST_X + c] - GUz[r * ST_X + c], DZ(k));
         const R diff = nu * (u_xx + u_zz);
-        const int ip = wrapx(i + 1, nx);
-        const R dpdx = fdiv(w.pres[gidx(k, ip)] - w.pres[q], dx);
-        const R dup = u_phys_2d ? w.d_u2d[q] : (k == 0 ? w.d_u_sf[D.offx + i] : R(0.0));
-        w.R_rhou[q] = adv - dpdx + diff + dup;
+        const R dpdx = fdiv(c_presx[u] - c_pres[u], dx);
+        w.R_rhou[q] = adv - dpdx + diff + c_dup[u];
     }
     __syncthreads();
 
@@ -222,10 +269,12 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
     }
     __syncthreads();
     const R zf_sp = R(D.zf_sponge), H = R(D.H);
-    for (int e = tid; e < ST_Z * ST_X; e += ST_THREADS) {
+#pragma unroll
+    for (int u = 0; u < NCL; ++u) {
+        const int e = tid + u * ST_THREADS;
         const int r = e / ST_X, c = e - r * ST_X;
         const int k = k0 + r, i = i0 + c;
-        if (k >= nz || i >= nx) continue;
+        if (e >= ST_Z * ST_X || k >= nz || i >= nx) continue;
         const int64_t q = gidx(k, i);
         if (k == ke) {
             w.R_rhow[q] = R(0.0);
@@ -240,15 +289,15 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         const R w_zz = fdiv(VZ[(r + 1) * ST_X + c] - VZ[r * ST_X + c], zz);
         const R diff = nu * (w_xx + w_zz);
         R sigma = R(0.0);
-        if (R(zfd[k]) > (R(1.0) - zf_sp) * H)
-            sigma = (KC::GRAV / R(30.0)) * ((R(zfd[k]) - (R(1.0) - zf_sp) * H) / (zf_sp * H));
+        if (c_zf[u] > (R(1.0) - zf_sp) * H)
+            sigma = (KC::GRAV / R(30.0)) * ((c_zf[u] - (R(1.0) - zf_sp) * H) / (zf_sp * H));
         const R sponge = -sigma * RW(k, i);
-        const R dpdz = fdiv(w.pres[q + nx] - w.pres[q], zz);
+        const R dpdz = fdiv(c_presz[u] - c_pres[u], zz);
         R res;
         if constexpr (is_mixed<S>()) {
             // -dp/dz + buoyancy = B_bar - dp'/dz - g (rho'_k + rho'_k+1)/2
-            const R buoy = -KC::GRAV * (in.f[RHO][q] + in.f[RHO][q + nx]) * R(0.5);
-            res = adv + ((R(zv.B_bar[D.offz + k]) - dpdz) + buoy) + diff + sponge;
+            const R buoy = -KC::GRAV * (c_rho[u] + c_rhoz[u]) * R(0.5);
+            res = adv + ((c_bb[u] - dpdz) + buoy) + diff + sponge;
         } else {
             const R buoy = -KC::GRAV * (RHL(k, i) + RHL(k + 1, i)) * R(0.5);
             res = adv - dpdz + buoy + diff + sponge;
