@@ -358,7 +358,9 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     const int64_t c = G(k, i);
     const double *rho = s.f[RHO];
     const double rrho = 1.0 / rho[c];
-    double lrho = lo.f[RHO][c];  // final update: rho_out = rho + rho'
+    // the final update's adjoints, read before any store
+    const double lo_r = lo.f[RHO][c], lo_u = lo.f[RHOU][c], lo_t = lo.f[RT][c], lo_w = lo.f[RHOW][c];
+    double lrho = lo_r;  // final update: rho_out = rho + rho'
     double lmx = 0.0, lmz = 0.0;
     // ---- the three scalars
 #pragma unroll 1
@@ -369,8 +371,8 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
         auto Q = [&](int kk, int ii) { return qa[G(kk, ii)]; };
         auto LFx = [&](int j) { return dt * rdx * (L[G(k, j + 1)] - L[G(k, j)]); };
         auto LFz = [&](int kk) { return dt * (L[G(kk + 1, i)] * gz[kk + 1].x - L[G(kk, i)] * gz[kk].x); };
-        ls0.f[QV + sc][c] += L[c];
-        lph.f[3 + sc][c] += dt * L[c];
+        // this scalar's read-modify-write values; the stores follow all its loads
+        const double Lc = L[c], s0v = ls0.f[QV + sc][c], phv = lph.f[3 + sc][c];
         double Lq = 0.0;
         // x faces j+1/2, j = i-2..i+1, read q(j-1..j+2); cell i is q slot i-j+1.
         // Face i+1/2 (o = 2) is this cell's own: its value is d flux / d mom_x.
@@ -423,7 +425,6 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
             lrho += lD * gq * (kk == k - 1 ? dzd[kk] : dzd[kk + 1]) * rden;
         }
         const double Lqr = Lq * rrho;
-        ls.f[QV + sc][c] = Lqr;
         lrho -= Lqr * qa[c];
         // this cell owns face i+1/2 (mom_x) and face k+1/2 (mom_z)
         lmx += LFx(i) * s_own_x;
@@ -434,6 +435,9 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
             if (k == 0) a.linb[D.offx * 3 + (int64_t)sc * nx + i] = lf * clo_own;
             if (k == ke - 1) a.lint[D.offx * 3 + (int64_t)sc * nx + i] = lf * chi_own;
         }
+        ls.f[QV + sc][c] = Lqr;
+        ls0.f[QV + sc][c] = s0v + Lc;
+        lph.f[3 + sc][c] = phv + dt * Lc;
     }
     a.Lmx[c] = lmx;
     a.Lmz[c] = lmz;
@@ -445,13 +449,13 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     }
     // final update (moist.jl:1522-1539): identity parts and the perturbation adjoints
     ls.f[RHO][c] = lrho;
-    ls.f[RHOU][c] = lo.f[RHOU][c];
-    ls.f[RT][c] = lo.f[RT][c];
-    ls.f[RHOW][c] = lo.f[RHOW][c];
-    a.Lrp[c] = lo.f[RHO][c];
-    Lrup[c] = lo.f[RHOU][c];
-    a.Lrtp[c] = lo.f[RT][c];
-    a.Lrwp[c] = k <= ke - 1 ? lo.f[RHOW][c] : 0.0;
+    ls.f[RHOU][c] = lo_u;
+    ls.f[RT][c] = lo_t;
+    ls.f[RHOW][c] = lo_w;
+    a.Lrp[c] = lo_r;
+    Lrup[c] = lo_u;
+    a.Lrtp[c] = lo_t;
+    a.Lrwp[c] = k <= ke - 1 ? lo_w : 0.0;
 }
 
 // Sum of the positivity-limiter adjoints of every column into rho_q at
@@ -540,6 +544,7 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
         pos = sw > EPS && f + 1 > 1;
         neg = !pos && sw < -EPS && f + 1 < ke;
     };
+    const double lth0 = a.Lth[G(k)], lrwp0 = k <= ke - 1 ? a.Lrwp[G(k)] : 0.0;
     double lth = 0.0;
     for (int f = k - 2; f <= k + 1; ++f) {
         if (f < 0 || f > ke - 1) continue;
@@ -552,7 +557,7 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
         brm(f, pos, neg);
         lth += dt_s * gz[f + 1].x * a.Lrtp[G(f + 1)] * w * upw_coef(pos, neg, k - f, w0, w1);
     }
-    a.Lth[G(k)] += lth;
+    double lrwp = 0.0;
     if (k <= ke - 1) {
         const double rden = gz[k].z;
         const double w0 = dzd[k + 1] * rden, w1 = dzd[k] * rden;
@@ -562,9 +567,12 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
         brm(k, pos, neg);
         const double gm = upw_val(pos, neg, THv(k - 1), THv(k), THv(k + 1), THv(k + 2), w0, w1);
         const int64_t q = G(k), qp = G(k + 1);
-        a.Lrwp[q] += a.Lmz[q] * rns - dt_s * gz[k].x * a.Lrtp[q] * gp + dt_s * gz[k + 1].x * a.Lrtp[qp] * gm -
-                     dt_s * gz[k].x * a.Lrp[q] + dt_s * gz[k + 1].x * a.Lrp[qp];
+        lrwp = a.Lmz[q] * rns - dt_s * gz[k].x * a.Lrtp[q] * gp + dt_s * gz[k + 1].x * a.Lrtp[qp] * gm -
+               dt_s * gz[k].x * a.Lrp[q] + dt_s * gz[k + 1].x * a.Lrp[qp];
     }
+    // read-modify-writes last, after every load of the kernel
+    a.Lth[G(k)] = lth0 + lth;
+    if (k <= ke - 1) a.Lrwp[G(k)] = lrwp0 + lrwp;
 }
 
 // Inclusive scans over the CH chunks of one column inside a wave (lane =
@@ -757,7 +765,10 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
         return rtp_s[q] + fact_tp * inc;
     };
     const int64_t q = G(k);
-    double lrt = 0.0, lth = 0.0, Lrp_tp = 0.0, Lrtp_tp = 0.0;
+    // this cell's read-modify-write values, read before any store
+    const double Lrt0 = a.Lrt[q], Lth0 = a.Lth[q], lr = a.Lrp[q], lt = a.Lrtp[q], LRr0 = a.LRr[q],
+                 LRt0 = a.LRt[q], LRw0 = k <= n - 1 ? a.LRw[q] : 0.0;
+    double lrt = 0.0, lth = 0.0, Lrp_tp = 0.0, Lrtp_tp = 0.0, lws = 0.0, lRw = 0.0;
     if (k <= n - 1) {
         const double y = yv(k);
         const double thk = W.th(k), rz1 = gz[k].y, rzd = rz1 * gz[k].x;
@@ -768,12 +779,11 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
         const double rt2h = (RT[q] + RT[G(k + 1)]) * 0.5;
         Lrtp_tp += dt_s * y * rt2h * rz1;
         Lrp_tp += -dt_s * y * GRAV * 0.5;
-        a.LRw[q] += dt_s * y;
+        lRw = dt_s * y;
         lth += Lthc(k) * dzd[k + 1] * gz[k].z;
         // rho_w' before the solve
-        double lws = y * (1.0 - fact * W.b(k));
+        lws = y * (1.0 - fact * W.b(k));
         if (k >= 1) lws += -fact * W.a(k) * y - fact * W.c(k - 1) * yv(k - 1);
-        a.Lrwp[q] = lws;
     }
     if (k >= 1) {
         const int kb = k - 1;
@@ -788,16 +798,19 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
         Lrp_tp += -dt_s * yb * GRAV * 0.5;
         lth += Lthc(kb) * dzd[kb] * gz[kb].z;
     }
-    a.Lrt[q] += lrt;
-    a.Lth[q] += lth;
-    const double lr = a.Lrp[q], lt = a.Lrtp[q];
     const double lir = lr + fact_tp * Lrp_tp, lit = lt + fact_tp * Lrtp_tp;
+    if (k <= n - 1) {
+        a.LRw[q] = LRw0 + lRw;
+        a.Lrwp[q] = lws;
+    }
+    a.Lrt[q] = Lrt0 + lrt;
+    a.Lth[q] = Lth0 + lth;
     a.Lrp[q] = lr + Lrp_tp;
     a.Lrtp[q] = lt + Lrtp_tp;
     a.Linc_r[q] = lir;
     a.Linc_t[q] = lit;
-    a.LRr[q] += dt_s * lir;
-    a.LRt[q] += dt_s * lit;
+    a.LRr[q] = LRr0 + dt_s * lir;
+    a.LRt[q] = LRt0 + dt_s * lit;
 }
 
 // ---- acoustic substep, horizontal part (moist.jl:1360-1408) -----------------------
@@ -826,33 +839,34 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
                lF * 0.5 * (TH[q] + TH[qp]);
     };
     const int64_t c = G(i), cm = G(i - 1), cp = G(i + 1);
+    // this cell's read-modify-write values, read before any store
+    const double Lth0 = a.Lth[c], LRu0 = a.LRu[c], Lrt0 = a.Lrt[c], Lrtp0 = a.Lrtp[c], pend0 = a.pend[c];
     // theta adjoint of the rho_theta' flux (faces i+1/2 and i-1/2)
+    double lth;
     {
         const double lFi = dt_s * rdx * (a.Linc_t[cp] - a.Linc_t[c]);
         const double lFm = dt_s * rdx * (a.Linc_t[c] - a.Linc_t[cm]);
-        a.Lth[c] += 0.5 * (lFi * rup_n[c] + lFm * rup_n[cm]);
+        lth = 0.5 * (lFi * rup_n[c] + lFm * rup_n[cm]);
     }
     const double Li = lrup(i), Lm = lrup(i - 1);
     // rho_theta' at the start of this substep: add the damping adjoint from the next substep
-    double lt = a.Lrtp[c] + a.pend[c];
+    double lt = Lrtp0 + pend0;
     // rho_u' update: rup_n = rup_s + dt_s (R_rho_u - rt2h ((1 + g) gx - g gxo))
     auto Gx = [&](int j) {
         const double gx = (rtp_s[G(j + 1)] - rtp_s[G(j)]) * rdx;
         const double gxo = (rtp_prev[G(j + 1)] - rtp_prev[G(j)]) * rdx;
         return gx + gam * (gx - gxo);
     };
-    a.LRu[c] += dt_s * Li;
-    a.Lrt[c] += -0.5 * dt_s * (Li * Gx(i) + Lm * Gx(i - 1));
+    const double lrt = -0.5 * dt_s * (Li * Gx(i) + Lm * Gx(i - 1));
     const double lGi = -dt_s * (RT[c] + RT[cp]) * 0.5 * Li;
     const double lGm = -dt_s * (RT[cm] + RT[c]) * 0.5 * Lm;
     lt += (1.0 + gam) * rdx * (lGm - lGi);
     const double lo = -gam * rdx * (lGm - lGi);
-    if (first_substep) {
-        lt += lo;
-        a.pend[c] = 0.0;
-    } else {
-        a.pend[c] = lo;
-    }
+    if (first_substep) lt += lo;
+    a.Lth[c] = Lth0 + lth;
+    a.LRu[c] = LRu0 + dt_s * Li;
+    a.Lrt[c] = Lrt0 + lrt;
+    a.pend[c] = first_substep ? 0.0 : lo;
     a.Lrtp[c] = lt;
     Lrup_out[c] = Li;
 }
@@ -882,22 +896,27 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     auto W = [&](int kk, int ii) { return a.wf[G(kk, ii)]; };  // kk <= ke - 1
     double l_r = 0.0, l_u = 0.0, l_w = 0.0, LUc = 0.0, LWc = 0.0, lp = 0.0, lth = 0.0;
 
+    // this cell's read-modify-write values, read before any store (stored at the end)
+    double s0r = ls0.f[RHO][c], s0u = ls0.f[RHOU][c], s0t = ls0.f[RT][c], s0w = ls0.f[RHOW][c];
+    double sr = ls.f[RHO][c], su = ls.f[RHOU][c], sw = ls.f[RHOW][c], st = ls.f[RT][c];
+    double p0 = lph.f[0][c], p1 = lph.f[1][c], p2 = lph.f[2][c];
+    const double Lth0 = a.Lth[c];
     // perturbation init and momentum copies
     {
         const double Lrp = a.Lrp[c], Lrt = a.Lrtp[c], Lrw = k <= ke - 1 ? a.Lrwp[c] : 0.0, Lru = Lrup[c];
-        ls0.f[RHO][c] += Lrp;
-        ls0.f[RHOU][c] += Lru;
-        ls0.f[RT][c] += Lrt;
-        ls0.f[RHOW][c] += Lrw;
+        s0r += Lrp;
+        s0u += Lru;
+        s0t += Lrt;
+        s0w += Lrw;
         l_r -= Lrp;
         l_u += -Lru + a.Lmx[c];
-        ls.f[RT][c] -= Lrt;
+        st -= Lrt;
         l_w += -Lrw + a.Lmz[c];
     }
     // physics tendencies enter R_rho, R_rho_u, R_rho_theta additively
-    lph.f[0][c] += a.LRr[c];
-    lph.f[1][c] += a.LRu[c];
-    lph.f[2][c] += a.LRt[c];
+    p0 += a.LRr[c];
+    p1 += a.LRu[c];
+    p2 += a.LRt[c];
 
     // ---- R_rho (moist.jl:1079-1086)
     l_u += -a.LRr[c] * rdx + a.LRr[G(k, i + 1)] * rdx;
@@ -1066,13 +1085,21 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
                                 kk + 2 <= ke ? TH[G(kk + 2, i)] : 0.0, dzd[kk + 1] * rden, dzd[kk] * rden);
     }
 
-    ls.f[RHO][c] += l_r;
-    ls.f[RHOU][c] += l_u;
-    ls.f[RHOW][c] += l_w;
+    ls0.f[RHO][c] = s0r;
+    ls0.f[RHOU][c] = s0u;
+    ls0.f[RT][c] = s0t;
+    ls0.f[RHOW][c] = s0w;
+    lph.f[0][c] = p0;
+    lph.f[1][c] = p1;
+    lph.f[2][c] = p2;
+    ls.f[RT][c] = st;
+    ls.f[RHO][c] = sr + l_r;
+    ls.f[RHOU][c] = su + l_u;
+    ls.f[RHOW][c] = sw + l_w;
     a.LU[c] = LUc;
     a.LW[c] = k <= ke - 1 ? LWc : 0.0;
     a.Lpres[c] = lp;
-    a.Lth[c] += lth;
+    a.Lth[c] = Lth0 + lth;
 }
 
 // ---- face velocities and stage diagnostics (moist.jl:1044-1069, 1091-1097, 1184-1189)
@@ -1133,13 +1160,16 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     double g[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) g[j] = lp * pres.d[j] + lt * theta.d[j] + lr2 * rt2.d[j];
-    ls.f[RHO][c] += l_r + g[0];
-    ls.f[RHOU][c] += l_u;
-    ls.f[RHOW][c] += l_w;
-    ls.f[RT][c] += g[1];
-    ls.f[QV][c] += g[2];
-    ls.f[QC][c] += g[3];
-    ls.f[QR][c] += g[4];
+    double o[7];
+#pragma unroll
+    for (int f = 0; f < 7; ++f) o[f] = ls.f[f][c];  // all read before the first store
+    ls.f[RHO][c] = o[RHO] + (l_r + g[0]);
+    ls.f[RHOU][c] = o[RHOU] + l_u;
+    ls.f[RHOW][c] = o[RHOW] + l_w;
+    ls.f[RT][c] = o[RT] + g[1];
+    ls.f[QV][c] = o[QV] + g[2];
+    ls.f[QC][c] = o[QC] + g[3];
+    ls.f[QR][c] = o[QR] + g[4];
 }
 
 // ---- Kessler microphysics of one cell with dual numbers (moist.jl:554-671) -------
