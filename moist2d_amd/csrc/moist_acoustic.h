@@ -79,6 +79,18 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
     const R EPS = K<R>::EPS;
     const R dt_s = stage_dt<R>(D, div, dt_ovr) / R(nsub);
     const R gam = R(D.gamma_d);
+    // Repeated divisors: the mixed form divides by the hardware reciprocal
+    // (fdiv's float form), so taking each reciprocal once gives the same
+    // values; the FP64 form keeps IEEE division.
+    auto rcp_of = [](R b) -> R {
+        if constexpr (is_mixed<S>()) return __builtin_amdgcn_rcpf(b);
+        else return R(1.0) / b;
+    };
+    auto DIV = [](R a, R b, R rb) -> R {
+        if constexpr (is_mixed<S>()) return a * rb;
+        else return a / b;
+    };
+    const R rdx = rcp_of(dx), rns = rcp_of(R(nsub)), r6 = rcp_of(R(6.0));
     const R beta = R(D.beta);
     // fact_tp of the explicit part is a Float64 expression (moist.jl:1375)
     const double fact_tp_d = double(R(0.5)) * (1.0 + double(beta));
@@ -136,16 +148,16 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
             const R old_c = ab.rtp_old[cc], old_m = ab.rtp_old[cm], old_p = ab.rtp_old[cx];
             const R rt2_c = w.rt2[cc], rt2_m = w.rt2[cm], rt2_p = w.rt2[cx];
             // rho_u' at i+1/2 and i-1/2 (moist.jl:1362-1367)
-            R gx = fdiv(cur_p - cur_c, dx);
-            R gxo = fdiv(old_p - old_c, dx);
+            R gx = DIV(cur_p - cur_c, dx, rdx);
+            R gxo = DIV(old_p - old_c, dx, rdx);
             gx = gx + gam * (gx - gxo);
             const R ru_i = ab.rup_in[cc] + dt_s * (w.R_rhou[cc] - (rt2_c + rt2_p) * R(0.5) * gx);
-            R gm = fdiv(cur_c - cur_m, dx);
-            R gmo = fdiv(old_c - old_m, dx);
+            R gm = DIV(cur_c - cur_m, dx, rdx);
+            R gmo = DIV(old_c - old_m, dx, rdx);
             gm = gm + gam * (gm - gmo);
             const R ru_m = ab.rup_in[cm] + dt_s * (w.R_rhou[cm] - (rt2_m + rt2_c) * R(0.5) * gm);
             // explicit rho' (moist.jl:1383-1386)
-            R adv = fdiv(-(ru_i - ru_m), dx);
+            R adv = DIV(-(ru_i - ru_m), dx, rdx);
             R inc = dt_s * (w.R_rho[cc] + adv);
             const R rpc = w.rp[cc];
             R rp_tp;
@@ -156,7 +168,7 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
             const R th_c = w.theta[cc];
             const R fip = (w.theta[cx] + th_c) * R(0.5) * ru_i;
             const R fim = (th_c + w.theta[cm]) * R(0.5) * ru_m;
-            adv = fdiv(-(fip - fim), dx);
+            adv = DIV(-(fip - fim), dx, rdx);
             inc = dt_s * (w.R_rt[cc] + adv);
             R rtp_tp;
             if constexpr (is_mixed<S>()) rtp_tp = cur_c + R(fact_tp_d) * inc;
@@ -178,7 +190,7 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
             tp_tp = rtp_tp;
             // rho_u' and its running mean mom_x (moist.jl:1453-1455)
             ab.rup_out[cc] = ru_i;
-            w.mom_x[cc] = mx_in + fdiv(ru_i, R(nsub));
+            w.mom_x[cc] = mx_in + DIV(ru_i, R(nsub), rns);
         }
     }
     // theta two levels below and above the chunk, rt2pres one level above
@@ -245,14 +257,22 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
         cc = j < nz - 2 ? -dt_tp2 * (r2k * thh[l + 2] * (gy * rdz1) + gw) : 0.0;
     };
 
+    // coefficients of the chunk's faces, once for the right-hand sides, the
+    // Moebius composites and the forward substitution
+    double ca[L], cb[L], ccf[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        ca[l] = cb[l] = ccf[l] = 0.0;
+        if (l < nf) coef(l, ca[l], cb[l], ccf[l]);
+    }
+
     // ---- right-hand sides (moist.jl:1008-1021)
 #pragma unroll
     for (int l = 0; l < L; ++l) {
         dd[l] = 0.0;
         if (l < nf) {
             const int j = k0 + l;
-            double a, b, cc;
-            coef(l, a, b, cc);
+            const double a = ca[l], b = cb[l], cc = ccf[l];
             const double rwj = double(rwv[l]);
             double d = rwj + double(drw[l]);
             if (j > 0) d -= a * fact * rwj;
@@ -279,9 +299,7 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
 #pragma unroll
         for (int l = 0; l < L; ++l) {
             if (l < nf) {
-                double a, b, cc;
-                coef(l, a, b, cc);
-                b += 1.0;
+                const double a = ca[l], b = cb[l] + 1.0, cc = ccf[l];
                 // [num; den] <- [[0, c], [-a, b]] [num; den]
                 const double n00 = cc * m10, n01 = cc * m11;
                 const double n10 = -a * m00 + b * m10, n11 = -a * m01 + b * m11;
@@ -345,9 +363,7 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
             cpv[l] = 0.0;
             pv[l] = 0.0;
             if (l < nf) {
-                double a, b, cc;
-                coef(l, a, b, cc);
-                b += 1.0;
+                const double a = ca[l], b = cb[l] + 1.0, cc = ccf[l];
                 const double rm = 1.0 / (b - a * cprev);
                 const double cp = cc * rm;
                 const double y = (dd[l] - a * yprev) * rm;
@@ -468,7 +484,7 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
             const int64_t cj = base + (int64_t)(k0 + l) * nx + i;
             xr[l] = R(x);
             w.rwp[cj] = xr[l];
-            w.mom_z[cj] = mzv[l] + fdiv(xr[l], R(nsub));
+            w.mom_z[cj] = mzv[l] + DIV(xr[l], R(nsub), rns);
         }
     }
     const R xb = R(x_below);
@@ -481,25 +497,25 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
             const int64_t cc = base + (int64_t)k * nx + i;
             const R wkp = k == ke ? R(0.0) : xr[l];
             const R wkm = k == 0 ? R(0.0) : (l > 0 ? xr[l > 0 ? l - 1 : 0] : xb);
-            const R dzk = R(LDZ(k));
-            R adv = fdiv(-(wkp - wkm), dzk);
+            const R dzk = R(LDZ(k)), rdzk = rcp_of(dzk);
+            R adv = DIV(-(wkp - wkm), dzk, rdzk);
             w.rp[cc] = rp_e[l] + dt_s * adv;
             const R t_m2 = thx[l], t_m1 = thx[l + 1], t_0 = thx[l + 2], t_p1 = thx[l + 3], t_p2 = thx[l + 4];
             R Fkp, Fkm;
             if (k == ke) Fkp = R(0.0);
             else {
-                if (wkp > EPS && k > 0) Fkp = fdiv(wkp * upw3(t_m1, t_0, t_p1), R(6.0));
-                else if (swv[l + 1] < -EPS && k < ke - 1) Fkp = fdiv(wkp * upw3(t_p2, t_p1, t_0), R(6.0));
+                if (wkp > EPS && k > 0) Fkp = DIV(wkp * upw3(t_m1, t_0, t_p1), R(6.0), r6);
+                else if (swv[l + 1] < -EPS && k < ke - 1) Fkp = DIV(wkp * upw3(t_p2, t_p1, t_0), R(6.0), r6);
                 else Fkp = wkp * f2h(t_0, t_p1, dzk, R(LDZ(k + 1)));
             }
             if (k == 0) Fkm = R(0.0);
             else {
                 const R sw = swv[l];
-                if (sw > EPS && k > 1) Fkm = fdiv(wkm * upw3(t_m2, t_m1, t_0), R(6.0));
-                else if (sw < -EPS && k < ke) Fkm = fdiv(wkm * upw3(t_p1, t_0, t_m1), R(6.0));
+                if (sw > EPS && k > 1) Fkm = DIV(wkm * upw3(t_m2, t_m1, t_0), R(6.0), r6);
+                else if (sw < -EPS && k < ke) Fkm = DIV(wkm * upw3(t_p1, t_0, t_m1), R(6.0), r6);
                 else Fkm = wkm * f2h(t_m1, t_0, R(LDZ(k - 1)), dzk);
             }
-            adv = fdiv(-(Fkp - Fkm), dzk);
+            adv = DIV(-(Fkp - Fkm), dzk, rdzk);
             ab.rtp_out[cc] = rtp_e[l] + dt_s * adv;
         }
     }
