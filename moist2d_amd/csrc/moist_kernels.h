@@ -104,18 +104,20 @@ __global__ void __launch_bounds__(256) k_diag(const Dom *__restrict__ doms, cons
         pres = KC::P0 * m_pow(r * rt / KC::P0, cp / cv);
         pfull = pres;
     }
-    w.pres[c] = pres;
-    w.theta[c] = fdiv(rt, rho);
-    w.rt2[c] = fdiv(fdiv(cp, cv) * pfull, rt);
+    // neighbour reads (i+1 and k+1) happen before the first store
     const int64_t cip = D.off2 + (int64_t)k * nx + ip;
-    w.uf[c] = fdiv(in.f[RHOU][c] * R(2.0), rho + full_rho(in, cip, zv, D, k));
+    const R uf = fdiv(in.f[RHOU][c] * R(2.0), rho + full_rho(in, cip, zv, D, k));
+    R wf = R(0.0);
     if (k < nz - 1) {
         const int64_t ckp = c + nx;
         const double *dz = zv.dz + D.offz;
-        w.wf[c] = fdiv(in.f[RHOW][c], f2h(rho, full_rho(in, ckp, zv, D, k + 1), R(dz[k]), R(dz[k + 1])));
-    } else {
-        w.wf[c] = R(0.0);
+        wf = fdiv(in.f[RHOW][c], f2h(rho, full_rho(in, ckp, zv, D, k + 1), R(dz[k]), R(dz[k + 1])));
     }
+    w.pres[c] = pres;
+    w.theta[c] = fdiv(rt, rho);
+    w.rt2[c] = fdiv(fdiv(cp, cv) * pfull, rt);
+    w.uf[c] = uf;
+    w.wf[c] = wf;
 }
 
 #include "moist_acoustic.h"
@@ -269,11 +271,20 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
     const int fl = D.flags;
     const R rho_k = full_rho(s, c, zv, D, k);
     const R rt = full_rt(s, c, zv, D, k);
+    // every other global read of the kernel, before its first store: the
+    // rain and density one level up (sedimentation), radiation and surface terms
+    const int kup = k < ke ? k + 1 : k;
+    const int64_t cu = D.off2 + (int64_t)kup * nx + i;
+    const R qr_c = s.f[QR][c], qr_u = s.f[QR][cu], rho_u = full_rho(s, cu, zv, D, kup);
+    const R th_rd = (fl & F_RD) ? w.d_th_rd[c] : R(0.0);
+    const bool sf = (k == 0) && (fl & F_SF);
+    const int64_t x = D.offx + i;
+    const R qv_sf = sf ? w.d_qv_sf[x] : R(0.0), th_sf = sf ? w.d_th_sf[x] : R(0.0);
     R dqv = R(0.0), dqc = R(0.0), dqr = R(0.0), dth = R(0.0);
     if (fl & F_MP) {
         const R rinv = fdiv(R(1.0), rho_k);
         R theta_k = rt * rinv;
-        R qv = s.f[QV][c] * rinv, qc = s.f[QC][c] * rinv, qr = s.f[QR][c] * rinv;
+        R qv = s.f[QV][c] * rinv, qc = s.f[QC][c] * rinv, qr = qr_c * rinv;
         const R qd = R(1.0) - qv - qc - qr;
         R r = KC::Rd * qd + KC::Rv * qv;
         R cp = KC::Cpd * qd + KC::Cpv * qv + KC::Cpl * (qc + qr);
@@ -346,32 +357,26 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
             dth -= re * lhe;
         }
     }
-    R dsed = R(0.0);
-    if ((fl & F_MP) && (fl & F_SED)) {
-        const R vt_k = term_velocity(s.f[QR][c], rho_k);
-        const R fkm = s.f[QR][c] * vt_k;
+    R dsed = R(0.0), fkm = R(0.0);
+    const bool sed = (fl & F_MP) && (fl & F_SED);
+    if (sed) {
+        const R vt_k = term_velocity(qr_c, rho_k);
+        fkm = qr_c * vt_k;
         R fkp = R(0.0);
-        if (k < ke) {
-            const int64_t cu = c + nx;
-            fkp = s.f[QR][cu] * term_velocity(s.f[QR][cu], full_rho(s, cu, zv, D, k + 1));
-        }
+        if (k < ke) fkp = qr_u * term_velocity(qr_u, rho_u);
         dsed = fdiv(fkp - fkm, R(zv.dz[D.offz + k]));
-        if (k == 0) {
-            const int64_t x = D.offx + i;
-            if constexpr (is_mixed<S>()) w.prec[x] += double(fkm) * D.dt;
-            else w.prec[x] += fkm * dt;
-        }
     }
-    const R th_rd = (fl & F_RD) ? w.d_th_rd[c] : R(0.0);
-    const bool sf = (k == 0) && (fl & F_SF);
-    const int64_t x = D.offx + i;
     R d_rho = dsed;
-    if (sf) d_rho += w.d_qv_sf[x];
+    if (sf) d_rho += qv_sf;
     R d_rt = dth + th_rd + fdiv(d_rho * rt, rho_k);
-    if (sf) d_rt += w.d_th_sf[x];
+    if (sf) d_rt += th_sf;
     R d_qv = dqv;
-    if (sf) d_qv += w.d_qv_sf[x];
+    if (sf) d_qv += qv_sf;
     const R d_qr = dqr + dsed;
+    if (sed && k == 0) {  // accumulated rain (FP64)
+        if constexpr (is_mixed<S>()) w.prec[x] += double(fkm) * D.dt;
+        else w.prec[x] += fkm * dt;
+    }
     w.d_rho[c] = d_rho;
     w.d_rt[c] = d_rt;
     w.d_qv[c] = d_qv;
