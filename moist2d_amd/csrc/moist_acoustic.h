@@ -47,6 +47,7 @@ template <class S> struct AcBuf {
     S *rtp_out;        // rho_theta' after the substep
     double *cin;       // c' entering each chunk, [chunk][column]
     int64_t cin_stride;
+    const S *rwp_in;   // rho_w' before this substep (the result goes to Work::rwp; may be the same array)
 };
 
 // WPE: minimum waves per SIMD the register allocation must allow (1: no
@@ -121,7 +122,7 @@ __global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ do
 #pragma unroll
         for (int q = 0; q < L + 1; ++q) {
             const int j = k0 + q, kq = k0 - 1 + q;
-            const R rv = w.rwp[at(j)], sv = in.f[RHOW][at(kq)];
+            const R rv = ab.rwp_in[at(j)], sv = in.f[RHOW][at(kq)];
             rwv[q] = (nl > 0 && j <= nz - 2) ? rv : R(0.0);
             swv[q] = (nl > 0 && kq >= 0 && kq < nz) ? sv : R(0.0);
         }

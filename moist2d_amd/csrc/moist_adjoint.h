@@ -253,9 +253,11 @@ __device__ __forceinline__ double upw_val(bool pos, bool neg, double vm1, double
 }
 
 // ---- adjoint work arrays of one stage --------------------------------------------
-struct AdjW {
-    // forward record of the stage (FP64)
-    const double *theta, *pres, *rt2, *R_rho, *R_rhou, *R_rhow, *R_rt, *mom_x, *mom_z;
+// RC: type of the forward record (the recording pass's form: double, or
+// float in the mixed form); every adjoint is FP64.
+template <class RC> struct AdjW {
+    // forward record of the stage
+    const RC *theta, *pres, *rt2, *R_rho, *R_rhou, *R_rhow, *R_rt, *mom_x, *mom_z;
     // adjoints
     double *Lth, *Lrt, *Lpres, *LU, *LW, *LRr, *LRu, *LRw, *LRt, *Lmx, *Lmz;
     double *Lrp, *Lrwp, *Lrtp, *pend, *Linc_r, *Linc_t;
@@ -268,6 +270,20 @@ struct AdjW {
 __global__ void ka_fill(double *p, int64_t n, double v) {
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
         p[q] = v;
+}
+
+// Zeroing of up to 16 arrays of `len` doubles in one launch (graph-safe).
+struct FillList {
+    double *p[16];
+    int n;
+    int64_t len;
+};
+__global__ void ka_fill_many(FillList L) {
+    const int64_t total = (int64_t)L.n * L.len;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        const int a = (int)(q / L.len);
+        L.p[a][q - (int64_t)a * L.len] = 0.0;
+    }
 }
 
 // Full FP64 state from a mixed-form state (FP32 deviations; rho and
@@ -322,8 +338,9 @@ __global__ void __launch_bounds__(256) ka_loss(const Dom *__restrict__ doms, SSe
 // q = rho_q / rho of the three scalars, u = 2 rho_u / (rho_i + rho_i+1) and
 // w = rho_w / f2h(rho_k, rho_k+1) of the stage input, with the forward's
 // expressions.
+template <class RC>
 __global__ void __launch_bounds__(AD_X *AD_Z)
-    ka_pre(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s, AdjW a) {
+    ka_pre(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s, AdjW<RC> a) {
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
@@ -342,9 +359,10 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
 // lo: adjoint of the stage output; writes ls (stage input, first writer of the
 // stage), adds into ls0 (rho_q) and lph (q tendencies); writes Lmx, Lmz and
 // the substep adjoints Lrp, Lrtp, Lrwp, Lrup (= lo of rho, rho_theta, rho_w, rho_u).
+template <class RC>
 __global__ void __launch_bounds__(AD_X *AD_Z)
     ka_scalar(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s, SSet<double> lo,
-              SSet<double> ls, SSet<double> ls0, SSet<double> lph, AdjW a, double *Lrup, int div) {
+              SSet<double> ls, SSet<double> ls0, SSet<double> lph, AdjW<RC> a, double *Lrup, int div) {
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
@@ -460,7 +478,8 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
 
 // Sum of the positivity-limiter adjoints of every column into rho_q at
 // (0, lin_col) and (ke, lin_col).  One block per member.
-__global__ void __launch_bounds__(256) ka_lin(const Dom *__restrict__ doms, AdjW a, SSet<double> ls) {
+template <class RC>
+__global__ void __launch_bounds__(256) ka_lin(const Dom *__restrict__ doms, AdjW<RC> a, SSet<double> ls) {
     const Dom &D = doms[blockIdx.x];
     __shared__ double rb[256], rt[256];
     for (int sc = 0; sc < 3; ++sc) {
@@ -492,11 +511,11 @@ __global__ void __launch_bounds__(256) ka_lin(const Dom *__restrict__ doms, AdjW
 // A = tridiag(a, b + 1, c) (implicit_correction!, moist.jl:974-1030).
 // Divisions by grid quantities use the per-level reciprocals of the geometry
 // table gz: {1/dz_k, 1/(z_k+1 - z_k), 1/(dz_k + dz_k+1), g/(dz_k + dz_k+1)}.
-struct WpCoef {
+template <class RC> struct WpCoef {
     const Dom &D;
     const double *zd, *dzd;
     const double4 *gz;
-    const double *TH, *RT;
+    const RC *TH, *RT;
     int nx, i;
     double dt_tp2;
     __device__ int64_t G(int kk) const { return D.off2 + (int64_t)kk * nx + i; }
@@ -517,9 +536,10 @@ struct WpCoef {
 // (1) Adjoint of rho_w' after the solve: the momentum average, the rho'
 // and rho_theta' vertical updates (with the Q5 branch tests of the
 // forward); theta adjoint of the rho_theta' vertical flux.  Per cell.
+template <class RC>
 __global__ void __launch_bounds__(AD_X *AD_Z)
-    ka_ac_v1(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s, AdjW a,
-             const double *__restrict__ rwp_n, int div, int nsub) {
+    ka_ac_v1(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s, AdjW<RC> a,
+             const RC *__restrict__ rwp_n, int div, int nsub) {
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
@@ -531,7 +551,8 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     const double EPS = 1e-12, rns = 1.0 / double(nsub);
     const double dt_s = D.dt / double(div) / double(nsub);
     auto G = [&](int kk) -> int64_t { return D.off2 + (int64_t)kk * nx + i; };
-    const double *TH = a.theta, *rw = s.f[RHOW];
+    const RC *TH = a.theta;
+    const double *rw = s.f[RHOW];
     auto THv = [&](int kk) { return kk >= 0 && kk <= ke ? TH[G(kk)] : 0.0; };
     // face f+1/2 as fkp of cell f: positive branch tests rwp_n[f], negative s.rho_w[f]
     auto brp = [&](int f, bool &pos, bool &neg) {
@@ -621,9 +642,9 @@ __device__ __forceinline__ void mob_norm(double &m00, double &m01, double &m10, 
 // the chunks.  `factor` (first reverse substep of a stage) first computes
 // c'_j = c_j / (b_j + 1 - a_j c'_j-1) with a Moebius scan and stores c', 1/m
 // and a.  groups: {member, first column} of every wave.
-template <int CH>
+template <int CH, class RC>
 __global__ void __launch_bounds__(256)
-    ka_ac_solve(const Dom *__restrict__ doms, const int2 *__restrict__ groups, int ngroups, ZVec zv, AdjW a, int div,
+    ka_ac_solve(const Dom *__restrict__ doms, const int2 *__restrict__ groups, int ngroups, ZVec zv, AdjW<RC> a, int div,
                 int nsub, int factor) {
     constexpr int CW = 64 / CH;
     const int wid = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -638,7 +659,7 @@ __global__ void __launch_bounds__(256)
     const bool live = i < nx;
     const double dt_s = D.dt / double(div) / double(nsub);
     const double fact_tp = 0.5 * (1.0 + D.beta);
-    const WpCoef W{D, zv.z + D.offz, zv.dz + D.offz, zv.geo + D.offz, a.theta, a.rt2, nx, live ? i : 0,
+    const WpCoef<RC> W{D, zv.z + D.offz, zv.dz + D.offz, zv.geo + D.offz, a.theta, a.rt2, nx, live ? i : 0,
                    (dt_s * fact_tp) * (dt_s * fact_tp)};
     const int L = (n + CH - 1) / CH, j0 = ch * L, j1 = live ? min(j0 + L, n) : j0;
     if (factor) {
@@ -720,10 +741,11 @@ __global__ void __launch_bounds__(256)
 // (3) Matrix and right-hand-side adjoints of the solve, the adjoint of
 // rho_w' before it, drwp = dt_s (R_rho_w - rt2h buoy + grav) and the explicit
 // part (rp_tp = rp + f inc, rp += inc; the same for rho_theta').  Per cell.
+template <class RC>
 __global__ void __launch_bounds__(AD_X *AD_Z)
-    ka_ac_v2(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, AdjW a,
-             const double *__restrict__ rtp_s, const double *__restrict__ rup_n, const double *__restrict__ rwp_s,
-             const double *__restrict__ rwp_n, int div, int nsub) {
+    ka_ac_v2(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, AdjW<RC> a,
+             const RC *__restrict__ rtp_s, const RC *__restrict__ rup_n, const RC *__restrict__ rwp_s,
+             const RC *__restrict__ rwp_n, int div, int nsub) {
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
@@ -736,9 +758,9 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     const double fact_tp = 0.5 * (1.0 + D.beta), fact_tm = 0.5 * (1.0 - D.beta);
     const double fact = fact_tm / fact_tp;
     const double4 *gz = zv.geo + D.offz;
-    const WpCoef W{D, zd, dzd, gz, a.theta, a.rt2, nx, i, (dt_s * fact_tp) * (dt_s * fact_tp)};
+    const WpCoef<RC> W{D, zd, dzd, gz, a.theta, a.rt2, nx, i, (dt_s * fact_tp) * (dt_s * fact_tp)};
     const double dt_tp2 = W.dt_tp2;
-    const double *RT = a.rt2, *TH = a.theta;
+    const RC *RT = a.rt2, *TH = a.theta;
     auto G = [&](int kk) -> int64_t { return W.G(kk); };
     auto Gx = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
     auto yv = [&](int j) { return j >= 0 && j <= n - 1 ? a.Y[G(j)] : 0.0; };
@@ -817,10 +839,11 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
 // Per cell.  Lrup_in: adjoint of rho_u' after this substep (from later
 // substeps and the final update); Lrup_out: adjoint before it.  pend: the
 // damping term's adjoint for rho_theta' at the start of the previous substep.
+template <class RC>
 __global__ void __launch_bounds__(AD_X *AD_Z)
-    ka_ac_hor(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, AdjW a, const double *__restrict__ Lrup_in,
-              double *__restrict__ Lrup_out, const double *__restrict__ rtp_s, const double *__restrict__ rtp_prev,
-              const double *__restrict__ rup_n, int div, int nsub, int first_substep) {
+    ka_ac_hor(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, AdjW<RC> a,
+              const double *__restrict__ Lrup_in, double *__restrict__ Lrup_out, const RC *__restrict__ rtp_s,
+              const RC *__restrict__ rtp_prev, const RC *__restrict__ rup_n, int div, int nsub, int first_substep) {
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
@@ -830,7 +853,7 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     const double dx = D.dx, gam = D.gamma_d, rdx = 1.0 / dx;
     const double dt_s = D.dt / double(div) / double(nsub), rns = 1.0 / double(nsub);
     auto G = [&](int ii) -> int64_t { return D.off2 + (int64_t)k * nx + wrapn(ii, nx); };
-    const double *TH = a.theta, *RT = a.rt2;
+    const RC *TH = a.theta, *RT = a.rt2;
     // adjoint of rho_u'(j) after the substep, all contributions of this substep included
     auto lrup = [&](int j) {
         const int64_t q = G(j), qp = G(j + 1);
@@ -876,9 +899,10 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
 // stage input (rho, rho_u, rho_w), to the face velocities (LU, LW), pres and
 // theta; adds the perturbation adjoints (rho' = s0 - s etc.) and the momentum
 // copies.
+template <class RC>
 __global__ void __launch_bounds__(AD_X *AD_Z)
     ka_slow(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s, SSet<double> ls,
-            SSet<double> ls0, SSet<double> lph, AdjW a, const double *__restrict__ Lrup) {
+            SSet<double> ls0, SSet<double> lph, AdjW<RC> a, const double *__restrict__ Lrup) {
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
@@ -891,7 +915,7 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     auto G = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
     const int64_t c = G(k, i);
     const double *rho = s.f[RHO], *ru = s.f[RHOU], *rw = s.f[RHOW];
-    const double *TH = a.theta;
+    const RC *TH = a.theta;
     auto U = [&](int kk, int ii) { return a.uf[G(kk, ii)]; };
     auto W = [&](int kk, int ii) { return a.wf[G(kk, ii)]; };  // kk <= ke - 1
     double l_r = 0.0, l_u = 0.0, l_w = 0.0, LUc = 0.0, LWc = 0.0, lp = 0.0, lth = 0.0;
@@ -1103,9 +1127,10 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
 }
 
 // ---- face velocities and stage diagnostics (moist.jl:1044-1069, 1091-1097, 1184-1189)
+template <class RC>
 __global__ void __launch_bounds__(AD_X *AD_Z)
     ka_diag(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s, SSet<double> ls,
-            AdjW a) {
+            AdjW<RC> a) {
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
@@ -1275,7 +1300,7 @@ __device__ __forceinline__ void kessler_v(const Dom &D, V rho_k, V rt, V rqv, V 
 // Adds into ls; accumulates dJ/dK per cell into gk (3 fields).
 __global__ void __launch_bounds__(AD_X *AD_Z)
     ka_physics(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s,
-               SSet<double> lph, SSet<double> ls, double *gk0, double *gk1, double *gk2) {
+               SSet<double> lph, SSet<double> ls, SSet<double> ls0, double *gk0, double *gk1, double *gk2) {
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
@@ -1297,6 +1322,11 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     auto flux = [&](int kk) { return rqr[G(kk)] * vt(kk); };
     double sedc = 0.0;
     if (sed) sedc = ((k < ke ? flux(k + 1) : 0.0) - flux(k)) / dzd[k];
+    // ls += ls0 (the step input is also every stage's s0) plus the physics
+    // adjoints below; all seven read first, written at the end
+    double o[7];
+#pragma unroll
+    for (int f = 0; f < 7; ++f) o[f] = ls.f[f][c] + ls0.f[f][c];
     double l_r = 0.0, l_rt = 0.0, l_qr = 0.0;
     // d_rho_theta = d_th_mp + d_rho rho_theta / rho with d_rho = d_qr_sed
     l_rt += lph.f[2][c] * sedc / rho[c];
@@ -1319,8 +1349,8 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
             }
             l_r += g[0];
             l_rt += g[1];
-            ls.f[QV][c] += g[2];
-            ls.f[QC][c] += g[3];
+            o[QV] += g[2];
+            o[QC] += g[3];
             l_qr += g[4];
         }
         {
@@ -1356,9 +1386,11 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
             l_r += -lF * C.B_term * rqr[c] * v / rho[c];
         }
     }
-    ls.f[RHO][c] += l_r;
-    ls.f[RT][c] += l_rt;
-    ls.f[QR][c] += l_qr;
+    o[RHO] += l_r;
+    o[RT] += l_rt;
+    o[QR] += l_qr;
+#pragma unroll
+    for (int f = 0; f < 7; ++f) ls.f[f][c] = o[f];
 }
 
 // Per-member sums of the parameter gradients (one block per member).
@@ -1384,10 +1416,4 @@ __global__ void __launch_bounds__(256) ka_reduce_k(const Dom *__restrict__ doms,
     }
     if (threadIdx.x == 0)
         for (int j = 0; j < 3; ++j) out[3 * blockIdx.x + j] = r[j][0];
-}
-
-// y += x over n doubles
-__global__ void ka_axpy(double *y, const double *x, int64_t n) {
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
-        y[q] += x[q];
 }

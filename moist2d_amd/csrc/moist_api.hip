@@ -20,6 +20,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <initializer_list>
 #include <memory>
 #include <string>
 #include <vector>
@@ -324,6 +325,7 @@ void enqueue_stage(moist_handle *h, int in_set, int out_set, int s0_set, int div
         ab.rtp_out = rtp[nxt];
         ab.cin = h->cin;
         ab.cin_stride = h->nxs;
+        ab.rwp_in = w.rwp;  // updated in place
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (h->ev_on) {
             e0 = next_event(h);
@@ -639,15 +641,20 @@ struct AdjState {
     int nsteps = 0, ckpt = 0, ncp = 0, mode = 0;
     // checkpoint: the 7 state slots in the handle's form (cp_words doubles), then prec
     size_t cp_stride = 0, cp_words = 0;
-    double *cp = nullptr, *seg = nullptr, *SC = nullptr;
-    // FP64 pools of the recording forward pass: the handle's own pools in the
-    // FP64 form; separate pools of the same layout in the mixed form
-    double *pool64 = nullptr, *pool1_64 = nullptr;
-    // forward records: blocks of {stage-2 input, stage-3 input, stage records,
-    // substep records}; one block per step of a segment when they fit in
-    // memory (`segrec`), else one block recomputed per step
-    double *recs = nullptr, *cur = nullptr;
-    size_t rb_stride = 0;
+    double *cp = nullptr, *SC = nullptr;
+    // step inputs of one segment in the handle's form (7 fields per step,
+    // seg_bytes apart); the mixed form widens one of them to FP64 into `snb`
+    // when the step is reversed
+    char *seg = nullptr;
+    size_t seg_bytes = 0;
+    double *snb = nullptr;
+    // forward records: blocks of {stage-2 input, stage-3 input (FP64), stage
+    // records, substep records (in the handle's form, rec_elem bytes)}; one
+    // block per step of a segment when they fit in memory (`segrec`), else one
+    // block recomputed per step
+    char *recs = nullptr, *cur = nullptr;
+    size_t rb_bytes = 0;
+    int rec_elem = 8;
     int segrec = 0, nblocks = 1;
     double *lam = nullptr, *lam2 = nullptr, *ls0 = nullptr, *lph = nullptr, *wk = nullptr, *lim = nullptr;
     double *gk = nullptr, *tgt = nullptr, *fin = nullptr, *J = nullptr, *gK = nullptr;
@@ -668,6 +675,8 @@ void adj_free(moist_handle *h) {
     h->adj = nullptr;
 }
 
+size_t rec_in_bytes(const moist_handle *h) { return 14 * (size_t)h->n2 * sizeof(double); }
+
 int adj_alloc(moist_handle *h, int nsteps, int ckpt) {
     if (h->adj && h->adj->nsteps == nsteps && h->adj->ckpt == ckpt && h->adj->mode == h->adj_mode) return 0;
     adj_free(h);
@@ -681,19 +690,20 @@ int adj_alloc(moist_handle *h, int nsteps, int ckpt) {
     const bool mixed = h->precision == MOIST_MIXED;
     A.cp_words = (7 * n2 * h->elem + 7) / 8;
     A.cp_stride = A.cp_words + h->nxs;
+    A.rec_elem = (int)h->elem;
+    A.seg_bytes = (7 * n2 * h->elem + 255) / 256 * 256;
     int nrec_sub = 0;
     for (int st = 0; st < 3; ++st) {
         A.sub_off[st] = nrec_sub;
         nrec_sub += h->ns_stage[st] + 1;
     }
-    A.rb_stride = (14 + 3 * (size_t)NREC + 3 * (size_t)nrec_sub) * n2;
+    A.rb_bytes = (rec_in_bytes(h) + (3 * (size_t)NREC + 3 * (size_t)nrec_sub) * n2 * A.rec_elem + 255) / 256 * 256;
     {
         // records of a whole segment if they take at most a quarter of the free memory
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) == hipSuccess) {
-            const size_t need = A.rb_stride * sizeof(double) * (size_t)ckpt;
-            const size_t other =
-                (A.cp_stride * A.ncp + 7 * n2 * (size_t)(ckpt + 1) + (mixed ? NSLOT + 60 : 60) * n2) * sizeof(double);
+            const size_t need = A.rb_bytes * (size_t)ckpt;
+            const size_t other = (A.cp_stride * A.ncp + 60 * n2) * sizeof(double) + A.seg_bytes * (size_t)ckpt;
             A.segrec = need + other < fr / 4 ? 1 : 0;
         }
         if (A.mode) A.segrec = A.mode == 2 ? 1 : 0;
@@ -705,19 +715,16 @@ int adj_alloc(moist_handle *h, int nsteps, int ckpt) {
         A.bytes += n * sizeof(double);
         return 0;
     };
-    if (al(A.cp, A.cp_stride * A.ncp) || al(A.seg, 7 * n2 * (size_t)ckpt) || al(A.SC, 7 * n2) ||
-        al(A.recs, A.rb_stride * (size_t)A.nblocks) || al(A.lam, 7 * n2) ||
+    double *seg = nullptr, *recs = nullptr;
+    if (al(A.cp, A.cp_stride * A.ncp) || al(seg, A.seg_bytes * (size_t)ckpt / 8) || al(A.SC, 7 * n2) ||
+        al(recs, A.rb_bytes * (size_t)A.nblocks / 8) || al(A.lam, 7 * n2) ||
         al(A.lam2, 7 * n2) || al(A.ls0, 7 * n2) || al(A.lph, 7 * n2) || al(A.wk, NWORK * n2) ||
         al(A.lim, 6 * h->nxs) || al(A.gk, 3 * n2) || al(A.tgt, n2) || al(A.fin, A.cp_stride) || al(A.J, h->nm) ||
         al(A.gK, 3 * h->nm))
         return -1;
-    if (mixed) {
-        if (al(A.pool64, (size_t)NSLOT * n2) || al(A.pool1_64, (size_t)NSLOT1 * h->nxs)) return -1;
-        HIPCHK(h, hipMemset(A.pool1_64, 0, (size_t)NSLOT1 * h->nxs * sizeof(double)));
-    } else {
-        A.pool64 = (double *)h->pool;
-        A.pool1_64 = (double *)h->pool1;
-    }
+    if (mixed && al(A.snb, 7 * n2)) return -1;
+    A.seg = (char *)seg;
+    A.recs = (char *)recs;
     A.cur = A.recs;
     A.J_h.assign(h->nm, 0.0);
     A.gK_h.assign(3 * h->nm, 0.0);
@@ -729,34 +736,42 @@ SSet<double> dset(double *base, size_t n2) {
     for (int f = 0; f < 7; ++f) s.f[f] = base + f * n2;
     return s;
 }
+SSet<float> fset(float *base, size_t n2) {
+    SSet<float> s;
+    for (int f = 0; f < 7; ++f) s.f[f] = base + f * n2;
+    return s;
+}
 
 // Layout of a record block (current block h->adj->cur): stage-2 input (7
-// fields), stage-3 input (7), per stage the NREC stage records, then per
-// stage and substep n (n = 0: initial) rho_theta', rho_u', rho_w' after it.
-double *rec_in(moist_handle *h, int st) { return h->adj->cur + (size_t)(st - 1) * 7 * h->n2; }  // st = 1, 2
-double *rec_st(moist_handle *h, int st, int r) { return h->adj->cur + (14 + (size_t)st * NREC + r) * h->n2; }
-double *rec_sub(moist_handle *h, int st, int n, int f) {
-    return h->adj->cur + (14 + 3 * (size_t)NREC + (size_t)3 * (h->adj->sub_off[st] + n) + f) * h->n2;
+// fields, FP64), stage-3 input (7, FP64), per stage the NREC stage records,
+// then per stage and substep n (n = 0: initial) rho_theta', rho_u', rho_w'
+// after it; the records are of the recording pass's type RC (the handle's).
+double *rec_in(moist_handle *h, int st) {  // st = 1, 2
+    return (double *)(h->adj->cur + (size_t)(st - 1) * 7 * h->n2 * sizeof(double));
 }
-double *rec_rtp(moist_handle *h, int st, int n) { return rec_sub(h, st, n, 0); }
-double *rec_rup(moist_handle *h, int st, int n) { return rec_sub(h, st, n, 1); }
-double *rec_rwp(moist_handle *h, int st, int n) { return rec_sub(h, st, n, 2); }
+template <class RC> RC *rec_st(moist_handle *h, int st, int r) {
+    return (RC *)(h->adj->cur + rec_in_bytes(h) + ((size_t)st * NREC + r) * h->n2 * sizeof(RC));
+}
+template <class RC> RC *rec_sub(moist_handle *h, int st, int n, int f) {
+    return (RC *)(h->adj->cur + rec_in_bytes(h) +
+                  (3 * (size_t)NREC + (size_t)3 * (h->adj->sub_off[st] + n) + f) * h->n2 * sizeof(RC));
+}
+template <class RC> RC *rec_rtp(moist_handle *h, int st, int n) { return rec_sub<RC>(h, st, n, 0); }
+template <class RC> RC *rec_rup(moist_handle *h, int st, int n) { return rec_sub<RC>(h, st, n, 1); }
+template <class RC> RC *rec_rwp(moist_handle *h, int st, int n) { return rec_sub<RC>(h, st, n, 2); }
 double *wk(moist_handle *h, int w) { return h->adj->wk + (size_t)w * h->n2; }
-// FP64 slots of the recording forward pass
-double *slot64(moist_handle *h, int s) { return h->adj->pool64 + (size_t)s * h->n2; }
-Work<double> work64(moist_handle *h) { return work<double>(h, h->adj->pool64, h->adj->pool1_64); }
 
-AdjW adj_w(moist_handle *h, int st) {
-    AdjW a;
-    a.theta = rec_st(h, st, REC_TH);
-    a.pres = rec_st(h, st, REC_PRES);
-    a.rt2 = rec_st(h, st, REC_RT2);
-    a.R_rho = rec_st(h, st, REC_RR);
-    a.R_rhou = rec_st(h, st, REC_RRU);
-    a.R_rhow = rec_st(h, st, REC_RRW);
-    a.R_rt = rec_st(h, st, REC_RRT);
-    a.mom_x = rec_st(h, st, REC_MX);
-    a.mom_z = rec_st(h, st, REC_MZ);
+template <class RC> AdjW<RC> adj_w(moist_handle *h, int st) {
+    AdjW<RC> a;
+    a.theta = rec_st<RC>(h, st, REC_TH);
+    a.pres = rec_st<RC>(h, st, REC_PRES);
+    a.rt2 = rec_st<RC>(h, st, REC_RT2);
+    a.R_rho = rec_st<RC>(h, st, REC_RR);
+    a.R_rhou = rec_st<RC>(h, st, REC_RRU);
+    a.R_rhow = rec_st<RC>(h, st, REC_RRW);
+    a.R_rt = rec_st<RC>(h, st, REC_RRT);
+    a.mom_x = rec_st<RC>(h, st, REC_MX);
+    a.mom_z = rec_st<RC>(h, st, REC_MZ);
     a.Lth = wk(h, W_TH);
     a.Lrt = wk(h, W_RT);
     a.Lpres = wk(h, W_PRES);
@@ -791,129 +806,165 @@ void fill(moist_handle *h, double *p, int64_t n, double v = 0.0) {
     hipLaunchKernelGGL(ka_fill, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0, h->stream, p, n, v);
 }
 
-void copy_set(moist_handle *h, double *dst, const double *src, size_t n) {
-    hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, h->stream);
+// Zeroes the arrays of `ps` (each `len` doubles) in one launch.
+void fill_zero(moist_handle *h, std::initializer_list<double *> ps, int64_t len) {
+    FillList L{};
+    L.n = 0;
+    L.len = len;
+    for (double *p : ps) L.p[L.n++] = p;
+    const int64_t blocks = std::min<int64_t>((L.n * len + 255) / 256, 4096);
+    hipLaunchKernelGGL(ka_fill_many, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0, h->stream, L);
 }
 
-// One stage of the forward pass with every intermediate the adjoint reads
-// kept in the stage record (same kernels as enqueue_stage).
-void enqueue_stage_rec(moist_handle *h, int st, SSet<double> in, SSet<double> out, SSet<double> s0, int div) {
+void copy_bytes(moist_handle *h, void *dst, const void *src, size_t n) {
+    hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, h->stream);
+}
+void copy_set(moist_handle *h, double *dst, const double *src, size_t n) { copy_bytes(h, dst, src, n * sizeof(double)); }
+
+// One stage of the forward pass in the handle's form S with every
+// intermediate the adjoint reads kept in the stage record (the kernels of
+// enqueue_stage, their work arrays pointed into the record).
+template <class S> void enqueue_stage_rec(moist_handle *h, int st, SSet<S> in, SSet<S> out, SSet<S> s0, int div) {
     const int nsub = h->ns_stage[st];
     const dim3 b2(TILE_X, TILE_Z);
     const ZVec zv = zvec(h);
-    Work<double> w = work64(h);
-    w.theta = rec_st(h, st, REC_TH);
-    w.pres = rec_st(h, st, REC_PRES);
-    w.rt2 = rec_st(h, st, REC_RT2);
-    w.R_rho = rec_st(h, st, REC_RR);
-    w.R_rhou = rec_st(h, st, REC_RRU);
-    w.R_rhow = rec_st(h, st, REC_RRW);
-    w.R_rt = rec_st(h, st, REC_RRT);
-    w.mom_x = rec_st(h, st, REC_MX);
-    w.mom_z = rec_st(h, st, REC_MZ);
-    w.rup = rec_rup(h, st, 0);
-    hipLaunchKernelGGL(k_diag<double>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, w);
-    hipLaunchKernelGGL(k_slow<double>, dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles, zv, in,
-                       s0, w, rec_rtp(h, st, 0), 0);
-    copy_set(h, rec_rwp(h, st, 0), w.rwp, h->n2);
+    Work<S> w = work<S>(h);
+    w.theta = rec_st<S>(h, st, REC_TH);
+    w.pres = rec_st<S>(h, st, REC_PRES);
+    w.rt2 = rec_st<S>(h, st, REC_RT2);
+    w.R_rho = rec_st<S>(h, st, REC_RR);
+    w.R_rhou = rec_st<S>(h, st, REC_RRU);
+    w.R_rhow = rec_st<S>(h, st, REC_RRW);
+    w.R_rt = rec_st<S>(h, st, REC_RRT);
+    w.mom_x = rec_st<S>(h, st, REC_MX);
+    w.mom_z = rec_st<S>(h, st, REC_MZ);
+    w.rup = rec_rup<S>(h, st, 0);
+    w.rwp = rec_rwp<S>(h, st, 0);  // rho_w' of every substep goes straight to its record
+    hipLaunchKernelGGL(k_diag<S>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, w);
+    hipLaunchKernelGGL(k_slow<S>, dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles, zv, in,
+                       s0, w, rec_rtp<S>(h, st, 0), 0);
     for (int n = 0; n < nsub; ++n) {
-        AcBuf<double> ab;
-        ab.rup_in = rec_rup(h, st, n);
-        ab.rup_out = rec_rup(h, st, n + 1);
-        ab.rtp_cur = rec_rtp(h, st, n);
-        ab.rtp_old = rec_rtp(h, st, n > 0 ? n - 1 : 0);
-        ab.rtp_out = rec_rtp(h, st, n + 1);
+        AcBuf<S> ab;
+        ab.rup_in = rec_rup<S>(h, st, n);
+        ab.rup_out = rec_rup<S>(h, st, n + 1);
+        ab.rtp_cur = rec_rtp<S>(h, st, n);
+        ab.rtp_old = rec_rtp<S>(h, st, n > 0 ? n - 1 : 0);
+        ab.rtp_out = rec_rtp<S>(h, st, n + 1);
         ab.cin = h->cin;
         ab.cin_stride = h->nxs;
-        launch_acoustic<double>(h, zv, in, w, ab, div, 0.0, nsub, n == 0 ? 1 : 0);
-        copy_set(h, rec_rwp(h, st, n + 1), w.rwp, h->n2);
+        ab.rwp_in = rec_rwp<S>(h, st, n);
+        w.rwp = rec_rwp<S>(h, st, n + 1);
+        launch_acoustic<S>(h, zv, in, w, ab, div, 0.0, nsub, n == 0 ? 1 : 0);
     }
-    hipLaunchKernelGGL(k_scalar<double>, dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles, zv, in,
-                       s0, out, w, (const double *)rec_rup(h, st, nsub), (const double *)rec_rtp(h, st, nsub), div, 0.0,
+    hipLaunchKernelGGL(k_scalar<S>, dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles, zv, in,
+                       s0, out, w, (const S *)rec_rup<S>(h, st, nsub), (const S *)rec_rtp<S>(h, st, nsub), div, 0.0,
                        st);
 }
 
-// Forward pass of one step from the state `sn` into the current record
-// block: the stage-2 and -3 inputs are kept in the block; stage 3 writes the
-// separate set SC (the step's output).
-void enqueue_step_rec(moist_handle *h, const double *sn) {
+// FP64 form: forward pass of one step from the state `sn` into the current
+// record block; the stage-2 and -3 inputs are kept in the block; stage 3
+// writes the separate set SC (the step's output).
+void enqueue_step_rec64(moist_handle *h, const double *sn) {
     const size_t n2 = h->n2;
-    copy_set(h, slot64(h, S0_), sn, 7 * n2);  // slots S0_..S0_+6 are contiguous
+    copy_set(h, slot<double>(h, S0_), sn, 7 * n2);  // slots S0_..S0_+6 are contiguous
     const ZVec zv = zvec(h);
-    const Work<double> w = work64(h);
-    const SSet<double> s0 = dset(slot64(h, S0_), n2), ia = dset(rec_in(h, 1), n2), ib = dset(rec_in(h, 2), n2);
+    const Work<double> w = work<double>(h);
+    const SSet<double> s0 = sset<double>(h, S0_), ia = dset(rec_in(h, 1), n2), ib = dset(rec_in(h, 2), n2);
     hipLaunchKernelGGL(k_physics<double>, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d,
                        h->tiles2d, zv, s0, w, 0);
-    enqueue_stage_rec(h, 0, s0, ia, s0, 3);
-    enqueue_stage_rec(h, 1, ia, ib, s0, 2);
-    enqueue_stage_rec(h, 2, ib, dset(h->adj->SC, n2), s0, 1);
+    enqueue_stage_rec<double>(h, 0, s0, ia, s0, 3);
+    enqueue_stage_rec<double>(h, 1, ia, ib, s0, 2);
+    enqueue_stage_rec<double>(h, 2, ib, dset(h->adj->SC, n2), s0, 1);
 }
 
-// Adjoint of one step from its record (the current block): A.lam holds the
-// adjoint of the step output on entry and of the step input `sn` on exit.
-void enqueue_step_adj(moist_handle *h, const double *sn) {
+// Mixed form: one step of the mixed form from the handle's state, recording
+// into the current block: FP32 stage and substep records, the stage-2 and -3
+// inputs widened to FP64; the step's output replaces the state, as a plain
+// step does.
+void enqueue_step_rec_mixed(moist_handle *h) {
+    const size_t n2 = h->n2;
+    const ZVec zv = zvec(h);
+    const dim3 b2(TILE_X, TILE_Z);
+    const Work<float> w = work<float>(h);
+    const SSet<float> s0 = sset<float>(h, S0_), sa = sset<float>(h, SA_), sb = sset<float>(h, SB_);
+    hipLaunchKernelGGL(k_physics<float>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, s0, w, 0);
+    enqueue_stage_rec<float>(h, 0, s0, sa, s0, 3);
+    hipLaunchKernelGGL(ka_widen, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, sa,
+                       dset(rec_in(h, 1), n2));
+    enqueue_stage_rec<float>(h, 1, sa, sb, s0, 2);
+    hipLaunchKernelGGL(ka_widen, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, sb,
+                       dset(rec_in(h, 2), n2));
+    enqueue_stage_rec<float>(h, 2, sb, s0, s0, 1);
+}
+
+// Adjoint of one step from its record (the current block, records of type
+// RC): A.lam holds the adjoint of the step output on entry and of the step
+// input `sn` (FP64) on exit.
+template <class RC> void enqueue_step_adj(moist_handle *h, const double *sn) {
     AdjState &A = *h->adj;
     const size_t n2 = h->n2;
     const ZVec zv = zvec(h);
     const dim3 b2(TILE_X, TILE_Z);
-    fill(h, A.ls0, 7 * n2);
-    fill(h, A.lph, 7 * n2);
+    fill_zero(h, {A.ls0, A.lph}, 7 * n2);
     const SSet<double> ls0 = dset(A.ls0, n2), lph = dset(A.lph, n2);
     const int divs[3] = {3, 2, 1};
     double *lo = A.lam, *li = A.lam2;
     for (int st = 2; st >= 0; --st) {
         const SSet<double> in = dset(st == 0 ? const_cast<double *>(sn) : rec_in(h, st), n2);
         const int nsub = h->ns_stage[st], div = divs[st];
-        const AdjW a = adj_w(h, st);
-        for (int wi : {W_TH, W_RT, W_LRR, W_LRU, W_LRW, W_LRT, W_PEND}) fill(h, wk(h, wi), n2);
+        const AdjW<RC> a = adj_w<RC>(h, st);
+        fill_zero(h, {wk(h, W_TH), wk(h, W_RT), wk(h, W_LRR), wk(h, W_LRU), wk(h, W_LRW), wk(h, W_LRT), wk(h, W_PEND)},
+                  n2);
         double *Lrup[2] = {wk(h, W_LRUP0), wk(h, W_LRUP1)};
-        hipLaunchKernelGGL(ka_pre, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, a);
-        hipLaunchKernelGGL(ka_scalar, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, dset(lo, n2),
-                           dset(li, n2), ls0, lph, a, Lrup[0], div);
-        hipLaunchKernelGGL(ka_lin, dim3(h->nm), dim3(256), 0, h->stream, h->dom_d, a, dset(li, n2));
+        hipLaunchKernelGGL(ka_pre<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, a);
+        hipLaunchKernelGGL(ka_scalar<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in,
+                           dset(lo, n2), dset(li, n2), ls0, lph, a, Lrup[0], div);
+        hipLaunchKernelGGL(ka_lin<RC>, dim3(h->nm), dim3(256), 0, h->stream, h->dom_d, a, dset(li, n2));
         int cur = 0;
         for (int n = nsub; n >= 1; --n) {
             const int m = n - 1;
-            hipLaunchKernelGGL(ka_ac_v1, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, a,
-                               (const double *)rec_rwp(h, st, m + 1), div, nsub);
+            hipLaunchKernelGGL(ka_ac_v1<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, a,
+                               (const RC *)rec_rwp<RC>(h, st, m + 1), div, nsub);
+            const dim3 gs((h->nwgroups + 3) / 4), bs(256);
             if (h->wch == 4)
-                hipLaunchKernelGGL(ka_ac_solve<4>, dim3((h->nwgroups + 3) / 4), dim3(256), 0, h->stream, h->dom_d,
-                                   h->wgroups, h->nwgroups, zv, a, div, nsub, n == nsub ? 1 : 0);
+                hipLaunchKernelGGL((ka_ac_solve<4, RC>), gs, bs, 0, h->stream, h->dom_d, h->wgroups, h->nwgroups, zv,
+                                   a, div, nsub, n == nsub ? 1 : 0);
             else if (h->wch == 64)
-                hipLaunchKernelGGL(ka_ac_solve<64>, dim3((h->nwgroups + 3) / 4), dim3(256), 0, h->stream, h->dom_d,
-                                   h->wgroups, h->nwgroups, zv, a, div, nsub, n == nsub ? 1 : 0);
+                hipLaunchKernelGGL((ka_ac_solve<64, RC>), gs, bs, 0, h->stream, h->dom_d, h->wgroups, h->nwgroups,
+                                   zv, a, div, nsub, n == nsub ? 1 : 0);
             else
-                hipLaunchKernelGGL(ka_ac_solve<8>, dim3((h->nwgroups + 3) / 4), dim3(256), 0, h->stream, h->dom_d,
-                                   h->wgroups, h->nwgroups, zv, a, div, nsub, n == nsub ? 1 : 0);
-            hipLaunchKernelGGL(ka_ac_v2, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, a,
-                               (const double *)rec_rtp(h, st, m), (const double *)rec_rup(h, st, m + 1),
-                               (const double *)rec_rwp(h, st, m), (const double *)rec_rwp(h, st, m + 1), div, nsub);
-            hipLaunchKernelGGL(ka_ac_hor, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, a,
-                               (const double *)Lrup[cur], Lrup[1 - cur], (const double *)rec_rtp(h, st, m),
-                               (const double *)rec_rtp(h, st, m > 0 ? m - 1 : 0),
-                               (const double *)rec_rup(h, st, m + 1), div, nsub, n == 1 ? 1 : 0);
+                hipLaunchKernelGGL((ka_ac_solve<8, RC>), gs, bs, 0, h->stream, h->dom_d, h->wgroups, h->nwgroups, zv,
+                                   a, div, nsub, n == nsub ? 1 : 0);
+            hipLaunchKernelGGL(ka_ac_v2<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, a,
+                               (const RC *)rec_rtp<RC>(h, st, m), (const RC *)rec_rup<RC>(h, st, m + 1),
+                               (const RC *)rec_rwp<RC>(h, st, m), (const RC *)rec_rwp<RC>(h, st, m + 1), div, nsub);
+            hipLaunchKernelGGL(ka_ac_hor<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, a,
+                               (const double *)Lrup[cur], Lrup[1 - cur], (const RC *)rec_rtp<RC>(h, st, m),
+                               (const RC *)rec_rtp<RC>(h, st, m > 0 ? m - 1 : 0),
+                               (const RC *)rec_rup<RC>(h, st, m + 1), div, nsub, n == 1 ? 1 : 0);
             cur = 1 - cur;
         }
-        hipLaunchKernelGGL(ka_slow, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, dset(li, n2),
-                           ls0, lph, a, (const double *)Lrup[cur]);
-        hipLaunchKernelGGL(ka_diag, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, dset(li, n2), a);
+        hipLaunchKernelGGL(ka_slow<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in,
+                           dset(li, n2), ls0, lph, a, (const double *)Lrup[cur]);
+        hipLaunchKernelGGL(ka_diag<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in,
+                           dset(li, n2), a);
         std::swap(lo, li);
     }
     // lo: adjoint of the stage-1 input; the step input also enters as s0 of every stage and through physics
-    hipLaunchKernelGGL(ka_axpy, dim3(1024), dim3(256), 0, h->stream, lo, (const double *)A.ls0, (int64_t)(7 * n2));
     hipLaunchKernelGGL(ka_physics, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv,
-                       dset(const_cast<double *>(sn), n2), lph, dset(lo, n2), A.gk, A.gk + n2, A.gk + 2 * n2);
-    if (lo != A.lam) copy_set(h, A.lam, lo, 7 * n2);
+                       dset(const_cast<double *>(sn), n2), lph, dset(lo, n2), ls0, A.gk, A.gk + n2, A.gk + 2 * n2);
+    // the buffer holding the step input's adjoint becomes A.lam (no copy)
+    if (lo != A.lam) std::swap(A.lam, A.lam2);
 }
 
 // Checkpoints hold the state slots (S0_.. S0_+6, contiguous at the front of
 // the pool) in the handle's form, and the accumulated rain.
 void save_state(moist_handle *h, double *dst) {
-    hipMemcpyAsync(dst, h->pool, 7 * h->n2 * h->elem, hipMemcpyDeviceToDevice, h->stream);
+    copy_bytes(h, dst, h->pool, 7 * h->n2 * h->elem);
     copy_set(h, dst + h->adj->cp_words, h->prec, h->nxs);
 }
 void load_state(moist_handle *h, const double *src) {
-    hipMemcpyAsync(h->pool, src, 7 * h->n2 * h->elem, hipMemcpyDeviceToDevice, h->stream);
+    copy_bytes(h, h->pool, src, 7 * h->n2 * h->elem);
     copy_set(h, h->prec, src + h->adj->cp_words, h->nxs);
 }
 // The current state as a full FP64 state (7 fields) at `dst`.
@@ -924,12 +975,36 @@ void state64(moist_handle *h, double *dst) {
     else
         copy_set(h, dst, slot<double>(h, S0_), 7 * h->n2);
 }
+// Step input j of the segment: the state slots in the handle's form.
+void *seg_step(moist_handle *h, int j) { return h->adj->seg + h->adj->seg_bytes * (size_t)j; }
+// Reverse step j of the segment (its record block is current): FP64 input
+// as stored, or the mixed input widened to FP64.
+void reverse_step(moist_handle *h, int j) {
+    AdjState &A = *h->adj;
+    if (h->precision == MOIST_MIXED) {
+        hipLaunchKernelGGL(ka_widen, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d, h->tiles2d,
+                           zvec(h), fset((float *)seg_step(h, j), h->n2), dset(A.snb, h->n2));
+        enqueue_step_adj<float>(h, A.snb);
+    } else {
+        enqueue_step_adj<double>(h, (const double *)seg_step(h, j));
+    }
+}
+// Recording pass of step j of the segment (the mixed form from the handle's
+// state, which the pass advances; the FP64 form from the stored input, and
+// with `advance` its output becomes the state).
+void record_step(moist_handle *h, int j, bool advance) {
+    if (h->precision == MOIST_MIXED) {
+        enqueue_step_rec_mixed(h);
+    } else {
+        enqueue_step_rec64(h, (const double *)seg_step(h, j));
+        if (advance) copy_set(h, slot<double>(h, S0_), h->adj->SC, 7 * h->n2);
+    }
+}
 
 template <class H> int adjoint_run_t(moist_handle *h, int nsteps, int ckpt, const void *const *tgt, double *J) {
     if (adj_alloc(h, nsteps, ckpt)) return -1;
     AdjState &A = *h->adj;
     const size_t n2 = h->n2;
-    const bool mixed = h->precision == MOIST_MIXED;
     // theta targets
     for (int m = 0; m < h->nm; ++m) {
         const Dom &D = h->dom_h[m];
@@ -961,36 +1036,32 @@ template <class H> int adjoint_run_t(moist_handle *h, int nsteps, int ckpt, cons
     hipLaunchKernelGGL(ka_loss, dim3(h->nm), dim3(256), 0, h->stream, h->dom_d, dset(A.SC, n2),
                        (const double *)h->prec, (const double *)A.tgt, A.lam + RHO * n2, A.lam + RT * n2, A.J);
     // reverse sweep over the segments
+    const size_t state_bytes = 7 * n2 * h->elem;
     for (int sg = A.ncp - 1; sg >= 0; --sg) {
         const int n0 = sg * ckpt, n1 = std::min(nsteps, n0 + ckpt);
         load_state(h, A.cp + A.cp_stride * sg);
         if (A.segrec) {
             // one recording pass over the segment, then the reverse sweep from the records
             for (int n = n0; n < n1; ++n) {
-                double *sn = A.seg + (size_t)(n - n0) * 7 * n2;
-                state64(h, sn);
-                A.cur = A.recs + A.rb_stride * (size_t)(n - n0);
-                enqueue_step_rec(h, sn);
-                // next state: the recorded step's output (FP64), or a step of the mixed form
-                if (!mixed)
-                    copy_set(h, slot<double>(h, S0_), A.SC, 7 * n2);
-                else if (n + 1 < n1 && run_steps(h, 1))
-                    return -1;
+                copy_bytes(h, seg_step(h, n - n0), h->pool, state_bytes);
+                A.cur = A.recs + A.rb_bytes * (size_t)(n - n0);
+                record_step(h, n - n0, true);
             }
             for (int n = n1 - 1; n >= n0; --n) {
-                A.cur = A.recs + A.rb_stride * (size_t)(n - n0);
-                enqueue_step_adj(h, A.seg + (size_t)(n - n0) * 7 * n2);
+                A.cur = A.recs + A.rb_bytes * (size_t)(n - n0);
+                reverse_step(h, n - n0);
             }
         } else {
             // states of the segment, then per step (in reverse) its record and adjoint
             for (int n = n0; n < n1; ++n) {
-                state64(h, A.seg + (size_t)(n - n0) * 7 * n2);
+                copy_bytes(h, seg_step(h, n - n0), h->pool, state_bytes);
                 if (n + 1 < n1 && run_steps(h, 1)) return -1;
             }
             A.cur = A.recs;
             for (int n = n1 - 1; n >= n0; --n) {
-                enqueue_step_rec(h, A.seg + (size_t)(n - n0) * 7 * n2);
-                enqueue_step_adj(h, A.seg + (size_t)(n - n0) * 7 * n2);
+                if (h->precision == MOIST_MIXED) copy_bytes(h, h->pool, seg_step(h, n - n0), state_bytes);
+                record_step(h, n - n0, false);
+                reverse_step(h, n - n0);
             }
         }
         HIPCHK(h, hipGetLastError());
@@ -1407,12 +1478,12 @@ int moist_autotune(moist_handle *h, int reps, int *levels, int *threads, double 
                     if (h->precision == MOIST_MIXED) {
                         const Work<float> w = work<float>(h);
                         AcBuf<float> ab{slot<float>(h, RUP), slot<float>(h, RUP2), slot<float>(h, RTP0),
-                                        slot<float>(h, RTP0), slot<float>(h, RTP1), h->cin, h->nxs};
+                                        slot<float>(h, RTP0), slot<float>(h, RTP1), h->cin, h->nxs, w.rwp};
                         launch_acoustic<float>(h, zv, sset<float>(h, S0_), w, ab, 1, 0.0, 1, n == 0 ? 1 : 0);
                     } else {
                         const Work<double> w = work<double>(h);
                         AcBuf<double> ab{slot<double>(h, RUP), slot<double>(h, RUP2), slot<double>(h, RTP0),
-                                         slot<double>(h, RTP0), slot<double>(h, RTP1), h->cin, h->nxs};
+                                         slot<double>(h, RTP0), slot<double>(h, RTP1), h->cin, h->nxs, w.rwp};
                         launch_acoustic<double>(h, zv, sset<double>(h, S0_), w, ab, 1, 0.0, 1, n == 0 ? 1 : 0);
                     }
                 }
