@@ -188,10 +188,10 @@ int moist_set_acoustic_shape(moist_handle *h, int levels, int threads);
  * step's acoustic substeps.  theta_target[m]: host array (ka, ia) in the
  * reference layout (halo columns ignored).  J[m] receives the loss.  The
  * handle's state afterwards is the final forward state.  The trajectory
- * (forward pass, checkpoints, segment recomputation) advances in the
- * handle's form; each step is recorded and reversed in FP64 from that
- * trajectory's state, so a mixed-form handle gives the FP64 step's adjoint
- * along its mixed-form trajectory (checkpoints take half the memory). */
+ * (forward pass, checkpoints, segment recomputation) and the forward records
+ * are in the handle's form (FP32 records in the mixed form); the adjoint
+ * arithmetic is FP64 in both, so a mixed-form handle returns the gradient of
+ * its own loss, linearised at the mixed step's stage values. */
 int moist_adjoint_run(moist_handle *h, int nsteps, int ckpt, const void *const *theta_target, int elem_bytes,
                       double *J);
 /* dJ/d(initial rho, rho_u, rho_w, rho_theta, rho_qv, rho_qc, rho_qr) of one

@@ -172,8 +172,8 @@ class Engine:
 
     def adjoint(self, nsteps, theta_target, ckpt=50):
         """Loss and reverse sweep over `nsteps` large steps from the uploaded
-        state.  The trajectory advances in the handle's form (FP64 or mixed);
-        each step is linearised in FP64 about that trajectory's state.
+        state.  The trajectory and the forward records are in the handle's form
+        (FP64, or mixed with FP32 records); the adjoint arithmetic is FP64.
         theta_target: one (ka, ia) array per member.
         Returns J per member; gradients come from `adjoint_grad`."""
         tt = [np.asfortranarray(t, dtype=np.float64) for t in theta_target]
