@@ -268,21 +268,29 @@ int set_acoustic_shape(moist_handle *h, int L, int NT, int W = 1) {
     return 0;
 }
 
-template <class S, int L, int NT, int W>
+template <class S, bool P, int L, int NT, int W>
 void launch_ac(moist_handle *h, const ZVec &zv, const SSet<S> &in, const Work<S> &w, const AcBuf<S> &ab, int div,
                double dt_ovr, int nsub, int first) {
     // dynamic LDS: the per-level geometry (5 doubles per level of the tallest member)
-    hipLaunchKernelGGL((k_acoustic<S, L, NT, W>), dim3(h->natiles), dim3(NT), (unsigned)(40 * h->max_ka), h->stream,
-                       h->dom_d, h->atiles, zv, in, w, ab, div, dt_ovr, nsub, first);
+    if constexpr (P) {
+        static_assert(is_mixed<S>(), "the level-pair layout is the mixed form's");
+        hipLaunchKernelGGL((k_acoustic_pairs<L, NT, W>), dim3(h->natiles), dim3(NT), (unsigned)(40 * h->max_ka),
+                           h->stream, h->dom_d, h->atiles, zv, in, w, ab, div, dt_ovr, nsub, first);
+    } else {
+        hipLaunchKernelGGL((k_acoustic<S, L, NT, W>), dim3(h->natiles), dim3(NT), (unsigned)(40 * h->max_ka),
+                           h->stream, h->dom_d, h->atiles, zv, in, w, ab, div, dt_ovr, nsub, first);
+    }
 }
 
-template <class S>
+// P: layout of the work fields (level pairs on the mixed form's forward
+// path, x-fastest in the FP64 form and the adjoint's recording pass)
+template <class S, bool P>
 void launch_acoustic(moist_handle *h, const ZVec &zv, const SSet<S> &in, const Work<S> &w, const AcBuf<S> &ab,
                      int div, double dt_ovr, int nsub, int first) {
     const int key = (h->ac_L * 10000 + h->ac_NT) * 10 + (h->ac_W == 8 ? 8 : 1);
 #define AC_CASE(L_, NT_, W_)                                                                 \
     case (L_ * 10000 + NT_) * 10 + W_:                                                       \
-        launch_ac<S, L_, NT_, W_>(h, zv, in, w, ab, div, dt_ovr, nsub, first);               \
+        launch_ac<S, P, L_, NT_, W_>(h, zv, in, w, ab, div, dt_ovr, nsub, first);            \
         break;
     switch (key) {
         AC_CASE(2, 256, 1)
@@ -294,7 +302,7 @@ void launch_acoustic(moist_handle *h, const ZVec &zv, const SSet<S> &in, const W
         AC_CASE(2, 1024, 8)
         AC_CASE(4, 512, 8)
         AC_CASE(4, 1024, 8)
-    default: launch_ac<S, 4, 1024, 1>(h, zv, in, w, ab, div, dt_ovr, nsub, first); break;
+    default: launch_ac<S, P, 4, 1024, 1>(h, zv, in, w, ab, div, dt_ovr, nsub, first); break;
     }
 #undef AC_CASE
 }
@@ -309,9 +317,11 @@ void enqueue_stage(moist_handle *h, int in_set, int out_set, int s0_set, int div
     const SSet<S> in = sset<S>(h, in_set), out = sset<S>(h, out_set), s0 = sset<S>(h, s0_set);
     S *rtp[3] = {slot<S>(h, RTP0), slot<S>(h, RTP1), slot<S>(h, RTP2)};
     S *rup[2] = {slot<S>(h, RUP), slot<S>(h, RUP2)};
-    hipLaunchKernelGGL(k_diag<S>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, w);
-    hipLaunchKernelGGL(k_slow<S>, dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles, zv, in, s0,
-                       w, rtp[0], u_phys_2d);
+    // the work fields of the mixed form are in the level-pair layout
+    constexpr bool PL = is_mixed<S>();
+    hipLaunchKernelGGL((k_diag<S, PL>), dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, w);
+    hipLaunchKernelGGL((k_slow<S, PL>), dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles, zv,
+                       in, s0, w, rtp[0], u_phys_2d);
     // rho_theta' rotates over three buffers (current, previous, next) and
     // rho_u' over two, because every column reads its neighbours' values
     int cur = 0, old = 0, ru = 0;
@@ -332,15 +342,14 @@ void enqueue_stage(moist_handle *h, int in_set, int out_set, int s0_set, int div
             e1 = next_event(h);
             if (e0) hipEventRecord(e0, h->stream);
         }
-        launch_acoustic<S>(h, zv, in, w, ab, div, dt_ovr, nsub, n == 0 ? 1 : 0);
+        launch_acoustic<S, PL>(h, zv, in, w, ab, div, dt_ovr, nsub, n == 0 ? 1 : 0);
         if (h->ev_on && e1) hipEventRecord(e1, h->stream);
         old = cur;
         cur = nxt;
         ru = 1 - ru;
     }
-    hipLaunchKernelGGL(k_scalar<S>, dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles, zv, in,
-                       s0, out, w,
-                       (const S *)rup[ru], (const S *)rtp[cur], div, dt_ovr, stage);
+    hipLaunchKernelGGL((k_scalar<S, PL>), dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles,
+                       zv, in, s0, out, w, (const S *)rup[ru], (const S *)rtp[cur], div, dt_ovr, stage);
 }
 
 template <class S> void enqueue_step(moist_handle *h) {
@@ -840,9 +849,9 @@ template <class S> void enqueue_stage_rec(moist_handle *h, int st, SSet<S> in, S
     w.mom_z = rec_st<S>(h, st, REC_MZ);
     w.rup = rec_rup<S>(h, st, 0);
     w.rwp = rec_rwp<S>(h, st, 0);  // rho_w' of every substep goes straight to its record
-    hipLaunchKernelGGL(k_diag<S>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, w);
-    hipLaunchKernelGGL(k_slow<S>, dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles, zv, in,
-                       s0, w, rec_rtp<S>(h, st, 0), 0);
+    hipLaunchKernelGGL((k_diag<S, false>), dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, w);
+    hipLaunchKernelGGL((k_slow<S, false>), dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles, zv,
+                       in, s0, w, rec_rtp<S>(h, st, 0), 0);
     for (int n = 0; n < nsub; ++n) {
         AcBuf<S> ab;
         ab.rup_in = rec_rup<S>(h, st, n);
@@ -854,11 +863,11 @@ template <class S> void enqueue_stage_rec(moist_handle *h, int st, SSet<S> in, S
         ab.cin_stride = h->nxs;
         ab.rwp_in = rec_rwp<S>(h, st, n);
         w.rwp = rec_rwp<S>(h, st, n + 1);
-        launch_acoustic<S>(h, zv, in, w, ab, div, 0.0, nsub, n == 0 ? 1 : 0);
+        launch_acoustic<S, false>(h, zv, in, w, ab, div, 0.0, nsub, n == 0 ? 1 : 0);
     }
-    hipLaunchKernelGGL(k_scalar<S>, dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles, zv, in,
-                       s0, out, w, (const S *)rec_rup<S>(h, st, nsub), (const S *)rec_rtp<S>(h, st, nsub), div, 0.0,
-                       st);
+    hipLaunchKernelGGL((k_scalar<S, false>), dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles,
+                       zv, in, s0, out, w, (const S *)rec_rup<S>(h, st, nsub), (const S *)rec_rtp<S>(h, st, nsub), div,
+                       0.0, st);
 }
 
 // FP64 form: forward pass of one step from the state `sn` into the current
@@ -1160,7 +1169,9 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
             zfh.push_back(p->zf[k]);
             dzh.push_back(p->dz[k]);
         }
-        off2 += (int64_t)p->nx * p->nz;
+        // an odd nz is padded by one level: the level-pair layout of the
+        // acoustic's work fields stores the top level with a partner
+        off2 += (int64_t)p->nx * (p->nz + (p->nz & 1));
         offx += p->nx;
         offz += p->nz;
         h->max_ka = std::max(h->max_ka, p->nz);
@@ -1479,12 +1490,13 @@ int moist_autotune(moist_handle *h, int reps, int *levels, int *threads, double 
                         const Work<float> w = work<float>(h);
                         AcBuf<float> ab{slot<float>(h, RUP), slot<float>(h, RUP2), slot<float>(h, RTP0),
                                         slot<float>(h, RTP0), slot<float>(h, RTP1), h->cin, h->nxs, w.rwp};
-                        launch_acoustic<float>(h, zv, sset<float>(h, S0_), w, ab, 1, 0.0, 1, n == 0 ? 1 : 0);
+                        launch_acoustic<float, true>(h, zv, sset<float>(h, S0_), w, ab, 1, 0.0, 1, n == 0 ? 1 : 0);
                     } else {
                         const Work<double> w = work<double>(h);
                         AcBuf<double> ab{slot<double>(h, RUP), slot<double>(h, RUP2), slot<double>(h, RTP0),
                                          slot<double>(h, RTP0), slot<double>(h, RTP1), h->cin, h->nxs, w.rwp};
-                        launch_acoustic<double>(h, zv, sset<double>(h, S0_), w, ab, 1, 0.0, 1, n == 0 ? 1 : 0);
+                        launch_acoustic<double, false>(h, zv, sset<double>(h, S0_), w, ab, 1, 0.0, 1,
+                                                       n == 0 ? 1 : 0);
                     }
                 }
             };

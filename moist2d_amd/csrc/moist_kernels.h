@@ -44,6 +44,57 @@ template <class S> struct Work {
 
 template <class S> constexpr bool is_mixed() { return sizeof(S) == 4; }
 
+// Layout of the fields the acoustic substeps stream (theta, rt2pres, the four
+// slow tendencies, rho', rho_u', rho_w', rho_theta', mom_x, mom_z).  P = true
+// is the level-pair layout: levels 2m and 2m+1 of a column are adjacent, cell
+// (k, i) of a member at off2 + ((k >> 1) nx + i) 2 + (k & 1).  A lane that
+// holds two consecutive levels moves them with one 8-byte (FP32) or 16-byte
+// (FP64) access, and 16 consecutive columns of a level pair fill a 128-byte
+// line, where an x-fastest row of 16 FP32 columns fills half of one.  The
+// mixed form's forward path uses it; P = false is the x-fastest layout of
+// every other pool, of the FP64 form (whose 16-column rows already fill a
+// line) and of the adjoint's recording pass.  Member offsets are even and
+// an odd nz is padded by one level, so the pair of the top level stays
+// inside its member.
+template <bool P> __device__ __forceinline__ int64_t widx(const Dom &D, int k, int i) {
+    if constexpr (P) return D.off2 + ((int64_t)(k >> 1) * D.nx + i) * 2 + (k & 1);
+    else return D.off2 + (int64_t)k * D.nx + i;
+}
+template <class S> struct Vec2;
+template <> struct Vec2<float> { using T = float2; };
+template <> struct Vec2<double> { using T = double2; };
+// levels k and k+1 (k even) of column i; x-fastest reads level k+1 only when
+// it exists.  In FP32 the pair is read as one relaxed wavefront-scope atomic
+// 8-byte load: the compiler splits a plain float2 load whose halves are used
+// apart into two 4-byte loads, and an atomic load is never split.
+template <bool P, class S> __device__ __forceinline__ void ld2(const S *p, const Dom &D, int k, int i, S &a, S &b) {
+    if constexpr (P && sizeof(S) == 4) {
+        const uint64_t u = __hip_atomic_load((const uint64_t *)(p + widx<true>(D, k, i)), __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WAVEFRONT);
+        a = __builtin_bit_cast(float, (uint32_t)u);
+        b = __builtin_bit_cast(float, (uint32_t)(u >> 32));
+    } else if constexpr (P) {
+        const typename Vec2<S>::T v = *(const typename Vec2<S>::T *)(p + widx<true>(D, k, i));
+        a = v.x;
+        b = v.y;
+    } else {
+        a = p[widx<false>(D, k, i)];
+        b = p[widx<false>(D, min(k + 1, D.nz - 1), i)];
+    }
+}
+// stores levels k and k+1 (k even); x-fastest stores level k+1 only when it exists
+template <bool P, class S> __device__ __forceinline__ void st2(S *p, const Dom &D, int k, int i, S a, S b) {
+    if constexpr (P) {
+        typename Vec2<S>::T v;
+        v.x = a;
+        v.y = b;
+        *(typename Vec2<S>::T *)(p + widx<true>(D, k, i)) = v;
+    } else {
+        p[widx<false>(D, k, i)] = a;
+        if (k + 1 < D.nz) p[widx<false>(D, k + 1, i)] = b;
+    }
+}
+
 // Stage time step: dt/3, dt/2, dt in the compute type (moist.jl:1649-1657),
 // or an explicit dt for the stage-level entry.
 template <class R> __device__ __forceinline__ R stage_dt(const Dom &D, int div, double dt_ovr) {
@@ -64,9 +115,10 @@ __device__ __forceinline__ S full_rt(const SSet<S> &s, int64_t c, const ZVec &zv
 
 // =============================================================================
 // Stage diagnostics: theta, pressure (or its deviation), rt2pres, u and w at
-// faces (moist.jl:1048-1062, 1091-1097, 1184-1189).
+// faces (moist.jl:1048-1062, 1091-1097, 1184-1189).  theta and rt2pres are
+// stored in layout P (level pairs when true), the others x-fastest.
 // =============================================================================
-template <class S>
+template <class S, bool P>
 __global__ void __launch_bounds__(256) k_diag(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
                                               ZVec zv, SSet<S> in, Work<S> w) {
     using R = S;
@@ -114,13 +166,14 @@ __global__ void __launch_bounds__(256) k_diag(const Dom *__restrict__ doms, cons
         wf = fdiv(in.f[RHOW][c], f2h(rho, full_rho(in, ckp, zv, D, k + 1), R(dz[k]), R(dz[k + 1])));
     }
     w.pres[c] = pres;
-    w.theta[c] = fdiv(rt, rho);
-    w.rt2[c] = fdiv(fdiv(cp, cv) * pfull, rt);
+    w.theta[widx<P>(D, k, i)] = fdiv(rt, rho);
+    w.rt2[widx<P>(D, k, i)] = fdiv(fdiv(cp, cv) * pfull, rt);
     w.uf[c] = uf;
     w.wf[c] = wf;
 }
 
 #include "moist_acoustic.h"
+#include "moist_acoustic_pairs.h"
 
 #include "moist_scalar.h"
 #include "moist_slow.h"

@@ -1,7 +1,8 @@
 // moist_scalar.h — scalar transport and final update of one RK stage
 // (moist.jl:1469-1539).  Included by moist_kernels.h inside namespace mo.
 //
-// A block owns a tile of ST_X columns x ST_Z levels.  For each of qv, qc, qr
+// A block owns a tile of ST_X columns x ST_Z levels.  The work fields it reads
+// (mom_x, mom_z and the final perturbations) are in layout P.  For each of qv, qc, qr
 // it stages q = rho_q / rho and rho of the tile plus a 2-cell halo in LDS,
 // evaluates every face flux once (Koren advective flux, moist.jl:850-945, and
 // the diffusive flux, moist.jl:1494-1510), and updates each cell from the
@@ -13,7 +14,7 @@
 constexpr int ST_X = 64, ST_Z = 8, ST_THREADS = 256;
 constexpr int ST_HX = ST_X + 4, ST_HZ = ST_Z + 4;  // haloed tile
 
-template <class S>
+template <class S, bool P>
 __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
                                                        ZVec zv, SSet<S> in, SSet<S> s0, SSet<S> out, Work<S> w,
                                                        const S *__restrict__ rup_fin, const S *__restrict__ rtp_fin,
@@ -62,7 +63,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
         const int e = tid + u * ST_THREADS;
         const int r = e / (ST_X + 1), f = e - r * (ST_X + 1);
         const int k = k0 + r;
-        mfx[u] = (e < ST_Z * (ST_X + 1) && k < nz) ? w.mom_x[gidx(k, wrapx(i0 + f - 1, nx))] : R(0.0);
+        mfx[u] = (e < ST_Z * (ST_X + 1) && k < nz) ? w.mom_x[widx<P>(D, k, wrapx(i0 + f - 1, nx))] : R(0.0);
     }
 #pragma unroll
     for (int u = 0; u < NFZ; ++u) {
@@ -70,7 +71,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
         const int g = e / ST_X, c = e - g * ST_X;
         const int kk = k0 + g - 1, i = i0 + c;
         const bool ok = e < (ST_Z + 1) * ST_X && i < nx && kk >= 0 && kk <= ke - 1;
-        mfz[u] = ok ? w.mom_z[gidx(kk, i)] : R(0.0);
+        mfz[u] = ok ? w.mom_z[widx<P>(D, kk, i)] : R(0.0);
         const int kc = ok ? kk : 0, kc1 = ok ? kk + 1 : 0;
         fdz0[u] = R(dzd[kc]);
         fdz1[u] = R(dzd[kc1]);
@@ -106,11 +107,11 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
         const int e = tid + u * ST_THREADS;
         const int r = e / ST_X, c = e - r * ST_X;
         const int k = min(k0 + r, ke), i = min(i0 + c, nx - 1);
-        const int64_t q = gidx(k, i);
-        fin[u][0] = in.f[RHO][q] + w.rp[q];
-        fin[u][1] = in.f[RHOU][q] + rup_fin[q];
-        fin[u][2] = in.f[RT][q] + rtp_fin[q];
-        fin[u][3] = k < ke ? in.f[RHOW][q] + w.rwp[q] : in.f[RHOW][q];
+        const int64_t q = gidx(k, i), qw = widx<P>(D, k, i);
+        fin[u][0] = in.f[RHO][q] + w.rp[qw];
+        fin[u][1] = in.f[RHOU][q] + rup_fin[qw];
+        fin[u][2] = in.f[RT][q] + rtp_fin[qw];
+        fin[u][3] = k < ke ? in.f[RHOW][q] + w.rwp[qw] : in.f[RHOW][q];
     }
 
     const S *dph[3] = {w.d_qv, w.d_qc, w.d_qr};

@@ -18,7 +18,7 @@
 constexpr int SL_HX = ST_X + 4, SL_HZ = ST_Z + 4;
 constexpr int SL_FLUX = 5 * (ST_Z + 1) * (ST_X + 1);  // largest flux phase (w)
 
-template <class S>
+template <class S, bool P>
 __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
                                                      ZVec zv, SSet<S> in, SSet<S> s0, Work<S> w, S *rtp0,
                                                      int u_phys_2d) {
@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
             const int64_t q = gidx(ok ? k : 0, ok ? i : 0);
             v[u][0] = ok ? w.uf[q] : R(0);
             v[u][1] = ok ? w.wf[q] : R(0);
-            v[u][2] = ok ? w.theta[q] : R(0);
+            v[u][2] = ok ? w.theta[widx<P>(D, ok ? k : 0, ok ? i : 0)] : R(0);
             v[u][3] = ok ? full_rho(in, q, zv, D, k) : R(0);
             v[u][4] = ok ? in.f[RHOU][q] : R(0);
             v[u][5] = ok ? in.f[RHOW][q] : R(0);
@@ -87,13 +87,18 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
             (&lRW[0][0])[e] = v[u][5];
         }
     }
+    // interior cells of this thread: levels k0+cr0 and k0+cr0+1 (one level pair)
+    // of column i0+cc0, so its outputs leave as level pairs
+    static_assert(NCL == 2 && ST_Z * ST_X == 2 * ST_THREADS, "two interior cells per thread");
+    const int cr0 = 2 * (tid / ST_X), cc0 = tid % ST_X;
+    const int kc0 = k0 + cr0, ic0 = i0 + cc0;
+    const bool pair_ok = kc0 < nz && ic0 < nx;
     // per interior cell of this thread: the cell phases' global inputs
     R c_drho[NCL], c_drt[NCL], c_rp[NCL], c_rup[NCL], c_rwp[NCL], c_rtp[NCL];
     R c_pres[NCL], c_presx[NCL], c_presz[NCL], c_dup[NCL], c_rho[NCL], c_rhoz[NCL], c_zf[NCL], c_bb[NCL];
 #pragma unroll
     for (int u = 0; u < NCL; ++u) {
-        const int e = tid + u * ST_THREADS;
-        const int r = e / ST_X, c = e - r * ST_X;
+        const int r = cr0 + u, c = cc0;
         const int k = min(k0 + r, ke), i = min(i0 + c, nx - 1);
         const int kz = min(k + 1, ke);
         const int64_t q = gidx(k, i);
@@ -142,28 +147,37 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         FTz[e] = fl;
     }
     __syncthreads();
+    {
+        R o_rr[NCL], o_rt[NCL], o_mx[NCL], o_mz[NCL];
 #pragma unroll
-    for (int u = 0; u < NCL; ++u) {
-        const int e = tid + u * ST_THREADS;
-        const int r = e / ST_X, c = e - r * ST_X;
-        const int k = k0 + r, i = i0 + c;
-        if (e >= ST_Z * ST_X || k >= nz || i >= nx) continue;
-        const int64_t q = gidx(k, i);
-        // R_rho (moist.jl:1079-1086)
-        const R fkp = k == ke ? R(0.0) : RW(k, i);
-        const R fkm = k == 0 ? R(0.0) : RW(k - 1, i);
-        w.R_rho[q] = -(fdiv(RU(k, i) - RU(k, i - 1), dx) + fdiv(fkp - fkm, DZ(k))) + c_drho[u];
-        // R_rho_theta
-        const R adv = -(fdiv(FTx[r * (ST_X + 1) + c + 1] - FTx[r * (ST_X + 1) + c], dx) +
-                        fdiv(FTz[(r + 1) * ST_X + c] - FTz[r * ST_X + c], DZ(k)));
-        w.R_rt[q] = adv + c_drt[u];
-        // perturbations from the step start, mass-flux accumulators (moist.jl:1339-1348)
-        w.rp[q] = c_rp[u];
-        w.rup[q] = c_rup[u];
-        w.rwp[q] = c_rwp[u];
-        rtp0[q] = c_rtp[u];
-        w.mom_x[q] = RU(k, i);
-        w.mom_z[q] = RW(k, i);
+        for (int u = 0; u < NCL; ++u) {
+            const int r = cr0 + u, c = cc0;
+            const int k = k0 + r, i = i0 + c;
+            o_rr[u] = o_rt[u] = o_mx[u] = o_mz[u] = R(0.0);
+            if (k >= nz || i >= nx) continue;
+            // R_rho (moist.jl:1079-1086)
+            const R fkp = k == ke ? R(0.0) : RW(k, i);
+            const R fkm = k == 0 ? R(0.0) : RW(k - 1, i);
+            o_rr[u] = -(fdiv(RU(k, i) - RU(k, i - 1), dx) + fdiv(fkp - fkm, DZ(k))) + c_drho[u];
+            // R_rho_theta
+            const R adv = -(fdiv(FTx[r * (ST_X + 1) + c + 1] - FTx[r * (ST_X + 1) + c], dx) +
+                            fdiv(FTz[(r + 1) * ST_X + c] - FTz[r * ST_X + c], DZ(k)));
+            o_rt[u] = adv + c_drt[u];
+            // mass-flux accumulators start from the stage input (moist.jl:1339-1348)
+            o_mx[u] = RU(k, i);
+            o_mz[u] = RW(k, i);
+        }
+        if (pair_ok) {
+            st2<P>(w.R_rho, D, kc0, ic0, o_rr[0], o_rr[1]);
+            st2<P>(w.R_rt, D, kc0, ic0, o_rt[0], o_rt[1]);
+            // perturbations from the step start (moist.jl:1339-1348)
+            st2<P>(w.rp, D, kc0, ic0, c_rp[0], c_rp[1]);
+            st2<P>(w.rup, D, kc0, ic0, c_rup[0], c_rup[1]);
+            st2<P>(w.rwp, D, kc0, ic0, c_rwp[0], c_rwp[1]);
+            st2<P>(rtp0, D, kc0, ic0, c_rtp[0], c_rtp[1]);
+            st2<P>(w.mom_x, D, kc0, ic0, o_mx[0], o_mx[1]);
+            st2<P>(w.mom_z, D, kc0, ic0, o_mz[0], o_mz[1]);
+        }
     }
     __syncthreads();
 
@@ -203,13 +217,13 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         GUz[e] = gz;
     }
     __syncthreads();
+    R o_ru[NCL];
 #pragma unroll
     for (int u = 0; u < NCL; ++u) {
-        const int e = tid + u * ST_THREADS;
-        const int r = e / ST_X, c = e - r * ST_X;
+        const int r = cr0 + u, c = cc0;
         const int k = k0 + r, i = i0 + c;
-        if (e >= ST_Z * ST_X || k >= nz || i >= nx) continue;
-        const int64_t q = gidx(k, i);
+        o_ru[u] = R(0.0);
+        if (k >= nz || i >= nx) continue;
         const int x0 = r * (ST_X + 1) + c;
         const R druudx = fdiv(CUx[x0 + 1] - CUx[x0], dx);
         const R drwudz = fdiv(CUz[(r + 1) * ST_X + c] - CUz[r * ST_X + c], DZ(k));
@@ -218,8 +232,9 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         const R u_zz = fdiv(GUz[(r + 1) * ST_X + c] - GUz[r * ST_X + c], DZ(k));
         const R diff = nu * (u_xx + u_zz);
         const R dpdx = fdiv(c_presx[u] - c_pres[u], dx);
-        w.R_rhou[q] = adv - dpdx + diff + c_dup[u];
+        o_ru[u] = adv - dpdx + diff + c_dup[u];
     }
+    if (pair_ok) st2<P>(w.R_rhou, D, kc0, ic0, o_ru[0], o_ru[1]);
     __syncthreads();
 
     // ---------------------------------------------------------------- R_rho_w
@@ -269,17 +284,13 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
     }
     __syncthreads();
     const R zf_sp = R(D.zf_sponge), H = R(D.H);
+    R o_rw[NCL];
 #pragma unroll
     for (int u = 0; u < NCL; ++u) {
-        const int e = tid + u * ST_THREADS;
-        const int r = e / ST_X, c = e - r * ST_X;
+        const int r = cr0 + u, c = cc0;
         const int k = k0 + r, i = i0 + c;
-        if (e >= ST_Z * ST_X || k >= nz || i >= nx) continue;
-        const int64_t q = gidx(k, i);
-        if (k == ke) {
-            w.R_rhow[q] = R(0.0);
-            continue;
-        }
+        o_rw[u] = R(0.0);
+        if (k >= nz || i >= nx || k == ke) continue;
         const int x0 = r * (ST_X + 1) + c;
         const R zz = Z(k + 1) - Z(k);
         const R drwudx = fdiv(XZ[x0 + 1] - XD[x0], dx);
@@ -302,6 +313,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
             const R buoy = -KC::GRAV * (RHL(k, i) + RHL(k + 1, i)) * R(0.5);
             res = adv - dpdz + buoy + diff + sponge;
         }
-        w.R_rhow[q] = res;
+        o_rw[u] = res;
     }
+    if (pair_ok) st2<P>(w.R_rhow, D, kc0, ic0, o_rw[0], o_rw[1]);
 }
