@@ -308,9 +308,12 @@ void launch_acoustic(moist_handle *h, const ZVec &zv, const SSet<S> &in, const W
 }
 
 // ---- one RK stage ------------------------------------------------------------
+// diag_in: the stage diagnostics of the input are still to be computed
+// (k_diag); otherwise the previous stage's k_scalar wrote them.  diag_out:
+// this stage's k_scalar writes those of its output for the next stage.
 template <class S>
 void enqueue_stage(moist_handle *h, int in_set, int out_set, int s0_set, int div, double dt_ovr, int nsub,
-                   int stage, int u_phys_2d) {
+                   int stage, int u_phys_2d, bool diag_in, bool diag_out) {
     const dim3 b2(TILE_X, TILE_Z);
     const ZVec zv = zvec(h);
     const Work<S> w = work<S>(h);
@@ -319,7 +322,10 @@ void enqueue_stage(moist_handle *h, int in_set, int out_set, int s0_set, int div
     S *rup[2] = {slot<S>(h, RUP), slot<S>(h, RUP2)};
     // the work fields of the mixed form are in the level-pair layout
     constexpr bool PL = is_mixed<S>();
-    hipLaunchKernelGGL((k_diag<S, PL>), dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, w);
+    // only the mixed form's k_scalar writes the next stage's diagnostics
+    if (!PL) diag_in = true, diag_out = false;
+    if (diag_in)
+        hipLaunchKernelGGL((k_diag<S, PL>), dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, w);
     hipLaunchKernelGGL((k_slow<S, PL>), dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles, zv,
                        in, s0, w, rtp[0], u_phys_2d);
     // rho_theta' rotates over three buffers (current, previous, next) and
@@ -349,7 +355,8 @@ void enqueue_stage(moist_handle *h, int in_set, int out_set, int s0_set, int div
         ru = 1 - ru;
     }
     hipLaunchKernelGGL((k_scalar<S, PL>), dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles,
-                       zv, in, s0, out, w, (const S *)rup[ru], (const S *)rtp[cur], div, dt_ovr, stage);
+                       zv, in, s0, out, w, (const S *)rup[ru], (const S *)rtp[cur], div, dt_ovr, stage,
+                       diag_out ? w.theta : nullptr, w.pres, w.rt2);
 }
 
 template <class S> void enqueue_step(moist_handle *h) {
@@ -363,9 +370,10 @@ template <class S> void enqueue_step(moist_handle *h) {
                            s, w);
     hipLaunchKernelGGL(k_physics<S>, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d, h->tiles2d,
                        zv, s, w, h->store_diag);
-    enqueue_stage<S>(h, S0_, SA_, S0_, 3, 0.0, h->ns_stage[0], 0, 0);
-    enqueue_stage<S>(h, SA_, SB_, S0_, 2, 0.0, h->ns_stage[1], 1, 0);
-    enqueue_stage<S>(h, SB_, S0_, S0_, 1, 0.0, h->ns_stage[2], 2, 0);
+    // stages 2 and 3 read the diagnostics the previous stage's k_scalar wrote
+    enqueue_stage<S>(h, S0_, SA_, S0_, 3, 0.0, h->ns_stage[0], 0, 0, true, true);
+    enqueue_stage<S>(h, SA_, SB_, S0_, 2, 0.0, h->ns_stage[1], 1, 0, false, true);
+    enqueue_stage<S>(h, SB_, S0_, S0_, 1, 0.0, h->ns_stage[2], 2, 0, false, false);
     hipLaunchKernelGGL(k_step_done, dim3(1), dim3(64), 0, h->stream, h->step_ctr);
 }
 
@@ -575,7 +583,7 @@ int compute_step_t(moist_handle *h, int m, const void *const phys[6], const void
     }
     HIPCHK(h, hipGetLastError());
     if (reset_nan(h)) return -1;
-    enqueue_stage<S>(h, S0_, SA_, SB_, 1, dt, ns, 0, 1);
+    enqueue_stage<S>(h, S0_, SA_, SB_, 1, dt, ns, 0, 1, true, false);
     // the output becomes the current state; the input's partner columns are the halos
     for (int f = 0; f < 7; ++f)
         HIPCHK(h, hipMemcpyAsync(slot<S>(h, SB_ + f), slot<S>(h, S0_ + f), h->n2 * sizeof(S),
@@ -867,7 +875,7 @@ template <class S> void enqueue_stage_rec(moist_handle *h, int st, SSet<S> in, S
     }
     hipLaunchKernelGGL((k_scalar<S, false>), dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles,
                        zv, in, s0, out, w, (const S *)rec_rup<S>(h, st, nsub), (const S *)rec_rtp<S>(h, st, nsub), div,
-                       0.0, st);
+                       0.0, st, (S *)nullptr, (S *)nullptr, (S *)nullptr);
 }
 
 // FP64 form: forward pass of one step from the state `sn` into the current

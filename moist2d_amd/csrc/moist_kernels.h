@@ -114,31 +114,30 @@ __device__ __forceinline__ S full_rt(const SSet<S> &s, int64_t c, const ZVec &zv
 }
 
 // =============================================================================
-// Stage diagnostics: theta, pressure (or its deviation), rt2pres, u and w at
-// faces (moist.jl:1048-1062, 1091-1097, 1184-1189).  theta and rt2pres are
-// stored in layout P (level pairs when true), the others x-fastest.
+// Stage diagnostics: theta, pressure (or its deviation), rt2pres
+// (moist.jl:1048-1062, 1091-1097), and in the x-fastest variant also u and w
+// at faces (moist.jl:1091-1097, 1184-1189); the pair-layout variant (the
+// mixed form) leaves those to k_slow, which forms them from the tile it stages.
 // =============================================================================
-template <class S, bool P>
-__global__ void __launch_bounds__(256) k_diag(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
-                                              ZVec zv, SSet<S> in, Work<S> w) {
+// One cell from its stored state values (rho and rho_theta are deviations
+// from the base state in the mixed form).  k_diag and the epilogue of
+// k_scalar (the next stage's diagnostics) share it.
+template <class S>
+__device__ __forceinline__ void diag_point(const Dom &D, const ZVec &zv, int k, S rho_s, S rt_s, S rqv, S rqc,
+                                           S rqr, S &theta, S &pres, S &rt2) {
     using R = S;
     using KC = K<R>;
-    const int4 t = tiles[blockIdx.x];
-    if (t.x < 0) return;  // padding tile of the XCD ordering
-    const Dom &D = doms[t.x];
-    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
-    const int nx = D.nx, nz = D.nz;
-    if (i >= nx || k >= nz) return;
-    const int64_t c = D.off2 + (int64_t)k * nx + i;
-    const int ip = wrapx(i + 1, nx);
-    const R rho = full_rho(in, c, zv, D, k);
-    const R rt = full_rt(in, c, zv, D, k);
-    const R qv = fdiv(in.f[QV][c], rho), qc = fdiv(in.f[QC][c], rho), qr = fdiv(in.f[QR][c], rho);
+    R rho = rho_s, rt = rt_s;
+    if constexpr (is_mixed<S>()) {
+        rho = rho_s + S(zv.rho_bar[D.offz + k]);
+        rt = rt_s + S(zv.rt_bar[D.offz + k]);
+    }
+    const R qv = fdiv(rqv, rho), qc = fdiv(rqc, rho), qr = fdiv(rqr, rho);
     const R qd = R(1.0) - qv - qc - qr;
     const R r = KC::Rd * qd + KC::Rv * qv;
     const R cp = KC::Cpd * qd + KC::Cpv * qv + KC::Cpl * (qc + qr);
     const R cv = cp - r;
-    R pres, pfull;
+    R pfull;
     if constexpr (is_mixed<S>()) {
         // p = P0 exp(g ln(r rt/P0)), p_bar = P0 exp(g_bar ln(Rd rt_bar/P0));
         // ln(r rt / (Rd rt_bar)) = log1p(dr) + log1p(rt'/rt_bar),
@@ -148,7 +147,7 @@ __global__ void __launch_bounds__(256) k_diag(const Dom *__restrict__ doms, cons
         const R dg = fdiv(KC::Cpd * dR - KC::Rd * dCp, cv * KC::Cvd);
         const R gam = fdiv(cp, cv);
         const R rtb = R(zv.rt_bar[D.offz + k]);
-        const R E = dg * R(zv.L_bar[D.offz + k]) + gam * (m_log1p(fdiv(dR, KC::Rd)) + m_log1p(fdiv(in.f[RT][c], rtb)));
+        const R E = dg * R(zv.L_bar[D.offz + k]) + gam * (m_log1p(fdiv(dR, KC::Rd)) + m_log1p(fdiv(rt_s, rtb)));
         const double pb = zv.p_bar[D.offz + k];
         pres = R(pb) * m_expm1(E);
         pfull = R(pb) + pres;
@@ -156,20 +155,42 @@ __global__ void __launch_bounds__(256) k_diag(const Dom *__restrict__ doms, cons
         pres = KC::P0 * m_pow(r * rt / KC::P0, cp / cv);
         pfull = pres;
     }
-    // neighbour reads (i+1 and k+1) happen before the first store
-    const int64_t cip = D.off2 + (int64_t)k * nx + ip;
-    const R uf = fdiv(in.f[RHOU][c] * R(2.0), rho + full_rho(in, cip, zv, D, k));
-    R wf = R(0.0);
-    if (k < nz - 1) {
-        const int64_t ckp = c + nx;
-        const double *dz = zv.dz + D.offz;
-        wf = fdiv(in.f[RHOW][c], f2h(rho, full_rho(in, ckp, zv, D, k + 1), R(dz[k]), R(dz[k + 1])));
+    theta = fdiv(rt, rho);
+    rt2 = fdiv(fdiv(cp, cv) * pfull, rt);
+}
+
+// theta and rt2pres are stored in layout P (level pairs when true), the
+// pressure and the face velocities x-fastest.
+template <class S, bool P>
+__global__ void __launch_bounds__(256) k_diag(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
+                                              ZVec zv, SSet<S> in, Work<S> w) {
+    const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;  // padding tile of the XCD ordering
+    const Dom &D = doms[t.x];
+    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+    const int nx = D.nx, nz = D.nz;
+    if (i >= nx || k >= nz) return;
+    const int64_t c = D.off2 + (int64_t)k * nx + i;
+    S th, pres, rt2;
+    diag_point<S>(D, zv, k, in.f[RHO][c], in.f[RT][c], in.f[QV][c], in.f[QC][c], in.f[QR][c], th, pres, rt2);
+    S uf = S(0.0), wf = S(0.0);
+    if constexpr (!P) {
+        // neighbour reads (i+1 and k+1) happen before the first store
+        const S rho = full_rho(in, c, zv, D, k);
+        const int64_t cip = D.off2 + (int64_t)k * nx + wrapx(i + 1, nx);
+        uf = fdiv(in.f[RHOU][c] * S(2.0), rho + full_rho(in, cip, zv, D, k));
+        if (k < nz - 1) {
+            const double *dz = zv.dz + D.offz;
+            wf = fdiv(in.f[RHOW][c], f2h(rho, full_rho(in, c + nx, zv, D, k + 1), S(dz[k]), S(dz[k + 1])));
+        }
     }
     w.pres[c] = pres;
-    w.theta[widx<P>(D, k, i)] = fdiv(rt, rho);
-    w.rt2[widx<P>(D, k, i)] = fdiv(fdiv(cp, cv) * pfull, rt);
-    w.uf[c] = uf;
-    w.wf[c] = wf;
+    w.theta[widx<P>(D, k, i)] = th;
+    w.rt2[widx<P>(D, k, i)] = rt2;
+    if constexpr (!P) {
+        w.uf[c] = uf;
+        w.wf[c] = wf;
+    }
 }
 
 #include "moist_acoustic.h"

@@ -2,7 +2,11 @@
 // (moist.jl:1469-1539).  Included by moist_kernels.h inside namespace mo.
 //
 // A block owns a tile of ST_X columns x ST_Z levels.  The work fields it reads
-// (mom_x, mom_z and the final perturbations) are in layout P.  For each of qv, qc, qr
+// (mom_x, mom_z and the final perturbations) are in layout P.  In the
+// pair-layout variant each thread's two interior cells are the levels of one
+// level pair of a column, and, given dg_th, it also writes the stage
+// diagnostics of its output (theta and rt2pres in level pairs, pressure
+// x-fastest), which the next stage then reads instead of running k_diag on it.  For each of qv, qc, qr
 // it stages q = rho_q / rho and rho of the tile plus a 2-cell halo in LDS,
 // evaluates every face flux once (Koren advective flux, moist.jl:850-945, and
 // the diffusive flux, moist.jl:1494-1510), and updates each cell from the
@@ -18,7 +22,8 @@ template <class S, bool P>
 __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
                                                        ZVec zv, SSet<S> in, SSet<S> s0, SSet<S> out, Work<S> w,
                                                        const S *__restrict__ rup_fin, const S *__restrict__ rtp_fin,
-                                                       int div, double dt_ovr, int stage) {
+                                                       int div, double dt_ovr, int stage, S *dg_th, S *dg_pres,
+                                                       S *dg_rt2) {
     using R = S;
     __shared__ R l_rho[ST_HZ][ST_HX];
     __shared__ R l_q[ST_HZ][ST_HX];
@@ -48,6 +53,12 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
     constexpr int NFX = (ST_Z * (ST_X + 1) + ST_THREADS - 1) / ST_THREADS;
     constexpr int NFZ = ((ST_Z + 1) * ST_X + ST_THREADS - 1) / ST_THREADS;
     constexpr int NCL = (ST_Z * ST_X + ST_THREADS - 1) / ST_THREADS;
+    static_assert(NCL == 2 && ST_Z * ST_X == 2 * ST_THREADS && ST_THREADS % ST_X == 0, "two cells a thread");
+    // interior cell u of this thread as a flat tile index e = r ST_X + c:
+    // rows tid/ST_X and tid/ST_X + ST_THREADS/ST_X, or in the pair-layout
+    // variant the two levels 2 (tid/ST_X), +1 of one column, so their
+    // work-field accesses move level pairs
+
     int64_t gst[NSTG];  // staged cell (-1: outside the domain's levels or the tile)
 #pragma unroll
     for (int u = 0; u < NSTG; ++u) {
@@ -79,7 +90,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
     }
 #pragma unroll
     for (int u = 0; u < NCL; ++u) {
-        const int e = tid + u * ST_THREADS;
+        const int e = P ? (2 * (tid / ST_X) + u) * ST_X + tid % ST_X : tid + u * ST_THREADS;
         const int k = k0 + e / ST_X;
         cdz[u] = R(dzd[min(k, ke)]);
     }
@@ -102,19 +113,38 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
 
     // the final update's inputs (read-only here), loaded with the rest
     R fin[NCL][4];
+    if constexpr (P) {
+        const int kq = min(k0 + 2 * (tid / ST_X), (nz - 1) & ~1), iq = min(i0 + tid % ST_X, nx - 1);
+        R p_rp[2], p_ru[2], p_rt[2], p_rw[2];
+        ld2<true>(w.rp, D, kq, iq, p_rp[0], p_rp[1]);
+        ld2<true>(rup_fin, D, kq, iq, p_ru[0], p_ru[1]);
+        ld2<true>(rtp_fin, D, kq, iq, p_rt[0], p_rt[1]);
+        ld2<true>(w.rwp, D, kq, iq, p_rw[0], p_rw[1]);
 #pragma unroll
-    for (int u = 0; u < NCL; ++u) {
-        const int e = tid + u * ST_THREADS;
-        const int r = e / ST_X, c = e - r * ST_X;
-        const int k = min(k0 + r, ke), i = min(i0 + c, nx - 1);
-        const int64_t q = gidx(k, i), qw = widx<P>(D, k, i);
-        fin[u][0] = in.f[RHO][q] + w.rp[qw];
-        fin[u][1] = in.f[RHOU][q] + rup_fin[qw];
-        fin[u][2] = in.f[RT][q] + rtp_fin[qw];
-        fin[u][3] = k < ke ? in.f[RHOW][q] + w.rwp[qw] : in.f[RHOW][q];
+        for (int u = 0; u < NCL; ++u) {
+            const int k = min(k0 + 2 * (tid / ST_X) + u, ke);
+            const int64_t q = gidx(k, iq);
+            fin[u][0] = in.f[RHO][q] + p_rp[u];
+            fin[u][1] = in.f[RHOU][q] + p_ru[u];
+            fin[u][2] = in.f[RT][q] + p_rt[u];
+            fin[u][3] = k < ke ? in.f[RHOW][q] + p_rw[u] : in.f[RHOW][q];
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < NCL; ++u) {
+            const int e = tid + u * ST_THREADS;
+            const int r = e / ST_X, c = e - r * ST_X;
+            const int k = min(k0 + r, ke), i = min(i0 + c, nx - 1);
+            const int64_t q = gidx(k, i);
+            fin[u][0] = in.f[RHO][q] + w.rp[q];
+            fin[u][1] = in.f[RHOU][q] + rup_fin[q];
+            fin[u][2] = in.f[RT][q] + rtp_fin[q];
+            fin[u][3] = k < ke ? in.f[RHOW][q] + w.rwp[q] : in.f[RHOW][q];
+        }
     }
 
     const S *dph[3] = {w.d_qv, w.d_qc, w.d_qr};
+    R qn[3][P ? NCL : 1];  // the updated rho_q of this thread's cells (pair variant's epilogue)
     int nan_f = 0;
     for (int sc = 0; sc < 3; ++sc) {
         const S *qraw = in.f[QV + sc];
@@ -127,7 +157,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
         R s0v[NCL], dpv[NCL];
 #pragma unroll
         for (int u = 0; u < NCL; ++u) {
-            const int e = tid + u * ST_THREADS;
+            const int e = P ? (2 * (tid / ST_X) + u) * ST_X + tid % ST_X : tid + u * ST_THREADS;
             const int r = e / ST_X, c = e - r * ST_X;
             const int k = k0 + r, i = i0 + c;
             const bool ok = e < ST_Z * ST_X && k < nz && i < nx;
@@ -223,7 +253,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
         // cells (moist.jl:1480-1516)
 #pragma unroll
         for (int u = 0; u < NCL; ++u) {
-            const int e = tid + u * ST_THREADS;
+            const int e = P ? (2 * (tid / ST_X) + u) * ST_X + tid % ST_X : tid + u * ST_THREADS;
             if (e >= ST_Z * ST_X) continue;
             const int r = e / ST_X, c = e - r * ST_X;
             const int k = k0 + r, i = i0 + c;
@@ -237,14 +267,23 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
             const R diff_z = fdiv(kap * (l_dz[r + 1][c] - l_dz[r][c]), dzk);
             const R v = s0v[u] + dt * (adv + (diff_x + diff_z) + dpv[u]);
             out.f[QV + sc][q] = v;
+            if constexpr (P) {
+                if (sc == 0) qn[0][u] = v;
+                else if (sc == 1) qn[1][u] = v;
+                else qn[2][u] = v;
+            }
             if (isnan(v) && !nan_f) nan_f = 5 + sc;
         }
     }
 
     // final update of rho, rho_u, rho_theta, rho_w (moist.jl:1522-1539) and NaN checks
+    R dth[P ? NCL : 1], drt2[P ? NCL : 1];  // the next stage's theta and rt2pres (pair variant)
+    if constexpr (P)
+#pragma unroll
+        for (int u = 0; u < NCL; ++u) dth[u] = drt2[u] = R(0.0);
 #pragma unroll
     for (int u = 0; u < NCL; ++u) {
-        const int e = tid + u * ST_THREADS;
+        const int e = P ? (2 * (tid / ST_X) + u) * ST_X + tid % ST_X : tid + u * ST_THREADS;
         const int r = e / ST_X, c = e - r * ST_X;
         const int k = k0 + r, i = i0 + c;
         if (e >= ST_Z * ST_X || k >= nz || i >= nx) continue;
@@ -260,9 +299,24 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
         else if (isnan(vw)) f = 3;
         else if (isnan(vt)) f = 4;
         if (f) report_nan(w.nan_key, ((*w.step_ctr * 3 + stage) << 4) | f);
+        // the next stage's diagnostics of the updated cell (dg_th null: none)
+        if constexpr (P) {
+            if (dg_th) {
+                R pres;
+                diag_point<S>(D, zv, k, vr, vt, qn[0][u], qn[1][u], qn[2][u], dth[u], pres, drt2[u]);
+                dg_pres[q] = pres;
+            }
+        }
         // stale halo values surface_fluxes! reads at the next step (moist.jl:723,736)
         if (k == 0 && i == nx - 1) w.sfh[2 * m + 0] = double(in.f[RHOU][q]);
         if (k == 0 && i == 0) w.sfh[2 * m + 1] = double(full_rho(in, q, zv, D, 0));
+    }
+    if constexpr (P) {
+        const int kq = k0 + 2 * (tid / ST_X), iq = i0 + tid % ST_X;
+        if (dg_th && kq < nz && iq < nx) {
+            st2<true>(dg_th, D, kq, iq, dth[0], dth[1]);
+            st2<true>(dg_rt2, D, kq, iq, drt2[0], drt2[1]);
+        }
     }
     // scalar NaN codes (5..7) come after the state fields of the same stage
     if (nan_f) report_nan(w.nan_key, ((*w.step_ctr * 3 + stage) << 4) | nan_f);

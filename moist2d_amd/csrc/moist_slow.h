@@ -2,8 +2,10 @@
 // acoustic start values (moist.jl:1339-1348).  Included by moist_kernels.h
 // inside namespace mo.
 //
-// A block owns ST_X columns x ST_Z levels.  It stages u and w at faces, theta,
-// rho, rho_u and rho_w of the tile plus a 2-cell halo in LDS, then evaluates
+// A block owns ST_X columns x ST_Z levels.  It stages u and w at faces (formed
+// here from rho, rho_u and rho_w in the pair-layout variant, read from k_diag's
+// output otherwise), theta, rho, rho_u and rho_w of the tile plus a 2-cell
+// halo in LDS, then evaluates
 // every flux once at the face, centre or corner where the reference defines
 // it and forms each tendency from flux differences:
 //   R_rho_theta: x- and z-face fluxes (moist.jl:1287-1329)
@@ -61,13 +63,28 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
             const int r = e / SL_HX, c = e - r * SL_HX;
             const int k = k0 - 2 + r, i = wrapx(i0 - 2 + c, nx);
             const bool ok = e < SL_HZ * SL_HX && k >= 0 && k < nz;
-            const int64_t q = gidx(ok ? k : 0, ok ? i : 0);
-            v[u][0] = ok ? w.uf[q] : R(0);
-            v[u][1] = ok ? w.wf[q] : R(0);
-            v[u][2] = ok ? w.theta[widx<P>(D, ok ? k : 0, ok ? i : 0)] : R(0);
-            v[u][3] = ok ? full_rho(in, q, zv, D, k) : R(0);
-            v[u][4] = ok ? in.f[RHOU][q] : R(0);
-            v[u][5] = ok ? in.f[RHOW][q] : R(0);
+            const int kc = ok ? k : 0, ic = ok ? i : 0;
+            const int64_t q = gidx(kc, ic);
+            const R rho = full_rho(in, q, zv, D, kc), ru = in.f[RHOU][q], rw = in.f[RHOW][q];
+            R uf, wf;
+            if constexpr (P) {
+                // u and w at the faces from rho of the neighbours i+1 and k+1,
+                // the stage diagnostics' expressions (moist.jl:1091-1097, 1184-1189)
+                const R rho_ip = full_rho(in, gidx(kc, wrapx(ic + 1, nx)), zv, D, kc);
+                const int kp = min(kc + 1, ke);
+                const R rho_kp = full_rho(in, gidx(kp, ic), zv, D, kp);
+                uf = fdiv(ru * R(2.0), rho + rho_ip);
+                wf = kc < nz - 1 ? fdiv(rw, f2h(rho, rho_kp, R(dzd[kc]), R(dzd[kp]))) : R(0.0);
+            } else {
+                uf = w.uf[q];
+                wf = w.wf[q];
+            }
+            v[u][0] = ok ? uf : R(0);
+            v[u][1] = ok ? wf : R(0);
+            v[u][2] = ok ? w.theta[widx<P>(D, kc, ic)] : R(0);
+            v[u][3] = ok ? rho : R(0);
+            v[u][4] = ok ? ru : R(0);
+            v[u][5] = ok ? rw : R(0);
         }
         if (tid < SL_HZ) {
             const int k = k0 - 2 + tid;
