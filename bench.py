@@ -55,7 +55,7 @@ def step_passes(ns):
     return 10 + 3 * (23 + 26) + ACOUSTIC_PASSES * n_sub
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_acoustic_coldpool_8192x256_mixed.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_acoustic_pairs_coldpool_8192x256_mixed.json")
 
 
 def measured_traffic(kernel, nx, nz):
@@ -74,8 +74,13 @@ def measured_traffic(kernel, nx, nz):
 AUTOTUNE = True
 
 
+FIXED_SHAPE = None  # (levels, threads) from --shape: no autotuning
+
+
 def tune(eng):
     """Pick the acoustic kernel's tile shape for this handle (None when disabled)."""
+    if FIXED_SHAPE:
+        return {"levels_per_chunk": FIXED_SHAPE[0], "threads": FIXED_SHAPE[1], "us_per_launch": None}
     if not AUTOTUNE:
         return None
     L, NT, us = eng.autotune(reps=10)
@@ -85,11 +90,15 @@ def tune(eng):
 def run_form(p, s, precision, steps, warmup, graph=True):
     """Warm up on a throwaway engine, then time `steps` large steps from t = 0."""
     shape = None
+    if FIXED_SHAPE:
+        shape = tune(None)
     if warmup:
         eng = core.Engine(p, precision)
         eng.store_phys_diag(False)
         eng.upload(s)
-        shape = tune(eng)
+        shape = shape or tune(eng)
+        if shape:
+            eng.set_acoustic_shape(shape["levels_per_chunk"], shape["threads"])
         eng.use_graph(graph)
         eng.time_steps(warmup)
         eng.close()
@@ -361,6 +370,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=30, help="0 skips the 256-member ensemble")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workloads")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default acoustic tile shape")
+    ap.add_argument("--shape", type=int, nargs=2, metavar=("LEVELS", "THREADS"),
+                    help="fix the acoustic tile shape of every form (no autotuning), e.g. for profiling")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -369,8 +380,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    global AUTOTUNE
+    global AUTOTUNE, FIXED_SHAPE
     AUTOTUNE = not args.no_autotune
+    FIXED_SHAPE = tuple(args.shape) if args.shape else None
     if args.copy_calibration:
         copy_calibration()
     p, s = ic.coldpool(np.float64, nx=args.nx, nz=args.nz)
@@ -425,7 +437,8 @@ def main():
     achieved = alg_bytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     sh = mixed["acoustic_shape"] or {"levels_per_chunk": 4, "threads": 1024}
     nt = sh["threads"]
-    kname = f"mo::k_acoustic<float, {sh['levels_per_chunk']}, {abs(nt)}, {8 if nt < 0 else 1}>"
+    # the mixed form's acoustic kernel (work fields in level pairs)
+    kname = f"mo::k_acoustic_pairs<{sh['levels_per_chunk']}, {abs(nt)}, {8 if nt < 0 else 1}>"
     roof = {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname, p.Nx, p.Nz),
             "avg_launch_ms": kms, "launches": nlaunch, "alg_bytes_per_launch": alg_bytes}
