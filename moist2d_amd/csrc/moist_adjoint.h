@@ -647,7 +647,7 @@ __global__ void __launch_bounds__(256)
     ka_ac_solve(const Dom *__restrict__ doms, const int2 *__restrict__ groups, int ngroups, ZVec zv, AdjW<RC> a, int div,
                 int nsub, int factor) {
     constexpr int CW = 64 / CH;
-    const int wid = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int wid = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (wid >= ngroups) return;
     const int2 t = groups[wid];
     if (t.x < 0) return;
@@ -1297,10 +1297,13 @@ __device__ __forceinline__ void kessler_v(const Dom &D, V rho_k, V rt, V rqv, V 
 
 // ---- physics adjoint (moist.jl:554-707, 1632-1644), do_sf = do_rd = false -------
 // lph: adjoint of the six tendencies; ldprec: adjoint of d_prec (dx * dt).
-// Adds into ls; accumulates dJ/dK per cell into gk (3 fields).
+// lout = ls + ls0 + the physics adjoints (lout may be another buffer than ls:
+// each cell reads and writes only itself); accumulates dJ/dK per cell into gk
+// (3 fields).
 __global__ void __launch_bounds__(AD_X *AD_Z)
     ka_physics(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s,
-               SSet<double> lph, SSet<double> ls, SSet<double> ls0, double *gk0, double *gk1, double *gk2) {
+               SSet<double> lph, SSet<double> ls, SSet<double> ls0, SSet<double> lout, double *gk0, double *gk1,
+               double *gk2) {
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
@@ -1390,7 +1393,7 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     o[RT] += l_rt;
     o[QR] += l_qr;
 #pragma unroll
-    for (int f = 0; f < 7; ++f) ls.f[f][c] = o[f];
+    for (int f = 0; f < 7; ++f) lout.f[f][c] = o[f];
 }
 
 // Per-member sums of the parameter gradients (one block per member).

@@ -942,7 +942,9 @@ template <class RC> void enqueue_step_adj(moist_handle *h, const double *sn) {
             const int m = n - 1;
             hipLaunchKernelGGL(ka_ac_v1<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, a,
                                (const RC *)rec_rwp<RC>(h, st, m + 1), div, nsub);
-            const dim3 gs((h->nwgroups + 3) / 4), bs(256);
+            // one wave (one column group) per block: the waves are independent,
+            // and small grids then reach every CU
+            const dim3 gs(h->nwgroups), bs(64);
             if (h->wch == 4)
                 hipLaunchKernelGGL((ka_ac_solve<4, RC>), gs, bs, 0, h->stream, h->dom_d, h->wgroups, h->nwgroups, zv,
                                    a, div, nsub, n == nsub ? 1 : 0);
@@ -967,11 +969,13 @@ template <class RC> void enqueue_step_adj(moist_handle *h, const double *sn) {
                            dset(li, n2), a);
         std::swap(lo, li);
     }
-    // lo: adjoint of the stage-1 input; the step input also enters as s0 of every stage and through physics
+    // lo: adjoint of the stage-1 input (A.lam2: the three stages alternate
+    // the two buffers, starting from A.lam); the step input also enters as s0
+    // of every stage and through physics.  The step input's adjoint goes to
+    // A.lam, so every reverse step reads and writes the same buffers.
     hipLaunchKernelGGL(ka_physics, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv,
-                       dset(const_cast<double *>(sn), n2), lph, dset(lo, n2), ls0, A.gk, A.gk + n2, A.gk + 2 * n2);
-    // the buffer holding the step input's adjoint becomes A.lam (no copy)
-    if (lo != A.lam) std::swap(A.lam, A.lam2);
+                       dset(const_cast<double *>(sn), n2), lph, dset(lo, n2), ls0, dset(A.lam, n2), A.gk,
+                       A.gk + n2, A.gk + 2 * n2);
 }
 
 // Checkpoints hold the state slots (S0_.. S0_+6, contiguous at the front of
@@ -1018,6 +1022,28 @@ void record_step(moist_handle *h, int j, bool advance) {
     }
 }
 
+// Step j of a segment's recording pass with segment records: the step input
+// saved, then the step recorded (which advances the state).
+void seg_record(moist_handle *h, int j) {
+    AdjState &A = *h->adj;
+    copy_bytes(h, seg_step(h, j), h->pool, 7 * h->n2 * h->elem);
+    A.cur = A.recs + A.rb_bytes * (size_t)j;
+    record_step(h, j, true);
+}
+// Reverse step j of a segment from its records, or, with per-step records,
+// its recording pass first (from the stored step input).
+void seg_reverse(moist_handle *h, int j) {
+    AdjState &A = *h->adj;
+    if (A.segrec) {
+        A.cur = A.recs + A.rb_bytes * (size_t)j;
+    } else {
+        if (h->precision == MOIST_MIXED) copy_bytes(h, h->pool, seg_step(h, j), 7 * h->n2 * h->elem);
+        A.cur = A.recs;
+        record_step(h, j, false);
+    }
+    reverse_step(h, j);
+}
+
 template <class H> int adjoint_run_t(moist_handle *h, int nsteps, int ckpt, const void *const *tgt, double *J) {
     if (adj_alloc(h, nsteps, ckpt)) return -1;
     AdjState &A = *h->adj;
@@ -1059,28 +1085,15 @@ template <class H> int adjoint_run_t(moist_handle *h, int nsteps, int ckpt, cons
         load_state(h, A.cp + A.cp_stride * sg);
         if (A.segrec) {
             // one recording pass over the segment, then the reverse sweep from the records
-            for (int n = n0; n < n1; ++n) {
-                copy_bytes(h, seg_step(h, n - n0), h->pool, state_bytes);
-                A.cur = A.recs + A.rb_bytes * (size_t)(n - n0);
-                record_step(h, n - n0, true);
-            }
-            for (int n = n1 - 1; n >= n0; --n) {
-                A.cur = A.recs + A.rb_bytes * (size_t)(n - n0);
-                reverse_step(h, n - n0);
-            }
+            for (int n = n0; n < n1; ++n) seg_record(h, n - n0);
         } else {
             // states of the segment, then per step (in reverse) its record and adjoint
             for (int n = n0; n < n1; ++n) {
                 copy_bytes(h, seg_step(h, n - n0), h->pool, state_bytes);
                 if (n + 1 < n1 && run_steps(h, 1)) return -1;
             }
-            A.cur = A.recs;
-            for (int n = n1 - 1; n >= n0; --n) {
-                if (h->precision == MOIST_MIXED) copy_bytes(h, h->pool, seg_step(h, n - n0), state_bytes);
-                record_step(h, n - n0, false);
-                reverse_step(h, n - n0);
-            }
         }
+        for (int n = n1 - 1; n >= n0; --n) seg_reverse(h, n - n0);
         HIPCHK(h, hipGetLastError());
     }
     hipLaunchKernelGGL(ka_reduce_k, dim3(h->nm), dim3(256), 0, h->stream, h->dom_d, (const double *)A.gk,
