@@ -272,19 +272,11 @@ __global__ void ka_fill(double *p, int64_t n, double v) {
         p[q] = v;
 }
 
-// Zeroing of up to 16 arrays of `len` doubles in one launch (graph-safe).
+// Arrays a per-cell kernel clears at its cells (up to 24).
 struct FillList {
-    double *p[16];
+    double *p[24];
     int n;
-    int64_t len;
 };
-__global__ void ka_fill_many(FillList L) {
-    const int64_t total = (int64_t)L.n * L.len;
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
-        const int a = (int)(q / L.len);
-        L.p[a][q - (int64_t)a * L.len] = 0.0;
-    }
-}
 
 // Full FP64 state from a mixed-form state (FP32 deviations; rho and
 // rho_theta about the FP64 horizontal-mean base state), one cell per thread.
@@ -334,13 +326,59 @@ __global__ void __launch_bounds__(256) ka_loss(const Dom *__restrict__ doms, SSe
     if (threadIdx.x == 0) J[m] = red[0];
 }
 
+// Sum of the positivity-limiter adjoints of every column into rho_q at
+// (0, lin_col) and (ke, lin_col) of one member, by one 256-thread block.
+__device__ __forceinline__ void lin_sum(const Dom &D, const double *linb, const double *lint, SSet<double> ls,
+                                        int tid, double *rb, double *rt) {
+    for (int sc = 0; sc < 3; ++sc) {
+        double b = 0.0, tp = 0.0;
+        for (int i = tid; i < D.nx; i += 256) {
+            b += linb[D.offx * 3 + (int64_t)sc * D.nx + i];
+            tp += lint[D.offx * 3 + (int64_t)sc * D.nx + i];
+        }
+        rb[tid] = b;
+        rt[tid] = tp;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (tid < w) {
+                rb[tid] += rb[tid + w];
+                rt[tid] += rt[tid + w];
+            }
+            __syncthreads();
+        }
+        if (tid == 0 && D.nz >= 2) {
+            ls.f[QV + sc][D.off2 + D.lin_col] += rb[0];
+            ls.f[QV + sc][D.off2 + (int64_t)(D.nz - 1) * D.nx + D.lin_col] += rt[0];
+        }
+        __syncthreads();
+    }
+}
+// The limiter sums of the stage reversed before, run by ka_pre's blocks past
+// its ntiles tiles (one per member, when the launch has them): they add into
+// that stage's input adjoint, which the next stage's ka_scalar reads after
+// ka_pre.
+struct LinSum {
+    const double *linb, *lint;
+    SSet<double> ls;
+    int ntiles;
+};
+
 // ---- per-stage point values the gathers read many times -----------------------------
 // q = rho_q / rho of the three scalars, u = 2 rho_u / (rho_i + rho_i+1) and
 // w = rho_w / f2h(rho_k, rho_k+1) of the stage input, with the forward's
-// expressions.
+// expressions.  The stage's accumulators (`zero`) are cleared at
+// the cell: ka_pre is the stage's first kernel and none of them is read or
+// accumulated before it has run everywhere.
 template <class RC>
 __global__ void __launch_bounds__(AD_X *AD_Z)
-    ka_pre(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s, AdjW<RC> a) {
+    ka_pre(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s, AdjW<RC> a,
+           FillList zero, LinSum lin) {
+    if ((int)blockIdx.x >= lin.ntiles) {  // the blocks past the tiles: limiter sums
+        __shared__ double rb[256], rt[256];
+        lin_sum(doms[blockIdx.x - lin.ntiles], lin.linb, lin.lint, lin.ls, threadIdx.y * AD_X + threadIdx.x, rb,
+                rt);
+        return;
+    }
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
@@ -353,6 +391,7 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     for (int sc = 0; sc < 3; ++sc) a.q[sc][c] = s.f[QV + sc][c] / rho[c];
     a.uf[c] = s.f[RHOU][c] * 2.0 / (rho[c] + rho[cp]);
     a.wf[c] = k < nz - 1 ? s.f[RHOW][c] / f2h(rho[c], rho[c + nx], dzd[k], dzd[k + 1]) : 0.0;
+    for (int f = 0; f < zero.n; ++f) zero.p[f][c] = 0.0;
 }
 
 // ---- scalar transport + final update (moist.jl:1469-1539) ------------------------
@@ -476,34 +515,11 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     a.Lrwp[c] = k <= ke - 1 ? lo_w : 0.0;
 }
 
-// Sum of the positivity-limiter adjoints of every column into rho_q at
-// (0, lin_col) and (ke, lin_col).  One block per member.
+// Sum of the positivity-limiter adjoints into rho_q (lin_sum), one block per member.
 template <class RC>
 __global__ void __launch_bounds__(256) ka_lin(const Dom *__restrict__ doms, AdjW<RC> a, SSet<double> ls) {
-    const Dom &D = doms[blockIdx.x];
     __shared__ double rb[256], rt[256];
-    for (int sc = 0; sc < 3; ++sc) {
-        double b = 0.0, tp = 0.0;
-        for (int i = threadIdx.x; i < D.nx; i += blockDim.x) {
-            b += a.linb[D.offx * 3 + (int64_t)sc * D.nx + i];
-            tp += a.lint[D.offx * 3 + (int64_t)sc * D.nx + i];
-        }
-        rb[threadIdx.x] = b;
-        rt[threadIdx.x] = tp;
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) {
-                rb[threadIdx.x] += rb[threadIdx.x + w];
-                rt[threadIdx.x] += rt[threadIdx.x + w];
-            }
-            __syncthreads();
-        }
-        if (threadIdx.x == 0 && D.nz >= 2) {
-            ls.f[QV + sc][D.off2 + D.lin_col] += rb[0];
-            ls.f[QV + sc][D.off2 + (int64_t)(D.nz - 1) * D.nx + D.lin_col] += rt[0];
-        }
-        __syncthreads();
-    }
+    lin_sum(doms[blockIdx.x], a.linb, a.lint, ls, threadIdx.x, rb, rt);
 }
 
 // ---- acoustic substep, vertical part (moist.jl:1377-1446) ---------------------------

@@ -824,15 +824,6 @@ void fill(moist_handle *h, double *p, int64_t n, double v = 0.0) {
 }
 
 // Zeroes the arrays of `ps` (each `len` doubles) in one launch.
-void fill_zero(moist_handle *h, std::initializer_list<double *> ps, int64_t len) {
-    FillList L{};
-    L.n = 0;
-    L.len = len;
-    for (double *p : ps) L.p[L.n++] = p;
-    const int64_t blocks = std::min<int64_t>((L.n * len + 255) / 256, 4096);
-    hipLaunchKernelGGL(ka_fill_many, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0, h->stream, L);
-}
-
 void copy_bytes(moist_handle *h, void *dst, const void *src, size_t n) {
     hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, h->stream);
 }
@@ -922,7 +913,6 @@ template <class RC> void enqueue_step_adj(moist_handle *h, const double *sn) {
     const size_t n2 = h->n2;
     const ZVec zv = zvec(h);
     const dim3 b2(TILE_X, TILE_Z);
-    fill_zero(h, {A.ls0, A.lph}, 7 * n2);
     const SSet<double> ls0 = dset(A.ls0, n2), lph = dset(A.lph, n2);
     const int divs[3] = {3, 2, 1};
     double *lo = A.lam, *li = A.lam2;
@@ -930,13 +920,28 @@ template <class RC> void enqueue_step_adj(moist_handle *h, const double *sn) {
         const SSet<double> in = dset(st == 0 ? const_cast<double *>(sn) : rec_in(h, st), n2);
         const int nsub = h->ns_stage[st], div = divs[st];
         const AdjW<RC> a = adj_w<RC>(h, st);
-        fill_zero(h, {wk(h, W_TH), wk(h, W_RT), wk(h, W_LRR), wk(h, W_LRU), wk(h, W_LRW), wk(h, W_LRT), wk(h, W_PEND)},
-                  n2);
+        // the stage's accumulators, and in the step's first reverse stage the
+        // step-level ones (ls0, lph), are cleared by ka_pre
+        FillList zero{};
+        for (double *p : {wk(h, W_TH), wk(h, W_RT), wk(h, W_LRR), wk(h, W_LRU), wk(h, W_LRW), wk(h, W_LRT),
+                          wk(h, W_PEND)})
+            zero.p[zero.n++] = p;
+        if (st == 2)
+            for (int f = 0; f < 7; ++f) {
+                zero.p[zero.n++] = A.ls0 + f * n2;
+                zero.p[zero.n++] = A.lph + f * n2;
+            }
         double *Lrup[2] = {wk(h, W_LRUP0), wk(h, W_LRUP1)};
-        hipLaunchKernelGGL(ka_pre<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, a);
+        // the limiter sums of the stage reversed before (into its input adjoint, lo
+        // here) run in ka_pre's extra blocks; those of stage 1 in their own launch,
+        // since the physics adjoint reads lo next
+        const LinSum lin{a.linb, a.lint, dset(lo, n2), h->ntiles2d};
+        hipLaunchKernelGGL(ka_pre<RC>, dim3(h->ntiles2d + (st < 2 ? h->nm : 0)), b2, 0, h->stream, h->dom_d,
+                           h->tiles2d, zv, in, a, zero, lin);
         hipLaunchKernelGGL(ka_scalar<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in,
                            dset(lo, n2), dset(li, n2), ls0, lph, a, Lrup[0], div);
-        hipLaunchKernelGGL(ka_lin<RC>, dim3(h->nm), dim3(256), 0, h->stream, h->dom_d, a, dset(li, n2));
+        if (st == 0)
+            hipLaunchKernelGGL(ka_lin<RC>, dim3(h->nm), dim3(256), 0, h->stream, h->dom_d, a, dset(li, n2));
         int cur = 0;
         for (int n = nsub; n >= 1; --n) {
             const int m = n - 1;
