@@ -130,6 +130,34 @@ def test_ensemble_members_match_single_runs():
     eng.close()
 
 
+def test_mixed_ensemble_of_odd_and_even_columns_matches_single_runs():
+    """Mixed form (level-pair work fields, members padded to an even nz): an
+    ensemble of three members with nz 17, 13, 24 and nx 64, 70, 128 in one
+    handle gives each member the same state, bit for bit, as a handle of its
+    own, with graph replay and the default tile shape."""
+    members = [ic.bubble(np.float64, nx=nx, nz=nz, d=250.0, seed=m)
+               for m, (nx, nz) in enumerate([(64, 17), (70, 13), (128, 24)])]
+    eng = core.Engine([p for p, _ in members], core.MIXED)
+    eng.use_graph(True)
+    for m, (p, s) in enumerate(members):
+        eng.upload(s, member=m)
+    eng.step(4)
+    for m, (p, s) in enumerate(members):
+        out = s.copy()
+        eng.download(out, member=m)
+        one = core.Engine(p, core.MIXED)
+        one.use_graph(True)
+        one.upload(s)
+        one.step(4)
+        ref = s.copy()
+        one.download(ref)
+        one.close()
+        for n in PROG:
+            np.testing.assert_array_equal(interior(p, getattr(out, n)), interior(p, getattr(ref, n)), err_msg=n)
+        assert np.all(np.isfinite(interior(p, out.rho_w)))
+    eng.close()
+
+
 def test_ensemble_workload_members_match_oracle():
     """Members 0, 1 and 255 of the 256-member workload (seed 2) in one handle
     with all 256 domains, graph replay, 10 steps, FP64 form against the oracle.
