@@ -368,10 +368,12 @@ template <class S> void enqueue_step(moist_handle *h) {
     if (h->any_rd)
         hipLaunchKernelGGL(k_radiation<S>, dim3(h->nctiles), dim3(COL_TILE), 0, h->stream, h->dom_d, h->ctiles, zv,
                            s, w);
+    // in the mixed form k_physics also writes the first stage's diagnostics,
+    // and stages 2 and 3 read those the previous stage's k_scalar wrote
+    constexpr bool PL = is_mixed<S>();
     hipLaunchKernelGGL(k_physics<S>, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d, h->tiles2d,
-                       zv, s, w, h->store_diag);
-    // stages 2 and 3 read the diagnostics the previous stage's k_scalar wrote
-    enqueue_stage<S>(h, S0_, SA_, S0_, 3, 0.0, h->ns_stage[0], 0, 0, true, true);
+                       zv, s, w, h->store_diag, PL ? 1 : 0);
+    enqueue_stage<S>(h, S0_, SA_, S0_, 3, 0.0, h->ns_stage[0], 0, 0, !PL, true);
     enqueue_stage<S>(h, SA_, SB_, S0_, 2, 0.0, h->ns_stage[1], 1, 0, false, true);
     enqueue_stage<S>(h, SB_, S0_, S0_, 1, 0.0, h->ns_stage[2], 2, 0, false, false);
     hipLaunchKernelGGL(k_step_done, dim3(1), dim3(64), 0, h->stream, h->step_ctr);
@@ -879,7 +881,7 @@ void enqueue_step_rec64(moist_handle *h, const double *sn) {
     const Work<double> w = work<double>(h);
     const SSet<double> s0 = sset<double>(h, S0_), ia = dset(rec_in(h, 1), n2), ib = dset(rec_in(h, 2), n2);
     hipLaunchKernelGGL(k_physics<double>, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d,
-                       h->tiles2d, zv, s0, w, 0);
+                       h->tiles2d, zv, s0, w, 0, 0);
     enqueue_stage_rec<double>(h, 0, s0, ia, s0, 3);
     enqueue_stage_rec<double>(h, 1, ia, ib, s0, 2);
     enqueue_stage_rec<double>(h, 2, ib, dset(h->adj->SC, n2), s0, 1);
@@ -895,7 +897,7 @@ void enqueue_step_rec_mixed(moist_handle *h) {
     const dim3 b2(TILE_X, TILE_Z);
     const Work<float> w = work<float>(h);
     const SSet<float> s0 = sset<float>(h, S0_), sa = sset<float>(h, SA_), sb = sset<float>(h, SB_);
-    hipLaunchKernelGGL(k_physics<float>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, s0, w, 0);
+    hipLaunchKernelGGL(k_physics<float>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, s0, w, 0, 0);
     enqueue_stage_rec<float>(h, 0, s0, sa, s0, 3);
     hipLaunchKernelGGL(ka_widen, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, sa,
                        dset(rec_in(h, 1), n2));

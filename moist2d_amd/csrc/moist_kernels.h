@@ -331,7 +331,8 @@ __device__ __forceinline__ S term_velocity(S rho_qr, S rho) {
 
 template <class S>
 __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
-                                                 ZVec zv, SSet<S> s, Work<S> w, int store_diag) {
+                                                 ZVec zv, SSet<S> s, Work<S> w, int store_diag,
+                                                 int diag_pairs) {
     using R = S;
     using KC = K<R>;
     const int4 t = tiles[blockIdx.x];
@@ -354,6 +355,13 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
     const bool sf = (k == 0) && (fl & F_SF);
     const int64_t x = D.offx + i;
     const R qv_sf = sf ? w.d_qv_sf[x] : R(0.0), th_sf = sf ? w.d_th_sf[x] : R(0.0);
+    // mixed form: the first stage's diagnostics of this same state (what
+    // k_diag would compute), stored in the level-pair layout with the rest
+    R dg_th = R(0.0), dg_pres = R(0.0), dg_rt2 = R(0.0);
+    if constexpr (is_mixed<S>()) {
+        if (diag_pairs)
+            diag_point<S>(D, zv, k, s.f[RHO][c], s.f[RT][c], s.f[QV][c], s.f[QC][c], qr_c, dg_th, dg_pres, dg_rt2);
+    }
     R dqv = R(0.0), dqc = R(0.0), dqr = R(0.0), dth = R(0.0);
     if (fl & F_MP) {
         const R rinv = fdiv(R(1.0), rho_k);
@@ -456,6 +464,13 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
     w.d_qv[c] = d_qv;
     w.d_qc[c] = dqc;
     w.d_qr[c] = d_qr;
+    if constexpr (is_mixed<S>()) {
+        if (diag_pairs) {
+            w.pres[c] = dg_pres;
+            w.theta[widx<true>(D, k, i)] = dg_th;
+            w.rt2[widx<true>(D, k, i)] = dg_rt2;
+        }
+    }
     if (store_diag) {
         w.dg_th_mp[c] = dth;
         w.dg_qv_mp[c] = dqv;

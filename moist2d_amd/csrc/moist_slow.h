@@ -5,7 +5,8 @@
 // A block owns ST_X columns x ST_Z levels.  It stages u and w at faces (formed
 // here from rho, rho_u and rho_w in the pair-layout variant, read from k_diag's
 // output otherwise), theta, rho, rho_u and rho_w of the tile plus a 2-cell
-// halo in LDS, then evaluates
+// halo in LDS (theta, u and w take turns in one array, each written while the
+// previous one's last phase runs, so a CU holds one more block), then evaluates
 // every flux once at the face, centre or corner where the reference defines
 // it and forms each tendency from flux differences:
 //   R_rho_theta: x- and z-face fluxes (moist.jl:1287-1329)
@@ -26,7 +27,8 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
                                                      int u_phys_2d) {
     using R = S;
     using KC = K<R>;
-    __shared__ R lU[SL_HZ][SL_HX], lW[SL_HZ][SL_HX], lT[SL_HZ][SL_HX];
+    // lX holds theta in the R_rho_theta phase, then u, then w
+    __shared__ R lX[SL_HZ][SL_HX];
     __shared__ R lR[SL_HZ][SL_HX], lRU[SL_HZ][SL_HX], lRW[SL_HZ][SL_HX];
     __shared__ R lF[SL_FLUX];
     __shared__ R lDZ[SL_HZ], lZ[SL_HZ];  // dz and z of levels k0-2 .. k0+ST_Z+1
@@ -43,9 +45,9 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
     auto DZ = [&](int k) -> R { return lDZ[k - k0 + 2]; };
     auto Z = [&](int k) -> R { return lZ[k - k0 + 2]; };
     // LDS accessors in absolute (k, i) of the tile neighbourhood
-    auto U = [&](int k, int i) -> R { return lU[k - k0 + 2][i - i0 + 2]; };
-    auto W = [&](int k, int i) -> R { return lW[k - k0 + 2][i - i0 + 2]; };
-    auto TH = [&](int k, int i) -> R { return lT[k - k0 + 2][i - i0 + 2]; };
+    auto U = [&](int k, int i) -> R { return lX[k - k0 + 2][i - i0 + 2]; };
+    auto W = [&](int k, int i) -> R { return lX[k - k0 + 2][i - i0 + 2]; };
+    auto TH = [&](int k, int i) -> R { return lX[k - k0 + 2][i - i0 + 2]; };
     auto RHL = [&](int k, int i) -> R { return lR[k - k0 + 2][i - i0 + 2]; };
     auto RU = [&](int k, int i) -> R { return lRU[k - k0 + 2][i - i0 + 2]; };
     auto RW = [&](int k, int i) -> R { return lRW[k - k0 + 2][i - i0 + 2]; };
@@ -55,8 +57,9 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
     // together instead of one loop iteration at a time.
     constexpr int NSL = (SL_HZ * SL_HX + ST_THREADS - 1) / ST_THREADS;
     constexpr int NCL = (ST_Z * ST_X + ST_THREADS - 1) / ST_THREADS;
+    R xu[NSL], xw[NSL];  // staged u and w, held until lX takes them
     {
-        R v[NSL][6];
+        R v[NSL][4];
 #pragma unroll
         for (int u = 0; u < NSL; ++u) {
             const int e = tid + u * ST_THREADS;
@@ -79,12 +82,12 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
                 uf = w.uf[q];
                 wf = w.wf[q];
             }
-            v[u][0] = ok ? uf : R(0);
-            v[u][1] = ok ? wf : R(0);
-            v[u][2] = ok ? w.theta[widx<P>(D, kc, ic)] : R(0);
-            v[u][3] = ok ? rho : R(0);
-            v[u][4] = ok ? ru : R(0);
-            v[u][5] = ok ? rw : R(0);
+            xu[u] = ok ? uf : R(0);
+            xw[u] = ok ? wf : R(0);
+            v[u][0] = ok ? w.theta[widx<P>(D, kc, ic)] : R(0);
+            v[u][1] = ok ? rho : R(0);
+            v[u][2] = ok ? ru : R(0);
+            v[u][3] = ok ? rw : R(0);
         }
         if (tid < SL_HZ) {
             const int k = k0 - 2 + tid;
@@ -96,12 +99,10 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         for (int u = 0; u < NSL; ++u) {
             const int e = tid + u * ST_THREADS;
             if (e >= SL_HZ * SL_HX) continue;
-            (&lU[0][0])[e] = v[u][0];
-            (&lW[0][0])[e] = v[u][1];
-            (&lT[0][0])[e] = v[u][2];
-            (&lR[0][0])[e] = v[u][3];
-            (&lRU[0][0])[e] = v[u][4];
-            (&lRW[0][0])[e] = v[u][5];
+            (&lX[0][0])[e] = v[u][0];
+            (&lR[0][0])[e] = v[u][1];
+            (&lRU[0][0])[e] = v[u][2];
+            (&lRW[0][0])[e] = v[u][3];
         }
     }
     // interior cells of this thread: levels k0+cr0 and k0+cr0+1 (one level pair)
@@ -164,6 +165,12 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         FTz[e] = fl;
     }
     __syncthreads();
+    // theta's last reader is done: lX takes u for the R_rho_u phase
+#pragma unroll
+    for (int u = 0; u < NSL; ++u) {
+        const int e = tid + u * ST_THREADS;
+        if (e < SL_HZ * SL_HX) (&lX[0][0])[e] = xu[u];
+    }
     {
         R o_rr[NCL], o_rt[NCL], o_mx[NCL], o_mz[NCL];
 #pragma unroll
@@ -234,6 +241,12 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         GUz[e] = gz;
     }
     __syncthreads();
+    // u's last reader is done: lX takes w for the R_rho_w phase
+#pragma unroll
+    for (int u = 0; u < NSL; ++u) {
+        const int e = tid + u * ST_THREADS;
+        if (e < SL_HZ * SL_HX) (&lX[0][0])[e] = xw[u];
+    }
     R o_ru[NCL];
 #pragma unroll
     for (int u = 0; u < NCL; ++u) {
