@@ -1150,6 +1150,12 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
             *out = h.release();
             return -2;
         }
+        // kernels index a member's cells (nz padded to even) with int
+        if ((int64_t)p->nx * (p->nz + 1) > INT_MAX) {
+            h->err = "member too large: nx * nz must stay below 2^31 cells";
+            *out = h.release();
+            return -2;
+        }
         if (m == 0) h->ns = p->ns;
         if (p->ns != h->ns) {
             h->err = "all members must share ns";

@@ -130,3 +130,27 @@ def test_bf2002_initial_state():
     # the bubble: theta scaled by 1 + theta_pert cos^2 / 300 at its centre
     th = s.rho_theta[inner] / s.rho[inner]
     assert th[:, ~far].max() > th[:, far].max() or th[:, ~far].max() > 0
+
+
+def test_create_rejects_bad_members_before_touching_the_device():
+    """moist_create validates every member first (no HIP call on this path):
+    halo > nx and a member of 2^31 cells or more return -2 with a message."""
+    L = core.lib()
+    p = make_params(np.float64, Nx=16, Nz=8, Lx=1600.0, H=800.0, dz0=100.0, z_fact=1.0, ns=6, tout_int=1.0)
+
+    def create(mutate):
+        c = p.to_c()
+        mutate(c)
+        arr = (ctypes.c_void_p * 1)(ctypes.addressof(c))
+        h = ctypes.c_void_p()
+        rc = L.moist_create(arr, 1, core.FP64, ctypes.byref(h))
+        msg = L.moist_last_error(h).decode() if h.value else ""
+        if h.value:
+            L.moist_destroy(h)
+        return rc, msg
+
+    rc, msg = create(lambda c: setattr(c, "halo", 17))
+    assert rc == -2 and "invalid member params" in msg
+    # 2^28 columns x (8 + 1) levels = 2.4e9 cells > 2^31 - 1
+    rc, msg = create(lambda c: setattr(c, "nx", 1 << 28))
+    assert rc == -2 and "member too large" in msg
