@@ -309,8 +309,10 @@ void launch_acoustic(moist_handle *h, const ZVec &zv, const SSet<S> &in, const W
 
 // ---- one RK stage ------------------------------------------------------------
 // diag_in: the stage diagnostics of the input are still to be computed
-// (k_diag); otherwise the previous stage's k_scalar wrote them.  diag_out:
-// this stage's k_scalar writes those of its output for the next stage.
+// (k_diag); otherwise they are already there (in the mixed form's step,
+// written by k_physics for stage 1 and by the previous stage's k_scalar for
+// stages 2 and 3).  diag_out: this stage's k_scalar writes those of its
+// output for the next stage.
 template <class S>
 void enqueue_stage(moist_handle *h, int in_set, int out_set, int s0_set, int div, double dt_ovr, int nsub,
                    int stage, int u_phys_2d, bool diag_in, bool diag_out) {

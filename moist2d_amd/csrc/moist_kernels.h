@@ -120,8 +120,9 @@ __device__ __forceinline__ S full_rt(const SSet<S> &s, int64_t c, const ZVec &zv
 // mixed form) leaves those to k_slow, which forms them from the tile it stages.
 // =============================================================================
 // One cell from its stored state values (rho and rho_theta are deviations
-// from the base state in the mixed form).  k_diag and the epilogue of
-// k_scalar (the next stage's diagnostics) share it.
+// from the base state in the mixed form).  k_diag, k_physics (the mixed
+// form's first stage) and the epilogue of k_scalar (the next stage's
+// diagnostics) share it.
 template <class S>
 __device__ __forceinline__ void diag_point(const Dom &D, const ZVec &zv, int k, S rho_s, S rt_s, S rqv, S rqc,
                                            S rqr, S &theta, S &pres, S &rt2) {
