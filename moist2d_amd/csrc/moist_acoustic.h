@@ -48,6 +48,12 @@ template <class S> struct AcBuf {
     double *cin;       // c' entering each chunk, [chunk][column]
     int64_t cin_stride;
     const S *rwp_in;   // rho_w' before this substep (the result goes to Work::rwp; may be the same array)
+    // the last substep of a stage in the pair-layout kernel: the stage's
+    // final update (moist.jl:1522-1539), state = stage input + perturbation
+    // of rho, rho_u, rho_theta and rho_w, goes straight to these fields
+    // instead of the perturbations going back to memory
+    S *fin_r, *fin_ru, *fin_rt, *fin_rw;
+    int stage;  // RK stage of its NaN report
 };
 
 // WPE: minimum waves per SIMD the register allocation must allow (1: no

@@ -12,8 +12,14 @@
 // the pair's inputs in flight together, then its two levels, then the
 // pair's stores of rho_u' and mom_x.  The FP64 form keeps k_acoustic: its
 // 16-column rows already fill a line.
+//
+// FIN (the last substep of a stage): the stage's final update of rho, rho_u,
+// rho_theta and rho_w (moist.jl:1522-1539) is done here, on the values the
+// substep holds, and written to the stage output (x-fastest, AcBuf::fin_*)
+// with its NaN checks (moist.jl:1543-1556); the final perturbations, which
+// nothing reads after this substep, are not stored.
 
-template <int L, int NT, int WPE>
+template <int L, int NT, int WPE, bool FIN>
 __global__ void __launch_bounds__(NT, WPE)
     k_acoustic_pairs(const Dom *__restrict__ doms, const int4 *__restrict__ atiles, ZVec zv, SSet<float> in,
                      Work<float> w, AcBuf<float> ab, int div, double dt_ovr, int nsub, int first) {
@@ -67,6 +73,8 @@ __global__ void __launch_bounds__(NT, WPE)
     R rwv[L + 1];  // rho_w' at faces k0 .. k0+L
     R swv[L + 1];  // input rho_w at levels k0-1 .. k0+L-1 (upwind branch tests)
     R mzv[L];      // mom_z at faces k0 .. k0+L-1
+    R inr[FIN ? L : 1], inrt[FIN ? L : 1];  // FIN: the stage input's rho and rho_theta
+    int nanf = 5;  // FIN: lowest NaN code of this thread's final values (5: none)
     double dd[L], cpv[L], pv[L];
 #pragma unroll
     for (int q = 0; q < L + 4; ++q) thx[q] = R(0.0);
@@ -97,6 +105,14 @@ __global__ void __launch_bounds__(NT, WPE)
             if (!(nl > 0 && j <= nz - 2)) rwv[q] = R(0.0);
             swv[q] = (nl > 0 && kq >= 0 && kq < nz) ? sv : R(0.0);
         }
+        if constexpr (FIN) {
+#pragma unroll
+            for (int q = 0; q < L; ++q) {
+                const int64_t qq = at(k0 + q);
+                inr[q] = in.f[RHO][qq];
+                inrt[q] = in.f[RT][qq];
+            }
+        }
     };
     // ---- phase 1: rho_u', explicit rho' and rho_theta', d rho_w'
     // Inputs arrive as level pairs, one pair's loads in flight together; the
@@ -111,7 +127,13 @@ __global__ void __launch_bounds__(NT, WPE)
         const int kp = min(k0 + 2 * m, kev);
         R mx_[GL], cc_[GL], cm_[GL], cx_[GL], oc_[GL], om_[GL], ox_[GL], r2c_[GL], r2m_[GL], r2x_[GL];
         R ruc_[GL], rum_[GL], Fuc_[GL], Fum_[GL], Fr_[GL], rp_[GL], thc_[GL], thm_[GL], thp_[GL], Ft_[GL];
+        R iru_[GL];  // FIN: the stage input's rho_u of the pair's levels
         auto load_group = [&]() {
+            if constexpr (FIN) {
+                const int64_t q0 = base + (int64_t)kp * nx + ic;
+                iru_[0] = in.f[RHOU][q0];
+                iru_[1] = in.f[RHOU][kp + 1 <= ke ? q0 + nx : q0];
+            }
             ldg(w.mom_x, kp, ic, mx_);
             ldg(ab.rtp_cur, kp, ic, cc_);
             ldg(ab.rtp_cur, kp, im, cm_);
@@ -192,7 +214,19 @@ __global__ void __launch_bounds__(NT, WPE)
             }
         }
         if (GL * m < nl) {
-            stg(ab.rup_out, k0 + GL * m, i, rui);
+            if constexpr (FIN) {
+#pragma unroll
+                for (int h = 0; h < GL; ++h) {
+                    const int l = GL * m + h;
+                    if (l < nl) {
+                        const R v = iru_[h] + rui[h];
+                        ab.fin_ru[base + (int64_t)(k0 + l) * nx + i] = v;
+                        if (isnan(v)) nanf = min(nanf, 2);
+                    }
+                }
+            } else {
+                stg(ab.rup_out, k0 + GL * m, i, rui);
+            }
             stg(w.mom_x, k0 + GL * m, i, mxo);
         }
     }
@@ -494,9 +528,21 @@ __global__ void __launch_bounds__(NT, WPE)
 #pragma unroll
     for (int m = 0; m < L / 2; ++m) {
         if (2 * m < nf) {
-            st2<true>(w.rwp, D, k0 + 2 * m, i, xr[2 * m], xr[2 * m + 1]);
+            if constexpr (!FIN) st2<true>(w.rwp, D, k0 + 2 * m, i, xr[2 * m], xr[2 * m + 1]);
             st2<true>(w.mom_z, D, k0 + 2 * m, i, mzv[2 * m] + DIV(xr[2 * m], R(nsub), rns),
                       mzv[2 * m + 1] + DIV(xr[2 * m + 1], R(nsub), rns));
+        }
+    }
+    if constexpr (FIN) {
+        // rho_w of the stage output: the lid level keeps the input's value
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            if (l < nl) {
+                const int k = k0 + l;
+                const R v = k < ke ? swv[l + 1] + xr[l] : swv[l + 1];
+                ab.fin_rw[base + (int64_t)k * nx + i] = v;
+                if (isnan(v)) nanf = min(nanf, 3);
+            }
         }
     }
     const R xb = R(x_below);
@@ -528,13 +574,24 @@ __global__ void __launch_bounds__(NT, WPE)
             }
             adv = DIV(-(Fkp - Fkm), dzk, rdzk);
             rtp_e[l] = rtp_e[l] + dt_s * adv;
-            // stored as soon as a level pair is complete; an odd top level is
-            // stored with the padding level above it
-            if ((l & 1) || l == nl - 1) {
+            if constexpr (FIN) {
+                const int64_t q = base + (int64_t)k * nx + i;
+                const R vr = inr[l] + rp_e[l], vt = inrt[l] + rtp_e[l];
+                ab.fin_r[q] = vr;
+                ab.fin_rt[q] = vt;
+                if (isnan(vr)) nanf = 1;
+                else if (isnan(vt)) nanf = min(nanf, 4);
+            } else if ((l & 1) || l == nl - 1) {
+                // stored as soon as a level pair is complete; an odd top level
+                // is stored with the padding level above it
                 const int l0 = l & ~1;
                 st2<true>(w.rp, D, k0 + l0, i, rp_e[l0], rp_e[l0 + 1]);
                 st2<true>(ab.rtp_out, D, k0 + l0, i, rtp_e[l0], rtp_e[l0 + 1]);
             }
         }
+    }
+    if constexpr (FIN) {
+        // the NaN codes of the state fields, as the final update reports them
+        if (nanf < 5) report_nan(w.nan_key, ((*w.step_ctr * 3 + ab.stage) << 4) | nanf);
     }
 }

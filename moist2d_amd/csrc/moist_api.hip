@@ -270,12 +270,18 @@ int set_acoustic_shape(moist_handle *h, int L, int NT, int W = 1) {
 
 template <class S, bool P, int L, int NT, int W>
 void launch_ac(moist_handle *h, const ZVec &zv, const SSet<S> &in, const Work<S> &w, const AcBuf<S> &ab, int div,
-               double dt_ovr, int nsub, int first) {
+               double dt_ovr, int nsub, int first, bool fin) {
     // dynamic LDS: the per-level geometry (5 doubles per level of the tallest member)
     if constexpr (P) {
         static_assert(is_mixed<S>(), "the level-pair layout is the mixed form's");
-        hipLaunchKernelGGL((k_acoustic_pairs<L, NT, W>), dim3(h->natiles), dim3(NT), (unsigned)(40 * h->max_ka),
-                           h->stream, h->dom_d, h->atiles, zv, in, w, ab, div, dt_ovr, nsub, first);
+        if (fin)
+            hipLaunchKernelGGL((k_acoustic_pairs<L, NT, W, true>), dim3(h->natiles), dim3(NT),
+                               (unsigned)(40 * h->max_ka), h->stream, h->dom_d, h->atiles, zv, in, w, ab, div, dt_ovr,
+                               nsub, first);
+        else
+            hipLaunchKernelGGL((k_acoustic_pairs<L, NT, W, false>), dim3(h->natiles), dim3(NT),
+                               (unsigned)(40 * h->max_ka), h->stream, h->dom_d, h->atiles, zv, in, w, ab, div, dt_ovr,
+                               nsub, first);
     } else {
         hipLaunchKernelGGL((k_acoustic<S, L, NT, W>), dim3(h->natiles), dim3(NT), (unsigned)(40 * h->max_ka),
                            h->stream, h->dom_d, h->atiles, zv, in, w, ab, div, dt_ovr, nsub, first);
@@ -283,14 +289,16 @@ void launch_ac(moist_handle *h, const ZVec &zv, const SSet<S> &in, const Work<S>
 }
 
 // P: layout of the work fields (level pairs on the mixed form's forward
-// path, x-fastest in the FP64 form and the adjoint's recording pass)
+// path, x-fastest in the FP64 form and the adjoint's recording pass).
+// fin (P only): the last substep of a stage, which also does the stage's
+// final update of rho, rho_u, rho_theta, rho_w into ab.fin_*.
 template <class S, bool P>
 void launch_acoustic(moist_handle *h, const ZVec &zv, const SSet<S> &in, const Work<S> &w, const AcBuf<S> &ab,
-                     int div, double dt_ovr, int nsub, int first) {
+                     int div, double dt_ovr, int nsub, int first, bool fin = false) {
     const int key = (h->ac_L * 10000 + h->ac_NT) * 10 + (h->ac_W == 8 ? 8 : 1);
 #define AC_CASE(L_, NT_, W_)                                                                 \
     case (L_ * 10000 + NT_) * 10 + W_:                                                       \
-        launch_ac<S, P, L_, NT_, W_>(h, zv, in, w, ab, div, dt_ovr, nsub, first);            \
+        launch_ac<S, P, L_, NT_, W_>(h, zv, in, w, ab, div, dt_ovr, nsub, first, fin);       \
         break;
     switch (key) {
         AC_CASE(2, 256, 1)
@@ -302,7 +310,7 @@ void launch_acoustic(moist_handle *h, const ZVec &zv, const SSet<S> &in, const W
         AC_CASE(2, 1024, 8)
         AC_CASE(4, 512, 8)
         AC_CASE(4, 1024, 8)
-    default: launch_ac<S, P, 4, 1024, 1>(h, zv, in, w, ab, div, dt_ovr, nsub, first); break;
+    default: launch_ac<S, P, 4, 1024, 1>(h, zv, in, w, ab, div, dt_ovr, nsub, first, fin); break;
     }
 #undef AC_CASE
 }
@@ -335,7 +343,11 @@ void enqueue_stage(moist_handle *h, int in_set, int out_set, int s0_set, int div
     int cur = 0, old = 0, ru = 0;
     for (int n = 0; n < nsub; ++n) {
         const int nxt = cur == old ? (cur + 1) % 3 : 3 - cur - old;
-        AcBuf<S> ab;
+        // in the mixed form the last substep also does the stage's final
+        // update of rho, rho_u, rho_theta and rho_w (k_scalar then does the
+        // scalars' only)
+        const bool fin = PL && n == nsub - 1;
+        AcBuf<S> ab{};
         ab.rup_in = rup[ru];
         ab.rup_out = rup[1 - ru];
         ab.rtp_cur = rtp[cur];
@@ -344,14 +356,22 @@ void enqueue_stage(moist_handle *h, int in_set, int out_set, int s0_set, int div
         ab.cin = h->cin;
         ab.cin_stride = h->nxs;
         ab.rwp_in = w.rwp;  // updated in place
+        if (fin) {
+            ab.fin_r = out.f[RHO];
+            ab.fin_ru = out.f[RHOU];
+            ab.fin_rt = out.f[RT];
+            ab.fin_rw = out.f[RHOW];
+            ab.stage = stage;
+        }
+        // the per-launch events time the plain substeps (one kernel variant)
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (h->ev_on) {
+        if (h->ev_on && !fin) {
             e0 = next_event(h);
             e1 = next_event(h);
             if (e0) hipEventRecord(e0, h->stream);
         }
-        launch_acoustic<S, PL>(h, zv, in, w, ab, div, dt_ovr, nsub, n == 0 ? 1 : 0);
-        if (h->ev_on && e1) hipEventRecord(e1, h->stream);
+        launch_acoustic<S, PL>(h, zv, in, w, ab, div, dt_ovr, nsub, n == 0 ? 1 : 0, fin);
+        if (e1) hipEventRecord(e1, h->stream);
         old = cur;
         cur = nxt;
         ru = 1 - ru;
@@ -856,7 +876,7 @@ template <class S> void enqueue_stage_rec(moist_handle *h, int st, SSet<S> in, S
     hipLaunchKernelGGL((k_slow<S, false>), dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles, zv,
                        in, s0, w, rec_rtp<S>(h, st, 0), 0);
     for (int n = 0; n < nsub; ++n) {
-        AcBuf<S> ab;
+        AcBuf<S> ab{};
         ab.rup_in = rec_rup<S>(h, st, n);
         ab.rup_out = rec_rup<S>(h, st, n + 1);
         ab.rtp_cur = rec_rtp<S>(h, st, n);

@@ -3,10 +3,12 @@
 //
 // A block owns a tile of ST_X columns x ST_Z levels.  The work fields it reads
 // (mom_x, mom_z and the final perturbations) are in layout P.  In the
-// pair-layout variant each thread's two interior cells are the levels of one
-// level pair of a column, and, given dg_th, it also writes the stage
-// diagnostics of its output (theta and rt2pres in level pairs, pressure
-// x-fastest), which the next stage then reads instead of running k_diag on it.  For each of qv, qc, qr
+// pair-layout variant the last acoustic substep has already done the final
+// update of rho, rho_u, rho_theta and rho_w; each thread's two interior cells
+// are the levels of one level pair of a column, and, given dg_th, it writes
+// the stage diagnostics of its output (theta and rt2pres in level pairs,
+// pressure x-fastest), which the next stage then reads instead of running
+// k_diag on it.  For each of qv, qc, qr
 // it stages q = rho_q / rho and rho of the tile plus a 2-cell halo in LDS,
 // evaluates every face flux once (Koren advective flux, moist.jl:850-945, and
 // the diffusive flux, moist.jl:1494-1510), and updates each cell from the
@@ -17,6 +19,78 @@
 
 constexpr int ST_X = 64, ST_Z = 8, ST_THREADS = 256;
 constexpr int ST_HX = ST_X + 4, ST_HZ = ST_Z + 4;  // haloed tile
+
+// Koren face value at x-face i+1/2 (compute_limited_flux_x, moist.jl:850-892)
+// from q = rho_q / rho of cells i-1 .. i+2, q[0] being cell i (a row of the
+// staged tile).  Both upwind branches are b + 1/2 phi(r) (b - a) with
+// r = (c - b) / (b - a): (a, b, c) = (q_i-1, q_i, q_i+1) for w > EPS and
+// (q_i+2, q_i+1, q_i) for w < -EPS.  SEL selects the operands and evaluates
+// that expression once (a wave whose faces differ in sign does not run both
+// branches; the mixed form's kernel is VALU-bound); otherwise it branches as
+// the reference does.  Both give the same value bit for bit.
+template <bool SEL, class R> __device__ __forceinline__ R face_x(R w, const R *q, R EPS) {
+    const R s_i = q[0], s_ip1 = q[1];
+    if constexpr (SEL) {
+        const R s_im1 = q[-1], s_ip2 = q[2];
+        const bool pos = w > EPS, neg = w < -EPS;
+        const R a = pos ? s_im1 : s_ip2, b = pos ? s_i : s_ip1, c = pos ? s_ip1 : s_i;
+        const R r_num = c - b, r_den = b - a;
+        const R rr = m_fabs(r_den) < EPS ? R(1.0) : fdiv(r_num, r_den);
+        const R lim = b + R(0.5) * koren(rr) * r_den;
+        return (pos || neg) ? lim : R(0.5) * (s_i + s_ip1);
+    } else {
+        if (w > EPS) {
+            const R r_num = s_ip1 - s_i, r_den = s_i - q[-1];
+            const R rr = m_fabs(r_den) < EPS ? R(1.0) : fdiv(r_num, r_den);
+            return s_i + R(0.5) * koren(rr) * r_den;
+        } else if (w < -EPS) {
+            const R r_num = s_i - s_ip1, r_den = s_ip1 - q[2];
+            const R rr = m_fabs(r_den) < EPS ? R(1.0) : fdiv(r_num, r_den);
+            return s_ip1 + R(0.5) * koren(rr) * r_den;
+        }
+        return R(0.5) * (s_i + s_ip1);
+    }
+}
+
+// Koren face value at z-face kk+1/2 (compute_limited_flux_z, moist.jl:895-945),
+// q[0] being level kk of a column of the staged tile (rows ST_HX apart).  The
+// negative branch is q_kk+1 - 1/2 phi(r) (q_kk+1 - q_kk+2) (Q3: minus, where x
+// has plus); SEL evaluates b + h phi(r) (b - a) with h = +-1/2, and
+// -(1/2 phi) (b - a) is exactly -(1/2 phi (b - a)).  At the bottom face for
+// w > EPS and the top face for w < -EPS the dz-weighted value is capped by the
+// positivity limiter, which divides rho_q read through the reference's linear
+// index (row kk of the first halo column: q_lo, q_hi) by the large-step flux.
+template <bool SEL, class R>
+__device__ __forceinline__ R face_z(R w, const R *q, int kk, int ke, R dzk, R dzk1, R q_lo, R q_hi, R dtL,
+                                    R EPS) {
+    const R s_k = q[0], s_kp1 = q[ST_HX];
+    if constexpr (SEL) {
+        const R s_km1 = q[-ST_HX], s_kp2 = q[2 * ST_HX];
+        const bool pos = w > EPS, neg = w < -EPS;
+        const R fh = f2h(s_k, s_kp1, dzk, dzk1);
+        const R a = pos ? s_km1 : s_kp2, b = pos ? s_k : s_kp1, c = pos ? s_kp1 : s_k;
+        const R r_num = c - b, r_den = b - a;
+        const R rr = m_fabs(r_den) < EPS ? R(1.0) : fdiv(r_num, r_den);
+        const R h = pos ? R(0.5) : R(-0.5);
+        const R lim = b + h * koren(rr) * r_den;
+        const bool bdy = (pos && kk == 0) || (neg && kk == ke - 1);
+        const R cap = jmin(fh, fdiv(pos ? q_lo : q_hi, (pos ? w : -w) * dtL));
+        return bdy ? cap : ((pos || neg) ? lim : fh);
+    } else {
+        if (w > EPS) {
+            if (kk == 0) return jmin(f2h(s_k, s_kp1, dzk, dzk1), fdiv(q_lo, w * dtL));
+            const R r_num = s_kp1 - s_k, r_den = s_k - q[-ST_HX];
+            const R rr = m_fabs(r_den) < EPS ? R(1.0) : fdiv(r_num, r_den);
+            return s_k + R(0.5) * koren(rr) * r_den;
+        } else if (w < -EPS) {
+            if (kk == ke - 1) return jmin(f2h(s_k, s_kp1, dzk, dzk1), fdiv(q_hi, -w * dtL));
+            const R r_num = s_k - s_kp1, r_den = s_kp1 - q[2 * ST_HX];
+            const R rr = m_fabs(r_den) < EPS ? R(1.0) : fdiv(r_num, r_den);
+            return s_kp1 - R(0.5) * koren(rr) * r_den;
+        }
+        return f2h(s_k, s_kp1, dzk, dzk1);
+    }
+}
 
 template <class S, bool P>
 __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
@@ -111,23 +185,19 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
         }
     }
 
-    // the final update's inputs (read-only here), loaded with the rest
-    R fin[NCL][4];
+    // the final update's inputs (read-only here), loaded with the rest.  In
+    // the pair-layout variant the last acoustic substep has done the final
+    // update of rho, rho_u, rho_theta and rho_w already; it reads back only the
+    // updated rho and rho_theta, for the next stage's diagnostics.
+    R fin[NCL][P ? 2 : 4];
     if constexpr (P) {
-        const int kq = min(k0 + 2 * (tid / ST_X), (nz - 1) & ~1), iq = min(i0 + tid % ST_X, nx - 1);
-        R p_rp[2], p_ru[2], p_rt[2], p_rw[2];
-        ld2<true>(w.rp, D, kq, iq, p_rp[0], p_rp[1]);
-        ld2<true>(rup_fin, D, kq, iq, p_ru[0], p_ru[1]);
-        ld2<true>(rtp_fin, D, kq, iq, p_rt[0], p_rt[1]);
-        ld2<true>(w.rwp, D, kq, iq, p_rw[0], p_rw[1]);
+        const int iq = min(i0 + tid % ST_X, nx - 1);
 #pragma unroll
         for (int u = 0; u < NCL; ++u) {
             const int k = min(k0 + 2 * (tid / ST_X) + u, ke);
             const int64_t q = gidx(k, iq);
-            fin[u][0] = in.f[RHO][q] + p_rp[u];
-            fin[u][1] = in.f[RHOU][q] + p_ru[u];
-            fin[u][2] = in.f[RT][q] + p_rt[u];
-            fin[u][3] = k < ke ? in.f[RHOW][q] + p_rw[u] : in.f[RHOW][q];
+            fin[u][0] = dg_th ? out.f[RHO][q] : R(0.0);
+            fin[u][1] = dg_th ? out.f[RT][q] : R(0.0);
         }
     } else {
 #pragma unroll
@@ -183,21 +253,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
                 const R rhou_face = mfx[u];
                 const int lr = r + 2, lc = f + 1;  // LDS column of cell ii
                 const R s_i = l_q[lr][lc], s_ip1 = l_q[lr][lc + 1];
-                R s_face;
-                if (rhou_face > EPS) {
-                    const R s_im1 = l_q[lr][lc - 1];
-                    const R r_num = s_ip1 - s_i, r_den = s_i - s_im1;
-                    const R rr = m_fabs(r_den) < EPS ? R(1.0) : fdiv(r_num, r_den);
-                    s_face = s_i + R(0.5) * koren(rr) * r_den;
-                } else if (rhou_face < -EPS) {
-                    const R s_ip2 = l_q[lr][lc + 2];
-                    const R r_num = s_i - s_ip1, r_den = s_ip1 - s_ip2;
-                    const R rr = m_fabs(r_den) < EPS ? R(1.0) : fdiv(r_num, r_den);
-                    s_face = s_ip1 + R(0.5) * koren(rr) * r_den;
-                } else {
-                    s_face = R(0.5) * (s_i + s_ip1);
-                }
-                fx = s_face * rhou_face;
+                fx = face_x<P>(rhou_face, &l_q[lr][lc], EPS) * rhou_face;
                 // rho * dq/dx at the face (moist.jl:1494-1495)
                 dfx = fdiv((l_rho[lr][lc + 1] + l_rho[lr][lc]) * R(0.5) * (s_ip1 - s_i), dx);
             }
@@ -217,32 +273,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
                 const int lr = g + 1, lc = c + 2;  // LDS row of level kk
                 const R s_k = l_q[lr][lc], s_kp1 = l_q[lr + 1][lc];
                 const R dzk = fdz0[u], dzk1 = fdz1[u];
-                R s_face;
-                if (rhow_face > EPS) {
-                    if (kk == 0) {
-                        s_face = f2h(s_k, s_kp1, dzk, dzk1);
-                        // linear index into rho_q: row kk of the first halo column
-                        s_face = jmin(s_face, fdiv(q_lo, rhow_face * dtL));
-                    } else {
-                        const R s_km1 = l_q[lr - 1][lc];
-                        const R r_num = s_kp1 - s_k, r_den = s_k - s_km1;
-                        const R rr = m_fabs(r_den) < EPS ? R(1.0) : fdiv(r_num, r_den);
-                        s_face = s_k + R(0.5) * koren(rr) * r_den;
-                    }
-                } else if (rhow_face < -EPS) {
-                    if (kk == ke - 1) {
-                        s_face = f2h(s_k, s_kp1, dzk, dzk1);
-                        s_face = jmin(s_face, fdiv(q_hi, -rhow_face * dtL));
-                    } else {
-                        const R s_kp2 = l_q[lr + 2][lc];
-                        const R r_num = s_k - s_kp1, r_den = s_kp1 - s_kp2;
-                        const R rr = m_fabs(r_den) < EPS ? R(1.0) : fdiv(r_num, r_den);
-                        s_face = s_kp1 - R(0.5) * koren(rr) * r_den;
-                    }
-                } else {
-                    s_face = f2h(s_k, s_kp1, dzk, dzk1);
-                }
-                fz = s_face * rhow_face;
+                fz = face_z<P>(rhow_face, &l_q[lr][lc], kk, ke, dzk, dzk1, q_lo, q_hi, dtL, EPS) * rhow_face;
                 // rho * dq/dz at the face (moist.jl:1507-1508)
                 dfz = fdiv(f2h(l_rho[lr + 1][lc], l_rho[lr][lc], dzk1, dzk) * (s_kp1 - s_k), fzz[u]);
             }
@@ -276,7 +307,8 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
         }
     }
 
-    // final update of rho, rho_u, rho_theta, rho_w (moist.jl:1522-1539) and NaN checks
+    // final update of rho, rho_u, rho_theta, rho_w (moist.jl:1522-1539) and NaN
+    // checks (the pair variant: done by the last acoustic substep)
     R dth[P ? NCL : 1], drt2[P ? NCL : 1];  // the next stage's theta and rt2pres (pair variant)
     if constexpr (P)
 #pragma unroll
@@ -288,24 +320,25 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
         const int k = k0 + r, i = i0 + c;
         if (e >= ST_Z * ST_X || k >= nz || i >= nx) continue;
         const int64_t q = gidx(k, i);
-        const R vr = fin[u][0], vu = fin[u][1], vt = fin[u][2], vw = fin[u][3];
-        out.f[RHO][q] = vr;
-        out.f[RHOU][q] = vu;
-        out.f[RT][q] = vt;
-        out.f[RHOW][q] = vw;
-        int f = 0;
-        if (isnan(vr)) f = 1;
-        else if (isnan(vu)) f = 2;
-        else if (isnan(vw)) f = 3;
-        else if (isnan(vt)) f = 4;
-        if (f) report_nan(w.nan_key, ((*w.step_ctr * 3 + stage) << 4) | f);
-        // the next stage's diagnostics of the updated cell (dg_th null: none)
         if constexpr (P) {
+            // the next stage's diagnostics of the updated cell (dg_th null: none)
             if (dg_th) {
                 R pres;
-                diag_point<S>(D, zv, k, vr, vt, qn[0][u], qn[1][u], qn[2][u], dth[u], pres, drt2[u]);
+                diag_point<S>(D, zv, k, fin[u][0], fin[u][1], qn[0][u], qn[1][u], qn[2][u], dth[u], pres, drt2[u]);
                 dg_pres[q] = pres;
             }
+        } else {
+            const R vr = fin[u][0], vu = fin[u][1], vt = fin[u][2], vw = fin[u][3];
+            out.f[RHO][q] = vr;
+            out.f[RHOU][q] = vu;
+            out.f[RT][q] = vt;
+            out.f[RHOW][q] = vw;
+            int f = 0;
+            if (isnan(vr)) f = 1;
+            else if (isnan(vu)) f = 2;
+            else if (isnan(vw)) f = 3;
+            else if (isnan(vt)) f = 4;
+            if (f) report_nan(w.nan_key, ((*w.step_ctr * 3 + stage) << 4) | f);
         }
         // stale halo values surface_fluxes! reads at the next step (moist.jl:723,736)
         if (k == 0 && i == nx - 1) w.sfh[2 * m + 0] = double(in.f[RHOU][q]);

@@ -200,6 +200,24 @@ def test_nan_is_reported_like_the_reference():
         core.rk3_step(s, p)
 
 
+@pytest.mark.parametrize("precision", [core.FP64, core.MIXED])
+@pytest.mark.parametrize("field", ["rho", "rho_theta", "rho_qc"])
+def test_nan_report_names_the_first_field_in_both_forms(precision, field):
+    """The state fields' NaN checks run in k_scalar (FP64 form) or in the last
+    acoustic substep of the stage (mixed form); either way the first field in
+    the reference's order is named.  A NaN in rho_theta or rho_qc reaches the
+    pressure (through cp/cv and R), then rho_u' and the acoustic rho' update
+    within the first stage, so rho is the first field named, as in the
+    reference's dataflow."""
+    p, s = ic.bubble(np.float64, nx=64, nz=16)
+    getattr(s, field)[5, p.halo + 3] = np.nan
+    eng = core.Engine(p, precision)
+    eng.upload(s)
+    with pytest.raises(RuntimeError, match="NaN detected in rho$"):
+        eng.step(1)
+    eng.close()
+
+
 def test_conservation_coldpool_1024x256():
     p, s = ic.coldpool(np.float64, nx=1024, nz=256)
     for prec, tol in ((core.FP64, 1e-13), (core.MIXED, 1e-6)):
