@@ -19,6 +19,18 @@
 // neighbouring cells, so evaluating it once gives the same numbers.
 
 constexpr int SL_HX = ST_X + 4, SL_HZ = ST_Z + 4;
+
+// 3rd-order upwind flux of the reference, w (-m + 5c + 2p) / 6 with
+// (m, c, p) = (m0, c0, p0) when pos and (m1, c1, p1) when neg, else the
+// centred value: the operands are selected and the expression evaluated
+// once, so a wave whose points differ in direction does not run every
+// branch.  The value is the branch's own, bit for bit.
+template <class R>
+__device__ __forceinline__ R upw_flux(R w, bool pos, bool neg, R m0, R c0, R p0, R m1, R c1, R p1, R centred) {
+    const R m = pos ? m0 : m1, c = pos ? c0 : c1, p = pos ? p0 : p1;
+    const R up = fdiv(w * upw3(m, c, p), R(6.0));
+    return (pos || neg) ? up : centred;
+}
 constexpr int SL_FLUX = 5 * (ST_Z + 1) * (ST_X + 1);  // largest flux phase (w)
 
 template <class S, bool P>
@@ -146,9 +158,8 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         R fl = R(0);
         if (k < nz) {
             const R ru = RU(k, L);
-            if (ru > EPS) fl = fdiv(ru * upw3(TH(k, L - 1), TH(k, L), TH(k, L + 1)), R(6.0));
-            else if (ru < -EPS) fl = fdiv(ru * upw3(TH(k, L + 2), TH(k, L + 1), TH(k, L)), R(6.0));
-            else fl = ru * (TH(k, L) + TH(k, L + 1)) * R(0.5);
+            const R t_m = TH(k, L - 1), t_c = TH(k, L), t_p = TH(k, L + 1), t_pp = TH(k, L + 2);
+            fl = upw_flux(ru, ru > EPS, ru < -EPS, t_m, t_c, t_p, t_pp, t_p, t_c, ru * (t_c + t_p) * R(0.5));
         }
         FTx[e] = fl;
     }
@@ -158,9 +169,9 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         R fl = R(0);
         if (k >= 0 && k <= ke - 1) {
             const R rw = RW(k, i);
-            if (rw > EPS && k > 0) fl = fdiv(rw * upw3(TH(k - 1, i), TH(k, i), TH(k + 1, i)), R(6.0));
-            else if (rw < -EPS && k < ke - 1) fl = fdiv(rw * upw3(TH(k + 2, i), TH(k + 1, i), TH(k, i)), R(6.0));
-            else fl = rw * f2h(TH(k, i), TH(k + 1, i), DZ(k), DZ(k + 1));
+            const R t_m = TH(k - 1, i), t_c = TH(k, i), t_p = TH(k + 1, i), t_pp = TH(k + 2, i);
+            fl = upw_flux(rw, rw > EPS && k > 0, rw < -EPS && k < ke - 1, t_m, t_c, t_p, t_pp, t_p, t_c,
+                          rw * f2h(t_c, t_p, DZ(k), DZ(k + 1)));
         }
         FTz[e] = fl;
     }
@@ -216,9 +227,8 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         R fl = R(0), gx = R(0);
         if (k < nz) {
             const R rc = R(0.5) * (RU(k, i - 1) + RU(k, i));
-            if (rc > EPS) fl = fdiv(rc * upw3(U(k, i - 2), U(k, i - 1), U(k, i)), R(6.0));
-            else if (rc < -EPS) fl = fdiv(rc * upw3(U(k, i + 1), U(k, i), U(k, i - 1)), R(6.0));
-            else fl = rc * (U(k, i - 1) + U(k, i)) * R(0.5);
+            const R u_mm = U(k, i - 2), u_m = U(k, i - 1), u_c = U(k, i), u_p = U(k, i + 1);
+            fl = upw_flux(rc, rc > EPS, rc < -EPS, u_mm, u_m, u_c, u_p, u_c, u_m, rc * (u_m + u_c) * R(0.5));
             gx = fdiv(U(k, i) - U(k, i - 1), dx) * RHL(k, i);
         }
         CUx[e] = fl;
@@ -230,9 +240,9 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         R fl = R(0), gz = R(0);
         if (k >= 0 && k <= ke - 1) {
             const R rw = R(0.5) * (RW(k, i) + RW(k, i + 1));
-            if (rw > EPS && k > 0) fl = fdiv(rw * upw3(U(k - 1, i), U(k, i), U(k + 1, i)), R(6.0));
-            else if (rw < -EPS && k < ke - 1) fl = fdiv(rw * upw3(U(k + 2, i), U(k + 1, i), U(k, i)), R(6.0));
-            else fl = rw * f2h(U(k, i), U(k + 1, i), DZ(k), DZ(k + 1));
+            const R u_m = U(k - 1, i), u_c = U(k, i), u_p = U(k + 1, i), u_pp = U(k + 2, i);
+            fl = upw_flux(rw, rw > EPS && k > 0, rw < -EPS && k < ke - 1, u_m, u_c, u_p, u_pp, u_p, u_c,
+                          rw * f2h(u_c, u_p, DZ(k), DZ(k + 1)));
             const R u_kp = fdiv(U(k + 1, i) - U(k, i), Z(k + 1) - Z(k));
             const R rho_kp = R(0.25) * (RHL(k, i) + RHL(k + 1, i) + RHL(k, i + 1) + RHL(k + 1, i + 1));
             gz = u_kp * rho_kp;
@@ -304,9 +314,9 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         R zc = R(0), vz = R(0);
         if (k >= 0 && k < ke - 1) {
             const R rw = f2h(RW(k, i), RW(k + 1, i), DZ(k), DZ(k + 1));
-            if (rw > EPS && k > 0) zc = fdiv(rw * upw3(W(k - 1, i), W(k, i), W(k + 1, i)), R(6.0));
-            else if (rw < -EPS && k < ke - 2) zc = fdiv(rw * upw3(W(k + 2, i), W(k + 1, i), W(k, i)), R(6.0));
-            else zc = rw * f2h(W(k, i), W(k + 1, i), DZ(k), DZ(k + 1));
+            const R w_m = W(k - 1, i), w_c = W(k, i), w_p = W(k + 1, i), w_pp = W(k + 2, i);
+            zc = upw_flux(rw, rw > EPS && k > 0, rw < -EPS && k < ke - 2, w_m, w_c, w_p, w_pp, w_p, w_c,
+                          rw * f2h(w_c, w_p, DZ(k), DZ(k + 1)));
             vz = fdiv(W(k + 1, i) - W(k, i), DZ(k + 1)) * RHL(k + 1, i);
         }
         ZC[e] = zc;
