@@ -70,6 +70,12 @@ template <class R> __device__ __forceinline__ R jmin(R x, R y) {
     R am = signbit(diff) ? x : y;
     return (isnan(x) || isnan(y)) ? diff : am;
 }
+// In float, the IEEE 754-2019 minimum and maximum (gfx950 v_minimum3_f32,
+// v_maximum3_f32: one instruction) have exactly these semantics: a NaN
+// operand gives NaN, -0 orders below +0, and otherwise the smaller (larger)
+// value; only the NaN's payload can differ.
+__device__ __forceinline__ float jmax(float x, float y) { return __builtin_elementwise_maximum(x, y); }
+__device__ __forceinline__ float jmin(float x, float y) { return __builtin_elementwise_minimum(x, y); }
 
 // ---- math in the compute type -----------------------------------------------
 // Division: IEEE division in double (the FP64 form performs the reference's
