@@ -89,6 +89,19 @@ template <class R> __device__ __forceinline__ R f2h(R v1, R v2, R d1, R d2) {
     return fdiv(v1 * d2 + v2 * d1, d1 + d2);
 }
 
+// Division by a divisor used several times: rb = rcp_once(b) once, then
+// fdiv_r(a, b, rb).  The float form's division is a times the hardware
+// reciprocal, so this gives the same values as fdiv with one reciprocal; in
+// double it is the IEEE division (rb is not used).
+__device__ __forceinline__ float rcp_once(float b) { return __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double rcp_once(double) { return 0.0; }
+__device__ __forceinline__ float fdiv_r(float a, float, float rb) { return a * rb; }
+__device__ __forceinline__ double fdiv_r(double a, double b, double) { return a / b; }
+// f2h with rs = rcp_once(d1 + d2) (the same for d2 + d1)
+template <class R> __device__ __forceinline__ R f2h_r(R v1, R v2, R d1, R d2, R rs) {
+    return fdiv_r(v1 * d2 + v2 * d1, d1 + d2, rs);
+}
+
 // Koren (1993) limiter (moist.jl:844-847)
 template <class R> __device__ __forceinline__ R koren(R r) {
     return jmax(R(0.0), jmin(R(2.0) * r, jmin(fdiv(R(1.0) + R(2.0) * r, R(3.0)), R(2.0))));

@@ -60,14 +60,15 @@ template <bool SEL, class R> __device__ __forceinline__ R face_x(R w, const R *q
 // w > EPS and the top face for w < -EPS the dz-weighted value is capped by the
 // positivity limiter, which divides rho_q read through the reference's linear
 // index (row kk of the first halo column: q_lo, q_hi) by the large-step flux.
+// rs = rcp_once(dzk + dzk1).
 template <bool SEL, class R>
-__device__ __forceinline__ R face_z(R w, const R *q, int kk, int ke, R dzk, R dzk1, R q_lo, R q_hi, R dtL,
+__device__ __forceinline__ R face_z(R w, const R *q, int kk, int ke, R dzk, R dzk1, R rs, R q_lo, R q_hi, R dtL,
                                     R EPS) {
     const R s_k = q[0], s_kp1 = q[ST_HX];
     if constexpr (SEL) {
         const R s_km1 = q[-ST_HX], s_kp2 = q[2 * ST_HX];
         const bool pos = w > EPS, neg = w < -EPS;
-        const R fh = f2h(s_k, s_kp1, dzk, dzk1);
+        const R fh = f2h_r(s_k, s_kp1, dzk, dzk1, rs);
         const R a = pos ? s_km1 : s_kp2, b = pos ? s_k : s_kp1, c = pos ? s_kp1 : s_k;
         const R r_num = c - b, r_den = b - a;
         const R rr = m_fabs(r_den) < EPS ? R(1.0) : fdiv(r_num, r_den);
@@ -78,17 +79,17 @@ __device__ __forceinline__ R face_z(R w, const R *q, int kk, int ke, R dzk, R dz
         return bdy ? cap : ((pos || neg) ? lim : fh);
     } else {
         if (w > EPS) {
-            if (kk == 0) return jmin(f2h(s_k, s_kp1, dzk, dzk1), fdiv(q_lo, w * dtL));
+            if (kk == 0) return jmin(f2h_r(s_k, s_kp1, dzk, dzk1, rs), fdiv(q_lo, w * dtL));
             const R r_num = s_kp1 - s_k, r_den = s_k - q[-ST_HX];
             const R rr = m_fabs(r_den) < EPS ? R(1.0) : fdiv(r_num, r_den);
             return s_k + R(0.5) * koren(rr) * r_den;
         } else if (w < -EPS) {
-            if (kk == ke - 1) return jmin(f2h(s_k, s_kp1, dzk, dzk1), fdiv(q_hi, -w * dtL));
+            if (kk == ke - 1) return jmin(f2h_r(s_k, s_kp1, dzk, dzk1, rs), fdiv(q_hi, -w * dtL));
             const R r_num = s_k - s_kp1, r_den = s_kp1 - q[2 * ST_HX];
             const R rr = m_fabs(r_den) < EPS ? R(1.0) : fdiv(r_num, r_den);
             return s_kp1 - R(0.5) * koren(rr) * r_den;
         }
-        return f2h(s_k, s_kp1, dzk, dzk1);
+        return f2h_r(s_k, s_kp1, dzk, dzk1, rs);
     }
 }
 
@@ -101,6 +102,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
     using R = S;
     __shared__ R l_rho[ST_HZ][ST_HX];
     __shared__ R l_q[ST_HZ][ST_HX];
+    __shared__ R l_ri[ST_HZ][ST_HX];  // 1 / rho for q = rho_q / rho (float form)
     __shared__ R l_fx[ST_Z][ST_X + 1];   // advective flux at x-face i0+f-1/2
     __shared__ R l_dx[ST_Z][ST_X + 1];   // diffusive flux at the same faces
     __shared__ R l_fz[ST_Z + 1][ST_X];   // advective flux at z-face k0+g-1/2
@@ -143,6 +145,10 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
     }
     R mfx[NFX], mfz[NFZ];  // mass flux at the x and z faces of this thread
     R fdz0[NFZ], fdz1[NFZ], fzz[NFZ], cdz[NCL];  // dz below / above each z face, z spacing, dz of each cell
+    // their reciprocals for the float form's divisions (rcp_once), the same
+    // for the three scalars
+    R rsz[NFZ], rzz[NFZ], rcz[NCL];
+    const R rdx = rcp_once(dx);
 #pragma unroll
     for (int u = 0; u < NFX; ++u) {
         const int e = tid + u * ST_THREADS;
@@ -161,15 +167,19 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
         fdz0[u] = R(dzd[kc]);
         fdz1[u] = R(dzd[kc1]);
         fzz[u] = R(zd[kc1]) - R(zd[kc]);
+        rsz[u] = rcp_once(fdz0[u] + fdz1[u]);
+        rzz[u] = rcp_once(fzz[u]);
     }
 #pragma unroll
     for (int u = 0; u < NCL; ++u) {
         const int e = P ? (2 * (tid / ST_X) + u) * ST_X + tid % ST_X : tid + u * ST_THREADS;
         const int k = k0 + e / ST_X;
         cdz[u] = R(dzd[min(k, ke)]);
+        rcz[u] = rcp_once(cdz[u]);
     }
 
-    // rho of the haloed tile (full density), once for the three scalars
+    // rho of the haloed tile (full density), once for the three scalars, and
+    // its reciprocal for q = rho_q / rho (float form)
     {
         R v[NSTG];
 #pragma unroll
@@ -181,7 +191,10 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
 #pragma unroll
         for (int u = 0; u < NSTG; ++u) {
             const int e = tid + u * ST_THREADS;
-            if (e < ST_HZ * ST_HX) (&l_rho[0][0])[e] = v[u];
+            if (e < ST_HZ * ST_HX) {
+                (&l_rho[0][0])[e] = v[u];
+                if constexpr (is_mixed<R>()) (&l_ri[0][0])[e] = rcp_once(v[u]);
+            }
         }
     }
 
@@ -238,7 +251,8 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
 #pragma unroll
         for (int u = 0; u < NSTG; ++u) {
             const int e = tid + u * ST_THREADS;
-            if (e < ST_HZ * ST_HX) (&l_q[0][0])[e] = gst[u] >= 0 ? fdiv(qv_[u], (&l_rho[0][0])[e]) : R(0.0);
+            if (e < ST_HZ * ST_HX)
+                (&l_q[0][0])[e] = gst[u] >= 0 ? fdiv_r(qv_[u], (&l_rho[0][0])[e], (&l_ri[0][0])[e]) : R(0.0);
         }
         __syncthreads();
         // x faces: face f lies between cells i0+f-1 and i0+f (compute_limited_flux_x at ii = i0+f-1)
@@ -255,7 +269,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
                 const R s_i = l_q[lr][lc], s_ip1 = l_q[lr][lc + 1];
                 fx = face_x<P>(rhou_face, &l_q[lr][lc], EPS) * rhou_face;
                 // rho * dq/dx at the face (moist.jl:1494-1495)
-                dfx = fdiv((l_rho[lr][lc + 1] + l_rho[lr][lc]) * R(0.5) * (s_ip1 - s_i), dx);
+                dfx = fdiv_r((l_rho[lr][lc + 1] + l_rho[lr][lc]) * R(0.5) * (s_ip1 - s_i), dx, rdx);
             }
             l_fx[r][f] = fx;
             l_dx[r][f] = dfx;
@@ -273,9 +287,11 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
                 const int lr = g + 1, lc = c + 2;  // LDS row of level kk
                 const R s_k = l_q[lr][lc], s_kp1 = l_q[lr + 1][lc];
                 const R dzk = fdz0[u], dzk1 = fdz1[u];
-                fz = face_z<P>(rhow_face, &l_q[lr][lc], kk, ke, dzk, dzk1, q_lo, q_hi, dtL, EPS) * rhow_face;
+                fz = face_z<P>(rhow_face, &l_q[lr][lc], kk, ke, dzk, dzk1, rsz[u], q_lo, q_hi, dtL, EPS) *
+                     rhow_face;
                 // rho * dq/dz at the face (moist.jl:1507-1508)
-                dfz = fdiv(f2h(l_rho[lr + 1][lc], l_rho[lr][lc], dzk1, dzk) * (s_kp1 - s_k), fzz[u]);
+                dfz = fdiv_r(f2h_r(l_rho[lr + 1][lc], l_rho[lr][lc], dzk1, dzk, rsz[u]) * (s_kp1 - s_k), fzz[u],
+                             rzz[u]);
             }
             l_fz[g][c] = fz;
             l_dz[g][c] = dfz;
@@ -291,11 +307,11 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
             if (k >= nz || i >= nx) continue;
             const int64_t q = gidx(k, i);
             const R dzk = cdz[u];
-            const R adv_x = fdiv(l_fx[r][c + 1] - l_fx[r][c], dx);
-            const R adv_z = fdiv(l_fz[r + 1][c] - l_fz[r][c], dzk);
+            const R adv_x = fdiv_r(l_fx[r][c + 1] - l_fx[r][c], dx, rdx);
+            const R adv_z = fdiv_r(l_fz[r + 1][c] - l_fz[r][c], dzk, rcz[u]);
             const R adv = -(adv_x + adv_z);
-            const R diff_x = fdiv(kap * (l_dx[r][c + 1] - l_dx[r][c]), dx);
-            const R diff_z = fdiv(kap * (l_dz[r + 1][c] - l_dz[r][c]), dzk);
+            const R diff_x = fdiv_r(kap * (l_dx[r][c + 1] - l_dx[r][c]), dx, rdx);
+            const R diff_z = fdiv_r(kap * (l_dz[r + 1][c] - l_dz[r][c]), dzk, rcz[u]);
             const R v = s0v[u] + dt * (adv + (diff_x + diff_z) + dpv[u]);
             out.f[QV + sc][q] = v;
             if constexpr (P) {
