@@ -678,6 +678,16 @@ __global__ void __launch_bounds__(256)
     const WpCoef<RC> W{D, zv.z + D.offz, zv.dz + D.offz, zv.geo + D.offz, a.theta, a.rt2, nx, live ? i : 0,
                    (dt_s * fact_tp) * (dt_s * fact_tp)};
     const int L = (n + CH - 1) / CH, j0 = ch * L, j1 = live ? min(j0 + L, n) : j0;
+    // Chunks of at most LR rows (one column per wave up to nz = 257) keep
+    // their c', 1/m, a, right-hand side and w in registers: one load phase
+    // and one store phase instead of a memory round trip between the scans.
+    // The arithmetic is the same, in the same order.  (L is the same for the
+    // whole wave: its columns belong to one member.)
+    constexpr int LR = 4;
+    const bool regs = L <= LR;
+    double cpv[LR], rmv[LR], acv[LR];
+#pragma unroll
+    for (int q = 0; q < LR; ++q) cpv[q] = rmv[q] = acv[q] = 0.0;
     if (factor) {
         double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;
         for (int j = j0; j < j1; ++j) {
@@ -708,16 +718,106 @@ __global__ void __launch_bounds__(256)
         // c' entering this chunk: the chunks below applied to c'_-1 = 0, i.e. (p, q) = (0, 1)
         const double e01 = __shfl_up(m01, CW), e11 = __shfl_up(m11, CW);
         double cprev = ch == 0 ? 0.0 : e01 / e11;
-        for (int j = j0; j < j1; ++j) {
-            const double aj = j > 0 ? W.a(j) : 0.0;
-            const double m = W.b(j) + 1.0 - aj * cprev;
-            const double rm = 1.0 / m;
-            const double cp = j < n - 1 ? W.c(j) * rm : 0.0;
-            a.cpr[W.G(j)] = cp;
-            a.rmi[W.G(j)] = rm;
-            a.aco[W.G(j)] = aj;
-            cprev = cp;
+        if (regs) {
+#pragma unroll
+            for (int q = 0; q < LR; ++q) {
+                const int j = j0 + q;
+                if (j < j1) {
+                    const double aj = j > 0 ? W.a(j) : 0.0;
+                    const double m = W.b(j) + 1.0 - aj * cprev;
+                    const double rm = 1.0 / m;
+                    const double cp = j < n - 1 ? W.c(j) * rm : 0.0;
+                    a.cpr[W.G(j)] = cp;
+                    a.rmi[W.G(j)] = rm;
+                    a.aco[W.G(j)] = aj;
+                    cpv[q] = cp;
+                    rmv[q] = rm;
+                    acv[q] = aj;
+                    cprev = cp;
+                }
+            }
+        } else {
+            for (int j = j0; j < j1; ++j) {
+                const double aj = j > 0 ? W.a(j) : 0.0;
+                const double m = W.b(j) + 1.0 - aj * cprev;
+                const double rm = 1.0 / m;
+                const double cp = j < n - 1 ? W.c(j) * rm : 0.0;
+                a.cpr[W.G(j)] = cp;
+                a.rmi[W.G(j)] = rm;
+                a.aco[W.G(j)] = aj;
+                cprev = cp;
+            }
         }
+    }
+    if (regs) {
+        double lx[LR];  // right-hand side
+#pragma unroll
+        for (int q = 0; q < LR; ++q) {
+            const int j = j0 + q;
+            lx[q] = j < j1 ? a.Lrwp[W.G(j)] : 0.0;
+            if (!factor && j < j1) {
+                cpv[q] = a.cpr[W.G(j)];
+                rmv[q] = a.rmi[W.G(j)];
+                acv[q] = a.aco[W.G(j)];
+            }
+        }
+        // c' of row j0-1 (last row of the chunk below, which is full) and a of
+        // row j1 (first row of the chunk above), from the neighbouring lanes
+        double cp_last = 0.0;
+#pragma unroll
+        for (int q = 0; q < LR; ++q)
+            if (q == L - 1) cp_last = cpv[q];
+        const double cp_below = __shfl_up(cp_last, CW), a_above = __shfl_down(acv[0], CW);
+        // forward sweep w_j = Lx_j - c'_j-1 w_j-1
+        double A = 1.0, B = 0.0;
+#pragma unroll
+        for (int q = 0; q < LR; ++q) {
+            const int j = j0 + q;
+            if (j < j1) {
+                const double al = j > 0 ? -(q == 0 ? cp_below : cpv[q > 0 ? q - 1 : 0]) : 0.0;
+                B = al * B + lx[q];
+                A = al * A;
+            }
+        }
+        scan_affine_up<CH>(A, B, ch);
+        double wv = __shfl_up(B, CW);
+        if (ch == 0) wv = 0.0;
+#pragma unroll
+        for (int q = 0; q < LR; ++q) {
+            const int j = j0 + q;
+            if (j < j1) {
+                const double al = j > 0 ? -(q == 0 ? cp_below : cpv[q > 0 ? q - 1 : 0]) : 0.0;
+                wv = al * wv + lx[q];
+                lx[q] = wv;  // w
+            }
+        }
+        // back substitution y_j = rm_j w_j - rm_j a_j+1 y_j+1, chunks from the top
+        A = 1.0;
+        B = 0.0;
+#pragma unroll
+        for (int q = LR - 1; q >= 0; --q) {
+            const int j = j0 + q;
+            if (j < j1) {
+                const double rm = rmv[q];
+                const double al = j < n - 1 ? -rm * (q + 1 < L ? acv[q < LR - 1 ? q + 1 : 0] : a_above) : 0.0;
+                B = al * B + rm * lx[q];
+                A = al * A;
+            }
+        }
+        scan_affine_down<CH>(A, B, ch);
+        double yv = __shfl_down(B, CW);
+        if (ch == CH - 1) yv = 0.0;
+#pragma unroll
+        for (int q = LR - 1; q >= 0; --q) {
+            const int j = j0 + q;
+            if (j < j1) {
+                const double rm = rmv[q];
+                const double al = j < n - 1 ? -rm * (q + 1 < L ? acv[q < LR - 1 ? q + 1 : 0] : a_above) : 0.0;
+                yv = al * yv + rm * lx[q];
+                a.Y[W.G(j)] = yv;
+            }
+        }
+        return;
     }
     // forward sweep w_j = Lx_j - c'_j-1 w_j-1
     double A = 1.0, B = 0.0;
