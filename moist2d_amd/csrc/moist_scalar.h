@@ -26,8 +26,8 @@ constexpr int ST_HX = ST_X + 4, ST_HZ = ST_Z + 4;  // haloed tile
 // r = (c - b) / (b - a): (a, b, c) = (q_i-1, q_i, q_i+1) for w > EPS and
 // (q_i+2, q_i+1, q_i) for w < -EPS.  SEL selects the operands and evaluates
 // that expression once (a wave whose faces differ in sign does not run both
-// branches; the mixed form's kernel is VALU-bound); otherwise it branches as
-// the reference does.  Both give the same value bit for bit.
+// branches; the float kernels are VALU-bound); the FP64 form branches as the
+// reference does, with fewer registers.  Both give the same value bit for bit.
 template <bool SEL, class R> __device__ __forceinline__ R face_x(R w, const R *q, R EPS) {
     const R s_i = q[0], s_ip1 = q[1];
     if constexpr (SEL) {
@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
                 const R rhou_face = mfx[u];
                 const int lr = r + 2, lc = f + 1;  // LDS column of cell ii
                 const R s_i = l_q[lr][lc], s_ip1 = l_q[lr][lc + 1];
-                fx = face_x<P>(rhou_face, &l_q[lr][lc], EPS) * rhou_face;
+                fx = face_x<is_mixed<S>()>(rhou_face, &l_q[lr][lc], EPS) * rhou_face;
                 // rho * dq/dx at the face (moist.jl:1494-1495)
                 dfx = fdiv_r((l_rho[lr][lc + 1] + l_rho[lr][lc]) * R(0.5) * (s_ip1 - s_i), dx, rdx);
             }
@@ -287,7 +287,8 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
                 const int lr = g + 1, lc = c + 2;  // LDS row of level kk
                 const R s_k = l_q[lr][lc], s_kp1 = l_q[lr + 1][lc];
                 const R dzk = fdz0[u], dzk1 = fdz1[u];
-                fz = face_z<P>(rhow_face, &l_q[lr][lc], kk, ke, dzk, dzk1, rsz[u], q_lo, q_hi, dtL, EPS) *
+                fz = face_z<is_mixed<S>()>(rhow_face, &l_q[lr][lc], kk, ke, dzk, dzk1, rsz[u], q_lo, q_hi, dtL,
+                                           EPS) *
                      rhow_face;
                 // rho * dq/dz at the face (moist.jl:1507-1508)
                 dfz = fdiv_r(f2h_r(l_rho[lr + 1][lc], l_rho[lr][lc], dzk1, dzk, rsz[u]) * (s_kp1 - s_k), fzz[u],
