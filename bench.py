@@ -82,8 +82,11 @@ FIXED_SHAPE = None  # (levels, threads) from --shape: no autotuning
 
 
 def tune(eng):
-    """Pick the acoustic kernel's tile shape for this handle (None when disabled)."""
+    """Pick the acoustic kernel's tile shape for this handle (None when disabled);
+    a shape fixed with --shape is set on the handle as it is."""
     if FIXED_SHAPE:
+        if eng is not None:
+            eng.set_acoustic_shape(FIXED_SHAPE[0], FIXED_SHAPE[1])
         return {"levels_per_chunk": FIXED_SHAPE[0], "threads": FIXED_SHAPE[1], "us_per_launch": None}
     if not AUTOTUNE:
         return None
@@ -378,6 +381,10 @@ def main():
                     help="fix the acoustic tile shape of every form (no autotuning), e.g. for profiling")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (counter calibration under rocprofv3)")
+    ap.add_argument("--adjoint-profile", type=int, metavar="STEPS", default=0,
+                    help="only run the 512x128 adjoint workload over STEPS steps (checkpoints every 50 steps, "
+                         "no FD check; both forms, or the mixed one with --no-fp64) and print its times; "
+                         "for a kernel trace of the reverse sweep")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -387,6 +394,18 @@ def main():
     global AUTOTUNE, FIXED_SHAPE
     AUTOTUNE = not args.no_autotune
     FIXED_SHAPE = tuple(args.shape) if args.shape else None
+    if args.adjoint_profile > 0:
+        n = args.adjoint_profile
+        p, s = ic.adjoint_bubble(np.float64, steps=n)
+        tt = ic.theta_target(p)
+        res = {}
+        forms = (("mixed", core.MIXED),) if args.no_fp64 else (("fp64", core.FP64), ("mixed", core.MIXED))
+        for name, prec in forms:
+            eng, out, _, _ = _adjoint_form(p, s, tt, prec, n, 50)
+            eng.close()
+            res[name] = {k: out[k] for k in ("forward_ms", "reverse_ms", "adjoint_forward_ratio", "acoustic_shape")}
+        print(json.dumps({"workload": "adjoint_bubble_512x128", "steps": n, "ckpt": 50, **res}), flush=True)
+        return
     if args.copy_calibration:
         copy_calibration()
     p, s = ic.coldpool(np.float64, nx=args.nx, nz=args.nz)
