@@ -305,3 +305,45 @@ def test_fp64_edge_cases(kw):
     _print("   oracle 1-ulp sensitivity", sens)
     for n, e in errs.items():
         assert e <= max(TOL_FP64, 2.0 * sens[n]), (n, e, sens[n])
+
+
+MIXED_EDGE_CASES = [
+    dict(beta_offcenter=0.6),  # off-centering terms active
+    dict(ns=1),                # one substep a stage: the first substep is also the last (the final update)
+    dict(ns=8),                # 3, 4, 8 substeps
+    dict(nz=3),                # one level pair and a padded top level
+    dict(nx=3),                # periodic wrap on 3 columns
+    dict(nx=70, nz=17),        # ragged tiles, odd nz (padded level pair)
+]
+
+
+@pytest.mark.parametrize("kw", MIXED_EDGE_CASES)
+def test_mixed_edge_cases(kw):
+    """The mixed form's kernels (level-pair work fields, the last substep's
+    final update) on the FP64 edge cases, against the FP64 oracle after 6
+    steps, and bit-identical between direct launches and graph replay.  The
+    bar is FP32 rounding: rel-L2 <= 1e-4 per field, or 30 times what rounding
+    the initial state once to FP32 changes in the oracle's own result (the
+    mixed form rounds at every step; on a 3-level column the oracle's rho_w
+    moves 2e-11 under a 1-ulp change of rho_theta, so FP32 rounding there is
+    felt at the 1e-3 level)."""
+    kw = dict(kw)
+    nx, nz = kw.pop("nx", 48), kw.pop("nz", 24)
+    p, s = ic.bubble(np.float64, nx=nx, nz=nz, d=250.0, **kw)
+    s.rho_u[...] = 4.0 * s.rho
+    ref = s.copy()
+    assert O.rk3_step(ref, p, 6) == 0
+    r32 = s.copy()
+    for n in PROG:
+        getattr(r32, n)[...] = getattr(s, n).astype(np.float32).astype(np.float64)
+    assert O.rk3_step(r32, p, 6) == 0
+    sens = field_errors(p, r32, ref)
+    out, _ = _run_gpu(p, s, 6, core.MIXED)
+    out_g, _ = _run_gpu(p, s, 6, core.MIXED, graph=True)
+    errs = field_errors(p, out, ref)
+    _print(f"mixed edge {kw} {nx}x{nz}", errs)
+    _print("   oracle change from FP32-rounded initial state", sens)
+    for n in PROG:
+        np.testing.assert_array_equal(getattr(out, n), getattr(out_g, n), err_msg=n)
+    for n, e in errs.items():
+        assert e <= max(1e-4, 30.0 * sens[n]), (n, e, sens[n])
