@@ -1109,6 +1109,28 @@ ORC_EXPORT int orc_compute_step_adj_f64(const moist_params *mp, const double *co
     return 0;
 }
 
+// The slow tendencies of compute_step! (moist.jl:1071-1332) at (phys, s):
+// out[0..3] = R_rho, R_rho_u, R_rho_w, R_rho_theta, out[4..5] = u and w at
+// the faces.  One compute_step! with ns = 1 runs on copies of s and its
+// record is returned, so the tendencies are exactly the step's own.
+ORC_EXPORT int orc_slow_tendencies_f64(const moist_params *mp, const double *const *phys, const double *const *s,
+                                       double *const *out) {
+    orc::G<double> g(mp);
+    const size_t N = g.n2();
+    orc::StBuf a, b;
+    for (int f = 0; f < 7; ++f) {
+        a.f[f].assign(s[f], s[f] + N);
+        b.f[f].assign(s[f], s[f] + N);
+    }
+    orc::St<double> av = a.view(), bv = b.view();
+    orc::Rec<double> rec;
+    int rc = orc::compute_step(g, phys[0], phys[1], phys[2], phys[3], phys[4], phys[5], av, bv, g.dt, 1,
+                               orc::ORDER_SNAPSHOT, &rec);
+    const std::vector<double> *src[6] = {&rec.R_rho, &rec.R_rho_u, &rec.R_rho_w, &rec.R_rho_theta, &rec.uf, &rec.wf};
+    for (int f = 0; f < 6; ++f) std::copy(src[f]->begin(), src[f]->end(), out[f]);
+    return rc;
+}
+
 ORC_EXPORT void orc_physics_fwd_f64(const moist_params *mp, const double *const *s, double *const *ph,
                                     double *dprec) {
     orc::G<double> g(mp);

@@ -66,6 +66,7 @@ def lib():
             ("rk3_step_adj", ctypes.c_int, [V, V, V, ctypes.c_double, V]),
             ("adjoint_run", ctypes.c_int, [V, V, ctypes.c_int, ctypes.c_int, V, V, V, V]),
             ("output_history", None, [V, V, V, V]),
+            ("slow_tendencies", ctypes.c_int, [V, V, V, V]),
         ):
             f = getattr(_lib, f"orc_{name}_f64")
             f.restype = res
@@ -230,6 +231,20 @@ def compute_step_adj(phys, s, s0, p, dt, ns, lam_out):
     if rc:
         raise RuntimeError(f"compute_step_adj failed ({rc})")
     return ls, ls0, lph
+
+
+def slow_tendencies(phys, s, p):
+    """The slow tendencies of compute_step! at (phys, s) (Float64):
+    dict of R_rho, R_rho_u, R_rho_w, R_rho_theta and the face velocities
+    u_face, w_face, as the step itself computes them."""
+    shape = s.rho.shape
+    out = [np.zeros(shape, order="F") for _ in range(6)]
+    cp = p.to_c()
+    rc = lib().orc_slow_tendencies_f64(ctypes.byref(cp), _f64(phys), _f64([getattr(s, n) for n in _PROG]),
+                                       _f64(out))
+    if rc:
+        raise RuntimeError(f"slow_tendencies: NaN code {rc}")
+    return dict(zip(("R_rho", "R_rho_u", "R_rho_w", "R_rho_theta", "u_face", "w_face"), out))
 
 
 def physics_fwd(s, p):

@@ -19,9 +19,10 @@ Pins by oracle function (reference lines in oracle/moist_oracle.cpp):
   compute_step!             runtests.jl:242-347 (mass, monotonicity, CwC); hydrostatic rest stays at
                             rest; translation invariance; flux-form conservation of mass and water
   rk3_step!                 runtests.jl:349-375 (finite, mass); stage structure vs three compute_step!
-  surface_fluxes!,          parity unpinned beyond invariants: no reference test or closed form covers
-  gray_radiation!           them; checked here for finiteness, sign and the telescoping of the LW flux
-                            divergence (sum of heating * cp * Pi * dz = net boundary flux)
+  surface_fluxes!,          signs and finiteness here; their closed forms (zero crossings, scaling,
+  gray_radiation!           one- and two-layer gray atmospheres, Newtonian relaxation) and those of
+                            viscosity, the sponge, scalar diffusion, rain evaporation, the history's
+                            KE and theta_e and init_state! are in test_oracle_pins.py
 """
 import math
 
@@ -625,7 +626,7 @@ def test_nan_check_order():
 
 
 # --------------------------------------------------- surface flux, radiation
-def test_gray_radiation_flux_divergence_telescopes():
+def test_gray_radiation_finite_and_cools_the_lower_troposphere():
     p = make_params(np.float64, Nx=3, Nz=30, H=20000.0, dz0=200.0, z_fact=1.08)
     s = ic.init_state(allocate_state(p), p, seed=2)
     d = np.zeros((p.ka, p.ia), order="F")
