@@ -4,8 +4,11 @@
 Prints one JSON line.  `value` is the mixed form (FP32 prognostic deviations,
 FP64 tridiagonal/base state/accumulators); the FP64 form of the same run is
 timed beside it (`fp64_form`), and the mixed form's error against the FP64
-form after the timed steps is stated (`mixed_vs_fp64`; the FP64 form itself
-matches the CPU oracle to <= 1e-11 per field in the parity tests).
+form after 600 s of model time (1200 large steps, whatever K is) is stated
+(`mixed_vs_fp64`; the FP64 form itself matches the CPU oracle to <= 1e-11 per
+field in the parity tests).  Every timed step stores the State diagnostics
+rk3_step! writes (moist.jl:1593-1630).  `dropin` times the host-State call
+core.rk3_step(s, p), one step per call as the reference's driver makes it.
 
 Timing: W warm-up steps, then exactly K large steps between two HIP events on
 the handle's stream with the state resident in HBM (the C ABI synchronizes the
@@ -101,7 +104,6 @@ def run_form(p, s, precision, steps, warmup, graph=True):
         shape = tune(None)
     if warmup:
         eng = core.Engine(p, precision)
-        eng.store_phys_diag(False)
         eng.upload(s)
         shape = shape or tune(eng)
         if shape:
@@ -110,7 +112,6 @@ def run_form(p, s, precision, steps, warmup, graph=True):
         eng.time_steps(warmup)
         eng.close()
     eng = core.Engine(p, precision)
-    eng.store_phys_diag(False)
     eng.upload(s)
     if shape:
         eng.set_acoustic_shape(shape["levels_per_chunk"], shape["threads"])
@@ -124,7 +125,6 @@ def run_form(p, s, precision, steps, warmup, graph=True):
 
 def kernel_pass(p, s, precision, steps, shape=None):
     eng = core.Engine(p, precision)
-    eng.store_phys_diag(False)
     eng.upload(s)
     if shape:
         eng.set_acoustic_shape(shape["levels_per_chunk"], shape["threads"])
@@ -146,7 +146,6 @@ def ensemble_run(steps, members=256, seed=2):
            "columns": sum(p.Nx for p in params), "dt": float(params[0].dt), "ns": params[0].ns, "steps": steps}
     for name, prec in (("mixed", core.MIXED), ("fp64", core.FP64)):
         eng = core.Engine(params, prec)
-        eng.store_phys_diag(False)
         for m, (_, s) in enumerate(mem):
             eng.upload(s, member=m)
         shape = tune(eng)
@@ -187,7 +186,6 @@ def _loss_np(p, a, tt):
 def _adjoint_form(p, s, tt, precision, steps, ckpt):
     """Forward timing and one checkpointed adjoint run in one form."""
     eng = core.Engine(p, precision)
-    eng.store_phys_diag(False)
     eng.use_graph(True)
     eng.upload(s)
     shape = tune(eng)
@@ -285,18 +283,11 @@ def batched_adjoint_bench(members=64, steps=300, ckpt=50, seed=4, forms=("mixed"
     reverse sweep of every member's loss in one batched launch sequence, in
     the mixed form (FP32 bulk, FP64 tridiagonal and step adjoints) and in the
     FP64 form."""
-    mem = []
-    for m in range(members):
-        u = [float(ic.rng_uniform(seed, m, 2000 + j, 0, 0)) for j in range(5)]
-        p, s = ic.bubble(np.float64, nx=1024, nz=128, d=100.0, dt=1.0, ns=6, steps=steps, amp=1.0 + 2.0 * u[0],
-                         xc=(0.3 + 0.4 * u[1]) * 1024 * 100.0, zc=1500.0 + 1000.0 * u[2], seed=seed, member=m,
-                         K_auto=5e-4 + 1.5e-3 * u[3])
-        mem.append((p, s, ic.theta_target(p, seed=seed, member=m)))
+    mem = ic.batched_adjoint(np.float64, members=members, seed=seed, steps=steps)
     cells = sum(p.Nx * p.Nz for p, _, _ in mem)
     res = {}
     for form in forms:
         eng = core.Engine([p for p, _, _ in mem], core.MIXED if form == "mixed" else core.FP64)
-        eng.store_phys_diag(False)
         eng.use_graph(True)
         for m, (p, s, _) in enumerate(mem):
             eng.upload(s, member=m)
@@ -359,6 +350,91 @@ def copy_calibration(nbytes=256 << 20):
     hip.hipFree(b)
 
 
+def _run_steps(p, s, precision, steps, shape=None):
+    """State of the device form after `steps` large steps from s (graph replay)."""
+    eng = core.Engine(p, precision)
+    eng.upload(s)
+    if shape:
+        eng.set_acoustic_shape(shape["levels_per_chunk"], shape["threads"])
+    eng.use_graph(True)
+    eng.time_steps(steps)
+    out = s.copy()
+    eng.download(out, phys=False)
+    diag = eng.diagnostics()
+    eng.close()
+    return out, diag
+
+
+def accuracy_leg(p, s, steps, shape_m=None, shape_d=None):
+    """Stated error of the mixed form: per-field rel-L2 against the FP64 form
+    after `steps` large steps from t = 0 (1200 x 0.5 s = 600 s), the drift of
+    total water + rain between them, and the FP32 sensitivity floor (the FP64
+    form from the initial state rounded once to FP32)."""
+    out_m, diag_m = _run_steps(p, s, core.MIXED, steps, shape_m)
+    out_d, diag_d = _run_steps(p, s, core.FP64, steps, shape_d)
+    is0 = p.is_ - 1
+    sl = np.s_[:, is0:is0 + p.Nx]
+    err = {n: rel_l2(getattr(out_m, n)[sl], getattr(out_d, n)[sl]) for n in PROGNOSTIC}
+    w0 = diag_d["total_water"] + diag_d["rain"]
+    err["total_water_plus_rain_rel_diff"] = (diag_m["total_water"] + diag_m["rain"] - w0) / w0
+    err["sim_seconds"] = steps * float(p.dt)
+    s32 = s.copy()
+    for n in PROGNOSTIC:
+        getattr(s32, n)[...] = getattr(s, n).astype(np.float32).astype(np.float64)
+    out_f, _ = _run_steps(p, s32, core.FP64, steps, shape_d)
+    floor = {n: rel_l2(getattr(out_f, n)[sl], getattr(out_d, n)[sl]) for n in PROGNOSTIC}
+    floor["what"] = "FP64 form from the FP32-rounded initial state vs FP64 form, same time"
+    return err, floor
+
+
+def fp32_step_floor(p, s, steps):
+    """The FP64 form with the prognostic state rounded to FP32 after every
+    large step (host round trip per step) against the plain FP64 form: the
+    error any scheme that stores its prognostics in FP32 carries, with FP64
+    arithmetic everywhere else."""
+    out_d, _ = _run_steps(p, s, core.FP64, steps)
+    eng = core.Engine(p, core.FP64)
+    a = s.copy()
+    for n in PROGNOSTIC:
+        getattr(a, n)[...] = getattr(s, n).astype(np.float32).astype(np.float64)
+    t0 = time.perf_counter()
+    for j in range(steps):
+        eng.upload(a)
+        eng.step(1)
+        eng.download(a, phys=False)
+        for n in PROGNOSTIC:
+            getattr(a, n)[...] = getattr(a, n).astype(np.float32).astype(np.float64)
+        if (j + 1) % 200 == 0:
+            print(f"fp32 step floor: {j + 1}/{steps} steps, {time.perf_counter() - t0:.0f} s", file=sys.stderr,
+                  flush=True)
+    eng.close()
+    is0 = p.is_ - 1
+    sl = np.s_[:, is0:is0 + p.Nx]
+    res = {n: rel_l2(getattr(a, n)[sl], getattr(out_d, n)[sl]) for n in PROGNOSTIC}
+    res["what"] = "FP64 form with the state rounded to FP32 after every large step vs FP64 form"
+    res["sim_seconds"] = steps * float(p.dt)
+    return res
+
+
+def dropin_leg(p, s, calls=5):
+    """The literal drop-in path: core.rk3_step(s, p) on a host State, one large
+    step per call as the reference's driver calls rk3_step! (moist.jl:1943-1944):
+    upload, step, download of the State and its diagnostics over PCIe, into a
+    handle cached after the first call."""
+    core.release_cached()
+    a = s.copy()
+    t0 = time.perf_counter()
+    core.rk3_step(a, p, precision=core.MIXED)
+    first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        core.rk3_step(a, p, precision=core.MIXED)
+    per = (time.perf_counter() - t0) / calls
+    core.release_cached()
+    return {"form": "mixed", "first_call_ms": first * 1e3, "ms_per_step": per * 1e3, "calls": calls,
+            "what": "host State in, host State out each call (PCIe-inclusive), handle cached"}
+
+
 def rel_l2(a, b):
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a - b))
@@ -379,6 +455,12 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default acoustic tile shape")
     ap.add_argument("--shape", type=int, nargs=2, metavar=("LEVELS", "THREADS"),
                     help="fix the acoustic tile shape of every form (no autotuning), e.g. for profiling")
+    ap.add_argument("--accuracy-steps", type=int, default=1200,
+                    help="large steps of the mixed-vs-FP64 accuracy leg (1200 x 0.5 s = 600 s)")
+    ap.add_argument("--fp32-step-floor", action="store_true",
+                    help="also run the FP64 form with the state rounded to FP32 after every step "
+                         "(host round trip per step, about a minute)")
+    ap.add_argument("--no-dropin", action="store_true", help="skip timing the host-State drop-in call")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (counter calibration under rocprofv3)")
     ap.add_argument("--adjoint-profile", type=int, metavar="STEPS", default=0,
@@ -415,9 +497,6 @@ def main():
     eng_m, ms_m, wall_m = run_form(p, s, core.MIXED, K, W)
     mixed = {"cell_steps_per_s": cells * K / (ms_m * 1e-3), "ms_per_step": ms_m / K, "wall_s": wall_m,
              "acoustic_shape": eng_m.shape}
-    out_m = s.copy()
-    eng_m.download(out_m, phys=False)
-    diag_m = eng_m.diagnostics()
     eng_m.close()
 
     fp64 = None
@@ -427,32 +506,13 @@ def main():
         eng_d, ms_d, wall_d = run_form(p, s, core.FP64, K, W)
         fp64 = {"value": cells * K / (ms_d * 1e-3), "ms_per_step": ms_d / K, "wall_s": wall_d,
                 "acoustic_shape": eng_d.shape}
-        out_d = s.copy()
-        eng_d.download(out_d, phys=False)
-        diag_d = eng_d.diagnostics()
         eng_d.close()
-        is0 = p.is_ - 1
-        sl = np.s_[:, is0:is0 + p.Nx]
-        err = {n: rel_l2(getattr(out_m, n)[sl], getattr(out_d, n)[sl]) for n in PROGNOSTIC}
-        w0 = diag_d["total_water"] + diag_d["rain"]
-        err["total_water_plus_rain_rel_diff"] = (diag_m["total_water"] + diag_m["rain"] - w0) / w0
-        err["sim_seconds"] = K * float(p.dt)
         mixed["speedup_vs_fp64"] = mixed["cell_steps_per_s"] / fp64["value"]
-        # sensitivity floor: the FP64 form started from the initial state rounded
-        # to FP32 differs from the FP64 run by this much at the same time
-        s32 = s.copy()
-        for n in PROGNOSTIC:
-            getattr(s32, n)[...] = getattr(s, n).astype(np.float32).astype(np.float64)
-        eng_f = core.Engine(p, core.FP64)
-        eng_f.store_phys_diag(False)
-        eng_f.upload(s32)
-        eng_f.use_graph(True)
-        eng_f.time_steps(K)
-        out_f = s.copy()
-        eng_f.download(out_f, phys=False)
-        eng_f.close()
-        floor = {n: rel_l2(getattr(out_f, n)[sl], getattr(out_d, n)[sl]) for n in PROGNOSTIC}
-        floor["what"] = "FP64 form from the FP32-rounded initial state vs FP64 form, same time"
+        err, floor = accuracy_leg(p, s, args.accuracy_steps, mixed["acoustic_shape"], fp64["acoustic_shape"])
+    if args.fp32_step_floor:
+        floor = dict(floor or {})
+        floor["per_step"] = fp32_step_floor(p, s, args.accuracy_steps)
+    dropin = None if args.no_dropin else dropin_leg(p, s)
 
     # dominant kernel: acoustic column kernel, timed live with HIP events
     kms, nlaunch = kernel_pass(p, s, core.MIXED, min(K, 40), mixed["acoustic_shape"])
@@ -494,11 +554,12 @@ def main():
         "dtype": "f32 (f64 tridiagonal, base state, accumulators)",
         "data": "synthetic",
         "config": {"workload": f"coldpool_{p.Nx}x{p.Nz}", "nx": p.Nx, "nz": p.Nz, "dx": float(p.dx),
-                   "dt": float(p.dt), "ns": p.ns, "form": "mixed", "hipgraph": True},
+                   "dt": float(p.dt), "ns": p.ns, "form": "mixed", "hipgraph": True, "phys_diag": True},
         "fp64_form": fp64,
         "mixed_vs_fp64": err,
         "fp64_sensitivity_floor": floor,
         "mixed_wall_s": mixed["wall_s"],
+        "dropin": dropin,
         "acoustic_shape": mixed["acoustic_shape"],
         "roofline": roof,
         "cpu_baseline": cpu,

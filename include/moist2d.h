@@ -77,7 +77,9 @@ typedef struct moist_handle moist_handle;
 /* Create a handle for an ensemble of n_members independent domains (n=1 for
  * a single domain).  Each member has its own params (nx, nz and grid may
  * differ; dt, ns and the switches may differ too).  Replaces allocate_state
- * (moist.jl:319) for the device.                                            */
+ * (moist.jl:319) for the device.  On failure *out may still hold a handle
+ * that carries the error text (moist_last_error); pass it to moist_destroy,
+ * which frees whatever was allocated before the failure.                    */
 int moist_create(const moist_params *const *members, int n_members,
                  int precision, moist_handle **out);
 

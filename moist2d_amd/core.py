@@ -240,31 +240,59 @@ def _precision_for(s: State, precision):
     return FP64 if s.rho.dtype == np.float64 else MIXED
 
 
+# Handles of the host-State entry points, kept between calls: the reference
+# calls rk3_step! once per step (moist.jl:1943-1944), so each call re-uploads
+# the State into a cached handle instead of creating one.  Keyed on the
+# Params values, the z grid and the precision; a few are kept, least
+# recently used first out.
+_CACHE: dict = {}
+_CACHE_MAX = 4
+
+
+def _params_key(p, precision):
+    vals = tuple(sorted((k, repr(v)) for k, v in p.values.items()))
+    grid = tuple(np.asarray(a, dtype=np.float64).tobytes() for a in (p.z, p.zf, p.dz))
+    return (np.dtype(p.T).str, vals, grid, precision)
+
+
+def _cached_engine(p, precision):
+    key = _params_key(p, precision)
+    eng = _CACHE.pop(key, None)
+    if eng is None:
+        while len(_CACHE) >= _CACHE_MAX:
+            _CACHE.pop(next(iter(_CACHE))).close()
+        eng = Engine(p, precision)
+    _CACHE[key] = eng  # most recently used last
+    return eng
+
+
+def release_cached() -> None:
+    """Free the device memory of the handles kept by rk3_step / compute_step."""
+    while _CACHE:
+        _CACHE.popitem()[1].close()
+
+
 def rk3_step(s: State, p, precision=None, nsteps=1):
     """rk3_step!(s, p) (moist.jl:1589-1683) on the GPU, in place on host State `s`.
 
     Float64 states run the FP64 form, Float32 states the mixed form, unless
     `precision` says otherwise.  `nsteps` > 1 advances several large steps
-    with one upload and one download.
+    with one upload and one download.  The device handle is created on the
+    first call for these Params and reused by later calls.
     """
-    eng = Engine(p, _precision_for(s, precision))
-    try:
-        eng.upload(s)
-        rc = eng.step(nsteps)
-        eng.download(s)
-    finally:
-        eng.close()
+    eng = _cached_engine(p, _precision_for(s, precision))
+    eng.upload(s)
+    rc = eng.step(nsteps)
+    eng.download(s)
     return rc
 
 
 def compute_step(d_rho_phys, d_rho_u_phys, d_rho_theta_phys, d_rho_qv_phys, d_rho_qc_phys, d_rho_qr_phys, s: State,
                  s0: State, p, dt, ns, precision=None):
-    """compute_step!(...) (moist.jl:1039-1580): one RK stage, in place on `s`."""
-    eng = Engine(p, _precision_for(s, precision))
-    try:
-        eng.upload(s)
-        eng.compute_step((d_rho_phys, d_rho_u_phys, d_rho_theta_phys, d_rho_qv_phys, d_rho_qc_phys, d_rho_qr_phys),
-                         s0, dt, ns)
-        eng.download(s, phys=False)
-    finally:
-        eng.close()
+    """compute_step!(...) (moist.jl:1039-1580): one RK stage, in place on `s`
+    (the device handle is cached as for rk3_step)."""
+    eng = _cached_engine(p, _precision_for(s, precision))
+    eng.upload(s)
+    eng.compute_step((d_rho_phys, d_rho_u_phys, d_rho_theta_phys, d_rho_qv_phys, d_rho_qc_phys, d_rho_qr_phys),
+                     s0, dt, ns)
+    eng.download(s, phys=False)

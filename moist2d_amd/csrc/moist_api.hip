@@ -423,6 +423,7 @@ int build_graph(moist_handle *h) {
 }
 
 int run_steps(moist_handle *h, int nsteps) {
+    h->ev_used = 0;  // the per-launch events of this call reuse the pool
     if (h->use_graph && !h->ev_on) {
         if (build_graph(h)) return -1;
         for (int n = 0; n < nsteps; ++n) HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
@@ -434,6 +435,20 @@ int run_steps(moist_handle *h, int nsteps) {
     return 0;
 }
 
+// HIP events that are destroyed with the scope, whatever path leaves it.
+struct ScopedEvents {
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    hipError_t create(int n) {
+        for (int j = 0; j < n; ++j)
+            if (hipError_t r = hipEventCreate(&e[j])) return r;
+        return hipSuccess;
+    }
+    ~ScopedEvents() {
+        for (hipEvent_t x : e)
+            if (x) hipEventDestroy(x);
+    }
+};
+
 int decode_nan(moist_handle *h) {
     int key = INT_MAX;
     HIPCHK(h, hipMemcpyAsync(&key, h->nan_key, sizeof(int), hipMemcpyDeviceToHost, h->stream));
@@ -442,9 +457,12 @@ int decode_nan(moist_handle *h) {
     return key & 15;
 }
 
+// The NaN key orders reports by (step of this call, stage, field): the step
+// counter restarts with the key at every call.
 int reset_nan(moist_handle *h) {
-    int key = INT_MAX;
+    static const int key = INT_MAX, zero = 0;
     HIPCHK(h, hipMemcpyAsync(h->nan_key, &key, sizeof(int), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->step_ctr, &zero, sizeof(int), hipMemcpyHostToDevice, h->stream));
     return 0;
 }
 
@@ -1086,10 +1104,9 @@ template <class H> int adjoint_run_t(moist_handle *h, int nsteps, int ckpt, cons
                            (const H *)h->staging, A.tgt, (const double *)nullptr, D.halo, ka);
         HIPCHK(h, hipGetLastError());
     }
-    hipEvent_t e0, e1, e2;
-    HIPCHK(h, hipEventCreate(&e0));
-    HIPCHK(h, hipEventCreate(&e1));
-    HIPCHK(h, hipEventCreate(&e2));
+    ScopedEvents ev;
+    HIPCHK(h, ev.create(3));
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1], e2 = ev.e[2];
     if (reset_nan(h)) return -1;
     HIPCHK(h, hipEventRecord(e0, h->stream));
     // forward with checkpoints
@@ -1138,9 +1155,6 @@ template <class H> int adjoint_run_t(moist_handle *h, int nsteps, int ckpt, cons
     hipEventElapsedTime(&f2, e1, e2);
     A.ms_fwd = f1;
     A.ms_bwd = f2;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    hipEventDestroy(e2);
     if (J)
         for (int m = 0; m < h->nm; ++m) J[m] = A.J_h[m];
     h->halo_src = -1;
@@ -1286,6 +1300,10 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
     h->nctiles = (int)tc.size();
 
     moist_handle *hp = h.get();
+    // Device set-up.  On a failure the handle is still returned (with the
+    // error text), so that the caller can read it and moist_destroy frees
+    // the stream and whatever was allocated before the failure.
+    auto setup = [&]() -> int {
 #define ALLOC(ptr, bytes)                                          \
     do {                                                           \
         HIPCHK(hp, hipMalloc((void **)&(ptr), (bytes)));           \
@@ -1348,8 +1366,11 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
 #undef ALLOC
     if (reset_nan(hp)) return -1;
     HIPCHK(hp, hipStreamSynchronize(hp->stream));
-    *out = h.release();
     return 0;
+    };
+    const int rc = setup();
+    *out = h.release();
+    return rc;
 }
 
 int moist_upload(moist_handle *h, int member, const void *const fields[7], const void *prec, const void *theta_ref,
@@ -1479,12 +1500,11 @@ int moist_set_option(moist_handle *h, int option, int value) {
 
 int moist_time_steps(moist_handle *h, int nsteps, double *ms) {
     if (!h || !ms || nsteps < 1) return -2;
-    hipEvent_t a, b;
-    HIPCHK(h, hipEventCreate(&a));
-    HIPCHK(h, hipEventCreate(&b));
+    ScopedEvents ev;
+    HIPCHK(h, ev.create(2));
+    hipEvent_t a = ev.e[0], b = ev.e[1];
     if (reset_nan(h)) return -1;
     if (h->use_graph && build_graph(h)) return -1;
-    h->ev_used = 0;
     HIPCHK(h, hipEventRecord(a, h->stream));
     if (run_steps(h, nsteps)) return -1;
     HIPCHK(h, hipEventRecord(b, h->stream));
@@ -1492,8 +1512,6 @@ int moist_time_steps(moist_handle *h, int nsteps, double *ms) {
     float f = 0.0f;
     HIPCHK(h, hipEventElapsedTime(&f, a, b));
     *ms = f;
-    hipEventDestroy(a);
-    hipEventDestroy(b);
     if (h->ev_on) {
         double tot = 0.0;
         int64_t n = 0;
@@ -1531,9 +1549,9 @@ int moist_autotune(moist_handle *h, int reps, int *levels, int *threads, double 
     const int L0 = h->ac_L, NT0 = h->ac_NT, W0 = h->ac_W;
     int bestL = L0, bestNT = NT0, bestW = W0;
     double best = 1e300, t0 = 1e300;
-    hipEvent_t a, b;
-    HIPCHK(h, hipEventCreate(&a));
-    HIPCHK(h, hipEventCreate(&b));
+    ScopedEvents ev;
+    HIPCHK(h, ev.create(2));
+    hipEvent_t a = ev.e[0], b = ev.e[1];
     const ZVec zv = zvec(h);
     for (const auto &sh : shapes) {
         const int L = sh[0], NT = sh[1], W = sh[2];
@@ -1573,8 +1591,6 @@ int moist_autotune(moist_handle *h, int reps, int *levels, int *threads, double 
             }
         }
     }
-    hipEventDestroy(a);
-    hipEventDestroy(b);
     // keep the current shape unless another is faster by more than 3 %
     if (!(best < 0.97 * t0)) {
         bestL = L0;

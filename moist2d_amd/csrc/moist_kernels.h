@@ -486,8 +486,11 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
     if (f) report_nan(w.nan_key, ((*w.step_ctr * 3) << 4) | f);
 }
 
+// Saturates below the largest step whose NaN key (step * 3 + stage) << 4 | f
+// fits an int: beyond 44.7 M steps in one call, later steps share its rank.
+constexpr int STEP_CTR_MAX = (INT_MAX >> 4) / 3 - 1;
 __global__ void k_step_done(int *step_ctr) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *step_ctr += 1;
+    if (threadIdx.x == 0 && blockIdx.x == 0 && *step_ctr < STEP_CTR_MAX) *step_ctr += 1;
 }
 
 // =============================================================================

@@ -228,6 +228,21 @@ def adjoint_bubble(T=np.float64, nx=512, nz=128, d=78.125, dt=0.5, ns=6, steps=6
     return bubble(T, nx=nx, nz=nz, d=d, dt=dt, ns=ns, steps=steps, seed=seed, **kw)
 
 
+def batched_adjoint(T=np.float64, members=64, seed=4, steps=300):
+    """Members of the 64 x 1024x128 batched adjoint workload: bubbles at
+    dx = dz = 100 m, dt = 1 s, ns = 6, with seeded amplitude U[1, 3] K,
+    centre x U[0.3, 0.7] Lx, z U[1.5, 2.5] km and K_auto ~ U[5e-4, 2e-3];
+    returns [(params, state, theta_target)]."""
+    out = []
+    for m in range(members):
+        u = [float(rng_uniform(seed, m, 2000 + j, 0, 0)) for j in range(5)]
+        p, s = bubble(T, nx=1024, nz=128, d=100.0, dt=1.0, ns=6, steps=steps, amp=1.0 + 2.0 * u[0],
+                      xc=(0.3 + 0.4 * u[1]) * 1024 * 100.0, zc=1500.0 + 1000.0 * u[2], seed=seed, member=m,
+                      K_auto=5e-4 + 1.5e-3 * u[3])
+        out.append((p, s, theta_target(p, seed=seed, member=m)))
+    return out
+
+
 def theta_target(p, seed=3, sigma=0.5, member=0):
     """Target theta of the adjoint workload's loss: the base profile plus a
     seeded Gaussian field of standard deviation `sigma` (K), in the (ka, ia)

@@ -91,6 +91,44 @@ def test_fp64_compute_step_entry():
         np.testing.assert_array_equal(getattr(out, n)[:, :p.halo], getattr(ref, n)[:, :p.halo])
 
 
+def test_mixed_compute_step_entry():
+    """The mixed form of moist_compute_step (core.compute_step's default for
+    Float32 states): explicit dt, s0 given apart from s, non-zero tendencies,
+    ns = 3, against the FP64 oracle.  Bar: FP32 rounding, i.e. rel-L2 <=
+    1e-5 per field, or 30 times what rounding s, s0 and the tendencies to
+    FP32 once changes in the oracle's own result."""
+    p, s = ic.bubble(np.float64, nx=64, nz=32, d=156.25)
+    s.rho_u[...] = 5.0 * s.rho
+    rng = np.random.default_rng(5)
+    s0 = s.copy()
+    O.rk3_step(s, p, 2)
+    phys = [np.asfortranarray(rng.normal(0, sc, size=s.rho.shape)) for sc in (1e-6, 1e-4, 1e-4, 1e-7, 1e-7, 1e-7)]
+    ref = s.copy()
+    assert O.compute_step(phys, ref, s0, p, 0.5, 3) == 0
+
+    def r32(st):
+        a = st.copy()
+        for n in PROG:
+            getattr(a, n)[...] = getattr(st, n).astype(np.float32).astype(np.float64)
+        return a
+
+    pert = r32(s)
+    phys32 = [np.asfortranarray(x.astype(np.float32).astype(np.float64)) for x in phys]
+    assert O.compute_step(phys32, pert, r32(s0), p, 0.5, 3) == 0
+    sens = field_errors(p, pert, ref)
+    eng = core.Engine(p, core.MIXED)
+    eng.upload(s)
+    eng.compute_step(phys, s0, 0.5, 3)
+    out = s.copy()
+    eng.download(out, phys=False)
+    eng.close()
+    errs = field_errors(p, out, ref)
+    _print("mixed compute_step", errs)
+    _print("   oracle change from FP32-rounded inputs", sens)
+    for n, e in errs.items():
+        assert e <= max(1e-5, 30.0 * sens[n]), (n, e, sens[n])
+
+
 def test_fp64_default_physics_surface_and_radiation():
     """Default Params switches (surface fluxes and gray radiation on) with the
     reference's init_state! profile."""
@@ -198,6 +236,36 @@ def test_nan_is_reported_like_the_reference():
     s.rho[5, p.halo + 3] = np.nan
     with pytest.raises(RuntimeError, match="NaN detected in rho$"):
         core.rk3_step(s, p)
+
+
+def test_dropin_step_reuses_its_handle():
+    """The reference's driver calls rk3_step!(s, p) once per step
+    (moist.jl:1943-1944): three such calls on host States reuse one cached
+    device handle and equal three oracle steps; a call that hits a NaN
+    leaves the handle usable for the next clean State."""
+    core.release_cached()
+    p, s = ic.bubble(np.float64, nx=64, nz=32, d=156.25)
+    ref = s.copy()
+    assert O.rk3_step(ref, p, 3) == 0
+    out = s.copy()
+    engines = []
+    for _ in range(3):
+        core.rk3_step(out, p)
+        engines.append(next(iter(core._CACHE.values())))
+    assert len(core._CACHE) == 1 and engines[0] is engines[1] is engines[2]
+    errs = field_errors(p, out, ref)
+    _print("drop-in rk3_step x3", errs)
+    for n, e in errs.items():
+        assert e <= TOL_FP64, (n, e)
+    bad = s.copy()
+    bad.rho_qr[4, p.halo + 5] = np.nan
+    with pytest.raises(RuntimeError, match="NaN detected in "):
+        core.rk3_step(bad, p)
+    again = s.copy()
+    core.rk3_step(again, p, nsteps=3)
+    for n in PROG:
+        np.testing.assert_array_equal(getattr(again, n), getattr(out, n), err_msg=n)
+    core.release_cached()
 
 
 @pytest.mark.parametrize("precision", [core.FP64, core.MIXED])
