@@ -124,6 +124,18 @@ int moist_compute_step(moist_handle *h, int member, const void *const phys[6],
 /* FP64 reductions of the current state of one member into out[MOIST_DIAG_COUNT]. */
 int moist_diagnostics(moist_handle *h, int member, double *out);
 
+/* moist_diagnostics without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under stream or graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The block
+ * partials go to a static device array of MOIST_NOALLOC_SCRATCH_DOUBLES
+ * doubles (8 MiB, one per GPU, held until the process ends); a handle that
+ * needs more (3 per 64x4 cell tile plus 4 per member: about 38 M cells)
+ * is rejected with -2.  Calls on one GPU share that array: calls that overlap
+ * in time on one GPU race, and must be serialized by the caller.  Same
+ * results as moist_diagnostics, bit for bit.                               */
+#define MOIST_NOALLOC_SCRATCH_DOUBLES (1 << 20)
+int moist_diagnostics_noalloc(moist_handle *h, int member, double *out);
+
 /* Capture one large step of all members as a hipGraph and replay it for
  * later moist_rk3_step calls (on = 1) or go back to direct launches (0).   */
 int moist_use_graph(moist_handle *h, int on);

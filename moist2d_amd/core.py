@@ -49,6 +49,7 @@ def lib():
             "moist_rk3_step": (ip, [vp, ip]),
             "moist_compute_step": (ip, [vp, ip, vp, vp, ctypes.c_double, ip, ip]),
             "moist_diagnostics": (ip, [vp, ip, ctypes.POINTER(ctypes.c_double)]),
+            "moist_diagnostics_noalloc": (ip, [vp, ip, ctypes.POINTER(ctypes.c_double)]),
             "moist_use_graph": (ip, [vp, ip]),
             "moist_set_option": (ip, [vp, ip, ip]),
             "moist_time_steps": (ip, [vp, ip, ctypes.POINTER(ctypes.c_double)]),
@@ -149,6 +150,13 @@ class Engine:
     def diagnostics(self, member=0) -> dict:
         out = (ctypes.c_double * 5)()
         self._check(lib().moist_diagnostics(self._h, member, out), "moist_diagnostics")
+        return dict(zip(DIAG_NAMES, list(out)))
+
+    def diagnostics_noalloc(self, member=0) -> dict:
+        """diagnostics() through moist_diagnostics_noalloc (static device scratch,
+        no allocation)."""
+        out = (ctypes.c_double * 5)()
+        self._check(lib().moist_diagnostics_noalloc(self._h, member, out), "moist_diagnostics_noalloc")
         return dict(zip(DIAG_NAMES, list(out)))
 
     def use_graph(self, on=True):

@@ -22,6 +22,7 @@
 #include <cstring>
 #include <initializer_list>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -64,10 +65,24 @@ template <class T> std::vector<T> xcd_order(const std::vector<T> &v, T pad) {
 
 }  // namespace
 
+// ---- static scratch of moist_diagnostics_noalloc ----------------------------
+// The block partials and per-member sums of the budget reduction live in this
+// file-scope device array instead of memory the library allocates.  Each GPU
+// holds its own copy until the process ends; host code uses the per-device
+// address from hipGetSymbolAddress (taken once per device in moist_create),
+// never the array's name, which on the host is only a stand-in symbol.
+static __device__ __align__(256) double scratch[MOIST_NOALLOC_SCRATCH_DOUBLES];
+namespace {
+constexpr int MAX_DEVICES = 64;
+double *g_scratch[MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+}  // namespace
+
 struct AdjState;
 
 struct moist_handle {
     int precision = MOIST_FP64;
+    int device = 0;  // HIP device current at moist_create
     int nm = 0;
     std::string err;
     hipStream_t stream = nullptr;
@@ -1309,6 +1324,19 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
         HIPCHK(hp, hipMalloc((void **)&(ptr), (bytes)));           \
         hp->dev_bytes += (int64_t)(bytes);                         \
     } while (0)
+    HIPCHK(hp, hipGetDevice(&hp->device));
+    if (hp->device < 0 || hp->device >= MAX_DEVICES) {
+        hp->err = "device index beyond the scratch table";
+        return -2;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch[hp->device]) {
+            void *ptr = nullptr;
+            HIPCHK(hp, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+            g_scratch[hp->device] = (double *)ptr;
+        }
+    }
     HIPCHK(hp, hipStreamCreate(&hp->stream));
     ALLOC(hp->dom_d, sizeof(Dom) * n_members);
     ALLOC(hp->tiles2d, sizeof(int4) * t2.size());
@@ -1459,6 +1487,47 @@ int moist_diagnostics(moist_handle *h, int member, double *out) {
     HIPCHK(h, hipGetLastError());
     double v[4];
     HIPCHK(h, hipMemcpyAsync(v, h->diag_all + 4 * member, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+    int key = INT_MAX;
+    HIPCHK(h, hipMemcpyAsync(&key, h->nan_key, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    out[MOIST_DIAG_DRY_MASS] = v[0];
+    out[MOIST_DIAG_TOTAL_WATER] = v[1];
+    out[MOIST_DIAG_RAIN] = v[2];
+    out[MOIST_DIAG_MASS] = v[3];
+    out[MOIST_DIAG_NAN] = key == INT_MAX ? 0.0 : double(key & 15);
+    return 0;
+}
+
+int moist_diagnostics_noalloc(moist_handle *h, int member, double *out) {
+    if (!h || !out) return -2;
+    if (int rc = check_member(h, member)) return rc;
+    const size_t need = 3 * (size_t)h->ntiles2d + 4 * (size_t)h->nm;
+    if (need > (size_t)MOIST_NOALLOC_SCRATCH_DOUBLES) {
+        h->err = "moist_diagnostics_noalloc: the handle needs " + std::to_string(need) +
+                 " doubles of scratch, above the static maximum of " +
+                 std::to_string((size_t)MOIST_NOALLOC_SCRATCH_DOUBLES);
+        return -2;
+    }
+    double *part = g_scratch[h->device];
+    if (!part) {
+        h->err = "moist_diagnostics_noalloc: no scratch address for this device";
+        return -1;
+    }
+    double *sums = part + 3 * (size_t)h->ntiles2d;
+    // every partial and sum is written before it is read within the call,
+    // so nothing needs clearing
+    const ZVec zv = zvec(h);
+    if (h->precision == MOIST_MIXED)
+        hipLaunchKernelGGL(k_budget<float>, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d,
+                           h->tiles2d, zv, sset<float>(h, S0_), part);
+    else
+        hipLaunchKernelGGL(k_budget<double>, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d,
+                           h->tiles2d, zv, sset<double>(h, S0_), part);
+    hipLaunchKernelGGL(k_budget_reduce, dim3(h->nm), dim3(256), 0, h->stream, h->dom_d, (const int4 *)h->tiles2d,
+                       h->ntiles2d, (const double *)part, (const double *)h->prec, sums);
+    HIPCHK(h, hipGetLastError());
+    double v[4];
+    HIPCHK(h, hipMemcpyAsync(v, sums + 4 * member, sizeof(v), hipMemcpyDeviceToHost, h->stream));
     int key = INT_MAX;
     HIPCHK(h, hipMemcpyAsync(&key, h->nan_key, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
