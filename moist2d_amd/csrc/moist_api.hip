@@ -338,7 +338,7 @@ void launch_acoustic(moist_handle *h, const ZVec &zv, const SSet<S> &in, const W
 // output for the next stage.
 template <class S>
 void enqueue_stage(moist_handle *h, int in_set, int out_set, int s0_set, int div, double dt_ovr, int nsub,
-                   int stage, int u_phys_2d, bool diag_in, bool diag_out) {
+                   int stage, int u_phys_2d, bool diag_in, bool diag_out, bool bump = false) {
     const dim3 b2(TILE_X, TILE_Z);
     const ZVec zv = zvec(h);
     const Work<S> w = work<S>(h);
@@ -352,7 +352,7 @@ void enqueue_stage(moist_handle *h, int in_set, int out_set, int s0_set, int div
     if (diag_in)
         hipLaunchKernelGGL((k_diag<S, PL>), dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, w);
     hipLaunchKernelGGL((k_slow<S, PL>), dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles, zv,
-                       in, s0, w, rtp[0], u_phys_2d);
+                       in, s0, w, rtp[0], u_phys_2d, bump ? h->step_ctr : (int *)nullptr);
     // rho_theta' rotates over three buffers (current, previous, next) and
     // rho_u' over two, because every column reads its neighbours' values
     int cur = 0, old = 0, ru = 0;
@@ -410,10 +410,9 @@ template <class S> void enqueue_step(moist_handle *h) {
     constexpr bool PL = is_mixed<S>();
     hipLaunchKernelGGL(k_physics<S>, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d, h->tiles2d,
                        zv, s, w, h->store_diag, PL ? 1 : 0);
-    enqueue_stage<S>(h, S0_, SA_, S0_, 3, 0.0, h->ns_stage[0], 0, 0, !PL, true);
+    enqueue_stage<S>(h, S0_, SA_, S0_, 3, 0.0, h->ns_stage[0], 0, 0, !PL, true, true);
     enqueue_stage<S>(h, SA_, SB_, S0_, 2, 0.0, h->ns_stage[1], 1, 0, false, true);
     enqueue_stage<S>(h, SB_, S0_, S0_, 1, 0.0, h->ns_stage[2], 2, 0, false, false);
-    hipLaunchKernelGGL(k_step_done, dim3(1), dim3(64), 0, h->stream, h->step_ctr);
 }
 
 void enqueue_step_any(moist_handle *h) {
@@ -907,7 +906,7 @@ template <class S> void enqueue_stage_rec(moist_handle *h, int st, SSet<S> in, S
     w.rwp = rec_rwp<S>(h, st, 0);  // rho_w' of every substep goes straight to its record
     hipLaunchKernelGGL((k_diag<S, false>), dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, w);
     hipLaunchKernelGGL((k_slow<S, false>), dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles, zv,
-                       in, s0, w, rec_rtp<S>(h, st, 0), 0);
+                       in, s0, w, rec_rtp<S>(h, st, 0), 0, (int *)nullptr);
     for (int n = 0; n < nsub; ++n) {
         AcBuf<S> ab{};
         ab.rup_in = rec_rup<S>(h, st, n);

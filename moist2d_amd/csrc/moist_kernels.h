@@ -42,6 +42,14 @@ template <class S> struct Work {
     const int *step_ctr;
 };
 
+// NaN reports rank by (step of the call, stage, field): key ((step * 3 + stage)
+// << 4) | field, atomically minimized.  The step count is advanced by the
+// first stage's k_slow, so a step's physics (which runs before it, and ranks
+// with that step's first stage) reads step - 1 and reports at 3 (step - 1) + 3.
+// The count saturates below the largest step whose key fits an int: beyond
+// 44.7 M steps in one call, later steps share its rank.
+constexpr int STEP_CTR_MAX = (INT_MAX >> 4) / 3 - 1;
+
 template <class S> constexpr bool is_mixed() { return sizeof(S) == 4; }
 
 // Layout of the fields the acoustic substeps stream (theta, rt2pres, the four
@@ -249,7 +257,7 @@ __global__ void __launch_bounds__(64) k_surface(const Dom *__restrict__ doms, co
     w.shflux[x] = sh * cp;
     w.d_qv_sf[x] = lh / dz0;
     w.lhflux[x] = lh * KC::Lv;
-    if (isnan(d_u)) report_nan(w.nan_key, ((*w.step_ctr * 3) << 4) | 8);
+    if (isnan(d_u)) report_nan(w.nan_key, ((*w.step_ctr * 3 + 3) << 4) | 8);
 }
 
 // =============================================================================
@@ -483,15 +491,9 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
     else if (isnan(d_qv)) f = 10;
     else if (isnan(dqc)) f = 11;
     else if (isnan(d_qr)) f = 12;
-    if (f) report_nan(w.nan_key, ((*w.step_ctr * 3) << 4) | f);
+    if (f) report_nan(w.nan_key, ((*w.step_ctr * 3 + 3) << 4) | f);
 }
 
-// Saturates below the largest step whose NaN key (step * 3 + stage) << 4 | f
-// fits an int: beyond 44.7 M steps in one call, later steps share its rank.
-constexpr int STEP_CTR_MAX = (INT_MAX >> 4) / 3 - 1;
-__global__ void k_step_done(int *step_ctr) {
-    if (threadIdx.x == 0 && blockIdx.x == 0 && *step_ctr < STEP_CTR_MAX) *step_ctr += 1;
-}
 
 // =============================================================================
 // Layout conversion between the host State arrays ((ka, ia) column-major,

@@ -36,7 +36,7 @@ constexpr int SL_FLUX = 5 * (ST_Z + 1) * (ST_X + 1);  // largest flux phase (w)
 template <class S, bool P>
 __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
                                                      ZVec zv, SSet<S> in, SSet<S> s0, Work<S> w, S *rtp0,
-                                                     int u_phys_2d) {
+                                                     int u_phys_2d, int *step_bump) {
     using R = S;
     using KC = K<R>;
     // lX holds theta in the R_rho_theta phase, then u, then w
@@ -45,6 +45,10 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
     __shared__ R lF[SL_FLUX];
     __shared__ R lDZ[SL_HZ], lZ[SL_HZ];  // dz and z of levels k0-2 .. k0+ST_Z+1
 
+    // the first stage of a large step advances the step count of the NaN key
+    // (no kernel of this launch reads it; the step's physics, which ran
+    // before, ranks with the step's first stage)
+    if (step_bump && blockIdx.x == 0 && threadIdx.x == 0 && *step_bump < STEP_CTR_MAX) *step_bump += 1;
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;  // padding tile of the XCD ordering (uniform per block)
     const Dom &D = doms[t.x];
