@@ -146,9 +146,13 @@ int moist_use_graph(moist_handle *h, int on);
  * records of its reverse sweep: 0 (default) whole-segment records when they
  * take at most a quarter of the free device memory, else per step; 1 per
  * step (each step's record recomputed from the segment's states); 2 whole
- * segment.  Both give the same gradient.                                    */
+ * segment.  Both give the same gradient.  MOIST_OPT_ADJ_COLUMN_KERNEL
+ * (default 1) runs the vertical part of each reverse acoustic substep as one
+ * kernel per column (members up to 257 levels); 0 runs it as three kernels.
+ * The arithmetic is the same.                                              */
 #define MOIST_OPT_PHYS_DIAG 0
 #define MOIST_OPT_ADJ_RECORDS 1
+#define MOIST_OPT_ADJ_COLUMN_KERNEL 2
 int moist_set_option(moist_handle *h, int option, int value);
 
 /* Time nsteps large steps with HIP events on the handle's stream (state
@@ -192,7 +196,10 @@ int moist_autotune(moist_handle *h, int reps, int *levels, int *threads, double 
  * autotuner reports threads the same way. */
 int moist_set_acoustic_shape(moist_handle *h, int levels, int threads);
 
-/* Reverse mode (do_sf = do_rd = false).  Replaces
+/* Reverse mode (every physics switch, surface fluxes and gray radiation
+ * included; the surface fluxes' halo reads at the domain's ends, copies of
+ * the previous step's stage-3 input, are differentiated as such, and the
+ * initial State's halo columns are taken as periodic copies).  Replaces
  * ForwardDiff / a ChainRulesCore rrule around the reference's driver loop
  * (run_sim, moist.jl:1930-1963, calling rk3_step!, moist.jl:1589): runs
  * nsteps forward from the uploaded state with checkpoints every `ckpt`

@@ -170,6 +170,11 @@ class Engine:
         v = {"auto": 0, "step": 1, "segment": 2}[mode]
         self._check(lib().moist_set_option(self._h, 1, v), "moist_set_option")
 
+    def adjoint_column_kernel(self, on=True):
+        """Vertical part of each reverse acoustic substep as one kernel per column
+        (default) or as three kernels (same arithmetic)."""
+        self._check(lib().moist_set_option(self._h, 2, 1 if on else 0), "moist_set_option")
+
     def kernel_events(self, on=True):
         self._check(lib().moist_kernel_events(self._h, 1 if on else 0), "moist_kernel_events")
 

@@ -122,6 +122,12 @@ __global__ void __launch_bounds__(NT, WPE)
     auto stg = [&](R *p, int k, int col, const R *v) { st2<true>(p, D, k, col, v[0], v[1]); };
     R tp_r0 = R(0.0), tp_t0 = R(0.0), tp_rp = R(0.0), tp_tp = R(0.0);
     R rrw[L];  // R_rho_w per level
+    // the level heights the buoyancy of phase 1 uses, loaded with the first
+    // pair's inputs (a load issued inside the pair's arithmetic would cost
+    // its own memory round trip)
+    double zpre[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) zpre[l] = zd[min(k0 + l, ke)];
 #pragma unroll
     for (int m = 0; m < L / GL; ++m) {
         const int kp = min(k0 + 2 * m, kev);
@@ -201,7 +207,7 @@ __global__ void __launch_bounds__(NT, WPE)
                     tp_t0 = rtp_tp;
                 } else {
                     // d rho_w' at face k-1 (moist.jl:1399-1404)
-                    const R buoy = fdiv(rtp_tp - tp_tp, R(zd[k]) - R(zd[k - 1]));
+                    const R buoy = fdiv(rtp_tp - tp_tp, R(zpre[l]) - R(zpre[l - 1]));
                     const R grav = -K<R>::GRAV * R(0.5) * (rp_tp + tp_rp);
                     const R rt2h = (rt2c[l - 1] + rt2_c) * R(0.5);
                     drw[l - 1] = dt_s * (rrw[l - 1] - rt2h * buoy + grav);

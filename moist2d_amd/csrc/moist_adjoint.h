@@ -951,6 +951,314 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     a.LRt[q] = LRt0 + dt_s * lit;
 }
 
+// ---- acoustic substep, vertical part, one kernel per column ------------------------
+// (1) + (2) + (3) of one reverse substep in one launch: one wave per column,
+// lane l owning the consecutive levels [l Lc, l Lc + Lc) and the solve's rows
+// [l Ls, l Ls + Ls) (Lc = ceil(nz / 64), Ls = ceil((nz - 1) / 64) <= 4).  The
+// column's inputs of the three parts are staged in LDS once; the solve's
+// right-hand side and its solution stay in LDS instead of a global round
+// trip, and two launches per substep are saved.  The arithmetic of every
+// value is that of ka_ac_v1, ka_ac_solve (one column per wave, CH = 64) and
+// ka_ac_v2, in the same order.  cols: {member, column} per block (one wave).
+// Dynamic LDS: AC_COL_ARRAYS doubles per level of the tallest member.
+constexpr int AC_COL_ARRAYS = 11;
+constexpr int AC_COL_MAX_NZ = 256;  // Lc, Ls <= 4: the solve's rows stay in registers
+// LCM: levels a lane may own (1, 2 or 4 for members up to 64, 128, 256 levels)
+template <class RC, int LCM>
+__global__ void __launch_bounds__(64)
+    ka_ac_col(const Dom *__restrict__ doms, const int2 *__restrict__ cols, ZVec zv, SSet<double> s, AdjW<RC> a,
+              const RC *__restrict__ rtp_s, const RC *__restrict__ rup_n, const RC *__restrict__ rwp_s,
+              const RC *__restrict__ rwp_n, int div, int nsub, int factor, int max_ka) {
+    extern __shared__ double lcol[];
+    const int2 t = cols[blockIdx.x];
+    if (t.x < 0) return;  // padding entry of the XCD ordering (uniform per block)
+    const Dom &D = doms[t.x];
+    const int nx = D.nx, nz = D.nz, ke = nz - 1, n = nz - 1;
+    if (n <= 0) return;
+    const int i = t.y, lane = threadIdx.x;
+    double *sRWN = lcol, *sRWS = lcol + max_ka, *sRW = lcol + 2 * max_ka, *sTH = lcol + 3 * max_ka,
+           *sRT = lcol + 4 * max_ka, *sLRTP = lcol + 5 * max_ka, *sLRP = lcol + 6 * max_ka,
+           *sLMZ = lcol + 7 * max_ka, *sTP = lcol + 8 * max_ka, *sRHS = lcol + 9 * max_ka, *sY = lcol + 10 * max_ka;
+    const double *dzd = zv.dz + D.offz;
+    const double4 *gz = zv.geo + D.offz;
+    const double EPS = 1e-12, rns = 1.0 / double(nsub);
+    const double GRAV = K<double>::GRAV, rdx = 1.0 / D.dx;
+    const double dt_s = D.dt / double(div) / double(nsub);
+    const double fact_tp = 0.5 * (1.0 + D.beta), fact_tm = 0.5 * (1.0 - D.beta);
+    const double fact = fact_tm / fact_tp;
+    const double dt_tp2 = (dt_s * fact_tp) * (dt_s * fact_tp);
+    const RC *TH = a.theta;
+    auto G = [&](int kk) -> int64_t { return D.off2 + (int64_t)kk * nx + i; };
+    auto Gx = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
+    const int Lc = (nz + 63) / 64, kc0 = lane * Lc;
+
+    // ---- stage the column (and read this lane's read-modify-write values)
+    double lth0[LCM], lrwp0[LCM], lrt0[LCM], lr0[LCM], lt0[LCM], lRr0[LCM], lRt0[LCM], lRw0[LCM];
+#pragma unroll
+    for (int u = 0; u < LCM; ++u) {
+        const int k = kc0 + u;
+        lth0[u] = lrwp0[u] = lrt0[u] = lr0[u] = lt0[u] = lRr0[u] = lRt0[u] = lRw0[u] = 0.0;
+        if (u < Lc && k < nz) {
+            const int64_t q = G(k);
+            sRWN[k] = rwp_n[q];
+            sRWS[k] = rwp_s[q];
+            sRW[k] = s.f[RHOW][q];
+            sTH[k] = TH[q];
+            sRT[k] = a.rt2[q];
+            sLRTP[k] = a.Lrtp[q];
+            sLRP[k] = a.Lrp[q];
+            sLMZ[k] = a.Lmz[q];
+            // off-centred rho_theta' of the forward explicit part (ka_ac_v2's rtp_tp)
+            const double fip = (TH[Gx(k, i + 1)] + TH[q]) * 0.5 * rup_n[q];
+            const double fim = (TH[q] + TH[Gx(k, i - 1)]) * 0.5 * rup_n[Gx(k, i - 1)];
+            const double inc = dt_s * (a.R_rt[q] + (-(fip - fim) * rdx));
+            sTP[k] = rtp_s[q] + fact_tp * inc;
+            lth0[u] = a.Lth[q];
+            lrwp0[u] = k <= ke - 1 ? a.Lrwp[q] : 0.0;
+            lrt0[u] = a.Lrt[q];
+            lr0[u] = a.Lrp[q];
+            lt0[u] = a.Lrtp[q];
+            lRr0[u] = a.LRr[q];
+            lRt0[u] = a.LRt[q];
+            lRw0[u] = k <= n - 1 ? a.LRw[q] : 0.0;
+        }
+    }
+    __syncthreads();
+
+    // ---- (1) adjoint of rho_w' after the solve (ka_ac_v1)
+    auto THv = [&](int kk) { return kk >= 0 && kk <= ke ? sTH[kk] : 0.0; };
+    auto brp = [&](int f, bool &pos, bool &neg) {
+        pos = sRWN[f] > EPS && f > 0;
+        neg = !pos && sRW[f] < -EPS && f < ke - 1;
+    };
+    auto brm = [&](int f, bool &pos, bool &neg) {
+        const double sw = sRW[f];
+        pos = sw > EPS && f + 1 > 1;
+        neg = !pos && sw < -EPS && f + 1 < ke;
+    };
+    double lth1[LCM];
+#pragma unroll
+    for (int u = 0; u < LCM; ++u) {
+        const int k = kc0 + u;
+        lth1[u] = 0.0;
+        if (!(u < Lc && k < nz)) continue;
+        double lth = 0.0;
+        for (int f = k - 2; f <= k + 1; ++f) {
+            if (f < 0 || f > ke - 1) continue;
+            const double rden = gz[f].z;
+            const double w0 = dzd[f + 1] * rden, w1 = dzd[f] * rden;
+            const double w = sRWN[f];
+            bool pos, neg;
+            brp(f, pos, neg);
+            lth += -dt_s * gz[f].x * sLRTP[f] * w * upw_coef(pos, neg, k - f, w0, w1);
+            brm(f, pos, neg);
+            lth += dt_s * gz[f + 1].x * sLRTP[f + 1] * w * upw_coef(pos, neg, k - f, w0, w1);
+        }
+        lth1[u] = lth0[u] + lth;
+        if (k <= ke - 1) {
+            const double rden = gz[k].z;
+            const double w0 = dzd[k + 1] * rden, w1 = dzd[k] * rden;
+            bool pos, neg;
+            brp(k, pos, neg);
+            const double gp = upw_val(pos, neg, THv(k - 1), THv(k), THv(k + 1), THv(k + 2), w0, w1);
+            brm(k, pos, neg);
+            const double gm = upw_val(pos, neg, THv(k - 1), THv(k), THv(k + 1), THv(k + 2), w0, w1);
+            const double lrwp = sLMZ[k] * rns - dt_s * gz[k].x * sLRTP[k] * gp + dt_s * gz[k + 1].x * sLRTP[k + 1] * gm -
+                                dt_s * gz[k].x * sLRP[k] + dt_s * gz[k + 1].x * sLRP[k + 1];
+            sRHS[k] = lrwp0[u] + lrwp;
+        }
+    }
+    __syncthreads();
+
+    // ---- (2) transposed solve A^T y = rhs (ka_ac_solve, one column per wave)
+    auto th = [&](int kk) { return (sTH[kk] * dzd[kk + 1] + sTH[kk + 1] * dzd[kk]) * gz[kk].z; };
+    auto Wa = [&](int kk) { return -dt_tp2 * (sRT[kk] * th(kk - 1) * (gz[kk].y * gz[kk].x) - gz[kk].w); };
+    auto Wb = [&](int kk) { return dt_tp2 * (sRT[kk] * gz[kk].x + sRT[kk + 1] * gz[kk + 1].x) * th(kk) * gz[kk].y; };
+    auto Wc = [&](int kk) { return -dt_tp2 * (sRT[kk + 1] * th(kk + 1) * (gz[kk].y * gz[kk + 1].x) + gz[kk].w); };
+    {
+        constexpr int CH = 64, LR = 4;
+        const int ch = lane;
+        const int L = (n + CH - 1) / CH, j0 = ch * L, j1 = min(j0 + L, n);
+        double cpv[LR], rmv[LR], acv[LR];
+#pragma unroll
+        for (int q = 0; q < LR; ++q) cpv[q] = rmv[q] = acv[q] = 0.0;
+        if (factor) {
+            double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;
+            for (int j = j0; j < j1; ++j) {
+                const double aj = j > 0 ? Wa(j) : 0.0, bj = Wb(j) + 1.0, cj = j < n - 1 ? Wc(j) : 0.0;
+                const double n00 = cj * m10, n01 = cj * m11;
+                const double n10 = -aj * m00 + bj * m10, n11 = -aj * m01 + bj * m11;
+                m00 = n00;
+                m01 = n01;
+                m10 = n10;
+                m11 = n11;
+                mob_norm(m00, m01, m10, m11);
+            }
+#pragma unroll
+            for (int d = 1; d < CH; d <<= 1) {
+                const double p00 = __shfl_up(m00, d), p01 = __shfl_up(m01, d), p10 = __shfl_up(m10, d),
+                             p11 = __shfl_up(m11, d);
+                if (ch >= d) {
+                    const double n00 = m00 * p00 + m01 * p10, n01 = m00 * p01 + m01 * p11;
+                    const double n10 = m10 * p00 + m11 * p10, n11 = m10 * p01 + m11 * p11;
+                    m00 = n00;
+                    m01 = n01;
+                    m10 = n10;
+                    m11 = n11;
+                    mob_norm(m00, m01, m10, m11);
+                }
+            }
+            const double e01 = __shfl_up(m01, 1), e11 = __shfl_up(m11, 1);
+            double cprev = ch == 0 ? 0.0 : e01 / e11;
+#pragma unroll
+            for (int q = 0; q < LR; ++q) {
+                const int j = j0 + q;
+                if (j < j1) {
+                    const double aj = j > 0 ? Wa(j) : 0.0;
+                    const double m = Wb(j) + 1.0 - aj * cprev;
+                    const double rm = 1.0 / m;
+                    const double cp = j < n - 1 ? Wc(j) * rm : 0.0;
+                    a.cpr[G(j)] = cp;
+                    a.rmi[G(j)] = rm;
+                    a.aco[G(j)] = aj;
+                    cpv[q] = cp;
+                    rmv[q] = rm;
+                    acv[q] = aj;
+                    cprev = cp;
+                }
+            }
+        }
+        double lx[LR];
+#pragma unroll
+        for (int q = 0; q < LR; ++q) {
+            const int j = j0 + q;
+            lx[q] = j < j1 ? sRHS[j] : 0.0;
+            if (!factor && j < j1) {
+                cpv[q] = a.cpr[G(j)];
+                rmv[q] = a.rmi[G(j)];
+                acv[q] = a.aco[G(j)];
+            }
+        }
+        double cp_last = 0.0;
+#pragma unroll
+        for (int q = 0; q < LR; ++q)
+            if (q == L - 1) cp_last = cpv[q];
+        const double cp_below = __shfl_up(cp_last, 1), a_above = __shfl_down(acv[0], 1);
+        double A = 1.0, B = 0.0;
+#pragma unroll
+        for (int q = 0; q < LR; ++q) {
+            const int j = j0 + q;
+            if (j < j1) {
+                const double al = j > 0 ? -(q == 0 ? cp_below : cpv[q > 0 ? q - 1 : 0]) : 0.0;
+                B = al * B + lx[q];
+                A = al * A;
+            }
+        }
+        scan_affine_up<CH>(A, B, ch);
+        double wv = __shfl_up(B, 1);
+        if (ch == 0) wv = 0.0;
+#pragma unroll
+        for (int q = 0; q < LR; ++q) {
+            const int j = j0 + q;
+            if (j < j1) {
+                const double al = j > 0 ? -(q == 0 ? cp_below : cpv[q > 0 ? q - 1 : 0]) : 0.0;
+                wv = al * wv + lx[q];
+                lx[q] = wv;
+            }
+        }
+        A = 1.0;
+        B = 0.0;
+#pragma unroll
+        for (int q = LR - 1; q >= 0; --q) {
+            const int j = j0 + q;
+            if (j < j1) {
+                const double rm = rmv[q];
+                const double al = j < n - 1 ? -rm * (q + 1 < L ? acv[q < LR - 1 ? q + 1 : 0] : a_above) : 0.0;
+                B = al * B + rm * lx[q];
+                A = al * A;
+            }
+        }
+        scan_affine_down<CH>(A, B, ch);
+        double yv = __shfl_down(B, 1);
+        if (ch == CH - 1) yv = 0.0;
+#pragma unroll
+        for (int q = LR - 1; q >= 0; --q) {
+            const int j = j0 + q;
+            if (j < j1) {
+                const double rm = rmv[q];
+                const double al = j < n - 1 ? -rm * (q + 1 < L ? acv[q < LR - 1 ? q + 1 : 0] : a_above) : 0.0;
+                yv = al * yv + rm * lx[q];
+                sY[j] = yv;
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- (3) matrix, right-hand-side and explicit-part adjoints (ka_ac_v2)
+    auto yv = [&](int j) { return j >= 0 && j <= n - 1 ? sY[j] : 0.0; };
+    auto xv = [&](int j) { return j >= 0 && j <= n - 1 ? sRWN[j] : 0.0; };
+    auto La = [&](int j) { return j >= 1 && j <= n - 1 ? -yv(j) * xv(j - 1) - fact * yv(j) * sRWS[j] : 0.0; };
+    auto Lb = [&](int j) { return j >= 0 && j <= n - 1 ? -yv(j) * xv(j) - fact * yv(j) * sRWS[j] : 0.0; };
+    auto Lc_ = [&](int j) { return j >= 0 && j < n - 1 ? -yv(j) * xv(j + 1) - fact * yv(j) * sRWS[j + 1] : 0.0; };
+    auto Lthc = [&](int j) {
+        if (j < 0 || j > n - 1) return 0.0;
+        double v = Lb(j) * dt_tp2 * (sRT[j] * gz[j].x + sRT[j + 1] * gz[j + 1].x) * gz[j].y;
+        if (j + 1 <= n - 1) v += La(j + 1) * (-dt_tp2 * sRT[j + 1] * (gz[j + 1].y * gz[j + 1].x));
+        if (j >= 1) v += Lc_(j - 1) * (-dt_tp2 * sRT[j] * (gz[j - 1].y * gz[j].x));
+        return v;
+    };
+#pragma unroll
+    for (int u = 0; u < LCM; ++u) {
+        const int k = kc0 + u;
+        if (!(u < Lc && k < nz)) continue;
+        const int64_t q = G(k);
+        const double lr = lr0[u], lt = lt0[u];
+        double lrt = 0.0, lth = 0.0, Lrp_tp = 0.0, Lrtp_tp = 0.0, lws = 0.0, lRw = 0.0;
+        if (k <= n - 1) {
+            const double y = yv(k);
+            const double thk = th(k), rz1 = gz[k].y, rzd = rz1 * gz[k].x;
+            lrt += Lb(k) * dt_tp2 * thk * rzd;
+            if (k >= 1) lrt += La(k) * (-dt_tp2 * th(k - 1) * rzd);
+            const double buoy = (sTP[k + 1] - sTP[k]) * rz1;
+            lrt += -0.5 * dt_s * y * buoy;
+            // the records' sum in their own type, as ka_ac_v2 forms it
+            const double rt2h = (RC(sRT[k]) + RC(sRT[k + 1])) * 0.5;
+            Lrtp_tp += dt_s * y * rt2h * rz1;
+            Lrp_tp += -dt_s * y * GRAV * 0.5;
+            lRw = dt_s * y;
+            lth += Lthc(k) * dzd[k + 1] * gz[k].z;
+            lws = y * (1.0 - fact * Wb(k));
+            if (k >= 1) lws += -fact * Wa(k) * y - fact * Wc(k - 1) * yv(k - 1);
+        }
+        if (k >= 1) {
+            const int kb = k - 1;
+            const double yb = yv(kb);
+            const double rz1 = gz[kb].y, rzd = rz1 * gz[k].x;
+            lrt += Lb(kb) * dt_tp2 * th(kb) * rzd;
+            if (kb < n - 1) lrt += Lc_(kb) * (-dt_tp2 * th(k) * rzd);
+            const double buoy = (sTP[k] - sTP[kb]) * rz1;
+            lrt += -0.5 * dt_s * yb * buoy;
+            const double rt2h = (RC(sRT[kb]) + RC(sRT[k])) * 0.5;
+            Lrtp_tp += -dt_s * yb * rt2h * rz1;
+            Lrp_tp += -dt_s * yb * GRAV * 0.5;
+            lth += Lthc(kb) * dzd[kb] * gz[kb].z;
+        }
+        const double lir = lr + fact_tp * Lrp_tp, lit = lt + fact_tp * Lrtp_tp;
+        if (k <= n - 1) {
+            a.LRw[q] = lRw0[u] + lRw;
+            a.Lrwp[q] = lws;
+        }
+        a.Lrt[q] = lrt0[u] + lrt;
+        a.Lth[q] = lth1[u] + lth;
+        a.Lrp[q] = lr + Lrp_tp;
+        a.Lrtp[q] = lt + Lrtp_tp;
+        a.Linc_r[q] = lir;
+        a.Linc_t[q] = lit;
+        a.LRr[q] = lRr0[u] + dt_s * lir;
+        a.LRt[q] = lRt0[u] + dt_s * lit;
+    }
+}
+
 // ---- acoustic substep, horizontal part (moist.jl:1360-1408) -----------------------
 // Per cell.  Lrup_in: adjoint of rho_u' after this substep (from later
 // substeps and the final update); Lrup_out: adjoint before it.  pend: the
@@ -1411,15 +1719,219 @@ __device__ __forceinline__ void kessler_v(const Dom &D, V rho_k, V rt, V rqv, V 
     out[3] = dth;
 }
 
-// ---- physics adjoint (moist.jl:554-707, 1632-1644), do_sf = do_rd = false -------
+// ---- surface fluxes (moist.jl:710-752): adjoint, one thread per column ---------
+// Layout of the per-column buffer sfb (SF_SLOTS x nxs doubles):
+//   0..5  adjoints of the surface cell's rho, rho_u, rho_theta, rho_qv, rho_qc, rho_qr
+//   6     adjoint of rho_u at the face left of the column (column i-1's)
+//   7     adjoint of rho of the cell right of the column (column i+1's)
+//   8     the forward d_rho_qv_sf (enters d_rho, and so d_rho_theta's d_rho rho_theta / rho)
+// At the domain's ends the left / right values are the stale halo copies
+// `stale` (2 per member) the forward read; their adjoints go to carry
+// (2 per member), to be added to the previous step's stage-3 input.
+constexpr int SF_SLOTS = 9;
+template <class V> __device__ __forceinline__ V vabs(const V &x) { return val(x) < 0.0 ? -x : x; }
+template <class V>
+__device__ __forceinline__ void surface_v(const Dom &D, double dz0, V rho, V ru, V rt, V rqv, V rqc, V rqr, V ru_l,
+                                          V rho_r, V out[3]) {
+    const K<double> c;
+    V qv = rqv / rho, qc = rqc / rho, qr = rqr / rho;
+    V qd = V(1.0) - qv - qc - qr;
+    V r = c.Rd * qd + c.Rv * qv;
+    V cp = c.Cpd * qd + c.Cpv * qv + c.Cpl * (qc + qr);
+    V cv = cp - r;
+    V uabs = vabs(ru_l + ru) * 0.5 / rho;
+    uabs = vmax(uabs, V(D.uabs_min));
+    V theta_ks = rt / rho;
+    V pi_ks = d_pow(r * rt / c.P0, r / cv);
+    V pi_surf = pi_ks + dz0 * 0.5 * c.GRAV / (cp * theta_ks);
+    V p_surf = c.P0 * d_pow(pi_surf, cp / r);
+    V rho_surf = p_surf / (r * D.temp_surf);
+    V theta_surf = D.temp_surf / pi_surf;
+    V u = ru / (rho + rho_r) * 2.0;
+    V mflux = -rho_surf * D.Cd * u * vabs(u);
+    out[0] = mflux / dz0;
+    V sh = rho_surf * D.Ch * uabs * (theta_surf - theta_ks) * pi_surf;
+    out[1] = sh / (pi_ks * dz0);
+    double es, des;
+    sat_vp<double>(D.temp_surf, es, des);
+    V tq = p_surf - (1.0 - c.EPSvap) * es;
+    V qsat = val(tq) > c.EPS ? c.EPSvap * es / tq : V(0.0);
+    V lh = rho_surf * D.Ce * uabs * (qsat - qv);
+    out[2] = lh / dz0;
+}
+
+__global__ void __launch_bounds__(64)
+    ka_surface(const Dom *__restrict__ doms, const int2 *__restrict__ ctiles, ZVec zv, SSet<double> s,
+               const double *__restrict__ stale, SSet<double> lph, double *sfb, int64_t nxs, double *carry) {
+    const int2 t = ctiles[blockIdx.x];
+    if (t.x < 0) return;
+    const int m = t.x;
+    const Dom &D = doms[m];
+    const int i = t.y + threadIdx.x, nx = D.nx;
+    if (i >= nx || !(D.flags & F_SF)) return;
+    typedef Dl<8> V;
+    const int64_t c = D.off2 + i;
+    const double ru_l = i == 0 ? stale[2 * m + 0] : s.f[RHOU][c - 1];
+    const double rho_r = i == nx - 1 ? stale[2 * m + 1] : s.f[RHO][c + 1];
+    V out[3];
+    surface_v<V>(D, zv.dz[D.offz], V::var(s.f[RHO][c], 0), V::var(s.f[RHOU][c], 1), V::var(s.f[RT][c], 2),
+                 V::var(s.f[QV][c], 3), V::var(s.f[QC][c], 4), V::var(s.f[QR][c], 5), V::var(ru_l, 6),
+                 V::var(rho_r, 7), out);
+    // d_rho_u_phys, d_rho_theta_phys and d_qv_phys at the ground take the three
+    // fluxes; d_rho takes d_qv_sf, which d_rho_theta sees through d_rho rho_theta / rho
+    const double lqv = lph.f[0][c] + lph.f[2][c] * s.f[RT][c] / s.f[RHO][c] + lph.f[3][c];
+    const double lo[3] = {lph.f[1][c], lph.f[2][c], lqv};
+    const int64_t x = D.offx + i;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        double g = 0.0;
+#pragma unroll
+        for (int o = 0; o < 3; ++o) g += lo[o] * out[o].d[j];
+        sfb[j * nxs + x] = g;
+    }
+    sfb[8 * nxs + x] = out[2].v;
+    if (i == 0) carry[2 * m + 0] = sfb[6 * nxs + x];
+    if (i == nx - 1) carry[2 * m + 1] = sfb[7 * nxs + x];
+}
+
+// The adjoints of the two stale halo reads of the step reversed just before
+// (carry) enter this step's stage-3 input adjoint l at (0, nx-1) for rho_u
+// and (0, 0) for rho.  One thread per member.
+__global__ void ka_stale_add(const Dom *__restrict__ doms, int nm, SSet<double> l, const double *carry) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nm) return;
+    const Dom &D = doms[m];
+    if (!(D.flags & F_SF)) return;
+    l.f[RHOU][D.off2 + D.nx - 1] += carry[2 * m + 0];
+    l.f[RHO][D.off2] += carry[2 * m + 1];
+}
+
+// ---- gray radiation (moist.jl:776-840): adjoint, one thread per column ----------
+// g = adjoint of d_rho_theta_rd (= that of d_rho_theta_phys); adds into l
+// (rho, rho_theta, rho_qv, rho_qc, rho_qr).  The upward long-wave recursion is
+// reversed from the top, the downward one from the ground; scratch (5 x n2):
+// the forward fluxes up at face k+1 and dn at face k, and the partial
+// adjoints of tau, eps and B = sigma T^4 of the first reversed recursion.
+__device__ __forceinline__ void rad_cell(const Dom &D, const ZVec &zv, const SSet<double> &s, int k, int64_t q,
+                                         double &eps, double &ex, double &B, Dl<5> &Td, Dl<5> &CP) {
+    const K<double> c;
+    typedef Dl<5> V;
+    V rho = V::var(s.f[RHO][q], 0), rt = V::var(s.f[RT][q], 1), rqv = V::var(s.f[QV][q], 2),
+      rqc = V::var(s.f[QC][q], 3), rqr = V::var(s.f[QR][q], 4);
+    V qv = rqv / rho, qc = rqc / rho, qr = rqr / rho;
+    V qd = V(1.0) - qv - qc - qr;
+    V r = c.Rd * qd + c.Rv * qv;
+    V cp = c.Cpd * qd + c.Cpv * qv + c.Cpl * (qc + qr);
+    V cv = cp - r;
+    V pex = d_pow(r * rt / c.P0, r / cv);
+    Td = rt / rho * pex;
+    CP = cp * pex;
+    const double dzk = zv.dz[D.offz + k];
+    const double dopt = D.k_abs_qv * s.f[QV][q] * dzk + D.k_abs_qc * s.f[QC][q] * dzk + D.k_abs_qr * s.f[QR][q] * dzk;
+    ex = exp(-dopt);
+    eps = 1.0 - ex;
+    const double tv = Td.v;
+    B = c.SIGMA * (tv * tv * tv * tv);
+}
+
+__global__ void __launch_bounds__(64)
+    ka_radiation(const Dom *__restrict__ doms, const int2 *__restrict__ ctiles, ZVec zv, SSet<double> s,
+                 const double *__restrict__ theta_ref, const double *__restrict__ g, SSet<double> l, double *sc0,
+                 double *sc1, double *sc2, double *sc3, double *sc4) {
+    const int2 t = ctiles[blockIdx.x];
+    if (t.x < 0) return;
+    const Dom &D = doms[t.x];
+    const int i = t.y + threadIdx.x, nx = D.nx, nz = D.nz;
+    if (i >= nx || !(D.flags & F_RD)) return;
+    const K<double> c;
+    const double *dzd = zv.dz + D.offz, *zd = zv.z + D.offz;
+    auto G = [&](int kk) -> int64_t { return D.off2 + (int64_t)kk * nx + i; };
+    double *UP = sc0, *DN = sc1, *LTAU = sc2, *LEPS = sc3, *LB = sc4;
+    const double ts = D.temp_surf;
+    const double up0 = D.surf_emissivity * c.SIGMA * (ts * ts * ts * ts);
+    // forward: up at faces 1..nz (stored at levels 0..nz-1), dn at faces 0..nz-1
+    double up = up0;
+    for (int k = 0; k < nz; ++k) {
+        double eps, ex, B;
+        Dl<5> Td, CP;
+        rad_cell(D, zv, s, k, G(k), eps, ex, B, Td, CP);
+        up = up * (1.0 - eps) + eps * B;
+        UP[G(k)] = up;
+    }
+    double dn = 0.0;
+    for (int k = nz - 1; k >= 0; --k) {
+        double eps, ex, B;
+        Dl<5> Td, CP;
+        rad_cell(D, zv, s, k, G(k), eps, ex, B, Td, CP);
+        dn = dn * (1.0 - eps) + eps * B;
+        DN[G(k)] = dn;
+    }
+    auto upf = [&](int j) { return j == 0 ? up0 : UP[G(j - 1)]; };   // up at face j
+    auto dnf = [&](int j) { return j == nz ? 0.0 : DN[G(j)]; };     // dn at face j
+    // direct adjoint of the net flux F = up - dn at face j: from the heating of
+    // the levels below (j-1) and above (j); heating hr_k = -divF_k / (cp Pi)_k
+    auto ldiv = [&](int k) {
+        double eps, ex, B;
+        Dl<5> Td, CP;
+        rad_cell(D, zv, s, k, G(k), eps, ex, B, Td, CP);
+        return -g[G(k)] / CP.v / dzd[k];
+    };
+    auto lF = [&](int j) { return (j >= 1 ? ldiv(j - 1) : 0.0) - (j <= nz - 1 ? ldiv(j) : 0.0); };
+    // up[k+1] = up[k] tau_k + eps_k B_k, reversed from the top
+    double Lup = lF(nz);  // total adjoint of up at face nz
+    for (int k = nz - 1; k >= 0; --k) {
+        double eps, ex, B;
+        Dl<5> Td, CP;
+        rad_cell(D, zv, s, k, G(k), eps, ex, B, Td, CP);
+        LTAU[G(k)] = Lup * upf(k);
+        LEPS[G(k)] = Lup * B;
+        LB[G(k)] = Lup * eps;
+        Lup = lF(k) + Lup * (1.0 - eps);
+    }
+    // dn[k] = dn[k+1] tau_k + eps_k B_k, reversed from the ground; then the
+    // cell's adjoints: tau = 1 - eps, eps = 1 - exp(-dopt), B = sigma T^4, cp Pi,
+    // and the Newtonian relaxation -(rho_theta - rho theta_ref) alpha
+    double Ldn = -lF(0);
+    const double dz_trans = 2.0e3;
+    for (int k = 0; k < nz; ++k) {
+        const int64_t q = G(k);
+        double eps, ex, B;
+        Dl<5> Td, CP;
+        rad_cell(D, zv, s, k, q, eps, ex, B, Td, CP);
+        const double ltau = LTAU[q] + Ldn * dnf(k + 1);
+        const double leps = LEPS[q] + Ldn * B;
+        const double lB = LB[q] + Ldn * eps;
+        Ldn = -lF(k + 1) + Ldn * (1.0 - eps);
+        const double divF = ((upf(k + 1) - dnf(k + 1)) - (upf(k) - dnf(k))) / dzd[k];
+        const double gk = g[q];
+        const double lcpp = gk * divF / (CP.v * CP.v);
+        const double ldopt = (leps - ltau) * ex;
+        const double tv = Td.v;
+        const double lT = lB * c.SIGMA * 4.0 * tv * tv * tv;
+        const double alpha =
+            0.5 * (1.0 + tanh((zd[k] - (D.z_tropopose + dz_trans * 0.5)) / dz_trans)) / D.tau_cooling;
+        const double dzk = dzd[k];
+        const double a0 = l.f[RHO][q], a1 = l.f[RT][q], a2 = l.f[QV][q], a3 = l.f[QC][q], a4 = l.f[QR][q];
+        l.f[RHO][q] = a0 + (lT * Td.d[0] + lcpp * CP.d[0]) + alpha * theta_ref[D.offz + k] * gk;
+        l.f[RT][q] = a1 + (lT * Td.d[1] + lcpp * CP.d[1]) - alpha * gk;
+        l.f[QV][q] = a2 + (lT * Td.d[2] + lcpp * CP.d[2]) + ldopt * D.k_abs_qv * dzk;
+        l.f[QC][q] = a3 + (lT * Td.d[3] + lcpp * CP.d[3]) + ldopt * D.k_abs_qc * dzk;
+        l.f[QR][q] = a4 + (lT * Td.d[4] + lcpp * CP.d[4]) + ldopt * D.k_abs_qr * dzk;
+    }
+}
+
+// ---- physics adjoint (moist.jl:554-707, 1632-1644) ----------------------------------
 // lph: adjoint of the six tendencies; ldprec: adjoint of d_prec (dx * dt).
 // lout = ls + ls0 + the physics adjoints (lout may be another buffer than ls:
 // each cell reads and writes only itself); accumulates dJ/dK per cell into gk
-// (3 fields).
+// (3 fields).  sfb (surface fluxes on): ka_surface's per-column adjoints,
+// gathered at the ground (own cell, the right column's left face, the left
+// column's right cell) and its forward d_qv_sf; null otherwise.  The gray
+// radiation's adjoint is added afterwards by ka_radiation.
 __global__ void __launch_bounds__(AD_X *AD_Z)
     ka_physics(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s,
                SSet<double> lph, SSet<double> ls, SSet<double> ls0, SSet<double> lout, double *gk0, double *gk1,
-               double *gk2) {
+               double *gk2, const double *__restrict__ sfb, int64_t nxs) {
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
@@ -1441,15 +1953,31 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     auto flux = [&](int kk) { return rqr[G(kk)] * vt(kk); };
     double sedc = 0.0;
     if (sed) sedc = ((k < ke ? flux(k + 1) : 0.0) - flux(k)) / dzd[k];
+    const bool sf = sfb && (D.flags & F_SF) && k == 0;
+    const int64_t x = D.offx + i;
+    // d_rho = d_qr_sed (+ d_qv_sf at the ground)
+    const double drho = sf ? sedc + sfb[8 * nxs + x] : sedc;
     // ls += ls0 (the step input is also every stage's s0) plus the physics
     // adjoints below; all seven read first, written at the end
     double o[7];
 #pragma unroll
     for (int f = 0; f < 7; ++f) o[f] = ls.f[f][c] + ls0.f[f][c];
     double l_r = 0.0, l_rt = 0.0, l_qr = 0.0;
-    // d_rho_theta = d_th_mp + d_rho rho_theta / rho with d_rho = d_qr_sed
-    l_rt += lph.f[2][c] * sedc / rho[c];
-    l_r += -lph.f[2][c] * sedc * rt[c] / (rho[c] * rho[c]);
+    // d_rho_theta = d_th_mp (+ d_th_rd) + d_rho rho_theta / rho (+ d_th_sf)
+    l_rt += lph.f[2][c] * drho / rho[c];
+    l_r += -lph.f[2][c] * drho * rt[c] / (rho[c] * rho[c]);
+    if (sf) {
+        // the surface fluxes' adjoints at this column, and those of the
+        // neighbours that read this cell (periodic: the ends read stale copies)
+        o[RHO] += sfb[x];
+        o[RHOU] += sfb[nxs + x];
+        o[RT] += sfb[2 * nxs + x];
+        o[QV] += sfb[3 * nxs + x];
+        o[QC] += sfb[4 * nxs + x];
+        o[QR] += sfb[5 * nxs + x];
+        if (i < nx - 1) o[RHOU] += sfb[6 * nxs + x + 1];
+        if (i > 0) o[RHO] += sfb[7 * nxs + x - 1];
+    }
     if (mp) {
         // partials w.r.t. the five state fields, then w.r.t. the three parameters
         // (two narrower dual evaluations of the same formula keep the registers in bounds)

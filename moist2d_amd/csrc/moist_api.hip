@@ -105,6 +105,9 @@ struct moist_handle {
                                            // minimum waves per SIMD of its register budget
     int tuned = 0;
     int2 *wgroups = nullptr;  // {member, i0} of the adjoint column solve's waves
+    int2 *acols = nullptr;    // {member, i} of every column (the fused acoustic adjoint, one wave each)
+    int nacols = 0;
+    int adj_col = 1;          // reverse substeps with the one-kernel-per-column vertical part
     int nwgroups = 0, wch = 8;  // chunks per column in those waves
     int natiles = 0;
     int nch_max = 0;
@@ -396,15 +399,20 @@ void enqueue_stage(moist_handle *h, int in_set, int out_set, int s0_set, int div
                        diag_out ? w.theta : nullptr, w.pres, w.rt2);
 }
 
-template <class S> void enqueue_step(moist_handle *h) {
-    const ZVec zv = zvec(h);
-    const Work<S> w = work<S>(h);
-    const SSet<S> s = sset<S>(h, S0_);
+// Surface fluxes and gray radiation of the step-start state s (flag-gated).
+template <class S> void enqueue_sf_rd(moist_handle *h, const ZVec &zv, const SSet<S> &s, const Work<S> &w) {
     if (h->any_sf)
         hipLaunchKernelGGL(k_surface<S>, dim3(h->nctiles), dim3(COL_TILE), 0, h->stream, h->dom_d, h->ctiles, zv, s, w);
     if (h->any_rd)
         hipLaunchKernelGGL(k_radiation<S>, dim3(h->nctiles), dim3(COL_TILE), 0, h->stream, h->dom_d, h->ctiles, zv,
                            s, w);
+}
+
+template <class S> void enqueue_step(moist_handle *h) {
+    const ZVec zv = zvec(h);
+    const Work<S> w = work<S>(h);
+    const SSet<S> s = sset<S>(h, S0_);
+    enqueue_sf_rd<S>(h, zv, s, w);
     // in the mixed form k_physics also writes the first stage's diagnostics,
     // and stages 2 and 3 read those the previous stage's k_scalar wrote
     constexpr bool PL = is_mixed<S>();
@@ -731,6 +739,10 @@ struct AdjState {
     int segrec = 0, nblocks = 1;
     double *lam = nullptr, *lam2 = nullptr, *ls0 = nullptr, *lph = nullptr, *wk = nullptr, *lim = nullptr;
     double *gk = nullptr, *tgt = nullptr, *fin = nullptr, *J = nullptr, *gK = nullptr;
+    // surface fluxes: the stale halo pair of every step of a segment (2 per
+    // member), the adjoints of a step's stale reads carried to the step
+    // before, and ka_surface's per-column buffer
+    double *sfh_seg = nullptr, *carry = nullptr, *sfb = nullptr;
     std::vector<double *> allocs;
     size_t bytes = 0;
     std::vector<double> J_h, gK_h;
@@ -762,7 +774,7 @@ int adj_alloc(moist_handle *h, int nsteps, int ckpt) {
     const size_t n2 = h->n2;
     const bool mixed = h->precision == MOIST_MIXED;
     A.cp_words = (7 * n2 * h->elem + 7) / 8;
-    A.cp_stride = A.cp_words + h->nxs;
+    A.cp_stride = A.cp_words + h->nxs + 2 * (size_t)h->nm;  // state, prec, the stale halo pair
     A.rec_elem = (int)h->elem;
     A.seg_bytes = (7 * n2 * h->elem + 255) / 256 * 256;
     int nrec_sub = 0;
@@ -796,6 +808,9 @@ int adj_alloc(moist_handle *h, int nsteps, int ckpt) {
         al(A.gK, 3 * h->nm))
         return -1;
     if (mixed && al(A.snb, 7 * n2)) return -1;
+    if (al(A.sfh_seg, 2 * (size_t)h->nm * ckpt) || al(A.carry, 2 * (size_t)h->nm) ||
+        al(A.sfb, SF_SLOTS * (size_t)h->nxs))
+        return -1;
     A.seg = (char *)seg;
     A.recs = (char *)recs;
     A.cur = A.recs;
@@ -934,6 +949,7 @@ void enqueue_step_rec64(moist_handle *h, const double *sn) {
     const ZVec zv = zvec(h);
     const Work<double> w = work<double>(h);
     const SSet<double> s0 = sset<double>(h, S0_), ia = dset(rec_in(h, 1), n2), ib = dset(rec_in(h, 2), n2);
+    enqueue_sf_rd<double>(h, zv, s0, w);
     hipLaunchKernelGGL(k_physics<double>, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d,
                        h->tiles2d, zv, s0, w, 0, 0);
     enqueue_stage_rec<double>(h, 0, s0, ia, s0, 3);
@@ -951,6 +967,7 @@ void enqueue_step_rec_mixed(moist_handle *h) {
     const dim3 b2(TILE_X, TILE_Z);
     const Work<float> w = work<float>(h);
     const SSet<float> s0 = sset<float>(h, S0_), sa = sset<float>(h, SA_), sb = sset<float>(h, SB_);
+    enqueue_sf_rd<float>(h, zv, s0, w);
     hipLaunchKernelGGL(k_physics<float>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, s0, w, 0, 0);
     enqueue_stage_rec<float>(h, 0, s0, sa, s0, 3);
     hipLaunchKernelGGL(ka_widen, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, sa,
@@ -964,7 +981,8 @@ void enqueue_step_rec_mixed(moist_handle *h) {
 // Adjoint of one step from its record (the current block, records of type
 // RC): A.lam holds the adjoint of the step output on entry and of the step
 // input `sn` (FP64) on exit.
-template <class RC> void enqueue_step_adj(moist_handle *h, const double *sn) {
+// stale: the step's stale halo pair (surface fluxes).
+template <class RC> void enqueue_step_adj(moist_handle *h, const double *sn, const double *stale) {
     AdjState &A = *h->adj;
     const size_t n2 = h->n2;
     const ZVec zv = zvec(h);
@@ -1001,23 +1019,46 @@ template <class RC> void enqueue_step_adj(moist_handle *h, const double *sn) {
         int cur = 0;
         for (int n = nsub; n >= 1; --n) {
             const int m = n - 1;
-            hipLaunchKernelGGL(ka_ac_v1<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in, a,
-                               (const RC *)rec_rwp<RC>(h, st, m + 1), div, nsub);
-            // one wave (one column group) per block: the waves are independent,
-            // and small grids then reach every CU
-            const dim3 gs(h->nwgroups), bs(64);
-            if (h->wch == 4)
-                hipLaunchKernelGGL((ka_ac_solve<4, RC>), gs, bs, 0, h->stream, h->dom_d, h->wgroups, h->nwgroups, zv,
-                                   a, div, nsub, n == nsub ? 1 : 0);
-            else if (h->wch == 64)
-                hipLaunchKernelGGL((ka_ac_solve<64, RC>), gs, bs, 0, h->stream, h->dom_d, h->wgroups, h->nwgroups,
-                                   zv, a, div, nsub, n == nsub ? 1 : 0);
-            else
-                hipLaunchKernelGGL((ka_ac_solve<8, RC>), gs, bs, 0, h->stream, h->dom_d, h->wgroups, h->nwgroups, zv,
-                                   a, div, nsub, n == nsub ? 1 : 0);
-            hipLaunchKernelGGL(ka_ac_v2<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, a,
-                               (const RC *)rec_rtp<RC>(h, st, m), (const RC *)rec_rup<RC>(h, st, m + 1),
-                               (const RC *)rec_rwp<RC>(h, st, m), (const RC *)rec_rwp<RC>(h, st, m + 1), div, nsub);
+            // the one-kernel-per-column form when one wave per column is also the
+            // three-kernel form's layout (few columns: the kernels are bound by
+            // latency, and launches and round trips are what the fusion saves);
+            // many columns keep the three kernels, whose waves hold 8 or 16
+            // adjacent columns and so coalesce their loads
+            if (h->adj_col && h->max_ka <= AC_COL_MAX_NZ && h->wch == 64) {
+                // the vertical part in one kernel per column
+                const unsigned lds = (unsigned)(AC_COL_ARRAYS * sizeof(double) * h->max_ka);
+                const RC *rtp_s = rec_rtp<RC>(h, st, m), *rup_n = rec_rup<RC>(h, st, m + 1),
+                         *rwp_s = rec_rwp<RC>(h, st, m), *rwp_n = rec_rwp<RC>(h, st, m + 1);
+                const int fac = n == nsub ? 1 : 0;
+                if (h->max_ka <= 64)
+                    hipLaunchKernelGGL((ka_ac_col<RC, 1>), dim3(h->nacols), dim3(64), lds, h->stream, h->dom_d,
+                                       h->acols, zv, in, a, rtp_s, rup_n, rwp_s, rwp_n, div, nsub, fac, h->max_ka);
+                else if (h->max_ka <= 128)
+                    hipLaunchKernelGGL((ka_ac_col<RC, 2>), dim3(h->nacols), dim3(64), lds, h->stream, h->dom_d,
+                                       h->acols, zv, in, a, rtp_s, rup_n, rwp_s, rwp_n, div, nsub, fac, h->max_ka);
+                else
+                    hipLaunchKernelGGL((ka_ac_col<RC, 4>), dim3(h->nacols), dim3(64), lds, h->stream, h->dom_d,
+                                       h->acols, zv, in, a, rtp_s, rup_n, rwp_s, rwp_n, div, nsub, fac, h->max_ka);
+            } else {
+                hipLaunchKernelGGL(ka_ac_v1<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in,
+                                   a, (const RC *)rec_rwp<RC>(h, st, m + 1), div, nsub);
+                // one wave (one column group) per block: the waves are independent,
+                // and small grids then reach every CU
+                const dim3 gs(h->nwgroups), bs(64);
+                if (h->wch == 4)
+                    hipLaunchKernelGGL((ka_ac_solve<4, RC>), gs, bs, 0, h->stream, h->dom_d, h->wgroups, h->nwgroups,
+                                       zv, a, div, nsub, n == nsub ? 1 : 0);
+                else if (h->wch == 64)
+                    hipLaunchKernelGGL((ka_ac_solve<64, RC>), gs, bs, 0, h->stream, h->dom_d, h->wgroups,
+                                       h->nwgroups, zv, a, div, nsub, n == nsub ? 1 : 0);
+                else
+                    hipLaunchKernelGGL((ka_ac_solve<8, RC>), gs, bs, 0, h->stream, h->dom_d, h->wgroups, h->nwgroups,
+                                       zv, a, div, nsub, n == nsub ? 1 : 0);
+                hipLaunchKernelGGL(ka_ac_v2<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, a,
+                                   (const RC *)rec_rtp<RC>(h, st, m), (const RC *)rec_rup<RC>(h, st, m + 1),
+                                   (const RC *)rec_rwp<RC>(h, st, m), (const RC *)rec_rwp<RC>(h, st, m + 1), div,
+                                   nsub);
+            }
             hipLaunchKernelGGL(ka_ac_hor<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, a,
                                (const double *)Lrup[cur], Lrup[1 - cur], (const RC *)rec_rtp<RC>(h, st, m),
                                (const RC *)rec_rtp<RC>(h, st, m > 0 ? m - 1 : 0),
@@ -1028,15 +1069,28 @@ template <class RC> void enqueue_step_adj(moist_handle *h, const double *sn) {
                            dset(li, n2), ls0, lph, a, (const double *)Lrup[cur]);
         hipLaunchKernelGGL(ka_diag<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in,
                            dset(li, n2), a);
+        // the stage-3 input is what the next step's surface fluxes read in the
+        // halos: the adjoints of those reads (reversed just before) enter here
+        if (st == 2 && h->any_sf)
+            hipLaunchKernelGGL(ka_stale_add, dim3((h->nm + 63) / 64), dim3(64), 0, h->stream, h->dom_d, h->nm,
+                               dset(li, n2), (const double *)A.carry);
         std::swap(lo, li);
     }
     // lo: adjoint of the stage-1 input (A.lam2: the three stages alternate
     // the two buffers, starting from A.lam); the step input also enters as s0
     // of every stage and through physics.  The step input's adjoint goes to
     // A.lam, so every reverse step reads and writes the same buffers.
-    hipLaunchKernelGGL(ka_physics, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv,
-                       dset(const_cast<double *>(sn), n2), lph, dset(lo, n2), ls0, dset(A.lam, n2), A.gk,
-                       A.gk + n2, A.gk + 2 * n2);
+    const SSet<double> snv = dset(const_cast<double *>(sn), n2);
+    if (h->any_sf)  // before ka_physics, which gathers its per-column adjoints
+        hipLaunchKernelGGL(ka_surface, dim3(h->nctiles), dim3(COL_TILE), 0, h->stream, h->dom_d, h->ctiles, zv, snv,
+                           stale, lph, A.sfb, (int64_t)h->nxs, A.carry);
+    hipLaunchKernelGGL(ka_physics, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, snv, lph,
+                       dset(lo, n2), ls0, dset(A.lam, n2), A.gk, A.gk + n2, A.gk + 2 * n2,
+                       h->any_sf ? (const double *)A.sfb : (const double *)nullptr, (int64_t)h->nxs);
+    if (h->any_rd)  // after it: the column kernel adds into the step input's adjoint
+        hipLaunchKernelGGL(ka_radiation, dim3(h->nctiles), dim3(COL_TILE), 0, h->stream, h->dom_d, h->ctiles, zv,
+                           snv, (const double *)h->theta_ref, (const double *)A.lph + 2 * n2, dset(A.lam, n2),
+                           wk(h, W_Y), wk(h, W_CPR), wk(h, W_RMI), wk(h, W_ACO), wk(h, W_PEND));
 }
 
 // Checkpoints hold the state slots (S0_.. S0_+6, contiguous at the front of
@@ -1044,11 +1098,15 @@ template <class RC> void enqueue_step_adj(moist_handle *h, const double *sn) {
 void save_state(moist_handle *h, double *dst) {
     copy_bytes(h, dst, h->pool, 7 * h->n2 * h->elem);
     copy_set(h, dst + h->adj->cp_words, h->prec, h->nxs);
+    copy_set(h, dst + h->adj->cp_words + h->nxs, h->sfh, 2 * h->nm);
 }
 void load_state(moist_handle *h, const double *src) {
     copy_bytes(h, h->pool, src, 7 * h->n2 * h->elem);
     copy_set(h, h->prec, src + h->adj->cp_words, h->nxs);
+    copy_set(h, h->sfh, src + h->adj->cp_words + h->nxs, 2 * h->nm);
 }
+// The stale halo pair surface_fluxes! reads at step j of the segment.
+double *seg_sfh(moist_handle *h, int j) { return h->adj->sfh_seg + 2 * (size_t)h->nm * j; }
 // The current state as a full FP64 state (7 fields) at `dst`.
 void state64(moist_handle *h, double *dst) {
     if (h->precision == MOIST_MIXED)
@@ -1066,9 +1124,9 @@ void reverse_step(moist_handle *h, int j) {
     if (h->precision == MOIST_MIXED) {
         hipLaunchKernelGGL(ka_widen, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d, h->tiles2d,
                            zvec(h), fset((float *)seg_step(h, j), h->n2), dset(A.snb, h->n2));
-        enqueue_step_adj<float>(h, A.snb);
+        enqueue_step_adj<float>(h, A.snb, seg_sfh(h, j));
     } else {
-        enqueue_step_adj<double>(h, (const double *)seg_step(h, j));
+        enqueue_step_adj<double>(h, (const double *)seg_step(h, j), seg_sfh(h, j));
     }
 }
 // Recording pass of step j of the segment (the mixed form from the handle's
@@ -1088,6 +1146,7 @@ void record_step(moist_handle *h, int j, bool advance) {
 void seg_record(moist_handle *h, int j) {
     AdjState &A = *h->adj;
     copy_bytes(h, seg_step(h, j), h->pool, 7 * h->n2 * h->elem);
+    copy_set(h, seg_sfh(h, j), h->sfh, 2 * h->nm);
     A.cur = A.recs + A.rb_bytes * (size_t)j;
     record_step(h, j, true);
 }
@@ -1099,6 +1158,7 @@ void seg_reverse(moist_handle *h, int j) {
         A.cur = A.recs + A.rb_bytes * (size_t)j;
     } else {
         if (h->precision == MOIST_MIXED) copy_bytes(h, h->pool, seg_step(h, j), 7 * h->n2 * h->elem);
+        copy_set(h, h->sfh, seg_sfh(h, j), 2 * h->nm);
         A.cur = A.recs;
         record_step(h, j, false);
     }
@@ -1138,7 +1198,9 @@ template <class H> int adjoint_run_t(moist_handle *h, int nsteps, int ckpt, cons
     state64(h, A.SC);
     hipLaunchKernelGGL(ka_loss, dim3(h->nm), dim3(256), 0, h->stream, h->dom_d, dset(A.SC, n2),
                        (const double *)h->prec, (const double *)A.tgt, A.lam + RHO * n2, A.lam + RT * n2, A.J);
-    // reverse sweep over the segments
+    // reverse sweep over the segments; the last step's stale halo reads feed
+    // no later step
+    fill(h, A.carry, 2 * h->nm);
     const size_t state_bytes = 7 * n2 * h->elem;
     for (int sg = A.ncp - 1; sg >= 0; --sg) {
         const int n0 = sg * ckpt, n1 = std::min(nsteps, n0 + ckpt);
@@ -1150,12 +1212,18 @@ template <class H> int adjoint_run_t(moist_handle *h, int nsteps, int ckpt, cons
             // states of the segment, then per step (in reverse) its record and adjoint
             for (int n = n0; n < n1; ++n) {
                 copy_bytes(h, seg_step(h, n - n0), h->pool, state_bytes);
+                copy_set(h, seg_sfh(h, n - n0), h->sfh, 2 * h->nm);
                 if (n + 1 < n1 && run_steps(h, 1)) return -1;
             }
         }
         for (int n = n1 - 1; n >= n0; --n) seg_reverse(h, n - n0);
         HIPCHK(h, hipGetLastError());
     }
+    // the first step's stale halo reads are the initial state's halo columns,
+    // taken as periodic copies of its interior
+    if (h->any_sf)
+        hipLaunchKernelGGL(ka_stale_add, dim3((h->nm + 63) / 64), dim3(64), 0, h->stream, h->dom_d, h->nm,
+                           dset(A.lam, n2), (const double *)A.carry);
     hipLaunchKernelGGL(ka_reduce_k, dim3(h->nm), dim3(256), 0, h->stream, h->dom_d, (const double *)A.gk,
                        (const double *)(A.gk + n2), (const double *)(A.gk + 2 * n2), A.gK);
     HIPCHK(h, hipEventRecord(e2, h->stream));
@@ -1309,6 +1377,11 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
         wc = xcd_order(wc, make_int2(-1, 0));
         h->nwgroups = (int)wc.size();
     }
+    std::vector<int2> acol;  // one entry per column of every member
+    for (int m = 0; m < n_members; ++m)
+        for (int i = 0; i < h->dom_h[m].nx; ++i) acol.push_back(make_int2(m, i));
+    acol = xcd_order(acol, make_int2(-1, 0));
+    h->nacols = (int)acol.size();
     h->natiles = (int)ta.size();
     h->ntiles2d = (int)t2.size();
     h->nctiles = (int)tc.size();
@@ -1343,6 +1416,8 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
     ALLOC(hp->atiles, sizeof(int4) * ta.size());
     ALLOC(hp->stiles, sizeof(int4) * ts.size());
     ALLOC(hp->wgroups, sizeof(int2) * wc.size());
+    ALLOC(hp->acols, sizeof(int2) * acol.size());
+    HIPCHK(hp, hipMemcpy(hp->acols, acol.data(), sizeof(int2) * acol.size(), hipMemcpyHostToDevice));
     HIPCHK(hp, hipMemcpy(hp->wgroups, wc.data(), sizeof(int2) * wc.size(), hipMemcpyHostToDevice));
     HIPCHK(hp, hipMemcpy(hp->stiles, ts.data(), sizeof(int4) * ts.size(), hipMemcpyHostToDevice));
     HIPCHK(hp, hipMemcpy(hp->atiles, ta.data(), sizeof(int4) * ta.size(), hipMemcpyHostToDevice));
@@ -1554,6 +1629,10 @@ int moist_set_option(moist_handle *h, int option, int value) {
         }
         return 0;
     }
+    if (option == MOIST_OPT_ADJ_COLUMN_KERNEL) {
+        h->adj_col = value ? 1 : 0;
+        return 0;
+    }
     if (option == MOIST_OPT_ADJ_RECORDS) {
         if (value < 0 || value > 2) {
             h->err = "adjoint record mode must be 0, 1 or 2";
@@ -1690,10 +1769,6 @@ int moist_set_acoustic_shape(moist_handle *h, int levels, int threads) {
 int moist_adjoint_run(moist_handle *h, int nsteps, int ckpt, const void *const *theta_target, int elem_bytes,
                       double *J) {
     if (!h || !theta_target || nsteps < 1 || ckpt < 1) return -2;
-    if (h->any_sf || h->any_rd) {
-        h->err = "the adjoint covers do_sf = do_rd = false";
-        return -2;
-    }
     for (int m = 0; m < h->nm; ++m)
         if (!theta_target[m]) return -2;
     if (elem_bytes == 8) return adjoint_run_t<double>(h, nsteps, ckpt, theta_target, J);
@@ -1756,7 +1831,7 @@ void moist_destroy(moist_handle *h) {
     if (h->gexec) hipGraphExecDestroy(h->gexec);
     for (auto e : h->ev_pool) hipEventDestroy(e);
     adj_free(h);
-    void *ptrs[] = {h->dom_d, h->tiles2d, h->stiles, h->ctiles, h->wgroups, h->hist_fields, h->hist_part, h->hist_stats, h->diag_all, h->z, h->zf, h->dz, h->geo, h->theta_ref, h->rho_bar, h->rt_bar,
+    void *ptrs[] = {h->dom_d, h->tiles2d, h->stiles, h->ctiles, h->wgroups, h->acols, h->hist_fields, h->hist_part, h->hist_stats, h->diag_all, h->z, h->zf, h->dz, h->geo, h->theta_ref, h->rho_bar, h->rt_bar,
                     h->p_bar, h->L_bar, h->B_bar, h->pool, h->pool1, h->atiles, h->cin, h->prec, h->sfh, h->nan_key,
                     h->step_ctr, h->staging, h->diag_out};
     for (void *p : ptrs)

@@ -175,8 +175,9 @@ def theta_profile(z, theta0=300.0, N=0.01):
 
 
 def _workload_params(T, nx, nz, d, dt, ns, steps, **kw):
+    kw = {"do_sf": False, "do_rd": False, **kw}
     return make_params(T, Nx=nx, Nz=nz, Lx=nx * d, H=nz * d, dz0=d, z_fact=1.0, ns=ns, tout_int=dt,
-                       t_end=steps * dt, do_sf=False, do_rd=False, **kw)
+                       t_end=steps * dt, **kw)
 
 
 def bubble(T=np.float64, nx=128, nz=64, d=156.25, dt=1.0, ns=6, steps=600, amp=2.0, radius=2000.0,

@@ -1132,34 +1132,35 @@ ORC_EXPORT int orc_slow_tendencies_f64(const moist_params *mp, const double *con
 }
 
 ORC_EXPORT void orc_physics_fwd_f64(const moist_params *mp, const double *const *s, double *const *ph,
-                                    double *dprec) {
+                                    double *dprec, const double *theta_ref) {
     orc::G<double> g(mp);
     orc::St<double> sv;
     for (int f = 0; f < 7; ++f) sv.f[f] = const_cast<double *>(s[f]);
     std::vector<double> v[6], dp;
-    orc::physics_fwd(g, sv, v, dp);
+    orc::physics_fwd(g, sv, theta_ref, v, dp);
     for (int f = 0; f < 6; ++f) std::copy(v[f].begin(), v[f].end(), ph[f]);
     std::copy(dp.begin(), dp.end(), dprec);
 }
 
 ORC_EXPORT void orc_physics_adj_f64(const moist_params *mp, const double *const *s, const double *const *lph,
-                                    const double *ldprec, double *const *ls, double *lK) {
+                                    const double *ldprec, double *const *ls, double *lK, const double *theta_ref,
+                                    double *stale) {
     orc::G<double> g(mp);
     orc::St<double> sv;
     for (int f = 0; f < 7; ++f) sv.f[f] = const_cast<double *>(s[f]);
-    orc::physics_adj(g, sv, lph, ldprec, ls, lK);
+    orc::physics_adj(g, sv, theta_ref, lph, ldprec, ls, lK, stale);
 }
 
+// s: the step input with its halo columns as the forward step sees them
+// (with surface fluxes on, two of them are read: they hold the previous
+// step's stage-3 input).  stale[2]: in/out, see rk3_step_adj.
 ORC_EXPORT int orc_rk3_step_adj_f64(const moist_params *mp, const double *const *s, double *const *lam,
-                                    double ldprec, double *lK) {
+                                    double ldprec, double *lK, const double *theta_ref, double *stale) {
     orc::G<double> g(mp);
     orc::StBuf a;
-    for (int f = 0; f < 7; ++f) {
-        a.f[f].assign(s[f], s[f] + g.n2());
-        orc::exchange_halo(a.f[f].data(), g);
-    }
+    for (int f = 0; f < 7; ++f) a.f[f].assign(s[f], s[f] + g.n2());
     orc::St<double> av = a.view();
-    return orc::rk3_step_adj(g, av, lam, ldprec, lK);
+    return orc::rk3_step_adj(g, av, theta_ref, lam, ldprec, lK, stale);
 }
 
 ORC_EXPORT void orc_output_history_f64(const moist_params *mp, double *const *fields, double *const *out,

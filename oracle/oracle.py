@@ -61,9 +61,9 @@ def lib():
         V = ctypes.c_void_p
         for name, res, args in (
             ("compute_step_adj", ctypes.c_int, [V, V, V, V, ctypes.c_double, ctypes.c_int, V, V, V, V]),
-            ("physics_fwd", None, [V, V, V, V]),
-            ("physics_adj", None, [V, V, V, V, V, V]),
-            ("rk3_step_adj", ctypes.c_int, [V, V, V, ctypes.c_double, V]),
+            ("physics_fwd", None, [V, V, V, V, V]),
+            ("physics_adj", None, [V, V, V, V, V, V, V, V]),
+            ("rk3_step_adj", ctypes.c_int, [V, V, V, ctypes.c_double, V, V, V]),
             ("adjoint_run", ctypes.c_int, [V, V, ctypes.c_int, ctypes.c_int, V, V, V, V]),
             ("output_history", None, [V, V, V, V]),
             ("slow_tendencies", ctypes.c_int, [V, V, V, V]),
@@ -247,37 +247,52 @@ def slow_tendencies(phys, s, p):
     return dict(zip(("R_rho", "R_rho_u", "R_rho_w", "R_rho_theta", "u_face", "w_face"), out))
 
 
+def _vp(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
 def physics_fwd(s, p):
-    """The six combined tendencies of rk3_step! (do_sf = do_rd = false) and d_prec."""
+    """The six combined tendencies of rk3_step! and d_prec (surface fluxes read
+    s's halo columns; radiation relaxes to s.theta_ref)."""
     shape = s.rho.shape
     ph = [np.zeros(shape, order="F") for _ in range(6)]
     dprec = np.zeros(shape[1])
     cp = p.to_c()
-    lib().orc_physics_fwd_f64(ctypes.byref(cp), _f64([getattr(s, n) for n in _PROG]), _f64(ph),
-                              dprec.ctypes.data_as(ctypes.c_void_p))
+    tr = np.ascontiguousarray(s.theta_ref, dtype=np.float64)
+    lib().orc_physics_fwd_f64(ctypes.byref(cp), _f64([getattr(s, n) for n in _PROG]), _f64(ph), _vp(dprec), _vp(tr))
     return ph, dprec
 
 
 def physics_adj(s, p, lam_phys, lam_dprec):
-    """Adjoint of physics_fwd: returns (lam_s[7], lam_K[3])."""
+    """Adjoint of physics_fwd: returns (lam_s[7], lam_K[3], lam_stale[2]); lam_stale
+    holds the adjoints of the two halo values the surface fluxes read (rho_u
+    left of the first column, rho right of the last)."""
     shape = s.rho.shape
     ls = [np.zeros(shape, order="F") for _ in range(7)]
     lK = np.zeros(3)
+    st = np.zeros(2)
     cp = p.to_c()
     lam_dprec = np.ascontiguousarray(lam_dprec, dtype=np.float64)
+    tr = np.ascontiguousarray(s.theta_ref, dtype=np.float64)
     lib().orc_physics_adj_f64(ctypes.byref(cp), _f64([getattr(s, n) for n in _PROG]), _f64(lam_phys),
-                              lam_dprec.ctypes.data_as(ctypes.c_void_p), _f64(ls),
-                              lK.ctypes.data_as(ctypes.c_void_p))
-    return ls, lK
+                              _vp(lam_dprec), _f64(ls), _vp(lK), _vp(tr), _vp(st))
+    return ls, lK, st
 
 
-def rk3_step_adj(s, p, lam, lam_dprec=0.0):
-    """Adjoint of one rk3_step! at s: `lam` (7 fields, adjoint of the step output)
-    is replaced by the adjoint of the step input; returns lam_K[3]."""
+def rk3_step_adj(s, p, lam, lam_dprec=0.0, stale=None):
+    """Adjoint of one rk3_step! at s (its halo columns as the forward step saw
+    them): `lam` (7 fields, adjoint of the step output) is replaced by the
+    adjoint of the step input; returns lam_K[3].  `stale` (2 values, in/out,
+    surface fluxes on): on entry the adjoints of the next step's two halo
+    reads, which are this step's stage-3 input at the ground; on exit those of
+    this step's own halo reads."""
     lK = np.zeros(3)
     cp = p.to_c()
+    tr = np.ascontiguousarray(s.theta_ref, dtype=np.float64)
+    st = np.zeros(2) if stale is None else stale
+    assert st.dtype == np.float64 and st.flags["C_CONTIGUOUS"] and st.size == 2
     rc = lib().orc_rk3_step_adj_f64(ctypes.byref(cp), _f64([getattr(s, n) for n in _PROG]), _f64(lam),
-                                    float(lam_dprec), lK.ctypes.data_as(ctypes.c_void_p))
+                                    float(lam_dprec), _vp(lK), _vp(tr), _vp(st))
     if rc:
         raise RuntimeError(f"rk3_step_adj failed ({rc})")
     return lK

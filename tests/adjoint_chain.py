@@ -26,6 +26,11 @@ def chain_adjoint(p, states, tt):
     lam[0][...] = -d * th / xN.rho
     lam[3][...] = d / xN.rho
     gK = np.zeros(3)
+    stale = np.zeros(2)  # adjoints of the surface fluxes' halo reads, carried one step back
     for x in reversed(states[:-1]):
-        gK += O.rk3_step_adj(x.copy(), p, lam, lam_dprec=p.dx * p.dt)  # prec += d_prec * dt
+        gK += O.rk3_step_adj(x.copy(), p, lam, lam_dprec=p.dx * p.dt, stale=stale)  # prec += d_prec * dt
+    if p.do_sf:
+        # the first step's halo reads are periodic copies of the initial state
+        lam[1][0, p.ie - 1] += stale[0]
+        lam[0][0, p.is_ - 1] += stale[1]
     return J, lam, gK

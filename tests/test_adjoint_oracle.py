@@ -97,7 +97,7 @@ def test_physics_adjoint_matches_fd():
     _wrap(p, s.rho_qv)
     Wp = [np.asfortranarray(rng.normal(size=s.rho.shape)) for _ in range(6)]
     wd = rng.normal(size=p.Nx + 2 * p.halo)
-    ls, lK = O.physics_adj(s, p, Wp, wd)
+    ls, lK, _ = O.physics_adj(s, p, Wp, wd)
     eps = 1e-4
 
     def jd(a, b):
@@ -263,3 +263,130 @@ def test_chain_of_step_adjoints_is_the_checkpointed_gradient():
     for f in range(7):
         np.testing.assert_allclose(g[f][:, h:h + p.Nx], go[f][:, h:h + p.Nx], rtol=1e-12, atol=1e-14 * np.abs(go[f]).max())
     np.testing.assert_allclose(gK, gKo, rtol=1e-12)
+
+
+# ---- surface fluxes and gray radiation (on by default in the reference) -----
+def _state_sf_rd(seed=7):
+    """A moist 16x12 state with wind, cloud and rain, surface fluxes and gray
+    radiation on (the reference's default switches), advanced 3 steps so the
+    halo columns hold what a step leaves there."""
+    rng = np.random.default_rng(seed)
+    p, s = ic.bubble(np.float64, nx=16, nz=12, d=200.0, do_sf=True, do_rd=True, temp_surf=302.0)
+    s.theta_ref[:] = (s.rho_theta / s.rho)[:, p.halo]
+    s.rho_u[...] = 6.0 * s.rho
+    s.rho_qc[...] += 1.5e-3 * s.rho * (1 + 0.2 * rng.random(s.rho.shape))
+    s.rho_qr[...] += 1e-4 * s.rho * (1 + 0.2 * rng.random(s.rho.shape))
+    for n in PROG:
+        _wrap(p, getattr(s, n))
+    assert O.rk3_step(s, p, 3) == 0
+    return p, s, rng
+
+
+def _halo_dot(p, stale, v_ru, v_rho):
+    """Contribution of a direction's two halo values the surface fluxes read."""
+    return stale[0] * v_ru[0, p.is_ - 2] + stale[1] * v_rho[0, p.ie]
+
+
+def test_physics_adjoint_with_surface_fluxes_and_radiation_matches_fd():
+    p, s, rng = _state_sf_rd()
+    Wp = [np.asfortranarray(rng.normal(size=s.rho.shape)) for _ in range(6)]
+    wd = rng.normal(size=p.Nx + 2 * p.halo)
+    ls, lK, stale = O.physics_adj(s, p, Wp, wd)
+    assert np.any(stale != 0)
+    eps = 1e-4
+
+    def jd(a, b):
+        (pa, da), (pb, db) = a, b
+        h = p.halo
+        return (sum(float(np.sum(_inner(p, Wp[f] * (pa[f] - pb[f])))) for f in range(6))
+                + float(np.sum((wd * (da - db))[h:h + p.Nx])))
+
+    for f, n in enumerate(PROG):
+        if n == "rho_w":
+            assert not np.any(ls[f]), n
+            continue
+        sc = 1e-4 if n in ("rho_qc", "rho_qr") else SCALES[n]
+        v = _dir(p, rng, s.rho.shape) * sc  # halo columns wrapped: the stale reads move too
+        sp, sm = s.copy(), s.copy()
+        getattr(sp, n)[...] += eps * v
+        getattr(sm, n)[...] -= eps * v
+        fd = jd(O.physics_fwd(sp, p), O.physics_fwd(sm, p)) / (2 * eps)
+        ad = float(np.sum(_inner(p, ls[f] * v)))
+        if n == "rho_u":
+            ad += _halo_dot(p, stale, v, np.zeros_like(v))
+        if n == "rho":
+            ad += _halo_dot(p, stale, np.zeros_like(v), v)
+        assert abs(fd - ad) <= 5e-5 * abs(fd), (n, fd, ad)
+
+
+def test_rk3_step_adjoint_with_surface_fluxes_and_radiation_matches_fd():
+    p, s, rng = _state_sf_rd(8)
+    W = [np.asfortranarray(rng.normal(size=s.rho.shape)) for _ in range(7)]
+    for w in W:
+        w[:, :p.halo] = 0
+        w[:, p.halo + p.Nx:] = 0
+    lam = [w.copy(order="F") for w in W]
+    stale = np.zeros(2)
+    O.rk3_step_adj(s, p, lam, stale=stale)
+    eps = 1e-4
+
+    def step(s_):
+        a = s_.copy()
+        assert O.rk3_step(a, p, 1) == 0
+        return a
+
+    for f, n in enumerate(PROG):
+        v = _dir(p, rng, s.rho.shape) * SCALES[n]
+        sp, sm = s.copy(), s.copy()
+        getattr(sp, n)[...] += eps * v
+        getattr(sm, n)[...] -= eps * v
+        a, b = step(sp), step(sm)
+        fd = sum(float(np.sum(W[g] * (getattr(a, m) - getattr(b, m)))) for g, m in enumerate(PROG)) / (2 * eps)
+        ad = float(np.sum(_inner(p, lam[f] * v)))
+        if n == "rho_u":
+            ad += _halo_dot(p, stale, v, np.zeros_like(v))
+        if n == "rho":
+            ad += _halo_dot(p, stale, np.zeros_like(v), v)
+        assert abs(fd - ad) <= 5e-5 * abs(fd), (n, fd, ad)
+
+
+def test_loss_gradient_with_surface_fluxes_and_radiation():
+    """The checkpointed gradient with both switches on: central FD (h = 1e-6
+    relative, 8 smooth directions of theta' and qv, and of rho_u, which the
+    surface drag and the stale halo reads see) against adjoint_run; the chain
+    of one-step adjoints (with the halo adjoints carried one step back) equals
+    it; checkpoint intervals do not change it."""
+    from adjoint_chain import chain_adjoint
+
+    p, s, rng = _state_sf_rd(9)
+    tt = np.asfortranarray(s.rho_theta / s.rho + rng.normal(0, 0.5, size=s.rho.shape))
+    nsteps = 8
+    J, grad, gK = O.adjoint_run(s.copy(), p, nsteps, tt, ckpt=3)
+    assert abs(J - _loss(p, s, tt, nsteps)) <= 1e-12 * abs(J)
+    eps = 1e-6
+    errs = []
+    for _ in range(3):
+        for f, n, sc in ((3, "rho_theta", 0.01), (4, "rho_qv", 1e-5), (1, "rho_u", 0.05)):
+            v = _smooth_dir(p, rng, s.rho.shape) * sc
+            sp, sm = s.copy(), s.copy()
+            getattr(sp, n)[...] += eps * v * s.rho
+            getattr(sm, n)[...] -= eps * v * s.rho
+            fd = (_loss(p, sp, tt, nsteps) - _loss(p, sm, tt, nsteps)) / (2 * eps)
+            ad = float(np.sum(_inner(p, grad[f] * s.rho * v)))
+            errs.append(abs(fd - ad) / abs(ad))
+    assert np.median(errs) <= 1e-5 and max(errs) <= 1e-4, errs
+    b = O.adjoint_run(s.copy(), p, nsteps, tt, ckpt=nsteps)
+    assert b[0] == J
+    for f in range(7):
+        np.testing.assert_array_equal(b[1][f], grad[f])
+    states = [s.copy()]
+    for _ in range(nsteps):
+        x = states[-1].copy()
+        O.rk3_step(x, p, 1)
+        states.append(x)
+    Jc, gc, gKc = chain_adjoint(p, states, tt)
+    assert abs(Jc - J) <= 1e-12 * abs(J)
+    h = p.halo
+    for f in range(7):
+        np.testing.assert_allclose(gc[f][:, h:h + p.Nx], grad[f][:, h:h + p.Nx], rtol=1e-12,
+                                   atol=1e-14 * np.abs(grad[f]).max())

@@ -259,3 +259,76 @@ def test_adjoint_edge_cases(kw):
     adjoint), unequal stage substeps, one substep, no divergence damping."""
     p, s, tt = _case(**kw)
     _compare(f"edge {kw}", p, s, tt, 4, 2)
+
+
+@pytest.mark.parametrize("precision", [core.FP64, core.MIXED])
+def test_adjoint_column_kernel_matches_three_kernels(precision):
+    """The vertical part of each reverse acoustic substep as one kernel per
+    column (the default) against the three-kernel form: the same arithmetic
+    in the same order (one column per wave in both), so the gradients are
+    bitwise equal."""
+    p, s, tt = _case()
+    out = []
+    for col in (True, False):
+        eng = core.Engine(p, precision)
+        eng.adjoint_column_kernel(col)
+        eng.upload(s)
+        J = eng.adjoint(7, [tt], ckpt=3)[0]
+        g, gK = eng.adjoint_grad(0)
+        eng.close()
+        out.append((J, g, gK))
+    assert out[0][0] == out[1][0]
+    for f in range(7):
+        np.testing.assert_array_equal(out[0][1][f], out[1][1][f], err_msg=PROG7[f])
+    np.testing.assert_array_equal(out[0][2], out[1][2])
+
+
+def _case_sf_rd(seed=7):
+    """The reference's default switches: surface fluxes and gray radiation on,
+    with wind, cloud and rain, spun up 3 oracle steps so the halo columns
+    (which the surface fluxes read at the domain's ends) hold what a step
+    leaves there."""
+    rng = np.random.default_rng(seed)
+    p, s = ic.bubble(np.float64, nx=16, nz=12, d=200.0, do_sf=True, do_rd=True, temp_surf=302.0)
+    s.theta_ref[:] = (s.rho_theta / s.rho)[:, p.halo]
+    s.rho_u[...] = 6.0 * s.rho
+    s.rho_qc[...] += 1.5e-3 * s.rho * (1 + 0.2 * rng.random(s.rho.shape))
+    s.rho_qr[...] += 1e-4 * s.rho * (1 + 0.2 * rng.random(s.rho.shape))
+    for n in PROG7:
+        _wrap(p, getattr(s, n))
+    assert O.rk3_step(s, p, 3) == 0
+    tt = np.asfortranarray(s.rho_theta / s.rho + rng.normal(0, 0.5, size=s.rho.shape))
+    return p, s, tt
+
+
+@pytest.mark.parametrize("records", ["segment", "step"])
+def test_adjoint_with_surface_fluxes_and_radiation(records):
+    """FP64 form against the oracle's adjoint_run with both switches on,
+    including the adjoints of the surface fluxes' stale halo reads carried
+    one step back."""
+    p, s, tt = _case_sf_rd()
+    _compare(f"sf+rd 9 steps ckpt 4, {records} records", p, s, tt, 9, 4, records=records)
+
+
+def test_mixed_adjoint_with_surface_fluxes_and_radiation():
+    """Mixed form with both switches on against the one-step adjoint chain
+    along its own trajectory (bars as for the other mixed-form chain tests)."""
+    p, s, tt = _case_sf_rd(8)
+    nsteps, ckpt = 5, 2
+    traj = _trajectory(p, s, nsteps, core.MIXED)
+    h = p.halo
+    # the first step's halo reads are the uploaded State's halo columns
+    for n in ("rho", "rho_u"):
+        getattr(traj[0], n)[:, :h] = getattr(s, n)[:, :h]
+        getattr(traj[0], n)[:, h + p.Nx:] = getattr(s, n)[:, h + p.Nx:]
+    Jc, gc, gKc = chain_adjoint(p, traj, tt)
+    J, g, gK = _device_adjoint(p, s, tt, nsteps, ckpt, core.MIXED)
+    errs = {n: _rel(g[f][:, h:h + p.Nx], gc[f][:, h:h + p.Nx]) for f, n in enumerate(PROG7)}
+    tot = np.sqrt(sum(np.linalg.norm(g[f][:, h:h + p.Nx] - gc[f][:, h:h + p.Nx]) ** 2 for f in range(7)) /
+                  sum(np.linalg.norm(gc[f][:, h:h + p.Nx]) ** 2 for f in range(7)))
+    print(f"mixed sf+rd: J {J:.12e} vs chain {Jc:.12e}; all fields {tot:.1e}; rel-L2 "
+          + " ".join(f"{k}={v:.1e}" for k, v in errs.items()), flush=True)
+    assert abs(J - Jc) <= 1e-12 * abs(Jc), (J, Jc)
+    assert tot <= 1e-3, tot
+    for n, e in errs.items():
+        assert e <= 5e-3, (n, e)
