@@ -957,11 +957,13 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
 // [l Ls, l Ls + Ls) (Lc = ceil(nz / 64), Ls = ceil((nz - 1) / 64) <= 4).  The
 // column's inputs of the three parts are staged in LDS once; the solve's
 // right-hand side and its solution stay in LDS instead of a global round
-// trip, and two launches per substep are saved.  The arithmetic of every
+// trip, and two launches per substep are saved.  The per-level geometry (dz
+// and the four grid factors) is staged in LDS with the column, so the loops
+// over neighbouring faces wait on no global load.  The arithmetic of every
 // value is that of ka_ac_v1, ka_ac_solve (one column per wave, CH = 64) and
 // ka_ac_v2, in the same order.  cols: {member, column} per block (one wave).
 // Dynamic LDS: AC_COL_ARRAYS doubles per level of the tallest member.
-constexpr int AC_COL_ARRAYS = 11;
+constexpr int AC_COL_ARRAYS = 16;
 constexpr int AC_COL_MAX_NZ = 256;  // Lc, Ls <= 4: the solve's rows stay in registers
 // LCM: levels a lane may own (1, 2 or 4 for members up to 64, 128, 256 levels)
 template <class RC, int LCM>
@@ -979,8 +981,12 @@ __global__ void __launch_bounds__(64)
     double *sRWN = lcol, *sRWS = lcol + max_ka, *sRW = lcol + 2 * max_ka, *sTH = lcol + 3 * max_ka,
            *sRT = lcol + 4 * max_ka, *sLRTP = lcol + 5 * max_ka, *sLRP = lcol + 6 * max_ka,
            *sLMZ = lcol + 7 * max_ka, *sTP = lcol + 8 * max_ka, *sRHS = lcol + 9 * max_ka, *sY = lcol + 10 * max_ka;
+    double *sGX = lcol + 11 * max_ka, *sGY = lcol + 12 * max_ka, *sGZ = lcol + 13 * max_ka, *sGW = lcol + 14 * max_ka,
+           *sDZ = lcol + 15 * max_ka;
     const double *dzd = zv.dz + D.offz;
     const double4 *gz = zv.geo + D.offz;
+    // the staged geometry of level kk
+    auto gzl = [&](int kk) { return make_double4(sGX[kk], sGY[kk], sGZ[kk], sGW[kk]); };
     const double EPS = 1e-12, rns = 1.0 / double(nsub);
     const double GRAV = K<double>::GRAV, rdx = 1.0 / D.dx;
     const double dt_s = D.dt / double(div) / double(nsub);
@@ -1008,6 +1014,12 @@ __global__ void __launch_bounds__(64)
             sLRTP[k] = a.Lrtp[q];
             sLRP[k] = a.Lrp[q];
             sLMZ[k] = a.Lmz[q];
+            const double4 g = gz[k];
+            sGX[k] = g.x;
+            sGY[k] = g.y;
+            sGZ[k] = g.z;
+            sGW[k] = g.w;
+            sDZ[k] = dzd[k];
             // off-centred rho_theta' of the forward explicit part (ka_ac_v2's rtp_tp)
             const double fip = (TH[Gx(k, i + 1)] + TH[q]) * 0.5 * rup_n[q];
             const double fim = (TH[q] + TH[Gx(k, i - 1)]) * 0.5 * rup_n[Gx(k, i - 1)];
@@ -1045,36 +1057,36 @@ __global__ void __launch_bounds__(64)
         double lth = 0.0;
         for (int f = k - 2; f <= k + 1; ++f) {
             if (f < 0 || f > ke - 1) continue;
-            const double rden = gz[f].z;
-            const double w0 = dzd[f + 1] * rden, w1 = dzd[f] * rden;
+            const double rden = gzl(f).z;
+            const double w0 = sDZ[f + 1] * rden, w1 = sDZ[f] * rden;
             const double w = sRWN[f];
             bool pos, neg;
             brp(f, pos, neg);
-            lth += -dt_s * gz[f].x * sLRTP[f] * w * upw_coef(pos, neg, k - f, w0, w1);
+            lth += -dt_s * gzl(f).x * sLRTP[f] * w * upw_coef(pos, neg, k - f, w0, w1);
             brm(f, pos, neg);
-            lth += dt_s * gz[f + 1].x * sLRTP[f + 1] * w * upw_coef(pos, neg, k - f, w0, w1);
+            lth += dt_s * gzl(f + 1).x * sLRTP[f + 1] * w * upw_coef(pos, neg, k - f, w0, w1);
         }
         lth1[u] = lth0[u] + lth;
         if (k <= ke - 1) {
-            const double rden = gz[k].z;
-            const double w0 = dzd[k + 1] * rden, w1 = dzd[k] * rden;
+            const double rden = gzl(k).z;
+            const double w0 = sDZ[k + 1] * rden, w1 = sDZ[k] * rden;
             bool pos, neg;
             brp(k, pos, neg);
             const double gp = upw_val(pos, neg, THv(k - 1), THv(k), THv(k + 1), THv(k + 2), w0, w1);
             brm(k, pos, neg);
             const double gm = upw_val(pos, neg, THv(k - 1), THv(k), THv(k + 1), THv(k + 2), w0, w1);
-            const double lrwp = sLMZ[k] * rns - dt_s * gz[k].x * sLRTP[k] * gp + dt_s * gz[k + 1].x * sLRTP[k + 1] * gm -
-                                dt_s * gz[k].x * sLRP[k] + dt_s * gz[k + 1].x * sLRP[k + 1];
+            const double lrwp = sLMZ[k] * rns - dt_s * gzl(k).x * sLRTP[k] * gp + dt_s * gzl(k + 1).x * sLRTP[k + 1] * gm -
+                                dt_s * gzl(k).x * sLRP[k] + dt_s * gzl(k + 1).x * sLRP[k + 1];
             sRHS[k] = lrwp0[u] + lrwp;
         }
     }
     __syncthreads();
 
     // ---- (2) transposed solve A^T y = rhs (ka_ac_solve, one column per wave)
-    auto th = [&](int kk) { return (sTH[kk] * dzd[kk + 1] + sTH[kk + 1] * dzd[kk]) * gz[kk].z; };
-    auto Wa = [&](int kk) { return -dt_tp2 * (sRT[kk] * th(kk - 1) * (gz[kk].y * gz[kk].x) - gz[kk].w); };
-    auto Wb = [&](int kk) { return dt_tp2 * (sRT[kk] * gz[kk].x + sRT[kk + 1] * gz[kk + 1].x) * th(kk) * gz[kk].y; };
-    auto Wc = [&](int kk) { return -dt_tp2 * (sRT[kk + 1] * th(kk + 1) * (gz[kk].y * gz[kk + 1].x) + gz[kk].w); };
+    auto th = [&](int kk) { return (sTH[kk] * sDZ[kk + 1] + sTH[kk + 1] * sDZ[kk]) * gzl(kk).z; };
+    auto Wa = [&](int kk) { return -dt_tp2 * (sRT[kk] * th(kk - 1) * (gzl(kk).y * gzl(kk).x) - gzl(kk).w); };
+    auto Wb = [&](int kk) { return dt_tp2 * (sRT[kk] * gzl(kk).x + sRT[kk + 1] * gzl(kk + 1).x) * th(kk) * gzl(kk).y; };
+    auto Wc = [&](int kk) { return -dt_tp2 * (sRT[kk + 1] * th(kk + 1) * (gzl(kk).y * gzl(kk + 1).x) + gzl(kk).w); };
     {
         constexpr int CH = 64, LR = 4;
         const int ch = lane;
@@ -1202,9 +1214,9 @@ __global__ void __launch_bounds__(64)
     auto Lc_ = [&](int j) { return j >= 0 && j < n - 1 ? -yv(j) * xv(j + 1) - fact * yv(j) * sRWS[j + 1] : 0.0; };
     auto Lthc = [&](int j) {
         if (j < 0 || j > n - 1) return 0.0;
-        double v = Lb(j) * dt_tp2 * (sRT[j] * gz[j].x + sRT[j + 1] * gz[j + 1].x) * gz[j].y;
-        if (j + 1 <= n - 1) v += La(j + 1) * (-dt_tp2 * sRT[j + 1] * (gz[j + 1].y * gz[j + 1].x));
-        if (j >= 1) v += Lc_(j - 1) * (-dt_tp2 * sRT[j] * (gz[j - 1].y * gz[j].x));
+        double v = Lb(j) * dt_tp2 * (sRT[j] * gzl(j).x + sRT[j + 1] * gzl(j + 1).x) * gzl(j).y;
+        if (j + 1 <= n - 1) v += La(j + 1) * (-dt_tp2 * sRT[j + 1] * (gzl(j + 1).y * gzl(j + 1).x));
+        if (j >= 1) v += Lc_(j - 1) * (-dt_tp2 * sRT[j] * (gzl(j - 1).y * gzl(j).x));
         return v;
     };
 #pragma unroll
@@ -1216,7 +1228,7 @@ __global__ void __launch_bounds__(64)
         double lrt = 0.0, lth = 0.0, Lrp_tp = 0.0, Lrtp_tp = 0.0, lws = 0.0, lRw = 0.0;
         if (k <= n - 1) {
             const double y = yv(k);
-            const double thk = th(k), rz1 = gz[k].y, rzd = rz1 * gz[k].x;
+            const double thk = th(k), rz1 = gzl(k).y, rzd = rz1 * gzl(k).x;
             lrt += Lb(k) * dt_tp2 * thk * rzd;
             if (k >= 1) lrt += La(k) * (-dt_tp2 * th(k - 1) * rzd);
             const double buoy = (sTP[k + 1] - sTP[k]) * rz1;
@@ -1226,14 +1238,14 @@ __global__ void __launch_bounds__(64)
             Lrtp_tp += dt_s * y * rt2h * rz1;
             Lrp_tp += -dt_s * y * GRAV * 0.5;
             lRw = dt_s * y;
-            lth += Lthc(k) * dzd[k + 1] * gz[k].z;
+            lth += Lthc(k) * sDZ[k + 1] * gzl(k).z;
             lws = y * (1.0 - fact * Wb(k));
             if (k >= 1) lws += -fact * Wa(k) * y - fact * Wc(k - 1) * yv(k - 1);
         }
         if (k >= 1) {
             const int kb = k - 1;
             const double yb = yv(kb);
-            const double rz1 = gz[kb].y, rzd = rz1 * gz[k].x;
+            const double rz1 = gzl(kb).y, rzd = rz1 * gzl(k).x;
             lrt += Lb(kb) * dt_tp2 * th(kb) * rzd;
             if (kb < n - 1) lrt += Lc_(kb) * (-dt_tp2 * th(k) * rzd);
             const double buoy = (sTP[k] - sTP[kb]) * rz1;
@@ -1241,7 +1253,7 @@ __global__ void __launch_bounds__(64)
             const double rt2h = (RC(sRT[kb]) + RC(sRT[k])) * 0.5;
             Lrtp_tp += -dt_s * yb * rt2h * rz1;
             Lrp_tp += -dt_s * yb * GRAV * 0.5;
-            lth += Lthc(kb) * dzd[kb] * gz[kb].z;
+            lth += Lthc(kb) * sDZ[kb] * gzl(kb).z;
         }
         const double lir = lr + fact_tp * Lrp_tp, lit = lt + fact_tp * Lrtp_tp;
         if (k <= n - 1) {

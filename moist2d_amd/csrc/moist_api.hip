@@ -396,7 +396,7 @@ void enqueue_stage(moist_handle *h, int in_set, int out_set, int s0_set, int div
     }
     hipLaunchKernelGGL((k_scalar<S, PL>), dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles,
                        zv, in, s0, out, w, (const S *)rup[ru], (const S *)rtp[cur], div, dt_ovr, stage,
-                       diag_out ? w.theta : nullptr, w.pres, w.rt2);
+                       diag_out ? w.theta : nullptr, w.pres, w.rt2, SSet<double>{});
 }
 
 // Surface fluxes and gray radiation of the step-start state s (flag-gated).
@@ -417,7 +417,7 @@ template <class S> void enqueue_step(moist_handle *h) {
     // and stages 2 and 3 read those the previous stage's k_scalar wrote
     constexpr bool PL = is_mixed<S>();
     hipLaunchKernelGGL(k_physics<S>, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d, h->tiles2d,
-                       zv, s, w, h->store_diag, PL ? 1 : 0);
+                       zv, s, w, h->store_diag, PL ? 1 : 0, SSet<double>{});
     enqueue_stage<S>(h, S0_, SA_, S0_, 3, 0.0, h->ns_stage[0], 0, 0, !PL, true, true);
     enqueue_stage<S>(h, SA_, SB_, S0_, 2, 0.0, h->ns_stage[1], 1, 0, false, true);
     enqueue_stage<S>(h, SB_, S0_, S0_, 1, 0.0, h->ns_stage[2], 2, 0, false, false);
@@ -724,11 +724,9 @@ struct AdjState {
     size_t cp_stride = 0, cp_words = 0;
     double *cp = nullptr, *SC = nullptr;
     // step inputs of one segment in the handle's form (7 fields per step,
-    // seg_bytes apart); the mixed form widens one of them to FP64 into `snb`
-    // when the step is reversed
+    // seg_bytes apart)
     char *seg = nullptr;
     size_t seg_bytes = 0;
-    double *snb = nullptr;
     // forward records: blocks of {stage-2 input, stage-3 input (FP64), stage
     // records, substep records (in the handle's form, rec_elem bytes)}; one
     // block per step of a segment when they fit in memory (`segrec`), else one
@@ -760,7 +758,10 @@ void adj_free(moist_handle *h) {
     h->adj = nullptr;
 }
 
-size_t rec_in_bytes(const moist_handle *h) { return 14 * (size_t)h->n2 * sizeof(double); }
+// FP64 inputs kept in a record block: those of stages 2 and 3, and in the
+// mixed form also the step input (widened)
+int rec_nin(const moist_handle *h) { return h->precision == MOIST_MIXED ? 3 : 2; }
+size_t rec_in_bytes(const moist_handle *h) { return 7 * (size_t)rec_nin(h) * h->n2 * sizeof(double); }
 
 int adj_alloc(moist_handle *h, int nsteps, int ckpt) {
     if (h->adj && h->adj->nsteps == nsteps && h->adj->ckpt == ckpt && h->adj->mode == h->adj_mode) return 0;
@@ -772,7 +773,6 @@ int adj_alloc(moist_handle *h, int nsteps, int ckpt) {
     A.mode = h->adj_mode;
     A.ncp = (nsteps + ckpt - 1) / ckpt;
     const size_t n2 = h->n2;
-    const bool mixed = h->precision == MOIST_MIXED;
     A.cp_words = (7 * n2 * h->elem + 7) / 8;
     A.cp_stride = A.cp_words + h->nxs + 2 * (size_t)h->nm;  // state, prec, the stale halo pair
     A.rec_elem = (int)h->elem;
@@ -807,7 +807,6 @@ int adj_alloc(moist_handle *h, int nsteps, int ckpt) {
         al(A.lim, 6 * h->nxs) || al(A.gk, 3 * n2) || al(A.tgt, n2) || al(A.fin, A.cp_stride) || al(A.J, h->nm) ||
         al(A.gK, 3 * h->nm))
         return -1;
-    if (mixed && al(A.snb, 7 * n2)) return -1;
     if (al(A.sfh_seg, 2 * (size_t)h->nm * ckpt) || al(A.carry, 2 * (size_t)h->nm) ||
         al(A.sfb, SF_SLOTS * (size_t)h->nxs))
         return -1;
@@ -830,12 +829,14 @@ SSet<float> fset(float *base, size_t n2) {
     return s;
 }
 
-// Layout of a record block (current block h->adj->cur): stage-2 input (7
-// fields, FP64), stage-3 input (7, FP64), per stage the NREC stage records,
-// then per stage and substep n (n = 0: initial) rho_theta', rho_u', rho_w'
-// after it; the records are of the recording pass's type RC (the handle's).
-double *rec_in(moist_handle *h, int st) {  // st = 1, 2
-    return (double *)(h->adj->cur + (size_t)(st - 1) * 7 * h->n2 * sizeof(double));
+// Layout of a record block (current block h->adj->cur): the FP64 inputs
+// (7 fields each: in the mixed form the step input, then the stage-2 and
+// stage-3 inputs; in the FP64 form the last two), per stage the NREC stage
+// records, then per stage and substep n (n = 0: initial) rho_theta', rho_u',
+// rho_w' after it; the records are of the recording pass's type RC (the
+// handle's).
+double *rec_in(moist_handle *h, int st) {  // st = 1, 2; 0 in the mixed form
+    return (double *)(h->adj->cur + (size_t)(st - 3 + rec_nin(h)) * 7 * h->n2 * sizeof(double));
 }
 template <class RC> RC *rec_st(moist_handle *h, int st, int r) {
     return (RC *)(h->adj->cur + rec_in_bytes(h) + ((size_t)st * NREC + r) * h->n2 * sizeof(RC));
@@ -903,7 +904,11 @@ void copy_set(moist_handle *h, double *dst, const double *src, size_t n) { copy_
 // One stage of the forward pass in the handle's form S with every
 // intermediate the adjoint reads kept in the stage record (the kernels of
 // enqueue_stage, their work arrays pointed into the record).
-template <class S> void enqueue_stage_rec(moist_handle *h, int st, SSet<S> in, SSet<S> out, SSet<S> s0, int div) {
+// wide: the stage output as full FP64 values (the mixed form's stage-2 and -3
+// inputs for the reverse sweep; null pointers: none).
+template <class S>
+void enqueue_stage_rec(moist_handle *h, int st, SSet<S> in, SSet<S> out, SSet<S> s0, int div,
+                       SSet<double> wide = SSet<double>{}) {
     const int nsub = h->ns_stage[st];
     const dim3 b2(TILE_X, TILE_Z);
     const ZVec zv = zvec(h);
@@ -937,7 +942,7 @@ template <class S> void enqueue_stage_rec(moist_handle *h, int st, SSet<S> in, S
     }
     hipLaunchKernelGGL((k_scalar<S, false>), dim3(h->nstiles), dim3(ST_THREADS), 0, h->stream, h->dom_d, h->stiles,
                        zv, in, s0, out, w, (const S *)rec_rup<S>(h, st, nsub), (const S *)rec_rtp<S>(h, st, nsub), div,
-                       0.0, st, (S *)nullptr, (S *)nullptr, (S *)nullptr);
+                       0.0, st, (S *)nullptr, (S *)nullptr, (S *)nullptr, wide);
 }
 
 // FP64 form: forward pass of one step from the state `sn` into the current
@@ -951,16 +956,17 @@ void enqueue_step_rec64(moist_handle *h, const double *sn) {
     const SSet<double> s0 = sset<double>(h, S0_), ia = dset(rec_in(h, 1), n2), ib = dset(rec_in(h, 2), n2);
     enqueue_sf_rd<double>(h, zv, s0, w);
     hipLaunchKernelGGL(k_physics<double>, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d,
-                       h->tiles2d, zv, s0, w, 0, 0);
+                       h->tiles2d, zv, s0, w, 0, 0, SSet<double>{});
     enqueue_stage_rec<double>(h, 0, s0, ia, s0, 3);
     enqueue_stage_rec<double>(h, 1, ia, ib, s0, 2);
     enqueue_stage_rec<double>(h, 2, ib, dset(h->adj->SC, n2), s0, 1);
 }
 
 // Mixed form: one step of the mixed form from the handle's state, recording
-// into the current block: FP32 stage and substep records, the stage-2 and -3
-// inputs widened to FP64; the step's output replaces the state, as a plain
-// step does.
+// into the current block: FP32 stage and substep records, and the step input
+// and the stage-2 and -3 inputs as FP64 (written by k_physics and the first
+// two stages' k_scalar, which hold those values); the step's output replaces
+// the state, as a plain step does.
 void enqueue_step_rec_mixed(moist_handle *h) {
     const size_t n2 = h->n2;
     const ZVec zv = zvec(h);
@@ -968,13 +974,10 @@ void enqueue_step_rec_mixed(moist_handle *h) {
     const Work<float> w = work<float>(h);
     const SSet<float> s0 = sset<float>(h, S0_), sa = sset<float>(h, SA_), sb = sset<float>(h, SB_);
     enqueue_sf_rd<float>(h, zv, s0, w);
-    hipLaunchKernelGGL(k_physics<float>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, s0, w, 0, 0);
-    enqueue_stage_rec<float>(h, 0, s0, sa, s0, 3);
-    hipLaunchKernelGGL(ka_widen, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, sa,
-                       dset(rec_in(h, 1), n2));
-    enqueue_stage_rec<float>(h, 1, sa, sb, s0, 2);
-    hipLaunchKernelGGL(ka_widen, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, sb,
-                       dset(rec_in(h, 2), n2));
+    hipLaunchKernelGGL(k_physics<float>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, s0, w, 0, 0,
+                       dset(rec_in(h, 0), n2));
+    enqueue_stage_rec<float>(h, 0, s0, sa, s0, 3, dset(rec_in(h, 1), n2));
+    enqueue_stage_rec<float>(h, 1, sa, sb, s0, 2, dset(rec_in(h, 2), n2));
     enqueue_stage_rec<float>(h, 2, sb, s0, s0, 1);
 }
 
@@ -1118,13 +1121,10 @@ void state64(moist_handle *h, double *dst) {
 // Step input j of the segment: the state slots in the handle's form.
 void *seg_step(moist_handle *h, int j) { return h->adj->seg + h->adj->seg_bytes * (size_t)j; }
 // Reverse step j of the segment (its record block is current): FP64 input
-// as stored, or the mixed input widened to FP64.
+// as stored, or in the mixed form as its recording pass widened it.
 void reverse_step(moist_handle *h, int j) {
-    AdjState &A = *h->adj;
     if (h->precision == MOIST_MIXED) {
-        hipLaunchKernelGGL(ka_widen, dim3(h->ntiles2d), dim3(TILE_X, TILE_Z), 0, h->stream, h->dom_d, h->tiles2d,
-                           zvec(h), fset((float *)seg_step(h, j), h->n2), dset(A.snb, h->n2));
-        enqueue_step_adj<float>(h, A.snb, seg_sfh(h, j));
+        enqueue_step_adj<float>(h, rec_in(h, 0), seg_sfh(h, j));
     } else {
         enqueue_step_adj<double>(h, (const double *)seg_step(h, j), seg_sfh(h, j));
     }
@@ -1145,7 +1145,8 @@ void record_step(moist_handle *h, int j, bool advance) {
 // saved, then the step recorded (which advances the state).
 void seg_record(moist_handle *h, int j) {
     AdjState &A = *h->adj;
-    copy_bytes(h, seg_step(h, j), h->pool, 7 * h->n2 * h->elem);
+    // the mixed form's reverse step reads its input from the record block
+    if (h->precision != MOIST_MIXED) copy_bytes(h, seg_step(h, j), h->pool, 7 * h->n2 * h->elem);
     copy_set(h, seg_sfh(h, j), h->sfh, 2 * h->nm);
     A.cur = A.recs + A.rb_bytes * (size_t)j;
     record_step(h, j, true);

@@ -341,7 +341,7 @@ __device__ __forceinline__ S term_velocity(S rho_qr, S rho) {
 template <class S>
 __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
                                                  ZVec zv, SSet<S> s, Work<S> w, int store_diag,
-                                                 int diag_pairs) {
+                                                 int diag_pairs, SSet<double> wide) {
     using R = S;
     using KC = K<R>;
     const int4 t = tiles[blockIdx.x];
@@ -364,6 +364,18 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
     const bool sf = (k == 0) && (fl & F_SF);
     const int64_t x = D.offx + i;
     const R qv_sf = sf ? w.d_qv_sf[x] : R(0.0), th_sf = sf ? w.d_th_sf[x] : R(0.0);
+    // wide (the mixed form's recording pass of the reverse sweep; null
+    // pointers: none): this state as full FP64 values, rho and rho_theta with
+    // their base state added
+    double wv[is_mixed<S>() ? 7 : 1];
+    if constexpr (is_mixed<S>()) {
+        if (wide.f[0]) {
+#pragma unroll
+            for (int f = 0; f < 7; ++f) wv[f] = double(s.f[f][c]);
+            wv[RHO] += zv.rho_bar[D.offz + k];
+            wv[RT] += zv.rt_bar[D.offz + k];
+        }
+    }
     // mixed form: the first stage's diagnostics of this same state (what
     // k_diag would compute), stored in the level-pair layout with the rest
     R dg_th = R(0.0), dg_pres = R(0.0), dg_rt2 = R(0.0);
@@ -479,6 +491,11 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
             w.theta[widx<true>(D, k, i)] = dg_th;
             w.rt2[widx<true>(D, k, i)] = dg_rt2;
         }
+    }
+    if constexpr (is_mixed<S>()) {
+        if (wide.f[0])
+#pragma unroll
+            for (int f = 0; f < 7; ++f) wide.f[f][c] = wv[f];
     }
     if (store_diag) {
         w.dg_th_mp[c] = dth;

@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
                                                        ZVec zv, SSet<S> in, SSet<S> s0, SSet<S> out, Work<S> w,
                                                        const S *__restrict__ rup_fin, const S *__restrict__ rtp_fin,
                                                        int div, double dt_ovr, int stage, S *dg_th, S *dg_pres,
-                                                       S *dg_rt2) {
+                                                       S *dg_rt2, SSet<double> wide) {
     using R = S;
     __shared__ R l_rho[ST_HZ][ST_HX];
     __shared__ R l_q[ST_HZ][ST_HX];
@@ -226,6 +226,10 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
         }
     }
 
+    // wide (x-fastest float variant, the adjoint's recording pass; null
+    // pointers: none): the output as full FP64 values, rho and rho_theta with
+    // their base state added, for the reverse sweep
+    constexpr bool WIDE = !P && is_mixed<S>();
     const S *dph[3] = {w.d_qv, w.d_qc, w.d_qr};
     R qn[3][P ? NCL : 1];  // the updated rho_q of this thread's cells (pair variant's epilogue)
     int nan_f = 0;
@@ -315,6 +319,9 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
             const R diff_z = fdiv_r(kap * (l_dz[r + 1][c] - l_dz[r][c]), dzk, rcz[u]);
             const R v = s0v[u] + dt * (adv + (diff_x + diff_z) + dpv[u]);
             out.f[QV + sc][q] = v;
+            if constexpr (WIDE) {
+                if (wide.f[0]) wide.f[QV + sc][q] = double(v);
+            }
             if constexpr (P) {
                 if (sc == 0) qn[0][u] = v;
                 else if (sc == 1) qn[1][u] = v;
@@ -350,6 +357,14 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
             out.f[RHOU][q] = vu;
             out.f[RT][q] = vt;
             out.f[RHOW][q] = vw;
+            if constexpr (WIDE) {
+                if (wide.f[0]) {
+                    wide.f[RHO][q] = double(vr) + zv.rho_bar[D.offz + k];
+                    wide.f[RHOU][q] = double(vu);
+                    wide.f[RT][q] = double(vt) + zv.rt_bar[D.offz + k];
+                    wide.f[RHOW][q] = double(vw);
+                }
+            }
             int f = 0;
             if (isnan(vr)) f = 1;
             else if (isnan(vu)) f = 2;
