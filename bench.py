@@ -211,9 +211,9 @@ def adjoint_bench(steps=600, ckpt=50, fd_dirs=32, cpu_steps=2):
     steps; FP64 form, and the mixed form (mixed-form trajectory, FP64 step
     adjoints along it).  Central FD of the GPU's own FP64 forward on 32
     seeded directions checks both gradients: smooth fields of unit RMS scaled
-    by the RMS of the initial theta' (theta - base profile) or qv, step
-    h = 1e-6 relative (an FP32 trajectory cannot resolve h = 1e-6, so the
-    mixed gradient is checked against the FP64 forward's differences)."""
+    by the RMS of the initial theta' (theta - base profile) or qv, steps
+    h = 1e-4 and 1e-6 relative (an FP32 trajectory cannot resolve such steps,
+    so the mixed gradient is checked against the FP64 forward's differences)."""
     p, s = ic.adjoint_bubble(np.float64, steps=steps)
     tt = ic.theta_target(p)
     eng, out, g, gK = _adjoint_form(p, s, tt, core.FP64, steps, ckpt)
@@ -222,46 +222,66 @@ def adjoint_bench(steps=600, ckpt=50, fd_dirs=32, cpu_steps=2):
     th0 = (s.rho_theta / s.rho)[:, hh:hh + p.Nx] - ic.theta_profile(p.z)[:, None]
     qv0 = (s.rho_qv / s.rho)[:, hh:hh + p.Nx]
     rms = {3: float(np.sqrt(np.mean(th0 ** 2))), 4: float(np.sqrt(np.mean(qv0 ** 2)))}
-    errs, fds = [], []
+    # central FD at two relative steps: h = 1e-4, where a sweep of h (1e-2 .. 1e-7)
+    # puts the FD's own error lowest (truncation below, roundoff of the
+    # 600-step losses above), and h = 1e-6
+    hs = (1e-4, 1e-6)
+    errs = {h: [] for h in hs}
+    fds = []
+    ads = []
     for d in range(fd_dirs):
         f, n = (3, "rho_theta") if d % 2 == 0 else (4, "rho_qv")
         v = _smooth_direction(p, rng)
         v *= rms[f] / np.sqrt(np.mean(v[:, hh:hh + p.Nx] ** 2))
         v = v * s.rho
-        eps = 1e-6
-        vals = []
-        for sgn in (1.0, -1.0):
-            sp = s.copy()
-            getattr(sp, n)[...] += sgn * eps * v
-            eng.upload(sp)
-            eng.step(steps)
-            a = s.copy()
-            eng.download(a, phys=False)
-            vals.append(_loss_np(p, a, tt))
-        fd = (vals[0] - vals[1]) / (2 * eps)
-        fds.append((f, v, fd))
         ad = float(np.sum(g[f][:, hh:hh + p.Nx] * v[:, hh:hh + p.Nx]))
-        errs.append(abs(fd - ad) / abs(ad))
+        ads.append(ad)
+        fd_h = {}
+        for eps in hs:
+            vals = []
+            for sgn in (1.0, -1.0):
+                sp = s.copy()
+                getattr(sp, n)[...] += sgn * eps * v
+                eng.upload(sp)
+                eng.step(steps)
+                a = s.copy()
+                eng.download(a, phys=False)
+                vals.append(_loss_np(p, a, tt))
+            fd_h[eps] = (vals[0] - vals[1]) / (2 * eps)
+            errs[eps].append(abs(fd_h[eps] - ad) / abs(ad))
+        fds.append((f, v, fd_h))
     eng.close()
     out = dict({"workload": f"adjoint_bubble_{p.Nx}x{p.Nz}", "form": "fp64", "steps": steps, "ckpt": ckpt,
                 "dt": float(p.dt), "ns": p.ns}, **out)
-    out["fd_check"] = {"directions": fd_dirs, "h": 1e-6, "controls": "theta' and qv at fixed rho",
-                       "direction_scale": {"theta_K": rms[3], "qv": rms[4]},
-                       "median_rel_err": float(np.median(errs)), "max_rel_err": float(max(errs))}
+
+    def fd_summary(e, h):
+        return {"directions": fd_dirs, "h": h, "median_rel_err": float(np.median(e)), "max_rel_err": float(max(e)),
+                "above_1e-5": int(sum(x > 1e-5 for x in e))}
+
+    out["fd_check"] = dict(fd_summary(errs[hs[0]], hs[0]), controls="theta' and qv at fixed rho",
+                           direction_scale={"theta_K": rms[3], "qv": rms[4]})
+    out["fd_check_small_h"] = fd_summary(errs[hs[1]], hs[1])
+    # per direction (seeded, reproducible): the adjoint's directional derivative and both FDs
+    out["fd_check"]["per_direction"] = {"adjoint": ads, "fd_h1e-4": [fd[hs[0]] for _, _, fd in fds],
+                                        "fd_h1e-6": [fd[hs[1]] for _, _, fd in fds]}
     # mixed form
     eng, mx, gm, gKm = _adjoint_form(p, s, tt, core.MIXED, steps, ckpt)
     eng.close()
-    errs_m = []
-    for f, v, fd in fds:
+    errs_m = {h: [] for h in hs}
+    ads_m = []
+    for f, v, fd_h in fds:
         ad = float(np.sum(gm[f][:, hh:hh + p.Nx] * v[:, hh:hh + p.Nx]))
-        errs_m.append(abs(fd - ad) / abs(ad))
+        ads_m.append(ad)
+        for h in hs:
+            errs_m[h].append(abs(fd_h[h] - ad) / abs(ad))
     mx["form"] = "mixed"
     mx["vs_fp64"] = {"J_rel": abs(mx["J"] - out["J"]) / abs(out["J"]),
                      "grad_theta_rel_l2": rel_l2(gm[3][:, hh:hh + p.Nx], g[3][:, hh:hh + p.Nx]),
                      "grad_qv_rel_l2": rel_l2(gm[4][:, hh:hh + p.Nx], g[4][:, hh:hh + p.Nx]),
                      "dJ_dK_rel": [abs(a - b) / max(abs(b), 1e-300) for a, b in zip(gKm, gK)]}
-    mx["fd_check_vs_fp64_forward"] = {"directions": fd_dirs, "median_rel_err": float(np.median(errs_m)),
-                                      "max_rel_err": float(max(errs_m))}
+    mx["fd_check_vs_fp64_forward"] = fd_summary(errs_m[hs[0]], hs[0])
+    mx["fd_check_vs_fp64_forward_small_h"] = fd_summary(errs_m[hs[1]], hs[1])
+    mx["fd_check_vs_fp64_forward"]["per_direction_adjoint"] = ads_m
     out["mixed"] = mx
     if cpu_steps:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

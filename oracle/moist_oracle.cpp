@@ -49,6 +49,7 @@
 #include <omp.h>
 #endif
 #include "../include/moist2d.h"
+#include "moist_dual.h"
 
 namespace orc {
 
@@ -544,13 +545,13 @@ static int compute_step(const G<T> &p, const T *d_rho_phys, const T *d_rho_u_phy
                 size_t q = p.ix(k, i);
                 T adv = -(rup[q] - rup[p.ix(k, i - 1)]) / dx;
                 T inc = dt_s * (R_rho[q] + adv);
-                rp_tp[k] = T(double(rp[q]) + fact_tp * double(inc));
+                rp_tp[k] = off_centre(rp[q], fact_tp, inc);
                 rp[q] += inc;
                 T fip = (TH(k, i + 1) + TH(k, i)) * T(0.5) * rup[q];
                 T fim = (TH(k, i) + TH(k, i - 1)) * T(0.5) * rup[p.ix(k, i - 1)];
                 adv = -(fip - fim) / dx;
                 inc = dt_s * (R_rho_theta[q] + adv);
-                rtp_tp[k] = T(double(rtp[q]) + fact_tp * double(inc));
+                rtp_tp[k] = off_centre(rtp[q], fact_tp, inc);
                 rtp[q] += inc;
             }
             for (int k = ks; k <= ke - 1; ++k) {
@@ -1173,6 +1174,46 @@ ORC_EXPORT int orc_adjoint_run_f64(const moist_params *mp, double *const *fields
                                    const double *theta_tgt, double *J, double *const *grad, double *gK) {
     orc::G<double> g(mp);
     return orc::adjoint_run(g, fields, nsteps, ckpt, theta_tgt, J, grad, gK);
+}
+
+// ---- forward-mode derivative of the 600-step loss (test infrastructure) --------
+// The loss of adjoint_run, J = dx sum(prec) + 1/2 sum (theta - theta_tgt)^2
+// after nsteps rk3_step!s, and its directional derivative along the tangent
+// `dir` of the 7 prognostic fields (with their halo columns) and dK of
+// (K_auto, Qc_threshold, K_accr): rk3_step! instantiated with dual numbers (FwdAD)
+// (moist_dual.h), branches following the primal trajectory.  fields: the
+// State in the order of the F_ enum (read only).
+ORC_EXPORT int orc_jvp_run_f64(const moist_params *mp, const double *const *fields, const double *const *dir,
+                               const double *dK, int nsteps, const double *theta_tgt, double *J, double *dJ) {
+    orc::G<orc::FwdAD> g(mp);
+    g.c.K_auto.d = dK[0];
+    g.c.Qc_threshold.d = dK[1];
+    g.c.K_accr.d = dK[2];
+    const size_t N = g.n2();
+    const size_t len[orc::F_COUNT] = {N, N, N, N, N, N, N, (size_t)g.ia, (size_t)g.ia, (size_t)g.ia, (size_t)g.ia,
+                                      (size_t)g.ka, N, N, N, N, N, (size_t)g.ia, (size_t)g.ia, (size_t)g.ia, N};
+    std::vector<std::vector<orc::FwdAD>> buf(orc::F_COUNT);
+    orc::FwdAD *F[orc::F_COUNT];
+    for (int f = 0; f < orc::F_COUNT; ++f) {
+        buf[f].resize(len[f]);
+        for (size_t q = 0; q < len[f]; ++q) buf[f][q] = orc::FwdAD(fields[f][q], f < 7 ? dir[f][q] : 0.0);
+        F[f] = buf[f].data();
+    }
+    for (int n = 0; n < nsteps; ++n) {
+        int rc = orc::rk3_step(g, F, orc::ORDER_SNAPSHOT);
+        if (rc) return rc;
+    }
+    orc::FwdAD j(0.0);
+    for (int i = g.is; i <= g.ie; ++i) j += g.dx * F[orc::F_PREC][i];
+    for (int i = g.is; i <= g.ie; ++i)
+        for (int k = g.ks; k <= g.ke; ++k) {
+            const size_t q = g.ix(k, i);
+            const orc::FwdAD d = F[3][q] / F[0][q] - orc::FwdAD(theta_tgt[q]);
+            j += orc::FwdAD(0.5) * d * d;
+        }
+    *J = j.v;
+    *dJ = j.d;
+    return 0;
 }
 
 ORC_EXPORT int orc_set_threads(int n) {

@@ -67,6 +67,7 @@ def lib():
             ("adjoint_run", ctypes.c_int, [V, V, ctypes.c_int, ctypes.c_int, V, V, V, V]),
             ("output_history", None, [V, V, V, V]),
             ("slow_tendencies", ctypes.c_int, [V, V, V, V]),
+            ("jvp_run", ctypes.c_int, [V, V, V, V, ctypes.c_int, V, V, V]),
         ):
             f = getattr(_lib, f"orc_{name}_f64")
             f.restype = res
@@ -314,6 +315,27 @@ def adjoint_run(s, p, nsteps, theta_target, ckpt=50):
     if rc:
         raise RuntimeError(f"adjoint_run failed ({rc})")
     return J.value, grad, gK
+
+
+def jvp_run(s, p, nsteps, theta_target, direction, dK=(0.0, 0.0, 0.0)):
+    """Forward-mode derivative of adjoint_run's loss: (J, dJ), dJ the
+    derivative of J along `direction` ({field name: array in the State's
+    layout} for any of the 7 prognostic fields) and dK (tangents of K_auto,
+    Qc_threshold, K_accr), from nsteps rk3_step!s run on dual numbers
+    (branches follow the primal trajectory, as ForwardDiff's do).  s is not
+    modified."""
+    shape = s.rho.shape
+    dirs = [np.asfortranarray(direction.get(n, np.zeros(shape)), dtype=np.float64) for n in _PROG]
+    J, dJ = ctypes.c_double(), ctypes.c_double()
+    cp = p.to_c()
+    arrs = [np.array(a, dtype=np.float64, order="F") for a in s.arrays()]
+    tt = np.asfortranarray(theta_target, dtype=np.float64)
+    dk = np.asarray(dK, dtype=np.float64)
+    rc = lib().orc_jvp_run_f64(ctypes.byref(cp), _f64(arrs), _f64(dirs), dk.ctypes.data_as(ctypes.c_void_p), nsteps,
+                               tt.ctypes.data_as(ctypes.c_void_p), ctypes.byref(J), ctypes.byref(dJ))
+    if rc:
+        raise RuntimeError(f"jvp_run failed ({rc})")
+    return J.value, dJ.value
 
 
 HIST_FIELDS = ("rho", "rhou", "rhow", "u", "w", "theta", "qv", "qc", "qr", "RH", "temp", "pres", "thetae")

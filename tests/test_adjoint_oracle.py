@@ -5,7 +5,12 @@ Pins: compute_step! adjoint per input field, physics adjoint (Kessler +
 sedimentation, with the three Kessler parameters), the adjoint of one
 rk3_step!, and the checkpointed gradient of the adjoint workload's loss
 (initial theta' and qv at fixed density, K_auto, Qc_threshold, K_accr),
-plus a Taylor-remainder test and checkpoint-interval invariance.
+plus a Taylor-remainder test and checkpoint-interval invariance.  The
+checkpointed gradient is also compared with forward-mode differentiation of
+the same restated step (rk3_step! on dual numbers, the counterpart of
+ForwardDiff.jl through moist.jl), which a branch switch inside a finite
+difference's step cannot blur; the forward mode itself is pinned by the
+closed-form derivative of the loss and by a one-step finite difference.
 
 States carry a smooth cloud and rain background so that no perturbation
 switches a branch the adjoint freezes (a field that is identically zero
@@ -390,3 +395,61 @@ def test_loss_gradient_with_surface_fluxes_and_radiation():
     for f in range(7):
         np.testing.assert_allclose(gc[f][:, h:h + p.Nx], grad[f][:, h:h + p.Nx], rtol=1e-12,
                                    atol=1e-14 * np.abs(grad[f]).max())
+
+
+def test_forward_mode_derivative_pins():
+    """The oracle's forward mode (rk3_step! on dual numbers, jvp_run) against
+    what fixes it independently: with no step the loss is
+    dx sum(prec) + 1/2 |theta - theta_target|^2, whose derivative along v is
+    sum((theta - tgt) dtheta) with dtheta = (v_rt - theta v_rho) / rho; after
+    one step it equals the central FD of the double forward (h = 1e-4 relative,
+    a smooth direction: below it the FD is lost in the roundoff of a loss of
+    ~200 whose directional derivative is ~0.03); the primal loss is the double
+    oracle's."""
+    p, s, rng, tt = _loss_setup()
+    h = p.halo
+    vr, vt = _dir(p, rng, s.rho.shape) * 1e-3, _dir(p, rng, s.rho.shape) * 1e-1
+    J0, dJ0 = O.jvp_run(s, p, 0, tt, {"rho": vr, "rho_theta": vt})
+    th = s.rho_theta / s.rho
+    dth = (vt - th * vr) / s.rho
+    ref = float(np.sum(_inner(p, (th - tt) * dth)))
+    assert abs(J0 - _loss(p, s, tt, 0)) <= 1e-13 * abs(J0)
+    assert abs(dJ0 - ref) <= 1e-12 * abs(ref), (dJ0, ref)
+    v = _smooth_dir(p, rng, s.rho.shape) * 0.01 * s.rho
+    J1, dJ1 = O.jvp_run(s, p, 1, tt, {"rho_theta": v})
+    assert abs(J1 - _loss(p, s, tt, 1)) <= 1e-13 * abs(J1)
+    eps = 1e-4
+    sp, sm = s.copy(), s.copy()
+    sp.rho_theta[...] += eps * v
+    sm.rho_theta[...] -= eps * v
+    fd = (_loss(p, sp, tt, 1) - _loss(p, sm, tt, 1)) / (2 * eps)
+    assert abs(fd - dJ1) <= 2e-6 * abs(dJ1), (fd, dJ1)
+
+
+@pytest.mark.parametrize("switches", ["kessler", "sf_rd"])
+def test_adjoint_equals_forward_mode_derivative(switches):
+    """The checkpointed adjoint against forward-mode differentiation of the
+    same restated step (dual numbers through rk3_step!, branches following the
+    primal trajectory: what ForwardDiff.jl does through moist.jl) on 12 steps:
+    each prognostic field's direction, a mixed direction and the three Kessler
+    parameters agree to 1e-11.  Unlike a finite difference this sees no
+    branch switch, so it pins the adjoint where the loss has kinks."""
+    if switches == "kessler":
+        p, s, rng, tt = _loss_setup()
+    else:
+        p, s, rng = _state_sf_rd(11)
+        tt = np.asfortranarray(s.rho_theta / s.rho + rng.normal(0, 0.5, size=s.rho.shape))
+    nsteps = 12
+    J, grad, gK = O.adjoint_run(s.copy(), p, nsteps, tt, ckpt=5)
+    dirs = [{n: _dir(p, rng, s.rho.shape) * SCALES[n]} for n in PROG]
+    dirs.append({n: _dir(p, rng, s.rho.shape) * SCALES[n] for n in PROG})
+    for d in dirs:
+        Jf, dJ = O.jvp_run(s, p, nsteps, tt, d)
+        assert Jf == J
+        ad = sum(float(np.sum(_inner(p, grad[PROG.index(n)] * v))) for n, v in d.items())
+        assert abs(ad - dJ) <= 1e-11 * abs(dJ), (list(d), ad, dJ)
+    for j in range(3):
+        dk = [0.0, 0.0, 0.0]
+        dk[j] = 1.0
+        _, dJ = O.jvp_run(s, p, nsteps, tt, {}, dK=dk)
+        assert abs(gK[j] - dJ) <= 1e-11 * abs(dJ), (j, gK[j], dJ)

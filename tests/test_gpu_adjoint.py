@@ -1,7 +1,8 @@
 """GPU reverse mode (moist_adjoint_run / moist_adjoint_grad) against the
-oracle's adjoint (itself pinned by central finite differences in
-test_adjoint_oracle.py) and against central finite differences of the GPU's
-own forward step.
+oracle's adjoint (itself pinned by central finite differences and by the
+oracle's forward mode in test_adjoint_oracle.py), against the oracle's
+forward-mode derivative (dual numbers, the ForwardDiff counterpart) and
+against central finite differences of the GPU's own forward step.
 
 FP64 form: bars are the loss to 1e-12 relative; per-field gradient rel-L2
 <= 1e-8 (the oracle and the GPU run the same FP64 operations in a different
@@ -187,6 +188,45 @@ def test_adjoint_bubble_64x32():
     rng = np.random.default_rng(9)
     tt = np.asfortranarray(s.rho_theta / s.rho + rng.normal(0, 0.5, size=s.rho.shape))
     _compare("bubble 64x32, 20 steps", p, s, tt, 20, 6, tol=1e-7)
+
+
+def test_adjoint_equals_forward_mode_derivative_64x32():
+    """The device's FP64 adjoint against forward-mode differentiation of the
+    oracle (dual numbers through the restated rk3_step!, branches following
+    the primal trajectory, as ForwardDiff.jl through moist.jl) on the 64x32
+    bubble with cloud and rain, 20 steps, checkpoints every 6: directions of
+    theta', qv and rho_u and the three Kessler parameters, to 1e-8 (the two
+    trajectories differ at roundoff)."""
+    p, s = ic.bubble(np.float64, nx=64, nz=32, d=312.5, dt=1.0)
+    O.rk3_step(s, p, 30)
+    rng = np.random.default_rng(12)
+    # cloud above the autoconversion threshold and rain everywhere, so that
+    # the three Kessler parameters act
+    s.rho_qc[...] += 1.5e-3 * s.rho * (1 + 0.2 * rng.random(s.rho.shape))
+    s.rho_qr[...] += 1e-4 * s.rho * (1 + 0.2 * rng.random(s.rho.shape))
+    for n in PROG7:
+        _wrap(p, getattr(s, n))
+    tt = np.asfortranarray(s.rho_theta / s.rho + rng.normal(0, 0.5, size=s.rho.shape))
+    nsteps = 20
+    eng = core.Engine(p, core.FP64)
+    eng.upload(s)
+    J = eng.adjoint(nsteps, [tt], ckpt=6)[0]
+    g, gK = eng.adjoint_grad(0)
+    eng.close()
+    h = p.halo
+    for f, n, sc in ((3, "rho_theta", 0.1), (4, "rho_qv", 1e-4), (1, "rho_u", 0.5)):
+        v = np.asfortranarray(_wrap(p, rng.normal(size=s.rho.shape)) * sc * s.rho)
+        Jf, dJ = O.jvp_run(s, p, nsteps, tt, {n: v})
+        ad = float(np.sum(g[f][:, h:h + p.Nx] * v[:, h:h + p.Nx]))
+        print(f"{n}: adjoint {ad:.12e} forward mode {dJ:.12e} rel {abs(ad - dJ) / abs(dJ):.1e}", flush=True)
+        assert abs(Jf - J) <= 1e-12 * abs(J)
+        assert abs(ad - dJ) <= 1e-8 * abs(dJ), (n, ad, dJ)
+    for j in range(3):
+        dk = [0.0, 0.0, 0.0]
+        dk[j] = 1.0
+        _, dJ = O.jvp_run(s, p, nsteps, tt, {}, dK=dk)
+        print(f"K{j}: adjoint {gK[j]:.12e} forward mode {dJ:.12e}", flush=True)
+        assert dJ != 0.0 and abs(gK[j] - dJ) <= 1e-8 * abs(dJ), (j, gK[j], dJ)
 
 
 def test_adjoint_ensemble_members_match_single_runs():
