@@ -478,7 +478,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--nx", type=int, default=8192)
     ap.add_argument("--nz", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=64, help="large steps of the CPU baseline (about 15 s on 16 host threads)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=30, help="0 skips the 256-member ensemble")
