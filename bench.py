@@ -494,6 +494,9 @@ def main():
     ap.add_argument("--no-dropin", action="store_true", help="skip timing the host-State drop-in call")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (counter calibration under rocprofv3)")
+    ap.add_argument("--batched-profile", type=int, metavar="STEPS", default=0,
+                    help="only run the 64-member batched adjoint over STEPS steps (checkpoints every 50, mixed "
+                         "form; both forms without --no-fp64) and print its times; for a kernel trace")
     ap.add_argument("--adjoint-profile", type=int, metavar="STEPS", default=0,
                     help="only run the 512x128 adjoint workload over STEPS steps (checkpoints every 50 steps, "
                          "no FD check; both forms, or the mixed one with --no-fp64) and print its times; "
@@ -507,6 +510,10 @@ def main():
     global AUTOTUNE, FIXED_SHAPE
     AUTOTUNE = not args.no_autotune
     FIXED_SHAPE = tuple(args.shape) if args.shape else None
+    if args.batched_profile > 0:
+        forms = ("mixed",) if args.no_fp64 else ("mixed", "fp64")
+        print(json.dumps(batched_adjoint_bench(steps=args.batched_profile, forms=forms)), flush=True)
+        return
     if args.adjoint_profile > 0:
         n = args.adjoint_profile
         p, s = ic.adjoint_bubble(np.float64, steps=n)

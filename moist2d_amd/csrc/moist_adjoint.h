@@ -398,104 +398,224 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
 // lo: adjoint of the stage output; writes ls (stage input, first writer of the
 // stage), adds into ls0 (rho_q) and lph (q tendencies); writes Lmx, Lmz and
 // the substep adjoints Lrp, Lrtp, Lrwp, Lrup (= lo of rho, rho_theta, rho_w, rho_u).
+//
+// A block owns the AD_X x AD_Z cells of its tile.  Per scalar it stages in
+// LDS q = rho_q / rho with a 3-cell halo and the scalar's output adjoint L
+// with a 2-cell halo (rho with a 1-cell halo once for the three), evaluates
+// the Koren partials of every x and z face that touches the tile once, and
+// keeps per face the four products (L-difference * mass flux) * partial that
+// its four cells gather; each cell then sums them in the order of the
+// per-cell expression sum_faces LF * m * c[slot], so the sums are the same
+// numbers as if every cell evaluated its four faces itself.
 template <class RC>
 __global__ void __launch_bounds__(AD_X *AD_Z)
     ka_scalar(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s, SSet<double> lo,
               SSet<double> ls, SSet<double> ls0, SSet<double> lph, AdjW<RC> a, double *Lrup, int div) {
+    constexpr int QX = AD_X + 6, QZ = AD_Z + 6;  // q: columns i0-3 .. i0+AD_X+2, levels k0-3 .. k0+AD_Z+2
+    constexpr int LX = AD_X + 4, LZ = AD_Z + 4;  // L: i0-2 .. i0+AD_X+1, k0-2 .. k0+AD_Z+1
+    constexpr int RX = AD_X + 2, RZ = AD_Z + 2;  // rho: i0-1 .. i0+AD_X, k0-1 .. k0+AD_Z
+    constexpr int FX = AD_X + 3;                 // x faces j+1/2, j = i0-2 .. i0+AD_X, of levels k0 .. k0+AD_Z-1
+    constexpr int FZ = AD_Z + 3;                 // z faces kk+1/2, kk = k0-2 .. k0+AD_Z, of columns i0 ..
+    constexpr int NT = AD_X * AD_Z;
+    __shared__ double sQ[QZ][QX], sL[LZ][LX], sR[RZ][RX];
+    __shared__ double sPx[AD_Z][FX][4], sSx[AD_Z][FX];  // x faces: (LF m) c[slot], the face value
+    __shared__ double sPz[FZ][AD_X][4];                  // z faces: (LF m) c[slot]
+    __shared__ double sOz[AD_Z][AD_X][3];                // each cell's own z face: value (0 if capped), clo, chi
+    __shared__ double sMx[AD_Z][FX], sMz[FZ][AD_X];       // face mass fluxes, the same for the three scalars
+    __shared__ double sG[QZ][4];                         // levels k0-3 ..: 1/dz, 1/dzh, 1/(dz+dz'), dz
+
     const int4 t = tiles[blockIdx.x];
-    if (t.x < 0) return;
+    if (t.x < 0) return;  // padding tile of the XCD ordering (uniform per block)
     const Dom &D = doms[t.x];
-    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+    const int i0 = t.y, k0 = t.z;
+    const int tx = threadIdx.x, tz = threadIdx.y, tid = tz * AD_X + tx;
+    const int i = i0 + tx, k = k0 + tz;
     const int nx = D.nx, nz = D.nz, ke = nz - 1;
-    if (i >= nx || k >= nz) return;
+    const bool cell = i < nx && k < nz;
     const double *dzd = zv.dz + D.offz;
     const double dx = D.dx, kap = D.kappa, dt = D.dt / double(div), rdx = 1.0 / dx;
     const double4 *gz = zv.geo + D.offz;  // per level: 1/dz, 1/(z_k+1 - z_k), 1/(dz_k + dz_k+1)
     auto G = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
-    const int64_t c = G(k, i);
+    const int64_t c = G(min(k, ke), i);
     const double *rho = s.f[RHO];
-    const double rrho = 1.0 / rho[c];
+    // once for the three scalars: rho of the tile with a 1-cell halo, the
+    // face mass fluxes and the per-level geometry
+    for (int e = tid; e < RZ * RX; e += NT) {
+        const int r = e / RX, kk = k0 - 1 + r;
+        (&sR[0][0])[e] = kk >= 0 && kk <= ke ? rho[G(kk, i0 - 1 + e - r * RX)] : 0.0;
+    }
+    for (int e = tid; e < AD_Z * FX; e += NT) {
+        const int r = e / FX, kk = k0 + r;
+        (&sMx[0][0])[e] = kk <= ke ? double(a.mom_x[G(kk, i0 - 2 + e - r * FX)]) : 0.0;
+    }
+    for (int e = tid; e < FZ * AD_X; e += NT) {
+        const int g = e / AD_X, kk = k0 - 2 + g, ii = i0 + e - g * AD_X;
+        (&sMz[0][0])[e] = kk >= 0 && kk <= ke - 1 && ii < nx ? double(a.mom_z[G(kk, ii)]) : 0.0;
+    }
+    if (tid < QZ) {
+        const int kk = k0 - 3 + tid;
+        if (kk >= 0 && kk <= ke) {
+            const double4 gv = gz[kk];
+            sG[tid][0] = gv.x;
+            sG[tid][1] = gv.y;
+            sG[tid][2] = gv.z;
+            sG[tid][3] = dzd[kk];
+        }
+    }
     // the final update's adjoints, read before any store
-    const double lo_r = lo.f[RHO][c], lo_u = lo.f[RHOU][c], lo_t = lo.f[RT][c], lo_w = lo.f[RHOW][c];
+    double lo_r = 0.0, lo_u = 0.0, lo_t = 0.0, lo_w = 0.0, rrho = 0.0;
+    if (cell) {
+        lo_r = lo.f[RHO][c];
+        lo_u = lo.f[RHOU][c];
+        lo_t = lo.f[RT][c];
+        lo_w = lo.f[RHOW][c];
+        rrho = 1.0 / rho[c];
+    }
     double lrho = lo_r;  // final update: rho_out = rho + rho'
     double lmx = 0.0, lmz = 0.0;
+    auto SQ = [&](int kk, int ii) { return sQ[kk - k0 + 3][ii - i0 + 3]; };
+    auto SL = [&](int kk, int ii) { return sL[kk - k0 + 2][ii - i0 + 2]; };
+    auto SR = [&](int kk, int ii) { return sR[kk - k0 + 1][ii - i0 + 1]; };
+    // staged level geometry: 1/dz, 1/(z_k+1 - z_k), 1/(dz_k + dz_k+1), dz of level kk
+    auto GX = [&](int kk) { return sG[kk - k0 + 3][0]; };
+    auto GY = [&](int kk) { return sG[kk - k0 + 3][1]; };
+    auto GZ = [&](int kk) { return sG[kk - k0 + 3][2]; };
+    auto DZ = [&](int kk) { return sG[kk - k0 + 3][3]; };
+    // a scalar's q and L tiles are loaded into registers one scalar ahead
+    // (while the previous scalar's faces and cells are computed) and written
+    // to LDS at the top of its iteration
+    constexpr int NQ = (QZ * QX + NT - 1) / NT, NL = (LZ * LX + NT - 1) / NT;
+    double qn[NQ], ln[NL];
+    auto prefetch = [&](int sc) {
+        const double *qa = a.q[sc], *L = lo.f[QV + sc];
+#pragma unroll
+        for (int u = 0; u < NQ; ++u) {
+            const int e = tid + u * NT, r = e / QX, kk = k0 - 3 + r;
+            qn[u] = e < QZ * QX && kk >= 0 && kk <= ke ? qa[G(kk, i0 - 3 + e - r * QX)] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < NL; ++u) {
+            const int e = tid + u * NT, r = e / LX, kk = k0 - 2 + r;
+            ln[u] = e < LZ * LX && kk >= 0 && kk <= ke ? L[G(kk, i0 - 2 + e - r * LX)] : 0.0;
+        }
+    };
+    prefetch(0);
     // ---- the three scalars
 #pragma unroll 1
     for (int sc = 0; sc < 3; ++sc) {
         const double *qr = s.f[QV + sc];
-        const double *L = lo.f[QV + sc];
-        const double *qa = a.q[sc];
-        auto Q = [&](int kk, int ii) { return qa[G(kk, ii)]; };
-        auto LFx = [&](int j) { return dt * rdx * (L[G(k, j + 1)] - L[G(k, j)]); };
-        auto LFz = [&](int kk) { return dt * (L[G(kk + 1, i)] * gz[kk + 1].x - L[G(kk, i)] * gz[kk].x); };
-        // this scalar's read-modify-write values; the stores follow all its loads
-        const double Lc = L[c], s0v = ls0.f[QV + sc][c], phv = lph.f[3 + sc][c];
-        double Lq = 0.0;
-        // x faces j+1/2, j = i-2..i+1, read q(j-1..j+2); cell i is q slot i-j+1.
-        // Face i+1/2 (o = 2) is this cell's own: its value is d flux / d mom_x.
-        double s_own_x = 0.0;
-#pragma unroll
-        for (int o = 0; o < 4; ++o) {
-            const int j = i - 2 + o, slot = 3 - o;
-            double cf[4];
-            const double m = a.mom_x[G(k, j)];
-            const double sf = kcoef_x(Q(k, j - 1), Q(k, j), Q(k, j + 1), Q(k, j + 2), m, cf);
-            Lq += LFx(j) * m * cf[slot];
-            if (o == 2) s_own_x = sf;
+        // this scalar's read-modify-write values
+        double s0v = 0.0, phv = 0.0;
+        if (cell) {
+            s0v = ls0.f[QV + sc][c];
+            phv = lph.f[3 + sc][c];
         }
-        // z faces kk+1/2, kk = k-2..k+1 within 0..ke-1, read q(kk-1..kk+2); cell k is slot k-kk+1
-        double s_own_z = 0.0, clo_own = 0.0, chi_own = 0.0;
 #pragma unroll
-        for (int o = 0; o < 4; ++o) {
-            const int kk = k - 2 + o, slot = 3 - o;
-            if (kk < 0 || kk > ke - 1) continue;
-            double cf[4], clo, chi;
-            bool capped;
-            const double m = a.mom_z[G(kk, i)];
-            const double sf = kcoef_z(kk > 0 ? Q(kk - 1, i) : 0.0, Q(kk, i), Q(kk + 1, i),
-                                      kk + 2 <= ke ? Q(kk + 2, i) : 0.0, m, qr[G(kk, D.lin_col)],
-                                      qr[G(kk + 1, D.lin_col)], kk == 0, kk == ke - 1, dzd[kk], dzd[kk + 1],
-                                      gz[kk].z, D.dt, cf, capped, clo, chi);
-            Lq += LFz(kk) * m * cf[slot];
-            if (o == 2) {
-                s_own_z = capped ? 0.0 : sf;
-                clo_own = clo;
-                chi_own = chi;
+        for (int u = 0; u < NQ; ++u)
+            if (tid + u * NT < QZ * QX) (&sQ[0][0])[tid + u * NT] = qn[u];
+#pragma unroll
+        for (int u = 0; u < NL; ++u)
+            if (tid + u * NT < LZ * LX) (&sL[0][0])[tid + u * NT] = ln[u];
+        __syncthreads();
+        if (sc < 2) prefetch(sc + 1);
+        // x faces j+1/2 of the tile's levels (compute_limited_flux_x at j)
+        for (int e = tid; e < AD_Z * FX; e += NT) {
+            const int r = e / FX, f = e - r * FX;
+            const int kk = k0 + r, j = i0 - 2 + f;
+            double p[4] = {0.0, 0.0, 0.0, 0.0}, sf = 0.0;
+            if (kk <= ke) {
+                double cf[4];
+                const double m = sMx[r][f];
+                sf = kcoef_x(SQ(kk, j - 1), SQ(kk, j), SQ(kk, j + 1), SQ(kk, j + 2), m, cf);
+                const double A = dt * rdx * (SL(kk, j + 1) - SL(kk, j)) * m;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) p[q] = A * cf[q];
             }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sPx[r][f][q] = p[q];
+            sSx[r][f] = sf;
         }
-        // diffusive x faces j+1/2, j = i-1, i: D = (rho_j + rho_j+1) / 2 (q_j+1 - q_j) / dx
-        for (int j = i - 1; j <= i; ++j) {
-            const double lD = kap * dt * rdx * (L[G(k, j)] - L[G(k, j + 1)]);
-            const double r0 = rho[G(k, j)], r1 = rho[G(k, j + 1)];
-            Lq += lD * (r1 + r0) * 0.5 * rdx * (j == i - 1 ? 1.0 : -1.0);
-            lrho += lD * 0.5 * (Q(k, j + 1) - Q(k, j)) * rdx;
+        // z faces kk+1/2 of the tile's columns (compute_limited_flux_z at kk)
+        for (int e = tid; e < FZ * AD_X; e += NT) {
+            const int g = e / AD_X, cc = e - g * AD_X;
+            const int kk = k0 - 2 + g, ii = i0 + cc;
+            double p[4] = {0.0, 0.0, 0.0, 0.0}, own[3] = {0.0, 0.0, 0.0};
+            if (kk >= 0 && kk <= ke - 1 && ii < nx) {
+                double cf[4], clo, chi;
+                bool capped;
+                const double m = sMz[g][cc];
+                // the positivity limiters' rho_q (read through the linear index) only
+                // matter on the bottom and top faces
+                const bool bot = kk == 0, top = kk == ke - 1;
+                const double qlo = bot ? qr[G(kk, D.lin_col)] : 0.0, qhi = top ? qr[G(kk + 1, D.lin_col)] : 0.0;
+                const double sf = kcoef_z(kk > 0 ? SQ(kk - 1, ii) : 0.0, SQ(kk, ii), SQ(kk + 1, ii),
+                                          kk + 2 <= ke ? SQ(kk + 2, ii) : 0.0, m, qlo, qhi, bot, top, DZ(kk),
+                                          DZ(kk + 1), GZ(kk), D.dt, cf, capped, clo, chi);
+                const double B = dt * (SL(kk + 1, ii) * GX(kk + 1) - SL(kk, ii) * GX(kk)) * m;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) p[q] = B * cf[q];
+                own[0] = capped ? 0.0 : sf;
+                own[1] = clo;
+                own[2] = chi;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sPz[g][cc][q] = p[q];
+            if (g >= 2 && g < 2 + AD_Z)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) sOz[g - 2][cc][q] = own[q];
         }
-        // diffusive z faces kk+1/2, kk = k-1, k: D = f2h(rho) (q_kk+1 - q_kk) / (z_kk+1 - z_kk)
-        for (int kk = k - 1; kk <= k; ++kk) {
-            if (kk < 0 || kk > ke - 1) continue;
-            const double lD = kap * dt * (L[G(kk, i)] * gz[kk].x - L[G(kk + 1, i)] * gz[kk + 1].x);
-            const double r0 = rho[G(kk, i)], r1 = rho[G(kk + 1, i)];
-            const double rden = gz[kk].z, rzz = gz[kk].y;
-            const double rf = (r1 * dzd[kk] + r0 * dzd[kk + 1]) * rden;
-            const double gq = (Q(kk + 1, i) - Q(kk, i)) * rzz;
-            Lq += lD * rf * rzz * (kk == k - 1 ? 1.0 : -1.0);
-            lrho += lD * gq * (kk == k - 1 ? dzd[kk] : dzd[kk + 1]) * rden;
+        __syncthreads();
+        if (cell) {
+            double Lq = 0.0;
+            // x faces j+1/2, j = i-2..i+1; cell i is slot i-j+1
+#pragma unroll
+            for (int o = 0; o < 4; ++o) Lq += sPx[tz][tx + o][3 - o];
+            const double s_own_x = sSx[tz][tx + 2];
+            // z faces kk+1/2, kk = k-2..k+1 within 0..ke-1; cell k is slot k-kk+1
+#pragma unroll
+            for (int o = 0; o < 4; ++o) {
+                const int kk = k - 2 + o;
+                if (kk < 0 || kk > ke - 1) continue;
+                Lq += sPz[tz + o][tx][3 - o];
+            }
+            const double s_own_z = sOz[tz][tx][0], clo_own = sOz[tz][tx][1], chi_own = sOz[tz][tx][2];
+            // diffusive x faces j+1/2, j = i-1, i: D = (rho_j + rho_j+1) / 2 (q_j+1 - q_j) / dx
+            for (int j = i - 1; j <= i; ++j) {
+                const double lD = kap * dt * rdx * (SL(k, j) - SL(k, j + 1));
+                const double r0 = SR(k, j), r1 = SR(k, j + 1);
+                Lq += lD * (r1 + r0) * 0.5 * rdx * (j == i - 1 ? 1.0 : -1.0);
+                lrho += lD * 0.5 * (SQ(k, j + 1) - SQ(k, j)) * rdx;
+            }
+            // diffusive z faces kk+1/2, kk = k-1, k: D = f2h(rho) (q_kk+1 - q_kk) / (z_kk+1 - z_kk)
+            for (int kk = k - 1; kk <= k; ++kk) {
+                if (kk < 0 || kk > ke - 1) continue;
+                const double lD = kap * dt * (SL(kk, i) * GX(kk) - SL(kk + 1, i) * GX(kk + 1));
+                const double r0 = SR(kk, i), r1 = SR(kk + 1, i);
+                const double rden = GZ(kk), rzz = GY(kk);
+                const double rf = (r1 * DZ(kk) + r0 * DZ(kk + 1)) * rden;
+                const double gq = (SQ(kk + 1, i) - SQ(kk, i)) * rzz;
+                Lq += lD * rf * rzz * (kk == k - 1 ? 1.0 : -1.0);
+                lrho += lD * gq * (kk == k - 1 ? DZ(kk) : DZ(kk + 1)) * rden;
+            }
+            const double Lqr = Lq * rrho;
+            lrho -= Lqr * SQ(k, i);
+            // this cell owns face i+1/2 (mom_x) and face k+1/2 (mom_z)
+            lmx += dt * rdx * (SL(k, i + 1) - SL(k, i)) * s_own_x;
+            if (k <= ke - 1) {
+                const double lf = dt * (SL(k + 1, i) * GX(k + 1) - SL(k, i) * GX(k));
+                lmz += lf * s_own_z;
+                // linear-index positivity limiters: bottom face reads (0, lin_col), top face (ke, lin_col)
+                if (k == 0) a.linb[D.offx * 3 + (int64_t)sc * nx + i] = lf * clo_own;
+                if (k == ke - 1) a.lint[D.offx * 3 + (int64_t)sc * nx + i] = lf * chi_own;
+            }
+            const double Lc = SL(k, i);
+            ls.f[QV + sc][c] = Lqr;
+            ls0.f[QV + sc][c] = s0v + Lc;
+            lph.f[3 + sc][c] = phv + dt * Lc;
         }
-        const double Lqr = Lq * rrho;
-        lrho -= Lqr * qa[c];
-        // this cell owns face i+1/2 (mom_x) and face k+1/2 (mom_z)
-        lmx += LFx(i) * s_own_x;
-        if (k <= ke - 1) {
-            const double lf = LFz(k);
-            lmz += lf * s_own_z;
-            // linear-index positivity limiters: bottom face reads (0, lin_col), top face (ke, lin_col)
-            if (k == 0) a.linb[D.offx * 3 + (int64_t)sc * nx + i] = lf * clo_own;
-            if (k == ke - 1) a.lint[D.offx * 3 + (int64_t)sc * nx + i] = lf * chi_own;
-        }
-        ls.f[QV + sc][c] = Lqr;
-        ls0.f[QV + sc][c] = s0v + Lc;
-        lph.f[3 + sc][c] = phv + dt * Lc;
+        __syncthreads();  // the next scalar restages sQ and sL
     }
+    if (!cell) return;
     a.Lmx[c] = lmx;
     a.Lmz[c] = lmz;
     if (ke == 0) {
@@ -1335,25 +1455,63 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
 // stage input (rho, rho_u, rho_w), to the face velocities (LU, LW), pres and
 // theta; adds the perturbation adjoints (rho' = s0 - s etc.) and the momentum
 // copies.
+//
+// Every neighbour value a cell reads lies within two cells of it in x and z:
+// the block stages the ten fields it gathers from (the four tendency
+// adjoints, rho, rho_u, rho_w, u and w at the faces, theta) for its tile with
+// a 2-cell halo in LDS, loads issued together, and the cells then read them
+// there; the arithmetic is the per-cell gather's, value for value.
 template <class RC>
 __global__ void __launch_bounds__(AD_X *AD_Z)
     ka_slow(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s, SSet<double> ls,
             SSet<double> ls0, SSet<double> lph, AdjW<RC> a, const double *__restrict__ Lrup) {
+    constexpr int HX = AD_X + 4, HZ = AD_Z + 4, NH = HX * HZ, NT = AD_X * AD_Z, NS = (NH + NT - 1) / NT;
+    constexpr int NF = 10;
+    __shared__ double sh[NF][HZ][HX];  // LRu, LRw, LRt, LRr, rho, rho_u, rho_w, u, w, theta
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
-    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+    const int i0 = t.y, k0 = t.z;
+    const int i = i0 + threadIdx.x, k = k0 + threadIdx.y, tid = threadIdx.y * AD_X + threadIdx.x;
     const int nx = D.nx, nz = D.nz, ke = nz - 1;
+    auto G = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
+    {
+        const double *src[NF] = {a.LRu, a.LRw, a.LRt, a.LRr, s.f[RHO], s.f[RHOU], s.f[RHOW], a.uf, a.wf, nullptr};
+        double x[NS][NF];
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int e = tid + u * NT, r = e / HX, kk = k0 - 2 + r;
+            const bool ok = e < NH && kk >= 0 && kk <= ke;
+            const int64_t q = ok ? G(kk, i0 - 2 + e - r * HX) : 0;
+#pragma unroll
+            for (int f = 0; f < NF - 1; ++f) x[u][f] = ok ? src[f][q] : 0.0;
+            x[u][NF - 1] = ok ? double(a.theta[q]) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int e = tid + u * NT;
+            if (e < NH)
+#pragma unroll
+                for (int f = 0; f < NF; ++f) (&sh[f][0][0])[e] = x[u][f];
+        }
+    }
+    __syncthreads();
     if (i >= nx || k >= nz) return;
     const double *zd = zv.z + D.offz, *zfd = zv.zf + D.offz, *dzd = zv.dz + D.offz;
     const double dx = D.dx, EPS = 1e-12, GRAV = K<double>::GRAV, nu = D.nu, rdx = 1.0 / dx;
     const double4 *gz = zv.geo + D.offz;  // per level: 1/dz, 1/(z_k+1 - z_k), 1/(dz_k + dz_k+1)
-    auto G = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
     const int64_t c = G(k, i);
-    const double *rho = s.f[RHO], *ru = s.f[RHOU], *rw = s.f[RHOW];
-    const RC *TH = a.theta;
-    auto U = [&](int kk, int ii) { return a.uf[G(kk, ii)]; };
-    auto W = [&](int kk, int ii) { return a.wf[G(kk, ii)]; };  // kk <= ke - 1
+    // the staged value of field f at (kk, ii), |kk - k| <= 2, |ii - i| <= 2
+    auto SH = [&](int f, int kk, int ii) { return sh[f][kk - k0 + 2][ii - i0 + 2]; };
+    auto LRU = [&](int kk, int ii) { return SH(0, kk, ii); };
+    auto LRT = [&](int kk, int ii) { return SH(2, kk, ii); };
+    auto LRR = [&](int kk, int ii) { return SH(3, kk, ii); };
+    auto RHOv = [&](int kk, int ii) { return SH(4, kk, ii); };
+    auto RU = [&](int kk, int ii) { return SH(5, kk, ii); };
+    auto RW = [&](int kk, int ii) { return SH(6, kk, ii); };
+    auto U = [&](int kk, int ii) { return SH(7, kk, ii); };
+    auto W = [&](int kk, int ii) { return SH(8, kk, ii); };  // kk <= ke - 1
+    auto THv = [&](int kk, int ii) { return SH(9, kk, ii); };
     double l_r = 0.0, l_u = 0.0, l_w = 0.0, LUc = 0.0, LWc = 0.0, lp = 0.0, lth = 0.0;
 
     // this cell's read-modify-write values, read before any store (stored at the end)
@@ -1379,14 +1537,14 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     p2 += a.LRt[c];
 
     // ---- R_rho (moist.jl:1079-1086)
-    l_u += -a.LRr[c] * rdx + a.LRr[G(k, i + 1)] * rdx;
-    if (k < ke) l_w += -a.LRr[c] * gz[k].x + a.LRr[G(k + 1, i)] * gz[k + 1].x;
+    l_u += -a.LRr[c] * rdx + LRR(k, i + 1) * rdx;
+    if (k < ke) l_w += -a.LRr[c] * gz[k].x + LRR(k + 1, i) * gz[k + 1].x;
 
     // ---- R_rho_u (moist.jl:1098-1179)
     // x flux F(j) between u-faces j and j+1, m = (rho_u_j + rho_u_j+1)/2
     for (int j = i - 2; j <= i + 1; ++j) {
-        const double lF = (a.LRu[G(k, j + 1)] - a.LRu[G(k, j)]) * rdx;
-        const double m = 0.5 * (ru[G(k, j)] + ru[G(k, j + 1)]);
+        const double lF = (LRU(k, j + 1) - LRU(k, j)) * rdx;
+        const double m = 0.5 * (RU(k, j) + RU(k, j + 1));
         const bool pos = m > EPS, neg = m < -EPS;
         LUc += lF * m * upw_coef(pos, neg, i - j, 0.5, 0.5);
         if (j == i - 1 || j == i) {
@@ -1397,8 +1555,8 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     // z flux at corner (kk+1/2, face i), kk = k-2..k+1 within 0..ke-1
     for (int kk = k - 2; kk <= k + 1; ++kk) {
         if (kk < 0 || kk > ke - 1) continue;
-        const double lF = a.LRu[G(kk + 1, i)] * gz[kk + 1].x - a.LRu[G(kk, i)] * gz[kk].x;
-        const double m = 0.5 * (rw[G(kk, i)] + rw[G(kk, i + 1)]);
+        const double lF = LRU(kk + 1, i) * gz[kk + 1].x - LRU(kk, i) * gz[kk].x;
+        const double m = 0.5 * (RW(kk, i) + RW(kk, i + 1));
         const bool pos = m > EPS && kk > 0, neg = !pos && m < -EPS && kk < ke - 1;
         const double rden = gz[kk].z;
         LUc += lF * m * upw_coef(pos, neg, k - kk, dzd[kk + 1] * rden, dzd[kk] * rden);
@@ -1406,8 +1564,8 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     // rho_w at (k, i) enters the corner fluxes (k+1/2, i) and (k+1/2, i-1)
     if (k <= ke - 1) {
         for (int ii = i - 1; ii <= i; ++ii) {
-            const double lF = a.LRu[G(k + 1, ii)] * gz[k + 1].x - a.LRu[G(k, ii)] * gz[k].x;
-            const double m = 0.5 * (rw[G(k, ii)] + rw[G(k, ii + 1)]);
+            const double lF = LRU(k + 1, ii) * gz[k + 1].x - LRU(k, ii) * gz[k].x;
+            const double m = 0.5 * (RW(k, ii) + RW(k, ii + 1));
             const bool pos = m > EPS && k > 0, neg = !pos && m < -EPS && k < ke - 1;
             const double rden = gz[k].z;
             const double g = upw_val(pos, neg, k > 0 ? U(k - 1, ii) : 0.0, U(k, ii), U(k + 1, ii),
@@ -1417,50 +1575,50 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     }
     // viscous x flux Dx(j) = (U_j+1 - U_j) / dx * rho_j+1, j = i-1, i
     for (int j = i - 1; j <= i; ++j) {
-        const double lD = nu * (a.LRu[G(k, j)] - a.LRu[G(k, j + 1)]) * rdx;
-        LUc += lD * rho[G(k, j + 1)] * rdx * (j == i - 1 ? 1.0 : -1.0);
+        const double lD = nu * (LRU(k, j) - LRU(k, j + 1)) * rdx;
+        LUc += lD * RHOv(k, j + 1) * rdx * (j == i - 1 ? 1.0 : -1.0);
         if (j == i - 1) l_r += lD * (U(k, i) - U(k, i - 1)) * rdx;
     }
     // viscous z flux Dz(kk) = (U_kk+1 - U_kk) / zz * rho at the corner, kk = k-1, k
     for (int kk = k - 1; kk <= k; ++kk) {
         if (kk < 0 || kk > ke - 1) continue;
-        const double lD = nu * (a.LRu[G(kk, i)] * gz[kk].x - a.LRu[G(kk + 1, i)] * gz[kk + 1].x);
-        const double rc = 0.25 * (rho[G(kk, i)] + rho[G(kk + 1, i)] + rho[G(kk, i + 1)] + rho[G(kk + 1, i + 1)]);
+        const double lD = nu * (LRU(kk, i) * gz[kk].x - LRU(kk + 1, i) * gz[kk + 1].x);
+        const double rc = 0.25 * (RHOv(kk, i) + RHOv(kk + 1, i) + RHOv(kk, i + 1) + RHOv(kk + 1, i + 1));
         LUc += lD * rc * gz[kk].y * (kk == k - 1 ? 1.0 : -1.0);
     }
     // rho(k, i) in the corner densities of the z viscous fluxes (kk, ii), kk = k-1, k; ii = i-1, i
     for (int kk = k - 1; kk <= k; ++kk) {
         if (kk < 0 || kk > ke - 1) continue;
         for (int ii = i - 1; ii <= i; ++ii) {
-            const double lD = nu * (a.LRu[G(kk, ii)] * gz[kk].x - a.LRu[G(kk + 1, ii)] * gz[kk + 1].x);
+            const double lD = nu * (LRU(kk, ii) * gz[kk].x - LRU(kk + 1, ii) * gz[kk + 1].x);
             l_r += lD * (U(kk + 1, ii) - U(kk, ii)) * gz[kk].y * 0.25;
         }
     }
     // pressure gradient (p_i+1 - p_i) / dx
-    lp += a.LRu[c] * rdx - a.LRu[G(k, i - 1)] * rdx;
+    lp += a.LRu[c] * rdx - LRU(k, i - 1) * rdx;
 
     // ---- R_rho_w (moist.jl:1190-1279), cells k' <= ke - 1
     const double sponge_base = (1.0 - D.zf_sponge) * D.H;
-    auto LRw = [&](int kk, int ii) { return a.LRw[G(kk, ii)]; };
+    auto LRw = [&](int kk, int ii) { return SH(1, kk, ii); };
     if (k <= ke - 1) {
         // fip(k, j): m = f2h(rho_u(k,j), rho_u(k+1,j), z_k, z_k+1), reads W(j-1..j+2)
         const double rzs = 1.0 / (zd[k] + zd[k + 1]);
         for (int j = i - 2; j <= i + 1; ++j) {
             const double lF = -LRw(k, j) * rdx;
-            const double m = (ru[G(k, j)] * zd[k + 1] + ru[G(k + 1, j)] * zd[k]) * rzs;
+            const double m = (RU(k, j) * zd[k + 1] + RU(k + 1, j) * zd[k]) * rzs;
             LWc += lF * m * upw_coef(m > EPS, m < -EPS, i - j, 0.5, 0.5);
         }
         // fim(k, j): m = f2h(rho_u(k,j-1), rho_u(k+1,j-1), dz), reads W(j-2..j+1): interface between j-1 and j
         const double rds = gz[k].z;
         for (int j = i - 1; j <= i + 2; ++j) {
             const double lF = LRw(k, j) * rdx;
-            const double m = (ru[G(k, j - 1)] * dzd[k + 1] + ru[G(k + 1, j - 1)] * dzd[k]) * rds;
+            const double m = (RU(k, j - 1) * dzd[k + 1] + RU(k + 1, j - 1) * dzd[k]) * rds;
             LWc += lF * m * upw_coef(m > EPS, m < -EPS, i - (j - 1), 0.5, 0.5);
         }
         // viscous x flux Ex(j) = (W_j+1 - W_j) / dx * rho at corner (k+1/2, j+1/2), j = i-1, i
         for (int j = i - 1; j <= i; ++j) {
             const double lE = nu * (LRw(k, j) - LRw(k, j + 1)) * rdx;
-            const double rc = 0.25 * (rho[G(k, j)] + rho[G(k + 1, j)] + rho[G(k, j + 1)] + rho[G(k + 1, j + 1)]);
+            const double rc = 0.25 * (RHOv(k, j) + RHOv(k + 1, j) + RHOv(k, j + 1) + RHOv(k + 1, j + 1));
             LWc += lE * rc * rdx * (j == i - 1 ? 1.0 : -1.0);
         }
         // sponge
@@ -1473,7 +1631,7 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
         if (kk < 0 || kk >= ke - 1) continue;
         const double lF = LRw(kk + 1, i) * gz[kk + 1].y - LRw(kk, i) * gz[kk].y;
         const double rds = gz[kk].z;
-        const double m = (rw[G(kk, i)] * dzd[kk + 1] + rw[G(kk + 1, i)] * dzd[kk]) * rds;
+        const double m = (RW(kk, i) * dzd[kk + 1] + RW(kk + 1, i) * dzd[kk]) * rds;
         const bool pos = m > EPS && kk > 0, neg = !pos && m < -EPS && kk < ke - 2;
         if (k <= ke - 1) LWc += lF * m * upw_coef(pos, neg, k - kk, dzd[kk + 1] * rds, dzd[kk] * rds);
         if (kk == k || kk == k - 1) {
@@ -1486,7 +1644,7 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     for (int kk = k - 1; kk <= k; ++kk) {
         if (kk < 0 || kk >= ke - 1) continue;
         const double lE = nu * (LRw(kk, i) * gz[kk].y - LRw(kk + 1, i) * gz[kk + 1].y);
-        if (k <= ke - 1) LWc += lE * rho[G(kk + 1, i)] * gz[kk + 1].x * (kk == k - 1 ? 1.0 : -1.0);
+        if (k <= ke - 1) LWc += lE * RHOv(kk + 1, i) * gz[kk + 1].x * (kk == k - 1 ? 1.0 : -1.0);
         if (kk == k - 1) l_r += lE * (W(k, i) - W(k - 1, i)) * gz[k].x;
     }
     // rho(k, i) in the corner densities of the x viscous fluxes Ex(k', j), k' = k-1, k; j = i-1, i
@@ -1503,14 +1661,14 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
         {
             const double lF = -LRw(kp, i) * rdx;
             const double rzs = 1.0 / (zd[kp] + zd[kp + 1]);
-            const double m = (ru[G(kp, i)] * zd[kp + 1] + ru[G(kp + 1, i)] * zd[kp]) * rzs;
+            const double m = (RU(kp, i) * zd[kp + 1] + RU(kp + 1, i) * zd[kp]) * rzs;
             const double g = upw_val(m > EPS, m < -EPS, W(kp, i - 1), W(kp, i), W(kp, i + 1), W(kp, i + 2), 0.5, 0.5);
             l_u += lF * g * (kp == k ? zd[kp + 1] : zd[kp]) * rzs;
         }
         {
             const double lF = LRw(kp, i + 1) * rdx;
             const double rds = gz[kp].z;
-            const double m = (ru[G(kp, i)] * dzd[kp + 1] + ru[G(kp + 1, i)] * dzd[kp]) * rds;
+            const double m = (RU(kp, i) * dzd[kp + 1] + RU(kp + 1, i) * dzd[kp]) * rds;
             const double g = upw_val(m > EPS, m < -EPS, W(kp, i - 1), W(kp, i), W(kp, i + 1), W(kp, i + 2), 0.5, 0.5);
             l_u += lF * g * (kp == k ? dzd[kp + 1] : dzd[kp]) * rds;
         }
@@ -1527,22 +1685,22 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
 
     // ---- R_rho_theta (moist.jl:1284-1332)
     for (int j = i - 2; j <= i + 1; ++j) {
-        const double lF = (a.LRt[G(k, j + 1)] - a.LRt[G(k, j)]) * rdx;
-        const double m = ru[G(k, j)];
+        const double lF = (LRT(k, j + 1) - LRT(k, j)) * rdx;
+        const double m = RU(k, j);
         const bool pos = m > EPS, neg = m < -EPS;
         lth += lF * m * upw_coef(pos, neg, i - j, 0.5, 0.5);
-        if (j == i) l_u += lF * upw_val(pos, neg, TH[G(k, i - 1)], TH[c], TH[G(k, i + 1)], TH[G(k, i + 2)], 0.5, 0.5);
+        if (j == i) l_u += lF * upw_val(pos, neg, THv(k, i - 1), THv(k, i), THv(k, i + 1), THv(k, i + 2), 0.5, 0.5);
     }
     for (int kk = k - 2; kk <= k + 1; ++kk) {
         if (kk < 0 || kk > ke - 1) continue;
-        const double lF = a.LRt[G(kk + 1, i)] * gz[kk + 1].x - a.LRt[G(kk, i)] * gz[kk].x;
-        const double m = rw[G(kk, i)];
+        const double lF = LRT(kk + 1, i) * gz[kk + 1].x - LRT(kk, i) * gz[kk].x;
+        const double m = RW(kk, i);
         const bool pos = m > EPS && kk > 0, neg = !pos && m < -EPS && kk < ke - 1;
         const double rden = gz[kk].z;
         lth += lF * m * upw_coef(pos, neg, k - kk, dzd[kk + 1] * rden, dzd[kk] * rden);
         if (kk == k)
-            l_w += lF * upw_val(pos, neg, kk > 0 ? TH[G(kk - 1, i)] : 0.0, TH[G(kk, i)], TH[G(kk + 1, i)],
-                                kk + 2 <= ke ? TH[G(kk + 2, i)] : 0.0, dzd[kk + 1] * rden, dzd[kk] * rden);
+            l_w += lF * upw_val(pos, neg, kk > 0 ? THv(kk - 1, i) : 0.0, THv(kk, i), THv(kk + 1, i),
+                                kk + 2 <= ke ? THv(kk + 2, i) : 0.0, dzd[kk + 1] * rden, dzd[kk] * rden);
     }
 
     ls0.f[RHO][c] = s0r;
