@@ -671,33 +671,72 @@ template <class RC> struct WpCoef {
 
 // (1) Adjoint of rho_w' after the solve: the momentum average, the rho'
 // and rho_theta' vertical updates (with the Q5 branch tests of the
-// forward); theta adjoint of the rho_theta' vertical flux.  Per cell.
+// forward); theta adjoint of the rho_theta' vertical flux.  Per cell.  A
+// cell reads levels k-2 .. k+2 of its column: the block stages the fields
+// it reads for its tile and two levels either side in LDS, in their own
+// types, and the cells evaluate the per-cell expressions on them.
 template <class RC>
 __global__ void __launch_bounds__(AD_X *AD_Z)
     ka_ac_v1(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s, AdjW<RC> a,
              const RC *__restrict__ rwp_n, int div, int nsub) {
+    constexpr int HZ = AD_Z + 4, NH = HZ * AD_X, NT = AD_X * AD_Z, NS = (NH + NT - 1) / NT;
+    __shared__ RC sR[2][HZ][AD_X];      // rwp_n, theta
+    __shared__ double sD[4][HZ][AD_X];  // rho_w, Lrtp, Lrp, Lmz
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
-    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+    const int i0 = t.y, k0 = t.z;
+    const int i = i0 + threadIdx.x, k = k0 + threadIdx.y, tid = threadIdx.y * AD_X + threadIdx.x;
     const int nx = D.nx, nz = D.nz, ke = nz - 1;
+    {
+        RC xr[NS][2];
+        double xd[NS][4];
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int e = tid + u * NT, r = e / AD_X, kk = k0 - 2 + r, ii = i0 + e - r * AD_X;
+            const bool ok = e < NH && kk >= 0 && kk <= ke && ii < nx;
+            const int64_t q = ok ? D.off2 + (int64_t)kk * nx + ii : 0;
+            xr[u][0] = ok ? rwp_n[q] : RC(0);
+            xr[u][1] = ok ? a.theta[q] : RC(0);
+            xd[u][0] = ok ? s.f[RHOW][q] : 0.0;
+            xd[u][1] = ok ? a.Lrtp[q] : 0.0;
+            xd[u][2] = ok ? a.Lrp[q] : 0.0;
+            xd[u][3] = ok ? a.Lmz[q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int e = tid + u * NT;
+            if (e < NH) {
+#pragma unroll
+                for (int f = 0; f < 2; ++f) (&sR[f][0][0])[e] = xr[u][f];
+#pragma unroll
+                for (int f = 0; f < 4; ++f) (&sD[f][0][0])[e] = xd[u][f];
+            }
+        }
+    }
+    __syncthreads();
     if (i >= nx || k >= nz) return;
     const double *dzd = zv.dz + D.offz;
     const double4 *gz = zv.geo + D.offz;  // per level: 1/dz, 1/(z_k+1 - z_k), 1/(dz_k + dz_k+1)
     const double EPS = 1e-12, rns = 1.0 / double(nsub);
     const double dt_s = D.dt / double(div) / double(nsub);
     auto G = [&](int kk) -> int64_t { return D.off2 + (int64_t)kk * nx + i; };
-    const RC *TH = a.theta;
-    const double *rw = s.f[RHOW];
-    auto THv = [&](int kk) { return kk >= 0 && kk <= ke ? TH[G(kk)] : 0.0; };
+    // staged values of level kk of this column
+    auto RWN = [&](int kk) { return sR[0][kk - k0 + 2][threadIdx.x]; };
+    auto TH = [&](int kk) { return sR[1][kk - k0 + 2][threadIdx.x]; };
+    auto RW = [&](int kk) { return sD[0][kk - k0 + 2][threadIdx.x]; };
+    auto LRTP = [&](int kk) { return sD[1][kk - k0 + 2][threadIdx.x]; };
+    auto LRP = [&](int kk) { return sD[2][kk - k0 + 2][threadIdx.x]; };
+    auto LMZ = [&](int kk) { return sD[3][kk - k0 + 2][threadIdx.x]; };
+    auto THv = [&](int kk) { return kk >= 0 && kk <= ke ? TH(kk) : 0.0; };
     // face f+1/2 as fkp of cell f: positive branch tests rwp_n[f], negative s.rho_w[f]
     auto brp = [&](int f, bool &pos, bool &neg) {
-        pos = rwp_n[G(f)] > EPS && f > 0;
-        neg = !pos && rw[G(f)] < -EPS && f < ke - 1;
+        pos = RWN(f) > EPS && f > 0;
+        neg = !pos && RW(f) < -EPS && f < ke - 1;
     };
     // face f+1/2 as fkm of cell f+1: both branches test s.rho_w[f]
     auto brm = [&](int f, bool &pos, bool &neg) {
-        const double sw = rw[G(f)];
+        const double sw = RW(f);
         pos = sw > EPS && f + 1 > 1;
         neg = !pos && sw < -EPS && f + 1 < ke;
     };
@@ -707,12 +746,12 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
         if (f < 0 || f > ke - 1) continue;
         const double rden = gz[f].z;
         const double w0 = dzd[f + 1] * rden, w1 = dzd[f] * rden;
-        const double w = rwp_n[G(f)];
+        const double w = RWN(f);
         bool pos, neg;
         brp(f, pos, neg);
-        lth += -dt_s * gz[f].x * a.Lrtp[G(f)] * w * upw_coef(pos, neg, k - f, w0, w1);
+        lth += -dt_s * gz[f].x * LRTP(f) * w * upw_coef(pos, neg, k - f, w0, w1);
         brm(f, pos, neg);
-        lth += dt_s * gz[f + 1].x * a.Lrtp[G(f + 1)] * w * upw_coef(pos, neg, k - f, w0, w1);
+        lth += dt_s * gz[f + 1].x * LRTP(f + 1) * w * upw_coef(pos, neg, k - f, w0, w1);
     }
     double lrwp = 0.0;
     if (k <= ke - 1) {
@@ -723,15 +762,13 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
         const double gp = upw_val(pos, neg, THv(k - 1), THv(k), THv(k + 1), THv(k + 2), w0, w1);
         brm(k, pos, neg);
         const double gm = upw_val(pos, neg, THv(k - 1), THv(k), THv(k + 1), THv(k + 2), w0, w1);
-        const int64_t q = G(k), qp = G(k + 1);
-        lrwp = a.Lmz[q] * rns - dt_s * gz[k].x * a.Lrtp[q] * gp + dt_s * gz[k + 1].x * a.Lrtp[qp] * gm -
-               dt_s * gz[k].x * a.Lrp[q] + dt_s * gz[k + 1].x * a.Lrp[qp];
+        lrwp = LMZ(k) * rns - dt_s * gz[k].x * LRTP(k) * gp + dt_s * gz[k + 1].x * LRTP(k + 1) * gm -
+               dt_s * gz[k].x * LRP(k) + dt_s * gz[k + 1].x * LRP(k + 1);
     }
     // read-modify-writes last, after every load of the kernel
     a.Lth[G(k)] = lth0 + lth;
     if (k <= ke - 1) a.Lrwp[G(k)] = lrwp0 + lrwp;
 }
-
 // Inclusive scans over the CH chunks of one column inside a wave (lane =
 // column + CW * chunk, CW = 64 / CH): affine maps x -> A x + B from the
 // bottom chunk up or from the top chunk down, and 2x2 matrices (Moebius
@@ -977,81 +1014,123 @@ __global__ void __launch_bounds__(256)
 // (3) Matrix and right-hand-side adjoints of the solve, the adjoint of
 // rho_w' before it, drwp = dt_s (R_rho_w - rt2h buoy + grav) and the explicit
 // part (rp_tp = rp + f inc, rp += inc; the same for rho_theta').  Per cell.
+// A cell reads levels k-2 .. k+1 of its column and, for the off-centred
+// rho_theta', the columns i-1 and i+1: the block stages the eight fields it
+// reads for its tile (levels k0-2 .. k0+AD_Z, columns i0-1 .. i0+AD_X) in
+// LDS, in their own types, and the cells evaluate the per-cell expressions
+// on them.
 template <class RC>
 __global__ void __launch_bounds__(AD_X *AD_Z)
     ka_ac_v2(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, AdjW<RC> a,
              const RC *__restrict__ rtp_s, const RC *__restrict__ rup_n, const RC *__restrict__ rwp_s,
              const RC *__restrict__ rwp_n, int div, int nsub) {
+    constexpr int HX = AD_X + 2, HZ = AD_Z + 3, NH = HX * HZ, NT = AD_X * AD_Z, NS = (NH + NT - 1) / NT;
+    __shared__ double sY[HZ][HX];
+    __shared__ RC sF[7][HZ][HX];  // rwp_n, rwp_s, theta, rt2pres, rup_n, R_rho_theta, rtp_s
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
-    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+    const int i0 = t.y, k0 = t.z;
+    const int i = i0 + threadIdx.x, k = k0 + threadIdx.y, tid = threadIdx.y * AD_X + threadIdx.x;
     const int nx = D.nx, nz = D.nz, n = nz - 1;
-    if (i >= nx || k >= nz || n <= 0) return;
-    const double *zd = zv.z + D.offz, *dzd = zv.dz + D.offz;
+    if (n <= 0) return;  // uniform per block
+    auto Gx = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
+    {
+        const RC *src[7] = {rwp_n, rwp_s, a.theta, a.rt2, rup_n, a.R_rt, rtp_s};
+        double y[NS];
+        RC x[NS][7];
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int e = tid + u * NT, r = e / HX, kk = k0 - 2 + r;
+            const bool ok = e < NH && kk >= 0 && kk <= n;
+            const int64_t q = ok ? Gx(kk, i0 - 1 + e - r * HX) : 0;
+            y[u] = ok ? a.Y[q] : 0.0;
+#pragma unroll
+            for (int f = 0; f < 7; ++f) x[u][f] = ok ? src[f][q] : RC(0);
+        }
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int e = tid + u * NT;
+            if (e < NH) {
+                (&sY[0][0])[e] = y[u];
+#pragma unroll
+                for (int f = 0; f < 7; ++f) (&sF[f][0][0])[e] = x[u][f];
+            }
+        }
+    }
+    __syncthreads();
+    if (i >= nx || k >= nz) return;
+    const double *dzd = zv.dz + D.offz;
     const double GRAV = K<double>::GRAV, dx = D.dx, rdx = 1.0 / dx;
     const double dt_s = D.dt / double(div) / double(nsub);
     const double fact_tp = 0.5 * (1.0 + D.beta), fact_tm = 0.5 * (1.0 - D.beta);
     const double fact = fact_tm / fact_tp;
     const double4 *gz = zv.geo + D.offz;
-    const WpCoef<RC> W{D, zd, dzd, gz, a.theta, a.rt2, nx, i, (dt_s * fact_tp) * (dt_s * fact_tp)};
-    const double dt_tp2 = W.dt_tp2;
-    const RC *RT = a.rt2, *TH = a.theta;
-    auto G = [&](int kk) -> int64_t { return W.G(kk); };
-    auto Gx = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
-    auto yv = [&](int j) { return j >= 0 && j <= n - 1 ? a.Y[G(j)] : 0.0; };
-    auto xv = [&](int j) { return j >= 0 && j <= n - 1 ? rwp_n[G(j)] : 0.0; };
-    auto La = [&](int j) { return j >= 1 && j <= n - 1 ? -yv(j) * xv(j - 1) - fact * yv(j) * rwp_s[G(j)] : 0.0; };
-    auto Lb = [&](int j) { return j >= 0 && j <= n - 1 ? -yv(j) * xv(j) - fact * yv(j) * rwp_s[G(j)] : 0.0; };
-    auto Lc = [&](int j) {
-        return j >= 0 && j < n - 1 ? -yv(j) * xv(j + 1) - fact * yv(j) * rwp_s[G(j + 1)] : 0.0;
-    };
+    const double dt_tp2 = (dt_s * fact_tp) * (dt_s * fact_tp);
+    // staged values of level kk (column i, or ii for theta and rup_n)
+    auto SF = [&](int f, int kk, int ii) -> RC { return sF[f][kk - k0 + 2][ii - i0 + 1]; };
+    auto RWN = [&](int kk) { return SF(0, kk, i); };
+    auto RWS = [&](int kk) { return SF(1, kk, i); };
+    auto TH = [&](int kk, int ii) { return SF(2, kk, ii); };
+    auto RT = [&](int kk) { return SF(3, kk, i); };
+    auto RUN = [&](int kk, int ii) { return SF(4, kk, ii); };
+    auto RRT = [&](int kk) { return SF(5, kk, i); };
+    auto RTS = [&](int kk) { return SF(6, kk, i); };
+    // the w-p system's coefficients of row kk (implicit_correction!)
+    auto th = [&](int kk) { return (TH(kk, i) * dzd[kk + 1] + TH(kk + 1, i) * dzd[kk]) * gz[kk].z; };
+    auto Wa = [&](int kk) { return -dt_tp2 * (RT(kk) * th(kk - 1) * (gz[kk].y * gz[kk].x) - gz[kk].w); };
+    auto Wb = [&](int kk) { return dt_tp2 * (RT(kk) * gz[kk].x + RT(kk + 1) * gz[kk + 1].x) * th(kk) * gz[kk].y; };
+    auto Wc = [&](int kk) { return -dt_tp2 * (RT(kk + 1) * th(kk + 1) * (gz[kk].y * gz[kk + 1].x) + gz[kk].w); };
+    auto yv = [&](int j) { return j >= 0 && j <= n - 1 ? sY[j - k0 + 2][i - i0 + 1] : 0.0; };
+    auto xv = [&](int j) { return j >= 0 && j <= n - 1 ? double(RWN(j)) : 0.0; };
+    auto La = [&](int j) { return j >= 1 && j <= n - 1 ? -yv(j) * xv(j - 1) - fact * yv(j) * RWS(j) : 0.0; };
+    auto Lb = [&](int j) { return j >= 0 && j <= n - 1 ? -yv(j) * xv(j) - fact * yv(j) * RWS(j) : 0.0; };
+    auto Lc = [&](int j) { return j >= 0 && j < n - 1 ? -yv(j) * xv(j + 1) - fact * yv(j) * RWS(j + 1) : 0.0; };
     // adjoint of the face theta th_j (j = 0..n-1)
     auto Lthc = [&](int j) {
         if (j < 0 || j > n - 1) return 0.0;
-        double v = Lb(j) * dt_tp2 * (RT[G(j)] * gz[j].x + RT[G(j + 1)] * gz[j + 1].x) * gz[j].y;
-        if (j + 1 <= n - 1) v += La(j + 1) * (-dt_tp2 * RT[G(j + 1)] * (gz[j + 1].y * gz[j + 1].x));
-        if (j >= 1) v += Lc(j - 1) * (-dt_tp2 * RT[G(j)] * (gz[j - 1].y * gz[j].x));
+        double v = Lb(j) * dt_tp2 * (RT(j) * gz[j].x + RT(j + 1) * gz[j + 1].x) * gz[j].y;
+        if (j + 1 <= n - 1) v += La(j + 1) * (-dt_tp2 * RT(j + 1) * (gz[j + 1].y * gz[j + 1].x));
+        if (j >= 1) v += Lc(j - 1) * (-dt_tp2 * RT(j) * (gz[j - 1].y * gz[j].x));
         return v;
     };
     // off-centred rho_theta' of the forward explicit part
     auto rtp_tp = [&](int kk) {
-        const int64_t q = G(kk);
-        const double fip = (TH[Gx(kk, i + 1)] + TH[q]) * 0.5 * rup_n[q];
-        const double fim = (TH[q] + TH[Gx(kk, i - 1)]) * 0.5 * rup_n[Gx(kk, i - 1)];
-        const double inc = dt_s * (a.R_rt[q] + (-(fip - fim) * rdx));
-        return rtp_s[q] + fact_tp * inc;
+        const double fip = (TH(kk, i + 1) + TH(kk, i)) * 0.5 * RUN(kk, i);
+        const double fim = (TH(kk, i) + TH(kk, i - 1)) * 0.5 * RUN(kk, i - 1);
+        const double inc = dt_s * (RRT(kk) + (-(fip - fim) * rdx));
+        return RTS(kk) + fact_tp * inc;
     };
-    const int64_t q = G(k);
+    const int64_t q = Gx(k, i);
     // this cell's read-modify-write values, read before any store
     const double Lrt0 = a.Lrt[q], Lth0 = a.Lth[q], lr = a.Lrp[q], lt = a.Lrtp[q], LRr0 = a.LRr[q],
                  LRt0 = a.LRt[q], LRw0 = k <= n - 1 ? a.LRw[q] : 0.0;
     double lrt = 0.0, lth = 0.0, Lrp_tp = 0.0, Lrtp_tp = 0.0, lws = 0.0, lRw = 0.0;
     if (k <= n - 1) {
         const double y = yv(k);
-        const double thk = W.th(k), rz1 = gz[k].y, rzd = rz1 * gz[k].x;
+        const double thk = th(k), rz1 = gz[k].y, rzd = rz1 * gz[k].x;
         lrt += Lb(k) * dt_tp2 * thk * rzd;
-        if (k >= 1) lrt += La(k) * (-dt_tp2 * W.th(k - 1) * rzd);
+        if (k >= 1) lrt += La(k) * (-dt_tp2 * th(k - 1) * rzd);
         const double buoy = (rtp_tp(k + 1) - rtp_tp(k)) * rz1;
         lrt += -0.5 * dt_s * y * buoy;
-        const double rt2h = (RT[q] + RT[G(k + 1)]) * 0.5;
+        const double rt2h = (RT(k) + RT(k + 1)) * 0.5;
         Lrtp_tp += dt_s * y * rt2h * rz1;
         Lrp_tp += -dt_s * y * GRAV * 0.5;
         lRw = dt_s * y;
         lth += Lthc(k) * dzd[k + 1] * gz[k].z;
         // rho_w' before the solve
-        lws = y * (1.0 - fact * W.b(k));
-        if (k >= 1) lws += -fact * W.a(k) * y - fact * W.c(k - 1) * yv(k - 1);
+        lws = y * (1.0 - fact * Wb(k));
+        if (k >= 1) lws += -fact * Wa(k) * y - fact * Wc(k - 1) * yv(k - 1);
     }
     if (k >= 1) {
         const int kb = k - 1;
         const double yb = yv(kb);
         const double rz1 = gz[kb].y, rzd = rz1 * gz[k].x;
-        lrt += Lb(kb) * dt_tp2 * W.th(kb) * rzd;
-        if (kb < n - 1) lrt += Lc(kb) * (-dt_tp2 * W.th(k) * rzd);
+        lrt += Lb(kb) * dt_tp2 * th(kb) * rzd;
+        if (kb < n - 1) lrt += Lc(kb) * (-dt_tp2 * th(k) * rzd);
         const double buoy = (rtp_tp(k) - rtp_tp(kb)) * rz1;
         lrt += -0.5 * dt_s * yb * buoy;
-        const double rt2h = (RT[G(kb)] + RT[q]) * 0.5;
+        const double rt2h = (RT(kb) + RT(k)) * 0.5;
         Lrtp_tp += -dt_s * yb * rt2h * rz1;
         Lrp_tp += -dt_s * yb * GRAV * 0.5;
         lth += Lthc(kb) * dzd[kb] * gz[kb].z;
@@ -1395,50 +1474,92 @@ __global__ void __launch_bounds__(64)
 // Per cell.  Lrup_in: adjoint of rho_u' after this substep (from later
 // substeps and the final update); Lrup_out: adjoint before it.  pend: the
 // damping term's adjoint for rho_theta' at the start of the previous substep.
+// A cell reads columns i-1 .. i+1 of its level: the block stages the fields
+// it reads for its tile and a column either side in LDS, in their own types,
+// and the cells evaluate the per-cell expressions on them.
 template <class RC>
 __global__ void __launch_bounds__(AD_X *AD_Z)
     ka_ac_hor(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, AdjW<RC> a,
               const double *__restrict__ Lrup_in, double *__restrict__ Lrup_out, const RC *__restrict__ rtp_s,
               const RC *__restrict__ rtp_prev, const RC *__restrict__ rup_n, int div, int nsub, int first_substep) {
+    constexpr int HX = AD_X + 2, NH = HX * AD_Z, NT = AD_X * AD_Z, NS = (NH + NT - 1) / NT;
+    __shared__ RC sR[5][AD_Z][HX];      // theta, rt2pres, rtp_s, rtp_prev, rup_n
+    __shared__ double sD[4][AD_Z][HX];  // Linc_t, Linc_r, Lrup_in, Lmx
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
-    const int i = t.y + threadIdx.x, k = t.z + threadIdx.y;
+    const int i0 = t.y, k0 = t.z;
+    const int i = i0 + threadIdx.x, k = k0 + threadIdx.y, tid = threadIdx.y * AD_X + threadIdx.x;
     const int nx = D.nx, nz = D.nz;
+    auto Gk = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
+    {
+        const RC *sr[5] = {a.theta, a.rt2, rtp_s, rtp_prev, rup_n};
+        const double *sd[4] = {a.Linc_t, a.Linc_r, Lrup_in, a.Lmx};
+        RC xr[NS][5];
+        double xd[NS][4];
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int e = tid + u * NT, r = e / HX, kk = k0 + r;
+            const bool ok = e < NH && kk < nz;
+            const int64_t q = ok ? Gk(kk, i0 - 1 + e - r * HX) : 0;
+#pragma unroll
+            for (int f = 0; f < 5; ++f) xr[u][f] = ok ? sr[f][q] : RC(0);
+#pragma unroll
+            for (int f = 0; f < 4; ++f) xd[u][f] = ok ? sd[f][q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int e = tid + u * NT;
+            if (e < NH) {
+#pragma unroll
+                for (int f = 0; f < 5; ++f) (&sR[f][0][0])[e] = xr[u][f];
+#pragma unroll
+                for (int f = 0; f < 4; ++f) (&sD[f][0][0])[e] = xd[u][f];
+            }
+        }
+    }
+    __syncthreads();
     if (i >= nx || k >= nz) return;
     const double dx = D.dx, gam = D.gamma_d, rdx = 1.0 / dx;
     const double dt_s = D.dt / double(div) / double(nsub), rns = 1.0 / double(nsub);
-    auto G = [&](int ii) -> int64_t { return D.off2 + (int64_t)k * nx + wrapn(ii, nx); };
-    const RC *TH = a.theta, *RT = a.rt2;
+    // staged values of column ii (i-1 .. i+1) of this level
+    const int r = threadIdx.y;
+    auto TH = [&](int ii) { return sR[0][r][ii - i0 + 1]; };
+    auto RT = [&](int ii) { return sR[1][r][ii - i0 + 1]; };
+    auto RTS = [&](int ii) { return sR[2][r][ii - i0 + 1]; };
+    auto RTP = [&](int ii) { return sR[3][r][ii - i0 + 1]; };
+    auto RUN = [&](int ii) { return sR[4][r][ii - i0 + 1]; };
+    auto LIT = [&](int ii) { return sD[0][r][ii - i0 + 1]; };
+    auto LIR = [&](int ii) { return sD[1][r][ii - i0 + 1]; };
+    auto LRUP = [&](int ii) { return sD[2][r][ii - i0 + 1]; };
+    auto LMX = [&](int ii) { return sD[3][r][ii - i0 + 1]; };
     // adjoint of rho_u'(j) after the substep, all contributions of this substep included
     auto lrup = [&](int j) {
-        const int64_t q = G(j), qp = G(j + 1);
-        const double lF = dt_s * rdx * (a.Linc_t[qp] - a.Linc_t[q]);
-        return Lrup_in[q] + a.Lmx[q] * rns - dt_s * rdx * a.Linc_r[q] + dt_s * rdx * a.Linc_r[qp] +
-               lF * 0.5 * (TH[q] + TH[qp]);
+        const double lF = dt_s * rdx * (LIT(j + 1) - LIT(j));
+        return LRUP(j) + LMX(j) * rns - dt_s * rdx * LIR(j) + dt_s * rdx * LIR(j + 1) + lF * 0.5 * (TH(j) + TH(j + 1));
     };
-    const int64_t c = G(i), cm = G(i - 1), cp = G(i + 1);
+    const int64_t c = Gk(k, i);
     // this cell's read-modify-write values, read before any store
     const double Lth0 = a.Lth[c], LRu0 = a.LRu[c], Lrt0 = a.Lrt[c], Lrtp0 = a.Lrtp[c], pend0 = a.pend[c];
     // theta adjoint of the rho_theta' flux (faces i+1/2 and i-1/2)
     double lth;
     {
-        const double lFi = dt_s * rdx * (a.Linc_t[cp] - a.Linc_t[c]);
-        const double lFm = dt_s * rdx * (a.Linc_t[c] - a.Linc_t[cm]);
-        lth = 0.5 * (lFi * rup_n[c] + lFm * rup_n[cm]);
+        const double lFi = dt_s * rdx * (LIT(i + 1) - LIT(i));
+        const double lFm = dt_s * rdx * (LIT(i) - LIT(i - 1));
+        lth = 0.5 * (lFi * RUN(i) + lFm * RUN(i - 1));
     }
     const double Li = lrup(i), Lm = lrup(i - 1);
     // rho_theta' at the start of this substep: add the damping adjoint from the next substep
     double lt = Lrtp0 + pend0;
     // rho_u' update: rup_n = rup_s + dt_s (R_rho_u - rt2h ((1 + g) gx - g gxo))
     auto Gx = [&](int j) {
-        const double gx = (rtp_s[G(j + 1)] - rtp_s[G(j)]) * rdx;
-        const double gxo = (rtp_prev[G(j + 1)] - rtp_prev[G(j)]) * rdx;
+        const double gx = (RTS(j + 1) - RTS(j)) * rdx;
+        const double gxo = (RTP(j + 1) - RTP(j)) * rdx;
         return gx + gam * (gx - gxo);
     };
     const double lrt = -0.5 * dt_s * (Li * Gx(i) + Lm * Gx(i - 1));
-    const double lGi = -dt_s * (RT[c] + RT[cp]) * 0.5 * Li;
-    const double lGm = -dt_s * (RT[cm] + RT[c]) * 0.5 * Lm;
+    const double lGi = -dt_s * (RT(i) + RT(i + 1)) * 0.5 * Li;
+    const double lGm = -dt_s * (RT(i - 1) + RT(i)) * 0.5 * Lm;
     lt += (1.0 + gam) * rdx * (lGm - lGi);
     const double lo = -gam * rdx * (lGm - lGi);
     if (first_substep) lt += lo;
@@ -1792,13 +1913,18 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
 }
 
 // ---- Kessler microphysics of one cell with dual numbers (moist.jl:554-671) -------
+// In two parts: the saturation adjustment (three iterations), which the
+// three Kessler parameters do not enter, and autoconversion, accretion and
+// rain evaporation, which they do.  KessSat carries what the second part
+// reads of the first.
+template <class V> struct KessSat {
+    V rho_k, theta_k, qv, qc, qr, qd, qsat, dqv, dqc, dth;
+};
 template <class V>
-__device__ __forceinline__ void kessler_v(const Dom &D, V rho_k, V rt, V rqv, V rqc, V rqr, V Ka, V Qc0, V Kacc,
-                                          V out[4]) {
+__device__ __forceinline__ KessSat<V> kessler_sat(const Dom &D, V rho_k, V rt, V rqv, V rqc, V rqr) {
     const K<double> c;
     const double dt = D.dt;
-    const bool do_auto = D.flags & F_AUTO, do_accr = D.flags & F_ACCR, do_revap = D.flags & F_REVAP;
-    V dqv(0.0), dqc(0.0), dqr(0.0), dth(0.0);
+    V dqv(0.0), dqc(0.0), dth(0.0);
     V rinv = V(1.0) / rho_k;
     V theta_k = rt * rinv;
     V qv = rqv * rinv, qc = rqc * rinv, qr = rqr * rinv;
@@ -1846,6 +1972,15 @@ __device__ __forceinline__ void kessler_v(const Dom &D, V rho_k, V rt, V rqv, V 
     dqv = dqv - rate;
     dqc = dqc + rate;
     dth = dth + rate * lhe;
+    return KessSat<V>{rho_k, theta_k, qv, qc, qr, qd, qsat, dqv, dqc, dth};
+}
+template <class V>
+__device__ __forceinline__ void kessler_rest(const Dom &D, const KessSat<V> &k0, V Ka, V Qc0, V Kacc, V out[4]) {
+    const K<double> c;
+    const double dt = D.dt;
+    const bool do_auto = D.flags & F_AUTO, do_accr = D.flags & F_ACCR, do_revap = D.flags & F_REVAP;
+    const V rho_k = k0.rho_k, theta_k = k0.theta_k, qv = k0.qv, qd = k0.qd, qsat = k0.qsat;
+    V qc = k0.qc, qr = k0.qr, dqv = k0.dqv, dqc = k0.dqc, dqr(0.0), dth = k0.dth;
     if (do_auto) {
         V ra = Ka * vmax(V(0.0), qc - Qc0);
         ra = vmin(ra, qc / dt);
@@ -1876,17 +2011,22 @@ __device__ __forceinline__ void kessler_v(const Dom &D, V rho_k, V rt, V rqv, V 
         re = re * rho_k;
         dqr = dqr - re;
         dqv = dqv + re;
-        r = c.Rd * qd + c.Rv * qv;
-        cp = c.Cpd * qd + c.Cpv * qv + c.Cpl * (qc + qr);
-        cv = cp - r;
-        Pi = d_pow(r * rho_k * theta_k / c.P0, r / cv);
-        lhe = c.Lv / (cp * Pi);
+        const V r = c.Rd * qd + c.Rv * qv;
+        const V cp = c.Cpd * qd + c.Cpv * qv + c.Cpl * (qc + qr);
+        const V cv = cp - r;
+        const V Pi = d_pow(r * rho_k * theta_k / c.P0, r / cv);
+        const V lhe = c.Lv / (cp * Pi);
         dth = dth - re * lhe;
     }
     out[0] = dqv;
     out[1] = dqc;
     out[2] = dqr;
     out[3] = dth;
+}
+// The saturation part's values as constants of another dual type.
+template <class W, class V> __device__ __forceinline__ KessSat<W> kessler_const(const KessSat<V> &k) {
+    return KessSat<W>{W(val(k.rho_k)), W(val(k.theta_k)), W(val(k.qv)), W(val(k.qc)), W(val(k.qr)), W(val(k.qd)),
+                      W(val(k.qsat)), W(val(k.dqv)), W(val(k.dqc)), W(val(k.dth))};
 }
 
 // ---- surface fluxes (moist.jl:710-752): adjoint, one thread per column ---------
@@ -2150,13 +2290,18 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     }
     if (mp) {
         // partials w.r.t. the five state fields, then w.r.t. the three parameters
-        // (two narrower dual evaluations of the same formula keep the registers in bounds)
+        // (two narrower dual evaluations keep the registers in bounds)
         const double lo[4] = {lph.f[3][c], lph.f[4][c], lph.f[5][c], lph.f[2][c]};
+        // (the saturation adjustment, which the parameters do not enter, runs
+        // once: the parameter pass starts from its values)
+        KessSat<Dl<3>> satK;
         {
             typedef Dl<5> V;
             V out[4];
-            kessler_v<V>(D, V::var(rho[c], 0), V::var(rt[c], 1), V::var(s.f[QV][c], 2), V::var(s.f[QC][c], 3),
-                         V::var(rqr[c], 4), V(D.K_auto), V(D.Qc_threshold), V(D.K_accr), out);
+            const KessSat<V> sat = kessler_sat<V>(D, V::var(rho[c], 0), V::var(rt[c], 1), V::var(s.f[QV][c], 2),
+                                                  V::var(s.f[QC][c], 3), V::var(rqr[c], 4));
+            satK = kessler_const<Dl<3>>(sat);
+            kessler_rest<V>(D, sat, V(D.K_auto), V(D.Qc_threshold), V(D.K_accr), out);
             double g[5];
 #pragma unroll
             for (int j = 0; j < 5; ++j) {
@@ -2173,8 +2318,7 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
         {
             typedef Dl<3> V;
             V out[4];
-            kessler_v<V>(D, V(rho[c]), V(rt[c]), V(s.f[QV][c]), V(s.f[QC][c]), V(rqr[c]), V::var(D.K_auto, 0),
-                         V::var(D.Qc_threshold, 1), V::var(D.K_accr, 2), out);
+            kessler_rest<V>(D, satK, V::var(D.K_auto, 0), V::var(D.Qc_threshold, 1), V::var(D.K_accr, 2), out);
             double g[3];
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
