@@ -144,7 +144,8 @@ int moist_use_graph(moist_handle *h, int on);
  * diagnostics of moist_download_phys at every step; 0 skips those writes.
  * MOIST_OPT_ADJ_RECORDS selects how moist_adjoint_run keeps the forward
  * records of its reverse sweep: 0 (default) whole-segment records when they
- * take at most a quarter of the free device memory, else per step; 1 per
+ * fit in the free device memory less a margin (a tenth of it, at least
+ * 8 GiB), else per step; 1 per
  * step (each step's record recomputed from the segment's states); 2 whole
  * segment.  Both give the same gradient.  MOIST_OPT_ADJ_COLUMN_KERNEL
  * (default 1) runs the vertical part of each reverse acoustic substep as one

@@ -784,12 +784,15 @@ int adj_alloc(moist_handle *h, int nsteps, int ckpt) {
     }
     A.rb_bytes = (rec_in_bytes(h) + (3 * (size_t)NREC + 3 * (size_t)nrec_sub) * n2 * A.rec_elem + 255) / 256 * 256;
     {
-        // records of a whole segment if they take at most a quarter of the free memory
+        // records of a whole segment when they fit in the free memory with a
+        // margin (a tenth of it, at least 8 GiB): one recording pass per step
+        // instead of a state pass and a recording pass
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) == hipSuccess) {
             const size_t need = A.rb_bytes * (size_t)ckpt;
             const size_t other = (A.cp_stride * A.ncp + 60 * n2) * sizeof(double) + A.seg_bytes * (size_t)ckpt;
-            A.segrec = need + other < fr / 4 ? 1 : 0;
+            const size_t margin = std::max(fr / 10, (size_t)8 << 30);
+            A.segrec = fr > margin && need + other < fr - margin ? 1 : 0;
         }
         if (A.mode) A.segrec = A.mode == 2 ? 1 : 0;
         A.nblocks = A.segrec ? ckpt : 1;
