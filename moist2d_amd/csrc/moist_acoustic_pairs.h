@@ -18,9 +18,25 @@
 // substep holds, and written to the stage output (x-fastest, AcBuf::fin_*)
 // with its NaN checks (moist.jl:1543-1556); the final perturbations, which
 // nothing reads after this substep, are not stored.
+//
+// Addressing: every field is read and written through the member's base
+// pointer and 32-bit byte offsets (ldb/ldb2/stb/stb2); the thread's level
+// pairs at columns i, i-1 and i+1 have one offset register each, shared by
+// all the fields.
 
+// 1/x in FP64 from the hardware reciprocal and two Newton steps (within an
+// ulp of the IEEE quotient; a third of the instructions of an IEEE division).
+__device__ __forceinline__ double rcp64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return r;
+}
+
+// At least 4 waves per SIMD (at most 128 VGPRs): two 512-thread workgroups
+// share a CU.
 template <int L, int NT, int WPE, bool FIN>
-__global__ void __launch_bounds__(NT, WPE)
+__global__ void __launch_bounds__(NT, (WPE > 4 ? WPE : 4))
     k_acoustic_pairs(const Dom *__restrict__ doms, const int4 *__restrict__ atiles, ZVec zv, SSet<float> in,
                      Work<float> w, AcBuf<float> ab, int div, double dt_ovr, int nsub, int first) {
     using S = float;
@@ -43,7 +59,6 @@ __global__ void __launch_bounds__(NT, WPE)
     const int nl = (col_ok && k0 < nz) ? min(L, nz - k0) : 0;  // levels owned
     const int nf = max(0, min(nl, nz - 1 - k0));                // faces owned (j <= nz-2)
     const int im = col_ok ? wrapx(i - 1, nx) : 0, ip = col_ok ? wrapx(i + 1, nx) : 0;
-    const int64_t base = D.off2;
     const double *zd = zv.z + D.offz, *dzd = zv.dz + D.offz;
     const R dx = R(D.dx);
     const R EPS = K<R>::EPS;
@@ -65,6 +80,27 @@ __global__ void __launch_bounds__(NT, WPE)
     const int64_t cidx = (int64_t)c * ab.cin_stride + D.offx + i;
     double cin_v = 0.0;
     if (!first && nl > 0) cin_v = ab.cin[cidx];
+    const int ic = col_ok ? i : 0;
+    const int kev = (nz - 1) & ~1;  // highest level-pair start of the member
+    // the member's fields (uniform base pointers)
+    const int64_t o2 = D.off2;
+    float *const f_mx = w.mom_x + o2, *const f_mz = w.mom_z + o2, *const f_rp = w.rp + o2;
+    const float *const f_cur = ab.rtp_cur + o2, *const f_old = ab.rtp_old + o2, *const f_rt2 = w.rt2 + o2,
+                       *const f_rup = ab.rup_in + o2, *const f_Ru = w.R_rhou + o2, *const f_Rr = w.R_rho + o2,
+                       *const f_th = w.theta + o2, *const f_Rt = w.R_rt + o2, *const f_Rw = w.R_rhow + o2,
+                       *const f_rwi = ab.rwp_in + o2, *const f_srw = in.f[RHOW] + o2;
+    // byte offsets of the thread's level pairs at columns i, i-1, i+1
+    const uint32_t rowb = 8u * (uint32_t)nx;  // bytes per level-pair row
+    uint32_t bc[L / 2], bm[L / 2], bp[L / 2];
+#pragma unroll
+    for (int m = 0; m < L / 2; ++m) {
+        const uint32_t r = (uint32_t)(min(k0 + 2 * m, kev) >> 1) * rowb;
+        bc[m] = r + 8u * (uint32_t)ic;
+        bm[m] = r + 8u * (uint32_t)im;
+        bp[m] = r + 8u * (uint32_t)ip;
+    }
+    // byte offset of level k (clamped to the member) of column i, x-fastest
+    auto bx = [&](int k) -> uint32_t { return 4u * ((uint32_t)min(max(k, 0), ke) * (uint32_t)nx + (uint32_t)ic); };
 
     // Loads of the later phases (after the first barrier) are issued before
     // it, right after phase 1's: the per-level geometry goes to LDS, the few
@@ -80,46 +116,44 @@ __global__ void __launch_bounds__(NT, WPE)
     for (int q = 0; q < L + 4; ++q) thx[q] = R(0.0);
 #pragma unroll
     for (int q = 0; q < L + 1; ++q) rt2c[q] = R(0.0);
-    const int ic = col_ok ? i : 0;
-    const int kev = (nz - 1) & ~1;  // highest level-pair start of the member
     // later-phase inputs: rho_w' at the faces, mom_z, the input rho_w, issued
     // after phase 1 (before it they would stay live across it and spill)
     auto load_late = [&]() {
-        auto at = [&](int k) -> int64_t { return base + (int64_t)min(max(k, 0), ke) * nx + ic; };
 #pragma unroll
         for (int m = 0; m < L / 2; ++m) {
-            const int kp = min(k0 + 2 * m, kev);
             R r0, r1, m0, m1;
-            ld2<true>(ab.rwp_in, D, kp, ic, r0, r1);
-            ld2<true>(w.mom_z, D, kp, ic, m0, m1);
+            ldb2(f_rwi, bc[m], r0, r1);
+            ldb2(f_mz, bc[m], m0, m1);
             rwv[2 * m] = r0;
             rwv[2 * m + 1] = r1;
             mzv[2 * m] = 2 * m < nf ? m0 : R(0.0);
             mzv[2 * m + 1] = 2 * m + 1 < nf ? m1 : R(0.0);
         }
-        rwv[L] = ab.rwp_in[widx<true>(D, min(k0 + L, ke), ic)];
+        rwv[L] = ldb(f_rwi, 4u * moff<true>(nx, min(k0 + L, ke), ic));
 #pragma unroll
         for (int q = 0; q < L + 1; ++q) {
             const int j = k0 + q, kq = k0 - 1 + q;
-            const R sv = in.f[RHOW][at(kq)];
+            const R sv = ldb(f_srw, bx(kq));
             if (!(nl > 0 && j <= nz - 2)) rwv[q] = R(0.0);
             swv[q] = (nl > 0 && kq >= 0 && kq < nz) ? sv : R(0.0);
         }
         if constexpr (FIN) {
 #pragma unroll
             for (int q = 0; q < L; ++q) {
-                const int64_t qq = at(k0 + q);
-                inr[q] = in.f[RHO][qq];
-                inrt[q] = in.f[RT][qq];
+                const uint32_t qq = bx(k0 + q);
+                inr[q] = ldb(in.f[RHO] + o2, qq);
+                inrt[q] = ldb(in.f[RT] + o2, qq);
             }
         }
     };
     // ---- phase 1: rho_u', explicit rho' and rho_theta', d rho_w'
-    // Inputs arrive as level pairs, one pair's loads in flight together; the
-    // pair's stores of rho_u' and mom_x end its iteration.
-    constexpr int GL = 2;
-    auto ldg = [&](const R *p, int k, int col, R *v) { ld2<true>(p, D, k, col, v[0], v[1]); };
-    auto stg = [&](R *p, int k, int col, const R *v) { st2<true>(p, D, k, col, v[0], v[1]); };
+    // Inputs arrive as level pairs, one pair's loads in flight together.  The
+    // loads of the next pair (after the last pair: the later phases' inputs
+    // and the level geometry) are issued before the current pair's stores of
+    // rho_u' and mom_x: a load issued after a store to an array the compiler
+    // cannot tell apart waits for that store to complete.
+    constexpr int GL = 2, NP = L / GL;
+    auto ldg = [&](const R *p, uint32_t b, R *v) { ldb2(p, b, v[0], v[1]); };
     R tp_r0 = R(0.0), tp_t0 = R(0.0), tp_rp = R(0.0), tp_tp = R(0.0);
     R rrw[L];  // R_rho_w per level
     // the level heights the buoyancy of phase 1 uses, loaded with the first
@@ -128,41 +162,59 @@ __global__ void __launch_bounds__(NT, WPE)
     double zpre[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) zpre[l] = zd[min(k0 + l, ke)];
+    R mx_[NP][GL], cc_[NP][GL], cm_[NP][GL], cx_[NP][GL], oc_[NP][GL], om_[NP][GL], ox_[NP][GL];
+    R r2c_[NP][GL], r2m_[NP][GL], r2x_[NP][GL], ruc_[NP][GL], rum_[NP][GL], Fuc_[NP][GL], Fum_[NP][GL];
+    R Fr_[NP][GL], rp_[NP][GL], thc_[NP][GL], thm_[NP][GL], thp_[NP][GL], Ft_[NP][GL];
+    R iru_[NP][GL];  // FIN: the stage input's rho_u of the pair's levels
+    auto load_group = [&](int m) {
+        if constexpr (FIN) {
+            const int kp = min(k0 + 2 * m, kev);
+            iru_[m][0] = ldb(in.f[RHOU] + o2, bx(kp));
+            iru_[m][1] = ldb(in.f[RHOU] + o2, bx(kp + 1));
+        }
+        const uint32_t oc = bc[m], om = bm[m], op = bp[m];
+        ldg(f_mx, oc, mx_[m]);
+        ldg(f_cur, oc, cc_[m]);
+        ldg(f_cur, om, cm_[m]);
+        ldg(f_cur, op, cx_[m]);
+        ldg(f_old, oc, oc_[m]);
+        ldg(f_old, om, om_[m]);
+        ldg(f_old, op, ox_[m]);
+        ldg(f_rt2, oc, r2c_[m]);
+        ldg(f_rt2, om, r2m_[m]);
+        ldg(f_rt2, op, r2x_[m]);
+        ldg(f_rup, oc, ruc_[m]);
+        ldg(f_rup, om, rum_[m]);
+        ldg(f_Ru, oc, Fuc_[m]);
+        ldg(f_Ru, om, Fum_[m]);
+        ldg(f_Rr, oc, Fr_[m]);
+        ldg(f_rp, oc, rp_[m]);
+        ldg(f_th, oc, thc_[m]);
+        ldg(f_th, om, thm_[m]);
+        ldg(f_th, op, thp_[m]);
+        ldg(f_Rt, oc, Ft_[m]);
+        ldg(f_Rw, oc, rrw + GL * m);
+    };
+    // theta two levels below and above the chunk (the level pairs k0-2 and
+    // k0+L), rt2pres one level above, and the first level of the per-level
+    // geometry this thread stages (z, dz, 1/dz, 1/(z_k+1 - z_k),
+    // g/(dz_k + dz_k+1)); issued with the later phases' inputs
+    R a0, a1, b0, b1, r2top;
+    double4 g0;
+    double z0, dz0;
+    const int q0 = min(t, nz - 1);
+    auto load_rest = [&]() {
+        load_late();
+        ldb2(f_th, 4u * moff<true>(nx, max(k0 - 2, 0), ic), a0, a1);
+        ldb2(f_th, 4u * moff<true>(nx, min(k0 + L, kev), ic), b0, b1);
+        r2top = ldb(f_rt2, 4u * moff<true>(nx, min(k0 + L, ke), ic));
+        g0 = zv.geo[D.offz + q0];
+        z0 = zd[q0];
+        dz0 = dzd[q0];
+    };
+    load_group(0);
 #pragma unroll
-    for (int m = 0; m < L / GL; ++m) {
-        const int kp = min(k0 + 2 * m, kev);
-        R mx_[GL], cc_[GL], cm_[GL], cx_[GL], oc_[GL], om_[GL], ox_[GL], r2c_[GL], r2m_[GL], r2x_[GL];
-        R ruc_[GL], rum_[GL], Fuc_[GL], Fum_[GL], Fr_[GL], rp_[GL], thc_[GL], thm_[GL], thp_[GL], Ft_[GL];
-        R iru_[GL];  // FIN: the stage input's rho_u of the pair's levels
-        auto load_group = [&]() {
-            if constexpr (FIN) {
-                const int64_t q0 = base + (int64_t)kp * nx + ic;
-                iru_[0] = in.f[RHOU][q0];
-                iru_[1] = in.f[RHOU][kp + 1 <= ke ? q0 + nx : q0];
-            }
-            ldg(w.mom_x, kp, ic, mx_);
-            ldg(ab.rtp_cur, kp, ic, cc_);
-            ldg(ab.rtp_cur, kp, im, cm_);
-            ldg(ab.rtp_cur, kp, ip, cx_);
-            ldg(ab.rtp_old, kp, ic, oc_);
-            ldg(ab.rtp_old, kp, im, om_);
-            ldg(ab.rtp_old, kp, ip, ox_);
-            ldg(w.rt2, kp, ic, r2c_);
-            ldg(w.rt2, kp, im, r2m_);
-            ldg(w.rt2, kp, ip, r2x_);
-            ldg(ab.rup_in, kp, ic, ruc_);
-            ldg(ab.rup_in, kp, im, rum_);
-            ldg(w.R_rhou, kp, ic, Fuc_);
-            ldg(w.R_rhou, kp, im, Fum_);
-            ldg(w.R_rho, kp, ic, Fr_);
-            ldg(w.rp, kp, ic, rp_);
-            ldg(w.theta, kp, ic, thc_);
-            ldg(w.theta, kp, im, thm_);
-            ldg(w.theta, kp, ip, thp_);
-            ldg(w.R_rt, kp, ic, Ft_);
-            ldg(w.R_rhow, kp, ic, rrw + GL * m);
-        };
-        load_group();
+    for (int m = 0; m < NP; ++m) {
         R rui[GL], mxo[GL];  // rho_u' at face i+1/2 and mom_x of the pair
 #pragma unroll
         for (int h = 0; h < GL; ++h) {
@@ -173,31 +225,30 @@ __global__ void __launch_bounds__(NT, WPE)
             rui[h] = R(0.0);
             mxo[h] = R(0.0);
             if (l < nl) {
-                const int k = k0 + l;
-                const R cur_c = cc_[h], cur_m = cm_[h], cur_p = cx_[h];
-                const R old_c = oc_[h], old_m = om_[h], old_p = ox_[h];
-                const R rt2_c = r2c_[h], rt2_m = r2m_[h], rt2_p = r2x_[h];
+                const R cur_c = cc_[m][h], cur_m = cm_[m][h], cur_p = cx_[m][h];
+                const R old_c = oc_[m][h], old_m = om_[m][h], old_p = ox_[m][h];
+                const R rt2_c = r2c_[m][h], rt2_m = r2m_[m][h], rt2_p = r2x_[m][h];
                 // rho_u' at i+1/2 and i-1/2 (moist.jl:1362-1367)
                 R gx = DIV(cur_p - cur_c, dx, rdx);
                 R gxo = DIV(old_p - old_c, dx, rdx);
                 gx = gx + gam * (gx - gxo);
-                const R ru_i = ruc_[h] + dt_s * (Fuc_[h] - (rt2_c + rt2_p) * R(0.5) * gx);
+                const R ru_i = ruc_[m][h] + dt_s * (Fuc_[m][h] - (rt2_c + rt2_p) * R(0.5) * gx);
                 R gm = DIV(cur_c - cur_m, dx, rdx);
                 R gmo = DIV(old_c - old_m, dx, rdx);
                 gm = gm + gam * (gm - gmo);
-                const R ru_m = rum_[h] + dt_s * (Fum_[h] - (rt2_m + rt2_c) * R(0.5) * gm);
+                const R ru_m = rum_[m][h] + dt_s * (Fum_[m][h] - (rt2_m + rt2_c) * R(0.5) * gm);
                 // explicit rho' (moist.jl:1383-1386)
                 R adv = DIV(-(ru_i - ru_m), dx, rdx);
-                R inc = dt_s * (Fr_[h] + adv);
-                const R rpc = rp_[h];
+                R inc = dt_s * (Fr_[m][h] + adv);
+                const R rpc = rp_[m][h];
                 const R rp_tp = rpc + R(fact_tp_d) * inc;
                 rp_e[l] = rpc + inc;
                 // explicit rho_theta' (moist.jl:1389-1394)
-                const R th_c = thc_[h];
-                const R fip = (thp_[h] + th_c) * R(0.5) * ru_i;
-                const R fim = (th_c + thm_[h]) * R(0.5) * ru_m;
+                const R th_c = thc_[m][h];
+                const R fip = (thp_[m][h] + th_c) * R(0.5) * ru_i;
+                const R fim = (th_c + thm_[m][h]) * R(0.5) * ru_m;
                 adv = DIV(-(fip - fim), dx, rdx);
-                inc = dt_s * (Ft_[h] + adv);
+                inc = dt_s * (Ft_[m][h] + adv);
                 const R rtp_tp = cur_c + R(fact_tp_d) * inc;
                 rtp_e[l] = cur_c + inc;
                 thx[l + 2] = th_c;
@@ -216,63 +267,65 @@ __global__ void __launch_bounds__(NT, WPE)
                 tp_tp = rtp_tp;
                 // rho_u' and its running mean mom_x (moist.jl:1453-1455)
                 rui[h] = ru_i;
-                mxo[h] = mx_[h] + DIV(ru_i, R(nsub), rns);
+                mxo[h] = mx_[m][h] + DIV(ru_i, R(nsub), rns);
             }
         }
+        if (m + 1 < NP) load_group(m + 1 < NP ? m + 1 : 0);
+        else load_rest();
         if (GL * m < nl) {
             if constexpr (FIN) {
 #pragma unroll
                 for (int h = 0; h < GL; ++h) {
                     const int l = GL * m + h;
                     if (l < nl) {
-                        const R v = iru_[h] + rui[h];
-                        ab.fin_ru[base + (int64_t)(k0 + l) * nx + i] = v;
+                        const R v = iru_[m][h] + rui[h];
+                        stb(ab.fin_ru + o2, bx(k0 + l), v);
                         if (isnan(v)) nanf = min(nanf, 2);
                     }
                 }
             } else {
-                stg(ab.rup_out, k0 + GL * m, i, rui);
+                stb2(ab.rup_out + o2, bc[m], rui[0], rui[1]);
             }
-            stg(w.mom_x, k0 + GL * m, i, mxo);
+            stb2(f_mx, bc[m], mxo[0], mxo[1]);
         }
     }
-    load_late();
-    // theta two levels below and above the chunk (the level pairs k0-2 and
-    // k0+L), rt2pres one level above, R_rho_w at the face below level k0+L
+    // R_rho_w at the face below level k0+L, and the halo values now in place
+    // (branch-free selects, so no use waits on a load inside a branch)
     const R rrw_top = rrw[L - 1];
     {
-        R a0, a1, b0, b1;
-        ld2<true>(w.theta, D, max(k0 - 2, 0), ic, a0, a1);
-        ld2<true>(w.theta, D, min(k0 + L, kev), ic, b0, b1);
-        const R r2top = w.rt2[widx<true>(D, min(k0 + L, ke), ic)];
-        if (nl > 0) {
-            if (k0 >= 2) thx[0] = a0;
-            if (k0 >= 1) thx[1] = a1;
-            if (nl == L) {
-                if (k0 + L < nz) {
-                    thx[L + 2] = b0;
-                    rt2c[L] = r2top;
-                }
-                if (k0 + L + 1 < nz) thx[L + 3] = b1;
-            }
-        }
+        const bool lo2 = nl > 0 && k0 >= 2, lo1 = nl > 0 && k0 >= 1;
+        const bool hi = nl == L && k0 + L < nz, hi1 = nl == L && k0 + L + 1 < nz;
+        thx[0] = lo2 ? a0 : thx[0];
+        thx[1] = lo1 ? a1 : thx[1];
+        thx[L + 2] = hi ? b0 : thx[L + 2];
+        rt2c[L] = hi ? r2top : rt2c[L];
+        thx[L + 3] = hi1 ? b1 : thx[L + 3];
     }
-    // geometry for the phases after the barrier, per level: z, dz, 1/dz, 1/(z_k+1 - z_k), g/(dz_k + dz_k+1)
-    for (int q = t; q < nz; q += NT) {
+    if (t < nz) {
+        double *lq = lgeo + 5u * (uint32_t)t;
+        lq[0] = z0;
+        lq[1] = dz0;
+        lq[2] = g0.x;
+        lq[3] = g0.y;
+        lq[4] = g0.w;
+    }
+    for (int q = t + NT; q < nz; q += NT) {
         const double4 g = zv.geo[D.offz + q];
-        lgeo[5 * q + 0] = zd[q];
-        lgeo[5 * q + 1] = dzd[q];
-        lgeo[5 * q + 2] = g.x;
-        lgeo[5 * q + 3] = g.y;
-        lgeo[5 * q + 4] = g.w;
+        double *lq = lgeo + 5u * (uint32_t)q;
+        lq[0] = zd[q];
+        lq[1] = dzd[q];
+        lq[2] = g.x;
+        lq[3] = g.y;
+        lq[4] = g.w;
     }
     // ---- LDS round A: off-centred values of the first level of each chunk
     // (and the geometry staged above)
     ldt[t] = tp_r0;
     ldt[NT + t] = tp_t0;
     __syncthreads();
-    auto LZ = [&](int k) { return lgeo[5 * k]; };
-    auto LDZ = [&](int k) { return lgeo[5 * k + 1]; };
+    // per-level geometry by 32-bit LDS index
+    auto LZ = [&](int k) { return lgeo[5u * (uint32_t)k]; };
+    auto LDZ = [&](int k) { return lgeo[5u * (uint32_t)k + 1u]; };
     if (nl == L && k0 + L < nz) {
         const int k = k0 + L;
         const R up_r = ldt[t + TX], up_t = ldt[NT + t + TX];
@@ -297,8 +350,9 @@ __global__ void __launch_bounds__(NT, WPE)
     // and g/(dz_j + dz_j+1) from the staged per-level table
     auto coef = [&](int l, double &a, double &b, double &cc) {
         const int j = k0 + l;
-        const double gx = lgeo[5 * j + 2], gy = lgeo[5 * j + 3], gw = lgeo[5 * j + 4];
-        const double rdz1 = lgeo[5 * (j + 1) + 2];
+        const double *lj = lgeo + 5u * (uint32_t)j;
+        const double gx = lj[2], gy = lj[3], gw = lj[4];
+        const double rdz1 = lj[5 + 2];
         const double r2j = double(rt2c[l]), r2k = double(rt2c[l + 1]);
         b = dt_tp2 * (r2j * gx + r2k * rdz1) * thh[l + 1] * gy;
         a = j > 0 ? -dt_tp2 * (r2j * thh[l] * (gy * gx) - gw) : 0.0;
@@ -351,7 +405,7 @@ __global__ void __launch_bounds__(NT, WPE)
                 // [num; den] <- [[0, c], [-a, b]] [num; den]
                 const double n00 = cc * m10, n01 = cc * m11;
                 const double n10 = -a * m00 + b * m10, n11 = -a * m01 + b * m11;
-                const double sc = 1.0 / fmax(fmax(fabs(n00), fabs(n01)), fmax(fabs(n10), fabs(n11)));
+                const double sc = __builtin_amdgcn_rcp(fmax(fmax(fabs(n00), fabs(n01)), fmax(fabs(n10), fabs(n11))));
                 m00 = n00 * sc;
                 m01 = n01 * sc;
                 m10 = n10 * sc;
@@ -365,7 +419,7 @@ __global__ void __launch_bounds__(NT, WPE)
             if (cw >= d) {
                 const double n0 = m00 * b0 + m01 * b2, n1 = m00 * b1 + m01 * b3;
                 const double n2 = m10 * b0 + m11 * b2, n3 = m10 * b1 + m11 * b3;
-                const double sc = 1.0 / fmax(fmax(fabs(n0), fabs(n1)), fmax(fabs(n2), fabs(n3)));
+                const double sc = __builtin_amdgcn_rcp(fmax(fmax(fabs(n0), fabs(n1)), fmax(fabs(n2), fabs(n3))));
                 m00 = n0 * sc;
                 m01 = n1 * sc;
                 m10 = n2 * sc;
@@ -393,7 +447,7 @@ __global__ void __launch_bounds__(NT, WPE)
         for (int gp = 0; gp < grp; ++gp) {
             const double *M = agm + 4 * (gp * TX + jj);
             const double v0 = M[0] * u0 + M[1] * u1, v1 = M[2] * u0 + M[3] * u1;
-            const double sc = 1.0 / fmax(fabs(v0), fabs(v1));
+            const double sc = __builtin_amdgcn_rcp(fmax(fabs(v0), fabs(v1)));
             u0 = v0 * sc;
             u1 = v1 * sc;
         }
@@ -412,7 +466,7 @@ __global__ void __launch_bounds__(NT, WPE)
             pv[l] = 0.0;
             if (l < nf) {
                 const double a = ca[l], b = cb[l] + 1.0, cc = ccf[l];
-                const double rm = 1.0 / (b - a * cprev);
+                const double rm = rcp64(b - a * cprev);
                 const double cp = cc * rm;
                 const double y = (dd[l] - a * yprev) * rm;
                 P = (-a * P) * rm;
@@ -534,8 +588,8 @@ __global__ void __launch_bounds__(NT, WPE)
 #pragma unroll
     for (int m = 0; m < L / 2; ++m) {
         if (2 * m < nf) {
-            if constexpr (!FIN) st2<true>(w.rwp, D, k0 + 2 * m, i, xr[2 * m], xr[2 * m + 1]);
-            st2<true>(w.mom_z, D, k0 + 2 * m, i, mzv[2 * m] + DIV(xr[2 * m], R(nsub), rns),
+            if constexpr (!FIN) stb2(w.rwp + o2, bc[m], xr[2 * m], xr[2 * m + 1]);
+            stb2(f_mz, bc[m], mzv[2 * m] + DIV(xr[2 * m], R(nsub), rns),
                       mzv[2 * m + 1] + DIV(xr[2 * m + 1], R(nsub), rns));
         }
     }
@@ -546,7 +600,7 @@ __global__ void __launch_bounds__(NT, WPE)
             if (l < nl) {
                 const int k = k0 + l;
                 const R v = k < ke ? swv[l + 1] + xr[l] : swv[l + 1];
-                ab.fin_rw[base + (int64_t)k * nx + i] = v;
+                stb(ab.fin_rw + o2, bx(k), v);
                 if (isnan(v)) nanf = min(nanf, 3);
             }
         }
@@ -581,18 +635,18 @@ __global__ void __launch_bounds__(NT, WPE)
             adv = DIV(-(Fkp - Fkm), dzk, rdzk);
             rtp_e[l] = rtp_e[l] + dt_s * adv;
             if constexpr (FIN) {
-                const int64_t q = base + (int64_t)k * nx + i;
+                const uint32_t q = bx(k);
                 const R vr = inr[l] + rp_e[l], vt = inrt[l] + rtp_e[l];
-                ab.fin_r[q] = vr;
-                ab.fin_rt[q] = vt;
+                stb(ab.fin_r + o2, q, vr);
+                stb(ab.fin_rt + o2, q, vt);
                 if (isnan(vr)) nanf = 1;
                 else if (isnan(vt)) nanf = min(nanf, 4);
             } else if ((l & 1) || l == nl - 1) {
                 // stored as soon as a level pair is complete; an odd top level
                 // is stored with the padding level above it
                 const int l0 = l & ~1;
-                st2<true>(w.rp, D, k0 + l0, i, rp_e[l0], rp_e[l0 + 1]);
-                st2<true>(ab.rtp_out, D, k0 + l0, i, rtp_e[l0], rtp_e[l0 + 1]);
+                stb2(f_rp, bc[l0 / 2], rp_e[l0], rp_e[l0 + 1]);
+                stb2(ab.rtp_out + o2, bc[l0 / 2], rtp_e[l0], rtp_e[l0 + 1]);
             }
         }
     }

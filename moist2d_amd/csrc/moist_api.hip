@@ -1272,9 +1272,10 @@ int moist_create(const moist_params *const *members, int n_members, int precisio
             *out = h.release();
             return -2;
         }
-        // kernels index a member's cells (nz padded to even) with int
-        if ((int64_t)p->nx * (p->nz + 1) > INT_MAX) {
-            h->err = "member too large: nx * nz must stay below 2^31 cells";
+        // kernels address a member's cells (nz padded to even) by 32-bit byte
+        // offsets from its base: a field of one member stays below 2^32 bytes
+        if ((int64_t)p->nx * (p->nz + 1) * h->elem > (int64_t)UINT32_MAX) {
+            h->err = "member too large: nx * (nz + 1) * (4 or 8 bytes) must stay below 2^32 bytes";
             *out = h.release();
             return -2;
         }

@@ -103,6 +103,36 @@ template <bool P, class S> __device__ __forceinline__ void st2(S *p, const Dom &
     }
 }
 
+// ---- member-relative addressing ----------------------------------------------
+// A block works on one member, so it can address a field through the member's
+// base pointer (uniform: one scalar register pair) and a 32-bit byte offset
+// per lane, the form gfx950 loads and stores take directly (global_load v,
+// v_off, s[base]): no 64-bit address arithmetic per access, and one offset
+// register serves every field at the same cell.  moist_create keeps each
+// member below 2^32 bytes a field.
+// element offset of cell (k, i) within its member, layout P
+template <bool P> __device__ __forceinline__ uint32_t moff(int nx, int k, int i) {
+    if constexpr (P) return ((uint32_t)(k >> 1) * (uint32_t)nx + (uint32_t)i) * 2u + (uint32_t)(k & 1);
+    else return (uint32_t)k * (uint32_t)nx + (uint32_t)i;
+}
+template <class S> __device__ __forceinline__ S ldb(const S *b, uint32_t boff) {
+    return *(const S *)((const char *)b + boff);
+}
+template <class S> __device__ __forceinline__ void stb(S *b, uint32_t boff, S v) { *(S *)((char *)b + boff) = v; }
+// a level pair (8 bytes; the same relaxed atomic load as ld2, never split)
+__device__ __forceinline__ void ldb2(const float *b, uint32_t boff, float &x, float &y) {
+    const uint64_t u = __hip_atomic_load((const uint64_t *)((const char *)b + boff), __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_WAVEFRONT);
+    x = __builtin_bit_cast(float, (uint32_t)u);
+    y = __builtin_bit_cast(float, (uint32_t)(u >> 32));
+}
+__device__ __forceinline__ void stb2(float *b, uint32_t boff, float x, float y) {
+    float2 v;
+    v.x = x;
+    v.y = y;
+    *(float2 *)((char *)b + boff) = v;
+}
+
 // Stage time step: dt/3, dt/2, dt in the compute type (moist.jl:1649-1657),
 // or an explicit dt for the stage-level entry.
 template <class R> __device__ __forceinline__ R stage_dt(const Dom &D, int div, double dt_ovr) {

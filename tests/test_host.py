@@ -134,7 +134,8 @@ def test_bf2002_initial_state():
 
 def test_create_rejects_bad_members_before_touching_the_device():
     """moist_create validates every member first (no HIP call on this path):
-    halo > nx and a member of 2^31 cells or more return -2 with a message."""
+    halo > nx and a member whose field takes 2^32 bytes or more return -2
+    with a message."""
     L = core.lib()
     p = make_params(np.float64, Nx=16, Nz=8, Lx=1600.0, H=800.0, dz0=100.0, z_fact=1.0, ns=6, tout_int=1.0)
 
@@ -151,6 +152,6 @@ def test_create_rejects_bad_members_before_touching_the_device():
 
     rc, msg = create(lambda c: setattr(c, "halo", 17))
     assert rc == -2 and "invalid member params" in msg
-    # 2^28 columns x (8 + 1) levels = 2.4e9 cells > 2^31 - 1
-    rc, msg = create(lambda c: setattr(c, "nx", 1 << 28))
+    # 2^26 columns x (8 + 1) levels x 8 bytes = 4.8e9 bytes >= 2^32
+    rc, msg = create(lambda c: setattr(c, "nx", 1 << 26))
     assert rc == -2 and "member too large" in msg
