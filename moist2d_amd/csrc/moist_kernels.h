@@ -381,16 +381,27 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
     const int nx = D.nx, nz = D.nz, ke = nz - 1;
     if (i >= nx || k >= nz) return;
     const int64_t c = D.off2 + (int64_t)k * nx + i;
+    // fields by the member's base pointer and 32-bit byte offsets
+    const int64_t o2 = D.off2;
+    const uint32_t q = (uint32_t)sizeof(S) * moff<false>(nx, k, i);
+    auto L = [&](const S *f, uint32_t b) -> R { return ldb(f + o2, b); };
     const R dt = R(D.dt);
     const int fl = D.flags;
-    const R rho_k = full_rho(s, c, zv, D, k);
-    const R rt = full_rt(s, c, zv, D, k);
+    const R s_rho = L(s.f[RHO], q), s_rt = L(s.f[RT], q);
+    R rho_k = s_rho, rt = s_rt;
+    if constexpr (is_mixed<S>()) {
+        rho_k = rho_k + S(zv.rho_bar[D.offz + k]);
+        rt = rt + S(zv.rt_bar[D.offz + k]);
+    }
     // every other global read of the kernel, before its first store: the
     // rain and density one level up (sedimentation), radiation and surface terms
     const int kup = k < ke ? k + 1 : k;
-    const int64_t cu = D.off2 + (int64_t)kup * nx + i;
-    const R qr_c = s.f[QR][c], qr_u = s.f[QR][cu], rho_u = full_rho(s, cu, zv, D, kup);
-    const R th_rd = (fl & F_RD) ? w.d_th_rd[c] : R(0.0);
+    const uint32_t qu = (uint32_t)sizeof(S) * moff<false>(nx, kup, i);
+    const R qr_c = L(s.f[QR], q), qr_u = L(s.f[QR], qu);
+    R rho_u = L(s.f[RHO], qu);
+    if constexpr (is_mixed<S>()) rho_u = rho_u + S(zv.rho_bar[D.offz + kup]);
+    const R s_qv = L(s.f[QV], q), s_qc = L(s.f[QC], q);
+    const R th_rd = (fl & F_RD) ? L(w.d_th_rd, q) : R(0.0);
     const bool sf = (k == 0) && (fl & F_SF);
     const int64_t x = D.offx + i;
     const R qv_sf = sf ? w.d_qv_sf[x] : R(0.0), th_sf = sf ? w.d_th_sf[x] : R(0.0);
@@ -411,13 +422,13 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
     R dg_th = R(0.0), dg_pres = R(0.0), dg_rt2 = R(0.0);
     if constexpr (is_mixed<S>()) {
         if (diag_pairs)
-            diag_point<S>(D, zv, k, s.f[RHO][c], s.f[RT][c], s.f[QV][c], s.f[QC][c], qr_c, dg_th, dg_pres, dg_rt2);
+            diag_point<S>(D, zv, k, s_rho, s_rt, s_qv, s_qc, qr_c, dg_th, dg_pres, dg_rt2);
     }
     R dqv = R(0.0), dqc = R(0.0), dqr = R(0.0), dth = R(0.0);
     if (fl & F_MP) {
         const R rinv = fdiv(R(1.0), rho_k);
         R theta_k = rt * rinv;
-        R qv = s.f[QV][c] * rinv, qc = s.f[QC][c] * rinv, qr = qr_c * rinv;
+        R qv = s_qv * rinv, qc = s_qc * rinv, qr = qr_c * rinv;
         const R qd = R(1.0) - qv - qc - qr;
         R r = KC::Rd * qd + KC::Rv * qv;
         R cp = KC::Cpd * qd + KC::Cpv * qv + KC::Cpl * (qc + qr);
@@ -510,16 +521,17 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
         if constexpr (is_mixed<S>()) w.prec[x] += double(fkm) * D.dt;
         else w.prec[x] += fkm * dt;
     }
-    w.d_rho[c] = d_rho;
-    w.d_rt[c] = d_rt;
-    w.d_qv[c] = d_qv;
-    w.d_qc[c] = dqc;
-    w.d_qr[c] = d_qr;
+    stb(w.d_rho + o2, q, d_rho);
+    stb(w.d_rt + o2, q, d_rt);
+    stb(w.d_qv + o2, q, d_qv);
+    stb(w.d_qc + o2, q, dqc);
+    stb(w.d_qr + o2, q, d_qr);
     if constexpr (is_mixed<S>()) {
         if (diag_pairs) {
-            w.pres[c] = dg_pres;
-            w.theta[widx<true>(D, k, i)] = dg_th;
-            w.rt2[widx<true>(D, k, i)] = dg_rt2;
+            const uint32_t qp = 4u * moff<true>(nx, k, i);
+            stb(w.pres + o2, q, dg_pres);
+            stb(w.theta + o2, qp, dg_th);
+            stb(w.rt2 + o2, qp, dg_rt2);
         }
     }
     if constexpr (is_mixed<S>()) {
@@ -528,10 +540,10 @@ __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, c
             for (int f = 0; f < 7; ++f) wide.f[f][c] = wv[f];
     }
     if (store_diag) {
-        w.dg_th_mp[c] = dth;
-        w.dg_qv_mp[c] = dqv;
-        w.dg_qr_mp[c] = dqr;
-        w.dg_qr_sed[c] = dsed;
+        stb(w.dg_th_mp + o2, q, dth);
+        stb(w.dg_qv_mp + o2, q, dqv);
+        stb(w.dg_qr_mp + o2, q, dqr);
+        stb(w.dg_qr_sed + o2, q, dsed);
     }
     int f = 0;
     if (isnan(d_rt)) f = 9;

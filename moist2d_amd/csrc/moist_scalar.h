@@ -93,8 +93,9 @@ __device__ __forceinline__ R face_z(R w, const R *q, int kk, int ke, R dzk, R dz
     }
 }
 
+// 6 waves per SIMD in float (80 VGPRs), 4 in double (128)
 template <class S, bool P>
-__global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
+__global__ void __launch_bounds__(ST_THREADS, is_mixed<S>() ? 6 : 4) k_scalar(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
                                                        ZVec zv, SSet<S> in, SSet<S> s0, SSet<S> out, Work<S> w,
                                                        const S *__restrict__ rup_fin, const S *__restrict__ rtp_fin,
                                                        int div, double dt_ovr, int stage, S *dg_th, S *dg_pres,
@@ -120,11 +121,16 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
     const R dt = stage_dt<R>(D, div, dt_ovr);
     const R dtL = R(D.dt);  // the positivity limiters use the large step (moist.jl:917,930)
     auto gidx = [&](int k, int i) -> int64_t { return D.off2 + (int64_t)k * nx + i; };
+    // member-relative byte offsets: x-fastest fields (bo) and work fields in layout P (bw)
+    const int64_t o2 = D.off2;
+    auto bo = [&](int k, int i) -> uint32_t { return (uint32_t)sizeof(S) * moff<false>(nx, k, i); };
+    auto bw = [&](int k, int i) -> uint32_t { return (uint32_t)sizeof(S) * moff<P>(nx, k, i); };
+    constexpr uint32_t NOCELL = 0xffffffffu;
 
     // Work items of the strided loops, fixed per thread: staged cells of the
-    // haloed tile, x faces, z faces and interior cells.  Their global indices
-    // and the face mass fluxes (the same for the three scalars) are set up
-    // once, so every loop issues its loads together.
+    // haloed tile, x faces, z faces and interior cells.  Their offsets and the
+    // face mass fluxes (the same for the three scalars) are set up once, so
+    // every loop issues its loads together.
     constexpr int NSTG = (ST_HZ * ST_HX + ST_THREADS - 1) / ST_THREADS;
     constexpr int NFX = (ST_Z * (ST_X + 1) + ST_THREADS - 1) / ST_THREADS;
     constexpr int NFZ = ((ST_Z + 1) * ST_X + ST_THREADS - 1) / ST_THREADS;
@@ -135,13 +141,24 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
     // variant the two levels 2 (tid/ST_X), +1 of one column, so their
     // work-field accesses move level pairs
 
-    int64_t gst[NSTG];  // staged cell (-1: outside the domain's levels or the tile)
+    uint32_t gst[NSTG];  // staged cell (NOCELL: outside the domain's levels or the tile)
 #pragma unroll
     for (int u = 0; u < NSTG; ++u) {
         const int e = tid + u * ST_THREADS;
         const int r = e / ST_HX, cidx = e - r * ST_HX;
         const int k = k0 - 2 + r, i = wrapx(i0 - 2 + cidx, nx);
-        gst[u] = (e < ST_HZ * ST_HX && k >= 0 && k < nz) ? gidx(k, i) : -1;
+        gst[u] = (e < ST_HZ * ST_HX && k >= 0 && k < nz) ? bo(k, i) : NOCELL;
+    }
+    // the thread's interior cells: offsets (clamped into the member) and whether they exist
+    uint32_t ocl[NCL];
+    bool okc[NCL];
+#pragma unroll
+    for (int u = 0; u < NCL; ++u) {
+        const int e = P ? (2 * (tid / ST_X) + u) * ST_X + tid % ST_X : tid + u * ST_THREADS;
+        const int r = e / ST_X, c = e - r * ST_X;
+        const int k = k0 + r, i = i0 + c;
+        okc[u] = e < ST_Z * ST_X && k < nz && i < nx;
+        ocl[u] = bo(min(k, ke), min(i, nx - 1));
     }
     R mfx[NFX], mfz[NFZ];  // mass flux at the x and z faces of this thread
     R fdz0[NFZ], fdz1[NFZ], fzz[NFZ], cdz[NCL];  // dz below / above each z face, z spacing, dz of each cell
@@ -154,7 +171,8 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
         const int e = tid + u * ST_THREADS;
         const int r = e / (ST_X + 1), f = e - r * (ST_X + 1);
         const int k = k0 + r;
-        mfx[u] = (e < ST_Z * (ST_X + 1) && k < nz) ? w.mom_x[widx<P>(D, k, wrapx(i0 + f - 1, nx))] : R(0.0);
+        const R v = ldb(w.mom_x + o2, bw(min(k, ke), wrapx(i0 + f - 1, nx)));
+        mfx[u] = (e < ST_Z * (ST_X + 1) && k < nz) ? v : R(0.0);
     }
 #pragma unroll
     for (int u = 0; u < NFZ; ++u) {
@@ -162,7 +180,8 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
         const int g = e / ST_X, c = e - g * ST_X;
         const int kk = k0 + g - 1, i = i0 + c;
         const bool ok = e < (ST_Z + 1) * ST_X && i < nx && kk >= 0 && kk <= ke - 1;
-        mfz[u] = ok ? w.mom_z[widx<P>(D, kk, i)] : R(0.0);
+        const R v = ldb(w.mom_z + o2, bw(min(max(kk, 0), ke), min(i, nx - 1)));
+        mfz[u] = ok ? v : R(0.0);
         const int kc = ok ? kk : 0, kc1 = ok ? kk + 1 : 0;
         fdz0[u] = R(dzd[kc]);
         fdz1[u] = R(dzd[kc1]);
@@ -186,7 +205,9 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
         for (int u = 0; u < NSTG; ++u) {
             const int e = tid + u * ST_THREADS;
             const int k = k0 - 2 + e / ST_HX;
-            v[u] = gst[u] >= 0 ? full_rho(in, gst[u], zv, D, k) : R(0.0);
+            R r = gst[u] != NOCELL ? ldb(in.f[RHO] + o2, gst[u]) : R(0.0);
+            if constexpr (is_mixed<S>()) r = gst[u] != NOCELL ? r + S(zv.rho_bar[D.offz + min(max(k, 0), ke)]) : r;
+            v[u] = r;
         }
 #pragma unroll
         for (int u = 0; u < NSTG; ++u) {
@@ -208,9 +229,9 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
 #pragma unroll
         for (int u = 0; u < NCL; ++u) {
             const int k = min(k0 + 2 * (tid / ST_X) + u, ke);
-            const int64_t q = gidx(k, iq);
-            fin[u][0] = dg_th ? out.f[RHO][q] : R(0.0);
-            fin[u][1] = dg_th ? out.f[RT][q] : R(0.0);
+            const uint32_t q = bo(k, iq);
+            fin[u][0] = dg_th ? ldb(out.f[RHO] + o2, q) : R(0.0);
+            fin[u][1] = dg_th ? ldb(out.f[RT] + o2, q) : R(0.0);
         }
     } else {
 #pragma unroll
@@ -218,11 +239,11 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
             const int e = tid + u * ST_THREADS;
             const int r = e / ST_X, c = e - r * ST_X;
             const int k = min(k0 + r, ke), i = min(i0 + c, nx - 1);
-            const int64_t q = gidx(k, i);
-            fin[u][0] = in.f[RHO][q] + w.rp[q];
-            fin[u][1] = in.f[RHOU][q] + rup_fin[q];
-            fin[u][2] = in.f[RT][q] + rtp_fin[q];
-            fin[u][3] = k < ke ? in.f[RHOW][q] + w.rwp[q] : in.f[RHOW][q];
+            const uint32_t q = bo(k, i);
+            fin[u][0] = ldb(in.f[RHO] + o2, q) + ldb(w.rp + o2, q);
+            fin[u][1] = ldb(in.f[RHOU] + o2, q) + ldb(rup_fin + o2, q);
+            fin[u][2] = ldb(in.f[RT] + o2, q) + ldb(rtp_fin + o2, q);
+            fin[u][3] = k < ke ? ldb(in.f[RHOW] + o2, q) + ldb(w.rwp + o2, q) : ldb(in.f[RHOW] + o2, q);
         }
     }
 
@@ -233,30 +254,34 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
     const S *dph[3] = {w.d_qv, w.d_qc, w.d_qr};
     R qn[3][P ? NCL : 1];  // the updated rho_q of this thread's cells (pair variant's epilogue)
     int nan_f = 0;
-    for (int sc = 0; sc < 3; ++sc) {
-        const S *qraw = in.f[QV + sc];
-        // this scalar's global values: staged cells, limiter inputs, and the
-        // s0 value and tendency of the interior cells
-        R qv_[NSTG];
+    // a scalar's global values: staged cells, limiter inputs, and the s0 value
+    // and tendency of the interior cells.  The next scalar's are loaded before
+    // this scalar's stores (a load issued after a store to an array the
+    // compiler cannot tell apart would wait for the store to complete).
+    R qv_[NSTG], q_lo, q_hi, s0v[NCL], dpv[NCL];
+    R nq_lo = R(0.0), nq_hi = R(0.0), ns0v[NCL] = {}, ndpv[NCL] = {};
+    auto load_scalar = [&](int sc, R *qv, R &lo, R &hi, R *s0x, R *dpx) {
+        const S *qraw = in.f[QV + sc] + o2;
 #pragma unroll
-        for (int u = 0; u < NSTG; ++u) qv_[u] = gst[u] >= 0 ? R(qraw[gst[u]]) : R(0.0);
-        const R q_lo = R(qraw[gidx(0, D.lin_col)]), q_hi = R(qraw[gidx(ke, D.lin_col)]);
-        R s0v[NCL], dpv[NCL];
+        for (int u = 0; u < NSTG; ++u) qv[u] = gst[u] != NOCELL ? ldb(qraw, gst[u]) : R(0.0);
+        lo = ldb(qraw, bo(0, D.lin_col));
+        hi = ldb(qraw, bo(ke, D.lin_col));
 #pragma unroll
         for (int u = 0; u < NCL; ++u) {
-            const int e = P ? (2 * (tid / ST_X) + u) * ST_X + tid % ST_X : tid + u * ST_THREADS;
-            const int r = e / ST_X, c = e - r * ST_X;
-            const int k = k0 + r, i = i0 + c;
-            const bool ok = e < ST_Z * ST_X && k < nz && i < nx;
-            s0v[u] = ok ? s0.f[QV + sc][gidx(k, i)] : R(0.0);
-            dpv[u] = ok ? dph[sc][gidx(k, i)] : R(0.0);
+            const R a = ldb(s0.f[QV + sc] + o2, ocl[u]), b = ldb(dph[sc] + o2, ocl[u]);
+            s0x[u] = okc[u] ? a : R(0.0);
+            dpx[u] = okc[u] ? b : R(0.0);
         }
+    };
+    load_scalar(0, qv_, q_lo, q_hi, s0v, dpv);
+#pragma unroll
+    for (int sc = 0; sc < 3; ++sc) {
         __syncthreads();  // l_rho ready; previous scalar's fluxes consumed
 #pragma unroll
         for (int u = 0; u < NSTG; ++u) {
             const int e = tid + u * ST_THREADS;
             if (e < ST_HZ * ST_HX)
-                (&l_q[0][0])[e] = gst[u] >= 0 ? fdiv_r(qv_[u], (&l_rho[0][0])[e], (&l_ri[0][0])[e]) : R(0.0);
+                (&l_q[0][0])[e] = gst[u] != NOCELL ? fdiv_r(qv_[u], (&l_rho[0][0])[e], (&l_ri[0][0])[e]) : R(0.0);
         }
         __syncthreads();
         // x faces: face f lies between cells i0+f-1 and i0+f (compute_limited_flux_x at ii = i0+f-1)
@@ -302,6 +327,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
             l_dz[g][c] = dfz;
         }
         __syncthreads();
+        if (sc < 2) load_scalar(sc + 1, qv_, nq_lo, nq_hi, ns0v, ndpv);
         // cells (moist.jl:1480-1516)
 #pragma unroll
         for (int u = 0; u < NCL; ++u) {
@@ -310,7 +336,6 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
             const int r = e / ST_X, c = e - r * ST_X;
             const int k = k0 + r, i = i0 + c;
             if (k >= nz || i >= nx) continue;
-            const int64_t q = gidx(k, i);
             const R dzk = cdz[u];
             const R adv_x = fdiv_r(l_fx[r][c + 1] - l_fx[r][c], dx, rdx);
             const R adv_z = fdiv_r(l_fz[r + 1][c] - l_fz[r][c], dzk, rcz[u]);
@@ -318,9 +343,9 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
             const R diff_x = fdiv_r(kap * (l_dx[r][c + 1] - l_dx[r][c]), dx, rdx);
             const R diff_z = fdiv_r(kap * (l_dz[r + 1][c] - l_dz[r][c]), dzk, rcz[u]);
             const R v = s0v[u] + dt * (adv + (diff_x + diff_z) + dpv[u]);
-            out.f[QV + sc][q] = v;
+            stb(out.f[QV + sc] + o2, ocl[u], v);
             if constexpr (WIDE) {
-                if (wide.f[0]) wide.f[QV + sc][q] = double(v);
+                if (wide.f[0]) wide.f[QV + sc][gidx(k, i)] = double(v);
             }
             if constexpr (P) {
                 if (sc == 0) qn[0][u] = v;
@@ -328,6 +353,13 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
                 else qn[2][u] = v;
             }
             if (isnan(v) && !nan_f) nan_f = 5 + sc;
+        }
+        q_lo = nq_lo;
+        q_hi = nq_hi;
+#pragma unroll
+        for (int u = 0; u < NCL; ++u) {
+            s0v[u] = ns0v[u];
+            dpv[u] = ndpv[u];
         }
     }
 
@@ -343,26 +375,27 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
         const int r = e / ST_X, c = e - r * ST_X;
         const int k = k0 + r, i = i0 + c;
         if (e >= ST_Z * ST_X || k >= nz || i >= nx) continue;
-        const int64_t q = gidx(k, i);
+        const uint32_t q = ocl[u];
         if constexpr (P) {
             // the next stage's diagnostics of the updated cell (dg_th null: none)
             if (dg_th) {
                 R pres;
                 diag_point<S>(D, zv, k, fin[u][0], fin[u][1], qn[0][u], qn[1][u], qn[2][u], dth[u], pres, drt2[u]);
-                dg_pres[q] = pres;
+                stb(dg_pres + o2, q, pres);
             }
         } else {
             const R vr = fin[u][0], vu = fin[u][1], vt = fin[u][2], vw = fin[u][3];
-            out.f[RHO][q] = vr;
-            out.f[RHOU][q] = vu;
-            out.f[RT][q] = vt;
-            out.f[RHOW][q] = vw;
+            stb(out.f[RHO] + o2, q, vr);
+            stb(out.f[RHOU] + o2, q, vu);
+            stb(out.f[RT] + o2, q, vt);
+            stb(out.f[RHOW] + o2, q, vw);
             if constexpr (WIDE) {
                 if (wide.f[0]) {
-                    wide.f[RHO][q] = double(vr) + zv.rho_bar[D.offz + k];
-                    wide.f[RHOU][q] = double(vu);
-                    wide.f[RT][q] = double(vt) + zv.rt_bar[D.offz + k];
-                    wide.f[RHOW][q] = double(vw);
+                    const int64_t g = gidx(k, i);
+                    wide.f[RHO][g] = double(vr) + zv.rho_bar[D.offz + k];
+                    wide.f[RHOU][g] = double(vu);
+                    wide.f[RT][g] = double(vt) + zv.rt_bar[D.offz + k];
+                    wide.f[RHOW][g] = double(vw);
                 }
             }
             int f = 0;
@@ -373,14 +406,14 @@ __global__ void __launch_bounds__(ST_THREADS) k_scalar(const Dom *__restrict__ d
             if (f) report_nan(w.nan_key, ((*w.step_ctr * 3 + stage) << 4) | f);
         }
         // stale halo values surface_fluxes! reads at the next step (moist.jl:723,736)
-        if (k == 0 && i == nx - 1) w.sfh[2 * m + 0] = double(in.f[RHOU][q]);
-        if (k == 0 && i == 0) w.sfh[2 * m + 1] = double(full_rho(in, q, zv, D, 0));
+        if (k == 0 && i == nx - 1) w.sfh[2 * m + 0] = double(in.f[RHOU][gidx(0, i)]);
+        if (k == 0 && i == 0) w.sfh[2 * m + 1] = double(full_rho(in, gidx(0, 0), zv, D, 0));
     }
     if constexpr (P) {
         const int kq = k0 + 2 * (tid / ST_X), iq = i0 + tid % ST_X;
         if (dg_th && kq < nz && iq < nx) {
-            st2<true>(dg_th, D, kq, iq, dth[0], dth[1]);
-            st2<true>(dg_rt2, D, kq, iq, drt2[0], drt2[1]);
+            stb2(dg_th + o2, 4u * moff<true>(nx, kq, iq), dth[0], dth[1]);
+            stb2(dg_rt2 + o2, 4u * moff<true>(nx, kq, iq), drt2[0], drt2[1]);
         }
     }
     // scalar NaN codes (5..7) come after the state fields of the same stage

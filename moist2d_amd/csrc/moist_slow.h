@@ -41,9 +41,12 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
     using KC = K<R>;
     // lX holds theta in the R_rho_theta phase, then u, then w
     __shared__ R lX[SL_HZ][SL_HX];
-    __shared__ R lR[SL_HZ][SL_HX], lRU[SL_HZ][SL_HX], lRW[SL_HZ][SL_HX];
+    // full rho of the tile with one more column and level than the others
+    // (the pair-layout variant forms u and w at the faces from it)
+    __shared__ R lR[SL_HZ + 1][SL_HX + 1];
+    __shared__ R lRU[SL_HZ][SL_HX], lRW[SL_HZ][SL_HX];
     __shared__ R lF[SL_FLUX];
-    __shared__ R lDZ[SL_HZ], lZ[SL_HZ];  // dz and z of levels k0-2 .. k0+ST_Z+1
+    __shared__ R lDZ[SL_HZ + 1], lZ[SL_HZ];  // dz of levels k0-2 .. k0+ST_Z+2, z of k0-2 .. k0+ST_Z+1
 
     // the first stage of a large step advances the step count of the NaN key
     // (no kernel of this launch reads it; the step's physics, which ran
@@ -70,55 +73,64 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
 
     // Staged tile and every global value the cell phases read are loaded
     // up front (fixed trip counts), so the loads of a thread are in flight
-    // together instead of one loop iteration at a time.
+    // together instead of one loop iteration at a time.  Fields are addressed
+    // through the member's base pointer and 32-bit byte offsets.
+    const int64_t o2 = D.off2;
+    auto bo = [&](int k, int i) -> uint32_t { return (uint32_t)sizeof(S) * moff<false>(nx, k, i); };
     constexpr int NSL = (SL_HZ * SL_HX + ST_THREADS - 1) / ST_THREADS;
+    constexpr int NSR = ((SL_HZ + 1) * (SL_HX + 1) + ST_THREADS - 1) / ST_THREADS;  // rho items
+    static_assert(NSR == NSL, "the rho tile's extra column and level fit the staging items");
     constexpr int NCL = (ST_Z * ST_X + ST_THREADS - 1) / ST_THREADS;
     R xu[NSL], xw[NSL];  // staged u and w, held until lX takes them
     {
         R v[NSL][4];
 #pragma unroll
         for (int u = 0; u < NSL; ++u) {
+            // item u: cell (r, c) of the rho tile (SL_HX + 1 wide), and, when
+            // inside the SL_HZ x SL_HX tile, the same cell of the other arrays
             const int e = tid + u * ST_THREADS;
-            const int r = e / SL_HX, c = e - r * SL_HX;
+            const int r = e / (SL_HX + 1), c = e - r * (SL_HX + 1);
             const int k = k0 - 2 + r, i = wrapx(i0 - 2 + c, nx);
-            const bool ok = e < SL_HZ * SL_HX && k >= 0 && k < nz;
-            const int kc = ok ? k : 0, ic = ok ? i : 0;
-            const int64_t q = gidx(kc, ic);
-            const R rho = full_rho(in, q, zv, D, kc), ru = in.f[RHOU][q], rw = in.f[RHOW][q];
-            R uf, wf;
-            if constexpr (P) {
-                // u and w at the faces from rho of the neighbours i+1 and k+1,
-                // the stage diagnostics' expressions (moist.jl:1091-1097, 1184-1189)
-                const R rho_ip = full_rho(in, gidx(kc, wrapx(ic + 1, nx)), zv, D, kc);
-                const int kp = min(kc + 1, ke);
-                const R rho_kp = full_rho(in, gidx(kp, ic), zv, D, kp);
-                uf = fdiv(ru * R(2.0), rho + rho_ip);
-                wf = kc < nz - 1 ? fdiv(rw, f2h(rho, rho_kp, R(dzd[kc]), R(dzd[kp]))) : R(0.0);
-            } else {
-                uf = w.uf[q];
-                wf = w.wf[q];
+            const bool okr = e < (SL_HZ + 1) * (SL_HX + 1) && k >= 0 && k < nz;
+            const bool in_t = r < SL_HZ && c < SL_HX;
+            const int kc = okr ? k : 0, ic = okr ? i : 0;
+            const uint32_t q = bo(kc, ic);
+            R rho = ldb(in.f[RHO] + o2, q);
+            if constexpr (is_mixed<S>()) rho = rho + S(zv.rho_bar[D.offz + kc]);
+            const R ru = ldb(in.f[RHOU] + o2, q), rw = ldb(in.f[RHOW] + o2, q);
+            const R th = ldb(w.theta + o2, (uint32_t)sizeof(S) * moff<P>(nx, kc, ic));
+            if constexpr (!P) {
+                // k_diag's u and w at the faces, item u of the SL_HZ x SL_HX tile
+                const int r2 = e / SL_HX, c2 = e - r2 * SL_HX;
+                const int k2 = k0 - 2 + r2;
+                const bool ok2 = e < SL_HZ * SL_HX && k2 >= 0 && k2 < nz;
+                const uint32_t q2 = bo(ok2 ? k2 : 0, ok2 ? wrapx(i0 - 2 + c2, nx) : 0);
+                xu[u] = ok2 ? ldb(w.uf + o2, q2) : R(0);
+                xw[u] = ok2 ? ldb(w.wf + o2, q2) : R(0);
             }
-            xu[u] = ok ? uf : R(0);
-            xw[u] = ok ? wf : R(0);
-            v[u][0] = ok ? w.theta[widx<P>(D, kc, ic)] : R(0);
-            v[u][1] = ok ? rho : R(0);
-            v[u][2] = ok ? ru : R(0);
-            v[u][3] = ok ? rw : R(0);
+            v[u][0] = okr && in_t ? th : R(0);
+            v[u][1] = okr ? rho : R(0);
+            v[u][2] = okr && in_t ? ru : R(0);
+            v[u][3] = okr && in_t ? rw : R(0);
         }
-        if (tid < SL_HZ) {
+        if (tid < SL_HZ + 1) {
             const int k = k0 - 2 + tid;
             const bool ok = k >= 0 && k < nz;
             lDZ[tid] = ok ? R(dzd[k]) : R(1);
-            lZ[tid] = ok ? R(zd[k]) : R(0);
+            if (tid < SL_HZ) lZ[tid] = ok ? R(zd[k]) : R(0);
         }
 #pragma unroll
         for (int u = 0; u < NSL; ++u) {
             const int e = tid + u * ST_THREADS;
-            if (e >= SL_HZ * SL_HX) continue;
-            (&lX[0][0])[e] = v[u][0];
+            if (e >= (SL_HZ + 1) * (SL_HX + 1)) continue;
+            const int r = e / (SL_HX + 1), c = e - r * (SL_HX + 1);
             (&lR[0][0])[e] = v[u][1];
-            (&lRU[0][0])[e] = v[u][2];
-            (&lRW[0][0])[e] = v[u][3];
+            if (r < SL_HZ && c < SL_HX) {
+                const int et = r * SL_HX + c;
+                (&lX[0][0])[et] = v[u][0];
+                (&lRU[0][0])[et] = v[u][2];
+                (&lRW[0][0])[et] = v[u][3];
+            }
         }
     }
     // interior cells of this thread: levels k0+cr0 and k0+cr0+1 (one level pair)
@@ -127,6 +139,15 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
     const int cr0 = 2 * (tid / ST_X), cc0 = tid % ST_X;
     const int kc0 = k0 + cr0, ic0 = i0 + cc0;
     const bool pair_ok = kc0 < nz && ic0 < nx;
+    // byte offset of the thread's level pair in the work fields' layout P
+    const uint32_t bpw = (uint32_t)sizeof(S) * moff<P>(nx, min(kc0, ke), min(ic0, nx - 1));
+    auto st_pair = [&](S *f, S a, S b) {
+        if constexpr (P) stb2(f + o2, bpw, a, b);
+        else {
+            stb(f + o2, bpw, a);
+            if (kc0 + 1 < nz) stb(f + o2, bpw + (uint32_t)sizeof(S) * (uint32_t)nx, b);
+        }
+    };
     // per interior cell of this thread: the cell phases' global inputs
     R c_drho[NCL], c_drt[NCL], c_rp[NCL], c_rup[NCL], c_rwp[NCL], c_rtp[NCL];
     R c_pres[NCL], c_presx[NCL], c_presz[NCL], c_dup[NCL], c_rho[NCL], c_rhoz[NCL], c_zf[NCL], c_bb[NCL];
@@ -135,23 +156,43 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         const int r = cr0 + u, c = cc0;
         const int k = min(k0 + r, ke), i = min(i0 + c, nx - 1);
         const int kz = min(k + 1, ke);
-        const int64_t q = gidx(k, i);
-        c_drho[u] = w.d_rho[q];
-        c_drt[u] = w.d_rt[q];
-        c_rp[u] = s0.f[RHO][q] - in.f[RHO][q];
-        c_rup[u] = s0.f[RHOU][q] - in.f[RHOU][q];
-        c_rwp[u] = s0.f[RHOW][q] - in.f[RHOW][q];
-        c_rtp[u] = s0.f[RT][q] - in.f[RT][q];
-        c_pres[u] = w.pres[q];
-        c_presx[u] = w.pres[gidx(k, wrapx(i + 1, nx))];
-        c_presz[u] = w.pres[gidx(kz, i)];
-        c_dup[u] = u_phys_2d ? w.d_u2d[q] : (k == 0 ? w.d_u_sf[D.offx + i] : R(0.0));
-        c_rho[u] = in.f[RHO][q];
-        c_rhoz[u] = in.f[RHO][gidx(kz, i)];
+        const uint32_t q = bo(k, i), qx = bo(k, wrapx(i + 1, nx)), qz = bo(kz, i);
+        c_drho[u] = ldb(w.d_rho + o2, q);
+        c_drt[u] = ldb(w.d_rt + o2, q);
+        c_rp[u] = ldb(s0.f[RHO] + o2, q) - ldb(in.f[RHO] + o2, q);
+        c_rup[u] = ldb(s0.f[RHOU] + o2, q) - ldb(in.f[RHOU] + o2, q);
+        c_rwp[u] = ldb(s0.f[RHOW] + o2, q) - ldb(in.f[RHOW] + o2, q);
+        c_rtp[u] = ldb(s0.f[RT] + o2, q) - ldb(in.f[RT] + o2, q);
+        c_pres[u] = ldb(w.pres + o2, q);
+        c_presx[u] = ldb(w.pres + o2, qx);
+        c_presz[u] = ldb(w.pres + o2, qz);
+        c_dup[u] = u_phys_2d ? ldb(w.d_u2d + o2, q) : (k == 0 ? w.d_u_sf[D.offx + i] : R(0.0));
+        c_rho[u] = ldb(in.f[RHO] + o2, q);
+        c_rhoz[u] = ldb(in.f[RHO] + o2, qz);
         c_zf[u] = R(zfd[k]);
         c_bb[u] = R(zv.B_bar[D.offz + k]);
     }
     __syncthreads();
+    if constexpr (P) {
+        // u and w at the faces from the staged rho, rho_u, rho_w and the rho of
+        // the neighbours i+1 and k+1 (the stage diagnostics' expressions,
+        // moist.jl:1091-1097, 1184-1189)
+#pragma unroll
+        for (int u = 0; u < NSL; ++u) {
+            const int e = tid + u * ST_THREADS;
+            const int r = e / SL_HX, c = e - r * SL_HX;
+            const int k = k0 - 2 + r;
+            const bool ok = e < SL_HZ * SL_HX && k >= 0 && k < nz;
+            R uf = R(0), wf = R(0);
+            if (ok) {
+                const R rho = lR[r][c], rho_ip = lR[r][c + 1], rho_kp = lR[r + 1][c];
+                uf = fdiv(lRU[r][c] * R(2.0), rho + rho_ip);
+                wf = k < nz - 1 ? fdiv(lRW[r][c], f2h(rho, rho_kp, lDZ[r], lDZ[r + 1])) : R(0.0);
+            }
+            xu[u] = uf;
+            xw[u] = wf;
+        }
+    }
 
     // ---------------------------------------------------------------- R_rho_theta
     R *FTx = lF;                          // [ST_Z][ST_X+1]: x-face between i0+f-1 and i0+f
@@ -207,15 +248,15 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
             o_mz[u] = RW(k, i);
         }
         if (pair_ok) {
-            st2<P>(w.R_rho, D, kc0, ic0, o_rr[0], o_rr[1]);
-            st2<P>(w.R_rt, D, kc0, ic0, o_rt[0], o_rt[1]);
+            st_pair(w.R_rho, o_rr[0], o_rr[1]);
+            st_pair(w.R_rt, o_rt[0], o_rt[1]);
             // perturbations from the step start (moist.jl:1339-1348)
-            st2<P>(w.rp, D, kc0, ic0, c_rp[0], c_rp[1]);
-            st2<P>(w.rup, D, kc0, ic0, c_rup[0], c_rup[1]);
-            st2<P>(w.rwp, D, kc0, ic0, c_rwp[0], c_rwp[1]);
-            st2<P>(rtp0, D, kc0, ic0, c_rtp[0], c_rtp[1]);
-            st2<P>(w.mom_x, D, kc0, ic0, o_mx[0], o_mx[1]);
-            st2<P>(w.mom_z, D, kc0, ic0, o_mz[0], o_mz[1]);
+            st_pair(w.rp, c_rp[0], c_rp[1]);
+            st_pair(w.rup, c_rup[0], c_rup[1]);
+            st_pair(w.rwp, c_rwp[0], c_rwp[1]);
+            st_pair(rtp0, c_rtp[0], c_rtp[1]);
+            st_pair(w.mom_x, o_mx[0], o_mx[1]);
+            st_pair(w.mom_z, o_mz[0], o_mz[1]);
         }
     }
     __syncthreads();
@@ -278,7 +319,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         const R dpdx = fdiv(c_presx[u] - c_pres[u], dx);
         o_ru[u] = adv - dpdx + diff + c_dup[u];
     }
-    if (pair_ok) st2<P>(w.R_rhou, D, kc0, ic0, o_ru[0], o_ru[1]);
+    if (pair_ok) st_pair(w.R_rhou, o_ru[0], o_ru[1]);
     __syncthreads();
 
     // ---------------------------------------------------------------- R_rho_w
@@ -359,5 +400,5 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         }
         o_rw[u] = res;
     }
-    if (pair_ok) st2<P>(w.R_rhow, D, kc0, ic0, o_rw[0], o_rw[1]);
+    if (pair_ok) st_pair(w.R_rhow, o_rw[0], o_rw[1]);
 }
