@@ -137,6 +137,18 @@ __device__ __forceinline__ double m_fabs(double a) { return fabs(a); }
 __device__ __forceinline__ float m_fabs(float a) { return fabsf(a); }
 __device__ __forceinline__ double m_log1p(double a) { return log1p(a); }
 __device__ __forceinline__ float m_log1p(float a) { return log1pf(a); }
+// log1p(a) + log1p(b) = log1p(a + b + ab).  In float from the hardware log2
+// (v_log_f32) by Goldberg's form log(u) x / (u - 1), u = 1 + x, whose rounding
+// of 1 + x cancels (Goldberg 1991, "What every computer scientist should know
+// about floating-point arithmetic", Theorem 4): a few ulp, where the library
+// log1pf takes some 120 instructions.
+__device__ __forceinline__ double m_log1p_sum(double a, double b) { return log1p(a) + log1p(b); }
+__device__ __forceinline__ float m_log1p_sum(float a, float b) {
+    const float x = a + b + a * b;
+    const float u = 1.0f + x, d = u - 1.0f;
+    const float l = __builtin_amdgcn_logf(u) * 0.69314718055994531f;
+    return d == 0.0f ? x : l * (x * __builtin_amdgcn_rcpf(d));
+}
 __device__ __forceinline__ double m_expm1(double a) { return expm1(a); }
 __device__ __forceinline__ float m_expm1(float a) { return expm1f(a); }
 

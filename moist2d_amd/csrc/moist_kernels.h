@@ -186,7 +186,7 @@ __device__ __forceinline__ void diag_point(const Dom &D, const ZVec &zv, int k, 
         const R dg = fdiv(KC::Cpd * dR - KC::Rd * dCp, cv * KC::Cvd);
         const R gam = fdiv(cp, cv);
         const R rtb = R(zv.rt_bar[D.offz + k]);
-        const R E = dg * R(zv.L_bar[D.offz + k]) + gam * (m_log1p(fdiv(dR, KC::Rd)) + m_log1p(fdiv(rt_s, rtb)));
+        const R E = dg * R(zv.L_bar[D.offz + k]) + gam * m_log1p_sum(fdiv(dR, KC::Rd), fdiv(rt_s, rtb));
         const double pb = zv.p_bar[D.offz + k];
         pres = R(pb) * m_expm1(E);
         pfull = R(pb) + pres;
