@@ -75,8 +75,11 @@ __device__ __forceinline__ R face_z(R w, const R *q, int kk, int ke, R dzk, R dz
         const R h = pos ? R(0.5) : R(-0.5);
         const R lim = b + h * koren(rr) * r_den;
         const bool bdy = (pos && kk == 0) || (neg && kk == ke - 1);
-        const R cap = jmin(fh, fdiv(pos ? q_lo : q_hi, (pos ? w : -w) * dtL));
-        return bdy ? cap : ((pos || neg) ? lim : fh);
+        R v = (pos || neg) ? lim : fh;
+        // the positivity cap's division only where it applies (the bottom
+        // and top faces: most waves skip it)
+        if (bdy) v = jmin(fh, fdiv(pos ? q_lo : q_hi, (pos ? w : -w) * dtL));
+        return v;
     } else {
         if (w > EPS) {
             if (kk == 0) return jmin(f2h_r(s_k, s_kp1, dzk, dzk1, rs), fdiv(q_lo, w * dtL));

@@ -63,6 +63,8 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
     auto gidx = [&](int k, int i) -> int64_t { return D.off2 + (int64_t)k * nx + i; };
     auto DZ = [&](int k) -> R { return lDZ[k - k0 + 2]; };
     auto Z = [&](int k) -> R { return lZ[k - k0 + 2]; };
+    // the float form divides by dx as a times its reciprocal: taken once
+    const R rdx = rcp_once(dx);
     // LDS accessors in absolute (k, i) of the tile neighbourhood
     auto U = [&](int k, int i) -> R { return lX[k - k0 + 2][i - i0 + 2]; };
     auto W = [&](int k, int i) -> R { return lX[k - k0 + 2][i - i0 + 2]; };
@@ -238,9 +240,9 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
             // R_rho (moist.jl:1079-1086)
             const R fkp = k == ke ? R(0.0) : RW(k, i);
             const R fkm = k == 0 ? R(0.0) : RW(k - 1, i);
-            o_rr[u] = -(fdiv(RU(k, i) - RU(k, i - 1), dx) + fdiv(fkp - fkm, DZ(k))) + c_drho[u];
+            o_rr[u] = -(fdiv_r(RU(k, i) - RU(k, i - 1), dx, rdx) + fdiv(fkp - fkm, DZ(k))) + c_drho[u];
             // R_rho_theta
-            const R adv = -(fdiv(FTx[r * (ST_X + 1) + c + 1] - FTx[r * (ST_X + 1) + c], dx) +
+            const R adv = -(fdiv_r(FTx[r * (ST_X + 1) + c + 1] - FTx[r * (ST_X + 1) + c], dx, rdx) +
                             fdiv(FTz[(r + 1) * ST_X + c] - FTz[r * ST_X + c], DZ(k)));
             o_rt[u] = adv + c_drt[u];
             // mass-flux accumulators start from the stage input (moist.jl:1339-1348)
@@ -274,7 +276,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
             const R rc = R(0.5) * (RU(k, i - 1) + RU(k, i));
             const R u_mm = U(k, i - 2), u_m = U(k, i - 1), u_c = U(k, i), u_p = U(k, i + 1);
             fl = upw_flux(rc, rc > EPS, rc < -EPS, u_mm, u_m, u_c, u_p, u_c, u_m, rc * (u_m + u_c) * R(0.5));
-            gx = fdiv(U(k, i) - U(k, i - 1), dx) * RHL(k, i);
+            gx = fdiv_r(U(k, i) - U(k, i - 1), dx, rdx) * RHL(k, i);
         }
         CUx[e] = fl;
         GUx[e] = gx;
@@ -310,13 +312,13 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         o_ru[u] = R(0.0);
         if (k >= nz || i >= nx) continue;
         const int x0 = r * (ST_X + 1) + c;
-        const R druudx = fdiv(CUx[x0 + 1] - CUx[x0], dx);
+        const R druudx = fdiv_r(CUx[x0 + 1] - CUx[x0], dx, rdx);
         const R drwudz = fdiv(CUz[(r + 1) * ST_X + c] - CUz[r * ST_X + c], DZ(k));
         const R adv = -druudx - drwudz;
-        const R u_xx = fdiv(GUx[x0 + 1] - GUx[x0], dx);
+        const R u_xx = fdiv_r(GUx[x0 + 1] - GUx[x0], dx, rdx);
         const R u_zz = fdiv(GUz[(r + 1) * ST_X + c] - GUz[r * ST_X + c], DZ(k));
         const R diff = nu * (u_xx + u_zz);
-        const R dpdx = fdiv(c_presx[u] - c_pres[u], dx);
+        const R dpdx = fdiv_r(c_presx[u] - c_pres[u], dx, rdx);
         o_ru[u] = adv - dpdx + diff + c_dup[u];
     }
     if (pair_ok) st_pair(w.R_rhou, o_ru[0], o_ru[1]);
@@ -345,7 +347,7 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
             if (ud > EPS) xd = fdiv(ud * s_p, R(6.0));
             else if (ud < -EPS) xd = fdiv(ud * s_m, R(6.0));
             else xd = ud * s_c;
-            const R w_ip = fdiv(W(k, L + 1) - W(k, L), dx);
+            const R w_ip = fdiv_r(W(k, L + 1) - W(k, L), dx, rdx);
             const R rho_ip = R(0.25) * (RHL(k, L) + RHL(k + 1, L) + RHL(k, L + 1) + RHL(k + 1, L + 1));
             vx = w_ip * rho_ip;
         }
@@ -378,10 +380,10 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
         if (k >= nz || i >= nx || k == ke) continue;
         const int x0 = r * (ST_X + 1) + c;
         const R zz = Z(k + 1) - Z(k);
-        const R drwudx = fdiv(XZ[x0 + 1] - XD[x0], dx);
+        const R drwudx = fdiv_r(XZ[x0 + 1] - XD[x0], dx, rdx);
         const R drwwdz = fdiv(ZC[(r + 1) * ST_X + c] - ZC[r * ST_X + c], zz);
         const R adv = -drwudx - drwwdz;
-        const R w_xx = fdiv(VX[x0 + 1] - VX[x0], dx);
+        const R w_xx = fdiv_r(VX[x0 + 1] - VX[x0], dx, rdx);
         const R w_zz = fdiv(VZ[(r + 1) * ST_X + c] - VZ[r * ST_X + c], zz);
         const R diff = nu * (w_xx + w_zz);
         R sigma = R(0.0);
