@@ -372,3 +372,36 @@ def test_mixed_adjoint_with_surface_fluxes_and_radiation():
     assert tot <= 1e-3, tot
     for n, e in errs.items():
         assert e <= 5e-3, (n, e)
+
+
+def test_two_handles_in_sequence_with_graph_replay():
+    """The sequence bench.py's adjoint leg runs in one process: an FP64-form
+    adjoint whose forward steps replay a captured graph, that handle
+    destroyed, then a mixed-form handle's adjoint, also with graph replay.
+    Each handle captures its own graph against its own pools.  The FP64 run
+    matches the oracle's loss; the mixed run equals, bit for bit, the same
+    adjoint on a handle that launches directly."""
+    p, s, tt = _case(nx=32, nz=16)
+    Jo, _, _ = O.adjoint_run(s.copy(), p, 4, tt, ckpt=2)
+    print("FP64 adjoint with graph replay, 32x16, 4 steps", flush=True)
+    eng = core.Engine(p, core.FP64)
+    eng.use_graph(True)
+    eng.upload(s)
+    J64 = eng.adjoint(4, [tt], ckpt=2)[0]
+    eng.close()
+    assert abs(J64 - Jo) <= 1e-12 * abs(Jo)
+    out = []
+    for graph in (True, False):
+        print(f"mixed adjoint, graph replay {graph}", flush=True)
+        eng = core.Engine(p, core.MIXED)
+        eng.use_graph(graph)
+        eng.upload(s)
+        J = eng.adjoint(4, [tt], ckpt=2)[0]
+        g, gK = eng.adjoint_grad(0)
+        eng.close()
+        out.append((J, g, gK))
+    assert out[0][0] == out[1][0]
+    for f in range(7):
+        np.testing.assert_array_equal(out[0][1][f], out[1][1][f], err_msg=PROG7[f])
+    np.testing.assert_array_equal(out[0][2], out[1][2])
+    assert np.isfinite(out[0][0]) and all(np.all(np.isfinite(a)) for a in out[0][1])
