@@ -682,12 +682,24 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     constexpr int HZ = AD_Z + 4, NH = HZ * AD_X, NT = AD_X * AD_Z, NS = (NH + NT - 1) / NT;
     __shared__ RC sR[2][HZ][AD_X];      // rwp_n, theta
     __shared__ double sD[4][HZ][AD_X];  // rho_w, Lrtp, Lrp, Lmz
+    __shared__ double4 sG[HZ];          // per-level geometry of the tile's levels
+    __shared__ double sDZ[HZ];
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
     const int i0 = t.y, k0 = t.z;
     const int i = i0 + threadIdx.x, k = k0 + threadIdx.y, tid = threadIdx.y * AD_X + threadIdx.x;
     const int nx = D.nx, nz = D.nz, ke = nz - 1;
+    // this cell's read-modify-write values and the tile's level geometry,
+    // loaded with the staged fields (before the barrier)
+    const bool cell_ok = i < nx && k < nz;
+    const int64_t qc = D.off2 + (int64_t)min(k, ke) * nx + min(i, nx - 1);
+    const double lth0 = cell_ok ? a.Lth[qc] : 0.0, lrwp0 = cell_ok && k <= ke - 1 ? a.Lrwp[qc] : 0.0;
+    if (tid < HZ) {
+        const int kk = min(max(k0 - 2 + tid, 0), ke);
+        sG[tid] = zv.geo[D.offz + kk];
+        sDZ[tid] = zv.dz[D.offz + kk];
+    }
     {
         RC xr[NS][2];
         double xd[NS][4];
@@ -716,8 +728,9 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     }
     __syncthreads();
     if (i >= nx || k >= nz) return;
-    const double *dzd = zv.dz + D.offz;
-    const double4 *gz = zv.geo + D.offz;  // per level: 1/dz, 1/(z_k+1 - z_k), 1/(dz_k + dz_k+1)
+    // per level (staged): 1/dz, 1/(z_k+1 - z_k), 1/(dz_k + dz_k+1), and dz
+    auto gzl = [&](int kk) { return sG[kk - k0 + 2]; };
+    auto dzl = [&](int kk) { return sDZ[kk - k0 + 2]; };
     const double EPS = 1e-12, rns = 1.0 / double(nsub);
     const double dt_s = D.dt / double(div) / double(nsub);
     auto G = [&](int kk) -> int64_t { return D.off2 + (int64_t)kk * nx + i; };
@@ -740,30 +753,29 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
         pos = sw > EPS && f + 1 > 1;
         neg = !pos && sw < -EPS && f + 1 < ke;
     };
-    const double lth0 = a.Lth[G(k)], lrwp0 = k <= ke - 1 ? a.Lrwp[G(k)] : 0.0;
     double lth = 0.0;
     for (int f = k - 2; f <= k + 1; ++f) {
         if (f < 0 || f > ke - 1) continue;
-        const double rden = gz[f].z;
-        const double w0 = dzd[f + 1] * rden, w1 = dzd[f] * rden;
+        const double rden = gzl(f).z;
+        const double w0 = dzl(f + 1) * rden, w1 = dzl(f) * rden;
         const double w = RWN(f);
         bool pos, neg;
         brp(f, pos, neg);
-        lth += -dt_s * gz[f].x * LRTP(f) * w * upw_coef(pos, neg, k - f, w0, w1);
+        lth += -dt_s * gzl(f).x * LRTP(f) * w * upw_coef(pos, neg, k - f, w0, w1);
         brm(f, pos, neg);
-        lth += dt_s * gz[f + 1].x * LRTP(f + 1) * w * upw_coef(pos, neg, k - f, w0, w1);
+        lth += dt_s * gzl(f + 1).x * LRTP(f + 1) * w * upw_coef(pos, neg, k - f, w0, w1);
     }
     double lrwp = 0.0;
     if (k <= ke - 1) {
-        const double rden = gz[k].z;
-        const double w0 = dzd[k + 1] * rden, w1 = dzd[k] * rden;
+        const double rden = gzl(k).z;
+        const double w0 = dzl(k + 1) * rden, w1 = dzl(k) * rden;
         bool pos, neg;
         brp(k, pos, neg);
         const double gp = upw_val(pos, neg, THv(k - 1), THv(k), THv(k + 1), THv(k + 2), w0, w1);
         brm(k, pos, neg);
         const double gm = upw_val(pos, neg, THv(k - 1), THv(k), THv(k + 1), THv(k + 2), w0, w1);
-        lrwp = LMZ(k) * rns - dt_s * gz[k].x * LRTP(k) * gp + dt_s * gz[k + 1].x * LRTP(k + 1) * gm -
-               dt_s * gz[k].x * LRP(k) + dt_s * gz[k + 1].x * LRP(k + 1);
+        lrwp = LMZ(k) * rns - dt_s * gzl(k).x * LRTP(k) * gp + dt_s * gzl(k + 1).x * LRTP(k + 1) * gm -
+               dt_s * gzl(k).x * LRP(k) + dt_s * gzl(k + 1).x * LRP(k + 1);
     }
     // read-modify-writes last, after every load of the kernel
     a.Lth[G(k)] = lth0 + lth;
@@ -1027,6 +1039,9 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     constexpr int HX = AD_X + 2, HZ = AD_Z + 3, NH = HX * HZ, NT = AD_X * AD_Z, NS = (NH + NT - 1) / NT;
     __shared__ double sY[HZ][HX];
     __shared__ RC sF[7][HZ][HX];  // rwp_n, rwp_s, theta, rt2pres, rup_n, R_rho_theta, rtp_s
+    constexpr int GZ = AD_Z + 6;  // levels k0-3 .. k0+AD_Z+2 of the staged geometry
+    __shared__ double4 sG[GZ];
+    __shared__ double sDZ[GZ];
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
@@ -1035,6 +1050,18 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     const int nx = D.nx, nz = D.nz, n = nz - 1;
     if (n <= 0) return;  // uniform per block
     auto Gx = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
+    // this cell's read-modify-write values and the tile's level geometry,
+    // loaded with the staged fields (before the barrier)
+    const bool cell_ok = i < nx && k < nz;
+    const int64_t q = D.off2 + (int64_t)min(k, n) * nx + min(i, nx - 1);
+    const double Lrt0 = cell_ok ? a.Lrt[q] : 0.0, Lth0 = cell_ok ? a.Lth[q] : 0.0, lr = cell_ok ? a.Lrp[q] : 0.0,
+                 lt = cell_ok ? a.Lrtp[q] : 0.0, LRr0 = cell_ok ? a.LRr[q] : 0.0, LRt0 = cell_ok ? a.LRt[q] : 0.0,
+                 LRw0 = cell_ok && k <= n - 1 ? a.LRw[q] : 0.0;
+    if (tid < GZ) {
+        const int kk = min(max(k0 - 3 + tid, 0), n);
+        sG[tid] = zv.geo[D.offz + kk];
+        sDZ[tid] = zv.dz[D.offz + kk];
+    }
     {
         const RC *src[7] = {rwp_n, rwp_s, a.theta, a.rt2, rup_n, a.R_rt, rtp_s};
         double y[NS];
@@ -1060,12 +1087,23 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     }
     __syncthreads();
     if (i >= nx || k >= nz) return;
-    const double *dzd = zv.dz + D.offz;
+    // the staged per-level geometry, indexed by level: gz[kk] and dzd[kk]
+    struct LG {
+        const double4 *g;
+        int o;
+        __device__ const double4 &operator[](int kk) const { return g[kk - o]; }
+    };
+    struct LD {
+        const double *g;
+        int o;
+        __device__ double operator[](int kk) const { return g[kk - o]; }
+    };
+    const LG gz{sG, k0 - 3};
+    const LD dzd{sDZ, k0 - 3};
     const double GRAV = K<double>::GRAV, dx = D.dx, rdx = 1.0 / dx;
     const double dt_s = D.dt / double(div) / double(nsub);
     const double fact_tp = 0.5 * (1.0 + D.beta), fact_tm = 0.5 * (1.0 - D.beta);
     const double fact = fact_tm / fact_tp;
-    const double4 *gz = zv.geo + D.offz;
     const double dt_tp2 = (dt_s * fact_tp) * (dt_s * fact_tp);
     // staged values of level kk (column i, or ii for theta and rup_n)
     auto SF = [&](int f, int kk, int ii) -> RC { return sF[f][kk - k0 + 2][ii - i0 + 1]; };
@@ -1101,10 +1139,6 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
         const double inc = dt_s * (RRT(kk) + (-(fip - fim) * rdx));
         return RTS(kk) + fact_tp * inc;
     };
-    const int64_t q = Gx(k, i);
-    // this cell's read-modify-write values, read before any store
-    const double Lrt0 = a.Lrt[q], Lth0 = a.Lth[q], lr = a.Lrp[q], lt = a.Lrtp[q], LRr0 = a.LRr[q],
-                 LRt0 = a.LRt[q], LRw0 = k <= n - 1 ? a.LRw[q] : 0.0;
     double lrt = 0.0, lth = 0.0, Lrp_tp = 0.0, Lrtp_tp = 0.0, lws = 0.0, lRw = 0.0;
     if (k <= n - 1) {
         const double y = yv(k);
