@@ -2,8 +2,9 @@
 
 `python -m moist2d_amd.build` compiles the HIP sources under csrc/ into
 moist2d_amd/libmoist2d_hip.so.  FP contraction is off so that the FP64 form
-performs the same IEEE operations as the reference's Float64 arithmetic;
-packed FP32 instructions are disabled.
+performs the same IEEE operations as the reference's Float64 arithmetic; the
+mixed form's step kernels (csrc/moist_mixed.hip) turn it on for their own
+translation unit with a pragma.  Packed FP32 instructions are disabled.
 """
 from __future__ import annotations
 
@@ -12,7 +13,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "moist_api.hip")
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("moist_api.hip", "moist_mixed.hip")]
 OUT = os.path.join(HERE, "libmoist2d_hip.so")
 
 HIPCC_FLAGS = [
@@ -25,7 +26,7 @@ HIPCC_FLAGS = [
 def build(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = OUT + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, SRC]
+    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, *SRCS]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

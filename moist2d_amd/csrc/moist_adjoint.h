@@ -2272,7 +2272,10 @@ __global__ void __launch_bounds__(64)
 // gathered at the ground (own cell, the right column's left face, the left
 // column's right cell) and its forward d_qv_sf; null otherwise.  The gray
 // radiation's adjoint is added afterwards by ka_radiation.
-__global__ void __launch_bounds__(AD_X *AD_Z)
+// At least 2 waves per SIMD (at most 256 registers; the dual-number Kessler
+// pass wants 262, and at one wave per SIMD nothing hides its latencies): a few
+// values spill, and the 64-member 1024x128 reverse step ran 4% faster.
+__global__ void __launch_bounds__(AD_X *AD_Z, 2)
     ka_physics(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s,
                SSet<double> lph, SSet<double> ls, SSet<double> ls0, SSet<double> lout, double *gk0, double *gk1,
                double *gk2, const double *__restrict__ sfb, int64_t nxs) {

@@ -29,6 +29,7 @@
 #include "../../include/moist2d.h"
 #include "moist_kernels.h"
 namespace mo {
+MO_MIXED_KERNELS(extern)  // compiled in moist_mixed.hip
 #include "moist_adjoint.h"
 #include "moist_history.h"
 }
@@ -1213,11 +1214,22 @@ template <class H> int adjoint_run_t(moist_handle *h, int nsteps, int ckpt, cons
             // one recording pass over the segment, then the reverse sweep from the records
             for (int n = n0; n < n1; ++n) seg_record(h, n - n0);
         } else {
-            // states of the segment, then per step (in reverse) its record and adjoint
+            // states of the segment, then per step (in reverse) its record and adjoint.
+            // The mixed form advances by its recording pass (into the one record
+            // block, rewritten later): those kernels, not the step's level-pair
+            // ones, make the states the reverse steps start from, as with segment
+            // records, so both modes replay the same values (the two layouts'
+            // kernels round differently under FP contraction).
             for (int n = n0; n < n1; ++n) {
                 copy_bytes(h, seg_step(h, n - n0), h->pool, state_bytes);
                 copy_set(h, seg_sfh(h, n - n0), h->sfh, 2 * h->nm);
-                if (n + 1 < n1 && run_steps(h, 1)) return -1;
+                if (n + 1 >= n1) continue;
+                if (h->precision == MOIST_MIXED) {
+                    A.cur = A.recs;
+                    record_step(h, n - n0, true);
+                } else if (run_steps(h, 1)) {
+                    return -1;
+                }
             }
         }
         for (int n = n1 - 1; n >= n0; --n) seg_reverse(h, n - n0);

@@ -372,6 +372,12 @@ template <class S>
 __global__ void __launch_bounds__(256) k_physics(const Dom *__restrict__ doms, const int4 *__restrict__ tiles,
                                                  ZVec zv, SSet<S> s, Work<S> w, int store_diag,
                                                  int diag_pairs, SSet<double> wide) {
+    // No FP contraction here, in either form: the condensate of the
+    // saturation adjustment is qv_init - qv, and with qv_init = rho_qv / rho
+    // fused into that subtraction it would be the product's rounding error
+    // (about 1e-9 kg/m^3 of cloud water in every cell of a subsaturated
+    // column) instead of zero.
+#pragma clang fp contract(off)
     using R = S;
     using KC = K<R>;
     const int4 t = tiles[blockIdx.x];
@@ -593,5 +599,47 @@ __global__ void k_download(Dom D, const S *__restrict__ src, const S *__restrict
     if (base) v += base[k];
     host[(int64_t)ih * ka + k] = H(v);
 }
+
+// =============================================================================
+// The float instantiations of the step kernels (the mixed form's step, and the
+// FP32 recording pass of its adjoint) live in a translation unit of their own,
+// moist_mixed.hip, compiled with FP contraction: a * b + c becomes one fused
+// multiply-add, rounded once instead of twice (fewer instructions, and no less
+// accurate).  The FP64 form keeps the reference's separately rounded
+// operations: moist_api.hip, compiled without contraction, declares these
+// instantiations extern (MO_MIXED_KERNELS(extern)) and so instantiates none of
+// them itself.  Every float step kernel the host launches must be listed here.
+// =============================================================================
+#define MO_AC_ARGS const Dom *, const int4 *, ZVec, SSet<float>, Work<float>, AcBuf<float>, int, double, int, int
+#define MO_AC_SHAPE(EXT, L_, NT_, W_)                                                       \
+    EXT template __global__ void k_acoustic<float, L_, NT_, W_>(MO_AC_ARGS);                \
+    EXT template __global__ void k_acoustic_pairs<L_, NT_, W_, false>(MO_AC_ARGS);          \
+    EXT template __global__ void k_acoustic_pairs<L_, NT_, W_, true>(MO_AC_ARGS);
+#define MO_SCALAR_ARGS                                                                                       \
+    const Dom *, const int4 *, ZVec, SSet<float>, SSet<float>, SSet<float>, Work<float>, const float *,   \
+        const float *, int, double, int, float *, float *, float *, SSet<double>
+#define MO_MIXED_KERNELS(EXT)                                                                                   \
+    EXT template __global__ void k_diag<float, false>(const Dom *, const int4 *, ZVec, SSet<float>, Work<float>); \
+    EXT template __global__ void k_diag<float, true>(const Dom *, const int4 *, ZVec, SSet<float>, Work<float>);  \
+    EXT template __global__ void k_surface<float>(const Dom *, const int2 *, ZVec, SSet<float>, Work<float>);     \
+    EXT template __global__ void k_radiation<float>(const Dom *, const int2 *, ZVec, SSet<float>, Work<float>);   \
+    EXT template __global__ void k_physics<float>(const Dom *, const int4 *, ZVec, SSet<float>, Work<float>, int, \
+                                                  int, SSet<double>);                                            \
+    EXT template __global__ void k_slow<float, false>(const Dom *, const int4 *, ZVec, SSet<float>, SSet<float>,  \
+                                                      Work<float>, float *, int, int *);                         \
+    EXT template __global__ void k_slow<float, true>(const Dom *, const int4 *, ZVec, SSet<float>, SSet<float>,   \
+                                                     Work<float>, float *, int, int *);                          \
+    EXT template __global__ void k_scalar<float, false>(MO_SCALAR_ARGS);                                        \
+    EXT template __global__ void k_scalar<float, true>(MO_SCALAR_ARGS);                                         \
+    MO_AC_SHAPE(EXT, 2, 256, 1)                                                                                   \
+    MO_AC_SHAPE(EXT, 2, 512, 1)                                                                                   \
+    MO_AC_SHAPE(EXT, 2, 1024, 1)                                                                                  \
+    MO_AC_SHAPE(EXT, 4, 256, 1)                                                                                   \
+    MO_AC_SHAPE(EXT, 4, 512, 1)                                                                                   \
+    MO_AC_SHAPE(EXT, 4, 1024, 1)                                                                                  \
+    MO_AC_SHAPE(EXT, 2, 512, 8)                                                                                   \
+    MO_AC_SHAPE(EXT, 2, 1024, 8)                                                                                  \
+    MO_AC_SHAPE(EXT, 4, 512, 8)                                                                                   \
+    MO_AC_SHAPE(EXT, 4, 1024, 8)
 
 }  // namespace mo
