@@ -1185,22 +1185,28 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
 }
 
 // ---- acoustic substep, vertical part, one kernel per column ------------------------
-// (1) + (2) + (3) of one reverse substep in one launch: one wave per column,
-// lane l owning the consecutive levels [l Lc, l Lc + Lc) and the solve's rows
-// [l Ls, l Ls + Ls) (Lc = ceil(nz / 64), Ls = ceil((nz - 1) / 64) <= 4).  The
-// column's inputs of the three parts are staged in LDS once; the solve's
-// right-hand side and its solution stay in LDS instead of a global round
-// trip, and two launches per substep are saved.  The per-level geometry (dz
-// and the four grid factors) is staged in LDS with the column, so the loops
-// over neighbouring faces wait on no global load.  The arithmetic of every
-// value is that of ka_ac_v1, ka_ac_solve (one column per wave, CH = 64) and
-// ka_ac_v2, in the same order.  cols: {member, column} per block (one wave).
+// (1) + (2) + (3) of one reverse substep in one launch: NW waves (one or two)
+// per column, thread t owning the consecutive levels [t Lc, t Lc + Lc) and the
+// solve's rows [t Ls, t Ls + Ls) (Lc = ceil(nz / 64 NW), Ls = ceil((nz - 1) /
+// 64 NW) <= 4).  The column's inputs of the three parts are staged in LDS
+// once; the solve's right-hand side and its solution stay in LDS instead of a
+// global round trip, and two launches per substep are saved.  The per-level
+// geometry (dz and the four grid factors) is staged in LDS with the column, so
+// the loops over neighbouring faces wait on no global load.  With one wave the
+// arithmetic of every value is that of ka_ac_v1, ka_ac_solve (one column per
+// wave, CH = 64) and ka_ac_v2, in the same order.  Two waves per column (taller
+// columns: half the levels per thread, and twice the waves on few columns)
+// split the solve into 128 chunks: the scans run within each wave by shuffles,
+// and the lower wave's aggregate (bottom-up scans) or the upper wave's
+// (top-down) is composed in through LDS, which regroups the scans' products
+// (rounding differences only).  cols: {member, column} per block.
 // Dynamic LDS: AC_COL_ARRAYS doubles per level of the tallest member.
 constexpr int AC_COL_ARRAYS = 16;
 constexpr int AC_COL_MAX_NZ = 256;  // Lc, Ls <= 4: the solve's rows stay in registers
-// LCM: levels a lane may own (1, 2 or 4 for members up to 64, 128, 256 levels)
-template <class RC, int LCM>
-__global__ void __launch_bounds__(64)
+// LCM: levels a thread may own; NW: waves per column (members up to 64 levels:
+// <1, 1>; up to 128: <1, 2>; up to 256: <2, 2>)
+template <class RC, int LCM, int NW>
+__global__ void __launch_bounds__(64 * NW)
     ka_ac_col(const Dom *__restrict__ doms, const int2 *__restrict__ cols, ZVec zv, SSet<double> s, AdjW<RC> a,
               const RC *__restrict__ rtp_s, const RC *__restrict__ rup_n, const RC *__restrict__ rwp_s,
               const RC *__restrict__ rwp_n, int div, int nsub, int factor, int max_ka) {
@@ -1210,7 +1216,13 @@ __global__ void __launch_bounds__(64)
     const Dom &D = doms[t.x];
     const int nx = D.nx, nz = D.nz, ke = nz - 1, n = nz - 1;
     if (n <= 0) return;
-    const int i = t.y, lane = threadIdx.x;
+    static_assert(NW == 1 || NW == 2, "one or two waves per column");
+    const int i = t.y, lane = threadIdx.x, wl = lane & 63, wave = lane >> 6;
+    // the scans' values that cross the wave boundary (NW = 2): the Moebius
+    // aggregate of the lower wave (0-3), the bottom-up affine aggregate of the
+    // lower wave (4-5), the top-down one of the upper wave (6-7), c' of the
+    // lower wave's last chunk (8) and a of the upper wave's first row (9)
+    __shared__ double xw[10];
     double *sRWN = lcol, *sRWS = lcol + max_ka, *sRW = lcol + 2 * max_ka, *sTH = lcol + 3 * max_ka,
            *sRT = lcol + 4 * max_ka, *sLRTP = lcol + 5 * max_ka, *sLRP = lcol + 6 * max_ka,
            *sLMZ = lcol + 7 * max_ka, *sTP = lcol + 8 * max_ka, *sRHS = lcol + 9 * max_ka, *sY = lcol + 10 * max_ka;
@@ -1229,7 +1241,7 @@ __global__ void __launch_bounds__(64)
     const RC *TH = a.theta;
     auto G = [&](int kk) -> int64_t { return D.off2 + (int64_t)kk * nx + i; };
     auto Gx = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
-    const int Lc = (nz + 63) / 64, kc0 = lane * Lc;
+    const int Lc = (nz + 64 * NW - 1) / (64 * NW), kc0 = lane * Lc;
 
     // ---- stage the column (and read this lane's read-modify-write values)
     double lth0[LCM], lrwp0[LCM], lrt0[LCM], lr0[LCM], lt0[LCM], lRr0[LCM], lRt0[LCM], lRw0[LCM];
@@ -1321,7 +1333,7 @@ __global__ void __launch_bounds__(64)
     auto Wb = [&](int kk) { return dt_tp2 * (sRT[kk] * gzl(kk).x + sRT[kk + 1] * gzl(kk + 1).x) * th(kk) * gzl(kk).y; };
     auto Wc = [&](int kk) { return -dt_tp2 * (sRT[kk + 1] * th(kk + 1) * (gzl(kk).y * gzl(kk + 1).x) + gzl(kk).w); };
     {
-        constexpr int CH = 64, LR = 4;
+        constexpr int CH = 64 * NW, LR = 4;
         const int ch = lane;
         const int L = (n + CH - 1) / CH, j0 = ch * L, j1 = min(j0 + L, n);
         double cpv[LR], rmv[LR], acv[LR];
@@ -1340,10 +1352,10 @@ __global__ void __launch_bounds__(64)
                 mob_norm(m00, m01, m10, m11);
             }
 #pragma unroll
-            for (int d = 1; d < CH; d <<= 1) {
+            for (int d = 1; d < 64; d <<= 1) {
                 const double p00 = __shfl_up(m00, d), p01 = __shfl_up(m01, d), p10 = __shfl_up(m10, d),
                              p11 = __shfl_up(m11, d);
-                if (ch >= d) {
+                if (wl >= d) {
                     const double n00 = m00 * p00 + m01 * p10, n01 = m00 * p01 + m01 * p11;
                     const double n10 = m10 * p00 + m11 * p10, n11 = m10 * p01 + m11 * p11;
                     m00 = n00;
@@ -1353,7 +1365,31 @@ __global__ void __launch_bounds__(64)
                     mob_norm(m00, m01, m10, m11);
                 }
             }
-            const double e01 = __shfl_up(m01, 1), e11 = __shfl_up(m11, 1);
+            if constexpr (NW == 2) {
+                // the upper wave's composites applied after the lower wave's aggregate
+                if (lane == 63) {
+                    xw[0] = m00;
+                    xw[1] = m01;
+                    xw[2] = m10;
+                    xw[3] = m11;
+                }
+                __syncthreads();
+                if (wave == 1) {
+                    const double p00 = xw[0], p01 = xw[1], p10 = xw[2], p11 = xw[3];
+                    const double n00 = m00 * p00 + m01 * p10, n01 = m00 * p01 + m01 * p11;
+                    const double n10 = m10 * p00 + m11 * p10, n11 = m10 * p01 + m11 * p11;
+                    m00 = n00;
+                    m01 = n01;
+                    m10 = n10;
+                    m11 = n11;
+                    mob_norm(m00, m01, m10, m11);
+                }
+            }
+            double e01 = __shfl_up(m01, 1), e11 = __shfl_up(m11, 1);
+            if (NW == 2 && lane == 64) {
+                e01 = xw[1];
+                e11 = xw[3];
+            }
             double cprev = ch == 0 ? 0.0 : e01 / e11;
 #pragma unroll
             for (int q = 0; q < LR; ++q) {
@@ -1388,7 +1424,14 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
         for (int q = 0; q < LR; ++q)
             if (q == L - 1) cp_last = cpv[q];
-        const double cp_below = __shfl_up(cp_last, 1), a_above = __shfl_down(acv[0], 1);
+        double cp_below = __shfl_up(cp_last, 1), a_above = __shfl_down(acv[0], 1);
+        if constexpr (NW == 2) {
+            if (lane == 63) xw[8] = cp_last;
+            if (lane == 64) xw[9] = acv[0];
+            __syncthreads();
+            if (lane == 64) cp_below = xw[8];
+            if (lane == 63) a_above = xw[9];
+        }
         double A = 1.0, B = 0.0;
 #pragma unroll
         for (int q = 0; q < LR; ++q) {
@@ -1399,8 +1442,20 @@ __global__ void __launch_bounds__(64)
                 A = al * A;
             }
         }
-        scan_affine_up<CH>(A, B, ch);
+        scan_affine_up<64>(A, B, wl);
+        if constexpr (NW == 2) {
+            if (lane == 63) {
+                xw[4] = A;
+                xw[5] = B;
+            }
+            __syncthreads();
+            if (wave == 1) {
+                B = A * xw[5] + B;
+                A = A * xw[4];
+            }
+        }
         double wv = __shfl_up(B, 1);
+        if (NW == 2 && lane == 64) wv = xw[5];
         if (ch == 0) wv = 0.0;
 #pragma unroll
         for (int q = 0; q < LR; ++q) {
@@ -1423,8 +1478,20 @@ __global__ void __launch_bounds__(64)
                 A = al * A;
             }
         }
-        scan_affine_down<CH>(A, B, ch);
+        scan_affine_down<64>(A, B, wl);
+        if constexpr (NW == 2) {
+            if (lane == 64) {
+                xw[6] = A;
+                xw[7] = B;
+            }
+            __syncthreads();
+            if (wave == 0) {
+                B = A * xw[7] + B;
+                A = A * xw[6];
+            }
+        }
         double yv = __shfl_down(B, 1);
+        if (NW == 2 && lane == 63) yv = xw[7];
         if (ch == CH - 1) yv = 0.0;
 #pragma unroll
         for (int q = LR - 1; q >= 0; --q) {

@@ -1037,14 +1037,16 @@ template <class RC> void enqueue_step_adj(moist_handle *h, const double *sn, con
                 const RC *rtp_s = rec_rtp<RC>(h, st, m), *rup_n = rec_rup<RC>(h, st, m + 1),
                          *rwp_s = rec_rwp<RC>(h, st, m), *rwp_n = rec_rwp<RC>(h, st, m + 1);
                 const int fac = n == nsub ? 1 : 0;
+                // one wave per column up to 64 levels, two above (half the levels per
+                // thread, twice the waves)
                 if (h->max_ka <= 64)
-                    hipLaunchKernelGGL((ka_ac_col<RC, 1>), dim3(h->nacols), dim3(64), lds, h->stream, h->dom_d,
+                    hipLaunchKernelGGL((ka_ac_col<RC, 1, 1>), dim3(h->nacols), dim3(64), lds, h->stream, h->dom_d,
                                        h->acols, zv, in, a, rtp_s, rup_n, rwp_s, rwp_n, div, nsub, fac, h->max_ka);
                 else if (h->max_ka <= 128)
-                    hipLaunchKernelGGL((ka_ac_col<RC, 2>), dim3(h->nacols), dim3(64), lds, h->stream, h->dom_d,
+                    hipLaunchKernelGGL((ka_ac_col<RC, 1, 2>), dim3(h->nacols), dim3(128), lds, h->stream, h->dom_d,
                                        h->acols, zv, in, a, rtp_s, rup_n, rwp_s, rwp_n, div, nsub, fac, h->max_ka);
                 else
-                    hipLaunchKernelGGL((ka_ac_col<RC, 4>), dim3(h->nacols), dim3(64), lds, h->stream, h->dom_d,
+                    hipLaunchKernelGGL((ka_ac_col<RC, 2, 2>), dim3(h->nacols), dim3(128), lds, h->stream, h->dom_d,
                                        h->acols, zv, in, a, rtp_s, rup_n, rwp_s, rwp_n, div, nsub, fac, h->max_ka);
             } else {
                 hipLaunchKernelGGL(ka_ac_v1<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in,

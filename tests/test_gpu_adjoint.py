@@ -301,6 +301,30 @@ def test_adjoint_edge_cases(kw):
     _compare(f"edge {kw}", p, s, tt, 4, 2)
 
 
+@pytest.mark.parametrize("nz", [100, 160])
+def test_adjoint_two_wave_column_kernel(nz):
+    """Columns above 64 levels take two waves in the column kernel (100
+    levels: one level per thread; 160: two).  Its solve then runs in 128
+    chunks, the lower wave's aggregate composed into the upper wave's scans
+    through LDS (and the upper wave's into the lower one's, top down), where
+    the three-kernel form runs 64 chunks in one wave: the scans' products
+    group differently, so the two forms agree to rounding, not bitwise.  FP64
+    form against the oracle's adjoint (bar 1e-8, as every adjoint parity
+    case) and against the three-kernel form (bar 1e-11)."""
+    p, s, tt = _case(nz=nz)
+    g, gK = _compare(f"two-wave column kernel nz={nz}", p, s, tt, 3, 2)
+    eng = core.Engine(p, core.FP64)
+    eng.adjoint_column_kernel(False)
+    eng.upload(s)
+    eng.adjoint(3, [tt], ckpt=2)
+    g3, gK3 = eng.adjoint_grad(0)
+    eng.close()
+    h = p.halo
+    for f, n in enumerate(PROG7):
+        e = _rel(g[f][:, h:h + p.Nx], g3[f][:, h:h + p.Nx])
+        assert e <= 1e-11, (n, e)
+
+
 @pytest.mark.parametrize("precision", [core.FP64, core.MIXED])
 def test_adjoint_column_kernel_matches_three_kernels(precision):
     """The vertical part of each reverse acoustic substep as one kernel per
