@@ -166,8 +166,10 @@ class Engine:
         self._check(lib().moist_set_option(self._h, 0, 1 if on else 0), "moist_set_option")
 
     def adjoint_records(self, mode):
-        """Forward records of the reverse sweep: "auto", "step" or "segment"."""
-        v = {"auto": 0, "step": 1, "segment": 2}[mode]
+        """Forward records of the reverse sweep: "auto", "step", "segment" or
+        "pipelined" (whole segments, one segment's recording pass beside the
+        next one's reverse sweep)."""
+        v = {"auto": 0, "step": 1, "segment": 2, "pipelined": 3}[mode]
         self._check(lib().moist_set_option(self._h, 1, v), "moist_set_option")
 
     def adjoint_column_kernel(self, on=True):

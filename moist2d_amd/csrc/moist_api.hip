@@ -138,7 +138,7 @@ struct moist_handle {
     int64_t ac_launches = 0;
     // reverse mode (allocated by the first moist_adjoint_run)
     AdjState *adj = nullptr;
-    int adj_mode = 0;  // forward records of the reverse sweep: 0 automatic, 1 per step, 2 per segment
+    int adj_mode = 0;  // forward records of the reverse sweep: 0 automatic, 1 per step, 2 per segment, 3 pipelined
     int64_t dev_bytes = 0;  // device memory of the state pools and tables
     // output_history: derived fields (allocated on first request), block partials, per-member stats
     double *hist_fields = nullptr, *hist_part = nullptr, *hist_stats = nullptr, *diag_all = nullptr;
@@ -736,6 +736,13 @@ struct AdjState {
     size_t rb_bytes = 0;
     int rec_elem = 8;
     int segrec = 0, nblocks = 1;
+    // pipelined segments: two halves of the segment buffers (records, step
+    // inputs, stale halo pairs); segment sg uses half sg & 1.  The recording
+    // pass of segment sg - 1 runs on stream s2 while stream `stream` reverses
+    // segment sg; events order each half's writer after its last reader.
+    int overlap = 0, half = 0;
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_go = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
     double *lam = nullptr, *lam2 = nullptr, *ls0 = nullptr, *lph = nullptr, *wk = nullptr, *lim = nullptr;
     double *gk = nullptr, *tgt = nullptr, *fin = nullptr, *J = nullptr, *gK = nullptr;
     // surface fluxes: the stale halo pair of every step of a segment (2 per
@@ -753,7 +760,14 @@ namespace {
 
 void adj_free(moist_handle *h) {
     if (!h->adj) return;
-    for (double *p : h->adj->allocs)
+    AdjState &A = *h->adj;
+    if (A.s2) {
+        hipStreamSynchronize(A.s2);
+        hipStreamDestroy(A.s2);
+    }
+    for (hipEvent_t e : {A.ev_go, A.ev_rec[0], A.ev_rec[1], A.ev_rev[0], A.ev_rev[1]})
+        if (e) hipEventDestroy(e);
+    for (double *p : A.allocs)
         if (p) hipFree(p);
     delete h->adj;
     h->adj = nullptr;
@@ -787,16 +801,28 @@ int adj_alloc(moist_handle *h, int nsteps, int ckpt) {
     {
         // records of a whole segment when they fit in the free memory with a
         // margin (a tenth of it, at least 8 GiB): one recording pass per step
-        // instead of a state pass and a recording pass
+        // instead of a state pass and a recording pass; those of two segments
+        // when they fit, so that one segment's recording pass runs beside the
+        // reverse sweep of the next (pipelined segments)
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) == hipSuccess) {
-            const size_t need = A.rb_bytes * (size_t)ckpt;
-            const size_t other = (A.cp_stride * A.ncp + 60 * n2) * sizeof(double) + A.seg_bytes * (size_t)ckpt;
+            const size_t seg1 = (A.rb_bytes + A.seg_bytes) * (size_t)ckpt;
+            const size_t other = (A.cp_stride * A.ncp + 64 * n2) * sizeof(double);
             const size_t margin = std::max(fr / 10, (size_t)8 << 30);
-            A.segrec = fr > margin && need + other < fr - margin ? 1 : 0;
+            A.segrec = fr > margin && seg1 + other < fr - margin ? 1 : 0;
+            A.overlap = A.segrec && 2 * seg1 + other < fr - margin ? 1 : 0;
         }
-        if (A.mode) A.segrec = A.mode == 2 ? 1 : 0;
+        if (A.mode) {
+            A.segrec = A.mode >= 2 ? 1 : 0;
+            A.overlap = A.mode == 3 ? 1 : 0;
+        }
         A.nblocks = A.segrec ? ckpt : 1;
+    }
+    const size_t halves = A.overlap ? 2 : 1;
+    if (A.overlap) {
+        HIPCHK(h, hipStreamCreate(&A.s2));
+        for (hipEvent_t *e : {&A.ev_go, &A.ev_rec[0], &A.ev_rec[1], &A.ev_rev[0], &A.ev_rev[1]})
+            HIPCHK(h, hipEventCreateWithFlags(e, hipEventDisableTiming));
     }
     auto al = [&](double *&p, size_t n) -> int {
         HIPCHK(h, hipMalloc((void **)&p, n * sizeof(double)));
@@ -805,13 +831,13 @@ int adj_alloc(moist_handle *h, int nsteps, int ckpt) {
         return 0;
     };
     double *seg = nullptr, *recs = nullptr;
-    if (al(A.cp, A.cp_stride * A.ncp) || al(seg, A.seg_bytes * (size_t)ckpt / 8) || al(A.SC, 7 * n2) ||
-        al(recs, A.rb_bytes * (size_t)A.nblocks / 8) || al(A.lam, 7 * n2) ||
+    if (al(A.cp, A.cp_stride * A.ncp) || al(seg, halves * A.seg_bytes * (size_t)ckpt / 8) || al(A.SC, 7 * n2) ||
+        al(recs, halves * A.rb_bytes * (size_t)A.nblocks / 8) || al(A.lam, 7 * n2) ||
         al(A.lam2, 7 * n2) || al(A.ls0, 7 * n2) || al(A.lph, 7 * n2) || al(A.wk, NWORK * n2) ||
         al(A.lim, 6 * h->nxs) || al(A.gk, 3 * n2) || al(A.tgt, n2) || al(A.fin, A.cp_stride) || al(A.J, h->nm) ||
         al(A.gK, 3 * h->nm))
         return -1;
-    if (al(A.sfh_seg, 2 * (size_t)h->nm * ckpt) || al(A.carry, 2 * (size_t)h->nm) ||
+    if (al(A.sfh_seg, halves * 2 * (size_t)h->nm * ckpt) || al(A.carry, 2 * (size_t)h->nm) ||
         al(A.sfb, SF_SLOTS * (size_t)h->nxs))
         return -1;
     A.seg = (char *)seg;
@@ -1115,7 +1141,9 @@ void load_state(moist_handle *h, const double *src) {
     copy_set(h, h->sfh, src + h->adj->cp_words + h->nxs, 2 * h->nm);
 }
 // The stale halo pair surface_fluxes! reads at step j of the segment.
-double *seg_sfh(moist_handle *h, int j) { return h->adj->sfh_seg + 2 * (size_t)h->nm * j; }
+double *seg_sfh(moist_handle *h, int j) {
+    return h->adj->sfh_seg + 2 * (size_t)h->nm * ((size_t)h->adj->half * h->adj->ckpt + j);
+}
 // The current state as a full FP64 state (7 fields) at `dst`.
 void state64(moist_handle *h, double *dst) {
     if (h->precision == MOIST_MIXED)
@@ -1125,7 +1153,13 @@ void state64(moist_handle *h, double *dst) {
         copy_set(h, dst, slot<double>(h, S0_), 7 * h->n2);
 }
 // Step input j of the segment: the state slots in the handle's form.
-void *seg_step(moist_handle *h, int j) { return h->adj->seg + h->adj->seg_bytes * (size_t)j; }
+void *seg_step(moist_handle *h, int j) {
+    return h->adj->seg + h->adj->seg_bytes * ((size_t)h->adj->half * h->adj->ckpt + j);
+}
+// Record block j of the current half.
+char *rec_block(moist_handle *h, int j) {
+    return h->adj->recs + h->adj->rb_bytes * ((size_t)h->adj->half * h->adj->nblocks + j);
+}
 // Reverse step j of the segment (its record block is current): FP64 input
 // as stored, or in the mixed form as its recording pass widened it.
 void reverse_step(moist_handle *h, int j) {
@@ -1154,7 +1188,7 @@ void seg_record(moist_handle *h, int j) {
     // the mixed form's reverse step reads its input from the record block
     if (h->precision != MOIST_MIXED) copy_bytes(h, seg_step(h, j), h->pool, 7 * h->n2 * h->elem);
     copy_set(h, seg_sfh(h, j), h->sfh, 2 * h->nm);
-    A.cur = A.recs + A.rb_bytes * (size_t)j;
+    A.cur = rec_block(h, j);
     record_step(h, j, true);
 }
 // Reverse step j of a segment from its records, or, with per-step records,
@@ -1162,7 +1196,7 @@ void seg_record(moist_handle *h, int j) {
 void seg_reverse(moist_handle *h, int j) {
     AdjState &A = *h->adj;
     if (A.segrec) {
-        A.cur = A.recs + A.rb_bytes * (size_t)j;
+        A.cur = rec_block(h, j);
     } else {
         if (h->precision == MOIST_MIXED) copy_bytes(h, h->pool, seg_step(h, j), 7 * h->n2 * h->elem);
         copy_set(h, h->sfh, seg_sfh(h, j), 2 * h->nm);
@@ -1209,7 +1243,43 @@ template <class H> int adjoint_run_t(moist_handle *h, int nsteps, int ckpt, cons
     // no later step
     fill(h, A.carry, 2 * h->nm);
     const size_t state_bytes = 7 * n2 * h->elem;
-    for (int sg = A.ncp - 1; sg >= 0; --sg) {
+    if (A.overlap) {
+        // pipelined segments: segment sg - 1's recording pass (stream s2, the
+        // half of segment sg + 1, once that segment's reverse sweep is done)
+        // runs beside segment sg's reverse sweep (the handle's stream, once
+        // sg's recording pass is done).  The two share no buffer: the
+        // recording pass writes the state, the work arrays and its half's
+        // segment buffers; the reverse sweep reads its half and writes the
+        // adjoint arrays.
+        hipStream_t sA = h->stream, sB = A.s2;
+        auto record_segment = [&](int sg) -> int {
+            const int n0 = sg * ckpt, n1 = std::min(nsteps, n0 + ckpt);
+            A.half = sg & 1;
+            h->stream = sB;
+            load_state(h, A.cp + A.cp_stride * sg);
+            for (int n = n0; n < n1; ++n) seg_record(h, n - n0);
+            h->stream = sA;
+            HIPCHK(h, hipEventRecord(A.ev_rec[sg & 1], sB));
+            return 0;
+        };
+        HIPCHK(h, hipEventRecord(A.ev_go, sA));
+        HIPCHK(h, hipStreamWaitEvent(sB, A.ev_go, 0));
+        if (record_segment(A.ncp - 1)) return -1;
+        for (int sg = A.ncp - 1; sg >= 0; --sg) {
+            const int n0 = sg * ckpt, n1 = std::min(nsteps, n0 + ckpt);
+            if (sg > 0) {
+                if (sg + 1 < A.ncp) HIPCHK(h, hipStreamWaitEvent(sB, A.ev_rev[(sg + 1) & 1], 0));
+                if (record_segment(sg - 1)) return -1;
+            }
+            HIPCHK(h, hipStreamWaitEvent(sA, A.ev_rec[sg & 1], 0));
+            A.half = sg & 1;
+            for (int n = n1 - 1; n >= n0; --n) seg_reverse(h, n - n0);
+            HIPCHK(h, hipEventRecord(A.ev_rev[sg & 1], sA));
+            HIPCHK(h, hipGetLastError());
+        }
+        A.half = 0;
+    }
+    for (int sg = A.overlap ? -1 : A.ncp - 1; sg >= 0; --sg) {
         const int n0 = sg * ckpt, n1 = std::min(nsteps, n0 + ckpt);
         load_state(h, A.cp + A.cp_stride * sg);
         if (A.segrec) {
@@ -1653,8 +1723,8 @@ int moist_set_option(moist_handle *h, int option, int value) {
         return 0;
     }
     if (option == MOIST_OPT_ADJ_RECORDS) {
-        if (value < 0 || value > 2) {
-            h->err = "adjoint record mode must be 0, 1 or 2";
+        if (value < 0 || value > 3) {
+            h->err = "adjoint record mode must be 0, 1, 2 or 3";
             return -2;
         }
         h->adj_mode = value;

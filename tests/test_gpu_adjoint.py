@@ -161,6 +161,24 @@ def test_mixed_adjoint_record_modes_agree():
     np.testing.assert_array_equal(a[2], b[2])
 
 
+@pytest.mark.parametrize("case", ["bubble", "sf_rd"])
+@pytest.mark.parametrize("precision", [core.FP64, core.MIXED])
+def test_pipelined_segments_match_segment_records(precision, case):
+    """Pipelined segments (one segment's recording pass on a second stream
+    beside the next segment's reverse sweep, the segment buffers in two
+    halves) run the same kernels on the same values as whole-segment records:
+    J and the gradients are bitwise equal.  7 steps, checkpoints every 2: four
+    segments, so both halves are written twice; with surface fluxes on, the
+    stale halo pairs of the segments' steps are in the halves too."""
+    p, s, tt = _case() if case == "bubble" else _case_sf_rd()
+    a = _device_adjoint(p, s, tt, 7, 2, precision, "segment")
+    b = _device_adjoint(p, s, tt, 7, 2, precision, "pipelined")
+    assert a[0] == b[0]
+    for f in range(7):
+        np.testing.assert_array_equal(a[1][f], b[1][f], err_msg=PROG7[f])
+    np.testing.assert_array_equal(a[2], b[2])
+
+
 def test_mixed_adjoint_close_to_fp64_adjoint():
     """The mixed-form gradient differs from the FP64 form's by what the
     trajectory's FP32 deviations change.  Measured on this case: the loss
