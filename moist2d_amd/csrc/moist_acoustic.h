@@ -57,9 +57,12 @@ template <class S> struct AcBuf {
 };
 
 // WPE: minimum waves per SIMD the register allocation must allow (1: no
-// constraint; 8 lets two 1024-thread workgroups share a CU).
+// constraint; 8 lets two 1024-thread workgroups share a CU).  The float
+// kernel (the mixed form's recording pass) keeps at least 4 (at most 128
+// VGPRs): with FP contraction it asked for 132, which left one 512-thread
+// workgroup per CU instead of two.
 template <class S, int L, int NT, int WPE>
-__global__ void __launch_bounds__(NT, WPE) k_acoustic(const Dom *__restrict__ doms, const int4 *__restrict__ atiles,
+__global__ void __launch_bounds__(NT, (is_mixed<S>() && WPE < 4) ? 4 : WPE) k_acoustic(const Dom *__restrict__ doms, const int4 *__restrict__ atiles,
                                                  ZVec zv, SSet<S> in, Work<S> w, AcBuf<S> ab, int div,
                                                  double dt_ovr, int nsub, int first) {
     using R = S;

@@ -859,6 +859,7 @@ __global__ void __launch_bounds__(256)
     for (int q = 0; q < LR; ++q) cpv[q] = rmv[q] = acv[q] = 0.0;
     if (factor) {
         double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;
+#pragma unroll 4
         for (int j = j0; j < j1; ++j) {
             const double aj = j > 0 ? W.a(j) : 0.0, bj = W.b(j) + 1.0, cj = j < n - 1 ? W.c(j) : 0.0;
             // M_j = [[0, c], [-a, b+1]] applied after the chunk composite
@@ -906,6 +907,7 @@ __global__ void __launch_bounds__(256)
                 }
             }
         } else {
+#pragma unroll 4
             for (int j = j0; j < j1; ++j) {
                 const double aj = j > 0 ? W.a(j) : 0.0;
                 const double m = W.b(j) + 1.0 - aj * cprev;
@@ -988,8 +990,11 @@ __global__ void __launch_bounds__(256)
         }
         return;
     }
-    // forward sweep w_j = Lx_j - c'_j-1 w_j-1
+    // forward sweep w_j = Lx_j - c'_j-1 w_j-1 (each sweep unrolled by four:
+    // the rows' loads do not depend on the recurrence, so four rows' loads are
+    // in flight together instead of one row's at a time)
     double A = 1.0, B = 0.0;
+#pragma unroll 4
     for (int j = j0; j < j1; ++j) {
         const double al = j > 0 ? -a.cpr[W.G(j - 1)] : 0.0;
         B = al * B + a.Lrwp[W.G(j)];
@@ -998,6 +1003,7 @@ __global__ void __launch_bounds__(256)
     scan_affine_up<CH>(A, B, ch);
     double wv = __shfl_up(B, CW);
     if (ch == 0) wv = 0.0;
+#pragma unroll 4
     for (int j = j0; j < j1; ++j) {
         const double al = j > 0 ? -a.cpr[W.G(j - 1)] : 0.0;
         wv = al * wv + a.Lrwp[W.G(j)];
@@ -1006,6 +1012,7 @@ __global__ void __launch_bounds__(256)
     // back substitution y_j = rm_j w_j - rm_j a_j+1 y_j+1, chunks from the top
     A = 1.0;
     B = 0.0;
+#pragma unroll 4
     for (int j = j1 - 1; j >= j0; --j) {
         const double rm = a.rmi[W.G(j)];
         const double al = j < n - 1 ? -rm * a.aco[W.G(j + 1)] : 0.0;
@@ -1015,6 +1022,7 @@ __global__ void __launch_bounds__(256)
     scan_affine_down<CH>(A, B, ch);
     double yv = __shfl_down(B, CW);
     if (ch == CH - 1) yv = 0.0;
+#pragma unroll 4
     for (int j = j1 - 1; j >= j0; --j) {
         const double rm = a.rmi[W.G(j)];
         const double al = j < n - 1 ? -rm * a.aco[W.G(j + 1)] : 0.0;
