@@ -1698,6 +1698,9 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     constexpr int HX = AD_X + 4, HZ = AD_Z + 4, NH = HX * HZ, NT = AD_X * AD_Z, NS = (NH + NT - 1) / NT;
     constexpr int NF = 10;
     __shared__ double sh[NF][HZ][HX];  // LRu, LRw, LRt, LRr, rho, rho_u, rho_w, u, w, theta
+    // per-level geometry of levels k0-2 .. k0+AD_Z+1: {1/dz, 1/(z_k+1 - z_k), 1/(dz_k + dz_k+1), g/..}, z, zf, dz
+    __shared__ double4 sG[HZ];
+    __shared__ double sZ[HZ], sZF[HZ], sDZ[HZ];
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;
     const Dom &D = doms[t.x];
@@ -1705,6 +1708,39 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     const int i = i0 + threadIdx.x, k = k0 + threadIdx.y, tid = threadIdx.y * AD_X + threadIdx.x;
     const int nx = D.nx, nz = D.nz, ke = nz - 1;
     auto G = [&](int kk, int ii) -> int64_t { return D.off2 + (int64_t)kk * nx + wrapn(ii, nx); };
+    // this cell's read-modify-write values (stored at the end) and the other
+    // per-cell inputs, loaded with the staged tile (before the barrier)
+    const bool cell_ok = i < nx && k < nz;
+    const int64_t c = D.off2 + (int64_t)min(k, ke) * nx + min(i, nx - 1);
+    double s0r = 0.0, s0u = 0.0, s0t = 0.0, s0w = 0.0, sr = 0.0, su = 0.0, sw = 0.0, st = 0.0, p0 = 0.0, p1 = 0.0,
+           p2 = 0.0, Lth0 = 0.0, Lrp = 0.0, Lrt = 0.0, Lrw = 0.0, Lru = 0.0, Lmx = 0.0, Lmz = 0.0;
+    if (cell_ok) {
+        s0r = ls0.f[RHO][c];
+        s0u = ls0.f[RHOU][c];
+        s0t = ls0.f[RT][c];
+        s0w = ls0.f[RHOW][c];
+        sr = ls.f[RHO][c];
+        su = ls.f[RHOU][c];
+        sw = ls.f[RHOW][c];
+        st = ls.f[RT][c];
+        p0 = lph.f[0][c];
+        p1 = lph.f[1][c];
+        p2 = lph.f[2][c];
+        Lth0 = a.Lth[c];
+        Lrp = a.Lrp[c];
+        Lrt = a.Lrtp[c];
+        Lrw = k <= ke - 1 ? a.Lrwp[c] : 0.0;
+        Lru = Lrup[c];
+        Lmx = a.Lmx[c];
+        Lmz = a.Lmz[c];
+    }
+    if (tid < HZ) {
+        const int kk = min(max(k0 - 2 + tid, 0), ke);
+        sG[tid] = zv.geo[D.offz + kk];
+        sZ[tid] = zv.z[D.offz + kk];
+        sZF[tid] = zv.zf[D.offz + kk];
+        sDZ[tid] = zv.dz[D.offz + kk];
+    }
     {
         const double *src[NF] = {a.LRu, a.LRw, a.LRt, a.LRr, s.f[RHO], s.f[RHOU], s.f[RHOW], a.uf, a.wf, nullptr};
         double x[NS][NF];
@@ -1726,11 +1762,21 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
         }
     }
     __syncthreads();
-    if (i >= nx || k >= nz) return;
-    const double *zd = zv.z + D.offz, *zfd = zv.zf + D.offz, *dzd = zv.dz + D.offz;
+    if (!cell_ok) return;
+    // the staged per-level geometry, indexed by level (kk - 2 <= ... <= k + 2)
+    struct LG {
+        const double4 *g;
+        int o;
+        __device__ const double4 &operator[](int kk) const { return g[kk - o]; }
+    };
+    struct LD {
+        const double *g;
+        int o;
+        __device__ double operator[](int kk) const { return g[kk - o]; }
+    };
+    const LD zd{sZ, k0 - 2}, zfd{sZF, k0 - 2}, dzd{sDZ, k0 - 2};
     const double dx = D.dx, EPS = 1e-12, GRAV = K<double>::GRAV, nu = D.nu, rdx = 1.0 / dx;
-    const double4 *gz = zv.geo + D.offz;  // per level: 1/dz, 1/(z_k+1 - z_k), 1/(dz_k + dz_k+1)
-    const int64_t c = G(k, i);
+    const LG gz{sG, k0 - 2};  // per level: 1/dz, 1/(z_k+1 - z_k), 1/(dz_k + dz_k+1)
     // the staged value of field f at (kk, ii), |kk - k| <= 2, |ii - i| <= 2
     auto SH = [&](int f, int kk, int ii) { return sh[f][kk - k0 + 2][ii - i0 + 2]; };
     auto LRU = [&](int kk, int ii) { return SH(0, kk, ii); };
@@ -1744,31 +1790,25 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     auto THv = [&](int kk, int ii) { return SH(9, kk, ii); };
     double l_r = 0.0, l_u = 0.0, l_w = 0.0, LUc = 0.0, LWc = 0.0, lp = 0.0, lth = 0.0;
 
-    // this cell's read-modify-write values, read before any store (stored at the end)
-    double s0r = ls0.f[RHO][c], s0u = ls0.f[RHOU][c], s0t = ls0.f[RT][c], s0w = ls0.f[RHOW][c];
-    double sr = ls.f[RHO][c], su = ls.f[RHOU][c], sw = ls.f[RHOW][c], st = ls.f[RT][c];
-    double p0 = lph.f[0][c], p1 = lph.f[1][c], p2 = lph.f[2][c];
-    const double Lth0 = a.Lth[c];
+    // this cell's tendency adjoints (staged)
+    const double LRr_c = LRR(k, i), LRu_c = LRU(k, i), LRt_c = LRT(k, i);
     // perturbation init and momentum copies
-    {
-        const double Lrp = a.Lrp[c], Lrt = a.Lrtp[c], Lrw = k <= ke - 1 ? a.Lrwp[c] : 0.0, Lru = Lrup[c];
-        s0r += Lrp;
-        s0u += Lru;
-        s0t += Lrt;
-        s0w += Lrw;
-        l_r -= Lrp;
-        l_u += -Lru + a.Lmx[c];
-        st -= Lrt;
-        l_w += -Lrw + a.Lmz[c];
-    }
+    s0r += Lrp;
+    s0u += Lru;
+    s0t += Lrt;
+    s0w += Lrw;
+    l_r -= Lrp;
+    l_u += -Lru + Lmx;
+    st -= Lrt;
+    l_w += -Lrw + Lmz;
     // physics tendencies enter R_rho, R_rho_u, R_rho_theta additively
-    p0 += a.LRr[c];
-    p1 += a.LRu[c];
-    p2 += a.LRt[c];
+    p0 += LRr_c;
+    p1 += LRu_c;
+    p2 += LRt_c;
 
     // ---- R_rho (moist.jl:1079-1086)
-    l_u += -a.LRr[c] * rdx + LRR(k, i + 1) * rdx;
-    if (k < ke) l_w += -a.LRr[c] * gz[k].x + LRR(k + 1, i) * gz[k + 1].x;
+    l_u += -LRr_c * rdx + LRR(k, i + 1) * rdx;
+    if (k < ke) l_w += -LRr_c * gz[k].x + LRR(k + 1, i) * gz[k + 1].x;
 
     // ---- R_rho_u (moist.jl:1098-1179)
     // x flux F(j) between u-faces j and j+1, m = (rho_u_j + rho_u_j+1)/2
@@ -1825,7 +1865,7 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
         }
     }
     // pressure gradient (p_i+1 - p_i) / dx
-    lp += a.LRu[c] * rdx - LRU(k, i - 1) * rdx;
+    lp += LRu_c * rdx - LRU(k, i - 1) * rdx;
 
     // ---- R_rho_w (moist.jl:1190-1279), cells k' <= ke - 1
     const double sponge_base = (1.0 - D.zf_sponge) * D.H;
