@@ -396,7 +396,8 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
 
 // ---- scalar transport + final update (moist.jl:1469-1539) ------------------------
 // lo: adjoint of the stage output; writes ls (stage input, first writer of the
-// stage), adds into ls0 (rho_q) and lph (q tendencies); writes Lmx, Lmz and
+// stage), adds into ls0 (rho_q) and lph (q tendencies) (writes them in the
+// step's first reverse stage, `first`: nothing cleared them); writes Lmx, Lmz and
 // the substep adjoints Lrp, Lrtp, Lrwp, Lrup (= lo of rho, rho_theta, rho_w, rho_u).
 //
 // A block owns the AD_X x AD_Z cells of its tile.  Per scalar it stages in
@@ -410,7 +411,7 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
 template <class RC>
 __global__ void __launch_bounds__(AD_X *AD_Z)
     ka_scalar(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s, SSet<double> lo,
-              SSet<double> ls, SSet<double> ls0, SSet<double> lph, AdjW<RC> a, double *Lrup, int div) {
+              SSet<double> ls, SSet<double> ls0, SSet<double> lph, AdjW<RC> a, double *Lrup, int div, int first) {
     constexpr int QX = AD_X + 6, QZ = AD_Z + 6;  // q: columns i0-3 .. i0+AD_X+2, levels k0-3 .. k0+AD_Z+2
     constexpr int LX = AD_X + 4, LZ = AD_Z + 4;  // L: i0-2 .. i0+AD_X+1, k0-2 .. k0+AD_Z+1
     constexpr int RX = AD_X + 2, RZ = AD_Z + 2;  // rho: i0-1 .. i0+AD_X, k0-1 .. k0+AD_Z
@@ -485,9 +486,16 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     // (while the previous scalar's faces and cells are computed) and written
     // to LDS at the top of its iteration
     constexpr int NQ = (QZ * QX + NT - 1) / NT, NL = (LZ * LX + NT - 1) / NT;
-    double qn[NQ], ln[NL];
+    double qn[NQ], ln[NL], s0n = 0.0, phn = 0.0;
     auto prefetch = [&](int sc) {
         const double *qa = a.q[sc], *L = lo.f[QV + sc];
+        // with the scalar's read-modify-write values, ahead of the previous
+        // scalar's stores (a load issued after a store waits for it); in the
+        // step's first reverse stage this kernel is their first writer
+        if (cell && !first) {
+            s0n = ls0.f[QV + sc][c];
+            phn = lph.f[3 + sc][c];
+        }
 #pragma unroll
         for (int u = 0; u < NQ; ++u) {
             const int e = tid + u * NT, r = e / QX, kk = k0 - 3 + r;
@@ -504,12 +512,8 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
 #pragma unroll 1
     for (int sc = 0; sc < 3; ++sc) {
         const double *qr = s.f[QV + sc];
-        // this scalar's read-modify-write values
-        double s0v = 0.0, phv = 0.0;
-        if (cell) {
-            s0v = ls0.f[QV + sc][c];
-            phv = lph.f[3 + sc][c];
-        }
+        // this scalar's read-modify-write values (prefetched)
+        const double s0v = s0n, phv = phn;
 #pragma unroll
         for (int u = 0; u < NQ; ++u)
             if (tid + u * NT < QZ * QX) (&sQ[0][0])[tid + u * NT] = qn[u];
@@ -1684,7 +1688,9 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
 // Gathers every contribution of R_rho, R_rho_u, R_rho_w, R_rho_theta to the
 // stage input (rho, rho_u, rho_w), to the face velocities (LU, LW), pres and
 // theta; adds the perturbation adjoints (rho' = s0 - s etc.) and the momentum
-// copies.
+// copies.  Its s0 and physics-tendency adjoints (ls0, lph) accumulate over the
+// three stages; in the step's first reverse stage (`first`) it is their first
+// writer and does not read them.
 //
 // Every neighbour value a cell reads lies within two cells of it in x and z:
 // the block stages the ten fields it gathers from (the four tendency
@@ -1694,7 +1700,7 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
 template <class RC>
 __global__ void __launch_bounds__(AD_X *AD_Z)
     ka_slow(const Dom *__restrict__ doms, const int4 *__restrict__ tiles, ZVec zv, SSet<double> s, SSet<double> ls,
-            SSet<double> ls0, SSet<double> lph, AdjW<RC> a, const double *__restrict__ Lrup) {
+            SSet<double> ls0, SSet<double> lph, AdjW<RC> a, const double *__restrict__ Lrup, int first) {
     constexpr int HX = AD_X + 4, HZ = AD_Z + 4, NH = HX * HZ, NT = AD_X * AD_Z, NS = (NH + NT - 1) / NT;
     constexpr int NF = 10;
     __shared__ double sh[NF][HZ][HX];  // LRu, LRw, LRt, LRr, rho, rho_u, rho_w, u, w, theta
@@ -1715,17 +1721,19 @@ __global__ void __launch_bounds__(AD_X *AD_Z)
     double s0r = 0.0, s0u = 0.0, s0t = 0.0, s0w = 0.0, sr = 0.0, su = 0.0, sw = 0.0, st = 0.0, p0 = 0.0, p1 = 0.0,
            p2 = 0.0, Lth0 = 0.0, Lrp = 0.0, Lrt = 0.0, Lrw = 0.0, Lru = 0.0, Lmx = 0.0, Lmz = 0.0;
     if (cell_ok) {
-        s0r = ls0.f[RHO][c];
-        s0u = ls0.f[RHOU][c];
-        s0t = ls0.f[RT][c];
-        s0w = ls0.f[RHOW][c];
+        if (!first) {  // in the step's first reverse stage this kernel is their first writer
+            s0r = ls0.f[RHO][c];
+            s0u = ls0.f[RHOU][c];
+            s0t = ls0.f[RT][c];
+            s0w = ls0.f[RHOW][c];
+            p0 = lph.f[0][c];
+            p1 = lph.f[1][c];
+            p2 = lph.f[2][c];
+        }
         sr = ls.f[RHO][c];
         su = ls.f[RHOU][c];
         sw = ls.f[RHOW][c];
         st = ls.f[RT][c];
-        p0 = lph.f[0][c];
-        p1 = lph.f[1][c];
-        p2 = lph.f[2][c];
         Lth0 = a.Lth[c];
         Lrp = a.Lrp[c];
         Lrt = a.Lrtp[c];

@@ -1027,17 +1027,13 @@ template <class RC> void enqueue_step_adj(moist_handle *h, const double *sn, con
         const SSet<double> in = dset(st == 0 ? const_cast<double *>(sn) : rec_in(h, st), n2);
         const int nsub = h->ns_stage[st], div = divs[st];
         const AdjW<RC> a = adj_w<RC>(h, st);
-        // the stage's accumulators, and in the step's first reverse stage the
-        // step-level ones (ls0, lph), are cleared by ka_pre
+        // the stage's accumulators are cleared by ka_pre; the step-level ones
+        // (ls0, lph) are not cleared: in the step's first reverse stage (st = 2)
+        // ka_scalar and ka_slow write them instead of adding to them (`first`)
         FillList zero{};
         for (double *p : {wk(h, W_TH), wk(h, W_RT), wk(h, W_LRR), wk(h, W_LRU), wk(h, W_LRW), wk(h, W_LRT),
                           wk(h, W_PEND)})
             zero.p[zero.n++] = p;
-        if (st == 2)
-            for (int f = 0; f < 7; ++f) {
-                zero.p[zero.n++] = A.ls0 + f * n2;
-                zero.p[zero.n++] = A.lph + f * n2;
-            }
         double *Lrup[2] = {wk(h, W_LRUP0), wk(h, W_LRUP1)};
         // the limiter sums of the stage reversed before (into its input adjoint, lo
         // here) run in ka_pre's extra blocks; those of stage 1 in their own launch,
@@ -1046,7 +1042,7 @@ template <class RC> void enqueue_step_adj(moist_handle *h, const double *sn, con
         hipLaunchKernelGGL(ka_pre<RC>, dim3(h->ntiles2d + (st < 2 ? h->nm : 0)), b2, 0, h->stream, h->dom_d,
                            h->tiles2d, zv, in, a, zero, lin);
         hipLaunchKernelGGL(ka_scalar<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in,
-                           dset(lo, n2), dset(li, n2), ls0, lph, a, Lrup[0], div);
+                           dset(lo, n2), dset(li, n2), ls0, lph, a, Lrup[0], div, st == 2 ? 1 : 0);
         if (st == 0)
             hipLaunchKernelGGL(ka_lin<RC>, dim3(h->nm), dim3(256), 0, h->stream, h->dom_d, a, dset(li, n2));
         int cur = 0;
@@ -1101,7 +1097,7 @@ template <class RC> void enqueue_step_adj(moist_handle *h, const double *sn, con
             cur = 1 - cur;
         }
         hipLaunchKernelGGL(ka_slow<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in,
-                           dset(li, n2), ls0, lph, a, (const double *)Lrup[cur]);
+                           dset(li, n2), ls0, lph, a, (const double *)Lrup[cur], st == 2 ? 1 : 0);
         hipLaunchKernelGGL(ka_diag<RC>, dim3(h->ntiles2d), b2, 0, h->stream, h->dom_d, h->tiles2d, zv, in,
                            dset(li, n2), a);
         // the stage-3 input is what the next step's surface fluxes read in the
