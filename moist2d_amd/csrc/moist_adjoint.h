@@ -23,6 +23,11 @@
 constexpr int AD_X = 64, AD_Z = 4;  // cell tile of the gather kernels
 
 // ---- dual numbers -------------------------------------------------------------
+// Value and N derivative parts.  The value parts take exactly the operations
+// the forward step takes (so frozen branch tests see its numbers); the
+// derivative parts of a product or quotient are contracted to fused
+// multiply-adds (`#pragma clang fp contract(fast)` in their loops only), and a
+// dual combined with a plain number touches only its own derivative parts.
 template <int N> struct Dl {
     double v;
     double d[N];
@@ -61,7 +66,10 @@ template <int N> __device__ __forceinline__ Dl<N> operator-(const Dl<N> &a) {
 template <int N> __device__ __forceinline__ Dl<N> operator*(const Dl<N> &a, const Dl<N> &b) {
     Dl<N> r(a.v * b.v);
 #pragma unroll
-    for (int j = 0; j < N; ++j) r.d[j] = a.d[j] * b.v + a.v * b.d[j];
+    for (int j = 0; j < N; ++j) {
+#pragma clang fp contract(fast)
+        r.d[j] = a.d[j] * b.v + a.v * b.d[j];
+    }
     return r;
 }
 // Quotients: the value is the correctly rounded quotient (the forward's
@@ -71,15 +79,45 @@ template <int N> __device__ __forceinline__ Dl<N> operator/(const Dl<N> &a, cons
     Dl<N> r(a.v / b.v);
     const double rb = 1.0 / b.v;
 #pragma unroll
-    for (int j = 0; j < N; ++j) r.d[j] = (a.d[j] - r.v * b.d[j]) * rb;
+    for (int j = 0; j < N; ++j) {
+#pragma clang fp contract(fast)
+        r.d[j] = (a.d[j] - r.v * b.d[j]) * rb;
+    }
     return r;
 }
-template <int N> __device__ __forceinline__ Dl<N> operator+(const Dl<N> &a, double b) { return a + Dl<N>(b); }
-template <int N> __device__ __forceinline__ Dl<N> operator+(double a, const Dl<N> &b) { return Dl<N>(a) + b; }
-template <int N> __device__ __forceinline__ Dl<N> operator-(const Dl<N> &a, double b) { return a - Dl<N>(b); }
-template <int N> __device__ __forceinline__ Dl<N> operator-(double a, const Dl<N> &b) { return Dl<N>(a) - b; }
-template <int N> __device__ __forceinline__ Dl<N> operator*(const Dl<N> &a, double b) { return a * Dl<N>(b); }
-template <int N> __device__ __forceinline__ Dl<N> operator*(double a, const Dl<N> &b) { return Dl<N>(a) * b; }
+template <int N> __device__ __forceinline__ Dl<N> operator+(const Dl<N> &a, double b) {
+    Dl<N> r = a;
+    r.v = a.v + b;
+    return r;
+}
+template <int N> __device__ __forceinline__ Dl<N> operator+(double a, const Dl<N> &b) {
+    Dl<N> r = b;
+    r.v = a + b.v;
+    return r;
+}
+template <int N> __device__ __forceinline__ Dl<N> operator-(const Dl<N> &a, double b) {
+    Dl<N> r = a;
+    r.v = a.v - b;
+    return r;
+}
+template <int N> __device__ __forceinline__ Dl<N> operator-(double a, const Dl<N> &b) {
+    Dl<N> r(a - b.v);
+#pragma unroll
+    for (int j = 0; j < N; ++j) r.d[j] = -b.d[j];
+    return r;
+}
+template <int N> __device__ __forceinline__ Dl<N> operator*(const Dl<N> &a, double b) {
+    Dl<N> r(a.v * b);
+#pragma unroll
+    for (int j = 0; j < N; ++j) r.d[j] = a.d[j] * b;
+    return r;
+}
+template <int N> __device__ __forceinline__ Dl<N> operator*(double a, const Dl<N> &b) {
+    Dl<N> r(a * b.v);
+#pragma unroll
+    for (int j = 0; j < N; ++j) r.d[j] = a * b.d[j];
+    return r;
+}
 template <int N> __device__ __forceinline__ Dl<N> operator/(const Dl<N> &a, double b) {
     Dl<N> r(a.v / b);
     const double rb = 1.0 / b;
@@ -112,7 +150,10 @@ template <int N> __device__ __forceinline__ Dl<N> d_pow(const Dl<N> &a, const Dl
     Dl<N> r(pow(a.v, b.v));
     const double la = log(a.v), ba = b.v / a.v;
 #pragma unroll
-    for (int j = 0; j < N; ++j) r.d[j] = r.v * (b.d[j] * la + ba * a.d[j]);
+    for (int j = 0; j < N; ++j) {
+#pragma clang fp contract(fast)
+        r.d[j] = r.v * (b.d[j] * la + ba * a.d[j]);
+    }
     return r;
 }
 template <int N> __device__ __forceinline__ double val(const Dl<N> &a) { return a.v; }
