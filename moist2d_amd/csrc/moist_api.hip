@@ -778,9 +778,21 @@ void adj_free(moist_handle *h) {
 int rec_nin(const moist_handle *h) { return h->precision == MOIST_MIXED ? 3 : 2; }
 size_t rec_in_bytes(const moist_handle *h) { return 7 * (size_t)rec_nin(h) * h->n2 * sizeof(double); }
 
+int adj_alloc_buffers(moist_handle *h, int nsteps, int ckpt);
+// The adjoint's buffers for (nsteps, ckpt) and the record mode; kept between
+// runs with the same three.  A failed allocation frees what it got, so a
+// later call starts over instead of finding a half-built state.
 int adj_alloc(moist_handle *h, int nsteps, int ckpt) {
     if (h->adj && h->adj->nsteps == nsteps && h->adj->ckpt == ckpt && h->adj->mode == h->adj_mode) return 0;
     adj_free(h);
+    if (adj_alloc_buffers(h, nsteps, ckpt)) {
+        adj_free(h);
+        (void)hipGetLastError();  // the failed call's error is in h->err; later checks must not see it
+        return -1;
+    }
+    return 0;
+}
+int adj_alloc_buffers(moist_handle *h, int nsteps, int ckpt) {
     h->adj = new AdjState();
     AdjState &A = *h->adj;
     A.nsteps = nsteps;

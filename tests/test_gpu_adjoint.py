@@ -179,6 +179,24 @@ def test_pipelined_segments_match_segment_records(precision, case):
     np.testing.assert_array_equal(a[2], b[2])
 
 
+def test_adjoint_buffers_that_do_not_fit_fail_cleanly_twice():
+    """An adjoint run whose checkpoints cannot fit in device memory (100,000
+    steps of the 512x128 bubble with a checkpoint every step: about 370 GB)
+    returns an error before any step runs, and so does the same call again:
+    the failed allocation leaves no half-built buffers behind for the second
+    call to find.  The handle then runs a small adjoint."""
+    p, s = ic.bubble(np.float64, nx=512, nz=128)
+    tt = np.asfortranarray(s.rho_theta / s.rho)
+    eng = core.Engine(p, core.FP64)
+    eng.upload(s)
+    for _ in range(2):
+        with pytest.raises(RuntimeError, match="moist_adjoint_run"):
+            eng.adjoint(100000, [tt], ckpt=1)
+    J = eng.adjoint(2, [tt], ckpt=1)[0]
+    assert np.isfinite(J)
+    eng.close()
+
+
 def test_mixed_adjoint_close_to_fp64_adjoint():
     """The mixed-form gradient differs from the FP64 form's by what the
     trajectory's FP32 deviations change.  Measured on this case: the loss
