@@ -146,9 +146,11 @@ int moist_use_graph(moist_handle *h, int on);
  * records of its reverse sweep: 1 per step (each step's record recomputed
  * from the segment's states); 2 whole segment; 3 whole segment, pipelined
  * (the records of two segments: one segment's recording pass runs on a
- * second stream beside the reverse sweep of the next); 0 (default) the first
- * of 3, 2, 1 whose buffers fit in the free device memory less a margin (a
- * tenth of it, at least 8 GiB).  All give the same gradient.
+ * second stream beside the reverse sweep of the next); 4 per step, pipelined
+ * (two record blocks: one step's recording pass on a second stream beside
+ * the reverse of the step after it); 0 (default) the first of 3, 2, 4, 1
+ * whose buffers fit in the free device memory less a margin (a tenth of it,
+ * at least 8 GiB).  All give the same gradient.
  * MOIST_OPT_ADJ_COLUMN_KERNEL
  * (default 1) runs the vertical part of each reverse acoustic substep as one
  * kernel per column (members up to 257 levels); 0 runs it as three kernels.

@@ -166,10 +166,11 @@ class Engine:
         self._check(lib().moist_set_option(self._h, 0, 1 if on else 0), "moist_set_option")
 
     def adjoint_records(self, mode):
-        """Forward records of the reverse sweep: "auto", "step", "segment" or
+        """Forward records of the reverse sweep: "auto", "step", "segment",
         "pipelined" (whole segments, one segment's recording pass beside the
-        next one's reverse sweep)."""
-        v = {"auto": 0, "step": 1, "segment": 2, "pipelined": 3}[mode]
+        next one's reverse sweep) or "step_pipelined" (per-step records, one
+        step's recording pass beside the reverse of the step after it)."""
+        v = {"auto": 0, "step": 1, "segment": 2, "pipelined": 3, "step_pipelined": 4}[mode]
         self._check(lib().moist_set_option(self._h, 1, v), "moist_set_option")
 
     def adjoint_column_kernel(self, on=True):

@@ -138,7 +138,9 @@ struct moist_handle {
     int64_t ac_launches = 0;
     // reverse mode (allocated by the first moist_adjoint_run)
     AdjState *adj = nullptr;
-    int adj_mode = 0;  // forward records of the reverse sweep: 0 automatic, 1 per step, 2 per segment, 3 pipelined
+    // forward records of the reverse sweep: 0 automatic, 1 per step, 2 per segment,
+    // 3 per segment pipelined, 4 per step pipelined
+    int adj_mode = 0;
     int64_t dev_bytes = 0;  // device memory of the state pools and tables
     // output_history: derived fields (allocated on first request), block partials, per-member stats
     double *hist_fields = nullptr, *hist_part = nullptr, *hist_stats = nullptr, *diag_all = nullptr;
@@ -740,6 +742,9 @@ struct AdjState {
     // inputs, stale halo pairs); segment sg uses half sg & 1.  The recording
     // pass of segment sg - 1 runs on stream s2 while stream `stream` reverses
     // segment sg; events order each half's writer after its last reader.
+    // With per-step records, pipelining keeps two record blocks: the recording
+    // pass of step j - 1 (stream s2, block (j - 1) mod 2) runs beside the
+    // reverse of step j (block j mod 2).
     int overlap = 0, half = 0;
     hipStream_t s2 = nullptr;
     hipEvent_t ev_go = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
@@ -822,15 +827,17 @@ int adj_alloc_buffers(moist_handle *h, int nsteps, int ckpt) {
             const size_t other = (A.cp_stride * A.ncp + 64 * n2) * sizeof(double);
             const size_t margin = std::max(fr / 10, (size_t)8 << 30);
             A.segrec = fr > margin && seg1 + other < fr - margin ? 1 : 0;
-            A.overlap = A.segrec && 2 * seg1 + other < fr - margin ? 1 : 0;
+            // pipelined: two segments' buffers, or (per-step records) two record blocks
+            const size_t step2 = 2 * A.rb_bytes + A.seg_bytes * (size_t)ckpt;
+            A.overlap = (A.segrec ? 2 * seg1 : step2) + other < fr - margin ? 1 : 0;
         }
         if (A.mode) {
-            A.segrec = A.mode >= 2 ? 1 : 0;
-            A.overlap = A.mode == 3 ? 1 : 0;
+            A.segrec = A.mode == 2 || A.mode == 3 ? 1 : 0;
+            A.overlap = A.mode == 3 || A.mode == 4 ? 1 : 0;
         }
-        A.nblocks = A.segrec ? ckpt : 1;
+        A.nblocks = A.segrec ? ckpt : A.overlap ? 2 : 1;
     }
-    const size_t halves = A.overlap ? 2 : 1;
+    const size_t halves = A.overlap && A.segrec ? 2 : 1;
     if (A.overlap) {
         HIPCHK(h, hipStreamCreate(&A.s2));
         for (hipEvent_t *e : {&A.ev_go, &A.ev_rec[0], &A.ev_rec[1], &A.ev_rev[0], &A.ev_rev[1]})
@@ -1251,7 +1258,7 @@ template <class H> int adjoint_run_t(moist_handle *h, int nsteps, int ckpt, cons
     // no later step
     fill(h, A.carry, 2 * h->nm);
     const size_t state_bytes = 7 * n2 * h->elem;
-    if (A.overlap) {
+    if (A.overlap && A.segrec) {
         // pipelined segments: segment sg - 1's recording pass (stream s2, the
         // half of segment sg + 1, once that segment's reverse sweep is done)
         // runs beside segment sg's reverse sweep (the handle's stream, once
@@ -1286,6 +1293,55 @@ template <class H> int adjoint_run_t(moist_handle *h, int nsteps, int ckpt, cons
             HIPCHK(h, hipGetLastError());
         }
         A.half = 0;
+    }
+    if (A.overlap && !A.segrec) {
+        // pipelined per-step records: the states of a segment (the handle's
+        // stream), then per step in reverse its recording pass on stream s2
+        // (block j mod 2, once the reverse of step j + 2 has read that block)
+        // beside the reverse of step j + 1.  Both passes of the pool (the state
+        // pass and the recording passes) belong to one segment at a time: the
+        // next segment's state pass follows the reverse of this one's step 0,
+        // which waited for its recording pass.
+        hipStream_t sA = h->stream, sB = A.s2;
+        auto record_async = [&](int j) -> int {
+            h->stream = sB;
+            if (h->precision == MOIST_MIXED) copy_bytes(h, h->pool, seg_step(h, j), state_bytes);
+            copy_set(h, h->sfh, seg_sfh(h, j), 2 * h->nm);
+            A.cur = rec_block(h, j & 1);
+            record_step(h, j, false);
+            h->stream = sA;
+            HIPCHK(h, hipEventRecord(A.ev_rec[j & 1], sB));
+            return 0;
+        };
+        for (int sg = A.ncp - 1; sg >= 0; --sg) {
+            const int n0 = sg * ckpt, n1 = std::min(nsteps, n0 + ckpt), J = n1 - n0;
+            load_state(h, A.cp + A.cp_stride * sg);
+            for (int n = n0; n < n1; ++n) {
+                copy_bytes(h, seg_step(h, n - n0), h->pool, state_bytes);
+                copy_set(h, seg_sfh(h, n - n0), h->sfh, 2 * h->nm);
+                if (n + 1 >= n1) continue;
+                if (h->precision == MOIST_MIXED) {
+                    A.cur = rec_block(h, 0);
+                    record_step(h, n - n0, true);
+                } else if (run_steps(h, 1)) {
+                    return -1;
+                }
+            }
+            HIPCHK(h, hipEventRecord(A.ev_go, sA));
+            HIPCHK(h, hipStreamWaitEvent(sB, A.ev_go, 0));
+            if (record_async(J - 1)) return -1;
+            for (int j = J - 1; j >= 0; --j) {
+                if (j > 0) {
+                    if (j + 1 <= J - 1) HIPCHK(h, hipStreamWaitEvent(sB, A.ev_rev[(j - 1) & 1], 0));
+                    if (record_async(j - 1)) return -1;
+                }
+                HIPCHK(h, hipStreamWaitEvent(sA, A.ev_rec[j & 1], 0));
+                A.cur = rec_block(h, j & 1);
+                reverse_step(h, j);
+                HIPCHK(h, hipEventRecord(A.ev_rev[j & 1], sA));
+            }
+            HIPCHK(h, hipGetLastError());
+        }
     }
     for (int sg = A.overlap ? -1 : A.ncp - 1; sg >= 0; --sg) {
         const int n0 = sg * ckpt, n1 = std::min(nsteps, n0 + ckpt);
@@ -1731,8 +1787,8 @@ int moist_set_option(moist_handle *h, int option, int value) {
         return 0;
     }
     if (option == MOIST_OPT_ADJ_RECORDS) {
-        if (value < 0 || value > 3) {
-            h->err = "adjoint record mode must be 0, 1, 2 or 3";
+        if (value < 0 || value > 4) {
+            h->err = "adjoint record mode must be 0 to 4";
             return -2;
         }
         h->adj_mode = value;

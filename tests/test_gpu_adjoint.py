@@ -179,6 +179,23 @@ def test_pipelined_segments_match_segment_records(precision, case):
     np.testing.assert_array_equal(a[2], b[2])
 
 
+@pytest.mark.parametrize("case", ["bubble", "sf_rd"])
+@pytest.mark.parametrize("precision", [core.FP64, core.MIXED])
+def test_pipelined_steps_match_step_records(precision, case):
+    """Per-step records with each step's recording pass on a second stream
+    beside the reverse of the step after it (two record blocks used in turn)
+    replay the same values as per-step records: J and the gradients are
+    bitwise equal.  7 steps, checkpoints every 3: segments of 3, 3 and 1
+    steps, so both blocks are reused within a segment and across segments."""
+    p, s, tt = _case() if case == "bubble" else _case_sf_rd()
+    a = _device_adjoint(p, s, tt, 7, 3, precision, "step")
+    b = _device_adjoint(p, s, tt, 7, 3, precision, "step_pipelined")
+    assert a[0] == b[0]
+    for f in range(7):
+        np.testing.assert_array_equal(a[1][f], b[1][f], err_msg=PROG7[f])
+    np.testing.assert_array_equal(a[2], b[2])
+
+
 def test_adjoint_buffers_that_do_not_fit_fail_cleanly_twice():
     """An adjoint run whose checkpoints cannot fit in device memory (100,000
     steps of the 512x128 bubble with a checkpoint every step: about 370 GB)
