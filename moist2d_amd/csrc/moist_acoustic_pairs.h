@@ -121,9 +121,9 @@ __global__ void __launch_bounds__(NT, (WPE > 4 ? WPE : 4))
     auto load_late = [&]() {
 #pragma unroll
         for (int m = 0; m < L / 2; ++m) {
-            R r0, r1, m0, m1;
+            R r0, r1, m0 = R(0.0), m1 = R(0.0);
             ldb2(f_rwi, bc[m], r0, r1);
-            ldb2(f_mz, bc[m], m0, m1);
+            if (!first) ldb2(f_mz, bc[m], m0, m1);
             rwv[2 * m] = r0;
             rwv[2 * m + 1] = r1;
             mzv[2 * m] = 2 * m < nf ? m0 : R(0.0);
@@ -136,6 +136,12 @@ __global__ void __launch_bounds__(NT, (WPE > 4 ? WPE : 4))
             const R sv = ldb(f_srw, bx(kq));
             if (!(nl > 0 && j <= nz - 2)) rwv[q] = R(0.0);
             swv[q] = (nl > 0 && kq >= 0 && kq < nz) ? sv : R(0.0);
+        }
+        if (first) {
+            // mom_z starts from the stage input's rho_w (moist.jl:1347), which
+            // swv holds at the chunk's faces
+#pragma unroll
+            for (int l = 0; l < L; ++l) mzv[l] = l < nf ? swv[l + 1] : R(0.0);
         }
         if constexpr (FIN) {
 #pragma unroll
@@ -173,7 +179,15 @@ __global__ void __launch_bounds__(NT, (WPE > 4 ? WPE : 4))
             iru_[m][1] = ldb(in.f[RHOU] + o2, bx(kp + 1));
         }
         const uint32_t oc = bc[m], om = bm[m], op = bp[m];
-        ldg(f_mx, oc, mx_[m]);
+        if (first) {
+            // mom_x starts from the stage input's rho_u (moist.jl:1346): read
+            // it there rather than from a copy (0 on the padding level of odd nz)
+            const int kp = min(k0 + 2 * m, kev);
+            mx_[m][0] = ldb(in.f[RHOU] + o2, bx(kp));
+            mx_[m][1] = kp + 1 < nz ? ldb(in.f[RHOU] + o2, bx(kp + 1)) : R(0.0);
+        } else {
+            ldg(f_mx, oc, mx_[m]);
+        }
         ldg(f_cur, oc, cc_[m]);
         ldg(f_cur, om, cm_[m]);
         ldg(f_cur, op, cx_[m]);

@@ -257,8 +257,12 @@ __global__ void __launch_bounds__(ST_THREADS) k_slow(const Dom *__restrict__ dom
             st_pair(w.rup, c_rup[0], c_rup[1]);
             st_pair(w.rwp, c_rwp[0], c_rwp[1]);
             st_pair(rtp0, c_rtp[0], c_rtp[1]);
-            st_pair(w.mom_x, o_mx[0], o_mx[1]);
-            st_pair(w.mom_z, o_mz[0], o_mz[1]);
+            // the pair-layout acoustic kernel's first substep reads the stage
+            // input itself for the accumulators' start
+            if constexpr (!P) {
+                st_pair(w.mom_x, o_mx[0], o_mx[1]);
+                st_pair(w.mom_z, o_mz[0], o_mz[1]);
+            }
         }
     }
     __syncthreads();
